@@ -1,0 +1,156 @@
+/* C ABI of the MI355X light-propagation library (libxlumina_hip.so).
+ *
+ * Plain C: device pointers, sizes, float64 scalars, a hipStream_t passed as
+ * void*.  Every call returns 0 on success or a positive status; the message of
+ * the last failure on the calling thread is xl_last_error().  No call
+ * allocates device memory: callers pass a workspace of at least
+ * xl_*_workspace_bytes(...) bytes.  Calls are asynchronous on `stream`.
+ *
+ * Arrays: complex64 fields are interleaved (re, im) float pairs, row-major
+ * [y][x], components `*_comp_stride` elements apart; masks are float32 [y][x];
+ * grids x, y are float64 device arrays (the reference keeps x, y as float64).
+ *
+ * Each entry point names the reference function it replaces (paths relative to
+ * the XLuminA repository root).
+ */
+#ifndef XLUMINA_HIP_H
+#define XLUMINA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define XL_OK 0
+#define XL_EINVAL 1     /* bad shape / argument (message says which)        */
+#define XL_EHIP 2       /* HIP runtime error                                */
+#define XL_EWORKSPACE 3 /* workspace smaller than *_workspace_bytes         */
+
+/* Jones masks fused before a propagation (applied in order, up to two) */
+#define XL_MASK_NONE 0
+#define XL_MASK_SLM 1        /* u * exp(i m0)                    optical_elements.py:87-103  */
+#define XL_MASK_SSLM 2       /* diag(exp(i m0), exp(i m1))       optical_elements.py:186-222 */
+#define XL_MASK_LCD 3        /* retarder(eta=m0, theta=m1)       optical_elements.py:266-305 */
+#define XL_MASK_LCD_SCALAR 4 /* retarder(eta=s0, theta=s1)       optical_elements.py:266-305 */
+#define XL_MASK_LP 5         /* polariser alpha=m0, Ez -> 0      optical_elements.py:307-332 */
+
+/* construction of the propagated Ez from (Ex, Ey) (incoming Ez discarded) */
+#define XL_EZ_NONE 0
+#define XL_EZ_VRS 1  /* (Ex X + Ey Y)/r           vectorized_optics.py:258-261 */
+#define XL_EZ_VCZT 2 /* (Ex X/r + Ey Y/r) z/r     vectorized_optics.py:341-344 */
+#define XL_EZ_HNA 3  /* (Ex X + Ey Y)/rho         optical_elements.py:538      */
+
+/* CZT variants */
+#define XL_CZT_RS 0    /* F0 * K_y (E F) K_x^T * z dx dy lambda   wave_optics.py:333-357   */
+#define XL_CZT_HNA 1   /* cte * K_y (RW-lens(E)) K_x^T            optical_elements.py:600-638 */
+#define XL_CZT_PLAIN 2 /* K_y E K_x^T                             wave_optics.py:359-371   */
+
+typedef struct xl_mask_stage {
+  int32_t kind;
+  int32_t reserved;
+  const float* m0; /* per-pixel parameter 0 (phase / eta / alpha) or NULL */
+  const float* m1; /* per-pixel parameter 1 (phi / theta) or NULL        */
+  double s0, s1;   /* constant eta, theta for XL_MASK_LCD_SCALAR           */
+} xl_mask_stage;
+
+typedef struct xl_rs_params {
+  const void* in;          /* nin complex64 components, n x n           */
+  int64_t in_comp_stride;
+  int32_t nin;             /* 1 (scalar) or 2/3 (Ex, Ey[, Ez])          */
+  int32_t ez_mode;         /* XL_EZ_NONE or XL_EZ_VRS                   */
+  void* out;               /* ncomp complex64 components, n x n         */
+  int64_t out_comp_stride;
+  float* intensity;        /* optional n x n sum_c |out_c|^2, or NULL   */
+  int32_t n;               /* grid side (square, as the reference)      */
+  int32_t adjoint;         /* 1: apply the adjoint operator (conj kernel) */
+  const double* x;         /* n float64 samples                         */
+  const double* y;
+  double z;                /* propagation distance (microns)            */
+  double k;                /* 2 pi / lambda                              */
+  double dx, dy;           /* x[1]-x[0], y[1]-y[0] (wave_optics.py:269-270) */
+  xl_mask_stage mask[2];
+} xl_rs_params;
+
+typedef struct xl_czt_params {
+  const void* in;          /* nin components, rows x cols of the input plane */
+  int64_t in_comp_stride;
+  int32_t nin;
+  int32_t ncomp;           /* components propagated (3 when Ez is built)     */
+  void* out;
+  int64_t out_comp_stride;
+  int32_t ny_in, nx_in;    /* input plane (forward)                           */
+  int32_t my, mx;          /* output plane (forward)                          */
+  int32_t mode;            /* XL_CZT_*                                        */
+  int32_t adjoint;         /* 1: in = (my, mx) cotangent, out = (ny_in, nx_in) */
+  int32_t ez_mode;         /* XL_EZ_NONE / XL_EZ_VCZT (XL_CZT_HNA builds its own) */
+  int32_t reserved;
+  const double* x;         /* input-plane grid (nx_in, ny_in)                  */
+  const double* y;
+  const double* xout;      /* output-plane grid (mx, my)                       */
+  const double* yout;
+  double z, wavelength, k;
+  double dm;               /* Dm of the Bluestein frequency window             */
+  double fy1, fy2, fx1, fx2;
+  double cte_re, cte_im;   /* constant output factor (forward)                 */
+  double hna_radius, hna_f;
+  xl_mask_stage mask[2];
+} xl_czt_params;
+
+typedef struct xl_pre_params {
+  /* forward pre-op: masks on (Ex, Ey) then XL_EZ_* / high-NA lens map;
+   * backward: cotangent of the pre-op output -> cotangents of Ex, Ey and of
+   * the per-pixel mask parameters. */
+  const void* in;          /* Ex, Ey (complex64, n_rows x n_cols)              */
+  int64_t in_comp_stride;
+  int32_t nin;
+  int32_t ncomp;           /* components of the pre-op output                 */
+  const void* gout;        /* backward: cotangent of the pre-op output         */
+  void* out;               /* forward: pre-op output; backward: grad of inputs */
+  int64_t comp_stride;
+  float* gparam[2][2];     /* backward: per-pixel grads of mask[s].m0 / m1, or NULL */
+  int32_t rows, cols;
+  int32_t ez_mode;         /* XL_EZ_*                                          */
+  int32_t hna;             /* 1: high-NA lens map (optical_elements.py:555-594) */
+  const double* x;
+  const double* y;
+  double z;
+  double hna_radius, hna_f;
+  xl_mask_stage mask[2];
+} xl_pre_params;
+
+const char* xl_last_error(void);
+int xl_abi_version(void);
+int xl_max_fft_length(void);
+/* Builds the float64-generated twiddle table on the current device (called
+ * once per device before any other call; later calls are no-ops). */
+int xl_init(void);
+
+/* RS_propagation_jit (wave_optics.py:281-289), VRS_propagation_jit
+ * (vectorized_optics.py:364-373) with Jones masks / Ez construction fused. */
+size_t xl_rs_workspace_bytes(int32_t n, int32_t ncomp);
+int xl_rs_propagate(const xl_rs_params* p, void* workspace, size_t workspace_bytes, void* stream);
+
+/* CZT_jit (wave_optics.py:333-357), VCZT_jit (vectorized_optics.py:375-384),
+ * CZT_for_high_NA_jit (wave_optics.py:359-371), VCZT_objective_lens
+ * (optical_elements.py:600-638); Bluestein FP32 form.  Returns XL_EINVAL for
+ * the shapes the reference cannot run (m + M - 1 a power of two, M = 1). */
+size_t xl_czt_workspace_bytes(const xl_czt_params* p);
+int xl_czt(const xl_czt_params* p, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Standalone pre-ops and their adjoint: SLM / sSLM / LCD / linear_polarizer
+ * (optical_elements.py:87-332), high_NA_objective_lens (:515-594). */
+int xl_pre_forward(const xl_pre_params* p, void* stream);
+int xl_pre_backward(const xl_pre_params* p, void* stream);
+
+/* transfer_function_RS (wave_optics.py:291-297) on meshgrid(x, y): out is
+ * ny x nx complex64. */
+int xl_transfer_function_rs(void* out, const double* x, const double* y, int32_t nx, int32_t ny, double z,
+                            double k, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XLUMINA_HIP_H */

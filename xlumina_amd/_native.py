@@ -1,0 +1,122 @@
+"""ctypes binding of libxlumina_hip.so (include/xlumina_hip.h).
+
+The product path has no fallback: if the library is missing or the device is
+not a ROCm GPU, calls raise ``NativeUnavailable``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+from . import build_lib
+
+
+class NativeUnavailable(RuntimeError):
+    pass
+
+
+class XlError(RuntimeError):
+    pass
+
+
+XL_MASK_NONE, XL_MASK_SLM, XL_MASK_SSLM, XL_MASK_LCD, XL_MASK_LCD_SCALAR, XL_MASK_LP = range(6)
+XL_EZ_NONE, XL_EZ_VRS, XL_EZ_VCZT, XL_EZ_HNA = range(4)
+XL_CZT_RS, XL_CZT_HNA, XL_CZT_PLAIN = range(3)
+
+
+class MaskStage(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("m0", C.c_void_p), ("m1", C.c_void_p),
+                ("s0", C.c_double), ("s1", C.c_double)]
+
+
+class RsParams(C.Structure):
+    _fields_ = [("inp", C.c_void_p), ("in_comp_stride", C.c_int64), ("nin", C.c_int32), ("ez_mode", C.c_int32),
+                ("out", C.c_void_p), ("out_comp_stride", C.c_int64), ("intensity", C.c_void_p),
+                ("n", C.c_int32), ("adjoint", C.c_int32), ("x", C.c_void_p), ("y", C.c_void_p),
+                ("z", C.c_double), ("k", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
+                ("mask", MaskStage * 2)]
+
+
+class CztParams(C.Structure):
+    _fields_ = [("inp", C.c_void_p), ("in_comp_stride", C.c_int64), ("nin", C.c_int32), ("ncomp", C.c_int32),
+                ("out", C.c_void_p), ("out_comp_stride", C.c_int64),
+                ("ny_in", C.c_int32), ("nx_in", C.c_int32), ("my", C.c_int32), ("mx", C.c_int32),
+                ("mode", C.c_int32), ("adjoint", C.c_int32), ("ez_mode", C.c_int32), ("reserved", C.c_int32),
+                ("x", C.c_void_p), ("y", C.c_void_p), ("xout", C.c_void_p), ("yout", C.c_void_p),
+                ("z", C.c_double), ("wavelength", C.c_double), ("k", C.c_double), ("dm", C.c_double),
+                ("fy1", C.c_double), ("fy2", C.c_double), ("fx1", C.c_double), ("fx2", C.c_double),
+                ("cte_re", C.c_double), ("cte_im", C.c_double), ("hna_radius", C.c_double), ("hna_f", C.c_double),
+                ("mask", MaskStage * 2)]
+
+
+class PreParams(C.Structure):
+    _fields_ = [("inp", C.c_void_p), ("in_comp_stride", C.c_int64), ("nin", C.c_int32), ("ncomp", C.c_int32),
+                ("gout", C.c_void_p), ("out", C.c_void_p), ("comp_stride", C.c_int64),
+                ("gparam", (C.c_void_p * 2) * 2), ("rows", C.c_int32), ("cols", C.c_int32),
+                ("ez_mode", C.c_int32), ("hna", C.c_int32), ("x", C.c_void_p), ("y", C.c_void_p),
+                ("z", C.c_double), ("hna_radius", C.c_double), ("hna_f", C.c_double), ("mask", MaskStage * 2)]
+
+
+# symbol -> (restype, argtypes); every declaration of include/xlumina_hip.h
+SIGNATURES = {
+    "xl_last_error": (C.c_char_p, []),
+    "xl_abi_version": (C.c_int, []),
+    "xl_max_fft_length": (C.c_int, []),
+    "xl_init": (C.c_int, []),
+    "xl_rs_workspace_bytes": (C.c_size_t, [C.c_int32, C.c_int32]),
+    "xl_rs_propagate": (C.c_int, [C.POINTER(RsParams), C.c_void_p, C.c_size_t, C.c_void_p]),
+    "xl_czt_workspace_bytes": (C.c_size_t, [C.POINTER(CztParams)]),
+    "xl_czt": (C.c_int, [C.POINTER(CztParams), C.c_void_p, C.c_size_t, C.c_void_p]),
+    "xl_pre_forward": (C.c_int, [C.POINTER(PreParams), C.c_void_p]),
+    "xl_pre_backward": (C.c_int, [C.POINTER(PreParams), C.c_void_p]),
+    "xl_transfer_function_rs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
+                                          C.c_double, C.c_double, C.c_void_p]),
+}
+
+_lib = None
+_lock = threading.Lock()
+_inited_devices: set = set()
+
+
+def load(path: str | None = None):
+    """Loads the shared library (no GPU needed)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        path = path or os.environ.get("XL_LIBRARY") or build_lib.lib_path()
+        if not os.path.exists(path):
+            raise NativeUnavailable(
+                f"{path} is missing: build it with `python -m xlumina_amd.build_lib` (hipcc, gfx950)")
+        lib = C.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = _lib.xl_last_error().decode(errors="replace")
+        if rc == 1:
+            raise ValueError(msg)
+        raise XlError(f"libxlumina_hip status {rc}: {msg}")
+
+
+def ensure_device(device) -> None:
+    """xl_init on `device` (float64 twiddle table); torch selects the device."""
+    import torch
+    if device.type != "cuda":
+        raise NativeUnavailable(f"the propagation kernels run on a ROCm GPU, got device {device}")
+    if not torch.version.hip:
+        raise NativeUnavailable("PyTorch is not a ROCm build")
+    idx = device.index if device.index is not None else torch.cuda.current_device()
+    if idx in _inited_devices:
+        return
+    lib = load()
+    with torch.cuda.device(idx):
+        check(lib.xl_init())
+    _inited_devices.add(idx)

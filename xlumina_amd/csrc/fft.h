@@ -1,0 +1,165 @@
+// LDS-staged Stockham FFT building blocks for gfx950 (64-wide waves).
+//
+// A workgroup runs FPW independent complex64 FFTs of a power-of-two length L
+// (64 <= L <= 4096), each on TPF = L/16 threads.  Every pass is radix 16
+// except a final radix 2/4/8 when log2(L) is not a multiple of 4.  Data lives
+// in LDS between passes (natural order in, natural order out); each thread
+// keeps its R points of a butterfly in registers.  LDS indices are padded by
+// one slot per 16 complex values so the stride-16 writes of the first pass
+// spread over the banks.
+//
+// Inter-pass twiddles come from a 4096-entry table e^{-2 pi i k/4096}
+// generated once per device in float64 (xl_fft_init_kernel).  A butterfly
+// loads w, w^2, w^4, w^8 and forms the other powers by complex products.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace xl {
+
+constexpr int TWN = 4096;                 // twiddle table length == max FFT length
+constexpr int MAX_FFT = 4096;
+
+__device__ float2 g_tw[TWN];              // e^{-2 pi i k / TWN}, float64-generated
+
+__global__ void xl_fft_init_kernel() {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < TWN) {
+    double s, c;
+    sincospi(-2.0 * (double)k / (double)TWN, &s, &c);
+    g_tw[k] = make_float2((float)c, (float)s);
+  }
+}
+
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+template <int L> struct LdsLen { static constexpr int value = L + L / 16; };
+
+__device__ __forceinline__ float2 operator+(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 operator-(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 operator*(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+template <bool INV> __device__ __forceinline__ float2 twmul(float2 v, float2 w) {
+  return INV ? cmulc(v, w) : cmul(v, w);
+}
+
+// cos / sin of 2 pi q / 16, q = 0..7
+__device__ constexpr float kC16[8] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508978f,
+                                      0.0f, -0.38268343236508978f, -0.70710678118654752f, -0.92387953251128674f};
+__device__ constexpr float kS16[8] = {0.0f, 0.38268343236508978f, 0.70710678118654752f, 0.92387953251128674f,
+                                      1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508978f};
+
+// v * e^{-+ 2 pi i q / 16} with q a compile-time constant after unrolling.
+template <bool INV> __device__ __forceinline__ float2 rot16(float2 v, int q) {
+  if (q == 0) return v;
+  if (q == 4) return INV ? make_float2(-v.y, v.x) : make_float2(v.y, -v.x);
+  const float c = kC16[q], s = INV ? kS16[q] : -kS16[q];
+  return make_float2(v.x * c - v.y * s, v.x * s + v.y * c);
+}
+
+// In-register DFT of R points (R = 1, 2, 4, 8, 16), natural order in and out.
+template <int R, bool INV> struct DFT {
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 e[R / 2], o[R / 2];
+#pragma unroll
+    for (int i = 0; i < R / 2; ++i) { e[i] = v[2 * i]; o[i] = v[2 * i + 1]; }
+    DFT<R / 2, INV>::run(e);
+    DFT<R / 2, INV>::run(o);
+#pragma unroll
+    for (int k = 0; k < R / 2; ++k) {
+      const float2 t = rot16<INV>(o[k], k * (16 / R));
+      v[k] = e[k] + t;
+      v[k + R / 2] = e[k] - t;
+    }
+  }
+};
+template <bool INV> struct DFT<1, INV> {
+  __device__ __forceinline__ static void run(float2*) {}
+};
+
+// w[r] = e^{-2 pi i r base / TWN}, r < R  (4 table loads, products for the rest)
+template <int R> __device__ __forceinline__ void twiddle_powers(float2* w, int base) {
+  w[0] = make_float2(1.f, 0.f);
+  if constexpr (R > 1) w[1] = g_tw[base];
+  if constexpr (R > 2) { w[2] = g_tw[2 * base]; w[3] = cmul(w[1], w[2]); }
+  if constexpr (R > 4) {
+    w[4] = g_tw[4 * base];
+    w[5] = cmul(w[1], w[4]); w[6] = cmul(w[2], w[4]); w[7] = cmul(w[3], w[4]);
+  }
+  if constexpr (R > 8) {
+    w[8] = g_tw[8 * base];
+#pragma unroll
+    for (int r = 9; r < 16; ++r) w[r] = cmul(w[r - 8], w[8]);
+  }
+}
+
+// One Stockham radix-R pass of an L-point FFT over TPF threads (thread t).
+template <int L, int R, int NS, int TPF, bool INV>
+__device__ __forceinline__ void stockham_pass(float2* s, int t) {
+  constexpr int NB = L / R;
+  constexpr int BPT = (NB + TPF - 1) / TPF;
+  float2 v[BPT][R];
+#pragma unroll
+  for (int b = 0; b < BPT; ++b) {
+    const int j = t + b * TPF;
+    if (NB % TPF == 0 || j < NB) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b][r] = s[lpad(j + r * NB)];
+      if constexpr (NS > 1) {
+        float2 w[R];
+        twiddle_powers<R>(w, (j & (NS - 1)) * (TWN / (NS * R)));
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[b][r] = twmul<INV>(v[b][r], w[r]);
+      }
+      DFT<R, INV>::run(v[b]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < BPT; ++b) {
+    const int j = t + b * TPF;
+    if (NB % TPF == 0 || j < NB) {
+      const int idxD = (j / NS) * NS * R + (j & (NS - 1));
+#pragma unroll
+      for (int r = 0; r < R; ++r) s[lpad(idxD + r * NS)] = v[b][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <int L, int TPF, bool INV, int NS = 1>
+__device__ __forceinline__ void fft_lds(float2* s, int t) {
+  if constexpr (NS < L) {
+    constexpr int rem = L / NS;
+    constexpr int R = rem >= 16 ? 16 : rem;
+    stockham_pass<L, R, NS, TPF, INV>(s, t);
+    fft_lds<L, TPF, INV, NS * R>(s, t);
+  }
+}
+
+// Block-index remap so that consecutive logical blocks (which share 128-byte
+// lines of the tiled intermediates) land on the same XCD: hardware dispatches
+// block b to XCD b % 8.
+__device__ __forceinline__ int xcd_remap(int b, int nblocks) {
+  constexpr int NX = 8;
+  if (nblocks % NX) return b;
+  return (b % NX) * (nblocks / NX) + b / NX;
+}
+
+// --- float64 phase helpers ---------------------------------------------------
+// e^{i 2 pi t} for a float64 number of turns t: the integer part is removed in
+// float64, the remainder in [-1/2, 1/2] goes to float32 sincospi.
+__device__ __forceinline__ float2 expi_turns(double t) {
+  const double f = t - rint(t);
+  float s, c;
+  sincospif((float)(2.0 * f), &s, &c);
+  return make_float2(c, s);
+}
+
+}  // namespace xl

@@ -1,0 +1,108 @@
+// Per-pixel Jones masks and Ez constructions, fused into the first pass of the
+// propagators and also run standalone.
+//
+//   SLM   optical_elements.py:87-103   u * e^{i phi}
+//   sSLM  optical_elements.py:141-152, :186-222   diag(e^{i alpha}, e^{i phi}), Ez kept
+//   LCD   optical_elements.py:122-139, :169-180, :266-305   general retarder, delta = 0,
+//         eta/theta per pixel (XL_MASK_LCD) or constant (XL_MASK_LCD_SCALAR), Ez kept
+//   LP    optical_elements.py:109-120, :307-332   [[c^2, cs], [cs, s^2]], Ez -> 0
+// Ez constructions (incoming Ez discarded):
+//   VRS   vectorized_optics.py:258-261   (Ex X + Ey Y) / r,  r = sqrt(X^2+Y^2+z^2)
+//   VCZT  vectorized_optics.py:341-344   (Ex X / r + Ey Y / r) z / r
+//   HNA   optical_elements.py:532-538   (Ex X + Ey Y) / rho
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace xl {
+
+enum { MASK_NONE = 0, MASK_SLM = 1, MASK_SSLM = 2, MASK_LCD = 3, MASK_LCD_SCALAR = 4, MASK_LP = 5 };
+enum { EZ_NONE = 0, EZ_VRS = 1, EZ_VCZT = 2, EZ_HNA = 3 };
+
+struct MaskStage {
+  int kind;
+  const float* m0;   // phase / eta / alpha, per pixel
+  const float* m1;   // phi / theta, per pixel
+  double s0, s1;     // constant eta, theta (MASK_LCD_SCALAR)
+};
+// Up to two masks applied in order (e.g. sSLM then LCD).
+struct MaskArgs {
+  MaskStage st[2];
+};
+
+__device__ __forceinline__ float2 c_mul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 c_add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 c_scale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 c_expi(float p) {
+  float s, c;
+  sincosf(p, &s, &c);
+  return make_float2(c, s);
+}
+
+// Retarder Jones entries for (eta, theta), delta = 0 (optical_elements.py:134-139):
+//   J00 = e^{-i eta/2} c^2 + e^{i eta/2} s^2, J01 = J10 = -2 i sin(eta/2) s c,
+//   J11 = e^{-i eta/2} s^2 + e^{i eta/2} c^2.
+__device__ __forceinline__ void lcd_jones(float eta, float theta, float2& j00, float2& j01, float2& j11) {
+  float sh, ch, st, ct;
+  sincosf(0.5f * eta, &sh, &ch);
+  sincosf(theta, &st, &ct);
+  const float c2 = ct * ct, s2 = st * st, sc = st * ct;
+  j00 = make_float2(ch * (c2 + s2), -sh * c2 + sh * s2);
+  j11 = make_float2(ch * (s2 + c2), -sh * s2 + sh * c2);
+  j01 = make_float2(0.f, -2.f * sh * sc);
+}
+
+__device__ __forceinline__ void apply_stage(const MaskStage& m, int64_t pix, float2& e0, float2& e1, float2& e2) {
+  switch (m.kind) {
+    case MASK_SLM:
+      e0 = c_mul(e0, c_expi(m.m0[pix]));
+      break;
+    case MASK_SSLM:
+      e0 = c_mul(e0, c_expi(m.m0[pix]));
+      e1 = c_mul(e1, c_expi(m.m1[pix]));
+      break;
+    case MASK_LCD:
+    case MASK_LCD_SCALAR: {
+      float2 j00, j01, j11;
+      const float eta = m.kind == MASK_LCD ? m.m0[pix] : (float)m.s0;
+      const float th = m.kind == MASK_LCD ? m.m1[pix] : (float)m.s1;
+      lcd_jones(eta, th, j00, j01, j11);
+      const float2 a = e0, b = e1;
+      e0 = c_add(c_mul(j00, a), c_mul(j01, b));
+      e1 = c_add(c_mul(j01, a), c_mul(j11, b));
+      break;
+    }
+    case MASK_LP: {
+      float s, c;
+      sincosf(m.m0[pix], &s, &c);
+      const float2 a = e0, b = e1;
+      e0 = c_add(c_scale(a, c * c), c_scale(b, c * s));
+      e1 = c_add(c_scale(a, c * s), c_scale(b, s * s));
+      e2 = make_float2(0.f, 0.f);
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+__device__ __forceinline__ void apply_mask(const MaskArgs& m, int64_t pix, float2& e0, float2& e1, float2& e2) {
+  apply_stage(m.st[0], pix, e0, e1, e2);
+  apply_stage(m.st[1], pix, e0, e1, e2);
+}
+
+__device__ __forceinline__ float2 build_ez(int mode, float2 ex, float2 ey, double X, double Y, double z) {
+  if (mode == EZ_VRS || mode == EZ_VCZT) {
+    const double r = sqrt(X * X + Y * Y + z * z);
+    double cx = X / r, cy = Y / r;
+    if (mode == EZ_VCZT) { cx *= z / r; cy *= z / r; }
+    return c_add(c_scale(ex, (float)cx), c_scale(ey, (float)cy));
+  }
+  // EZ_HNA: (Ex X + Ey Y) / rho; rho = 0 gives NaN as in the reference
+  const double rho = sqrt(X * X + Y * Y);
+  return c_add(c_scale(ex, (float)(X / rho)), c_scale(ey, (float)(Y / rho)));
+}
+
+}  // namespace xl

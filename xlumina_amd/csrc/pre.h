@@ -1,0 +1,203 @@
+// Standalone per-pixel pre-ops (Jones mask chain, Ez construction, high-NA
+// lens map) and their adjoint.  The forward form is the same arithmetic the
+// propagators fuse into their first pass; the adjoint is what the propagators'
+// backward runs after the adjoint convolution.
+//
+// Cotangent convention (PyTorch): for complex y, g = dL/dRe(y) + i dL/dIm(y);
+// a linear map y = A x sends g to A^H g; a real parameter p gets
+// Re(sum conj(g) dy/dp).
+#pragma once
+#include "fft.h"
+#include "masks.h"
+#include "rs.h"
+
+namespace xl {
+
+struct PreArgs {
+  const float2* in;
+  int64_t in_cs;
+  int nin, ncomp;
+  const float2* gout;
+  float2* out;
+  int64_t cs;
+  float* gparam[2][2];
+  int rows, cols, ez_mode, hna;
+  const double* x;
+  const double* y;
+  double z, hna_radius, hna_f, hna_stm2;
+  MaskArgs mask;
+};
+
+// Richards-Wolf lens factors (optical_elements.py:528-590): the 3x2 real map
+// (Ex', Ey') -> lens output, with Ez = (Ex' X + Ey' Y)/rho folded in.
+__device__ __forceinline__ void hna_map(double X, double Y, double radius, double f, double stm2, float A[3][2]) {
+  const double rho = sqrt(X * X + Y * Y);
+  const float cp = (float)(X / rho), sp = (float)(Y / rho);
+  float st, ct;
+  sincosf((float)(rho / f), &st, &ct);
+  const double u2 = (X * X + Y * Y) / (radius * radius);
+  const float G = u2 < 1.0 ? (float)(1.0 / sqrt(fabs(1.0 - u2 * stm2))) : 0.f;
+  const float w = sqrtf(fabsf(ct)) * G;
+  const float R[3][3] = {{ct * cp * cp + sp * sp, ct * cp * sp - sp * cp, -cp * st},
+                         {sp * ct * cp - cp * sp, ct * sp * sp + cp * cp, -sp * st},
+                         {st * cp, st * sp, ct}};
+  for (int i = 0; i < 3; ++i) {
+    A[i][0] = w * (R[i][0] + R[i][2] * cp);
+    A[i][1] = w * (R[i][1] + R[i][2] * sp);
+  }
+}
+
+__device__ __forceinline__ float re_conj_mul(float2 g, float2 v) { return g.x * v.x + g.y * v.y; }  // Re(conj(g) v)
+
+__global__ void pre_forward_kernel(PreArgs a) {
+  const int64_t npix = (int64_t)a.rows * a.cols;
+  for (int64_t pix = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; pix < npix;
+       pix += (int64_t)gridDim.x * blockDim.x) {
+    const int row = (int)(pix / a.cols), col = (int)(pix % a.cols);
+    float2 e0 = a.in[pix];
+    float2 e1 = a.nin > 1 ? a.in[a.in_cs + pix] : make_float2(0.f, 0.f);
+    float2 e2 = a.nin > 2 ? a.in[2 * a.in_cs + pix] : make_float2(0.f, 0.f);
+    apply_mask(a.mask, pix, e0, e1, e2);
+    if (a.hna) {
+      float A[3][2];
+      hna_map(a.x[col], a.y[row], a.hna_radius, a.hna_f, a.hna_stm2, A);
+      for (int i = 0; i < 3; ++i) a.out[i * a.cs + pix] = c_add(c_scale(e0, A[i][0]), c_scale(e1, A[i][1]));
+      continue;
+    }
+    if (a.ez_mode != EZ_NONE) e2 = build_ez(a.ez_mode, e0, e1, a.x[col], a.y[row], a.z);
+    a.out[pix] = e0;
+    if (a.ncomp > 1) a.out[a.cs + pix] = e1;
+    if (a.ncomp > 2) a.out[2 * a.cs + pix] = e2;
+  }
+}
+
+// adjoint of one mask stage: e_in -> (stage) -> e_out; g holds the cotangent of
+// e_out on entry and of e_in on exit; parameter cotangents go to gp0 / gp1.
+__device__ __forceinline__ void stage_backward(const MaskStage& m, int64_t pix, float2 e0, float2 e1, float2& g0,
+                                               float2& g1, float2& g2, float* gp0, float* gp1) {
+  switch (m.kind) {
+    case MASK_SLM: {
+      const float2 w = c_expi(m.m0[pix]);
+      const float2 o = c_mul(e0, w);
+      if (gp0) gp0[pix] = re_conj_mul(g0, make_float2(-o.y, o.x));   // d/dphi = i o
+      g0 = c_mul(g0, make_float2(w.x, -w.y));
+      break;
+    }
+    case MASK_SSLM: {
+      const float2 w0 = c_expi(m.m0[pix]), w1 = c_expi(m.m1[pix]);
+      const float2 o0 = c_mul(e0, w0), o1 = c_mul(e1, w1);
+      if (gp0) gp0[pix] = re_conj_mul(g0, make_float2(-o0.y, o0.x));
+      if (gp1) gp1[pix] = re_conj_mul(g1, make_float2(-o1.y, o1.x));
+      g0 = c_mul(g0, make_float2(w0.x, -w0.y));
+      g1 = c_mul(g1, make_float2(w1.x, -w1.y));
+      break;
+    }
+    case MASK_LCD:
+    case MASK_LCD_SCALAR: {
+      const float eta = m.kind == MASK_LCD ? m.m0[pix] : (float)m.s0;
+      const float th = m.kind == MASK_LCD ? m.m1[pix] : (float)m.s1;
+      float sh, ch, st, ct;
+      sincosf(0.5f * eta, &sh, &ch);
+      sincosf(th, &st, &ct);
+      const float c2 = ct * ct, s2 = st * st, sc = st * ct;
+      const float2 j00 = make_float2(ch * (c2 + s2), -sh * c2 + sh * s2);
+      const float2 j11 = make_float2(ch * (s2 + c2), -sh * s2 + sh * c2);
+      const float2 j01 = make_float2(0.f, -2.f * sh * sc);
+      // d/deta
+      const float2 a00 = make_float2(-0.5f * sh * (c2 + s2), 0.5f * ch * (s2 - c2));
+      const float2 a11 = make_float2(-0.5f * sh * (s2 + c2), 0.5f * ch * (c2 - s2));
+      const float2 a01 = make_float2(0.f, -ch * sc);
+      // d/dtheta: c^2 -> -2sc, s^2 -> 2sc, sc -> c^2 - s^2
+      const float2 b00 = make_float2(ch * (-2.f * sc + 2.f * sc), 4.f * sh * sc);
+      const float2 b11 = make_float2(ch * (2.f * sc - 2.f * sc), -4.f * sh * sc);
+      const float2 b01 = make_float2(0.f, -2.f * sh * (c2 - s2));
+      if (gp0)
+        gp0[pix] = re_conj_mul(g0, c_add(c_mul(a00, e0), c_mul(a01, e1))) +
+                   re_conj_mul(g1, c_add(c_mul(a01, e0), c_mul(a11, e1)));
+      if (gp1)
+        gp1[pix] = re_conj_mul(g0, c_add(c_mul(b00, e0), c_mul(b01, e1))) +
+                   re_conj_mul(g1, c_add(c_mul(b01, e0), c_mul(b11, e1)));
+      const float2 n0 = c_add(c_mul(conjf2(j00), g0), c_mul(conjf2(j01), g1));
+      const float2 n1 = c_add(c_mul(conjf2(j01), g0), c_mul(conjf2(j11), g1));
+      g0 = n0;
+      g1 = n1;
+      break;
+    }
+    case MASK_LP: {
+      float s, c;
+      sincosf(m.m0[pix], &s, &c);
+      if (gp0) {
+        const float d00 = -2.f * s * c, d01 = c * c - s * s, d11 = 2.f * s * c;
+        gp0[pix] = re_conj_mul(g0, c_add(c_scale(e0, d00), c_scale(e1, d01))) +
+                   re_conj_mul(g1, c_add(c_scale(e0, d01), c_scale(e1, d11)));
+      }
+      const float2 n0 = c_add(c_scale(g0, c * c), c_scale(g1, c * s));
+      const float2 n1 = c_add(c_scale(g0, c * s), c_scale(g1, s * s));
+      g0 = n0;
+      g1 = n1;
+      g2 = make_float2(0.f, 0.f);
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+__global__ void pre_backward_kernel(PreArgs a) {
+  const int64_t npix = (int64_t)a.rows * a.cols;
+  for (int64_t pix = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; pix < npix;
+       pix += (int64_t)gridDim.x * blockDim.x) {
+    const int row = (int)(pix / a.cols), col = (int)(pix % a.cols);
+    // forward recompute of the stage inputs
+    float2 x0 = a.in[pix];
+    float2 x1 = a.nin > 1 ? a.in[a.in_cs + pix] : make_float2(0.f, 0.f);
+    float2 x2 = a.nin > 2 ? a.in[2 * a.in_cs + pix] : make_float2(0.f, 0.f);
+    float2 m0 = x0, m1 = x1, m2 = x2;
+    apply_stage(a.mask.st[0], pix, m0, m1, m2);
+    // cotangent of the masked (Ex', Ey', Ez')
+    float2 g0 = a.gout[pix];
+    float2 g1 = a.ncomp > 1 ? a.gout[a.cs + pix] : make_float2(0.f, 0.f);
+    float2 g2 = a.ncomp > 2 ? a.gout[2 * a.cs + pix] : make_float2(0.f, 0.f);
+    if (a.hna) {
+      float A[3][2];
+      hna_map(a.x[col], a.y[row], a.hna_radius, a.hna_f, a.hna_stm2, A);
+      const float2 h0 = c_add(c_add(c_scale(g0, A[0][0]), c_scale(g1, A[1][0])), c_scale(g2, A[2][0]));
+      const float2 h1 = c_add(c_add(c_scale(g0, A[0][1]), c_scale(g1, A[1][1])), c_scale(g2, A[2][1]));
+      g0 = h0; g1 = h1; g2 = make_float2(0.f, 0.f);
+    } else if (a.ez_mode != EZ_NONE) {
+      const double X = a.x[col], Y = a.y[row], z = a.z;
+      float cx, cy;
+      if (a.ez_mode == EZ_HNA) {
+        const double rho = sqrt(X * X + Y * Y);
+        cx = (float)(X / rho); cy = (float)(Y / rho);
+      } else {
+        const double r = sqrt(X * X + Y * Y + z * z);
+        double dx = X / r, dy = Y / r;
+        if (a.ez_mode == EZ_VCZT) { dx *= z / r; dy *= z / r; }
+        cx = (float)dx; cy = (float)dy;
+      }
+      g0 = c_add(g0, c_scale(g2, cx));
+      g1 = c_add(g1, c_scale(g2, cy));
+      g2 = make_float2(0.f, 0.f);
+    }
+    stage_backward(a.mask.st[1], pix, m0, m1, g0, g1, g2, a.gparam[1][0], a.gparam[1][1]);
+    stage_backward(a.mask.st[0], pix, x0, x1, g0, g1, g2, a.gparam[0][0], a.gparam[0][1]);
+    if (a.out) {
+      a.out[pix] = g0;
+      if (a.nin > 1) a.out[a.in_cs + pix] = g1;
+      if (a.nin > 2) a.out[2 * a.in_cs + pix] = g2;
+    }
+  }
+}
+
+__global__ void transfer_function_kernel(float2* out, const double* x, const double* y, int nx, int ny, double z,
+                                         double k) {
+  const int64_t npix = (int64_t)nx * ny;
+  for (int64_t pix = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; pix < npix;
+       pix += (int64_t)gridDim.x * blockDim.x) {
+    const int row = (int)(pix / nx), col = (int)(pix % nx);
+    out[pix] = rs_kernel_value(x[col], y[row], z, k);
+  }
+}
+
+}  // namespace xl

@@ -1,0 +1,397 @@
+// C ABI of libxlumina_hip.so (declared in include/xlumina_hip.h).
+//
+// One translation unit: the kernels live in the headers included here, the
+// entry points below validate shapes on the host (float64 scalar set-up, as the
+// reference's build_grid / build_CZT_grid do in NumPy), carve the caller's
+// workspace and launch on the caller's stream.  Nothing here allocates device
+// memory or synchronises the stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "../../include/xlumina_hip.h"
+#include "czt.h"
+#include "fft.h"
+#include "masks.h"
+#include "pre.h"
+#include "rs.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) return fail(XL_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+  return XL_OK;
+}
+
+constexpr int kMaxDevices = 64;
+std::mutex g_init_mu;
+bool g_init_done[kMaxDevices] = {};
+
+int ensure_init() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return fail(XL_EHIP, "hipGetDevice failed");
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  if (g_init_done[dev]) return XL_OK;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(nullptr, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+    return fail(XL_EINVAL, "xl_init must run before graph capture");
+  hipLaunchKernelGGL(xl::xl_fft_init_kernel, dim3((xl::TWN + 255) / 256), dim3(256), 0, nullptr);
+  int rc = hip_check(hipGetLastError(), "twiddle init launch");
+  if (rc) return rc;
+  rc = hip_check(hipDeviceSynchronize(), "twiddle init");
+  if (rc) return rc;
+  g_init_done[dev] = true;
+  return XL_OK;
+}
+
+bool device_ready() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return false;
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  return g_init_done[dev];
+}
+
+int next_pow2(int64_t v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+xl::MaskArgs to_mask(const xl_mask_stage* m) {
+  xl::MaskArgs a;
+  for (int i = 0; i < 2; ++i) {
+    a.st[i].kind = m[i].kind;
+    a.st[i].m0 = m[i].m0;
+    a.st[i].m1 = m[i].m1;
+    a.st[i].s0 = m[i].s0;
+    a.st[i].s1 = m[i].s1;
+  }
+  return a;
+}
+
+int check_mask(const xl_mask_stage* m, int nin) {
+  for (int i = 0; i < 2; ++i) {
+    const int k = m[i].kind;
+    if (k < XL_MASK_NONE || k > XL_MASK_LP) return fail(XL_EINVAL, "unknown mask kind");
+    if (k == XL_MASK_NONE) continue;
+    if ((k == XL_MASK_SLM || k == XL_MASK_SSLM || k == XL_MASK_LCD || k == XL_MASK_LP) && !m[i].m0)
+      return fail(XL_EINVAL, "mask parameter m0 is NULL");
+    if ((k == XL_MASK_SSLM || k == XL_MASK_LCD) && !m[i].m1) return fail(XL_EINVAL, "mask parameter m1 is NULL");
+    if (k != XL_MASK_SLM && nin < 2) return fail(XL_EINVAL, "polarisation masks need Ex and Ey");
+  }
+  return XL_OK;
+}
+
+// ---------------------------------------------------------------- RS
+int rs_fft_len(int n) { return std::max(64, next_pow2(2 * (int64_t)n - 1)); }
+
+size_t rs_ws(int n, int ncomp) {
+  const int64_t P = rs_fft_len(n);
+  const int64_t nr4 = ((n + 3) / 4) * 4;
+  return (size_t)(ncomp * nr4 * P + (P / 2) * P) * sizeof(float2) + 256;
+}
+
+// ---------------------------------------------------------------- CZT
+struct AxisPlan {
+  int m, M, q;
+  double d1_dm, dl_dm, th_turns;
+};
+
+int plan_axis(int m, int M, double f1, double f2, double Dm, AxisPlan* ap) {
+  if (M < 2) return fail(XL_EINVAL, "CZT output length must be >= 2 (the reference mis-broadcasts M_out=1)");
+  if (m < 2) return fail(XL_EINVAL, "CZT input length must be >= 2");
+  const int64_t mp = (int64_t)m + M - 1;
+  if ((mp & (mp - 1)) == 0)
+    return fail(XL_EINVAL, "CZT with m + M_out - 1 = " + std::to_string(mp) +
+                               " a power of two is rejected (the reference's window slice is one sample short)");
+  const int q = std::max(64, next_pow2(mp));
+  if (q > xl::MAX_FFT)
+    return fail(XL_EINVAL, "CZT needs FFT length " + std::to_string(q) + " > " + std::to_string(xl::MAX_FFT) +
+                               " (m + M_out - 1 must be below 4096)");
+  // wave_optics.py:435-437, :391-393
+  const double D1 = f1 + (M * Dm + f2 - f1) / (2.0 * M);
+  const double D2 = f2 + (M * Dm + f2 - f1) / (2.0 * M);
+  const double delta = (D2 - D1) / M;
+  double th = -2.0 * M_PI * (D2 - D1) / (M * Dm);
+  th = std::atan2(std::sin(th), std::cos(th));   // principal branch of angle(W)
+  ap->m = m;
+  ap->M = M;
+  ap->q = q;
+  ap->d1_dm = D1 / Dm;
+  ap->dl_dm = delta / Dm;
+  ap->th_turns = th / (2.0 * M_PI);
+  return XL_OK;
+}
+
+struct CztPlan {
+  AxisPlan x, y;       // forward axes (x: nx_in -> mx, y: ny_in -> my)
+  int ncomp;
+  int rin, lin_x, lout_x, lout_y;
+  size_t tables_bytes, tiles_bytes;
+};
+
+int plan_czt(const xl_czt_params* p, CztPlan* cp) {
+  if (!p) return fail(XL_EINVAL, "null params");
+  if (p->ny_in < 2 || p->nx_in < 2 || p->my < 1 || p->mx < 1) return fail(XL_EINVAL, "bad CZT plane sizes");
+  if (!(p->dm != 0.0) || !std::isfinite(p->dm)) return fail(XL_EINVAL, "Dm must be finite and non-zero");
+  int rc = plan_axis(p->nx_in, p->mx, p->fx1, p->fx2, p->dm, &cp->x);
+  if (rc) return rc;
+  rc = plan_axis(p->ny_in, p->my, p->fy1, p->fy2, p->dm, &cp->y);
+  if (rc) return rc;
+  cp->ncomp = p->ncomp;
+  if (!p->adjoint) {
+    cp->rin = p->ny_in; cp->lin_x = p->nx_in; cp->lout_x = p->mx; cp->lout_y = p->my;
+  } else {
+    cp->rin = p->my; cp->lin_x = p->mx; cp->lout_x = p->nx_in; cp->lout_y = p->ny_in;
+  }
+  cp->tables_bytes = (size_t)2 * 3 * xl::MAX_FFT * sizeof(float2);
+  const int64_t r4 = ((cp->rin + 3) / 4) * 4, o4 = ((cp->lout_x + 3) / 4) * 4;
+  cp->tiles_bytes = (size_t)cp->ncomp * r4 * o4 * sizeof(float2);
+  return XL_OK;
+}
+
+template <typename F>
+int dispatch_q(int q, F&& f) {
+  switch (q) {
+    case 64: f(std::integral_constant<int, 64>()); return XL_OK;
+    case 128: f(std::integral_constant<int, 128>()); return XL_OK;
+    case 256: f(std::integral_constant<int, 256>()); return XL_OK;
+    case 512: f(std::integral_constant<int, 512>()); return XL_OK;
+    case 1024: f(std::integral_constant<int, 1024>()); return XL_OK;
+    case 2048: f(std::integral_constant<int, 2048>()); return XL_OK;
+    case 4096: f(std::integral_constant<int, 4096>()); return XL_OK;
+    default: return fail(XL_EINVAL, "unsupported FFT length " + std::to_string(q));
+  }
+}
+
+char* align256(char* p) { return (char*)(((uintptr_t)p + 255) & ~(uintptr_t)255); }
+
+}  // namespace
+
+extern "C" {
+
+const char* xl_last_error(void) { return g_err.c_str(); }
+int xl_abi_version(void) { return 1; }
+int xl_max_fft_length(void) { return xl::MAX_FFT; }
+int xl_init(void) { return ensure_init(); }
+
+size_t xl_rs_workspace_bytes(int32_t n, int32_t ncomp) {
+  if (n < 2 || ncomp < 1 || ncomp > 3) return 0;
+  return rs_ws(n, ncomp);
+}
+
+int xl_rs_propagate(const xl_rs_params* p, void* ws, size_t ws_bytes, void* stream) {
+  if (!p) return fail(XL_EINVAL, "null params");
+  if (!device_ready()) return fail(XL_EINVAL, "xl_init() has not run on this device");
+  const int n = p->n;
+  if (n < 2) return fail(XL_EINVAL, "grid side must be >= 2");
+  const int P = rs_fft_len(n);
+  if (P > xl::MAX_FFT) return fail(XL_EINVAL, "grid side " + std::to_string(n) + " needs FFT length > 4096");
+  if (p->nin < 1 || p->nin > 3) return fail(XL_EINVAL, "nin must be 1..3");
+  if (p->ez_mode != XL_EZ_NONE && p->ez_mode != XL_EZ_VRS) return fail(XL_EINVAL, "RS supports XL_EZ_NONE / XL_EZ_VRS");
+  if (p->ez_mode == XL_EZ_VRS && p->nin < 2) return fail(XL_EINVAL, "Ez construction needs Ex and Ey");
+  if (!p->in || !p->out || !p->x || !p->y) return fail(XL_EINVAL, "null pointer argument");
+  if (int rc = check_mask(p->mask, p->nin)) return rc;
+  if (p->adjoint && (p->mask[0].kind || p->mask[1].kind || p->ez_mode))
+    return fail(XL_EINVAL, "the adjoint convolution takes no masks / Ez construction (use xl_pre_backward)");
+  const int ncomp = p->ez_mode == XL_EZ_VRS ? 3 : p->nin;
+  const size_t need = rs_ws(n, ncomp);
+  if (!ws || ws_bytes < need) return fail(XL_EWORKSPACE, "RS workspace too small: need " + std::to_string(need));
+  char* w = align256((char*)ws);
+  const int64_t nr4 = ((n + 3) / 4) * 4;
+  xl::RsArgs a;
+  a.in = (const float2*)p->in;
+  a.in_cs = p->in_comp_stride;
+  a.out = (float2*)p->out;
+  a.out_cs = p->out_comp_stride;
+  a.intensity = p->intensity;
+  a.t = (float2*)w;
+  a.hr = a.t + ncomp * nr4 * P;
+  a.x = p->x;
+  a.y = p->y;
+  a.z = p->z;
+  a.k = p->k;
+  a.n = n;
+  a.nin = p->nin;
+  a.ncomp = ncomp;
+  a.ez_mode = p->ez_mode;
+  a.adjoint = p->adjoint;
+  a.mask = to_mask(p->mask);
+  a.scale = (float)(p->dx * p->dy / ((double)P * (double)P));
+  hipStream_t st = (hipStream_t)stream;
+  int rc = dispatch_q(P, [&](auto c) { (void)xl::rs_launch<decltype(c)::value>(a, st); });
+  if (rc) return rc;
+  return hip_check(hipGetLastError(), "RS launch");
+}
+
+size_t xl_czt_workspace_bytes(const xl_czt_params* p) {
+  CztPlan cp;
+  if (plan_czt(p, &cp)) return 0;
+  return cp.tables_bytes + cp.tiles_bytes + 512;
+}
+
+int xl_czt(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream) {
+  if (!device_ready()) return fail(XL_EINVAL, "xl_init() has not run on this device");
+  CztPlan cp;
+  if (int rc = plan_czt(p, &cp)) return rc;
+  if (!p->in || !p->out || !p->x || !p->y || !p->xout || !p->yout) return fail(XL_EINVAL, "null pointer argument");
+  if (p->nin < 1 || p->nin > 3 || p->ncomp < 1 || p->ncomp > 3) return fail(XL_EINVAL, "nin / ncomp must be 1..3");
+  if (p->mode < XL_CZT_RS || p->mode > XL_CZT_PLAIN) return fail(XL_EINVAL, "unknown CZT mode");
+  if (p->mode == XL_CZT_HNA && !p->adjoint && (p->nin < 2 || p->ncomp != 3))
+    return fail(XL_EINVAL, "high-NA CZT takes Ex, Ey and returns 3 components");
+  if (p->ez_mode != XL_EZ_NONE && p->ez_mode != XL_EZ_VCZT) return fail(XL_EINVAL, "CZT supports XL_EZ_NONE / XL_EZ_VCZT");
+  if (!p->adjoint && p->ez_mode == XL_EZ_VCZT && (p->nin < 2 || p->ncomp != 3))
+    return fail(XL_EINVAL, "Ez construction needs Ex, Ey and 3 output components");
+  if (!p->adjoint && p->ez_mode == XL_EZ_NONE && p->mode != XL_CZT_HNA && p->ncomp != p->nin)
+    return fail(XL_EINVAL, "ncomp must equal nin without Ez construction");
+  if (p->adjoint && (p->mask[0].kind || p->mask[1].kind || p->ez_mode || p->nin != p->ncomp))
+    return fail(XL_EINVAL, "the adjoint CZT takes ncomp cotangents and no masks / Ez (use xl_pre_backward)");
+  if (int rc = check_mask(p->mask, p->nin)) return rc;
+  const size_t need = cp.tables_bytes + cp.tiles_bytes + 512;
+  if (!ws || ws_bytes < need) return fail(XL_EWORKSPACE, "CZT workspace too small: need " + std::to_string(need));
+  char* w = align256((char*)ws);
+  float2* tab = (float2*)w;
+  float2* tiles = (float2*)align256((char*)(tab + 6 * xl::MAX_FFT));
+  hipStream_t st = (hipStream_t)stream;
+
+  // axis tables: x in tab[0..3Q), y in tab[3Q..6Q)
+  xl::CztArgs a;
+  std::memset(&a, 0, sizeof(a));
+  const AxisPlan* axp[2] = {&cp.x, &cp.y};
+  xl::CztAxis* axo[2] = {&a.ax, &a.ay};
+  for (int i = 0; i < 2; ++i) {
+    const AxisPlan& ap = *axp[i];
+    xl::CztSetupArgs s;
+    s.pre = tab + i * 3 * xl::MAX_FFT;
+    s.gf = s.pre + xl::MAX_FFT;
+    s.post = s.gf + xl::MAX_FFT;
+    s.m = ap.m;
+    s.M = ap.M;
+    s.q = ap.q;
+    s.adjoint = p->adjoint;
+    s.d1_dm = ap.d1_dm;
+    s.dl_dm = ap.dl_dm;
+    s.th_turns = ap.th_turns;
+    if (int rc = dispatch_q(ap.q, [&](auto c) { xl::czt_setup_launch<decltype(c)::value>(s, st); })) return rc;
+    xl::CztAxis& ax = *axo[i];
+    ax.q = ap.q;
+    ax.pre = s.pre;
+    ax.gf = s.gf;
+    ax.post = s.post;
+    ax.lin = p->adjoint ? ap.M : ap.m;
+    ax.lout = p->adjoint ? ap.m : ap.M;
+    ax.woff = p->adjoint ? 0 : ap.m;
+  }
+  a.in = (const float2*)p->in;
+  a.in_cs = p->in_comp_stride;
+  a.nin = p->nin;
+  a.out = (float2*)p->out;
+  a.out_cs = p->out_comp_stride;
+  a.t = tiles;
+  a.rin = cp.rin;
+  a.ncomp = p->ncomp;
+  a.ez_mode = p->adjoint ? XL_EZ_NONE : p->ez_mode;
+  a.z = p->z;
+  a.k = p->k;
+  a.hna_radius = p->hna_radius;
+  a.hna_f = p->hna_f;
+  a.hna_stm2 = p->hna_radius * p->hna_radius / (p->hna_radius * p->hna_radius + p->hna_f * p->hna_f);
+  a.mask = to_mask(p->mask);
+  const float2 cte = make_float2((float)p->cte_re, (float)p->cte_im);
+  const float2 ctec = make_float2((float)p->cte_re, (float)-p->cte_im);
+  if (!p->adjoint) {
+    a.px = p->x; a.py = p->y; a.qx = p->xout; a.qy = p->yout;
+    a.cte = cte;
+    if (p->mode == XL_CZT_RS) { a.pre_mode = xl::PRE_RS; a.post_mode = xl::POST_RS; }
+    else if (p->mode == XL_CZT_HNA) { a.pre_mode = xl::PRE_HNA; a.post_mode = xl::POST_CONST; }
+    else { a.pre_mode = xl::PRE_NONE; a.post_mode = xl::POST_NONE; }
+  } else {
+    a.px = p->xout; a.py = p->yout; a.qx = p->x; a.qy = p->y;
+    a.cte = ctec;
+    if (p->mode == XL_CZT_RS) { a.pre_mode = xl::PRE_CONJ_RS; a.post_mode = xl::POST_CONJ_RS; }
+    else if (p->mode == XL_CZT_HNA) { a.pre_mode = xl::PRE_CONST; a.post_mode = xl::POST_NONE; }
+    else { a.pre_mode = xl::PRE_NONE; a.post_mode = xl::POST_NONE; }
+  }
+  if (int rc = dispatch_q(a.ax.q, [&](auto c) { xl::czt_rows_launch<decltype(c)::value>(a, st); })) return rc;
+  if (int rc = dispatch_q(a.ay.q, [&](auto c) { xl::czt_cols_launch<decltype(c)::value>(a, st); })) return rc;
+  return hip_check(hipGetLastError(), "CZT launch");
+}
+
+static int pre_common(const xl_pre_params* p, xl::PreArgs* a) {
+  if (!p) return fail(XL_EINVAL, "null params");
+  if (!p->in || !p->x || !p->y) return fail(XL_EINVAL, "null pointer argument");
+  if (p->rows < 1 || p->cols < 1 || p->nin < 1 || p->nin > 3 || p->ncomp < 1 || p->ncomp > 3)
+    return fail(XL_EINVAL, "bad pre-op shape");
+  if (p->ez_mode < XL_EZ_NONE || p->ez_mode > XL_EZ_HNA) return fail(XL_EINVAL, "unknown Ez mode");
+  if ((p->ez_mode != XL_EZ_NONE || p->hna) && (p->nin < 2 || p->ncomp != 3))
+    return fail(XL_EINVAL, "Ez construction / lens map need Ex, Ey and 3 output components");
+  if (int rc = check_mask(p->mask, p->nin)) return rc;
+  std::memset(a, 0, sizeof(*a));
+  a->in = (const float2*)p->in;
+  a->in_cs = p->in_comp_stride;
+  a->nin = p->nin;
+  a->ncomp = p->ncomp;
+  a->gout = (const float2*)p->gout;
+  a->out = (float2*)p->out;
+  a->cs = p->comp_stride;
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) a->gparam[i][j] = p->gparam[i][j];
+  a->rows = p->rows;
+  a->cols = p->cols;
+  a->ez_mode = p->ez_mode;
+  a->hna = p->hna;
+  a->x = p->x;
+  a->y = p->y;
+  a->z = p->z;
+  a->hna_radius = p->hna_radius;
+  a->hna_f = p->hna_f;
+  a->hna_stm2 = p->hna_radius * p->hna_radius / (p->hna_radius * p->hna_radius + p->hna_f * p->hna_f);
+  a->mask = to_mask(p->mask);
+  return XL_OK;
+}
+
+static dim3 grid_for(int64_t npix) {
+  int64_t b = (npix + 255) / 256;
+  if (b > 65536) b = 65536;
+  return dim3((unsigned)std::max<int64_t>(b, 1));
+}
+
+int xl_pre_forward(const xl_pre_params* p, void* stream) {
+  xl::PreArgs a;
+  if (int rc = pre_common(p, &a)) return rc;
+  if (!p->out) return fail(XL_EINVAL, "null output");
+  hipLaunchKernelGGL(xl::pre_forward_kernel, grid_for((int64_t)p->rows * p->cols), dim3(256), 0,
+                     (hipStream_t)stream, a);
+  return hip_check(hipGetLastError(), "pre_forward launch");
+}
+
+int xl_pre_backward(const xl_pre_params* p, void* stream) {
+  xl::PreArgs a;
+  if (int rc = pre_common(p, &a)) return rc;
+  if (!p->gout) return fail(XL_EINVAL, "null cotangent");
+  hipLaunchKernelGGL(xl::pre_backward_kernel, grid_for((int64_t)p->rows * p->cols), dim3(256), 0,
+                     (hipStream_t)stream, a);
+  return hip_check(hipGetLastError(), "pre_backward launch");
+}
+
+int xl_transfer_function_rs(void* out, const double* x, const double* y, int32_t nx, int32_t ny, double z, double k,
+                            void* stream) {
+  if (!out || !x || !y || nx < 1 || ny < 1) return fail(XL_EINVAL, "bad transfer-function arguments");
+  hipLaunchKernelGGL(xl::transfer_function_kernel, grid_for((int64_t)nx * ny), dim3(256), 0, (hipStream_t)stream,
+                     (float2*)out, x, y, nx, ny, z, k);
+  return hip_check(hipGetLastError(), "transfer function launch");
+}
+
+}  // extern "C"
