@@ -1,0 +1,192 @@
+#!/usr/bin/env python
+"""Benchmark of the light-propagation hot path on one MI355X.
+
+Headline workload (one step): 2048^2 radially polarised beam through a
+per-pixel LCD retarder, VRS to z = 30 mm (Ex, Ey, Ez complex64), then VCZT
+zoom to a 512^2 ROI of 0.2 mm -- one full VRS + VCZT chain, inputs resident
+in HBM.  `value` = chains per second on the one GPU.
+
+Also reported (`secondary`): scalar RS 1024^2 (SLM fused, field + |E|^2)
+propagations/s and high-NA 2048^2 -> 256^2 forward+backward steps/s.
+
+`roofline` covers the kernel with the largest share of the timed region,
+timed live with HIP events recorded on its launch stream; `cpu_baseline` is
+the float64 NumPy/SciPy restatement of the reference algorithm (the oracle)
+on the host cores for a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "2048² VRS+CZT propagations/s & fwd+bwd steps/s vs plain FP32; p50/p99 latency"
+
+
+def kernel_bytes(name, n, P, ncomp, nin, mask_params, m_out, czt_rows_in, czt_cols_in, czt_nin):
+    """Algorithmic HBM bytes of one launch (streaming model: complex64 8 B,
+    float32 mask 4 B, each array read or written once per pass)."""
+    c8 = 8
+    if name == "rs_hrow":
+        return (P // 2) * P * c8
+    if name == "rs_urow":
+        return nin * n * n * c8 + mask_params * n * n * 4 + ncomp * n * P * c8
+    if name == "rs_col":
+        return (P // 2) * P * c8 + 2 * ncomp * n * P * c8
+    if name == "rs_final":
+        return ncomp * n * P * c8 + ncomp * n * n * c8
+    if name == "czt_rows":
+        return czt_nin * czt_rows_in * czt_cols_in * c8 + 3 * czt_rows_in * m_out * c8
+    if name == "czt_cols":
+        return 3 * czt_rows_in * m_out * c8 + 3 * m_out * m_out * c8
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the propagation path shards "
+              "independent requests / candidates with no exchange, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from xlumina_amd import _native as N
+    from xlumina_amd import workloads
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    N.ensure_device(dev)
+
+    w = workloads.vrs_vczt_2048(dev)
+    geom = w.geometry()
+    n = w.field.shape[-1]
+    P = 1 << (2 * n - 1).bit_length()
+    for _ in range(args.warmup):
+        w.run(geom)
+    torch.cuda.synchronize()
+    N.timing_enable(capacity=16 * (args.steps + 2))
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    s.record()
+    for _ in range(args.steps):
+        w.run(geom)
+    e.record()
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(e) / args.steps
+    kt = N.timing_collect()
+    N.timing_disable()
+    value = 1000.0 / ms
+
+    # roofline of the kernel with the largest share of the timed region
+    dom = max(kt.items(), key=lambda kv: kv[1][1])
+    name, (launches, total_ms) = dom
+    per_launch_ms = total_ms / launches
+    nbytes = kernel_bytes(name, n, P, 3, 2, 2, len(w.xout), n, n, 2)
+    achieved = nbytes / (per_launch_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes": nbytes, "avg_launch_us": round(per_launch_ms * 1e3, 2),
+                "kernels_ms_per_step": {k: round(v[1] / args.steps, 4) for k, v in kt.items()}}
+
+    secondary = {}
+    if not args.no_secondary:
+        secondary = run_secondary(dev, workloads, args)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(w)
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "VRS+VCZT chains/s (2048^2 -> 512^2)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "vrs_vczt_2048: 2048^2 radial beam, lambda 635 nm, window 2.4 mm, per-pixel LCD "
+                               "(seeded eta, theta U[0,2pi]), VRS z=30 mm, VCZT z=30 mm to 512^2 ROI of 0.2 mm",
+                   "form": "plain FP32 (LDS radix-16 FFT RS + Bluestein CZT)", "grid": n, "fft_len": P},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "secondary": secondary,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def _time(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(steps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / steps
+
+
+def run_secondary(dev, workloads, args):
+    import torch
+    out = {}
+    rs = workloads.scalar_rs_1024(dev)
+    ms = _time(lambda: rs.run(), max(10, args.steps), 3)
+    out["rs_1024_slm"] = {"value": round(1000.0 / ms, 2), "unit": "propagations/s", "ms": round(ms, 4)}
+    h = workloads.high_na_2048(dev)
+    geom = h.geometry()
+    alpha = h.alpha.clone().requires_grad_(True)
+    phi = h.phi.clone().requires_grad_(True)
+
+    def step():
+        L = h.loss(alpha, phi, geom)
+        L.backward()
+
+    ms = _time(step, max(5, args.steps // 4), 2)
+    out["high_na_fwd_bwd_2048_to_256"] = {"value": round(1000.0 / ms, 2), "unit": "fwd+bwd steps/s",
+                                          "ms": round(ms, 4), "form": "plain FP32 Bluestein"}
+    return out
+
+
+def cpu_baseline(w):
+    """One VRS + VCZT chain of the same workload through the float64 oracle."""
+    import numpy as np
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import xlumina_ref as R
+    cores = len(os.sched_getaffinity(0))
+    R.set_workers(cores)
+    Ex = w.field[0].cpu().numpy().astype(np.complex128)
+    Ey = w.field[1].cpu().numpy().astype(np.complex128)
+    eta = w.eta.cpu().numpy().astype(np.float64)
+    th = w.theta.cpu().numpy().astype(np.float64)
+    t0 = time.perf_counter()
+    mx, my, _ = R.LCD(Ex, Ey, None, eta, th)
+    (vx, vy, _), _ = R.vrs_propagation(mx, my, w.x, w.y, w.wavelength, w.z_vrs)
+    R.vczt(vx, vy, w.x, w.y, w.wavelength, w.z_czt, w.xout, w.yout)
+    dt = time.perf_counter() - t0
+    return {"value": round(1.0 / dt, 5), "unit": "VRS+VCZT chains/s", "cores": cores, "kind": "port",
+            "sample": f"1 chain at 2048^2 -> 512^2, complex128, scipy.fft workers={cores}, {dt:.1f} s"}
+
+
+if __name__ == "__main__":
+    main()

@@ -150,6 +150,19 @@ int xl_pre_backward(const xl_pre_params* p, void* stream);
 int xl_transfer_function_rs(void* out, const double* x, const double* y, int32_t nx, int32_t ny, double z,
                             double k, void* stream);
 
+/* Opt-in per-kernel timing: HIP events recorded on the launch stream around
+ * every kernel launch of the calls that follow (capacity = launches kept).
+ * xl_timing_collect waits on them, writes per-kernel totals (up to
+ * max_entries) and returns the number of distinct kernels (negative on error);
+ * it empties the record list.  Keep it off during graph capture. */
+typedef struct xl_kernel_time {
+  char name[32];
+  int64_t launches;
+  double total_ms;
+} xl_kernel_time;
+int xl_timing_enable(int32_t enable, int32_t capacity);
+int xl_timing_collect(xl_kernel_time* out, int32_t max_entries);
+
 #ifdef __cplusplus
 }
 #endif

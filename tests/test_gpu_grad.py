@@ -1,0 +1,165 @@
+"""Hand-written adjoints behind torch.autograd (GPU).
+
+Pins: the dot-product identity <A x, y> = <x, A^H y> for every propagator
+(with masks fused), and central finite differences of float64 oracle losses
+(eps = 1e-3) on 64 seeded mask pixels.  The reference's jax.grad cannot run
+here (JAX is absent); finite differences of the float64 restatement pin the
+same derivative.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import xlumina_ref as R
+from tests.conftest import rel_l2
+from xlumina_amd import _native as N
+from xlumina_amd import autograd as AG
+from xlumina_amd import ops, workloads
+from xlumina_amd.ops import MaskSpec
+
+pytestmark = pytest.mark.gpu
+
+
+def c128(t):
+    return t.detach().to(torch.complex128).flatten()
+
+
+def dot_identity(fn, x, cot):
+    """Re<fn(x), cot> vs Re<x, grad> with grad from autograd (= A^H cot)."""
+    x = x.clone().requires_grad_(True)
+    y = fn(x)
+    L = torch.real(torch.vdot(cot.flatten(), y.flatten()))
+    (g,) = torch.autograd.grad(L, x)
+    lhs = torch.vdot(c128(cot), c128(y))
+    rhs = torch.vdot(c128(g), c128(x))
+    return abs(complex(lhs) - complex(rhs)) / abs(complex(lhs))
+
+
+def test_rs_adjoint_identity(cuda_dev):
+    n = 256
+    x, y = R.space(300.0, n)
+    a = torch.randn((1, n, n), dtype=torch.complex64, device=cuda_dev)
+    c = torch.randn((1, n, n), dtype=torch.complex64, device=cuda_dev)
+    assert dot_identity(lambda u: AG.rs(u, x, y, 2000.0, 0.65), a, c) < 1e-5
+
+
+def test_czt_adjoint_identity(cuda_dev):
+    n = 200
+    x, y = R.space(300.0, n)
+    xo, yo = np.linspace(-30, 30, 77), np.linspace(-20, 25, 50)
+    g = ops.czt_geometry(x, y, xo, yo, 4000.0, 0.65)
+    a = torch.randn((1, n, n), dtype=torch.complex64, device=cuda_dev)
+    c = torch.randn((1, 50, 77), dtype=torch.complex64, device=cuda_dev)
+    assert dot_identity(lambda u: AG.czt(u, g), a, c) < 1e-5
+
+
+def test_vrs_with_masks_adjoint_identity(cuda_dev):
+    n = 128
+    rng = np.random.default_rng(0)
+    x, y = R.space(300.0, n)
+    a, p = (torch.tensor(v, dtype=torch.float32, device=cuda_dev) for v in rng.uniform(-3, 3, (2, n, n)))
+    masks = [MaskSpec(N.XL_MASK_SSLM, a, p), MaskSpec(N.XL_MASK_LCD_SCALAR, None, None, 0.3, 1.2)]
+    E = torch.randn((2, n, n), dtype=torch.complex64, device=cuda_dev)
+    c = torch.randn((3, n, n), dtype=torch.complex64, device=cuda_dev)
+    assert dot_identity(lambda u: AG.rs(u, x, y, 2000.0, 0.65, masks=masks, ez_mode=N.XL_EZ_VRS, nin=2), E, c) < 1e-5
+
+
+def test_hna_adjoint_identity(cuda_dev):
+    n = 128
+    x, y = R.space(1800.0, n)
+    xo = np.linspace(-2, 2, 40)
+    g = ops.hna_geometry(x, y, xo, xo, 0.65, 1800.0, 2000.0)
+    E = torch.randn((2, n, n), dtype=torch.complex64, device=cuda_dev)
+    c = torch.randn((3, 40, 40), dtype=torch.complex64, device=cuda_dev)
+    assert dot_identity(lambda u: AG.czt(u, g, nin=2), E, c) < 1e-5
+
+
+def _oracle_hna_loss(w, alpha, phi):
+    ex, ey = w.field[0].cpu().numpy().astype(np.complex128), w.field[1].cpu().numpy().astype(np.complex128)
+    sx, sy, _ = R.sSLM(ex, ey, None, alpha, phi)
+    lx, ly, _ = R.LCD(sx, sy, None, w.eta, w.theta)
+    E = R.vczt_objective_lens(lx, ly, w.x, w.y, w.wavelength, w.radius, w.f, w.xout, w.yout)
+    I = np.sum(np.abs(E) ** 2, 0)
+    U, V = np.meshgrid(w.xout, w.yout)
+    return float(np.sum(I * (U ** 2 + V ** 2)) / np.sum(I))
+
+
+def test_high_na_grad_vs_central_fd_64_pixels(cuda_dev):
+    """Second-moment loss through sSLM + LCD + VCZT_objective_lens (reduced
+    128^2 pupil so 128 float64 oracle evaluations stay fast)."""
+    w = workloads.high_na_2048(cuda_dev, n=128, m_out=64)
+    alpha = w.alpha.clone().requires_grad_(True)
+    phi = w.phi.clone().requires_grad_(True)
+    L = w.loss(alpha, phi)
+    ga, gp = torch.autograd.grad(L, (alpha, phi))
+    a0 = w.alpha.cpu().numpy().astype(np.float64)
+    p0 = w.phi.cpu().numpy().astype(np.float64)
+    assert abs(L.item() - _oracle_hna_loss(w, a0, p0)) < 1e-5 * abs(L.item())
+    rng = np.random.default_rng(4)
+    pix = rng.choice(128 * 128, 64, replace=False)
+    eps = 1e-3
+    fd_a, fd_p = [], []
+    for q in pix:
+        i, j = divmod(int(q), 128)
+        for arr, out in ((a0, fd_a), (p0, fd_p)):
+            arr[i, j] += eps
+            lp = _oracle_hna_loss(w, a0, p0)
+            arr[i, j] -= 2 * eps
+            lm = _oracle_hna_loss(w, a0, p0)
+            arr[i, j] += eps
+            out.append((lp - lm) / (2 * eps))
+    got_a = ga.cpu().numpy().ravel()[pix]
+    got_p = gp.cpu().numpy().ravel()[pix]
+    assert rel_l2(got_a, np.array(fd_a)) < 2e-3
+    assert rel_l2(got_p, np.array(fd_p)) < 2e-3
+
+
+def test_high_na_grad_full_size_directional(cuda_dev):
+    """Full 2048^2 pupil, 256^2 focal window: the autograd gradient along a
+    random direction on 64 pixels vs a float64 central difference."""
+    w = workloads.high_na_2048(cuda_dev)
+    alpha = w.alpha.clone().requires_grad_(True)
+    L = w.loss(alpha, w.phi)
+    (ga,) = torch.autograd.grad(L, (alpha,))
+    rng = np.random.default_rng(4)
+    pix = rng.choice(2048 * 2048, 64, replace=False)
+    v = np.zeros(2048 * 2048)
+    v[pix] = rng.standard_normal(64)
+    v = v.reshape(2048, 2048)
+    a0 = w.alpha.cpu().numpy().astype(np.float64)
+    p0 = w.phi.cpu().numpy().astype(np.float64)
+    eps = 1e-3
+    fd = (_oracle_hna_loss(w, a0 + eps * v, p0) - _oracle_hna_loss(w, a0 - eps * v, p0)) / (2 * eps)
+    got = float((ga.double().cpu().numpy() * v).sum())
+    assert abs(got - fd) < 5e-3 * abs(fd) + 1e-9
+
+
+def test_rs_slm_grad_vs_fd(cuda_dev):
+    n = 64
+    x, y = R.space(100.0, n)
+    rng = np.random.default_rng(9)
+    u = R.gaussian_beam(x, y, 0.65, (40.0, 40.0), 1.0)
+    phi0 = rng.uniform(-np.pi, np.pi, (n, n)).astype(np.float32)
+    tgt = rng.random((n, n))
+    field = torch.from_numpy(u.astype(np.complex64))[None].to(cuda_dev)
+    phi = torch.from_numpy(phi0).to(cuda_dev).requires_grad_(True)
+    out = AG.rs(field, x, y, 500.0, 0.65, masks=[MaskSpec(N.XL_MASK_SLM, phi)])
+    L = ((out[0].abs() ** 2 - torch.from_numpy(tgt).float().to(cuda_dev)) ** 2).mean()
+    (g,) = torch.autograd.grad(L, phi)
+
+    def oracle_loss(ph):
+        o, _ = R.rs_propagation(u * np.exp(1j * ph), x, y, 0.65, 500.0)
+        return float(np.mean((np.abs(o) ** 2 - tgt) ** 2))
+
+    p = phi0.astype(np.float64)
+    pix = rng.choice(n * n, 32, replace=False)
+    fd = []
+    for q in pix:
+        i, j = divmod(int(q), n)
+        p[i, j] += 1e-3
+        lp = oracle_loss(p)
+        p[i, j] -= 2e-3
+        lm = oracle_loss(p)
+        p[i, j] += 1e-3
+        fd.append((lp - lm) / 2e-3)
+    assert rel_l2(g.cpu().numpy().ravel()[pix], np.array(fd)) < 2e-3

@@ -1,0 +1,101 @@
+"""The C ABI library: loads without a GPU, exports every declared symbol, and
+the ctypes structures match the C layout (host-only calls, no kernels)."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+from tests.conftest import ROOT
+from xlumina_amd import _native as N
+from xlumina_amd import build_lib
+
+HEADER = os.path.join(ROOT, "include", "xlumina_hip.h")
+
+
+def header_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(xl_\w+)\s*\(", src)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(build_lib.lib_path()):
+        pytest.fail("libxlumina_hip.so is not built (run __graft_entry__.build())")
+    return N.load()
+
+
+def test_every_header_symbol_is_exported_and_bound(lib):
+    names = header_functions()
+    assert len(names) >= 10
+    for name in names:
+        assert hasattr(lib, name), name
+        assert name in N.SIGNATURES, f"{name} has no ctypes signature"
+    assert set(N.SIGNATURES) == set(names)
+
+
+def test_struct_layout_matches_c():
+    code = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "xlumina_hip.h"
+int main(void) {
+  printf("%zu %zu %zu %zu\n", sizeof(xl_mask_stage), sizeof(xl_rs_params), sizeof(xl_czt_params), sizeof(xl_pre_params));
+  printf("%zu %zu %zu\n", offsetof(xl_rs_params, mask), offsetof(xl_czt_params, mask), offsetof(xl_pre_params, mask));
+  printf("%zu %zu %zu\n", offsetof(xl_rs_params, dx), offsetof(xl_czt_params, hna_f), offsetof(xl_pre_params, gparam));
+  return 0;
+}
+'''
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "layout.c")
+        exe = os.path.join(d, "layout")
+        open(src, "w").write(code)
+        subprocess.run(["gcc", "-std=c99", "-I", os.path.dirname(HEADER), src, "-o", exe], check=True)
+        lines = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split("\n")
+    sizes = [int(v) for v in lines[0].split()]
+    offs = [int(v) for v in lines[1].split()]
+    offs2 = [int(v) for v in lines[2].split()]
+    assert sizes == [C.sizeof(N.MaskStage), C.sizeof(N.RsParams), C.sizeof(N.CztParams), C.sizeof(N.PreParams)]
+    assert offs == [N.RsParams.mask.offset, N.CztParams.mask.offset, N.PreParams.mask.offset]
+    assert offs2 == [N.RsParams.dx.offset, N.CztParams.hna_f.offset, N.PreParams.gparam.offset]
+
+
+def test_host_side_queries(lib):
+    assert lib.xl_abi_version() == 1
+    assert lib.xl_max_fft_length() == 4096
+    # RS workspace: ncomp * round4(n) * P + (P/2) * P complex64 (+ alignment slack)
+    n, P = 1024, 2048
+    assert lib.xl_rs_workspace_bytes(n, 1) == (n * P + (P // 2) * P) * 8 + 256
+    assert lib.xl_rs_workspace_bytes(1, 1) == 0
+    assert lib.xl_rs_workspace_bytes(4096, 1) > 0   # rejected later by the propagate call (P > 4096)
+
+
+def _czt_params(m, M):
+    p = N.CztParams()
+    p.nin = p.ncomp = 1
+    p.ny_in = p.nx_in = m
+    p.my = p.mx = M
+    p.dm = 100.0
+    p.fy1 = p.fx1 = 40.0
+    p.fy2 = p.fx2 = 60.0
+    return p
+
+
+def test_czt_plan_rejects_reference_failures(lib):
+    assert lib.xl_czt_workspace_bytes(C.byref(_czt_params(2048, 512))) > 0
+    p = _czt_params(2048, 2049)           # m + M - 1 = 4096: the reference raises
+    assert lib.xl_czt_workspace_bytes(C.byref(p)) == 0
+    assert b"power of two" in lib.xl_last_error()
+    assert lib.xl_czt_workspace_bytes(C.byref(_czt_params(64, 1))) == 0   # M_out = 1
+    assert lib.xl_czt_workspace_bytes(C.byref(_czt_params(4000, 200))) == 0   # FFT length > 4096
+
+
+def test_entry_points_refuse_without_init(lib):
+    # no kernel is launched: the call stops at the initialisation check
+    p = N.RsParams()
+    p.n = 64
+    rc = lib.xl_rs_propagate(C.byref(p), None, 0, None)
+    assert rc != 0

@@ -58,6 +58,10 @@ class PreParams(C.Structure):
                 ("z", C.c_double), ("hna_radius", C.c_double), ("hna_f", C.c_double), ("mask", MaskStage * 2)]
 
 
+class KernelTime(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_int64), ("total_ms", C.c_double)]
+
+
 # symbol -> (restype, argtypes); every declaration of include/xlumina_hip.h
 SIGNATURES = {
     "xl_last_error": (C.c_char_p, []),
@@ -72,6 +76,8 @@ SIGNATURES = {
     "xl_pre_backward": (C.c_int, [C.POINTER(PreParams), C.c_void_p]),
     "xl_transfer_function_rs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
                                           C.c_double, C.c_double, C.c_void_p]),
+    "xl_timing_enable": (C.c_int, [C.c_int32, C.c_int32]),
+    "xl_timing_collect": (C.c_int, [C.POINTER(KernelTime), C.c_int32]),
 }
 
 _lib = None
@@ -120,3 +126,22 @@ def ensure_device(device) -> None:
     with torch.cuda.device(idx):
         check(lib.xl_init())
     _inited_devices.add(idx)
+
+
+def timing_enable(capacity: int = 8192) -> None:
+    """Record HIP events around every kernel launch from now on."""
+    check(load().xl_timing_enable(1, capacity))
+
+
+def timing_disable() -> None:
+    check(load().xl_timing_enable(0, 0))
+
+
+def timing_collect() -> dict:
+    """{kernel name: (launches, total ms)} of the launches since the last collect."""
+    lib = load()
+    buf = (KernelTime * 32)()
+    n = lib.xl_timing_collect(buf, 32)
+    if n < 0:
+        check(-n)
+    return {buf[i].name.decode(): (int(buf[i].launches), float(buf[i].total_ms)) for i in range(min(n, 32))}

@@ -1,0 +1,233 @@
+"""torch.autograd.Function wrappers with hand-written adjoints.
+
+Each propagator is linear in the field, so its backward is the adjoint
+operator run by the same kernels:
+  * RS / VRS: convolution with conj(h) (the RS kernel array is even, so the
+    correlation is a convolution with the conjugated transform);
+  * CZT / VCZT / high-NA: conj(F) (K_y^H (conj(F0 c) G) conj(K_x)) through the
+    Bluestein kernels with adjoint tables;
+  * fused Jones masks / Ez construction / lens map: xl_pre_backward, which
+    also returns the cotangents of the per-pixel mask parameters.
+Only the inputs (field, mask parameters) are saved; masked intermediates are
+recomputed in the backward pass, so a chain keeps one field per stage alive.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Optional, Sequence, Tuple
+
+import torch
+
+from . import _native as N
+from . import ops
+from .ops import MaskSpec
+
+
+def _mask_structure(masks: Sequence[MaskSpec]):
+    """Static part of a mask chain (kinds and scalar values)."""
+    return tuple((m.kind, m.s0, m.s1) for m in masks)
+
+
+def _rebuild(struct, tensors):
+    out = []
+    for i, (kind, s0, s1) in enumerate(struct):
+        out.append(MaskSpec(kind, tensors[2 * i], tensors[2 * i + 1], s0, s1))
+    return out
+
+
+def _flat_params(masks: Sequence[MaskSpec]):
+    flat = [None, None, None, None]
+    for i, m in enumerate(masks[:2]):
+        flat[2 * i], flat[2 * i + 1] = m.p0, m.p1
+    return flat
+
+
+def _param_grads(ctx, struct, params, gp):
+    """Reduce per-pixel parameter cotangents to each parameter's shape."""
+    outs = []
+    for i in range(2):
+        for j in range(2):
+            p = params[2 * i + j]
+            if p is None or not ctx.needs_input_grad[2 + 2 * i + j]:
+                outs.append(None)
+                continue
+            g = gp[i][j]
+            if g is None:
+                outs.append(None)
+                continue
+            if g.shape != p.shape:
+                g = g.sum() if p.dim() == 0 else g.sum_to_size(p.shape)
+            outs.append(g.to(p.dtype))
+    return outs
+
+
+@dataclass
+class RSConfig:
+    x: object
+    y: object
+    z: float
+    wavelength: float
+    ez_mode: int
+    nin: int
+    mask_struct: tuple = field(default_factory=tuple)
+
+
+class _RSFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, cfg: RSConfig, src, p00, p01, p10, p11):
+        params = (p00, p01, p10, p11)
+        masks = _rebuild(cfg.mask_struct, params)
+        out = ops.rs_propagate(src, cfg.x, cfg.y, cfg.z, cfg.wavelength, masks=masks, ez_mode=cfg.ez_mode,
+                               nin=cfg.nin)
+        ctx.cfg = cfg
+        ctx.save_for_backward(src, *[p if p is not None else torch.empty(0) for p in params])
+        ctx.has = [p is not None for p in params]
+        return out
+
+    @staticmethod
+    def backward(ctx, gout):
+        cfg = ctx.cfg
+        saved = ctx.saved_tensors
+        src = saved[0]
+        params = [saved[1 + i] if ctx.has[i] else None for i in range(4)]
+        g_pre = ops.rs_propagate(gout.contiguous(), cfg.x, cfg.y, cfg.z, cfg.wavelength, adjoint=True)
+        masks = _rebuild(cfg.mask_struct, params)
+        need_params = ((ctx.needs_input_grad[2] and params[0] is not None, ctx.needs_input_grad[3] and params[1] is not None),
+                       (ctx.needs_input_grad[4] and params[2] is not None, ctx.needs_input_grad[5] and params[3] is not None))
+        if not masks and cfg.ez_mode == N.XL_EZ_NONE:
+            g_src, gp = g_pre, [[None, None], [None, None]]
+        else:
+            g_src, gp = ops.pre_backward(src, g_pre, cfg.x, cfg.y, cfg.z, masks=masks, ez_mode=cfg.ez_mode,
+                                         nin=cfg.nin, need_src=ctx.needs_input_grad[1], need_params=need_params)
+        if g_src is not None and ctx.needs_input_grad[1]:
+            if g_src.shape[0] != src.shape[0]:
+                full = torch.zeros_like(src)
+                full[: g_src.shape[0]] = g_src
+                g_src = full
+        else:
+            g_src = None
+        return (None, g_src, *_param_grads(ctx, cfg.mask_struct, params, gp))
+
+
+def rs(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Sequence[MaskSpec] = (),
+       ez_mode: int = N.XL_EZ_NONE, nin: Optional[int] = None) -> torch.Tensor:
+    """Differentiable RS / VRS propagation of src (C, n, n)."""
+    nin = src.shape[0] if nin is None else nin
+    cfg = RSConfig(x, y, float(z), float(wavelength), ez_mode, nin, _mask_structure(masks))
+    return _RSFunction.apply(cfg, src, *_flat_params(masks))
+
+
+@dataclass
+class CZTConfig:
+    geom: ops.CztGeometry
+    ez_mode: int
+    nin: int
+    mode: int
+    mask_struct: tuple = field(default_factory=tuple)
+
+
+class _CZTFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, cfg: CZTConfig, src, p00, p01, p10, p11):
+        params = (p00, p01, p10, p11)
+        masks = _rebuild(cfg.mask_struct, params)
+        out = ops.czt(src, cfg.geom, masks=masks, ez_mode=cfg.ez_mode, nin=cfg.nin, mode=cfg.mode)
+        ctx.cfg = cfg
+        ctx.save_for_backward(src, *[p if p is not None else torch.empty(0) for p in params])
+        ctx.has = [p is not None for p in params]
+        return out
+
+    @staticmethod
+    def backward(ctx, gout):
+        cfg = ctx.cfg
+        g = cfg.geom
+        saved = ctx.saved_tensors
+        src = saved[0]
+        params = [saved[1 + i] if ctx.has[i] else None for i in range(4)]
+        g_pre = ops.czt(gout.contiguous(), g, adjoint=True, mode=cfg.mode)
+        masks = _rebuild(cfg.mask_struct, params)
+        hna = cfg.mode == N.XL_CZT_HNA
+        need_params = ((ctx.needs_input_grad[2] and params[0] is not None, ctx.needs_input_grad[3] and params[1] is not None),
+                       (ctx.needs_input_grad[4] and params[2] is not None, ctx.needs_input_grad[5] and params[3] is not None))
+        if not masks and cfg.ez_mode == N.XL_EZ_NONE and not hna:
+            g_src, gp = g_pre, [[None, None], [None, None]]
+        else:
+            g_src, gp = ops.pre_backward(src, g_pre, g.x, g.y, g.z, masks=masks, ez_mode=cfg.ez_mode, hna=hna,
+                                         nin=cfg.nin, need_src=ctx.needs_input_grad[1], need_params=need_params,
+                                         hna_radius=g.hna_radius, hna_f=g.hna_f)
+        if g_src is not None and ctx.needs_input_grad[1]:
+            if g_src.shape[0] != src.shape[0]:
+                full = torch.zeros_like(src)
+                full[: g_src.shape[0]] = g_src
+                g_src = full
+        else:
+            g_src = None
+        return (None, g_src, *_param_grads(ctx, cfg.mask_struct, params, gp))
+
+
+def czt(src: torch.Tensor, geom: ops.CztGeometry, masks: Sequence[MaskSpec] = (), ez_mode: int = N.XL_EZ_NONE,
+        nin: Optional[int] = None, mode: Optional[int] = None) -> torch.Tensor:
+    """Differentiable CZT / VCZT / high-NA VCZT of src (C, ny, nx)."""
+    nin = src.shape[0] if nin is None else nin
+    mode = geom.mode if mode is None else mode
+    cfg = CZTConfig(geom, ez_mode, nin, mode, _mask_structure(masks))
+    return _CZTFunction.apply(cfg, src, *_flat_params(masks))
+
+
+@dataclass
+class PreConfig:
+    x: object
+    y: object
+    z: float
+    ez_mode: int
+    hna: bool
+    nin: int
+    ncomp: int
+    hna_radius: float
+    hna_f: float
+    mask_struct: tuple = field(default_factory=tuple)
+
+
+class _PreFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, cfg: PreConfig, src, p00, p01, p10, p11):
+        params = (p00, p01, p10, p11)
+        masks = _rebuild(cfg.mask_struct, params)
+        out = ops.pre_forward(src, cfg.x, cfg.y, cfg.z, masks=masks, ez_mode=cfg.ez_mode, hna=cfg.hna,
+                              ncomp=cfg.ncomp, nin=cfg.nin, hna_radius=cfg.hna_radius, hna_f=cfg.hna_f)
+        ctx.cfg = cfg
+        ctx.save_for_backward(src, *[p if p is not None else torch.empty(0) for p in params])
+        ctx.has = [p is not None for p in params]
+        return out
+
+    @staticmethod
+    def backward(ctx, gout):
+        cfg = ctx.cfg
+        saved = ctx.saved_tensors
+        src = saved[0]
+        params = [saved[1 + i] if ctx.has[i] else None for i in range(4)]
+        masks = _rebuild(cfg.mask_struct, params)
+        need_params = ((ctx.needs_input_grad[2] and params[0] is not None, ctx.needs_input_grad[3] and params[1] is not None),
+                       (ctx.needs_input_grad[4] and params[2] is not None, ctx.needs_input_grad[5] and params[3] is not None))
+        g_src, gp = ops.pre_backward(src, gout.contiguous(), cfg.x, cfg.y, cfg.z, masks=masks, ez_mode=cfg.ez_mode,
+                                     hna=cfg.hna, nin=cfg.nin, need_src=ctx.needs_input_grad[1],
+                                     need_params=need_params, hna_radius=cfg.hna_radius, hna_f=cfg.hna_f)
+        if g_src is not None and ctx.needs_input_grad[1]:
+            if g_src.shape[0] != src.shape[0]:
+                full = torch.zeros_like(src)
+                full[: g_src.shape[0]] = g_src
+                g_src = full
+        else:
+            g_src = None
+        return (None, g_src, *_param_grads(ctx, cfg.mask_struct, params, gp))
+
+
+def pre(src: torch.Tensor, x, y, z: float = 0.0, masks: Sequence[MaskSpec] = (), ez_mode: int = N.XL_EZ_NONE,
+        hna: bool = False, nin: Optional[int] = None, ncomp: Optional[int] = None, hna_radius: float = 0.0,
+        hna_f: float = 0.0) -> torch.Tensor:
+    """Differentiable standalone Jones masks / Ez construction / lens map."""
+    nin = src.shape[0] if nin is None else nin
+    if ncomp is None:
+        ncomp = 3 if (hna or ez_mode != N.XL_EZ_NONE) else src.shape[0]
+    cfg = PreConfig(x, y, float(z), ez_mode, hna, nin, ncomp, hna_radius, hna_f, _mask_structure(masks))
+    return _PreFunction.apply(cfg, src, *_flat_params(masks))

@@ -230,14 +230,17 @@ __global__ __launch_bounds__(Q / 4) void czt_cols_kernel(CztArgs a) {
 
 template <int Q>
 inline void czt_rows_launch(const CztArgs& a, hipStream_t st) {
+  KTimer t(K_CZT_ROWS, st);
   hipLaunchKernelGGL(czt_rows_kernel<Q>, dim3((a.rin + 3) / 4), dim3(Q / 4), 0, st, a);
 }
 template <int Q>
 inline void czt_cols_launch(const CztArgs& a, hipStream_t st) {
+  KTimer t(K_CZT_COLS, st);
   hipLaunchKernelGGL(czt_cols_kernel<Q>, dim3((a.ax.lout + 3) / 4), dim3(Q / 4), 0, st, a);
 }
 template <int Q>
 inline void czt_setup_launch(const CztSetupArgs& a, hipStream_t st) {
+  KTimer t(K_CZT_SETUP, st);
   hipLaunchKernelGGL(czt_setup_kernel<Q>, dim3(1), dim3(Q / 16), 0, st, a);
 }
 

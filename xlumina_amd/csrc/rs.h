@@ -23,6 +23,7 @@
 #pragma once
 #include "fft.h"
 #include "masks.h"
+#include "timing.h"
 
 namespace xl {
 
@@ -248,10 +249,10 @@ template <int P>
 inline hipError_t rs_launch(const RsArgs& a, hipStream_t st) {
   const int nrg = (a.n + 3) >> 2;
   dim3 blk(P / 4);
-  hipLaunchKernelGGL(rs_hrow_kernel<P>, dim3(P / 8), blk, 0, st, a);
-  hipLaunchKernelGGL(rs_urow_kernel<P>, dim3(nrg), blk, 0, st, a);
-  hipLaunchKernelGGL(rs_col_kernel<P>, dim3(P / 4), blk, 0, st, a);
-  hipLaunchKernelGGL(rs_final_kernel<P>, dim3(nrg), blk, 0, st, a);
+  { KTimer t(K_RS_HROW, st); hipLaunchKernelGGL(rs_hrow_kernel<P>, dim3(P / 8), blk, 0, st, a); }
+  { KTimer t(K_RS_UROW, st); hipLaunchKernelGGL(rs_urow_kernel<P>, dim3(nrg), blk, 0, st, a); }
+  { KTimer t(K_RS_COL, st); hipLaunchKernelGGL(rs_col_kernel<P>, dim3(P / 4), blk, 0, st, a); }
+  { KTimer t(K_RS_FINAL, st); hipLaunchKernelGGL(rs_final_kernel<P>, dim3(nrg), blk, 0, st, a); }
   return hipGetLastError();
 }
 

@@ -19,6 +19,7 @@
 #include "masks.h"
 #include "pre.h"
 #include "rs.h"
+#include "timing.h"
 
 namespace {
 
@@ -372,6 +373,7 @@ int xl_pre_forward(const xl_pre_params* p, void* stream) {
   xl::PreArgs a;
   if (int rc = pre_common(p, &a)) return rc;
   if (!p->out) return fail(XL_EINVAL, "null output");
+  xl::KTimer t(xl::K_PRE_FWD, (hipStream_t)stream);
   hipLaunchKernelGGL(xl::pre_forward_kernel, grid_for((int64_t)p->rows * p->cols), dim3(256), 0,
                      (hipStream_t)stream, a);
   return hip_check(hipGetLastError(), "pre_forward launch");
@@ -381,6 +383,7 @@ int xl_pre_backward(const xl_pre_params* p, void* stream) {
   xl::PreArgs a;
   if (int rc = pre_common(p, &a)) return rc;
   if (!p->gout) return fail(XL_EINVAL, "null cotangent");
+  xl::KTimer t(xl::K_PRE_BWD, (hipStream_t)stream);
   hipLaunchKernelGGL(xl::pre_backward_kernel, grid_for((int64_t)p->rows * p->cols), dim3(256), 0,
                      (hipStream_t)stream, a);
   return hip_check(hipGetLastError(), "pre_backward launch");
@@ -389,9 +392,57 @@ int xl_pre_backward(const xl_pre_params* p, void* stream) {
 int xl_transfer_function_rs(void* out, const double* x, const double* y, int32_t nx, int32_t ny, double z, double k,
                             void* stream) {
   if (!out || !x || !y || nx < 1 || ny < 1) return fail(XL_EINVAL, "bad transfer-function arguments");
+  xl::KTimer t(xl::K_TF, (hipStream_t)stream);
   hipLaunchKernelGGL(xl::transfer_function_kernel, grid_for((int64_t)nx * ny), dim3(256), 0, (hipStream_t)stream,
                      (float2*)out, x, y, nx, ny, z, k);
   return hip_check(hipGetLastError(), "transfer function launch");
+}
+
+int xl_timing_enable(int32_t enable, int32_t capacity) {
+  xl::TimingState& s = xl::timing_state();
+  std::lock_guard<std::mutex> lk(s.mu);
+  for (hipEvent_t e : s.pool) (void)hipEventDestroy(e);
+  s.pool.clear();
+  s.ids.clear();
+  s.used = 0;
+  s.dropped = 0;
+  s.on = false;
+  if (!enable) return XL_OK;
+  if (capacity < 1 || capacity > (1 << 20)) return fail(XL_EINVAL, "timing capacity must be 1..2^20");
+  s.pool.resize(2 * (size_t)capacity);
+  s.ids.resize(capacity);
+  for (auto& e : s.pool)
+    if (int rc = hip_check(hipEventCreate(&e), "hipEventCreate")) return rc;
+  s.on = true;
+  return XL_OK;
+}
+
+int xl_timing_collect(xl_kernel_time* out, int32_t max_entries) {
+  xl::TimingState& s = xl::timing_state();
+  std::lock_guard<std::mutex> lk(s.mu);
+  double tot[xl::K_COUNT] = {};
+  int64_t cnt[xl::K_COUNT] = {};
+  for (size_t i = 0; i < s.used; ++i) {
+    if (int rc = hip_check(hipEventSynchronize(s.pool[2 * i + 1]), "hipEventSynchronize")) return -rc;
+    float ms = 0.f;
+    if (int rc = hip_check(hipEventElapsedTime(&ms, s.pool[2 * i], s.pool[2 * i + 1]), "hipEventElapsedTime"))
+      return -rc;
+    tot[s.ids[i]] += ms;
+    cnt[s.ids[i]] += 1;
+  }
+  s.used = 0;
+  int n = 0;
+  for (int k = 0; k < xl::K_COUNT; ++k) {
+    if (!cnt[k]) continue;
+    if (out && n < max_entries) {
+      std::memset(out[n].name, 0, sizeof(out[n].name));
+      std::strncpy(out[n].name, xl::kernel_name(k), sizeof(out[n].name) - 1);
+      out[n].launches = cnt[k];
+      out[n].total_ms = tot[k];
+    }
+    ++n;
+  }
+  return n;
 }
 
 }  // extern "C"

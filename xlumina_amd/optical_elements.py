@@ -1,0 +1,142 @@
+"""Jones masks and the high-NA objective lens on the propagation path
+(drop-in for the corresponding functions of xlumina/optical_elements.py).
+
+SLM / sSLM / LCD / linear_polarizer return a light whose mask is pending: the
+next propagation fuses it into its first pass (two masks in a row fuse; a
+third materialises the first two).  Reading the field of such a light
+materialises it with the standalone pre-op kernel.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import _native as N
+from . import autograd as AG
+from . import ops
+from .ops import MaskSpec
+from .vectorized_optics import VectorizedLight
+from .wave_optics import ScalarLight, _grid
+from . import wave_optics as _wo
+
+
+def _param(p, light):
+    if p is None:
+        return None
+    if isinstance(p, torch.Tensor):
+        return p.to(light.device)
+    return torch.as_tensor(np.asarray(p, dtype=np.float32), device=light.device)
+
+
+# -- (1) scalar -------------------------------------------------------------------------
+
+def phase_scalar_SLM(phase):
+    """optical_elements.py:76-85: exp(i phase)."""
+    if isinstance(phase, torch.Tensor):
+        return torch.polar(torch.ones_like(phase), phase)
+    return np.exp(1j * np.asarray(phase))
+
+
+def SLM(input_field: ScalarLight, phase_array, shape: int):
+    """optical_elements.py:87-103: per-pixel phase mask -> (light, exp(i phase)).
+
+    The light's mask stays pending and is fused into its next propagation."""
+    phase = _param(phase_array, input_field)
+    if tuple(phase.shape) != (shape, shape):
+        raise ValueError(f"SLM phase array must be ({shape}, {shape})")
+    light = input_field._with_mask(MaskSpec(N.XL_MASK_SLM, phase))
+    return light, phase_scalar_SLM(phase)
+
+
+# -- (2) Jones matrices (2 x 2, scalar parameters) -------------------------------------------
+
+def jones_LP(alpha):
+    """optical_elements.py:109-120."""
+    a = torch.as_tensor(alpha, dtype=torch.float64)
+    c, s = torch.cos(a), torch.sin(a)
+    return torch.stack([torch.stack([c ** 2, c * s]), torch.stack([c * s, s ** 2])])
+
+
+def jones_general_retarder(eta, theta, delta):
+    """optical_elements.py:122-139."""
+    eta = torch.as_tensor(eta, dtype=torch.float64)
+    theta = torch.as_tensor(theta, dtype=torch.float64)
+    delta = torch.as_tensor(delta, dtype=torch.float64)
+    em, ep = torch.exp(-(eta / 2) * 1j), torch.exp((eta / 2) * 1j)
+    c, s = torch.cos(theta), torch.sin(theta)
+    return torch.stack([torch.stack([em * c ** 2 + ep * s ** 2, (em - ep) * torch.exp(-delta * 1j) * s * c]),
+                        torch.stack([(em - ep) * torch.exp(delta * 1j) * s * c, em * s ** 2 + ep * c ** 2])])
+
+
+def jones_sSLM(alpha, phi):
+    """optical_elements.py:141-152."""
+    a = torch.as_tensor(alpha, dtype=torch.float64)
+    p = torch.as_tensor(phi, dtype=torch.float64)
+    z = torch.zeros((), dtype=torch.complex128)
+    return torch.stack([torch.stack([torch.exp(1j * a), z]), torch.stack([z, torch.exp(1j * p)])])
+
+
+def jones_LCD(eta, theta):
+    """optical_elements.py:169-180."""
+    return jones_general_retarder(eta, theta, 0.0)
+
+
+# -- (3) polarisation devices ----------------------------------------------------------------
+
+def sSLM(input_field: VectorizedLight, alpha_array=None, phi_array=None) -> VectorizedLight:
+    """optical_elements.py:186-222: diag(e^{i alpha}, e^{i phi}) per pixel, Ez kept."""
+    return input_field._with_mask(MaskSpec(N.XL_MASK_SSLM, _param(alpha_array, input_field),
+                                           _param(phi_array, input_field)))
+
+
+def LCD(input_field: VectorizedLight, eta, theta) -> VectorizedLight:
+    """optical_elements.py:266-305: retarder with eta/theta constant (floats) or per pixel; Ez kept."""
+    if not isinstance(eta, torch.Tensor) and not isinstance(theta, torch.Tensor) \
+            and np.ndim(eta) == 0 and np.ndim(theta) == 0:
+        spec = MaskSpec(N.XL_MASK_LCD_SCALAR, None, None, float(eta), float(theta))
+    else:
+        spec = MaskSpec(N.XL_MASK_LCD, _param(eta, input_field), _param(theta, input_field))
+    return input_field._with_mask(spec)
+
+
+def linear_polarizer(input_field: VectorizedLight, alpha) -> VectorizedLight:
+    """optical_elements.py:307-332: [[c^2, cs], [cs, s^2]] per pixel; Ez becomes 0."""
+    return input_field._with_mask(MaskSpec(N.XL_MASK_LP, _param(alpha, input_field)))
+
+
+# -- high-NA objective lens ---------------------------------------------------------------------
+
+def high_NA_objective_lens(input_field: VectorizedLight, radius: float, f: float):
+    """optical_elements.py:515-553: (3, N, N) field after the lens and sin(theta_max)."""
+    sin_theta_max = radius / math.sqrt(radius ** 2 + f ** 2)
+    src, masks = input_field._pending()
+    out = AG.pre(src, input_field.x, input_field.y, masks=masks, hna=True, nin=2, ncomp=3, hna_radius=radius,
+                 hna_f=f)
+    return out, sin_theta_max
+
+
+def build_high_NA_VCZT_grid(f, r, wavelength, xin, xout, yout):
+    """optical_elements.py:640-672."""
+    xin, xout, yout = _grid(xin), _grid(xout), _grid(yout)
+    nx, ny = len(xout), len(yout)
+    Dm = f * wavelength * (len(xin) - 1) / (2 * r)
+    return nx, ny, Dm, yout[0] + Dm / 2, yout[-1] + Dm / 2, xout[0] + Dm / 2, xout[-1] + Dm / 2
+
+
+def VCZT_objective_lens(input_field: VectorizedLight, r: float, f: float, xout, yout) -> VectorizedLight:
+    """optical_elements.py:600-638: lens map + two Bluestein passes + -i sin^2(theta_max)/(f lambda),
+    all on the device with pending masks fused into the first pass."""
+    tic = time.perf_counter()
+    xout, yout = _grid(xout), _grid(yout)
+    src, masks = input_field._pending()
+    geom = ops.hna_geometry(input_field.x, input_field.y, xout, yout, input_field.wavelength, r, f)
+    out = AG.czt(src, geom, masks=masks, nin=2)
+    light = VectorizedLight(xout, yout, input_field.wavelength, input_field.device)
+    light._E = out
+    if _wo.VERBOSE:
+        print("Time taken to perform one VCZT propagation through objective lens (in seconds):  "
+              f"{(time.perf_counter() - tic):.4f}")
+    return light
