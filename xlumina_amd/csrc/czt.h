@@ -19,6 +19,7 @@
 // contiguously, runs the y axis and applies the output-plane factor.
 #pragma once
 #include "fft.h"
+#include "fftio.h"
 #include "masks.h"
 #include "rs.h"
 #include "pre.h"
@@ -41,7 +42,7 @@ struct CztArgs {
   int nin;
   float2* out;
   int64_t out_cs;
-  float2* t;           // tiles: ncomp x round4(rin) x round4(ax.lout)
+  float2* t;           // row-major intermediate: ncomp x rin x ax.lout
   int rin, ncomp;      // input rows (cols = ax.lin)
   int pre_mode, post_mode, ez_mode;
   const double* px;    // grid of the input plane (pre-op)
@@ -65,9 +66,14 @@ struct CztSetupArgs {
   double th_turns;     // angle(W) / (2 pi), principal branch
 };
 
-// One workgroup of q/16 threads builds one axis' tables.
+// One workgroup of q/16 threads builds one axis' tables (block b takes args b).
+struct CztSetupPair {
+  CztSetupArgs ax[2];
+};
+
 template <int Q>
-__global__ __launch_bounds__(Q / 16) void czt_setup_kernel(CztSetupArgs a) {
+__global__ __launch_bounds__(Q / 16) void czt_setup_kernel(CztSetupPair pair) {
+  const CztSetupArgs& a = pair.ax[blockIdx.x];
   constexpr int TPF = Q / 16;
   __shared__ float2 s[LdsLen<Q>::value];
   const int t = threadIdx.x;
@@ -106,142 +112,154 @@ __global__ __launch_bounds__(Q / 16) void czt_setup_kernel(CztSetupArgs a) {
   for (int i = t; i < Q; i += TPF) a.gf[i] = s[lpad(i)] * iq;
 }
 
-// Pre-op for pixel (row, col) of component c: returns the value fed to the x axis.
+// Pre-op families the row kernel is specialised for.
+enum { PREK_SIMPLE = 0, PREK_F = 1, PREK_HNA = 2 };
+
+// Pre-op for pixel (row, col) of component c: the value fed to the x axis.
+template <int K>
 __device__ __forceinline__ float2 czt_pre_value(const CztArgs& a, int c, int row, int col, float2 fpix) {
   const int64_t pix = (int64_t)row * a.ax.lin + col;
   float2 e0 = a.in[pix];
   float2 e1 = a.nin > 1 ? a.in[a.in_cs + pix] : make_float2(0.f, 0.f);
   float2 e2 = a.nin > 2 ? a.in[2 * a.in_cs + pix] : make_float2(0.f, 0.f);
   apply_mask(a.mask, pix, e0, e1, e2);
-  if (a.pre_mode == PRE_HNA) {
+  if constexpr (K == PREK_HNA) {
     // optical_elements.py:528-594: theta = rho/f, phi = atan2(Y, X), Richards-Wolf map
     float A[3][2];
     hna_map(a.px[col], a.py[row], a.hna_radius, a.hna_f, a.hna_stm2, A);
     return c_add(c_scale(e0, A[c][0]), c_scale(e1, A[c][1]));
+  } else {
+    float2 v = c == 0 ? e0 : (c == 1 ? e1 : e2);
+    if (c == 2 && a.ez_mode != EZ_NONE) v = build_ez(a.ez_mode, e0, e1, a.px[col], a.py[row], a.z);
+    if constexpr (K == PREK_F) v = c_mul(v, fpix);
+    else if (a.pre_mode == PRE_CONST) v = c_mul(v, a.cte);
+    return v;
   }
-  float2 v = c == 0 ? e0 : (c == 1 ? e1 : e2);
-  if (c == 2 && a.ez_mode != EZ_NONE) v = build_ez(a.ez_mode, e0, e1, a.px[col], a.py[row], a.z);
-  if (a.pre_mode == PRE_RS || a.pre_mode == PRE_CONJ_RS) v = c_mul(v, fpix);
-  else if (a.pre_mode == PRE_CONST) v = c_mul(v, a.cte);
-  return v;
 }
 
-template <int Q>
-__global__ __launch_bounds__(Q / 4) void czt_rows_kernel(CztArgs a) {
+// Transfer function of the input plane (or its conjugate times cte for the adjoint).
+__device__ __forceinline__ float2 czt_f_value(const CztArgs& a, int row, int col) {
+  float2 h = rs_kernel_value(a.px[col], a.py[row], a.z, a.k);
+  if (a.pre_mode == PRE_CONJ_RS) h = c_mul(conjf2(h), a.cte);
+  return h;
+}
+
+// One block per input row: component c runs its FFT on threads
+// [c*TPF, (c+1)*TPF) and LDS buffer c; the pre-op of each pixel (masks, Ez,
+// transfer function in float64 phase, lens map) is computed once by one
+// thread for all components.  The chirp spectrum is applied in registers
+// between the forward and inverse chains and only the output window is
+// stored: row-major intermediate [c][row][k].
+template <int Q, int K>
+__global__ __launch_bounds__(3 * Q / 16) void czt_rows_kernel(CztArgs a) {
   constexpr int TPF = Q / 16, LS = LdsLen<Q>::value;
-  constexpr int PPT = Q / TPF;
-  __shared__ float2 lds[4 * LS];
-  const int tid = threadIdx.x, f = tid / TPF, t = tid % TPF;
-  float2* s = lds + f * LS;
-  const int nrg = (a.rin + 3) >> 2;
-  const int rg = blockIdx.x;
-  const int row = rg * 4 + f;
-  const bool rowok = row < a.rin;
+  using HO = HandOff<Q, TPF>;
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int f = tid / TPF, t = tid % TPF;
+  const int row = blockIdx.x;
   const int lin = a.ax.lin, lout = a.ax.lout, woff = a.ax.woff;
-  const int nog = (lout + 3) >> 2;
-  // transfer function of the input plane, once per pixel (float64 phase)
-  float2 fp[PPT];
-#pragma unroll
-  for (int i = 0; i < PPT; ++i) {
-    const int col = t + i * TPF;
-    fp[i] = make_float2(1.f, 0.f);
-    if (rowok && col < lin && (a.pre_mode == PRE_RS || a.pre_mode == PRE_CONJ_RS)) {
-      float2 h = rs_kernel_value(a.px[col], a.py[row], a.z, a.k);
-      if (a.pre_mode == PRE_CONJ_RS) h = c_mul(conjf2(h), a.cte);
-      fp[i] = h;
+  const int nc = a.ncomp;
+  for (int col = tid; col < Q; col += nthr) {
+    if (col < lin) {
+      float2 fv = make_float2(1.f, 0.f);
+      if constexpr (K == PREK_F) fv = czt_f_value(a, row, col);
+      const float2 pre = a.ax.pre[col];
+      for (int c = 0; c < nc; ++c) lds[c * LS + lpad(col)] = c_mul(czt_pre_value<K>(a, c, row, col, fv), pre);
+    } else {
+      for (int c = 0; c < nc; ++c) lds[c * LS + lpad(col)] = make_float2(0.f, 0.f);
     }
   }
-  for (int c = 0; c < a.ncomp; ++c) {
+  __syncthreads();
+  float2* s = lds + f * LS;
+  typename HO::Type u;
+  NoIO none;
+  fft_forward_to_regs<Q, TPF, SRC_LDS>(s, t, none, u);
 #pragma unroll
-    for (int i = 0; i < PPT; ++i) {
-      const int col = t + i * TPF;
-      float2 v = make_float2(0.f, 0.f);
-      if (rowok && col < lin) v = c_mul(czt_pre_value(a, c, row, col, fp[i]), a.ax.pre[col]);
-      s[lpad(col)] = v;
-    }
-    __syncthreads();
-    fft_lds<Q, TPF, false>(s, t);
+  for (int b = 0; b < HO::BPT; ++b)
 #pragma unroll
-    for (int i = 0; i < PPT; ++i) {
-      const int idx = t + i * TPF;
-      s[lpad(idx)] = cmul(s[lpad(idx)], a.ax.gf[idx]);
-    }
-    __syncthreads();
-    fft_lds<Q, TPF, true>(s, t);
-    // window * post -> tiles [out group][row group]
-    float2* dst = a.t + (int64_t)c * nrg * 4 * (nog * 4);
-    for (int e = tid; e < nog * 16; e += Q / 4) {
-      const int og = e >> 4, rr = (e >> 2) & 3, cc = e & 3;
-      const int k = og * 4 + cc;
-      float2 v = make_float2(0.f, 0.f);
-      if (k < lout) v = cmul(lds[rr * LS + lpad(woff + k)], a.ax.post[k]);
-      dst[((int64_t)og * nrg + rg) * 16 + rr * 4 + cc] = v;
-    }
-    __syncthreads();
-  }
+    for (int q = 0; q < HO::R; ++q) u.v[b][q] = cmul(u.v[b][q], a.ax.gf[HO::index(t, b, q)]);
+  float2* dst = a.t + ((int64_t)f * a.rin + row) * lout;
+  auto store = [&](int idx, float2 v) {
+    const int k = idx - woff;
+    if (k >= 0 && k < lout) dst[k] = cmul(v, a.ax.post[k]);
+  };
+  fft_inverse_from_regs<Q, TPF, DST_FN>(s, t, u, store);
 }
 
+// Per block 4 output columns of the x axis (one per thread group) of one
+// component: the column of the row-major intermediate is loaded straight into
+// the first pass, the chirp spectrum applied in registers, and only the
+// output window stored with the output-plane factor.  Four blocks share each
+// 128 B line of the intermediate, so consecutive blocks stay on one XCD.
 template <int Q>
-__global__ __launch_bounds__(Q / 4) void czt_cols_kernel(CztArgs a) {
-  constexpr int TPF = Q / 16, LS = LdsLen<Q>::value;
-  constexpr int PPT = Q / TPF;
+__global__ __launch_bounds__(4 * FftThreads<Q>::value) void czt_cols_kernel(CztArgs a) {
+  constexpr int TPF = FftThreads<Q>::value, LS = LdsLen<Q>::value;
+  using HO = HandOff<Q, TPF>;
   __shared__ float2 lds[4 * LS];
   const int tid = threadIdx.x, f = tid / TPF, t = tid % TPF;
   float2* s = lds + f * LS;
-  const int nrg = (a.rin + 3) >> 2;
-  const int og = blockIdx.x;          // output columns 4og..4og+3 of the x axis
+  const int og = xcd_remap(blockIdx.x, gridDim.x);
+  const int c = blockIdx.y;
   const int lout_x = a.ax.lout;
-  const int nog = (lout_x + 3) >> 2;
+  const int kx = og * 4 + f;
+  const bool colok = kx < lout_x;
   const int lin = a.ay.lin, lout = a.ay.lout, woff = a.ay.woff;
-  for (int c = 0; c < a.ncomp; ++c) {
-    const float2* src = a.t + (int64_t)c * nrg * 4 * (nog * 4) + (int64_t)og * nrg * 16;
-    for (int e = tid; e < nrg * 16; e += Q / 4) {
-      const int r = (e >> 4) * 4 + ((e >> 2) & 3), cc = e & 3;
-      if (r < lin) lds[cc * LS + lpad(r)] = cmul(src[e], a.ay.pre[r]);
-    }
-    for (int e = tid; e < 4 * (Q - lin); e += Q / 4) {
-      const int cc = e & 3, r = lin + (e >> 2);
-      lds[cc * LS + lpad(r)] = make_float2(0.f, 0.f);
-    }
-    __syncthreads();
-    fft_lds<Q, TPF, false>(s, t);
+  const float2* src = a.t + (int64_t)c * a.rin * lout_x + kx;
+  auto load = [&](int r) -> float2 {
+    return (colok && r < lin) ? cmul(src[(int64_t)r * lout_x], a.ay.pre[r]) : make_float2(0.f, 0.f);
+  };
+  typename HO::Type u;
+  fft_forward_to_regs<Q, TPF, SRC_FN>(s, t, load, u);
 #pragma unroll
-    for (int i = 0; i < PPT; ++i) {
-      const int idx = t + i * TPF;
-      s[lpad(idx)] = cmul(s[lpad(idx)], a.ay.gf[idx]);
+  for (int b = 0; b < HO::BPT; ++b)
+#pragma unroll
+    for (int q = 0; q < HO::R; ++q) u.v[b][q] = cmul(u.v[b][q], a.ay.gf[HO::index(t, b, q)]);
+  float2* oc = a.out + (int64_t)c * a.out_cs;
+  auto store = [&](int idx, float2 v) {
+    const int ky = idx - woff;
+    if (colok && ky >= 0 && ky < lout) {
+      v = cmul(v, a.ay.post[ky]);
+      if (a.post_mode == POST_RS) v = c_mul(c_mul(v, rs_kernel_value(a.qx[kx], a.qy[ky], a.z, a.k)), a.cte);
+      else if (a.post_mode == POST_CONJ_RS) v = c_mul(v, conjf2(rs_kernel_value(a.qx[kx], a.qy[ky], a.z, a.k)));
+      else if (a.post_mode == POST_CONST) v = c_mul(v, a.cte);
+      oc[(int64_t)ky * lout_x + kx] = v;
     }
-    __syncthreads();
-    fft_lds<Q, TPF, true>(s, t);
-    float2* oc = a.out + (int64_t)c * a.out_cs;
-    for (int e = tid; e < lout * 4; e += Q / 4) {
-      const int ky = e >> 2, cc = e & 3;
-      const int kx = og * 4 + cc;
-      if (kx < lout_x) {
-        float2 v = cmul(lds[cc * LS + lpad(woff + ky)], a.ay.post[ky]);
-        if (a.post_mode == POST_RS) v = c_mul(c_mul(v, rs_kernel_value(a.qx[kx], a.qy[ky], a.z, a.k)), a.cte);
-        else if (a.post_mode == POST_CONJ_RS) v = c_mul(v, conjf2(rs_kernel_value(a.qx[kx], a.qy[ky], a.z, a.k)));
-        else if (a.post_mode == POST_CONST) v = c_mul(v, a.cte);
-        oc[(int64_t)ky * lout_x + kx] = v;
-      }
-    }
-    __syncthreads();
-  }
+  };
+  fft_inverse_from_regs<Q, TPF, DST_FN>(s, t, u, store);
 }
 
 template <int Q>
 inline void czt_rows_launch(const CztArgs& a, hipStream_t st) {
   KTimer t(K_CZT_ROWS, st);
-  hipLaunchKernelGGL(czt_rows_kernel<Q>, dim3((a.rin + 3) / 4), dim3(Q / 4), 0, st, a);
+  const dim3 g(a.rin), b(a.ncomp * Q / 16);
+  const size_t lds = (size_t)a.ncomp * LdsLen<Q>::value * sizeof(float2);
+  static bool attr_set = false;   // allow up to three 4096-point buffers of dynamic LDS
+  if (!attr_set) {
+    const int mx = 3 * LdsLen<Q>::value * (int)sizeof(float2);
+    (void)hipFuncSetAttribute((const void*)czt_rows_kernel<Q, PREK_F>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+    (void)hipFuncSetAttribute((const void*)czt_rows_kernel<Q, PREK_HNA>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+    (void)hipFuncSetAttribute((const void*)czt_rows_kernel<Q, PREK_SIMPLE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              mx);
+    attr_set = true;
+  }
+  if (a.pre_mode == PRE_RS || a.pre_mode == PRE_CONJ_RS)
+    hipLaunchKernelGGL((czt_rows_kernel<Q, PREK_F>), g, b, lds, st, a);
+  else if (a.pre_mode == PRE_HNA)
+    hipLaunchKernelGGL((czt_rows_kernel<Q, PREK_HNA>), g, b, lds, st, a);
+  else
+    hipLaunchKernelGGL((czt_rows_kernel<Q, PREK_SIMPLE>), g, b, lds, st, a);
 }
 template <int Q>
 inline void czt_cols_launch(const CztArgs& a, hipStream_t st) {
   KTimer t(K_CZT_COLS, st);
-  hipLaunchKernelGGL(czt_cols_kernel<Q>, dim3((a.ax.lout + 3) / 4), dim3(Q / 4), 0, st, a);
+  hipLaunchKernelGGL(czt_cols_kernel<Q>, dim3((a.ax.lout + 3) / 4, a.ncomp), dim3(4 * FftThreads<Q>::value), 0, st,
+                     a);
 }
 template <int Q>
-inline void czt_setup_launch(const CztSetupArgs& a, hipStream_t st) {
+inline void czt_setup_launch(const CztSetupPair& p, int nblocks, hipStream_t st) {
   KTimer t(K_CZT_SETUP, st);
-  hipLaunchKernelGGL(czt_setup_kernel<Q>, dim3(1), dim3(Q / 16), 0, st, a);
+  hipLaunchKernelGGL(czt_setup_kernel<Q>, dim3(nblocks), dim3(Q / 16), 0, st, p);
 }
 
 }  // namespace xl

@@ -1,7 +1,7 @@
 // LDS-staged Stockham FFT building blocks for gfx950 (64-wide waves).
 //
 // A workgroup runs FPW independent complex64 FFTs of a power-of-two length L
-// (64 <= L <= 4096), each on TPF = L/16 threads.  Every pass is radix 16
+// (64 <= L <= 4096), each on TPF = FftThreads<L> threads.  Every pass is radix 16
 // except a final radix 2/4/8 when log2(L) is not a multiple of 4.  Data lives
 // in LDS between passes (natural order in, natural order out); each thread
 // keeps its R points of a butterfly in registers.  LDS indices are padded by
@@ -33,6 +33,10 @@ __global__ void xl_fft_init_kernel() {
 
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
 template <int L> struct LdsLen { static constexpr int value = L + L / 16; };
+// Threads per FFT: one radix-16 butterfly per thread and pass up to L = 2048;
+// two at L = 4096, so that four 4096-point FFTs per block run on 512 threads
+// with up to 256 VGPRs each instead of spilling at 1024 threads (128 VGPRs).
+template <int L> struct FftThreads { static constexpr int value = L >= 4096 ? L / 32 : L / 16; };
 
 __device__ __forceinline__ float2 operator+(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 operator-(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -83,19 +87,38 @@ template <bool INV> struct DFT<1, INV> {
   __device__ __forceinline__ static void run(float2*) {}
 };
 
-// w[r] = e^{-2 pi i r base / TWN}, r < R  (4 table loads, products for the rest)
-template <int R> __device__ __forceinline__ void twiddle_powers(float2* w, int base) {
-  w[0] = make_float2(1.f, 0.f);
-  if constexpr (R > 1) w[1] = g_tw[base];
-  if constexpr (R > 2) { w[2] = g_tw[2 * base]; w[3] = cmul(w[1], w[2]); }
-  if constexpr (R > 4) {
-    w[4] = g_tw[4 * base];
-    w[5] = cmul(w[1], w[4]); w[6] = cmul(w[2], w[4]); w[7] = cmul(w[3], w[4]);
-  }
-  if constexpr (R > 8) {
-    w[8] = g_tw[8 * base];
-#pragma unroll
-    for (int r = 9; r < 16; ++r) w[r] = cmul(w[r - 8], w[8]);
+// v[r] *= w^r for r < R with w = e^{-+2 pi i base / TWN}: four table loads
+// (w, w^2, w^4, w^8), the other powers as products formed next to their use so
+// that only the four bases stay live.
+template <int R, bool INV> __device__ __forceinline__ void apply_twiddles(float2* v, int base) {
+  if constexpr (R >= 2) {
+    const float2 w1 = g_tw[base];
+    v[1] = twmul<INV>(v[1], w1);
+    if constexpr (R >= 4) {
+      const float2 w2 = g_tw[2 * base];
+      v[2] = twmul<INV>(v[2], w2);
+      const float2 w3 = cmul(w1, w2);
+      v[3] = twmul<INV>(v[3], w3);
+      if constexpr (R >= 8) {
+        const float2 w4 = g_tw[4 * base];
+        v[4] = twmul<INV>(v[4], w4);
+        v[5] = twmul<INV>(v[5], cmul(w1, w4));
+        v[6] = twmul<INV>(v[6], cmul(w2, w4));
+        const float2 w7 = cmul(w3, w4);
+        v[7] = twmul<INV>(v[7], w7);
+        if constexpr (R >= 16) {
+          const float2 w8 = g_tw[8 * base];
+          v[8] = twmul<INV>(v[8], w8);
+          v[9] = twmul<INV>(v[9], cmul(w1, w8));
+          v[10] = twmul<INV>(v[10], cmul(w2, w8));
+          v[11] = twmul<INV>(v[11], cmul(w3, w8));
+          v[12] = twmul<INV>(v[12], cmul(w4, w8));
+          v[13] = twmul<INV>(v[13], cmul(cmul(w1, w4), w8));
+          v[14] = twmul<INV>(v[14], cmul(cmul(w2, w4), w8));
+          v[15] = twmul<INV>(v[15], cmul(w7, w8));
+        }
+      }
+    }
   }
 }
 
@@ -111,12 +134,7 @@ __device__ __forceinline__ void stockham_pass(float2* s, int t) {
     if (NB % TPF == 0 || j < NB) {
 #pragma unroll
       for (int r = 0; r < R; ++r) v[b][r] = s[lpad(j + r * NB)];
-      if constexpr (NS > 1) {
-        float2 w[R];
-        twiddle_powers<R>(w, (j & (NS - 1)) * (TWN / (NS * R)));
-#pragma unroll
-        for (int r = 1; r < R; ++r) v[b][r] = twmul<INV>(v[b][r], w[r]);
-      }
+      if constexpr (NS > 1) apply_twiddles<R, INV>(v[b], (j & (NS - 1)) * (TWN / (NS * R)));
       DFT<R, INV>::run(v[b]);
     }
   }

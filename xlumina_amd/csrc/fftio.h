@@ -1,0 +1,163 @@
+// Stockham FFT passes with direct I/O at the ends of the pass chain.
+//
+// Each pass is the radix-R Stockham step of fft.h, but its source can be LDS,
+// a caller functor (global loads, masks, on-the-fly generation) or registers,
+// and its sink LDS, a caller functor (global stores of only the samples that
+// are needed) or registers.  The forward chain uses radices 16,...,16,REM and
+// the inverse chain REM,16,...,16, so the last forward pass leaves element
+// j + q*(L/R) in register v[b][q] (j = t + b*TPF) and the first inverse pass
+// reads exactly that layout: a frequency-domain product (transfer function,
+// chirp spectrum) is applied in registers between the two chains, with no
+// LDS round trip.
+//
+// Functor interfaces:
+//   load(int idx) -> float2        input sample idx in [0, L) (0 where padded)
+//   store(int idx, float2 v)       output sample idx (the functor skips unneeded ones)
+//   mul(int idx, int slot, float2 v) -> float2   frequency-domain product
+#pragma once
+#include "fft.h"
+
+namespace xl {
+
+constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x / 2); }
+
+template <int L> struct FftPlan {
+  static constexpr int LG = ilog2c(L);
+  static constexpr int N16 = LG / 4;
+  static constexpr int REM = 1 << (LG % 4);
+  static constexpr int NP = N16 + (REM > 1 ? 1 : 0);
+  static constexpr int radix(bool fwd, int i) {
+    return fwd ? (i < N16 ? 16 : REM) : ((REM > 1) ? (i == 0 ? REM : 16) : 16);
+  }
+  static constexpr int ns(bool fwd, int i) {
+    int p = 1;
+    for (int k = 0; k < i; ++k) p *= radix(fwd, k);
+    return p;
+  }
+  // register layout at the forward/inverse hand-off
+  static constexpr int RH = radix(true, NP - 1);
+  static constexpr int NBH = L / RH;
+};
+
+template <int BPT, int R> struct Regs {
+  float2 v[BPT][R];
+};
+
+enum { SRC_LDS = 0, SRC_FN = 1, SRC_REG = 2 };
+enum { DST_LDS = 0, DST_FN = 1, DST_REG = 2 };
+
+// One pass.  SRC_REG / DST_REG use `reg` (shape [BPT][R] of this pass).
+template <int L, int R, int NS, int TPF, bool INV, int SRC, int DST, class Load, class Store, class RegT>
+__device__ __forceinline__ void pass_io(float2* s, int t, Load& load, Store& store, RegT& reg) {
+  constexpr int NB = L / R;
+  constexpr int BPT = (NB + TPF - 1) / TPF;
+  float2 v[BPT][R];
+#pragma unroll
+  for (int b = 0; b < BPT; ++b) {
+    const int j = t + b * TPF;
+    if (NB % TPF == 0 || j < NB) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if constexpr (SRC == SRC_LDS) v[b][r] = s[lpad(j + r * NB)];
+        else if constexpr (SRC == SRC_FN) v[b][r] = load(j + r * NB);
+        else v[b][r] = reg.v[b][r];
+      }
+      if constexpr (NS > 1) apply_twiddles<R, INV>(v[b], (j & (NS - 1)) * (TWN / (NS * R)));
+      DFT<R, INV>::run(v[b]);
+    }
+  }
+  // LDS may still be read by other threads (this pass or the previous phase)
+  if constexpr (DST == DST_LDS) __syncthreads();
+#pragma unroll
+  for (int b = 0; b < BPT; ++b) {
+    const int j = t + b * TPF;
+    if (NB % TPF == 0 || j < NB) {
+      const int idxD = (j / NS) * NS * R + (j & (NS - 1));
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if constexpr (DST == DST_LDS) s[lpad(idxD + r * NS)] = v[b][r];
+        else if constexpr (DST == DST_FN) store(idxD + r * NS, v[b][r]);
+        else reg.v[b][r] = v[b][r];
+      }
+    }
+  }
+  if constexpr (DST == DST_LDS) __syncthreads();
+}
+
+struct NoIO {
+  __device__ float2 operator()(int) const { return make_float2(0.f, 0.f); }
+  __device__ void operator()(int, float2) const {}
+};
+
+// Pass chain i..NP-1 of the forward (FWD = true, radices 16..REM) or inverse
+// (radices REM..16) plan.  The first pass reads SRC, the last writes DST, the
+// others go through LDS.  Every pass that writes LDS first waits for all
+// threads (so a chain may follow any earlier LDS use); a chain ending in
+// DST_FN / DST_REG leaves LDS free without a final barrier.
+template <int L, int TPF, bool INV, bool FWD, int I, int SRC, int DST, class Load, class Store, class RegIn,
+          class RegOut>
+__device__ __forceinline__ void chain_io(float2* s, int t, Load& load, Store& store, RegIn& rin, RegOut& rout) {
+  using P = FftPlan<L>;
+  constexpr int R = P::radix(FWD, I), NS = P::ns(FWD, I);
+  constexpr bool first = I == 0, last = I == P::NP - 1;
+  constexpr int S = first ? SRC : SRC_LDS;
+  constexpr int D = last ? DST : DST_LDS;
+  if constexpr (first) {
+    pass_io<L, R, NS, TPF, INV, S, D>(s, t, load, store, rin);
+  } else if constexpr (last) {
+    pass_io<L, R, NS, TPF, INV, S, D>(s, t, load, store, rout);
+  } else {
+    pass_io<L, R, NS, TPF, INV, S, D>(s, t, load, store, rout);
+  }
+  if constexpr (!last) chain_io<L, TPF, INV, FWD, I + 1, SRC, DST>(s, t, load, store, rin, rout);
+}
+
+template <int L, int TPF> struct HandOff {
+  using P = FftPlan<L>;
+  static constexpr int R = P::RH;
+  static constexpr int NB = P::NBH;
+  static constexpr int BPT = (NB + TPF - 1) / TPF;
+  using Type = Regs<BPT, R>;
+  // element index held in slot (b, q)
+  __device__ static int index(int t, int b, int q) { return t + b * TPF + q * NB; }
+  __device__ static bool valid(int t, int b) { return NB % TPF == 0 || t + b * TPF < NB; }
+};
+
+// forward FFT: SRC -> registers (hand-off layout)
+template <int L, int TPF, int SRC, class Load>
+__device__ __forceinline__ void fft_forward_to_regs(float2* s, int t, Load& load,
+                                                    typename HandOff<L, TPF>::Type& out) {
+  NoIO none;
+  typename HandOff<L, TPF>::Type dummy;
+  chain_io<L, TPF, false, true, 0, SRC, DST_REG>(s, t, load, none, dummy, out);
+}
+
+// inverse FFT: registers (hand-off layout) -> DST
+template <int L, int TPF, int DST, class Store>
+__device__ __forceinline__ void fft_inverse_from_regs(float2* s, int t, typename HandOff<L, TPF>::Type& in,
+                                                      Store& store) {
+  NoIO none;
+  chain_io<L, TPF, true, false, 0, SRC_REG, DST>(s, t, none, store, in, in);
+}
+
+// Output layout of the last pass of a plan: slot (b, q) holds index j + q*NS,
+// j = t + b*TPF < NB (NS = NB = L / R_last).
+template <int L, int TPF, bool FWD> struct LastPass {
+  using P = FftPlan<L>;
+  static constexpr int R = P::radix(FWD, P::NP - 1);
+  static constexpr int NB = L / R;
+  static constexpr int BPT = (NB + TPF - 1) / TPF;
+  using Type = Regs<BPT, R>;
+  __device__ static int index(int t, int b, int q) { return t + b * TPF + q * NB; }
+  __device__ static bool valid(int t, int b) { return NB % TPF == 0 || t + b * TPF < NB; }
+};
+
+// full transform, SRC -> DST (no hand-off)
+template <int L, int TPF, bool INV, int SRC, int DST, class Load, class Store>
+__device__ __forceinline__ void fft_io(float2* s, int t, Load& load, Store& store,
+                                       typename LastPass<L, TPF, !INV>::Type& out) {
+  typename LastPass<L, TPF, !INV>::Type dummy;
+  chain_io<L, TPF, INV, !INV, 0, SRC, DST>(s, t, load, store, dummy, out);
+}
+
+}  // namespace xl

@@ -22,6 +22,7 @@
 //   rs_final  : row IFFT of T, keep columns 0..N-1 -> output (+ |E|^2)
 #pragma once
 #include "fft.h"
+#include "fftio.h"
 #include "masks.h"
 #include "timing.h"
 
@@ -62,33 +63,18 @@ __device__ __forceinline__ float2 rs_kernel_value(double X, double Y, double z, 
   return make_float2(fr * e.x - fi * e.y, fr * e.y + fi * e.x);
 }
 
-// tile index of element (r, c) for an array with nrg row groups
-__device__ __forceinline__ int64_t tile_idx(int r, int c, int nrg) {
-  return ((int64_t)(c >> 2) * nrg + (r >> 2)) * 16 + (r & 3) * 4 + (c & 3);
-}
-
-// Write 4 LDS rows (length >= ncols) to tiles at row group rg; thread-linear so
-// each group of 16 lanes writes one 128 B tile.
-template <int P>
-__device__ __forceinline__ void store_rows_to_tiles(float2* dst, const float2* lds, int rg, int nrg, int ncols,
-                                                    int tid, int nthr) {
-  constexpr int LS = LdsLen<P>::value;
-  const int total = ncols * 4;
-  for (int e = tid; e < total; e += nthr) {
-    const int cgp = e >> 4, rr = (e >> 2) & 3, cc = e & 3;
-    const int c = cgp * 4 + cc;
-    dst[((int64_t)cgp * nrg + rg) * 16 + rr * 4 + cc] = lds[rr * LS + lpad(c)];
-  }
+// Hr tile index of (row r, column c): P/8 row groups
+__device__ __forceinline__ int64_t hr_idx(int r, int c, int P) {
+  return ((int64_t)(c >> 2) * (P / 8) + (r >> 2)) * 16 + (r & 3) * 4 + (c & 3);
 }
 
 template <int P>
-__global__ __launch_bounds__(P / 4) void rs_hrow_kernel(RsArgs a) {
-  constexpr int TPF = P / 16, LS = LdsLen<P>::value;
+__global__ __launch_bounds__(4 * FftThreads<P>::value) void rs_hrow_kernel(RsArgs a) {
+  constexpr int TPF = FftThreads<P>::value, LS = LdsLen<P>::value;
   __shared__ float2 lds[4 * LS];
   const int tid = threadIdx.x, f = tid / TPF, t = tid % TPF;
   float2* s = lds + f * LS;
-  const int rg = blockIdx.x;             // rows 4rg..4rg+3 of 0..P/2-1
-  const int r = rg * 4 + f;
+  const int r = blockIdx.x * 4 + f;      // rows 0..P/2-1
   double dy;
   const bool rowok = lag_of(a.y, r, a.n, P, &dy);
   for (int c = t; c <= P / 2; c += TPF) {
@@ -99,27 +85,28 @@ __global__ __launch_bounds__(P / 4) void rs_hrow_kernel(RsArgs a) {
     if (c > 0 && c < P / 2) s[lpad(P - c)] = v;
   }
   __syncthreads();
-  fft_lds<P, TPF, false>(s, t);
-  store_rows_to_tiles<P>(a.hr, lds, rg, P / 8, P, tid, P / 4);
+  NoIO none;
+  auto store = [&](int c, float2 v) { a.hr[hr_idx(r, c, P)] = v; };
+  typename LastPass<P, TPF, true>::Type unused;
+  fft_io<P, TPF, false, SRC_LDS, DST_FN>(s, t, none, store, unused);
 }
 
+// One block per input row: the masks / Ez of each pixel are computed once by
+// one thread for all components, component c is transformed by threads
+// [c*TPF, (c+1)*TPF) in LDS buffer c.  Rows 4g..4g+3 run back to back on one
+// XCD, so the four 32 B row slices of each 128 B tile meet in the same L2.
 template <int P>
-__global__ __launch_bounds__(P / 4) void rs_urow_kernel(RsArgs a) {
+__global__ __launch_bounds__(3 * (P / 16)) void rs_urow_kernel(RsArgs a) {
   constexpr int TPF = P / 16, LS = LdsLen<P>::value;
-  constexpr int PPT = (P / 2 + TPF - 1) / TPF;   // pixels per thread (row length <= P/2)
-  __shared__ float2 lds[4 * LS];
-  const int tid = threadIdx.x, f = tid / TPF, t = tid % TPF;
-  float2* s = lds + f * LS;
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int f = tid / TPF, t = tid % TPF;
   const int nrg = (a.n + 3) >> 2;
-  const int rg = blockIdx.x;
-  const int row = rg * 4 + f;
-  const bool rowok = row < a.n;
-  float2 v[3][PPT];
-#pragma unroll
-  for (int i = 0; i < PPT; ++i) {
-    const int col = t + i * TPF;
+  const int row = xcd_remap(blockIdx.x, gridDim.x);
+  const int nc = a.ncomp;
+  for (int col = tid; col < P / 2; col += nthr) {
     float2 e0 = make_float2(0.f, 0.f), e1 = e0, e2 = e0;
-    if (rowok && col < a.n) {
+    if (row < a.n && col < a.n) {
       const int64_t pix = (int64_t)row * a.n + col;
       e0 = a.in[pix];
       if (a.nin > 1) e1 = a.in[a.in_cs + pix];
@@ -127,132 +114,170 @@ __global__ __launch_bounds__(P / 4) void rs_urow_kernel(RsArgs a) {
       apply_mask(a.mask, pix, e0, e1, e2);
       if (a.ez_mode != EZ_NONE) e2 = build_ez(a.ez_mode, e0, e1, a.x[col], a.y[row], a.z);
     }
-    v[0][i] = e0; v[1][i] = e1; v[2][i] = e2;
+    lds[lpad(col)] = e0;
+    lds[lpad(col + P / 2)] = make_float2(0.f, 0.f);
+    if (nc > 1) { lds[LS + lpad(col)] = e1; lds[LS + lpad(col + P / 2)] = make_float2(0.f, 0.f); }
+    if (nc > 2) { lds[2 * LS + lpad(col)] = e2; lds[2 * LS + lpad(col + P / 2)] = make_float2(0.f, 0.f); }
   }
-  for (int c = 0; c < a.ncomp; ++c) {
-#pragma unroll
-    for (int i = 0; i < PPT; ++i) {
-      const int col = t + i * TPF;
-      if (col < P / 2) {
-        s[lpad(col)] = c == 0 ? v[0][i] : (c == 1 ? v[1][i] : v[2][i]);
-        s[lpad(col + P / 2)] = make_float2(0.f, 0.f);
-      }
-    }
-    __syncthreads();
-    fft_lds<P, TPF, false>(s, t);
-    store_rows_to_tiles<P>(a.t + (int64_t)c * nrg * 4 * P, lds, rg, nrg, P, tid, P / 4);
-    __syncthreads();
-  }
+  __syncthreads();
+  const int rg = row >> 2, rr = row & 3;
+  float2* dst = a.t + (int64_t)f * nrg * 4 * P + rg * 16 + rr * 4;
+  NoIO none;
+  auto store = [&](int col, float2 v) { dst[(int64_t)(col >> 2) * nrg * 16 + (col & 3)] = v; };
+  typename LastPass<P, TPF, true>::Type unused;
+  fft_io<P, TPF, false, SRC_LDS, DST_FN>(lds + f * LS, t, none, store, unused);
 }
 
+// Column pass.  A block holds G = 4 / (1 + ncomp) column slots; each slot
+// has 1 + ncomp thread groups that run side by side: group 0 transforms the
+// kernel column (Hr, mirrored), groups 1..ncomp the component columns (T).
+// At the forward/inverse hand-off group 0 leaves the kernel spectrum in its
+// LDS buffer, the component groups multiply their register-held spectra by it
+// and run the inverse transform, storing rows 0..N-1 back in place.  Nothing is
+// held in registers across components, so each thread keeps one radix-16
+// butterfly (1024 threads at P = 4096) without spilling.
 template <int P>
 __global__ __launch_bounds__(P / 4) void rs_col_kernel(RsArgs a) {
   constexpr int TPF = P / 16, LS = LdsLen<P>::value;
-  constexpr int EPT = P / TPF;   // 16 elements per thread per column
+  using HO = HandOff<P, TPF>;
   __shared__ float2 lds[4 * LS];
-  const int tid = threadIdx.x, f = tid / TPF, t = tid % TPF;
-  float2* s = lds + f * LS;
-  const int cg = xcd_remap(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x, g = tid / TPF, t = tid % TPF;
+  float2* s = lds + g * LS;
+  const int ng = 1 + a.ncomp;           // groups per column slot
+  const int G = 4 / ng;                 // column slots per block
+  const int slot = g / ng, role = g - slot * ng;
+  const bool active = slot < G;
+  const int col = blockIdx.x * G + slot;
   const int nrg = (a.n + 3) >> 2;
-  // 1) Hr columns (rows 0..P/2-1 stored, mirrored to P/2+1..P-1, row P/2 zero)
-  {
-    const float2* src = a.hr + (int64_t)cg * (P / 8) * 16;
-    for (int e = tid; e < (P / 2) * 4; e += P / 4) {
-      const int r = (e >> 4) * 4 + ((e >> 2) & 3), cc = e & 3;
-      const float2 v = src[e];
-      lds[cc * LS + lpad(r)] = v;
-      if (r > 0) lds[cc * LS + lpad(P - r)] = v;
+  const int nr = nrg * 4;
+  const int cg = col >> 2, cc = col & 3;
+  const float2* hsrc = a.hr + (int64_t)cg * (P / 8) * 16 + cc;
+  float2* tc = a.t + (int64_t)(role > 0 ? role - 1 : 0) * nrg * 4 * P + (int64_t)cg * nrg * 16 + cc;
+  auto load = [&](int r) -> float2 {
+    if (!active) return make_float2(0.f, 0.f);
+    if (role == 0) {
+      const int rr = r <= P / 2 ? r : P - r;
+      return rr == P / 2 ? make_float2(0.f, 0.f) : hsrc[(rr >> 2) * 16 + (rr & 3) * 4];
     }
-    if (tid < 4) lds[tid * LS + lpad(P / 2)] = make_float2(0.f, 0.f);
+    return r < nr ? tc[(r >> 2) * 16 + (r & 3) * 4] : make_float2(0.f, 0.f);
+  };
+  typename HO::Type u;
+  fft_forward_to_regs<P, TPF, SRC_FN>(s, t, load, u);
+  // kernel spectrum of this slot -> its group-0 buffer (hand-off layout)
+  __syncthreads();
+  if (active && role == 0) {
+#pragma unroll
+    for (int b = 0; b < HO::BPT; ++b)
+#pragma unroll
+      for (int q = 0; q < HO::R; ++q) {
+        const float2 h = u.v[b][q] * a.scale;
+        s[lpad(HO::index(t, b, q))] = a.adjoint ? conjf2(h) : h;
+      }
   }
   __syncthreads();
-  fft_lds<P, TPF, false>(s, t);
-  float2 hh[EPT];
+  if (active && role > 0) {
+    const float2* hs = lds + (slot * ng) * LS;
 #pragma unroll
-  for (int i = 0; i < EPT; ++i) {
-    float2 h = s[lpad(t + i * TPF)] * a.scale;
-    hh[i] = a.adjoint ? conjf2(h) : h;
-  }
-  __syncthreads();
-  for (int c = 0; c < a.ncomp; ++c) {
-    float2* tc = a.t + (int64_t)c * nrg * 4 * P + (int64_t)cg * nrg * 16;
-    for (int e = tid; e < nrg * 16; e += P / 4) {
-      const int r = (e >> 4) * 4 + ((e >> 2) & 3), cc = e & 3;
-      lds[cc * LS + lpad(r)] = tc[e];
-    }
-    for (int e = tid; e < 4 * (P - nrg * 4); e += P / 4) {
-      const int cc = e & 3, r = nrg * 4 + (e >> 2);
-      lds[cc * LS + lpad(r)] = make_float2(0.f, 0.f);
-    }
-    __syncthreads();
-    fft_lds<P, TPF, false>(s, t);
+    for (int b = 0; b < HO::BPT; ++b)
 #pragma unroll
-    for (int i = 0; i < EPT; ++i) {
-      const int idx = lpad(t + i * TPF);
-      s[idx] = cmul(s[idx], hh[i]);
-    }
-    __syncthreads();
-    fft_lds<P, TPF, true>(s, t);
-    for (int e = tid; e < nrg * 16; e += P / 4) {
-      const int r = (e >> 4) * 4 + ((e >> 2) & 3), cc = e & 3;
-      tc[e] = lds[cc * LS + lpad(r)];
-    }
-    __syncthreads();
+      for (int q = 0; q < HO::R; ++q) u.v[b][q] = cmul(u.v[b][q], hs[lpad(HO::index(t, b, q))]);
   }
+  // the inverse passes write only this group's own buffer; group 0's kernel
+  // spectrum has been consumed once every group passed the barrier inside
+  auto store = [&](int r, float2 v) {
+    if (active && role > 0 && r < nr) tc[(r >> 2) * 16 + (r & 3) * 4] = v;
+  };
+  fft_inverse_from_regs<P, TPF, DST_FN>(s, t, u, store);
 }
 
+// One block per output row: component c is inverse-transformed by thread
+// group c from its LDS buffer (its row of the tiles staged there), the first N
+// samples are stored from registers, and |E|^2 is summed over the components
+// through LDS.  Rows 4g..4g+3 run back to back on one XCD so the 32 B row
+// slices of each 128 B tile are read from the same L2.
 template <int P>
-__global__ __launch_bounds__(P / 4) void rs_final_kernel(RsArgs a) {
+__global__ __launch_bounds__(3 * (P / 16)) void rs_final_kernel(RsArgs a) {
   constexpr int TPF = P / 16, LS = LdsLen<P>::value;
-  constexpr int PPT = (P / 2 + TPF - 1) / TPF;
-  __shared__ float2 lds[4 * LS];
-  const int tid = threadIdx.x, f = tid / TPF, t = tid % TPF;
-  float2* s = lds + f * LS;
+  using LP = LastPass<P, TPF, false>;
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int f = tid / TPF, t = tid % TPF;
   const int nrg = (a.n + 3) >> 2;
-  const int rg = blockIdx.x;
-  const int row = rg * 4 + f;
-  float inten[PPT];
-#pragma unroll
-  for (int i = 0; i < PPT; ++i) inten[i] = 0.f;
-  for (int c = 0; c < a.ncomp; ++c) {
-    const float2* tc = a.t + (int64_t)c * nrg * 4 * P;
-    for (int e = tid; e < 4 * P; e += P / 4) {
-      const int cgp = e >> 4, rr = (e >> 2) & 3, cc = e & 3;
-      lds[rr * LS + lpad(cgp * 4 + cc)] = tc[((int64_t)cgp * nrg + rg) * 16 + rr * 4 + cc];
-    }
-    __syncthreads();
-    fft_lds<P, TPF, true>(s, t);
-    if (row < a.n) {
-      float2* oc = a.out + (int64_t)c * a.out_cs + (int64_t)row * a.n;
-#pragma unroll
-      for (int i = 0; i < PPT; ++i) {
-        const int col = t + i * TPF;
-        if (col < a.n) {
-          const float2 v = s[lpad(col)];
-          oc[col] = v;
-          inten[i] += v.x * v.x + v.y * v.y;
-        }
-      }
-    }
-    __syncthreads();
+  const int row = xcd_remap(blockIdx.x, gridDim.x);
+  const int rg = row >> 2, rr = row & 3;
+  const int nc = a.ncomp;
+  for (int e = tid; e < nc * P; e += nthr) {
+    const int c = e / P, col = e - c * P;
+    lds[c * LS + lpad(col)] =
+        a.t[(int64_t)c * nrg * 4 * P + ((int64_t)(col >> 2) * nrg + rg) * 16 + rr * 4 + (col & 3)];
   }
-  if (a.intensity && row < a.n) {
+  __syncthreads();
+  NoIO none;
+  typename LP::Type o;
+  fft_io<P, TPF, true, SRC_LDS, DST_REG>(lds + f * LS, t, none, none, o);
+  const bool rowok = row < a.n;
+  if (rowok) {
+    float2* oc = a.out + (int64_t)f * a.out_cs + (int64_t)row * a.n;
 #pragma unroll
-    for (int i = 0; i < PPT; ++i) {
-      const int col = t + i * TPF;
-      if (col < a.n) a.intensity[(int64_t)row * a.n + col] = inten[i];
-    }
+    for (int b = 0; b < LP::BPT; ++b)
+#pragma unroll
+      for (int q = 0; q < LP::R; ++q) {
+        const int col = LP::index(t, b, q);
+        if (LP::valid(t, b) && col < a.n) oc[col] = o.v[b][q];
+      }
+  }
+  if (a.intensity) {
+    float* ib = reinterpret_cast<float*>(lds);   // |E_c|^2 of column col at ib[c * LS * 2 + col]
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < LP::BPT; ++b)
+#pragma unroll
+      for (int q = 0; q < LP::R; ++q) {
+        const int col = LP::index(t, b, q);
+        if (LP::valid(t, b) && col < a.n) ib[f * LS * 2 + col] = o.v[b][q].x * o.v[b][q].x + o.v[b][q].y * o.v[b][q].y;
+      }
+    __syncthreads();
+    if (rowok)
+      for (int col = tid; col < a.n; col += nthr) {
+        float sum = 0.f;
+        for (int c = 0; c < nc; ++c) sum += ib[c * LS * 2 + col];
+        a.intensity[(int64_t)row * a.n + col] = sum;
+      }
   }
 }
 
 template <int P>
 inline hipError_t rs_launch(const RsArgs& a, hipStream_t st) {
   const int nrg = (a.n + 3) >> 2;
-  dim3 blk(P / 4);
+  dim3 blk(4 * FftThreads<P>::value);
   { KTimer t(K_RS_HROW, st); hipLaunchKernelGGL(rs_hrow_kernel<P>, dim3(P / 8), blk, 0, st, a); }
-  { KTimer t(K_RS_UROW, st); hipLaunchKernelGGL(rs_urow_kernel<P>, dim3(nrg), blk, 0, st, a); }
-  { KTimer t(K_RS_COL, st); hipLaunchKernelGGL(rs_col_kernel<P>, dim3(P / 4), blk, 0, st, a); }
-  { KTimer t(K_RS_FINAL, st); hipLaunchKernelGGL(rs_final_kernel<P>, dim3(nrg), blk, 0, st, a); }
+  {
+    KTimer t(K_RS_UROW, st);
+    static bool attr_set = false;   // up to three 4096-point buffers of dynamic LDS
+    if (!attr_set) {
+      (void)hipFuncSetAttribute((const void*)rs_urow_kernel<P>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                3 * LdsLen<P>::value * (int)sizeof(float2));
+      attr_set = true;
+    }
+    hipLaunchKernelGGL(rs_urow_kernel<P>, dim3(nrg * 4), dim3(a.ncomp * (P / 16)), 
+                       (size_t)a.ncomp * LdsLen<P>::value * sizeof(float2), st, a);
+  }
+  {
+    KTimer t(K_RS_COL, st);
+    const int G = 4 / (1 + a.ncomp);
+    hipLaunchKernelGGL(rs_col_kernel<P>, dim3((P + G - 1) / G), dim3(P / 4), 0, st, a);
+  }
+  {
+    KTimer t(K_RS_FINAL, st);
+    static bool attr_set = false;
+    if (!attr_set) {
+      (void)hipFuncSetAttribute((const void*)rs_final_kernel<P>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                3 * LdsLen<P>::value * (int)sizeof(float2));
+      attr_set = true;
+    }
+    hipLaunchKernelGGL(rs_final_kernel<P>, dim3(nrg * 4), dim3(a.ncomp * (P / 16)),
+                       (size_t)a.ncomp * LdsLen<P>::value * sizeof(float2), st, a);
+  }
   return hipGetLastError();
 }
 

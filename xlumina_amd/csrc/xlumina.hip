@@ -158,8 +158,7 @@ int plan_czt(const xl_czt_params* p, CztPlan* cp) {
     cp->rin = p->my; cp->lin_x = p->mx; cp->lout_x = p->nx_in; cp->lout_y = p->ny_in;
   }
   cp->tables_bytes = (size_t)2 * 3 * xl::MAX_FFT * sizeof(float2);
-  const int64_t r4 = ((cp->rin + 3) / 4) * 4, o4 = ((cp->lout_x + 3) / 4) * 4;
-  cp->tiles_bytes = (size_t)cp->ncomp * r4 * o4 * sizeof(float2);
+  cp->tiles_bytes = (size_t)cp->ncomp * cp->rin * cp->lout_x * sizeof(float2);
   return XL_OK;
 }
 
@@ -272,9 +271,10 @@ int xl_czt(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream) {
   std::memset(&a, 0, sizeof(a));
   const AxisPlan* axp[2] = {&cp.x, &cp.y};
   xl::CztAxis* axo[2] = {&a.ax, &a.ay};
+  xl::CztSetupPair pair;
   for (int i = 0; i < 2; ++i) {
     const AxisPlan& ap = *axp[i];
-    xl::CztSetupArgs s;
+    xl::CztSetupArgs& s = pair.ax[i];
     s.pre = tab + i * 3 * xl::MAX_FFT;
     s.gf = s.pre + xl::MAX_FFT;
     s.post = s.gf + xl::MAX_FFT;
@@ -285,7 +285,6 @@ int xl_czt(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream) {
     s.d1_dm = ap.d1_dm;
     s.dl_dm = ap.dl_dm;
     s.th_turns = ap.th_turns;
-    if (int rc = dispatch_q(ap.q, [&](auto c) { xl::czt_setup_launch<decltype(c)::value>(s, st); })) return rc;
     xl::CztAxis& ax = *axo[i];
     ax.q = ap.q;
     ax.pre = s.pre;
@@ -294,6 +293,17 @@ int xl_czt(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream) {
     ax.lin = p->adjoint ? ap.M : ap.m;
     ax.lout = p->adjoint ? ap.m : ap.M;
     ax.woff = p->adjoint ? 0 : ap.m;
+  }
+  if (cp.x.q == cp.y.q) {
+    if (int rc = dispatch_q(cp.x.q, [&](auto c) { xl::czt_setup_launch<decltype(c)::value>(pair, 2, st); }))
+      return rc;
+  } else {
+    for (int i = 0; i < 2; ++i) {
+      xl::CztSetupPair one;
+      one.ax[0] = pair.ax[i];
+      if (int rc = dispatch_q(axp[i]->q, [&](auto c) { xl::czt_setup_launch<decltype(c)::value>(one, 1, st); }))
+        return rc;
+    }
   }
   a.in = (const float2*)p->in;
   a.in_cs = p->in_comp_stride;
