@@ -140,6 +140,13 @@ int xl_rs_propagate(const xl_rs_params* p, void* workspace, size_t workspace_byt
 size_t xl_czt_workspace_bytes(const xl_czt_params* p);
 int xl_czt(const xl_czt_params* p, void* workspace, size_t workspace_bytes, void* stream);
 
+/* Same operator as xl_czt (same params, forward and adjoint) as a dense
+ * zoom-DFT on MFMA: K_y G K_x^T with the kernels generated in float64 phase
+ * and split into bf16 hi + lo planes, FP32 accumulation (12 bf16 MFMAs per
+ * complex MAC); relative error ~1e-5.  Pays off for small output windows. */
+size_t xl_czt_gemm_workspace_bytes(const xl_czt_params* p);
+int xl_czt_gemm(const xl_czt_params* p, void* workspace, size_t workspace_bytes, void* stream);
+
 /* Standalone pre-ops and their adjoint: SLM / sSLM / LCD / linear_polarizer
  * (optical_elements.py:87-332), high_NA_objective_lens (:515-594). */
 int xl_pre_forward(const xl_pre_params* p, void* stream);

@@ -72,6 +72,8 @@ SIGNATURES = {
     "xl_rs_propagate": (C.c_int, [C.POINTER(RsParams), C.c_void_p, C.c_size_t, C.c_void_p]),
     "xl_czt_workspace_bytes": (C.c_size_t, [C.POINTER(CztParams)]),
     "xl_czt": (C.c_int, [C.POINTER(CztParams), C.c_void_p, C.c_size_t, C.c_void_p]),
+    "xl_czt_gemm_workspace_bytes": (C.c_size_t, [C.POINTER(CztParams)]),
+    "xl_czt_gemm": (C.c_int, [C.POINTER(CztParams), C.c_void_p, C.c_size_t, C.c_void_p]),
     "xl_pre_forward": (C.c_int, [C.POINTER(PreParams), C.c_void_p]),
     "xl_pre_backward": (C.c_int, [C.POINTER(PreParams), C.c_void_p]),
     "xl_transfer_function_rs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,

@@ -124,6 +124,7 @@ class CZTConfig:
     nin: int
     mode: int
     mask_struct: tuple = field(default_factory=tuple)
+    form: str = "bluestein"   # resolved implementation, used for forward and adjoint
 
 
 class _CZTFunction(torch.autograd.Function):
@@ -131,7 +132,7 @@ class _CZTFunction(torch.autograd.Function):
     def forward(ctx, cfg: CZTConfig, src, p00, p01, p10, p11):
         params = (p00, p01, p10, p11)
         masks = _rebuild(cfg.mask_struct, params)
-        out = ops.czt(src, cfg.geom, masks=masks, ez_mode=cfg.ez_mode, nin=cfg.nin, mode=cfg.mode)
+        out = ops.czt(src, cfg.geom, masks=masks, ez_mode=cfg.ez_mode, nin=cfg.nin, mode=cfg.mode, form=cfg.form)
         ctx.cfg = cfg
         ctx.save_for_backward(src, *[p if p is not None else torch.empty(0) for p in params])
         ctx.has = [p is not None for p in params]
@@ -144,7 +145,7 @@ class _CZTFunction(torch.autograd.Function):
         saved = ctx.saved_tensors
         src = saved[0]
         params = [saved[1 + i] if ctx.has[i] else None for i in range(4)]
-        g_pre = ops.czt(gout.contiguous(), g, adjoint=True, mode=cfg.mode)
+        g_pre = ops.czt(gout.contiguous(), g, adjoint=True, mode=cfg.mode, form=cfg.form)
         masks = _rebuild(cfg.mask_struct, params)
         hna = cfg.mode == N.XL_CZT_HNA
         need_params = ((ctx.needs_input_grad[2] and params[0] is not None, ctx.needs_input_grad[3] and params[1] is not None),
@@ -166,11 +167,11 @@ class _CZTFunction(torch.autograd.Function):
 
 
 def czt(src: torch.Tensor, geom: ops.CztGeometry, masks: Sequence[MaskSpec] = (), ez_mode: int = N.XL_EZ_NONE,
-        nin: Optional[int] = None, mode: Optional[int] = None) -> torch.Tensor:
-    """Differentiable CZT / VCZT / high-NA VCZT of src (C, ny, nx)."""
+        nin: Optional[int] = None, mode: Optional[int] = None, form: Optional[str] = None) -> torch.Tensor:
+    """Differentiable CZT / VCZT / high-NA VCZT of src (C, ny, nx); `form` as ops.czt."""
     nin = src.shape[0] if nin is None else nin
     mode = geom.mode if mode is None else mode
-    cfg = CZTConfig(geom, ez_mode, nin, mode, _mask_structure(masks))
+    cfg = CZTConfig(geom, ez_mode, nin, mode, _mask_structure(masks), ops.resolve_czt_form(geom, form))
     return _CZTFunction.apply(cfg, src, *_flat_params(masks))
 
 

@@ -1,0 +1,273 @@
+// Chirp z-transform as a dense zoom-DFT on MFMA (split-BF16, FP32 accumulate).
+//
+// The 2-D CZT of the reference (CZT_jit, wave_optics.py:333-357; high-NA form
+// optical_elements.py:600-638) is  out = post2d ( K_y G K_x^T )  with G the
+// pre-op'd field and, per axis, K[k][j] = post_k g[m+k-j] pre_j -- the same
+// float64 phases as the Bluestein tables (czt.h), so both forms compute the
+// same operator.  Here it runs as two complex GEMMs of the form C = A B^T
+// (A: M x K, B: N x K, both K-contiguous):
+//   stage 1: C1 = K_x [G_0; G_1; G_2]^T   (Mx x 3*Nyp)   components batched in N
+//   stage 2: C2 = K_y  B2^T, B2 row (c, kx) = C1[kx][c*Nyp : (c+1)*Nyp]
+// Operands are stored as four bf16 planes (re_hi, re_lo, im_hi, im_lo) with
+// x = hi + lo (hi = bf16(x), lo = bf16(x - hi)); each complex MAC is
+// 4 real products x 3 split terms (hi*hi, hi*lo, lo*hi) = 12 MFMAs
+// (v_mfma_f32_32x32x16_bf16), accumulated in FP32 as C_rr, C_ii, C_ri.
+// The adjoint uses the conjugate-transposed kernels with the same GEMMs.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fft.h"
+#include "masks.h"
+#include "pre.h"
+#include "rs.h"
+
+namespace xl {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(16))) float f32x16;
+
+constexpr int GBM = 128, GBN = 128, GBK = 32;   // block tile (complex), K step
+constexpr int GLDK = GBK + 8;                   // LDS row stride in bf16 (80 B: 16 B aligned, 2-way banks)
+
+__device__ __forceinline__ uint16_t f2bf(float x) {   // round to nearest even (finite inputs)
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16) | ((u & 0xffffu) ? 0x40 : 0);  // inf / nan
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// x -> (hi, lo) bf16 pair
+__device__ __forceinline__ void split_bf16(float x, uint16_t& hi, uint16_t& lo) {
+  hi = f2bf(x);
+  lo = f2bf(x - bf2f(hi));
+}
+
+// Four-plane split storage of a complex matrix.
+struct Planes {
+  uint16_t* p;
+  int64_t ps;   // plane stride (elements)
+  int64_t ld;   // row stride (elements)
+};
+
+__device__ __forceinline__ void store_split(const Planes& d, int64_t off, float2 v) {
+  uint16_t h, l;
+  split_bf16(v.x, h, l);
+  d.p[off] = h;
+  d.p[d.ps + off] = l;
+  split_bf16(v.y, h, l);
+  d.p[2 * d.ps + off] = h;
+  d.p[3 * d.ps + off] = l;
+}
+
+// ---------------------------------------------------------------------------
+// kernel matrices: K[k][j] = post_k g[m+k-j] pre_j as float64 turns
+// ---------------------------------------------------------------------------
+struct KgenArgs {
+  Planes dst;
+  int m, M;            // forward axis lengths (input m, output M)
+  int rows, cols;      // stored matrix (forward: M x colsp, adjoint: m x colsp)
+  int valid_cols;      // forward: m, adjoint: M
+  int adjoint;         // store conj(K)^T
+  double d1_dm, dl_dm, th_turns;
+};
+
+__device__ __forceinline__ double czt_pre_turns(const KgenArgs& a, int j) {
+  const double jj = (double)j;
+  return -jj * a.d1_dm + a.th_turns * 0.5 * jj * jj;
+}
+__device__ __forceinline__ double czt_post_turns(const KgenArgs& a, int k) {
+  const double kk = (double)k;
+  return a.th_turns * 0.5 * kk * kk - (a.d1_dm + kk * a.dl_dm) * (0.5 - 0.5 * a.m);
+}
+
+__global__ void kgen_kernel(KgenArgs a) {
+  const int64_t total = (int64_t)a.rows * a.dst.ld;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e / a.dst.ld), c = (int)(e % a.dst.ld);
+    float2 v = make_float2(0.f, 0.f);
+    if (c < a.valid_cols) {
+      const int k = a.adjoint ? c : r;   // output index of the forward axis
+      const int j = a.adjoint ? r : c;   // input index
+      const int s = a.m + k - j;          // g index, in [1, m+M-1]
+      if (!(a.M >= a.m && s == a.m + a.M - 1)) {
+        const double u = (double)(s - a.m + 1);
+        const double turns = czt_post_turns(a, k) - a.th_turns * 0.5 * u * u + czt_pre_turns(a, j);
+        v = expi_turns(a.adjoint ? -turns : turns);
+      }
+    }
+    store_split(a.dst, e, v);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pre-op: field -> split planes of G, rows (c, y) with per-component row stride
+// ---------------------------------------------------------------------------
+struct GemmPreArgs {
+  const float2* in;
+  int64_t in_cs;
+  int nin, ncomp;
+  int rows, cols, rowsp;    // input plane and padded rows per component
+  Planes dst;               // ld = padded cols
+  int pre_mode;             // PRE_* of czt.h
+  int ez_mode;
+  const double* px;
+  const double* py;
+  double z, k;
+  float2 cte;
+  double hna_radius, hna_f, hna_stm2;
+  MaskArgs mask;
+};
+
+__global__ void gemm_pre_kernel(GemmPreArgs a) {
+  const int64_t total = (int64_t)a.rowsp * a.dst.ld;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int row = (int)(e / a.dst.ld), col = (int)(e % a.dst.ld);
+    float2 v[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+    if (row < a.rows && col < a.cols) {
+      const int64_t pix = (int64_t)row * a.cols + col;
+      float2 e0 = a.in[pix];
+      float2 e1 = a.nin > 1 ? a.in[a.in_cs + pix] : make_float2(0.f, 0.f);
+      float2 e2 = a.nin > 2 ? a.in[2 * a.in_cs + pix] : make_float2(0.f, 0.f);
+      apply_mask(a.mask, pix, e0, e1, e2);
+      if (a.pre_mode == 2 /* PRE_HNA */) {
+        float A[3][2];
+        hna_map(a.px[col], a.py[row], a.hna_radius, a.hna_f, a.hna_stm2, A);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[c] = c_add(c_scale(e0, A[c][0]), c_scale(e1, A[c][1]));
+      } else {
+        v[0] = e0;
+        v[1] = e1;
+        v[2] = (a.ez_mode != EZ_NONE) ? build_ez(a.ez_mode, e0, e1, a.px[col], a.py[row], a.z) : e2;
+        if (a.pre_mode == 1 /* PRE_RS */ || a.pre_mode == 3 /* PRE_CONJ_RS */) {
+          float2 h = rs_kernel_value(a.px[col], a.py[row], a.z, a.k);
+          if (a.pre_mode == 3) h = c_mul(conjf2(h), a.cte);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) v[c] = c_mul(v[c], h);
+        } else if (a.pre_mode == 4 /* PRE_CONST */) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) v[c] = c_mul(v[c], a.cte);
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      if (c < a.ncomp) store_split(a.dst, (int64_t)c * a.rowsp * a.dst.ld + e, v[c]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// C = A B^T, complex, split-BF16 MFMA
+// ---------------------------------------------------------------------------
+struct GemmOperand {
+  const uint16_t* p;
+  int64_t ps;       // plane stride
+  int64_t ld;       // row stride within a batch
+  int rows;         // valid rows (others read as zero)
+  int rb;           // rows per batch: row r lives at (r % rb) * ld + (r / rb) * boff
+  int64_t boff;
+};
+
+struct GemmArgs {
+  GemmOperand A, B;
+  int M, N, K;       // K multiple of GBK (zero padded)
+  int mode;          // 0: split planes C, 1: complex output with post-op
+  Planes c;          // mode 0
+  // mode 1: output element (m, n) -> out[n / nb][m][n % nb] * post
+  float2* out;
+  int64_t out_cs;
+  int nb;            // output columns per component
+  int post_mode;     // POST_* of czt.h
+  const double* qx;
+  const double* qy;
+  double z, k;
+  float2 cte;
+};
+
+__device__ __forceinline__ int64_t gop_off(const GemmOperand& o, int r) {
+  return (int64_t)(r % o.rb) * o.ld + (int64_t)(r / o.rb) * o.boff;
+}
+
+__global__ __launch_bounds__(1024) void cgemm_kernel(GemmArgs a) {
+  // [operand][plane][row][k] bf16, single buffer; the next K step is prefetched into registers
+  __shared__ __attribute__((aligned(16))) uint16_t sm[2][4][GBM][GLDK];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w & 3, wn = w >> 2;                 // 4 x 4 waves of 32 x 32
+  const int bm = blockIdx.y * GBM, bn = blockIdx.x * GBN;
+  // global -> LDS mapping: 4096 chunks of 8 bf16 per K step, 4 per thread
+  uint4 pre[4];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int ch = tid + i * 1024;
+      const int op = ch >> 11, pl = (ch >> 9) & 3, r = (ch >> 2) & 127, kc = ch & 3;
+      const GemmOperand& o = op ? a.B : a.A;
+      const int gr = (op ? bn : bm) + r;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (gr < o.rows) v = *reinterpret_cast<const uint4*>(o.p + pl * o.ps + gop_off(o, gr) + k0 + kc * 8);
+      pre[i] = v;
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int ch = tid + i * 1024;
+      const int op = ch >> 11, pl = (ch >> 9) & 3, r = (ch >> 2) & 127, kc = ch & 3;
+      *reinterpret_cast<uint4*>(&sm[op][pl][r][kc * 8]) = pre[i];
+    }
+  };
+  f32x16 crr = {}, cii = {}, cri = {};
+  gload(0);
+  for (int k0 = 0; k0 < a.K; k0 += GBK) {
+    __syncthreads();
+    sstore();
+    __syncthreads();
+    if (k0 + GBK < a.K) gload(k0 + GBK);
+#pragma unroll
+    for (int kk = 0; kk < GBK; kk += 16) {
+      const int ra = wm * 32 + (lane & 31), rb = wn * 32 + (lane & 31), kq = kk + 8 * (lane >> 5);
+      const bf16x8 arh = *reinterpret_cast<const bf16x8*>(&sm[0][0][ra][kq]);
+      const bf16x8 arl = *reinterpret_cast<const bf16x8*>(&sm[0][1][ra][kq]);
+      const bf16x8 aih = *reinterpret_cast<const bf16x8*>(&sm[0][2][ra][kq]);
+      const bf16x8 ail = *reinterpret_cast<const bf16x8*>(&sm[0][3][ra][kq]);
+      const bf16x8 brh = *reinterpret_cast<const bf16x8*>(&sm[1][0][rb][kq]);
+      const bf16x8 brl = *reinterpret_cast<const bf16x8*>(&sm[1][1][rb][kq]);
+      const bf16x8 bih = *reinterpret_cast<const bf16x8*>(&sm[1][2][rb][kq]);
+      const bf16x8 bil = *reinterpret_cast<const bf16x8*>(&sm[1][3][rb][kq]);
+      crr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arl, brh, crr, 0, 0, 0);
+      crr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, brl, crr, 0, 0, 0);
+      crr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, brh, crr, 0, 0, 0);
+      cii = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ail, bih, cii, 0, 0, 0);
+      cii = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aih, bil, cii, 0, 0, 0);
+      cii = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aih, bih, cii, 0, 0, 0);
+      cri = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arl, bih, cri, 0, 0, 0);
+      cri = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, bil, cri, 0, 0, 0);
+      cri = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ail, brh, cri, 0, 0, 0);
+      cri = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aih, brl, cri, 0, 0, 0);
+      cri = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, bih, cri, 0, 0, 0);
+      cri = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aih, brh, cri, 0, 0, 0);
+    }
+  }
+  // epilogue: lane owns column n = lane % 32 and rows 8*(r/4) + 4*(lane/32) + r%4
+  const int n = bn + wn * 32 + (lane & 31);
+  if (n >= a.N) return;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int m = bm + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (m >= a.M) continue;
+    const float2 v = make_float2(crr[r] - cii[r], cri[r]);
+    if (a.mode == 0) {
+      store_split(a.c, (int64_t)m * a.c.ld + n, v);
+    } else {
+      const int comp = n / a.nb, kx = n - comp * a.nb;
+      float2 o = v;
+      if (a.post_mode == 1 /* POST_RS */) o = c_mul(c_mul(o, rs_kernel_value(a.qx[kx], a.qy[m], a.z, a.k)), a.cte);
+      else if (a.post_mode == 3 /* POST_CONJ_RS */) o = c_mul(o, conjf2(rs_kernel_value(a.qx[kx], a.qy[m], a.z, a.k)));
+      else if (a.post_mode == 2 /* POST_CONST */) o = c_mul(o, a.cte);
+      a.out[(int64_t)comp * a.out_cs + (int64_t)m * a.nb + kx] = o;
+    }
+  }
+}
+
+}  // namespace xl

@@ -15,6 +15,7 @@
 
 #include "../../include/xlumina_hip.h"
 #include "czt.h"
+#include "gemm.h"
 #include "fft.h"
 #include "masks.h"
 #include "pre.h"
@@ -338,6 +339,192 @@ int xl_czt(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream) {
   if (int rc = dispatch_q(a.ax.q, [&](auto c) { xl::czt_rows_launch<decltype(c)::value>(a, st); })) return rc;
   if (int rc = dispatch_q(a.ay.q, [&](auto c) { xl::czt_cols_launch<decltype(c)::value>(a, st); })) return rc;
   return hip_check(hipGetLastError(), "CZT launch");
+}
+
+static dim3 grid_for(int64_t npix);
+
+// ---------------------------------------------------------------- zoom-GEMM CZT
+namespace {
+struct GemmPlan {
+  CztPlan cz;
+  int nxp, nyp, mxp, myp;            // padded (x32) lengths
+  int a1_rows, a1_ld, a2_rows, a2_ld;  // kernel matrices of stage 1 / 2
+  int b_rowsp, b_ld;                 // pre-op planes: rows per component, row stride
+  int64_t c1_ld;                     // stage-1 output row stride
+  int c1_rows;
+  size_t ka_bytes, kb_bytes, g_bytes, c1_bytes;
+};
+
+int r32(int v) { return (v + 31) / 32 * 32; }
+
+int plan_gemm(const xl_czt_params* p, GemmPlan* g) {
+  if (int rc = plan_czt(p, &g->cz)) return rc;
+  g->nxp = r32(p->nx_in); g->nyp = r32(p->ny_in); g->mxp = r32(p->mx); g->myp = r32(p->my);
+  const int nc = p->ncomp;
+  if (!p->adjoint) {
+    g->a1_rows = p->mx; g->a1_ld = g->nxp;        // K_x
+    g->a2_rows = p->my; g->a2_ld = g->nyp;        // K_y
+    g->b_rowsp = g->nyp; g->b_ld = g->nxp;        // G
+    g->c1_rows = p->mx; g->c1_ld = (int64_t)nc * g->nyp;
+  } else {
+    g->a1_rows = p->nx_in; g->a1_ld = g->mxp;     // K_x^H
+    g->a2_rows = p->ny_in; g->a2_ld = g->myp;     // K_y^H
+    g->b_rowsp = g->myp; g->b_ld = g->mxp;        // G'
+    g->c1_rows = p->nx_in; g->c1_ld = (int64_t)nc * g->myp;
+  }
+  g->ka_bytes = (size_t)4 * g->a1_rows * g->a1_ld * 2;
+  g->kb_bytes = (size_t)4 * g->a2_rows * g->a2_ld * 2;
+  g->g_bytes = (size_t)4 * nc * g->b_rowsp * g->b_ld * 2;
+  g->c1_bytes = (size_t)4 * g->c1_rows * g->c1_ld * 2;
+  return XL_OK;
+}
+
+size_t gemm_ws(const GemmPlan& g) {
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  return al(g.ka_bytes) + al(g.kb_bytes) + al(g.g_bytes) + al(g.c1_bytes) + 256;
+}
+
+xl::Planes planes_at(char* base, int rows, int64_t ld) {
+  xl::Planes pl;
+  pl.p = (uint16_t*)base;
+  pl.ld = ld;
+  pl.ps = (int64_t)rows * ld;
+  return pl;
+}
+}  // namespace
+
+size_t xl_czt_gemm_workspace_bytes(const xl_czt_params* p) {
+  GemmPlan g;
+  if (plan_gemm(p, &g)) return 0;
+  return gemm_ws(g);
+}
+
+int xl_czt_gemm(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream) {
+  if (!device_ready()) return fail(XL_EINVAL, "xl_init() has not run on this device");
+  GemmPlan g;
+  if (int rc = plan_gemm(p, &g)) return rc;
+  if (!p->in || !p->out || !p->x || !p->y || !p->xout || !p->yout) return fail(XL_EINVAL, "null pointer argument");
+  if (p->nin < 1 || p->nin > 3 || p->ncomp < 1 || p->ncomp > 3) return fail(XL_EINVAL, "nin / ncomp must be 1..3");
+  if (p->mode < XL_CZT_RS || p->mode > XL_CZT_PLAIN) return fail(XL_EINVAL, "unknown CZT mode");
+  if (p->mode == XL_CZT_HNA && !p->adjoint && (p->nin < 2 || p->ncomp != 3))
+    return fail(XL_EINVAL, "high-NA CZT takes Ex, Ey and returns 3 components");
+  if (p->ez_mode != XL_EZ_NONE && p->ez_mode != XL_EZ_VCZT) return fail(XL_EINVAL, "CZT supports XL_EZ_NONE / XL_EZ_VCZT");
+  if (!p->adjoint && p->ez_mode == XL_EZ_VCZT && (p->nin < 2 || p->ncomp != 3))
+    return fail(XL_EINVAL, "Ez construction needs Ex, Ey and 3 output components");
+  if (!p->adjoint && p->ez_mode == XL_EZ_NONE && p->mode != XL_CZT_HNA && p->ncomp != p->nin)
+    return fail(XL_EINVAL, "ncomp must equal nin without Ez construction");
+  if (p->adjoint && (p->mask[0].kind || p->mask[1].kind || p->ez_mode || p->nin != p->ncomp))
+    return fail(XL_EINVAL, "the adjoint CZT takes ncomp cotangents and no masks / Ez (use xl_pre_backward)");
+  if (int rc = check_mask(p->mask, p->nin)) return rc;
+  const size_t need = gemm_ws(g);
+  if (!ws || ws_bytes < need) return fail(XL_EWORKSPACE, "zoom-GEMM workspace too small: need " + std::to_string(need));
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  char* w = align256((char*)ws);
+  xl::Planes ka = planes_at(w, g.a1_rows, g.a1_ld);
+  w += al(g.ka_bytes);
+  xl::Planes kb = planes_at(w, g.a2_rows, g.a2_ld);
+  w += al(g.kb_bytes);
+  xl::Planes gp = planes_at(w, p->ncomp * g.b_rowsp, g.b_ld);
+  w += al(g.g_bytes);
+  xl::Planes c1 = planes_at(w, g.c1_rows, g.c1_ld);
+  hipStream_t st = (hipStream_t)stream;
+  const int nc = p->ncomp;
+
+  // kernel matrices (x axis for stage 1, y axis for stage 2)
+  const AxisPlan* axp[2] = {&g.cz.x, &g.cz.y};
+  xl::Planes* kd[2] = {&ka, &kb};
+  const int krows[2] = {g.a1_rows, g.a2_rows};
+  for (int i = 0; i < 2; ++i) {
+    xl::KgenArgs k;
+    k.dst = *kd[i];
+    k.m = axp[i]->m;
+    k.M = axp[i]->M;
+    k.rows = krows[i];
+    k.cols = (int)kd[i]->ld;
+    k.valid_cols = p->adjoint ? axp[i]->M : axp[i]->m;
+    k.adjoint = p->adjoint;
+    k.d1_dm = axp[i]->d1_dm;
+    k.dl_dm = axp[i]->dl_dm;
+    k.th_turns = axp[i]->th_turns;
+    xl::KTimer t(xl::K_GEMM_KGEN, st);
+    hipLaunchKernelGGL(xl::kgen_kernel, grid_for((int64_t)k.rows * k.cols), dim3(256), 0, st, k);
+  }
+  // pre-op planes
+  {
+    xl::GemmPreArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.in = (const float2*)p->in;
+    a.in_cs = p->in_comp_stride;
+    a.nin = p->nin;
+    a.ncomp = nc;
+    a.rows = p->adjoint ? p->my : p->ny_in;
+    a.cols = p->adjoint ? p->mx : p->nx_in;
+    a.rowsp = g.b_rowsp;
+    a.dst = gp;
+    a.dst.ps = (int64_t)nc * g.b_rowsp * g.b_ld;
+    a.ez_mode = p->adjoint ? XL_EZ_NONE : p->ez_mode;
+    a.z = p->z;
+    a.k = p->k;
+    a.hna_radius = p->hna_radius;
+    a.hna_f = p->hna_f;
+    a.hna_stm2 = p->hna_radius * p->hna_radius / (p->hna_radius * p->hna_radius + p->hna_f * p->hna_f);
+    a.mask = to_mask(p->mask);
+    const float2 ctec = make_float2((float)p->cte_re, (float)-p->cte_im);
+    if (!p->adjoint) {
+      a.px = p->x; a.py = p->y;
+      a.pre_mode = p->mode == XL_CZT_RS ? xl::PRE_RS : (p->mode == XL_CZT_HNA ? xl::PRE_HNA : xl::PRE_NONE);
+    } else {
+      a.px = p->xout; a.py = p->yout;
+      a.cte = ctec;
+      a.pre_mode = p->mode == XL_CZT_RS ? xl::PRE_CONJ_RS : (p->mode == XL_CZT_HNA ? xl::PRE_CONST : xl::PRE_NONE);
+    }
+    xl::KTimer t(xl::K_GEMM_PRE, st);
+    hipLaunchKernelGGL(xl::gemm_pre_kernel, grid_for((int64_t)g.b_rowsp * g.b_ld), dim3(256), 0, st, a);
+  }
+  // stage 1: C1 = K1 G^T
+  {
+    xl::GemmArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.A = {ka.p, ka.ps, ka.ld, g.a1_rows, 1 << 30, 0};
+    a.B = {gp.p, (int64_t)nc * g.b_rowsp * g.b_ld, gp.ld, nc * g.b_rowsp, 1 << 30, 0};
+    a.M = g.a1_rows;
+    a.N = nc * g.b_rowsp;
+    a.K = (int)g.b_ld;
+    a.mode = 0;
+    a.c = c1;
+    xl::KTimer t(xl::K_GEMM_X, st);
+    hipLaunchKernelGGL(xl::cgemm_kernel, dim3((a.N + xl::GBN - 1) / xl::GBN, (a.M + xl::GBM - 1) / xl::GBM),
+                       dim3(1024), 0, st, a);
+  }
+  // stage 2: out = K2 B2^T, B2 row (c, j) = C1[j][c * rowsp ...]
+  {
+    xl::GemmArgs a;
+    std::memset(&a, 0, sizeof(a));
+    const int nb = g.a1_rows;   // stage-1 rows = output columns per component
+    a.A = {kb.p, kb.ps, kb.ld, g.a2_rows, 1 << 30, 0};
+    a.B = {c1.p, c1.ps, c1.ld, nc * nb, nb, (int64_t)g.b_rowsp};
+    a.M = g.a2_rows;
+    a.N = nc * nb;
+    a.K = g.b_rowsp;
+    a.mode = 1;
+    a.out = (float2*)p->out;
+    a.out_cs = p->out_comp_stride;
+    a.nb = nb;
+    a.z = p->z;
+    a.k = p->k;
+    if (!p->adjoint) {
+      a.qx = p->xout; a.qy = p->yout;
+      a.cte = make_float2((float)p->cte_re, (float)p->cte_im);
+      a.post_mode = p->mode == XL_CZT_RS ? xl::POST_RS : (p->mode == XL_CZT_HNA ? xl::POST_CONST : xl::POST_NONE);
+    } else {
+      a.qx = p->x; a.qy = p->y;
+      a.post_mode = p->mode == XL_CZT_RS ? xl::POST_CONJ_RS : xl::POST_NONE;
+    }
+    xl::KTimer t(xl::K_GEMM_Y, st);
+    hipLaunchKernelGGL(xl::cgemm_kernel, dim3((a.N + xl::GBN - 1) / xl::GBN, (a.M + xl::GBM - 1) / xl::GBM),
+                       dim3(1024), 0, st, a);
+  }
+  return hip_check(hipGetLastError(), "zoom-GEMM launch");
 }
 
 static int pre_common(const xl_pre_params* p, xl::PreArgs* a) {

@@ -251,13 +251,43 @@ def _czt_params(g: CztGeometry, dev, ncomp, nin, adjoint, ez_mode, mode=None):
     return p, keep
 
 
+CZT_FORMS = ("auto", "bluestein", "gemm")
+_czt_form = "auto"
+# Output-window size (per axis) below which "auto" picks the split-BF16 MFMA
+# zoom-DFT over the FP32 Bluestein FFT form (measured on MI355X; see bench.py).
+GEMM_CROSSOVER = 300
+
+
+def set_czt_form(form: str) -> None:
+    """Select the CZT implementation: 'bluestein' (FP32 FFT), 'gemm' (split-BF16
+    MFMA zoom-DFT) or 'auto' (gemm when both output axes are short)."""
+    global _czt_form
+    if form not in CZT_FORMS:
+        raise ValueError(f"CZT form must be one of {CZT_FORMS}")
+    _czt_form = form
+
+
+def get_czt_form() -> str:
+    return _czt_form
+
+
+def resolve_czt_form(g: "CztGeometry", form: Optional[str] = None) -> str:
+    form = form or _czt_form
+    if form == "auto":
+        small = max(len(g.xout), len(g.yout)) <= GEMM_CROSSOVER
+        return "gemm" if small else "bluestein"
+    return form
+
+
 def czt(src: torch.Tensor, g: CztGeometry, masks: Sequence[MaskSpec] = (), ez_mode: int = N.XL_EZ_NONE,
         nin: Optional[int] = None, ncomp: Optional[int] = None, adjoint: bool = False, mode: Optional[int] = None,
-        out: Optional[torch.Tensor] = None, workspace: Optional[torch.Tensor] = None) -> torch.Tensor:
+        out: Optional[torch.Tensor] = None, workspace: Optional[torch.Tensor] = None,
+        form: Optional[str] = None) -> torch.Tensor:
     """CZT_jit / VCZT_jit / CZT_for_high_NA_jit / VCZT_objective_lens core.
 
     Forward: src (C, ny, nx) -> (ncomp, my, mx).  Adjoint: src (ncomp, my, mx)
-    cotangent -> (ncomp, ny, nx) cotangent of the pre-op output.
+    cotangent -> (ncomp, ny, nx) cotangent of the pre-op output.  `form`:
+    'bluestein', 'gemm' or 'auto' (default: the module setting).
     """
     lib = N.load()
     src = _prep_field(src)
@@ -278,12 +308,15 @@ def czt(src: torch.Tensor, g: CztGeometry, masks: Sequence[MaskSpec] = (), ez_mo
     p.out = out.data_ptr()
     p.out_comp_stride = rows_out * cols_out
     keep += fill_masks(p.mask, masks, (rows_in, cols_in), dev, nin)
-    need = lib.xl_czt_workspace_bytes(C.byref(p))
+    use_gemm = resolve_czt_form(g, form) == "gemm"
+    wsfn = lib.xl_czt_gemm_workspace_bytes if use_gemm else lib.xl_czt_workspace_bytes
+    runfn = lib.xl_czt_gemm if use_gemm else lib.xl_czt
+    need = wsfn(C.byref(p))
     if need == 0:
-        N.check(lib.xl_czt(C.byref(p), None, 0, _stream(dev)))   # raises with the plan error
+        N.check(runfn(C.byref(p), None, 0, _stream(dev)))   # raises with the plan error
     if workspace is None or workspace.numel() < need:
         workspace = torch.empty(need, dtype=torch.uint8, device=dev)
-    N.check(lib.xl_czt(C.byref(p), workspace.data_ptr(), workspace.numel(), _stream(dev)))
+    N.check(runfn(C.byref(p), workspace.data_ptr(), workspace.numel(), _stream(dev)))
     del keep
     return out
 
