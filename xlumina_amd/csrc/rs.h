@@ -147,7 +147,8 @@ __global__ __launch_bounds__(P / 4) void rs_col_kernel(RsArgs a) {
   const int G = 4 / ng;                 // column slots per block
   const int slot = g / ng, role = g - slot * ng;
   const bool active = slot < G;
-  const int col = blockIdx.x * G + slot;
+  // the four columns of a tile run back to back on one XCD (shared 128 B lines)
+  const int col = xcd_remap(blockIdx.x, gridDim.x) * G + slot;
   const int nrg = (a.n + 3) >> 2;
   const int nr = nrg * 4;
   const int cg = col >> 2, cc = col & 3;
@@ -190,59 +191,52 @@ __global__ __launch_bounds__(P / 4) void rs_col_kernel(RsArgs a) {
   fft_inverse_from_regs<P, TPF, DST_FN>(s, t, u, store);
 }
 
-// One block per output row: component c is inverse-transformed by thread
-// group c from its LDS buffer (its row of the tiles staged there), the first N
-// samples are stored from registers, and |E|^2 is summed over the components
-// through LDS.  Rows 4g..4g+3 run back to back on one XCD so the 32 B row
-// slices of each 128 B tile are read from the same L2.
+// Per block 4 rows (one per thread group): the 4-row tiles of each component
+// are staged into LDS with whole-line loads, inverse-transformed in LDS, and
+// the first N samples stored; |E|^2 is accumulated over the components.
 template <int P>
-__global__ __launch_bounds__(3 * (P / 16)) void rs_final_kernel(RsArgs a) {
-  constexpr int TPF = P / 16, LS = LdsLen<P>::value;
-  using LP = LastPass<P, TPF, false>;
-  extern __shared__ float2 lds[];
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  const int f = tid / TPF, t = tid % TPF;
+__global__ __launch_bounds__(4 * FftThreads<P>::value) void rs_final_kernel(RsArgs a) {
+  constexpr int TPF = FftThreads<P>::value, LS = LdsLen<P>::value;
+  constexpr int NT = 4 * TPF;
+  constexpr int PPT = (P / 2 + TPF - 1) / TPF;
+  __shared__ float2 lds[4 * LS];
+  const int tid = threadIdx.x, f = tid / TPF, t = tid % TPF;
+  float2* s = lds + f * LS;
   const int nrg = (a.n + 3) >> 2;
-  const int row = xcd_remap(blockIdx.x, gridDim.x);
-  const int rg = row >> 2, rr = row & 3;
-  const int nc = a.ncomp;
-  for (int e = tid; e < nc * P; e += nthr) {
-    const int c = e / P, col = e - c * P;
-    lds[c * LS + lpad(col)] =
-        a.t[(int64_t)c * nrg * 4 * P + ((int64_t)(col >> 2) * nrg + rg) * 16 + rr * 4 + (col & 3)];
-  }
-  __syncthreads();
-  NoIO none;
-  typename LP::Type o;
-  fft_io<P, TPF, true, SRC_LDS, DST_REG>(lds + f * LS, t, none, none, o);
-  const bool rowok = row < a.n;
-  if (rowok) {
-    float2* oc = a.out + (int64_t)f * a.out_cs + (int64_t)row * a.n;
+  const int rg = blockIdx.x;
+  const int row = rg * 4 + f;
+  float inten[PPT];
 #pragma unroll
-    for (int b = 0; b < LP::BPT; ++b)
-#pragma unroll
-      for (int q = 0; q < LP::R; ++q) {
-        const int col = LP::index(t, b, q);
-        if (LP::valid(t, b) && col < a.n) oc[col] = o.v[b][q];
-      }
-  }
-  if (a.intensity) {
-    float* ib = reinterpret_cast<float*>(lds);   // |E_c|^2 of column col at ib[c * LS * 2 + col]
+  for (int i = 0; i < PPT; ++i) inten[i] = 0.f;
+  for (int c = 0; c < a.ncomp; ++c) {
+    const float2* tc = a.t + (int64_t)c * nrg * 4 * P;
+#pragma unroll 4
+    for (int e = tid; e < 4 * P; e += NT) {
+      const int cgp = e >> 4, rr = (e >> 2) & 3, cc = e & 3;
+      lds[rr * LS + lpad(cgp * 4 + cc)] = tc[((int64_t)cgp * nrg + rg) * 16 + rr * 4 + cc];
+    }
     __syncthreads();
+    fft_lds<P, TPF, true>(s, t);
+    if (row < a.n) {
+      float2* oc = a.out + (int64_t)c * a.out_cs + (int64_t)row * a.n;
 #pragma unroll
-    for (int b = 0; b < LP::BPT; ++b)
-#pragma unroll
-      for (int q = 0; q < LP::R; ++q) {
-        const int col = LP::index(t, b, q);
-        if (LP::valid(t, b) && col < a.n) ib[f * LS * 2 + col] = o.v[b][q].x * o.v[b][q].x + o.v[b][q].y * o.v[b][q].y;
+      for (int i = 0; i < PPT; ++i) {
+        const int col = t + i * TPF;
+        if (col < a.n) {
+          const float2 v = s[lpad(col)];
+          oc[col] = v;
+          inten[i] += v.x * v.x + v.y * v.y;
+        }
       }
+    }
     __syncthreads();
-    if (rowok)
-      for (int col = tid; col < a.n; col += nthr) {
-        float sum = 0.f;
-        for (int c = 0; c < nc; ++c) sum += ib[c * LS * 2 + col];
-        a.intensity[(int64_t)row * a.n + col] = sum;
-      }
+  }
+  if (a.intensity && row < a.n) {
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) {
+      const int col = t + i * TPF;
+      if (col < a.n) a.intensity[(int64_t)row * a.n + col] = inten[i];
+    }
   }
 }
 
@@ -267,17 +261,7 @@ inline hipError_t rs_launch(const RsArgs& a, hipStream_t st) {
     const int G = 4 / (1 + a.ncomp);
     hipLaunchKernelGGL(rs_col_kernel<P>, dim3((P + G - 1) / G), dim3(P / 4), 0, st, a);
   }
-  {
-    KTimer t(K_RS_FINAL, st);
-    static bool attr_set = false;
-    if (!attr_set) {
-      (void)hipFuncSetAttribute((const void*)rs_final_kernel<P>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                3 * LdsLen<P>::value * (int)sizeof(float2));
-      attr_set = true;
-    }
-    hipLaunchKernelGGL(rs_final_kernel<P>, dim3(nrg * 4), dim3(a.ncomp * (P / 16)),
-                       (size_t)a.ncomp * LdsLen<P>::value * sizeof(float2), st, a);
-  }
+  { KTimer t(K_RS_FINAL, st); hipLaunchKernelGGL(rs_final_kernel<P>, dim3(nrg), blk, 0, st, a); }
   return hipGetLastError();
 }
 
