@@ -53,6 +53,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--reference-copy", action="store_true",
+                    help="also run a 1 GiB device-to-device copy (scale reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the propagation path shards "
@@ -100,6 +102,13 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "algorithmic_bytes": nbytes, "avg_launch_us": round(per_launch_ms * 1e3, 2),
                 "kernels_ms_per_step": {k: round(v[1] / args.steps, 4) for k, v in kt.items()}}
+
+    if args.reference_copy:
+        a = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     secondary = {}
     if not args.no_secondary:

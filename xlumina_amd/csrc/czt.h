@@ -137,6 +137,32 @@ __device__ __forceinline__ float2 czt_pre_value(const CztArgs& a, int c, int row
   }
 }
 
+// All pre-op components of pixel (row, col) at once (loads, masks, Ez once).
+template <int K>
+__device__ __forceinline__ void czt_pre_values(const CztArgs& a, int row, int col, float2 fpix, float2 (&v)[3]) {
+  const int64_t pix = (int64_t)row * a.ax.lin + col;
+  float2 e0 = a.in[pix];
+  float2 e1 = a.nin > 1 ? a.in[a.in_cs + pix] : make_float2(0.f, 0.f);
+  float2 e2 = a.nin > 2 ? a.in[2 * a.in_cs + pix] : make_float2(0.f, 0.f);
+  apply_mask(a.mask, pix, e0, e1, e2);
+  if constexpr (K == PREK_HNA) {
+    float A[3][2];
+    hna_map(a.px[col], a.py[row], a.hna_radius, a.hna_f, a.hna_stm2, A);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] = c_add(c_scale(e0, A[c][0]), c_scale(e1, A[c][1]));
+  } else {
+    if (a.ez_mode != EZ_NONE) e2 = build_ez(a.ez_mode, e0, e1, a.px[col], a.py[row], a.z);
+    v[0] = e0;
+    v[1] = e1;
+    v[2] = e2;
+    float2 m = make_float2(1.f, 0.f);
+    if constexpr (K == PREK_F) m = fpix;
+    else if (a.pre_mode == PRE_CONST) m = a.cte;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] = c_mul(v[c], m);
+  }
+}
+
 // Transfer function of the input plane (or its conjugate times cte for the adjoint).
 __device__ __forceinline__ float2 czt_f_value(const CztArgs& a, int row, int col) {
   float2 h = rs_kernel_value(a.px[col], a.py[row], a.z, a.k);
@@ -161,14 +187,18 @@ __global__ __launch_bounds__(3 * Q / 16) void czt_rows_kernel(CztArgs a) {
   const int lin = a.ax.lin, lout = a.ax.lout, woff = a.ax.woff;
   const int nc = a.ncomp;
   for (int col = tid; col < Q; col += nthr) {
+    float2 v[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
     if (col < lin) {
       float2 fv = make_float2(1.f, 0.f);
       if constexpr (K == PREK_F) fv = czt_f_value(a, row, col);
+      czt_pre_values<K>(a, row, col, fv, v);
       const float2 pre = a.ax.pre[col];
-      for (int c = 0; c < nc; ++c) lds[c * LS + lpad(col)] = c_mul(czt_pre_value<K>(a, c, row, col, fv), pre);
-    } else {
-      for (int c = 0; c < nc; ++c) lds[c * LS + lpad(col)] = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[c] = c_mul(v[c], pre);
     }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      if (c < nc) lds[c * LS + lpad(col)] = v[c];
   }
   __syncthreads();
   float2* s = lds + f * LS;
@@ -187,22 +217,27 @@ __global__ __launch_bounds__(3 * Q / 16) void czt_rows_kernel(CztArgs a) {
   fft_inverse_from_regs<Q, TPF, DST_FN>(s, t, u, store);
 }
 
-// Per block 4 output columns of the x axis (one per thread group) of one
+// Per block CPB output columns of the x axis (one per thread group) of one
 // component: the column of the row-major intermediate is loaded straight into
 // the first pass, the chirp spectrum applied in registers, and only the
-// output window stored with the output-plane factor.  Four blocks share each
+// output window stored with the output-plane factor.  Neighbouring blocks share each
 // 128 B line of the intermediate, so consecutive blocks stay on one XCD.
+template <int Q> struct CztColsShape {
+  static constexpr int CPB = Q >= 4096 ? 2 : 4;   // columns per block (more blocks at 4096)
+};
+
 template <int Q>
-__global__ __launch_bounds__(4 * FftThreads<Q>::value) void czt_cols_kernel(CztArgs a) {
+__global__ __launch_bounds__(CztColsShape<Q>::CPB * FftThreads<Q>::value) void czt_cols_kernel(CztArgs a) {
   constexpr int TPF = FftThreads<Q>::value, LS = LdsLen<Q>::value;
+  constexpr int CPB = CztColsShape<Q>::CPB;
   using HO = HandOff<Q, TPF>;
-  __shared__ float2 lds[4 * LS];
+  __shared__ float2 lds[CPB * LS];
   const int tid = threadIdx.x, f = tid / TPF, t = tid % TPF;
   float2* s = lds + f * LS;
   const int og = xcd_remap(blockIdx.x, gridDim.x);
   const int c = blockIdx.y;
   const int lout_x = a.ax.lout;
-  const int kx = og * 4 + f;
+  const int kx = og * CPB + f;
   const bool colok = kx < lout_x;
   const int lin = a.ay.lin, lout = a.ay.lout, woff = a.ay.woff;
   const float2* src = a.t + (int64_t)c * a.rin * lout_x + kx;
@@ -253,8 +288,9 @@ inline void czt_rows_launch(const CztArgs& a, hipStream_t st) {
 template <int Q>
 inline void czt_cols_launch(const CztArgs& a, hipStream_t st) {
   KTimer t(K_CZT_COLS, st);
-  hipLaunchKernelGGL(czt_cols_kernel<Q>, dim3((a.ax.lout + 3) / 4, a.ncomp), dim3(4 * FftThreads<Q>::value), 0, st,
-                     a);
+  constexpr int CPB = CztColsShape<Q>::CPB;
+  hipLaunchKernelGGL(czt_cols_kernel<Q>, dim3((a.ax.lout + CPB - 1) / CPB, a.ncomp), dim3(CPB * FftThreads<Q>::value), 0,
+                     st, a);
 }
 template <int Q>
 inline void czt_setup_launch(const CztSetupPair& p, int nblocks, hipStream_t st) {
