@@ -6,8 +6,11 @@ per-pixel LCD retarder, VRS to z = 30 mm (Ex, Ey, Ez complex64), then VCZT
 zoom to a 512^2 ROI of 0.2 mm -- one full VRS + VCZT chain, inputs resident
 in HBM.  `value` = chains per second on the one GPU.
 
-Also reported (`secondary`): scalar RS 1024^2 (SLM fused, field + |E|^2)
-propagations/s and high-NA 2048^2 -> 256^2 forward+backward steps/s.
+Also reported (`secondary`): VCZT alone in both forms (FP32 Bluestein and
+split-BF16 MFMA), scalar RS 1024^2 (SLM fused, field + |E|^2)
+propagations/s, high-NA 2048^2 -> 256^2 forward+backward steps/s, and the
+serving loop on the mixed request stream (capacity, then Poisson arrivals at
+0.25-2x capacity with p50/p99 latency; summarised in `latency`).
 
 `roofline` covers the kernel with the largest share of the timed region,
 timed live with HIP events recorded on its launch stream; `cpu_baseline` is
@@ -53,6 +56,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--fourf-launches", type=int, default=2,
+                    help="timed 100-step launches of the batched 4f optimisation (0 skips it)")
+    ap.add_argument("--serve-requests", type=int, default=10000,
+                    help="requests per serving run (0 skips the serving sweep)")
     ap.add_argument("--reference-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (scale reference for HBM byte counters)")
     args = ap.parse_args()
@@ -110,12 +117,14 @@ def main():
         torch.cuda.synchronize()
         del a, b
 
+    _progress(f"headline: {value:.1f} chains/s ({ms:.4f} ms/step)")
     secondary = {}
     if not args.no_secondary:
         secondary = run_secondary(dev, workloads, args)
 
     cpu = None
     if not args.no_cpu_baseline:
+        _progress("CPU baseline (float64 oracle, one chain)")
         cpu = cpu_baseline(w)
 
     line = {
@@ -136,9 +145,26 @@ def main():
                    "form": "plain FP32 (LDS radix-16 FFT RS + Bluestein CZT)", "grid": n, "fft_len": P},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "latency": _latency(secondary.get("serving_mix")),
         "secondary": secondary,
     }
     print(json.dumps(line), flush=True)
+
+
+def _progress(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def _latency(sweep):
+    """p50/p99 of the mixed request stream at half and at full measured capacity."""
+    if not sweep:
+        return None
+    out = {"capacity_req_s": sweep["capacity"]["throughput_req_s"]}
+    for r in sweep["rates"]:
+        if r["fraction_of_capacity"] in (0.5, 1.0):
+            out[f"at_{r['fraction_of_capacity']}x"] = {"p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
+                                                      "throughput_req_s": r["throughput_req_s"]}
+    return out
 
 
 def _time(fn, steps, warmup):
@@ -173,6 +199,7 @@ def run_secondary(dev, workloads, args):
             ent["mfma_tflops"] = round(flops / (ms * 1e-3) / 1e12, 1)
             ent["mfma_frac_of_2500"] = round(flops / (ms * 1e-3) / 2.5e15, 4)
         out[f"vczt_2048_to_512_{form}"] = ent
+        _progress(f"vczt {form}: {ms:.4f} ms")
     rs = workloads.scalar_rs_1024(dev)
     ms = _time(lambda: rs.run(), max(10, args.steps), 3)
     out["rs_1024_slm"] = {"value": round(1000.0 / ms, 2), "unit": "propagations/s", "ms": round(ms, 4)}
@@ -192,6 +219,34 @@ def run_secondary(dev, workloads, args):
         ops.set_czt_form(prev)
         out[f"high_na_fwd_bwd_2048_to_256_{form}"] = {"value": round(1000.0 / ms, 2), "unit": "fwd+bwd steps/s",
                                                       "ms": round(ms, 4)}
+        _progress(f"high-NA fwd+bwd {form}: {ms:.4f} ms")
+    if args.fourf_launches > 0:
+        from xlumina_amd import fourf
+        opt = fourf.FourFOptimizer(dev, batch=64, n=1024, steps_per_launch=100)
+        first = opt.launch().clone()          # captures the 100-step graph, then runs it
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.fourf_launches):
+            last = opt.launch()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.fourf_launches
+        out["fourf_batched_opt_64x1024"] = {
+            "value": round(opt.candidate_steps_per_launch() / dt, 1), "unit": "candidate-steps/s",
+            "ms_per_launch": round(dt * 1e3, 2), "steps_per_launch": 100, "candidates": 64,
+            "mean_loss_step0": float(first[0].mean()), "mean_loss_last": float(last[-1].mean())}
+        del opt
+        torch.cuda.empty_cache()
+        _progress(f"4f batched optimisation: {dt * 1e3:.1f} ms per 100-step launch")
+    if args.serve_requests > 0:
+        from xlumina_amd import serving as S
+        srv = S.Server(dev)
+        sweep = S.rate_sweep(srv, count=args.serve_requests, fractions=(0.25, 0.5, 1.0, 2.0))
+        sweep["workload"] = (f"{args.serve_requests} requests, n in 512/1024/2048 (p .5/.3/.2), scalar/vectorial "
+                             "70/30, z ~ U[5, 100] mm, 20% gradient w.r.t. one phase mask, Poisson arrivals at "
+                             "fractions of the measured capacity")
+        sweep["hbm_pool_bytes"] = srv.used_bytes
+        out["serving_mix"] = sweep
+        _progress(f"serving: capacity {sweep['capacity']['throughput_req_s']} req/s")
     return out
 
 

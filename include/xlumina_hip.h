@@ -72,6 +72,8 @@ typedef struct xl_rs_params {
   double k;                /* 2 pi / lambda                              */
   double dx, dy;           /* x[1]-x[0], y[1]-y[0] (wave_optics.py:269-270) */
   xl_mask_stage mask[2];
+  const double* z_dev;     /* optional device copy of z read by the kernels instead of `z`
+                              (lets a captured HIP graph replay with a new distance) */
 } xl_rs_params;
 
 typedef struct xl_czt_params {
@@ -97,6 +99,13 @@ typedef struct xl_czt_params {
   double cte_re, cte_im;   /* constant output factor (forward)                 */
   double hna_radius, hna_f;
   xl_mask_stage mask[2];
+  /* Batch of independent problems sharing this geometry, run by the same
+   * launches (Bluestein form only).  batch <= 1: one problem. */
+  int32_t batch;
+  int32_t reserved2;
+  int64_t in_batch_stride;    /* complex elements between problems' inputs (0: one shared input) */
+  int64_t out_batch_stride;   /* complex elements between problems' outputs */
+  int64_t mask_batch_stride;  /* float elements between problems' per-pixel mask parameters (0: shared) */
 } xl_czt_params;
 
 typedef struct xl_pre_params {
@@ -119,6 +128,14 @@ typedef struct xl_pre_params {
   double z;
   double hna_radius, hna_f;
   xl_mask_stage mask[2];
+  const double* z_dev;     /* optional device copy of z (see xl_rs_params) */
+  int32_t batch;           /* independent problems per launch (<= 1: one) */
+  int32_t reserved;
+  int64_t in_batch_stride;     /* complex elements (0: one input shared by the batch) */
+  int64_t gout_batch_stride;   /* complex elements */
+  int64_t out_batch_stride;    /* complex elements */
+  int64_t mask_batch_stride;   /* float elements (0: shared mask parameters) */
+  int64_t gparam_batch_stride; /* float elements */
 } xl_pre_params;
 
 const char* xl_last_error(void);
@@ -151,6 +168,43 @@ int xl_czt_gemm(const xl_czt_params* p, void* workspace, size_t workspace_bytes,
  * (optical_elements.py:87-332), high_NA_objective_lens (:515-594). */
 int xl_pre_forward(const xl_pre_params* p, void* stream);
 int xl_pre_backward(const xl_pre_params* p, void* stream);
+
+/* MSE_Intensity (loss_functions.py:137-149) of a batch of detector fields
+ * against one target intensity |T|^2, and its cotangent with respect to the
+ * field: loss_b = scale * sum_p (|E_bp|^2 - T_p)^2 / npix,
+ * grad_bp = scale * 4 (|E_bp|^2 - T_p) E_bp / npix (dL/dRe + i dL/dIm).
+ * grad may alias field.  Deterministic two-stage reduction. */
+typedef struct xl_mse_params {
+  const void* field;           /* batch x npix complex64 */
+  int64_t field_batch_stride;
+  const float* target;         /* npix float32 target intensity */
+  void* grad;                  /* batch x npix complex64, or NULL */
+  int64_t grad_batch_stride;
+  float* loss;                 /* batch float32 losses, or NULL */
+  int64_t npix;
+  int32_t batch;
+  float scale;
+} xl_mse_params;
+size_t xl_mse_workspace_bytes(int64_t npix, int32_t batch);
+int xl_mse_intensity(const xl_mse_params* p, void* workspace, size_t workspace_bytes, void* stream);
+
+/* One Adam / AdamW update (optax.adam; optax.adamw as used by
+ * experiments/four_f_optimizer.py:112) of n float32 parameters in place:
+ * m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2, t = count + 1,
+ * p -= lr ((m / (1-b1^t)) / (sqrt(v / (1-b2^t)) + eps) + weight_decay p);
+ * then count += 1.  count lives in device memory so that a captured HIP
+ * graph of several steps can be replayed. */
+typedef struct xl_adam_params {
+  float* param;
+  const float* grad;
+  float* m;
+  float* v;
+  int64_t n;
+  int32_t* count;
+  float lr, b1, b2, eps, weight_decay;
+  int32_t reserved;
+} xl_adam_params;
+int xl_adam_step(const xl_adam_params* p, void* stream);
 
 /* transfer_function_RS (wave_optics.py:291-297) on meshgrid(x, y): out is
  * ny x nx complex64. */

@@ -46,6 +46,10 @@ int main(void) {
   printf("%zu %zu %zu %zu\n", sizeof(xl_mask_stage), sizeof(xl_rs_params), sizeof(xl_czt_params), sizeof(xl_pre_params));
   printf("%zu %zu %zu\n", offsetof(xl_rs_params, mask), offsetof(xl_czt_params, mask), offsetof(xl_pre_params, mask));
   printf("%zu %zu %zu\n", offsetof(xl_rs_params, dx), offsetof(xl_czt_params, hna_f), offsetof(xl_pre_params, gparam));
+  printf("%zu %zu %zu %zu %zu\n", offsetof(xl_rs_params, z_dev), offsetof(xl_czt_params, mask_batch_stride),
+         offsetof(xl_pre_params, z_dev), offsetof(xl_pre_params, gparam_batch_stride), sizeof(xl_mse_params));
+  printf("%zu %zu %zu %zu\n", offsetof(xl_mse_params, scale), sizeof(xl_adam_params), offsetof(xl_adam_params, count),
+         offsetof(xl_adam_params, weight_decay));
   return 0;
 }
 '''
@@ -61,10 +65,15 @@ int main(void) {
     assert sizes == [C.sizeof(N.MaskStage), C.sizeof(N.RsParams), C.sizeof(N.CztParams), C.sizeof(N.PreParams)]
     assert offs == [N.RsParams.mask.offset, N.CztParams.mask.offset, N.PreParams.mask.offset]
     assert offs2 == [N.RsParams.dx.offset, N.CztParams.hna_f.offset, N.PreParams.gparam.offset]
+    assert [int(v) for v in lines[3].split()] == [N.RsParams.z_dev.offset, N.CztParams.mask_batch_stride.offset,
+                                                  N.PreParams.z_dev.offset, N.PreParams.gparam_batch_stride.offset,
+                                                  C.sizeof(N.MseParams)]
+    assert [int(v) for v in lines[4].split()] == [N.MseParams.scale.offset, C.sizeof(N.AdamParams),
+                                                  N.AdamParams.count.offset, N.AdamParams.weight_decay.offset]
 
 
 def test_host_side_queries(lib):
-    assert lib.xl_abi_version() == 1
+    assert lib.xl_abi_version() == 2
     assert lib.xl_max_fft_length() == 4096
     # RS workspace: ncomp * round4(n) * P + (P/2) * P complex64 (+ alignment slack)
     n, P = 1024, 2048

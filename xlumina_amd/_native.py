@@ -35,7 +35,7 @@ class RsParams(C.Structure):
                 ("out", C.c_void_p), ("out_comp_stride", C.c_int64), ("intensity", C.c_void_p),
                 ("n", C.c_int32), ("adjoint", C.c_int32), ("x", C.c_void_p), ("y", C.c_void_p),
                 ("z", C.c_double), ("k", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
-                ("mask", MaskStage * 2)]
+                ("mask", MaskStage * 2), ("z_dev", C.c_void_p)]
 
 
 class CztParams(C.Structure):
@@ -47,7 +47,8 @@ class CztParams(C.Structure):
                 ("z", C.c_double), ("wavelength", C.c_double), ("k", C.c_double), ("dm", C.c_double),
                 ("fy1", C.c_double), ("fy2", C.c_double), ("fx1", C.c_double), ("fx2", C.c_double),
                 ("cte_re", C.c_double), ("cte_im", C.c_double), ("hna_radius", C.c_double), ("hna_f", C.c_double),
-                ("mask", MaskStage * 2)]
+                ("mask", MaskStage * 2), ("batch", C.c_int32), ("reserved2", C.c_int32),
+                ("in_batch_stride", C.c_int64), ("out_batch_stride", C.c_int64), ("mask_batch_stride", C.c_int64)]
 
 
 class PreParams(C.Structure):
@@ -55,7 +56,22 @@ class PreParams(C.Structure):
                 ("gout", C.c_void_p), ("out", C.c_void_p), ("comp_stride", C.c_int64),
                 ("gparam", (C.c_void_p * 2) * 2), ("rows", C.c_int32), ("cols", C.c_int32),
                 ("ez_mode", C.c_int32), ("hna", C.c_int32), ("x", C.c_void_p), ("y", C.c_void_p),
-                ("z", C.c_double), ("hna_radius", C.c_double), ("hna_f", C.c_double), ("mask", MaskStage * 2)]
+                ("z", C.c_double), ("hna_radius", C.c_double), ("hna_f", C.c_double), ("mask", MaskStage * 2),
+                ("z_dev", C.c_void_p), ("batch", C.c_int32), ("reserved", C.c_int32),
+                ("in_batch_stride", C.c_int64), ("gout_batch_stride", C.c_int64), ("out_batch_stride", C.c_int64),
+                ("mask_batch_stride", C.c_int64), ("gparam_batch_stride", C.c_int64)]
+
+
+class MseParams(C.Structure):
+    _fields_ = [("field", C.c_void_p), ("field_batch_stride", C.c_int64), ("target", C.c_void_p),
+                ("grad", C.c_void_p), ("grad_batch_stride", C.c_int64), ("loss", C.c_void_p),
+                ("npix", C.c_int64), ("batch", C.c_int32), ("scale", C.c_float)]
+
+
+class AdamParams(C.Structure):
+    _fields_ = [("param", C.c_void_p), ("grad", C.c_void_p), ("m", C.c_void_p), ("v", C.c_void_p),
+                ("n", C.c_int64), ("count", C.c_void_p), ("lr", C.c_float), ("b1", C.c_float), ("b2", C.c_float),
+                ("eps", C.c_float), ("weight_decay", C.c_float), ("reserved", C.c_int32)]
 
 
 class KernelTime(C.Structure):
@@ -78,6 +94,9 @@ SIGNATURES = {
     "xl_pre_backward": (C.c_int, [C.POINTER(PreParams), C.c_void_p]),
     "xl_transfer_function_rs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
                                           C.c_double, C.c_double, C.c_void_p]),
+    "xl_mse_workspace_bytes": (C.c_size_t, [C.c_int64, C.c_int32]),
+    "xl_mse_intensity": (C.c_int, [C.POINTER(MseParams), C.c_void_p, C.c_size_t, C.c_void_p]),
+    "xl_adam_step": (C.c_int, [C.POINTER(AdamParams), C.c_void_p]),
     "xl_timing_enable": (C.c_int, [C.c_int32, C.c_int32]),
     "xl_timing_collect": (C.c_int, [C.POINTER(KernelTime), C.c_int32]),
 }

@@ -54,7 +54,23 @@ struct CztArgs {
   double hna_radius, hna_f, hna_stm2;
   MaskArgs mask;
   CztAxis ax, ay;
+  int64_t in_bs, out_bs, t_bs, mask_bs;   // batch strides (blockIdx.y / .z selects the member)
 };
+
+// The arguments of batch member b: inputs, outputs, intermediate and
+// per-pixel mask parameters offset by their batch strides.
+__device__ __forceinline__ CztArgs czt_member(const CztArgs& a0, int b) {
+  CztArgs a = a0;
+  a.in += b * a0.in_bs;
+  a.out += b * a0.out_bs;
+  a.t += b * a0.t_bs;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    if (a.mask.st[i].m0) a.mask.st[i].m0 += b * a0.mask_bs;
+    if (a.mask.st[i].m1) a.mask.st[i].m1 += b * a0.mask_bs;
+  }
+  return a;
+}
 
 struct CztSetupArgs {
   float2* pre;
@@ -177,7 +193,8 @@ __device__ __forceinline__ float2 czt_f_value(const CztArgs& a, int row, int col
 // between the forward and inverse chains and only the output window is
 // stored: row-major intermediate [c][row][k].
 template <int Q, int K>
-__global__ __launch_bounds__(3 * Q / 16) void czt_rows_kernel(CztArgs a) {
+__global__ __launch_bounds__(3 * Q / 16) void czt_rows_kernel(CztArgs a0) {
+  const CztArgs a = czt_member(a0, blockIdx.y);
   constexpr int TPF = Q / 16, LS = LdsLen<Q>::value;
   using HO = HandOff<Q, TPF>;
   extern __shared__ float2 lds[];
@@ -227,7 +244,8 @@ template <int Q> struct CztColsShape {
 };
 
 template <int Q>
-__global__ __launch_bounds__(CztColsShape<Q>::CPB * FftThreads<Q>::value) void czt_cols_kernel(CztArgs a) {
+__global__ __launch_bounds__(CztColsShape<Q>::CPB * FftThreads<Q>::value) void czt_cols_kernel(CztArgs a0) {
+  const CztArgs a = czt_member(a0, blockIdx.z);
   constexpr int TPF = FftThreads<Q>::value, LS = LdsLen<Q>::value;
   constexpr int CPB = CztColsShape<Q>::CPB;
   using HO = HandOff<Q, TPF>;
@@ -265,9 +283,9 @@ __global__ __launch_bounds__(CztColsShape<Q>::CPB * FftThreads<Q>::value) void c
 }
 
 template <int Q>
-inline void czt_rows_launch(const CztArgs& a, hipStream_t st) {
+inline void czt_rows_launch(const CztArgs& a, int batch, hipStream_t st) {
   KTimer t(K_CZT_ROWS, st);
-  const dim3 g(a.rin), b(a.ncomp * Q / 16);
+  const dim3 g(a.rin, batch), b(a.ncomp * Q / 16);
   const size_t lds = (size_t)a.ncomp * LdsLen<Q>::value * sizeof(float2);
   static bool attr_set = false;   // allow up to three 4096-point buffers of dynamic LDS
   if (!attr_set) {
@@ -286,11 +304,11 @@ inline void czt_rows_launch(const CztArgs& a, hipStream_t st) {
     hipLaunchKernelGGL((czt_rows_kernel<Q, PREK_SIMPLE>), g, b, lds, st, a);
 }
 template <int Q>
-inline void czt_cols_launch(const CztArgs& a, hipStream_t st) {
+inline void czt_cols_launch(const CztArgs& a, int batch, hipStream_t st) {
   KTimer t(K_CZT_COLS, st);
   constexpr int CPB = CztColsShape<Q>::CPB;
-  hipLaunchKernelGGL(czt_cols_kernel<Q>, dim3((a.ax.lout + CPB - 1) / CPB, a.ncomp), dim3(CPB * FftThreads<Q>::value), 0,
-                     st, a);
+  hipLaunchKernelGGL(czt_cols_kernel<Q>, dim3((a.ax.lout + CPB - 1) / CPB, a.ncomp, batch),
+                     dim3(CPB * FftThreads<Q>::value), 0, st, a);
 }
 template <int Q>
 inline void czt_setup_launch(const CztSetupPair& p, int nblocks, hipStream_t st) {

@@ -25,8 +25,27 @@ struct PreArgs {
   const double* x;
   const double* y;
   double z, hna_radius, hna_f, hna_stm2;
+  const double* z_dev;   // if set, z is read from device memory
   MaskArgs mask;
+  int64_t in_bs, gout_bs, out_bs, mask_bs, gparam_bs;   // batch strides (blockIdx.y = member)
 };
+
+// The arguments of batch member b (pointers offset by the batch strides).
+__device__ __forceinline__ PreArgs pre_member(const PreArgs& a0, int b) {
+  PreArgs a = a0;
+  a.in += b * a0.in_bs;
+  if (a.gout) a.gout += b * a0.gout_bs;
+  if (a.out) a.out += b * a0.out_bs;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    if (a.mask.st[i].m0) a.mask.st[i].m0 += b * a0.mask_bs;
+    if (a.mask.st[i].m1) a.mask.st[i].m1 += b * a0.mask_bs;
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+      if (a.gparam[i][j]) a.gparam[i][j] += b * a0.gparam_bs;
+  }
+  return a;
+}
 
 // Richards-Wolf lens factors (optical_elements.py:528-590): the 3x2 real map
 // (Ex', Ey') -> lens output, with Ez = (Ex' X + Ey' Y)/rho folded in.
@@ -49,7 +68,8 @@ __device__ __forceinline__ void hna_map(double X, double Y, double radius, doubl
 
 __device__ __forceinline__ float re_conj_mul(float2 g, float2 v) { return g.x * v.x + g.y * v.y; }  // Re(conj(g) v)
 
-__global__ void pre_forward_kernel(PreArgs a) {
+__global__ void pre_forward_kernel(PreArgs a0) {
+  const PreArgs a = pre_member(a0, blockIdx.y);
   const int64_t npix = (int64_t)a.rows * a.cols;
   for (int64_t pix = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; pix < npix;
        pix += (int64_t)gridDim.x * blockDim.x) {
@@ -64,7 +84,7 @@ __global__ void pre_forward_kernel(PreArgs a) {
       for (int i = 0; i < 3; ++i) a.out[i * a.cs + pix] = c_add(c_scale(e0, A[i][0]), c_scale(e1, A[i][1]));
       continue;
     }
-    if (a.ez_mode != EZ_NONE) e2 = build_ez(a.ez_mode, e0, e1, a.x[col], a.y[row], a.z);
+    if (a.ez_mode != EZ_NONE) e2 = build_ez(a.ez_mode, e0, e1, a.x[col], a.y[row], a.z_dev ? *a.z_dev : a.z);
     a.out[pix] = e0;
     if (a.ncomp > 1) a.out[a.cs + pix] = e1;
     if (a.ncomp > 2) a.out[2 * a.cs + pix] = e2;
@@ -143,7 +163,8 @@ __device__ __forceinline__ void stage_backward(const MaskStage& m, int64_t pix, 
   }
 }
 
-__global__ void pre_backward_kernel(PreArgs a) {
+__global__ void pre_backward_kernel(PreArgs a0) {
+  const PreArgs a = pre_member(a0, blockIdx.y);
   const int64_t npix = (int64_t)a.rows * a.cols;
   for (int64_t pix = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; pix < npix;
        pix += (int64_t)gridDim.x * blockDim.x) {
@@ -165,7 +186,7 @@ __global__ void pre_backward_kernel(PreArgs a) {
       const float2 h1 = c_add(c_add(c_scale(g0, A[0][1]), c_scale(g1, A[1][1])), c_scale(g2, A[2][1]));
       g0 = h0; g1 = h1; g2 = make_float2(0.f, 0.f);
     } else if (a.ez_mode != EZ_NONE) {
-      const double X = a.x[col], Y = a.y[row], z = a.z;
+      const double X = a.x[col], Y = a.y[row], z = a.z_dev ? *a.z_dev : a.z;
       float cx, cy;
       if (a.ez_mode == EZ_HNA) {
         const double rho = sqrt(X * X + Y * Y);

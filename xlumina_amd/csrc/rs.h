@@ -39,6 +39,7 @@ struct RsArgs {
   const double* x;
   const double* y;
   double z, k;
+  const double* z_dev;    // if set, z is read from device memory
   float scale;            // dx*dy/P^2
   int n, nin, ncomp, ez_mode, adjoint;
   MaskArgs mask;
@@ -75,12 +76,13 @@ __global__ __launch_bounds__(4 * FftThreads<P>::value) void rs_hrow_kernel(RsArg
   const int tid = threadIdx.x, f = tid / TPF, t = tid % TPF;
   float2* s = lds + f * LS;
   const int r = blockIdx.x * 4 + f;      // rows 0..P/2-1
+  const double z = a.z_dev ? *a.z_dev : a.z;
   double dy;
   const bool rowok = lag_of(a.y, r, a.n, P, &dy);
   for (int c = t; c <= P / 2; c += TPF) {
     double dx;
     float2 v = make_float2(0.f, 0.f);
-    if (rowok && lag_of(a.x, c, a.n, P, &dx)) v = rs_kernel_value(dx, dy, a.z, a.k);
+    if (rowok && lag_of(a.x, c, a.n, P, &dx)) v = rs_kernel_value(dx, dy, z, a.k);
     s[lpad(c)] = v;
     if (c > 0 && c < P / 2) s[lpad(P - c)] = v;
   }
@@ -104,6 +106,7 @@ __global__ __launch_bounds__(3 * (P / 16)) void rs_urow_kernel(RsArgs a) {
   const int nrg = (a.n + 3) >> 2;
   const int row = xcd_remap(blockIdx.x, gridDim.x);
   const int nc = a.ncomp;
+  const double z = a.z_dev ? *a.z_dev : a.z;
   for (int col = tid; col < P / 2; col += nthr) {
     float2 e0 = make_float2(0.f, 0.f), e1 = e0, e2 = e0;
     if (row < a.n && col < a.n) {
@@ -112,7 +115,7 @@ __global__ __launch_bounds__(3 * (P / 16)) void rs_urow_kernel(RsArgs a) {
       if (a.nin > 1) e1 = a.in[a.in_cs + pix];
       if (a.nin > 2) e2 = a.in[2 * a.in_cs + pix];
       apply_mask(a.mask, pix, e0, e1, e2);
-      if (a.ez_mode != EZ_NONE) e2 = build_ez(a.ez_mode, e0, e1, a.x[col], a.y[row], a.z);
+      if (a.ez_mode != EZ_NONE) e2 = build_ez(a.ez_mode, e0, e1, a.x[col], a.y[row], z);
     }
     lds[lpad(col)] = e0;
     lds[lpad(col + P / 2)] = make_float2(0.f, 0.f);

@@ -15,6 +15,7 @@
 
 #include "../../include/xlumina_hip.h"
 #include "czt.h"
+#include "opt.h"
 #include "gemm.h"
 #include "fft.h"
 #include "masks.h"
@@ -139,7 +140,7 @@ int plan_axis(int m, int M, double f1, double f2, double Dm, AxisPlan* ap) {
 
 struct CztPlan {
   AxisPlan x, y;       // forward axes (x: nx_in -> mx, y: ny_in -> my)
-  int ncomp;
+  int ncomp, batch;
   int rin, lin_x, lout_x, lout_y;
   size_t tables_bytes, tiles_bytes;
 };
@@ -158,8 +159,10 @@ int plan_czt(const xl_czt_params* p, CztPlan* cp) {
   } else {
     cp->rin = p->my; cp->lin_x = p->mx; cp->lout_x = p->nx_in; cp->lout_y = p->ny_in;
   }
+  if (p->batch < 0 || p->batch > 65535) return fail(XL_EINVAL, "batch must be 0..65535");
+  cp->batch = std::max(1, (int)p->batch);
   cp->tables_bytes = (size_t)2 * 3 * xl::MAX_FFT * sizeof(float2);
-  cp->tiles_bytes = (size_t)cp->ncomp * cp->rin * cp->lout_x * sizeof(float2);
+  cp->tiles_bytes = (size_t)cp->batch * cp->ncomp * cp->rin * cp->lout_x * sizeof(float2);
   return XL_OK;
 }
 
@@ -184,7 +187,7 @@ char* align256(char* p) { return (char*)(((uintptr_t)p + 255) & ~(uintptr_t)255)
 extern "C" {
 
 const char* xl_last_error(void) { return g_err.c_str(); }
-int xl_abi_version(void) { return 1; }
+int xl_abi_version(void) { return 2; }
 int xl_max_fft_length(void) { return xl::MAX_FFT; }
 int xl_init(void) { return ensure_init(); }
 
@@ -224,6 +227,7 @@ int xl_rs_propagate(const xl_rs_params* p, void* ws, size_t ws_bytes, void* stre
   a.y = p->y;
   a.z = p->z;
   a.k = p->k;
+  a.z_dev = p->z_dev;
   a.n = n;
   a.nin = p->nin;
   a.ncomp = ncomp;
@@ -336,12 +340,17 @@ int xl_czt(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream) {
     else if (p->mode == XL_CZT_HNA) { a.pre_mode = xl::PRE_CONST; a.post_mode = xl::POST_NONE; }
     else { a.pre_mode = xl::PRE_NONE; a.post_mode = xl::POST_NONE; }
   }
-  if (int rc = dispatch_q(a.ax.q, [&](auto c) { xl::czt_rows_launch<decltype(c)::value>(a, st); })) return rc;
-  if (int rc = dispatch_q(a.ay.q, [&](auto c) { xl::czt_cols_launch<decltype(c)::value>(a, st); })) return rc;
+  const int batch = std::max(1, (int)p->batch);
+  a.in_bs = batch > 1 ? p->in_batch_stride : 0;
+  a.out_bs = batch > 1 ? p->out_batch_stride : 0;
+  a.mask_bs = batch > 1 ? p->mask_batch_stride : 0;
+  a.t_bs = (int64_t)cp.ncomp * cp.rin * cp.lout_x;
+  if (int rc = dispatch_q(a.ax.q, [&](auto c) { xl::czt_rows_launch<decltype(c)::value>(a, batch, st); })) return rc;
+  if (int rc = dispatch_q(a.ay.q, [&](auto c) { xl::czt_cols_launch<decltype(c)::value>(a, batch, st); })) return rc;
   return hip_check(hipGetLastError(), "CZT launch");
 }
 
-static dim3 grid_for(int64_t npix);
+static dim3 grid_for(int64_t npix, int batch = 1);
 
 // ---------------------------------------------------------------- zoom-GEMM CZT
 namespace {
@@ -403,6 +412,7 @@ int xl_czt_gemm(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream)
   if (!device_ready()) return fail(XL_EINVAL, "xl_init() has not run on this device");
   GemmPlan g;
   if (int rc = plan_gemm(p, &g)) return rc;
+  if (p->batch > 1) return fail(XL_EINVAL, "batched CZT runs in the Bluestein form (xl_czt)");
   if (!p->in || !p->out || !p->x || !p->y || !p->xout || !p->yout) return fail(XL_EINVAL, "null pointer argument");
   if (p->nin < 1 || p->nin > 3 || p->ncomp < 1 || p->ncomp > 3) return fail(XL_EINVAL, "nin / ncomp must be 1..3");
   if (p->mode < XL_CZT_RS || p->mode > XL_CZT_PLAIN) return fail(XL_EINVAL, "unknown CZT mode");
@@ -553,17 +563,26 @@ static int pre_common(const xl_pre_params* p, xl::PreArgs* a) {
   a->x = p->x;
   a->y = p->y;
   a->z = p->z;
+  a->z_dev = p->z_dev;
   a->hna_radius = p->hna_radius;
   a->hna_f = p->hna_f;
   a->hna_stm2 = p->hna_radius * p->hna_radius / (p->hna_radius * p->hna_radius + p->hna_f * p->hna_f);
   a->mask = to_mask(p->mask);
+  if (p->batch < 0 || p->batch > 65535) return fail(XL_EINVAL, "batch must be 0..65535");
+  if (p->batch > 1) {
+    a->in_bs = p->in_batch_stride;
+    a->gout_bs = p->gout_batch_stride;
+    a->out_bs = p->out_batch_stride;
+    a->mask_bs = p->mask_batch_stride;
+    a->gparam_bs = p->gparam_batch_stride;
+  }
   return XL_OK;
 }
 
-static dim3 grid_for(int64_t npix) {
+static dim3 grid_for(int64_t npix, int batch) {
   int64_t b = (npix + 255) / 256;
   if (b > 65536) b = 65536;
-  return dim3((unsigned)std::max<int64_t>(b, 1));
+  return dim3((unsigned)std::max<int64_t>(b, 1), (unsigned)std::max(batch, 1));
 }
 
 int xl_pre_forward(const xl_pre_params* p, void* stream) {
@@ -571,7 +590,7 @@ int xl_pre_forward(const xl_pre_params* p, void* stream) {
   if (int rc = pre_common(p, &a)) return rc;
   if (!p->out) return fail(XL_EINVAL, "null output");
   xl::KTimer t(xl::K_PRE_FWD, (hipStream_t)stream);
-  hipLaunchKernelGGL(xl::pre_forward_kernel, grid_for((int64_t)p->rows * p->cols), dim3(256), 0,
+  hipLaunchKernelGGL(xl::pre_forward_kernel, grid_for((int64_t)p->rows * p->cols, p->batch), dim3(256), 0,
                      (hipStream_t)stream, a);
   return hip_check(hipGetLastError(), "pre_forward launch");
 }
@@ -581,9 +600,67 @@ int xl_pre_backward(const xl_pre_params* p, void* stream) {
   if (int rc = pre_common(p, &a)) return rc;
   if (!p->gout) return fail(XL_EINVAL, "null cotangent");
   xl::KTimer t(xl::K_PRE_BWD, (hipStream_t)stream);
-  hipLaunchKernelGGL(xl::pre_backward_kernel, grid_for((int64_t)p->rows * p->cols), dim3(256), 0,
+  hipLaunchKernelGGL(xl::pre_backward_kernel, grid_for((int64_t)p->rows * p->cols, p->batch), dim3(256), 0,
                      (hipStream_t)stream, a);
   return hip_check(hipGetLastError(), "pre_backward launch");
+}
+
+size_t xl_mse_workspace_bytes(int64_t npix, int32_t batch) {
+  if (npix < 1 || batch < 1) return 0;
+  return (size_t)batch * xl::kMseBlocks * sizeof(float) + 256;
+}
+
+int xl_mse_intensity(const xl_mse_params* p, void* ws, size_t ws_bytes, void* stream) {
+  if (!p) return fail(XL_EINVAL, "null params");
+  if (!p->field || !p->target || p->npix < 1 || p->batch < 1 || p->batch > 65535)
+    return fail(XL_EINVAL, "bad MSE arguments");
+  if (!p->grad && !p->loss) return fail(XL_EINVAL, "MSE call asks for neither loss nor gradient");
+  const size_t need = xl_mse_workspace_bytes(p->npix, p->batch);
+  if (!ws || ws_bytes < need) return fail(XL_EWORKSPACE, "MSE workspace too small: need " + std::to_string(need));
+  xl::MseArgs a;
+  a.field = (const float2*)p->field;
+  a.fbs = p->field_batch_stride;
+  a.target = p->target;
+  a.grad = (float2*)p->grad;
+  a.gbs = p->grad_batch_stride;
+  a.partial = (float*)align256((char*)ws);
+  a.loss = p->loss;
+  a.npix = p->npix;
+  a.gscale = (float)(4.0 * p->scale / (double)p->npix);
+  a.lscale = (float)(p->scale / (double)p->npix);
+  hipStream_t st = (hipStream_t)stream;
+  xl::KTimer t(xl::K_MSE, st);
+  hipLaunchKernelGGL(xl::mse_kernel, dim3(xl::kMseBlocks, p->batch), dim3(xl::kMseThreads), 0, st, a);
+  if (p->loss)
+    hipLaunchKernelGGL(xl::mse_reduce_kernel, dim3(p->batch), dim3(xl::kMseThreads), 0, st, a, xl::kMseBlocks);
+  return hip_check(hipGetLastError(), "MSE launch");
+}
+
+int xl_adam_step(const xl_adam_params* p, void* stream) {
+  if (!p) return fail(XL_EINVAL, "null params");
+  if (!p->param || !p->grad || !p->m || !p->v || !p->count || p->n < 1) return fail(XL_EINVAL, "bad Adam arguments");
+  if (!(p->b1 >= 0.f && p->b1 < 1.f && p->b2 >= 0.f && p->b2 < 1.f)) return fail(XL_EINVAL, "Adam betas must be in [0, 1)");
+  xl::AdamArgs a;
+  a.p = p->param;
+  a.g = p->grad;
+  a.m = p->m;
+  a.v = p->v;
+  a.n = p->n;
+  a.count = p->count;
+  a.lr = p->lr;
+  a.b1 = p->b1;
+  a.b2 = p->b2;
+  a.eps = p->eps;
+  a.wd = p->weight_decay;
+  auto al16 = [](const void* q) { return ((uintptr_t)q & 15) == 0; };
+  a.vec = al16(a.p) && al16(a.g) && al16(a.m) && al16(a.v);
+  hipStream_t st = (hipStream_t)stream;
+  xl::KTimer t(xl::K_ADAM, st);
+  int64_t blocks = (p->n / 4 + 255) / 256;
+  blocks = std::min<int64_t>(std::max<int64_t>(blocks, 1), 8192);
+  hipLaunchKernelGGL(xl::adam_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(xl::adam_tick_kernel, dim3(1), dim3(64), 0, st, p->count);
+  return hip_check(hipGetLastError(), "Adam launch");
 }
 
 int xl_transfer_function_rs(void* out, const double* x, const double* y, int32_t nx, int32_t ny, double z, double k,

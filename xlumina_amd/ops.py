@@ -81,10 +81,16 @@ def _prep_param(p, shape, device) -> Optional[torch.Tensor]:
     return p.contiguous()
 
 
-def fill_masks(dst, masks: Sequence[MaskSpec], shape, device, nin: int):
-    """Fills the two xl_mask_stage slots; returns the tensors kept alive."""
+def fill_masks(dst, masks: Sequence[MaskSpec], shape, device, nin: int, batch: Optional[int] = None):
+    """Fills the two xl_mask_stage slots; returns the tensors kept alive.
+
+    With `batch`, a parameter may also be a (batch, rows, cols) float32 view
+    whose rows are contiguous; every such parameter must share one batch
+    stride, returned as the second value ((keep, mask_batch_stride))."""
     if len(masks) > 2:
         raise ValueError("at most two fused mask stages")
+    if batch is not None:
+        return _fill_masks_batched(dst, masks, shape, device, batch)
     keep = []
     for i in range(2):
         st = dst[i]
@@ -105,6 +111,38 @@ def fill_masks(dst, masks: Sequence[MaskSpec], shape, device, nin: int):
     return keep
 
 
+def _fill_masks_batched(dst, masks, shape, device, batch):
+    keep, strides = [], set()
+    simple = []
+    for m in masks:
+        ps = []
+        for p in (m.p0, m.p1):
+            if isinstance(p, torch.Tensor) and p.dim() == 3:
+                if (tuple(p.shape) != (batch,) + tuple(shape) or p.dtype != torch.float32 or p.device != device
+                        or p.stride(2) != 1 or p.stride(1) != shape[1]):
+                    raise ValueError(f"batched mask parameter must be float32 {(batch,) + tuple(shape)} with "
+                                     "contiguous rows on the field's device")
+                strides.add(p.stride(0))
+                keep.append(p)
+                ps.append(("batched", p))
+            else:
+                ps.append(("shared", p))
+        simple.append(ps)
+    if len(strides) > 1:
+        raise ValueError("batched mask parameters must share one batch stride")
+    if strides and any(kind == "shared" and p is not None for ps in simple for kind, p in ps):
+        raise ValueError("mix of batched and shared per-pixel mask parameters")
+    flat = [MaskSpec(m.kind, None if isinstance(m.p0, torch.Tensor) and m.p0.dim() == 3 else m.p0,
+                     None if isinstance(m.p1, torch.Tensor) and m.p1.dim() == 3 else m.p1, m.s0, m.s1)
+            for m in masks]
+    keep += fill_masks(dst, flat, shape, device, 1)
+    for i, ps in enumerate(simple):
+        for j, (kind, p) in enumerate(ps):
+            if kind == "batched":
+                setattr(dst[i], "m0" if j == 0 else "m1", p.data_ptr())
+    return keep, (strides.pop() if strides else 0)
+
+
 def quality_factor(x, y, wavelength, z) -> float:
     """wave_optics.py:187-191 (host float64, as the reference)."""
     x = np.asarray(x, dtype=np.float64)
@@ -115,6 +153,14 @@ def quality_factor(x, y, wavelength, z) -> float:
     rmax = math.sqrt(float((x ** 2).max()) + float((y ** 2).max()))
     dr_ideal = math.sqrt(wavelength ** 2 + rmax ** 2 + 2 * wavelength * math.sqrt(rmax ** 2 + z ** 2)) - rmax
     return dr_ideal / dr_real
+
+
+def _z_ptr(z_dev: Optional[torch.Tensor], dev):
+    if z_dev is None:
+        return None
+    if z_dev.dtype != torch.float64 or z_dev.device != dev or z_dev.numel() != 1:
+        raise ValueError("z_dev must be a one-element float64 tensor on the field's device")
+    return z_dev.data_ptr()
 
 
 # ---------------------------------------------------------------------------
@@ -128,12 +174,15 @@ def rs_workspace_bytes(n: int, ncomp: int) -> int:
 def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Sequence[MaskSpec] = (),
                  ez_mode: int = N.XL_EZ_NONE, nin: Optional[int] = None, intensity: bool = False,
                  adjoint: bool = False, out: Optional[torch.Tensor] = None,
-                 workspace: Optional[torch.Tensor] = None):
+                 workspace: Optional[torch.Tensor] = None, z_dev: Optional[torch.Tensor] = None,
+                 inten: Optional[torch.Tensor] = None):
     """RS_propagation_jit / VRS_propagation_jit (wave_optics.py:281-289,
     vectorized_optics.py:364-373) with fused masks and Ez construction.
 
     src: (C, n, n) complex64; `nin` components are read (default C).
     Returns the (ncomp, n, n) field, plus |E|^2 if `intensity`.
+    z_dev: optional float64 device scalar read by the kernels instead of `z`
+    (a captured graph then replays with whatever distance it holds).
     """
     lib = N.load()
     src = _prep_field(src)
@@ -146,7 +195,10 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
     xd, yd = device_grid(x, dev), device_grid(y, dev)
     if out is None:
         out = torch.empty((ncomp, n, n), dtype=torch.complex64, device=dev)
-    inten = torch.empty((n, n), dtype=torch.float32, device=dev) if intensity else None
+    if intensity and inten is None:
+        inten = torch.empty((n, n), dtype=torch.float32, device=dev)
+    if not intensity:
+        inten = None
     need = lib.xl_rs_workspace_bytes(n, ncomp)
     if workspace is None or workspace.numel() < need:
         workspace = torch.empty(need, dtype=torch.uint8, device=dev)
@@ -168,6 +220,7 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
     ya = np.asarray(y, dtype=np.float64) if not isinstance(y, torch.Tensor) else y.detach().cpu().double().numpy()
     p.dx = float(xa[1] - xa[0])
     p.dy = float(ya[1] - ya[0])
+    p.z_dev = _z_ptr(z_dev, dev)
     keep = fill_masks(p.mask, masks, (n, n), dev, nin)
     N.check(lib.xl_rs_propagate(C.byref(p), workspace.data_ptr(), workspace.numel(), _stream(dev)))
     del keep
@@ -282,13 +335,18 @@ def resolve_czt_form(g: "CztGeometry", form: Optional[str] = None) -> str:
 def czt(src: torch.Tensor, g: CztGeometry, masks: Sequence[MaskSpec] = (), ez_mode: int = N.XL_EZ_NONE,
         nin: Optional[int] = None, ncomp: Optional[int] = None, adjoint: bool = False, mode: Optional[int] = None,
         out: Optional[torch.Tensor] = None, workspace: Optional[torch.Tensor] = None,
-        form: Optional[str] = None) -> torch.Tensor:
+        form: Optional[str] = None, batch: Optional[int] = None) -> torch.Tensor:
     """CZT_jit / VCZT_jit / CZT_for_high_NA_jit / VCZT_objective_lens core.
 
     Forward: src (C, ny, nx) -> (ncomp, my, mx).  Adjoint: src (ncomp, my, mx)
     cotangent -> (ncomp, ny, nx) cotangent of the pre-op output.  `form`:
     'bluestein', 'gemm' or 'auto' (default: the module setting).
+    `batch`: B independent problems of this geometry in the same launches
+    (Bluestein form): src (B, C, ...) or one shared (C, ...), mask parameters
+    (B, rows, cols) views or shared, output (B, ncomp, ...).
     """
+    if batch is not None:
+        return _czt_batched(src, g, masks, ez_mode, nin, ncomp, adjoint, mode, out, workspace, batch)
     lib = N.load()
     src = _prep_field(src)
     dev = src.device
@@ -321,11 +379,72 @@ def czt(src: torch.Tensor, g: CztGeometry, masks: Sequence[MaskSpec] = (), ez_mo
     return out
 
 
+def _czt_batched(src, g, masks, ez_mode, nin, ncomp, adjoint, mode, out, workspace, batch):
+    lib = N.load()
+    if not isinstance(src, torch.Tensor) or src.device.type != "cuda" or src.dtype != torch.complex64:
+        raise N.NativeUnavailable("fields must be complex64 tensors on a ROCm GPU")
+    N.ensure_device(src.device)
+    dev = src.device
+    src = src.contiguous()
+    shared = src.dim() == 3
+    if not shared and (src.dim() != 4 or src.shape[0] != batch):
+        raise ValueError(f"batched CZT input must be (C, rows, cols) or ({batch}, C, rows, cols)")
+    cdim = src.shape[-3]
+    nin = cdim if nin is None else nin
+    mode_eff = g.mode if mode is None else mode
+    if ncomp is None:
+        ncomp = 3 if (ez_mode == N.XL_EZ_VCZT or mode_eff == N.XL_CZT_HNA) and not adjoint else nin
+    p, keep = _czt_params(g, dev, ncomp, nin, adjoint, ez_mode, mode)
+    rows_in, cols_in = (p.ny_in, p.nx_in) if not adjoint else (p.my, p.mx)
+    rows_out, cols_out = (p.my, p.mx) if not adjoint else (p.ny_in, p.nx_in)
+    if tuple(src.shape[-2:]) != (rows_in, cols_in):
+        raise ValueError(f"CZT input is {tuple(src.shape[-2:])}, expected {(rows_in, cols_in)}")
+    if out is None:
+        out = torch.empty((batch, ncomp, rows_out, cols_out), dtype=torch.complex64, device=dev)
+    if tuple(out.shape) != (batch, ncomp, rows_out, cols_out) or not out.is_contiguous():
+        raise ValueError("batched CZT output must be a contiguous (B, ncomp, rows, cols) tensor")
+    p.inp = src.data_ptr()
+    p.in_comp_stride = rows_in * cols_in
+    p.out = out.data_ptr()
+    p.out_comp_stride = rows_out * cols_out
+    p.batch = batch
+    p.in_batch_stride = 0 if shared else cdim * rows_in * cols_in
+    p.out_batch_stride = ncomp * rows_out * cols_out
+    mk, p.mask_batch_stride = fill_masks(p.mask, masks, (rows_in, cols_in), dev, nin, batch=batch)
+    keep += mk
+    need = lib.xl_czt_workspace_bytes(C.byref(p))
+    if need == 0:
+        N.check(lib.xl_czt(C.byref(p), None, 0, _stream(dev)))
+    if workspace is None or workspace.numel() < need:
+        workspace = torch.empty(need, dtype=torch.uint8, device=dev)
+    N.check(lib.xl_czt(C.byref(p), workspace.data_ptr(), workspace.numel(), _stream(dev)))
+    del keep
+    return out
+
+
+def czt_workspace_bytes(g: CztGeometry, ncomp: int = 1, nin: Optional[int] = None, adjoint: bool = False,
+                        batch: int = 1, ez_mode: int = N.XL_EZ_NONE) -> int:
+    """Workspace of the Bluestein CZT of geometry g (for preallocation)."""
+    p = N.CztParams()
+    p.nin, p.ncomp = (ncomp if nin is None else nin), ncomp
+    p.ny_in, p.nx_in = len(g.y), len(g.x)
+    p.my, p.mx = len(g.yout), len(g.xout)
+    p.mode, p.adjoint, p.ez_mode = g.mode, 1 if adjoint else 0, ez_mode
+    p.dm = g.dm
+    p.fy1, p.fy2 = g.fy
+    p.fx1, p.fx2 = g.fx
+    p.batch = batch
+    need = N.load().xl_czt_workspace_bytes(C.byref(p))
+    if need == 0:
+        raise ValueError(N.load().xl_last_error().decode())
+    return int(need)
+
+
 # ---------------------------------------------------------------------------
 # standalone pre-ops
 # ---------------------------------------------------------------------------
 
-def _pre_params(src, x, y, z, masks, ez_mode, hna, ncomp, nin, hna_radius, hna_f):
+def _pre_params(src, x, y, z, masks, ez_mode, hna, ncomp, nin, hna_radius, hna_f, z_dev=None):
     dev = src.device
     rows, cols = src.shape[-2:]
     p = N.PreParams()
@@ -341,6 +460,7 @@ def _pre_params(src, x, y, z, masks, ez_mode, hna, ncomp, nin, hna_radius, hna_f
     p.x, p.y = xd.data_ptr(), yd.data_ptr()
     p.z = float(z)
     p.hna_radius, p.hna_f = float(hna_radius), float(hna_f)
+    p.z_dev = _z_ptr(z_dev, dev)
     keep = [xd, yd] + fill_masks(p.mask, masks, (rows, cols), dev, nin)
     return p, keep
 
@@ -364,14 +484,17 @@ def pre_forward(src: torch.Tensor, x, y, z: float = 0.0, masks: Sequence[MaskSpe
 def pre_backward(src: torch.Tensor, gout: torch.Tensor, x, y, z: float = 0.0, masks: Sequence[MaskSpec] = (),
                  ez_mode: int = N.XL_EZ_NONE, hna: bool = False, nin: Optional[int] = None,
                  need_src: bool = True, need_params=((False, False), (False, False)),
-                 hna_radius: float = 0.0, hna_f: float = 0.0):
-    """Adjoint of pre_forward: returns (grad of src[:nin], [[g00, g01], [g10, g11]])."""
+                 hna_radius: float = 0.0, hna_f: float = 0.0, z_dev: Optional[torch.Tensor] = None,
+                 gparams=None):
+    """Adjoint of pre_forward: returns (grad of src[:nin], [[g00, g01], [g10, g11]]).
+    `gparams` may hold preallocated (rows, cols) float32 outputs in the
+    [[g00, g01], [g10, g11]] slots that are needed."""
     lib = N.load()
     src = _prep_field(src)
     gout = _prep_field(gout)
     nin = src.shape[0] if nin is None else nin
     ncomp = gout.shape[0]
-    p, keep = _pre_params(src, x, y, z, masks, ez_mode, hna, ncomp, nin, hna_radius, hna_f)
+    p, keep = _pre_params(src, x, y, z, masks, ez_mode, hna, ncomp, nin, hna_radius, hna_f, z_dev)
     p.gout = gout.data_ptr()
     rows, cols = src.shape[-2:]
     gsrc = torch.empty((nin, rows, cols), dtype=torch.complex64, device=src.device) if need_src else None
@@ -380,7 +503,9 @@ def pre_backward(src: torch.Tensor, gout: torch.Tensor, x, y, z: float = 0.0, ma
     for i in range(2):
         for j in range(2):
             if need_params[i][j]:
-                gp[i][j] = torch.empty((rows, cols), dtype=torch.float32, device=src.device)
+                given = gparams[i][j] if gparams is not None else None
+                gp[i][j] = given if given is not None else torch.empty((rows, cols), dtype=torch.float32,
+                                                                       device=src.device)
                 p.gparam[i][j] = gp[i][j].data_ptr()
             else:
                 p.gparam[i][j] = None
@@ -398,3 +523,130 @@ def transfer_function_rs(x, y, z: float, wavelength: float, device) -> torch.Ten
     N.check(lib.xl_transfer_function_rs(out.data_ptr(), xd.data_ptr(), yd.data_ptr(), len(xd), len(yd), float(z),
                                         2 * math.pi / wavelength, _stream(device)))
     return out
+
+
+def pre_backward_batched(src: torch.Tensor, gout: torch.Tensor, x, y, batch: int, z: float = 0.0,
+                         masks: Sequence[MaskSpec] = (), ez_mode: int = N.XL_EZ_NONE, nin: Optional[int] = None,
+                         need_src: bool = True, need_params=((False, False), (False, False)),
+                         gparams=None, out: Optional[torch.Tensor] = None):
+    """pre_backward over a batch in one launch.  src: (B, nin, rows, cols) or one
+    shared (nin, rows, cols); gout: (B, ncomp, rows, cols); mask parameters
+    (B, rows, cols) views or shared.  Returns (grad of src (B, nin, ...) or
+    None, [[g00, g01], [g10, g11]] each (B, rows, cols) or None); `gparams`
+    may hold preallocated (B, rows, cols) views sharing one batch stride."""
+    lib = N.load()
+    for t in (src, gout):
+        if not isinstance(t, torch.Tensor) or t.device.type != "cuda" or t.dtype != torch.complex64:
+            raise N.NativeUnavailable("fields must be complex64 tensors on a ROCm GPU")
+    dev = src.device
+    N.ensure_device(dev)
+    src, gout = src.contiguous(), gout.contiguous()
+    rows, cols = src.shape[-2:]
+    shared = src.dim() == 3
+    nin = src.shape[-3] if nin is None else nin
+    if gout.dim() != 4 or gout.shape[0] != batch:
+        raise ValueError(f"cotangent must be ({batch}, ncomp, rows, cols)")
+    ncomp = gout.shape[1]
+    p = N.PreParams()
+    p.inp = src.data_ptr()
+    p.in_comp_stride = rows * cols
+    p.nin, p.ncomp = nin, ncomp
+    p.comp_stride = rows * cols
+    p.rows, p.cols = rows, cols
+    p.ez_mode = ez_mode
+    xd, yd = device_grid(x, dev), device_grid(y, dev)
+    p.x, p.y = xd.data_ptr(), yd.data_ptr()
+    p.z = float(z)
+    p.gout = gout.data_ptr()
+    keep, p.mask_batch_stride = fill_masks(p.mask, masks, (rows, cols), dev, nin, batch=batch)
+    p.batch = batch
+    p.in_batch_stride = 0 if shared else src.shape[-3] * rows * cols
+    p.gout_batch_stride = ncomp * rows * cols
+    gsrc = None
+    if need_src:
+        gsrc = out if out is not None else torch.empty((batch, nin, rows, cols), dtype=torch.complex64, device=dev)
+        if tuple(gsrc.shape) != (batch, nin, rows, cols) or not gsrc.is_contiguous():
+            raise ValueError("source-gradient output must be a contiguous (B, nin, rows, cols) tensor")
+        p.out = gsrc.data_ptr()
+        p.out_batch_stride = nin * rows * cols
+    gp = [[None, None], [None, None]]
+    strides = set()
+    for i in range(2):
+        for j in range(2):
+            if need_params[i][j]:
+                t = gparams[i][j] if gparams is not None and gparams[i][j] is not None else \
+                    torch.empty((batch, rows, cols), dtype=torch.float32, device=dev)
+                if (tuple(t.shape) != (batch, rows, cols) or t.dtype != torch.float32 or t.stride(2) != 1
+                        or t.stride(1) != cols):
+                    raise ValueError("parameter-gradient outputs must be float32 (B, rows, cols) with contiguous rows")
+                strides.add(t.stride(0))
+                gp[i][j] = t
+                p.gparam[i][j] = t.data_ptr()
+            else:
+                p.gparam[i][j] = None
+    if len(strides) > 1:
+        raise ValueError("parameter-gradient outputs must share one batch stride")
+    p.gparam_batch_stride = strides.pop() if strides else 0
+    N.check(lib.xl_pre_backward(C.byref(p), _stream(dev)))
+    del keep
+    return gsrc, gp
+
+
+# ---------------------------------------------------------------------------
+# loss and optimiser steps of batched optimisation
+# ---------------------------------------------------------------------------
+
+def mse_intensity(field: torch.Tensor, target_intensity: torch.Tensor, grad: Optional[torch.Tensor] = None,
+                  loss: Optional[torch.Tensor] = None, want_grad: bool = True, scale: float = 1.0,
+                  workspace: Optional[torch.Tensor] = None):
+    """MSE_Intensity (loss_functions.py:137-149) per batch member of `field`
+    (B, rows, cols) against |T|^2 = target_intensity (rows, cols).  Returns
+    (loss (B,), cotangent (B, rows, cols) or None); `grad` may alias `field`."""
+    lib = N.load()
+    if field.dtype != torch.complex64 or field.device.type != "cuda" or not field.is_contiguous():
+        raise N.NativeUnavailable("field must be a contiguous complex64 tensor on a ROCm GPU")
+    dev = field.device
+    N.ensure_device(dev)
+    B = field.shape[0]
+    npix = field[0].numel()
+    tgt = target_intensity.to(device=dev, dtype=torch.float32).contiguous()
+    if tgt.numel() != npix:
+        raise ValueError("target must have one value per pixel")
+    if loss is None:
+        loss = torch.empty(B, dtype=torch.float32, device=dev)
+    if want_grad and grad is None:
+        grad = torch.empty_like(field)
+    need = lib.xl_mse_workspace_bytes(npix, B)
+    if workspace is None or workspace.numel() < need:
+        workspace = torch.empty(need, dtype=torch.uint8, device=dev)
+    p = N.MseParams()
+    p.field = field.data_ptr()
+    p.field_batch_stride = npix
+    p.target = tgt.data_ptr()
+    p.grad = grad.data_ptr() if want_grad else None
+    p.grad_batch_stride = npix
+    p.loss = loss.data_ptr()
+    p.npix = npix
+    p.batch = B
+    p.scale = scale
+    N.check(lib.xl_mse_intensity(C.byref(p), workspace.data_ptr(), workspace.numel(), _stream(dev)))
+    return loss, (grad if want_grad else None)
+
+
+def adam_step(param: torch.Tensor, grad: torch.Tensor, m: torch.Tensor, v: torch.Tensor, count: torch.Tensor,
+              lr: float, b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8, weight_decay: float = 0.0) -> None:
+    """In-place Adam / AdamW step (optax.adam / optax.adamw) over contiguous
+    float32 tensors; `count` is a one-element int32 device tensor, advanced."""
+    lib = N.load()
+    for t in (param, grad, m, v):
+        if t.dtype != torch.float32 or t.device.type != "cuda" or not t.is_contiguous() or t.numel() != param.numel():
+            raise ValueError("Adam tensors must be contiguous float32 GPU tensors of one size")
+    if count.dtype != torch.int32 or count.numel() != 1 or count.device != param.device:
+        raise ValueError("count must be a one-element int32 tensor on the parameters' device")
+    N.ensure_device(param.device)
+    p = N.AdamParams()
+    p.param, p.grad, p.m, p.v = param.data_ptr(), grad.data_ptr(), m.data_ptr(), v.data_ptr()
+    p.n = param.numel()
+    p.count = count.data_ptr()
+    p.lr, p.b1, p.b2, p.eps, p.weight_decay = lr, b1, b2, eps, weight_decay
+    N.check(lib.xl_adam_step(C.byref(p), _stream(param.device)))
