@@ -167,6 +167,20 @@ def _latency(sweep):
     return out
 
 
+def _kernel_split(fn, steps):
+    """Per-kernel ms per call of fn (HIP events around each launch)."""
+    import torch
+    from xlumina_amd import _native as N
+    torch.cuda.synchronize()
+    N.timing_enable(capacity=64 * steps)
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    kt = N.timing_collect()
+    N.timing_disable()
+    return {k: round(v[1] / steps, 4) for k, v in kt.items()}
+
+
 def _time(fn, steps, warmup):
     import torch
     for _ in range(warmup):
@@ -183,6 +197,7 @@ def _time(fn, steps, warmup):
 
 
 def run_secondary(dev, workloads, args):
+    import numpy as np
     import torch
     from xlumina_amd import _native as N
     from xlumina_amd import ops
@@ -194,12 +209,25 @@ def run_secondary(dev, workloads, args):
     for form in ("bluestein", "gemm"):
         ms = _time(lambda: ops.czt(vrs, g, ez_mode=N.XL_EZ_VCZT, nin=2, form=form), max(10, args.steps), 3)
         ent = {"value": round(1000.0 / ms, 2), "unit": "VCZT/s", "ms": round(ms, 4)}
+        ent["kernels_ms"] = _kernel_split(lambda: ops.czt(vrs, g, ez_mode=N.XL_EZ_VCZT, nin=2, form=form), 10)
         if form == "gemm":
             flops = 3 * 8 * (512 * 2048 * 2048 + 512 * 512 * 2048) * 3
             ent["mfma_tflops"] = round(flops / (ms * 1e-3) / 1e12, 1)
             ent["mfma_frac_of_2500"] = round(flops / (ms * 1e-3) / 2.5e15, 4)
         out[f"vczt_2048_to_512_{form}"] = ent
         _progress(f"vczt {form}: {ms:.4f} ms")
+    # CZT form crossover: VCZT 2048^2 -> M^2 for small output windows
+    cross = {}
+    for m_out in (64, 128, 192, 256):
+        xo = np.linspace(-100.0, 100.0, m_out)
+        gm = ops.czt_geometry(w.x, w.y, xo, xo, w.z_czt, w.wavelength)
+        row = {}
+        for form in ("bluestein", "gemm"):
+            row[form + "_ms"] = round(_time(lambda: ops.czt(vrs, gm, ez_mode=N.XL_EZ_VCZT, nin=2, form=form),
+                                            max(10, args.steps // 2), 3), 4)
+        cross[str(m_out)] = row
+    out["vczt_2048_form_crossover"] = cross
+    _progress(f"CZT form crossover: {cross}")
     rs = workloads.scalar_rs_1024(dev)
     ms = _time(lambda: rs.run(), max(10, args.steps), 3)
     out["rs_1024_slm"] = {"value": round(1000.0 / ms, 2), "unit": "propagations/s", "ms": round(ms, 4)}

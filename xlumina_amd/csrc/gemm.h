@@ -27,7 +27,7 @@ namespace xl {
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(16))) float f32x16;
 
-constexpr int GBM = 128, GBN = 128, GBK = 32;   // block tile (complex), K step
+constexpr int GBM = 128, GBK = 32;   // block tile rows (complex), K step; the tile width is 32 * WN
 constexpr int GLDK = GBK + 8;                   // LDS row stride in bf16 (80 B: 16 B aligned, 2-way banks)
 
 __device__ __forceinline__ uint16_t f2bf(float x) {   // round to nearest even (finite inputs)
@@ -183,58 +183,97 @@ struct GemmArgs {
   const double* qy;
   double z, k;
   float2 cte;
+  float2* partial;   // split-K (gridDim.z > 1): fp32 partial sums [split][m][n]
 };
 
 __device__ __forceinline__ int64_t gop_off(const GemmOperand& o, int r) {
   return (int64_t)(r % o.rb) * o.ld + (int64_t)(r / o.rb) * o.boff;
 }
 
-__global__ __launch_bounds__(1024) void cgemm_kernel(GemmArgs a) {
-  // [operand][plane][row][k] bf16, single buffer; the next K step is prefetched into registers
-  __shared__ __attribute__((aligned(16))) uint16_t sm[2][4][GBM][GLDK];
+// Writes output element (m, n) of the product (mode 0 or 1).
+__device__ __forceinline__ void gemm_store(const GemmArgs& a, int m, int n, float2 v) {
+  if (a.mode == 0) {
+    store_split(a.c, (int64_t)m * a.c.ld + n, v);
+  } else {
+    const int comp = n / a.nb, kx = n - comp * a.nb;
+    float2 o = v;
+    if (a.post_mode == 1 /* POST_RS */) o = c_mul(c_mul(o, rs_kernel_value(a.qx[kx], a.qy[m], a.z, a.k)), a.cte);
+    else if (a.post_mode == 3 /* POST_CONJ_RS */) o = c_mul(o, conjf2(rs_kernel_value(a.qx[kx], a.qy[m], a.z, a.k)));
+    else if (a.post_mode == 2 /* POST_CONST */) o = c_mul(o, a.cte);
+    a.out[(int64_t)comp * a.out_cs + (int64_t)m * a.nb + kx] = o;
+  }
+}
+
+// Block tile GBM x (32 WN) complex on 4 x WN waves of 32 x 32; blockIdx.z
+// takes an even share of the K steps (split-K) and then writes fp32 partial
+// sums for cgemm_reduce_kernel instead of the output.
+template <int WN>
+__global__ __launch_bounds__(256 * WN) void cgemm_kernel(GemmArgs a) {
+  constexpr int BN = 32 * WN, NT = 256 * WN;
+  constexpr int CH_A = 4 * GBM * (GBK / 8);   // chunks of 8 bf16 per K step: A planes
+  constexpr int CH = CH_A + 4 * BN * (GBK / 8);
+  constexpr int PER = (CH + NT - 1) / NT;
+  // [plane][row][k] bf16, single buffer; the next K step is prefetched into registers
+  __shared__ __attribute__((aligned(16))) uint16_t smA[4][GBM][GLDK];
+  __shared__ __attribute__((aligned(16))) uint16_t smB[4][BN][GLDK];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int wm = w & 3, wn = w >> 2;                 // 4 x 4 waves of 32 x 32
-  const int bm = blockIdx.y * GBM, bn = blockIdx.x * GBN;
-  // global -> LDS mapping: 4096 chunks of 8 bf16 per K step, 4 per thread
-  uint4 pre[4];
+  const int wm = w & 3, wn = w >> 2;
+  const int bm = blockIdx.y * GBM, bn = blockIdx.x * BN;
+  const int ksteps = a.K / GBK;
+  const int kbeg = (int)((int64_t)ksteps * blockIdx.z / gridDim.z);
+  const int kend = (int)((int64_t)ksteps * (blockIdx.z + 1) / gridDim.z);
+  uint4 pre[PER];
   auto gload = [&](int k0) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int ch = tid + i * 1024;
-      const int op = ch >> 11, pl = (ch >> 9) & 3, r = (ch >> 2) & 127, kc = ch & 3;
-      const GemmOperand& o = op ? a.B : a.A;
-      const int gr = (op ? bn : bm) + r;
+    for (int i = 0; i < PER; ++i) {
+      const int ch = tid + i * NT;
       uint4 v = make_uint4(0, 0, 0, 0);
-      if (gr < o.rows) v = *reinterpret_cast<const uint4*>(o.p + pl * o.ps + gop_off(o, gr) + k0 + kc * 8);
+      if (CH % NT == 0 || ch < CH) {
+        const bool isA = ch < CH_A;
+        const int c2 = isA ? ch : ch - CH_A;
+        const int rows = isA ? GBM : BN;
+        const int pl = c2 / (4 * rows), r = (c2 >> 2) % rows, kc = c2 & 3;
+        const GemmOperand& o = isA ? a.A : a.B;
+        const int gr = (isA ? bm : bn) + r;
+        if (gr < o.rows) v = *reinterpret_cast<const uint4*>(o.p + pl * o.ps + gop_off(o, gr) + k0 + kc * 8);
+      }
       pre[i] = v;
     }
   };
   auto sstore = [&]() {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int ch = tid + i * 1024;
-      const int op = ch >> 11, pl = (ch >> 9) & 3, r = (ch >> 2) & 127, kc = ch & 3;
-      *reinterpret_cast<uint4*>(&sm[op][pl][r][kc * 8]) = pre[i];
+    for (int i = 0; i < PER; ++i) {
+      const int ch = tid + i * NT;
+      if (CH % NT == 0 || ch < CH) {
+        if (ch < CH_A) {
+          const int pl = ch / (4 * GBM), r = (ch >> 2) % GBM, kc = ch & 3;
+          *reinterpret_cast<uint4*>(&smA[pl][r][kc * 8]) = pre[i];
+        } else {
+          const int c2 = ch - CH_A;
+          const int pl = c2 / (4 * BN), r = (c2 >> 2) % BN, kc = c2 & 3;
+          *reinterpret_cast<uint4*>(&smB[pl][r][kc * 8]) = pre[i];
+        }
+      }
     }
   };
   f32x16 crr = {}, cii = {}, cri = {};
-  gload(0);
-  for (int k0 = 0; k0 < a.K; k0 += GBK) {
+  gload(kbeg * GBK);
+  for (int ks = kbeg; ks < kend; ++ks) {
     __syncthreads();
     sstore();
     __syncthreads();
-    if (k0 + GBK < a.K) gload(k0 + GBK);
+    if (ks + 1 < kend) gload((ks + 1) * GBK);
 #pragma unroll
     for (int kk = 0; kk < GBK; kk += 16) {
       const int ra = wm * 32 + (lane & 31), rb = wn * 32 + (lane & 31), kq = kk + 8 * (lane >> 5);
-      const bf16x8 arh = *reinterpret_cast<const bf16x8*>(&sm[0][0][ra][kq]);
-      const bf16x8 arl = *reinterpret_cast<const bf16x8*>(&sm[0][1][ra][kq]);
-      const bf16x8 aih = *reinterpret_cast<const bf16x8*>(&sm[0][2][ra][kq]);
-      const bf16x8 ail = *reinterpret_cast<const bf16x8*>(&sm[0][3][ra][kq]);
-      const bf16x8 brh = *reinterpret_cast<const bf16x8*>(&sm[1][0][rb][kq]);
-      const bf16x8 brl = *reinterpret_cast<const bf16x8*>(&sm[1][1][rb][kq]);
-      const bf16x8 bih = *reinterpret_cast<const bf16x8*>(&sm[1][2][rb][kq]);
-      const bf16x8 bil = *reinterpret_cast<const bf16x8*>(&sm[1][3][rb][kq]);
+      const bf16x8 arh = *reinterpret_cast<const bf16x8*>(&smA[0][ra][kq]);
+      const bf16x8 arl = *reinterpret_cast<const bf16x8*>(&smA[1][ra][kq]);
+      const bf16x8 aih = *reinterpret_cast<const bf16x8*>(&smA[2][ra][kq]);
+      const bf16x8 ail = *reinterpret_cast<const bf16x8*>(&smA[3][ra][kq]);
+      const bf16x8 brh = *reinterpret_cast<const bf16x8*>(&smB[0][rb][kq]);
+      const bf16x8 brl = *reinterpret_cast<const bf16x8*>(&smB[1][rb][kq]);
+      const bf16x8 bih = *reinterpret_cast<const bf16x8*>(&smB[2][rb][kq]);
+      const bf16x8 bil = *reinterpret_cast<const bf16x8*>(&smB[3][rb][kq]);
       crr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arl, brh, crr, 0, 0, 0);
       crr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, brl, crr, 0, 0, 0);
       crr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, brh, crr, 0, 0, 0);
@@ -252,21 +291,25 @@ __global__ __launch_bounds__(1024) void cgemm_kernel(GemmArgs a) {
   // epilogue: lane owns column n = lane % 32 and rows 8*(r/4) + 4*(lane/32) + r%4
   const int n = bn + wn * 32 + (lane & 31);
   if (n >= a.N) return;
+  const bool split = gridDim.z > 1;
+  float2* part = a.partial + (int64_t)blockIdx.z * a.M * a.N;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int m = bm + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
     if (m >= a.M) continue;
     const float2 v = make_float2(crr[r] - cii[r], cri[r]);
-    if (a.mode == 0) {
-      store_split(a.c, (int64_t)m * a.c.ld + n, v);
-    } else {
-      const int comp = n / a.nb, kx = n - comp * a.nb;
-      float2 o = v;
-      if (a.post_mode == 1 /* POST_RS */) o = c_mul(c_mul(o, rs_kernel_value(a.qx[kx], a.qy[m], a.z, a.k)), a.cte);
-      else if (a.post_mode == 3 /* POST_CONJ_RS */) o = c_mul(o, conjf2(rs_kernel_value(a.qx[kx], a.qy[m], a.z, a.k)));
-      else if (a.post_mode == 2 /* POST_CONST */) o = c_mul(o, a.cte);
-      a.out[(int64_t)comp * a.out_cs + (int64_t)m * a.nb + kx] = o;
-    }
+    if (split) part[(int64_t)m * a.N + n] = v;
+    else gemm_store(a, m, n, v);
+  }
+}
+
+// Sums the split-K partials in a fixed order and writes the output.
+__global__ void cgemm_reduce_kernel(GemmArgs a, int splits) {
+  const int64_t total = (int64_t)a.M * a.N;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    float2 v = a.partial[e];
+    for (int s = 1; s < splits; ++s) v = v + a.partial[s * total + e];
+    gemm_store(a, (int)(e / a.N), (int)(e % a.N), v);
   }
 }
 

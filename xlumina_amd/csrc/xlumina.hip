@@ -361,10 +361,41 @@ struct GemmPlan {
   int b_rowsp, b_ld;                 // pre-op planes: rows per component, row stride
   int64_t c1_ld;                     // stage-1 output row stride
   int c1_rows;
-  size_t ka_bytes, kb_bytes, g_bytes, c1_bytes;
+  int wn[2], splits[2];              // tile width (32 * wn) and split-K factor per stage
+  size_t ka_bytes, kb_bytes, g_bytes, c1_bytes, part_bytes;
 };
 
 int r32(int v) { return (v + 31) / 32 * 32; }
+
+// Tile width and split-K factor for an M x N x K product: the grid should
+// fill the 256 CUs (blocks of 4 x WN waves at 4 waves per SIMD: WN = 2 fits
+// two blocks per CU, WN = 3 or 4 one).  Split-K costs an fp32 partial pass
+// that grows with the split count: a split is charged 0.1 + 0.01 per split
+// against the filled fraction, scaled down when the partials are small
+// (under 32 MiB).
+void choose_gemm_shape(int M, int N, int K, int* wn_out, int* splits_out) {
+  constexpr int kCUs = 256;
+  const int tm = (M + xl::GBM - 1) / xl::GBM;
+  const int ksteps = K / xl::GBK;
+  const int smax = std::max(1, std::min(16, ksteps / 4));
+  double best = -1.0;
+  for (int wn = 4; wn >= 2; --wn) {
+    const int tn = (N + 32 * wn - 1) / (32 * wn);
+    const int slots = kCUs * (wn == 2 ? 2 : 1);
+    for (int sp = 1; sp <= smax; ++sp) {
+      const int64_t blocks = (int64_t)tm * tn * sp;
+      const int64_t rounds = (blocks + slots - 1) / slots;
+      const double part = (double)sp * M * N * sizeof(float2);
+      const double charge = sp > 1 ? (0.1 + 0.01 * sp) * std::min(1.0, part / (32.0 * 1048576.0)) : 0.0;
+      const double score = (double)blocks / (double)(rounds * slots) - charge;
+      if (score > best + 1e-9) {
+        best = score;
+        *wn_out = wn;
+        *splits_out = sp;
+      }
+    }
+  }
+}
 
 int plan_gemm(const xl_czt_params* p, GemmPlan* g) {
   if (int rc = plan_czt(p, &g->cz)) return rc;
@@ -385,12 +416,31 @@ int plan_gemm(const xl_czt_params* p, GemmPlan* g) {
   g->kb_bytes = (size_t)4 * g->a2_rows * g->a2_ld * 2;
   g->g_bytes = (size_t)4 * nc * g->b_rowsp * g->b_ld * 2;
   g->c1_bytes = (size_t)4 * g->c1_rows * g->c1_ld * 2;
+  // stage 1: M = a1_rows, N = nc * b_rowsp, K = b_ld; stage 2: M = a2_rows, N = nc * a1_rows, K = b_rowsp
+  const int sm[2] = {g->a1_rows, g->a2_rows}, sn[2] = {nc * g->b_rowsp, nc * g->a1_rows}, sk[2] = {g->b_ld, g->b_rowsp};
+  g->part_bytes = 0;
+  for (int i = 0; i < 2; ++i) {
+    choose_gemm_shape(sm[i], sn[i], sk[i], &g->wn[i], &g->splits[i]);
+    if (g->splits[i] > 1)
+      g->part_bytes = std::max(g->part_bytes, (size_t)g->splits[i] * sm[i] * sn[i] * sizeof(float2));
+  }
   return XL_OK;
 }
 
 size_t gemm_ws(const GemmPlan& g) {
   auto al = [](size_t b) { return (b + 255) / 256 * 256; };
-  return al(g.ka_bytes) + al(g.kb_bytes) + al(g.g_bytes) + al(g.c1_bytes) + 256;
+  return al(g.ka_bytes) + al(g.kb_bytes) + al(g.g_bytes) + al(g.c1_bytes) + al(g.part_bytes) + 256;
+}
+
+// Launches C = A B^T with the stage's tile width and split-K factor (then the
+// partial reduction when split).
+void launch_cgemm(xl::GemmArgs& a, int wn, int splits, hipStream_t st) {
+  const dim3 grid((a.N + 32 * wn - 1) / (32 * wn), (a.M + xl::GBM - 1) / xl::GBM, splits);
+  if (wn == 4) hipLaunchKernelGGL(xl::cgemm_kernel<4>, grid, dim3(1024), 0, st, a);
+  else if (wn == 3) hipLaunchKernelGGL(xl::cgemm_kernel<3>, grid, dim3(768), 0, st, a);
+  else hipLaunchKernelGGL(xl::cgemm_kernel<2>, grid, dim3(512), 0, st, a);
+  if (splits > 1)
+    hipLaunchKernelGGL(xl::cgemm_reduce_kernel, grid_for((int64_t)a.M * a.N), dim3(256), 0, st, a, splits);
 }
 
 xl::Planes planes_at(char* base, int rows, int64_t ld) {
@@ -437,6 +487,8 @@ int xl_czt_gemm(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream)
   xl::Planes gp = planes_at(w, p->ncomp * g.b_rowsp, g.b_ld);
   w += al(g.g_bytes);
   xl::Planes c1 = planes_at(w, g.c1_rows, g.c1_ld);
+  w += al(g.c1_bytes);
+  float2* part = g.part_bytes ? (float2*)w : nullptr;
   hipStream_t st = (hipStream_t)stream;
   const int nc = p->ncomp;
 
@@ -502,9 +554,9 @@ int xl_czt_gemm(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream)
     a.K = (int)g.b_ld;
     a.mode = 0;
     a.c = c1;
+    a.partial = part;
     xl::KTimer t(xl::K_GEMM_X, st);
-    hipLaunchKernelGGL(xl::cgemm_kernel, dim3((a.N + xl::GBN - 1) / xl::GBN, (a.M + xl::GBM - 1) / xl::GBM),
-                       dim3(1024), 0, st, a);
+    launch_cgemm(a, g.wn[0], g.splits[0], st);
   }
   // stage 2: out = K2 B2^T, B2 row (c, j) = C1[j][c * rowsp ...]
   {
@@ -530,9 +582,9 @@ int xl_czt_gemm(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream)
       a.qx = p->x; a.qy = p->y;
       a.post_mode = p->mode == XL_CZT_RS ? xl::POST_CONJ_RS : xl::POST_NONE;
     }
+    a.partial = part;
     xl::KTimer t(xl::K_GEMM_Y, st);
-    hipLaunchKernelGGL(xl::cgemm_kernel, dim3((a.N + xl::GBN - 1) / xl::GBN, (a.M + xl::GBM - 1) / xl::GBM),
-                       dim3(1024), 0, st, a);
+    launch_cgemm(a, g.wn[1], g.splits[1], st);
   }
   return hip_check(hipGetLastError(), "zoom-GEMM launch");
 }

@@ -308,7 +308,7 @@ CZT_FORMS = ("auto", "bluestein", "gemm")
 _czt_form = "auto"
 # Output-window size (per axis) below which "auto" picks the split-BF16 MFMA
 # zoom-DFT over the FP32 Bluestein FFT form (measured on MI355X; see bench.py).
-GEMM_CROSSOVER = 300
+GEMM_CROSSOVER = 160
 
 
 def set_czt_form(form: str) -> None:
