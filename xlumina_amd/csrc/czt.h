@@ -42,7 +42,7 @@ struct CztArgs {
   int nin;
   float2* out;
   int64_t out_cs;
-  float2* t;           // row-major intermediate: ncomp x rin x ax.lout
+  float2* t;           // intermediate, per component rin4 x lout4 in 4x4 tiles (czt_tidx)
   int rin, ncomp;      // input rows (cols = ax.lin)
   int pre_mode, post_mode, ez_mode;
   const double* px;    // grid of the input plane (pre-op)
@@ -55,7 +55,15 @@ struct CztArgs {
   MaskArgs mask;
   CztAxis ax, ay;
   int64_t in_bs, out_bs, t_bs, mask_bs;   // batch strides (blockIdx.y / .z selects the member)
+  int rin4;            // rin rounded up to a multiple of 4
+  int64_t t_cs;        // component stride of the intermediate (rin4 * lout4)
 };
+
+// Offset of (row, k) in the tiled intermediate: 4 rows x 4 outputs per 128-byte
+// tile, tiles ordered [k / 4][row / 4], so the column pass reads whole lines.
+__device__ __forceinline__ int64_t czt_tidx(int rin4, int row, int k) {
+  return ((int64_t)(k >> 2) * (rin4 >> 2) + (row >> 2)) * 16 + (row & 3) * 4 + (k & 3);
+}
 
 // The arguments of batch member b: inputs, outputs, intermediate and
 // per-pixel mask parameters offset by their batch strides.
@@ -191,7 +199,8 @@ __device__ __forceinline__ float2 czt_f_value(const CztArgs& a, int row, int col
 // transfer function in float64 phase, lens map) is computed once by one
 // thread for all components.  The chirp spectrum is applied in registers
 // between the forward and inverse chains and only the output window is
-// stored: row-major intermediate [c][row][k].
+// stored, into the tiled intermediate (consecutive rows run on one XCD, so
+// the four rows of a tile meet in one L2).
 template <int Q, int K>
 __global__ __launch_bounds__(3 * Q / 16) void czt_rows_kernel(CztArgs a0) {
   const CztArgs a = czt_member(a0, blockIdx.y);
@@ -200,7 +209,7 @@ __global__ __launch_bounds__(3 * Q / 16) void czt_rows_kernel(CztArgs a0) {
   extern __shared__ float2 lds[];
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int f = tid / TPF, t = tid % TPF;
-  const int row = blockIdx.x;
+  const int row = xcd_remap(blockIdx.x, gridDim.x);
   const int lin = a.ax.lin, lout = a.ax.lout, woff = a.ax.woff;
   const int nc = a.ncomp;
   for (int col = tid; col < Q; col += nthr) {
@@ -226,19 +235,20 @@ __global__ __launch_bounds__(3 * Q / 16) void czt_rows_kernel(CztArgs a0) {
   for (int b = 0; b < HO::BPT; ++b)
 #pragma unroll
     for (int q = 0; q < HO::R; ++q) u.v[b][q] = cmul(u.v[b][q], a.ax.gf[HO::index(t, b, q)]);
-  float2* dst = a.t + ((int64_t)f * a.rin + row) * lout;
+  float2* dst = a.t + (int64_t)f * a.t_cs;
   auto store = [&](int idx, float2 v) {
     const int k = idx - woff;
-    if (k >= 0 && k < lout) dst[k] = cmul(v, a.ax.post[k]);
+    if (k >= 0 && k < lout) dst[czt_tidx(a.rin4, row, k)] = cmul(v, a.ax.post[k]);
   };
   fft_inverse_from_regs<Q, TPF, DST_FN>(s, t, u, store);
 }
 
 // Per block CPB output columns of the x axis (one per thread group) of one
-// component: the column of the row-major intermediate is loaded straight into
-// the first pass, the chirp spectrum applied in registers, and only the
-// output window stored with the output-plane factor.  Neighbouring blocks share each
-// 128 B line of the intermediate, so consecutive blocks stay on one XCD.
+// component: the column of the tiled intermediate is loaded straight into the
+// first pass, the chirp spectrum applied in registers, and only the output
+// window stored with the output-plane factor.  A 128 B tile line holds four
+// columns: the CPB groups of a block (and at CPB = 2 the neighbouring block,
+// kept on the same XCD) share it.
 template <int Q> struct CztColsShape {
   static constexpr int CPB = Q >= 4096 ? 2 : 4;   // columns per block (more blocks at 4096)
 };
@@ -258,9 +268,9 @@ __global__ __launch_bounds__(CztColsShape<Q>::CPB * FftThreads<Q>::value) void c
   const int kx = og * CPB + f;
   const bool colok = kx < lout_x;
   const int lin = a.ay.lin, lout = a.ay.lout, woff = a.ay.woff;
-  const float2* src = a.t + (int64_t)c * a.rin * lout_x + kx;
+  const float2* src = a.t + (int64_t)c * a.t_cs;
   auto load = [&](int r) -> float2 {
-    return (colok && r < lin) ? cmul(src[(int64_t)r * lout_x], a.ay.pre[r]) : make_float2(0.f, 0.f);
+    return (colok && r < lin) ? cmul(src[czt_tidx(a.rin4, r, kx)], a.ay.pre[r]) : make_float2(0.f, 0.f);
   };
   typename HO::Type u;
   fft_forward_to_regs<Q, TPF, SRC_FN>(s, t, load, u);

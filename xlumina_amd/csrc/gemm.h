@@ -61,6 +61,28 @@ __device__ __forceinline__ void store_split(const Planes& d, int64_t off, float2
   d.p[3 * d.ps + off] = l;
 }
 
+// Two consecutive elements of a split matrix, packed for 4-byte stores (a
+// wave then writes 256 contiguous bytes per plane while its loads of the
+// source stay lane-contiguous).
+struct Split2 {
+  uint32_t w[4];   // [plane]: element 0 in the low half, element 1 in the high half
+};
+__device__ __forceinline__ void split2_put(Split2& s, int i, float2 v) {
+  uint16_t h[4];
+  split_bf16(v.x, h[0], h[1]);
+  split_bf16(v.y, h[2], h[3]);
+#pragma unroll
+  for (int pl = 0; pl < 4; ++pl) {
+    if (i) s.w[pl] |= (uint32_t)h[pl] << 16;
+    else s.w[pl] = h[pl];
+  }
+}
+// off: element offset of the first of the two (even).
+__device__ __forceinline__ void store_split2(const Planes& d, int64_t off, const Split2& s) {
+#pragma unroll
+  for (int pl = 0; pl < 4; ++pl) *reinterpret_cast<uint32_t*>(d.p + pl * d.ps + off) = s.w[pl];
+}
+
 // ---------------------------------------------------------------------------
 // kernel matrices: K[k][j] = post_k g[m+k-j] pre_j as float64 turns
 // ---------------------------------------------------------------------------
@@ -82,22 +104,32 @@ __device__ __forceinline__ double czt_post_turns(const KgenArgs& a, int k) {
   return a.th_turns * 0.5 * kk * kk - (a.d1_dm + kk * a.dl_dm) * (0.5 - 0.5 * a.m);
 }
 
-__global__ void kgen_kernel(KgenArgs a) {
-  const int64_t total = (int64_t)a.rows * a.dst.ld;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int r = (int)(e / a.dst.ld), c = (int)(e % a.dst.ld);
-    float2 v = make_float2(0.f, 0.f);
-    if (c < a.valid_cols) {
-      const int k = a.adjoint ? c : r;   // output index of the forward axis
-      const int j = a.adjoint ? r : c;   // input index
-      const int s = a.m + k - j;          // g index, in [1, m+M-1]
-      if (!(a.M >= a.m && s == a.m + a.M - 1)) {
-        const double u = (double)(s - a.m + 1);
-        const double turns = czt_post_turns(a, k) - a.th_turns * 0.5 * u * u + czt_pre_turns(a, j);
-        v = expi_turns(a.adjoint ? -turns : turns);
-      }
+__device__ __forceinline__ float2 kgen_value(const KgenArgs& a, int r, int c) {
+  float2 v = make_float2(0.f, 0.f);
+  if (c < a.valid_cols) {
+    const int k = a.adjoint ? c : r;   // output index of the forward axis
+    const int j = a.adjoint ? r : c;   // input index
+    const int s = a.m + k - j;          // g index, in [1, m+M-1]
+    if (!(a.M >= a.m && s == a.m + a.M - 1)) {
+      const double u = (double)(s - a.m + 1);
+      const double turns = czt_post_turns(a, k) - a.th_turns * 0.5 * u * u + czt_pre_turns(a, j);
+      v = expi_turns(a.adjoint ? -turns : turns);
     }
-    store_split(a.dst, e, v);
+  }
+  return v;
+}
+
+// Two consecutive columns per thread (ld is a multiple of 32).
+__global__ void kgen_kernel(KgenArgs a) {
+  const int64_t total2 = (int64_t)a.rows * a.dst.ld / 2;
+  for (int64_t e2 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e2 < total2;
+       e2 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t e = e2 * 2;
+    const int r = (int)(e / a.dst.ld), c0 = (int)(e % a.dst.ld);
+    Split2 sp;
+    split2_put(sp, 0, kgen_value(a, r, c0));
+    split2_put(sp, 1, kgen_value(a, r, c0 + 1));
+    store_split2(a.dst, e, sp);
   }
 }
 
@@ -120,40 +152,54 @@ struct GemmPreArgs {
   MaskArgs mask;
 };
 
+// Pre-op values of pixel (row, col) for every component (zero in the padding).
+__device__ __forceinline__ void gemm_pre_values(const GemmPreArgs& a, int row, int col, float2 (&v)[3]) {
+  v[0] = v[1] = v[2] = make_float2(0.f, 0.f);
+  if (row >= a.rows || col >= a.cols) return;
+  const int64_t pix = (int64_t)row * a.cols + col;
+  float2 e0 = a.in[pix];
+  float2 e1 = a.nin > 1 ? a.in[a.in_cs + pix] : make_float2(0.f, 0.f);
+  float2 e2 = a.nin > 2 ? a.in[2 * a.in_cs + pix] : make_float2(0.f, 0.f);
+  apply_mask(a.mask, pix, e0, e1, e2);
+  if (a.pre_mode == 2 /* PRE_HNA */) {
+    float A[3][2];
+    hna_map(a.px[col], a.py[row], a.hna_radius, a.hna_f, a.hna_stm2, A);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] = c_add(c_scale(e0, A[c][0]), c_scale(e1, A[c][1]));
+  } else {
+    v[0] = e0;
+    v[1] = e1;
+    v[2] = (a.ez_mode != EZ_NONE) ? build_ez(a.ez_mode, e0, e1, a.px[col], a.py[row], a.z) : e2;
+    if (a.pre_mode == 1 /* PRE_RS */ || a.pre_mode == 3 /* PRE_CONJ_RS */) {
+      float2 h = rs_kernel_value(a.px[col], a.py[row], a.z, a.k);
+      if (a.pre_mode == 3) h = c_mul(conjf2(h), a.cte);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[c] = c_mul(v[c], h);
+    } else if (a.pre_mode == 4 /* PRE_CONST */) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[c] = c_mul(v[c], a.cte);
+    }
+  }
+}
+
+// Two consecutive columns per thread (ld is a multiple of 32), 4-byte plane stores.
 __global__ void gemm_pre_kernel(GemmPreArgs a) {
-  const int64_t total = (int64_t)a.rowsp * a.dst.ld;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int row = (int)(e / a.dst.ld), col = (int)(e % a.dst.ld);
-    float2 v[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
-    if (row < a.rows && col < a.cols) {
-      const int64_t pix = (int64_t)row * a.cols + col;
-      float2 e0 = a.in[pix];
-      float2 e1 = a.nin > 1 ? a.in[a.in_cs + pix] : make_float2(0.f, 0.f);
-      float2 e2 = a.nin > 2 ? a.in[2 * a.in_cs + pix] : make_float2(0.f, 0.f);
-      apply_mask(a.mask, pix, e0, e1, e2);
-      if (a.pre_mode == 2 /* PRE_HNA */) {
-        float A[3][2];
-        hna_map(a.px[col], a.py[row], a.hna_radius, a.hna_f, a.hna_stm2, A);
+  const int64_t total2 = (int64_t)a.rowsp * a.dst.ld / 2;
+  for (int64_t e2 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e2 < total2;
+       e2 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t e = e2 * 2;
+    const int row = (int)(e / a.dst.ld), col0 = (int)(e % a.dst.ld);
+    Split2 sp[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) v[c] = c_add(c_scale(e0, A[c][0]), c_scale(e1, A[c][1]));
-      } else {
-        v[0] = e0;
-        v[1] = e1;
-        v[2] = (a.ez_mode != EZ_NONE) ? build_ez(a.ez_mode, e0, e1, a.px[col], a.py[row], a.z) : e2;
-        if (a.pre_mode == 1 /* PRE_RS */ || a.pre_mode == 3 /* PRE_CONJ_RS */) {
-          float2 h = rs_kernel_value(a.px[col], a.py[row], a.z, a.k);
-          if (a.pre_mode == 3) h = c_mul(conjf2(h), a.cte);
+    for (int i = 0; i < 2; ++i) {
+      float2 v[3];
+      gemm_pre_values(a, row, col0 + i, v);
 #pragma unroll
-          for (int c = 0; c < 3; ++c) v[c] = c_mul(v[c], h);
-        } else if (a.pre_mode == 4 /* PRE_CONST */) {
-#pragma unroll
-          for (int c = 0; c < 3; ++c) v[c] = c_mul(v[c], a.cte);
-        }
-      }
+      for (int c = 0; c < 3; ++c) split2_put(sp[c], i, v[c]);
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-      if (c < a.ncomp) store_split(a.dst, (int64_t)c * a.rowsp * a.dst.ld + e, v[c]);
+      if (c < a.ncomp) store_split2(a.dst, (int64_t)c * a.rowsp * a.dst.ld + e, sp[c]);
   }
 }
 

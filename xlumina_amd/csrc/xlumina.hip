@@ -162,7 +162,7 @@ int plan_czt(const xl_czt_params* p, CztPlan* cp) {
   if (p->batch < 0 || p->batch > 65535) return fail(XL_EINVAL, "batch must be 0..65535");
   cp->batch = std::max(1, (int)p->batch);
   cp->tables_bytes = (size_t)2 * 3 * xl::MAX_FFT * sizeof(float2);
-  cp->tiles_bytes = (size_t)cp->batch * cp->ncomp * cp->rin * cp->lout_x * sizeof(float2);
+  cp->tiles_bytes = (size_t)cp->batch * cp->ncomp * ((cp->rin + 3) / 4 * 4) * ((cp->lout_x + 3) / 4 * 4) * sizeof(float2);
   return XL_OK;
 }
 
@@ -344,7 +344,9 @@ int xl_czt(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream) {
   a.in_bs = batch > 1 ? p->in_batch_stride : 0;
   a.out_bs = batch > 1 ? p->out_batch_stride : 0;
   a.mask_bs = batch > 1 ? p->mask_batch_stride : 0;
-  a.t_bs = (int64_t)cp.ncomp * cp.rin * cp.lout_x;
+  a.rin4 = (cp.rin + 3) / 4 * 4;
+  a.t_cs = (int64_t)a.rin4 * ((cp.lout_x + 3) / 4 * 4);
+  a.t_bs = (int64_t)cp.ncomp * a.t_cs;
   if (int rc = dispatch_q(a.ax.q, [&](auto c) { xl::czt_rows_launch<decltype(c)::value>(a, batch, st); })) return rc;
   if (int rc = dispatch_q(a.ay.q, [&](auto c) { xl::czt_cols_launch<decltype(c)::value>(a, batch, st); })) return rc;
   return hip_check(hipGetLastError(), "CZT launch");
@@ -509,7 +511,7 @@ int xl_czt_gemm(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream)
     k.dl_dm = axp[i]->dl_dm;
     k.th_turns = axp[i]->th_turns;
     xl::KTimer t(xl::K_GEMM_KGEN, st);
-    hipLaunchKernelGGL(xl::kgen_kernel, grid_for((int64_t)k.rows * k.cols), dim3(256), 0, st, k);
+    hipLaunchKernelGGL(xl::kgen_kernel, grid_for((int64_t)k.rows * k.cols / 2), dim3(256), 0, st, k);
   }
   // pre-op planes
   {
@@ -541,7 +543,7 @@ int xl_czt_gemm(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream)
       a.pre_mode = p->mode == XL_CZT_RS ? xl::PRE_CONJ_RS : (p->mode == XL_CZT_HNA ? xl::PRE_CONST : xl::PRE_NONE);
     }
     xl::KTimer t(xl::K_GEMM_PRE, st);
-    hipLaunchKernelGGL(xl::gemm_pre_kernel, grid_for((int64_t)g.b_rowsp * g.b_ld), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(xl::gemm_pre_kernel, grid_for((int64_t)g.b_rowsp * g.b_ld / 2), dim3(256), 0, st, a);
   }
   // stage 1: C1 = K1 G^T
   {
