@@ -38,6 +38,8 @@ def kernel_bytes(name, n, P, ncomp, nin, mask_params, m_out, czt_rows_in, czt_co
         return (P // 2) * P * c8
     if name == "rs_urow":
         return nin * n * n * c8 + mask_params * n * n * 4 + ncomp * n * P * c8
+    if name == "rs_hcol":
+        return 2 * (P // 2) * P * c8
     if name == "rs_col":
         return (P // 2) * P * c8 + 2 * ncomp * n * P * c8
     if name == "rs_final":

@@ -75,9 +75,9 @@ int main(void) {
 def test_host_side_queries(lib):
     assert lib.xl_abi_version() == 2
     assert lib.xl_max_fft_length() == 4096
-    # RS workspace: ncomp * round4(n) * P + (P/2) * P complex64 (+ alignment slack)
+    # RS workspace: ncomp * round4(n) * P + (P/2) * P + P complex64 (+ alignment slack)
     n, P = 1024, 2048
-    assert lib.xl_rs_workspace_bytes(n, 1) == (n * P + (P // 2) * P) * 8 + 256
+    assert lib.xl_rs_workspace_bytes(n, 1) == (n * P + (P // 2) * P + P) * 8 + 256
     assert lib.xl_rs_workspace_bytes(1, 1) == 0
     assert lib.xl_rs_workspace_bytes(4096, 1) > 0   # rejected later by the propagate call (P > 4096)
 

@@ -17,8 +17,9 @@
 // 128 B lines):
 //   rs_hrow   : H rows 0..P/2-1 generated in float64 phase -> row FFT -> Hr
 //   rs_urow   : masks + Ez on the input rows -> zero-padded row FFT -> T
-//   rs_col    : per 4 columns: FFT(Hr col) -> Hhat (registers), then for each
-//               component FFT(T col) * Hhat -> IFFT, keep rows 0..N-1 -> T
+//   rs_hcol   : per column: FFT(Hr col, mirrored) -> Hhat, in place (half spectrum)
+//   rs_col    : per (column, component): FFT(T col) * Hhat -> IFFT, keep rows
+//               0..N-1 -> T
 //   rs_final  : row IFFT of T, keep columns 0..N-1 -> output (+ |E|^2)
 #pragma once
 #include "fft.h"
@@ -35,7 +36,8 @@ struct RsArgs {
   int64_t out_cs;
   float* intensity;       // optional N x N, sum_c |out_c|^2
   float2* t;              // tiles, ncomp x NR4*4 x P
-  float2* hr;             // tiles, (P/2) x P
+  float2* hr;             // tiles, (P/2) x P: Hr after the row pass, Hhat (ky < P/2) after rs_hcol
+  float2* hmid;           // P: Hhat at ky = P/2
   const double* x;
   const double* y;
   double z, k;
@@ -131,65 +133,87 @@ __global__ __launch_bounds__(3 * (P / 16)) void rs_urow_kernel(RsArgs a) {
   fft_io<P, TPF, false, SRC_LDS, DST_FN>(lds + f * LS, t, none, store, unused);
 }
 
-// Column pass.  A block holds G = 4 / (1 + ncomp) column slots; each slot
-// has 1 + ncomp thread groups that run side by side: group 0 transforms the
-// kernel column (Hr, mirrored), groups 1..ncomp the component columns (T).
-// At the forward/inverse hand-off group 0 leaves the kernel spectrum in its
-// LDS buffer, the component groups multiply their register-held spectra by it
-// and run the inverse transform, storing rows 0..N-1 back in place.  Nothing is
-// held in registers across components, so each thread keeps one radix-16
-// butterfly (1024 threads at P = 4096) without spilling.
+// Column units per block for the per-column kernels: one column per block
+// once an FFT has a wave or more of threads, several below that.
+template <int P> struct ColUnits {
+  static constexpr int TPF = P / 16;
+  static constexpr int CPB = TPF >= 64 ? 1 : 64 / TPF;
+};
+
+// Kernel spectrum.  Per column: the Hr column (rows mirrored, row P/2 zero)
+// is transformed along y and the spectrum, times dx dy / P^2 (conjugated for
+// the adjoint), replaces the Hr column in place for ky < P/2; ky = P/2 goes
+// to hmid[col].  The array is even in y, so ky > P/2 is not stored
+// (Hhat(P - ky) = Hhat(ky)).
 template <int P>
-__global__ __launch_bounds__(P / 4) void rs_col_kernel(RsArgs a) {
-  constexpr int TPF = P / 16, LS = LdsLen<P>::value;
+__global__ __launch_bounds__(ColUnits<P>::CPB * (P / 16)) void rs_hcol_kernel(RsArgs a) {
+  constexpr int TPF = P / 16, LS = LdsLen<P>::value, CPB = ColUnits<P>::CPB;
   using HO = HandOff<P, TPF>;
-  __shared__ float2 lds[4 * LS];
-  const int tid = threadIdx.x, g = tid / TPF, t = tid % TPF;
+  __shared__ float2 lds[CPB * LS];
+  const int g = threadIdx.x / TPF, t = threadIdx.x % TPF;
   float2* s = lds + g * LS;
-  const int ng = 1 + a.ncomp;           // groups per column slot
-  const int G = 4 / ng;                 // column slots per block
-  const int slot = g / ng, role = g - slot * ng;
-  const bool active = slot < G;
-  // the four columns of a tile run back to back on one XCD (shared 128 B lines)
-  const int col = xcd_remap(blockIdx.x, gridDim.x) * G + slot;
-  const int nrg = (a.n + 3) >> 2;
-  const int nr = nrg * 4;
+  const int col = xcd_remap(blockIdx.x, gridDim.x) * CPB + g;
   const int cg = col >> 2, cc = col & 3;
-  const float2* hsrc = a.hr + (int64_t)cg * (P / 8) * 16 + cc;
-  float2* tc = a.t + (int64_t)(role > 0 ? role - 1 : 0) * nrg * 4 * P + (int64_t)cg * nrg * 16 + cc;
+  float2* hc = a.hr + (int64_t)cg * (P / 8) * 16 + cc;
   auto load = [&](int r) -> float2 {
-    if (!active) return make_float2(0.f, 0.f);
-    if (role == 0) {
-      const int rr = r <= P / 2 ? r : P - r;
-      return rr == P / 2 ? make_float2(0.f, 0.f) : hsrc[(rr >> 2) * 16 + (rr & 3) * 4];
-    }
-    return r < nr ? tc[(r >> 2) * 16 + (r & 3) * 4] : make_float2(0.f, 0.f);
+    const int rr = r <= P / 2 ? r : P - r;
+    return rr == P / 2 ? make_float2(0.f, 0.f) : hc[(rr >> 2) * 16 + (rr & 3) * 4];
   };
   typename HO::Type u;
   fft_forward_to_regs<P, TPF, SRC_FN>(s, t, load, u);
-  // kernel spectrum of this slot -> its group-0 buffer (hand-off layout)
-  __syncthreads();
-  if (active && role == 0) {
+#pragma unroll
+  for (int b = 0; b < HO::BPT; ++b)
+#pragma unroll
+    for (int q = 0; q < HO::R; ++q) {
+      const int ky = HO::index(t, b, q);
+      float2 h = u.v[b][q] * a.scale;
+      if (a.adjoint) h = conjf2(h);
+      if (ky < P / 2) hc[(ky >> 2) * 16 + (ky & 3) * 4] = h;
+      else if (ky == P / 2) a.hmid[col] = h;
+    }
+}
+
+// Column pass.  One (column, component) unit per thread group: the T column
+// (rows 0..N-1, zero padded) is transformed along y straight from the tiles,
+// multiplied in registers by the kernel spectrum of its column (read from
+// the in-place Hhat, mirrored for ky > P/2), inverse-transformed, and rows
+// 0..N-1 stored back in place.  A block holds one 4096-point FFT (35 KB of
+// LDS), so four blocks share a CU and their pass barriers interleave.  The
+// units of a column and of its tile neighbours run back to back on one XCD,
+// so the Hhat column and the 128 B tile lines are fetched once into its L2.
+template <int P>
+__global__ __launch_bounds__(ColUnits<P>::CPB * (P / 16)) void rs_col_kernel(RsArgs a) {
+  constexpr int TPF = P / 16, LS = LdsLen<P>::value, CPB = ColUnits<P>::CPB;
+  using HO = HandOff<P, TPF>;
+  __shared__ float2 lds[CPB * LS];
+  const int g = threadIdx.x / TPF, t = threadIdx.x % TPF;
+  float2* s = lds + g * LS;
+  const int unit = xcd_remap(blockIdx.x, gridDim.x) * CPB + g;
+  const int col = unit / a.ncomp, comp = unit - col * a.ncomp;
+  const bool active = col < P;
+  const int nrg = (a.n + 3) >> 2;
+  const int nr = nrg * 4;
+  const int cg = col >> 2, cc = col & 3;
+  const float2* hc = a.hr + (int64_t)cg * (P / 8) * 16 + cc;
+  float2* tc = a.t + (int64_t)comp * nrg * 4 * P + (int64_t)cg * nrg * 16 + cc;
+  auto load = [&](int r) -> float2 {
+    return (active && r < nr) ? tc[(r >> 2) * 16 + (r & 3) * 4] : make_float2(0.f, 0.f);
+  };
+  typename HO::Type u;
+  fft_forward_to_regs<P, TPF, SRC_FN>(s, t, load, u);
+  if (active) {
 #pragma unroll
     for (int b = 0; b < HO::BPT; ++b)
 #pragma unroll
       for (int q = 0; q < HO::R; ++q) {
-        const float2 h = u.v[b][q] * a.scale;
-        s[lpad(HO::index(t, b, q))] = a.adjoint ? conjf2(h) : h;
+        const int ky = HO::index(t, b, q);
+        const int kk = ky <= P / 2 ? ky : P - ky;
+        const float2 h = kk == P / 2 ? a.hmid[col] : hc[(kk >> 2) * 16 + (kk & 3) * 4];
+        u.v[b][q] = cmul(u.v[b][q], h);
       }
   }
-  __syncthreads();
-  if (active && role > 0) {
-    const float2* hs = lds + (slot * ng) * LS;
-#pragma unroll
-    for (int b = 0; b < HO::BPT; ++b)
-#pragma unroll
-      for (int q = 0; q < HO::R; ++q) u.v[b][q] = cmul(u.v[b][q], hs[lpad(HO::index(t, b, q))]);
-  }
-  // the inverse passes write only this group's own buffer; group 0's kernel
-  // spectrum has been consumed once every group passed the barrier inside
   auto store = [&](int r, float2 v) {
-    if (active && role > 0 && r < nr) tc[(r >> 2) * 16 + (r & 3) * 4] = v;
+    if (active && r < nr) tc[(r >> 2) * 16 + (r & 3) * 4] = v;
   };
   fft_inverse_from_regs<P, TPF, DST_FN>(s, t, u, store);
 }
@@ -260,9 +284,15 @@ inline hipError_t rs_launch(const RsArgs& a, hipStream_t st) {
                        (size_t)a.ncomp * LdsLen<P>::value * sizeof(float2), st, a);
   }
   {
+    KTimer t(K_RS_HCOL, st);
+    constexpr int CPB = ColUnits<P>::CPB;
+    hipLaunchKernelGGL(rs_hcol_kernel<P>, dim3(P / CPB), dim3(CPB * (P / 16)), 0, st, a);
+  }
+  {
     KTimer t(K_RS_COL, st);
-    const int G = 4 / (1 + a.ncomp);
-    hipLaunchKernelGGL(rs_col_kernel<P>, dim3((P + G - 1) / G), dim3(P / 4), 0, st, a);
+    constexpr int CPB = ColUnits<P>::CPB;
+    const int units = P * a.ncomp;
+    hipLaunchKernelGGL(rs_col_kernel<P>, dim3((units + CPB - 1) / CPB), dim3(CPB * (P / 16)), 0, st, a);
   }
   { KTimer t(K_RS_FINAL, st); hipLaunchKernelGGL(rs_final_kernel<P>, dim3(nrg), blk, 0, st, a); }
   return hipGetLastError();

@@ -28,6 +28,7 @@ enum KernelId {
   K_GEMM_X,
   K_MSE,
   K_ADAM,
+  K_RS_HCOL,
   K_COUNT
 };
 
@@ -35,7 +36,7 @@ inline const char* kernel_name(int id) {
   static const char* names[K_COUNT] = {"rs_hrow", "rs_urow", "rs_col", "rs_final", "czt_setup", "czt_rows",
                                        "czt_cols", "pre_forward", "pre_backward", "transfer_function",
                                        "gemm_kernel_gen", "gemm_pre", "gemm_y", "gemm_x", "mse_intensity",
-                                       "adam"};
+                                       "adam", "rs_hcol"};
   return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
 

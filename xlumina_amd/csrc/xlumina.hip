@@ -103,7 +103,7 @@ int rs_fft_len(int n) { return std::max(64, next_pow2(2 * (int64_t)n - 1)); }
 size_t rs_ws(int n, int ncomp) {
   const int64_t P = rs_fft_len(n);
   const int64_t nr4 = ((n + 3) / 4) * 4;
-  return (size_t)(ncomp * nr4 * P + (P / 2) * P) * sizeof(float2) + 256;
+  return (size_t)(ncomp * nr4 * P + (P / 2) * P + P) * sizeof(float2) + 256;
 }
 
 // ---------------------------------------------------------------- CZT
@@ -223,6 +223,7 @@ int xl_rs_propagate(const xl_rs_params* p, void* ws, size_t ws_bytes, void* stre
   a.intensity = p->intensity;
   a.t = (float2*)w;
   a.hr = a.t + ncomp * nr4 * P;
+  a.hmid = a.hr + (int64_t)(P / 2) * P;
   a.x = p->x;
   a.y = p->y;
   a.z = p->z;
