@@ -71,13 +71,24 @@ __device__ __forceinline__ int64_t hr_idx(int r, int c, int P) {
   return ((int64_t)(c >> 2) * (P / 8) + (r >> 2)) * 16 + (r & 3) * 4 + (c & 3);
 }
 
+// Rows or columns per block for the one-FFT-per-group kernels: an FFT runs on
+// P/16 threads (one radix-16 butterfly each per pass) in one LDS buffer of
+// 35 KB at P = 4096, so four blocks share a CU and their pass barriers
+// interleave; below a wave of threads per FFT a block takes several.
+template <int P> struct Units {
+  static constexpr int TPF = P / 16;
+  static constexpr int CPB = TPF >= 64 ? 1 : 64 / TPF;
+};
+
+// One kernel row per group: H of rows 0..P/2-1 generated in float64 phase
+// (the array is even in x: columns above P/2 mirror), row FFT, stored as Hr.
 template <int P>
-__global__ __launch_bounds__(4 * FftThreads<P>::value) void rs_hrow_kernel(RsArgs a) {
-  constexpr int TPF = FftThreads<P>::value, LS = LdsLen<P>::value;
-  __shared__ float2 lds[4 * LS];
-  const int tid = threadIdx.x, f = tid / TPF, t = tid % TPF;
-  float2* s = lds + f * LS;
-  const int r = blockIdx.x * 4 + f;      // rows 0..P/2-1
+__global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_hrow_kernel(RsArgs a) {
+  constexpr int TPF = P / 16, LS = LdsLen<P>::value, CPB = Units<P>::CPB;
+  __shared__ float2 lds[CPB * LS];
+  const int g = threadIdx.x / TPF, t = threadIdx.x % TPF;
+  float2* s = lds + g * LS;
+  const int r = xcd_remap(blockIdx.x, gridDim.x) * CPB + g;      // rows 0..P/2-1
   const double z = a.z_dev ? *a.z_dev : a.z;
   double dy;
   const bool rowok = lag_of(a.y, r, a.n, P, &dy);
@@ -133,21 +144,14 @@ __global__ __launch_bounds__(3 * (P / 16)) void rs_urow_kernel(RsArgs a) {
   fft_io<P, TPF, false, SRC_LDS, DST_FN>(lds + f * LS, t, none, store, unused);
 }
 
-// Column units per block for the per-column kernels: one column per block
-// once an FFT has a wave or more of threads, several below that.
-template <int P> struct ColUnits {
-  static constexpr int TPF = P / 16;
-  static constexpr int CPB = TPF >= 64 ? 1 : 64 / TPF;
-};
-
 // Kernel spectrum.  Per column: the Hr column (rows mirrored, row P/2 zero)
 // is transformed along y and the spectrum, times dx dy / P^2 (conjugated for
 // the adjoint), replaces the Hr column in place for ky < P/2; ky = P/2 goes
 // to hmid[col].  The array is even in y, so ky > P/2 is not stored
 // (Hhat(P - ky) = Hhat(ky)).
 template <int P>
-__global__ __launch_bounds__(ColUnits<P>::CPB * (P / 16)) void rs_hcol_kernel(RsArgs a) {
-  constexpr int TPF = P / 16, LS = LdsLen<P>::value, CPB = ColUnits<P>::CPB;
+__global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_hcol_kernel(RsArgs a) {
+  constexpr int TPF = P / 16, LS = LdsLen<P>::value, CPB = Units<P>::CPB;
   using HO = HandOff<P, TPF>;
   __shared__ float2 lds[CPB * LS];
   const int g = threadIdx.x / TPF, t = threadIdx.x % TPF;
@@ -182,8 +186,8 @@ __global__ __launch_bounds__(ColUnits<P>::CPB * (P / 16)) void rs_hcol_kernel(Rs
 // units of a column and of its tile neighbours run back to back on one XCD,
 // so the Hhat column and the 128 B tile lines are fetched once into its L2.
 template <int P>
-__global__ __launch_bounds__(ColUnits<P>::CPB * (P / 16)) void rs_col_kernel(RsArgs a) {
-  constexpr int TPF = P / 16, LS = LdsLen<P>::value, CPB = ColUnits<P>::CPB;
+__global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_col_kernel(RsArgs a) {
+  constexpr int TPF = P / 16, LS = LdsLen<P>::value, CPB = Units<P>::CPB;
   using HO = HandOff<P, TPF>;
   __shared__ float2 lds[CPB * LS];
   const int g = threadIdx.x / TPF, t = threadIdx.x % TPF;
@@ -269,9 +273,10 @@ __global__ __launch_bounds__(4 * FftThreads<P>::value) void rs_final_kernel(RsAr
 
 template <int P>
 inline hipError_t rs_launch(const RsArgs& a, hipStream_t st) {
-  const int nrg = (a.n + 3) >> 2;
-  dim3 blk(4 * FftThreads<P>::value);
-  { KTimer t(K_RS_HROW, st); hipLaunchKernelGGL(rs_hrow_kernel<P>, dim3(P / 8), blk, 0, st, a); }
+  constexpr int CPB = Units<P>::CPB;
+  const dim3 blk(CPB * (P / 16));
+  const int rows = ((a.n + 3) >> 2) * 4;
+  { KTimer t(K_RS_HROW, st); hipLaunchKernelGGL(rs_hrow_kernel<P>, dim3(P / 2 / CPB), blk, 0, st, a); }
   {
     KTimer t(K_RS_UROW, st);
     static bool attr_set = false;   // up to three 4096-point buffers of dynamic LDS
@@ -280,21 +285,19 @@ inline hipError_t rs_launch(const RsArgs& a, hipStream_t st) {
                                 3 * LdsLen<P>::value * (int)sizeof(float2));
       attr_set = true;
     }
-    hipLaunchKernelGGL(rs_urow_kernel<P>, dim3(nrg * 4), dim3(a.ncomp * (P / 16)), 
+    hipLaunchKernelGGL(rs_urow_kernel<P>, dim3(rows), dim3(a.ncomp * (P / 16)), 
                        (size_t)a.ncomp * LdsLen<P>::value * sizeof(float2), st, a);
   }
-  {
-    KTimer t(K_RS_HCOL, st);
-    constexpr int CPB = ColUnits<P>::CPB;
-    hipLaunchKernelGGL(rs_hcol_kernel<P>, dim3(P / CPB), dim3(CPB * (P / 16)), 0, st, a);
-  }
+  { KTimer t(K_RS_HCOL, st); hipLaunchKernelGGL(rs_hcol_kernel<P>, dim3(P / CPB), blk, 0, st, a); }
   {
     KTimer t(K_RS_COL, st);
-    constexpr int CPB = ColUnits<P>::CPB;
     const int units = P * a.ncomp;
-    hipLaunchKernelGGL(rs_col_kernel<P>, dim3((units + CPB - 1) / CPB), dim3(CPB * (P / 16)), 0, st, a);
+    hipLaunchKernelGGL(rs_col_kernel<P>, dim3((units + CPB - 1) / CPB), blk, 0, st, a);
   }
-  { KTimer t(K_RS_FINAL, st); hipLaunchKernelGGL(rs_final_kernel<P>, dim3(nrg), blk, 0, st, a); }
+  {
+    KTimer t(K_RS_FINAL, st);
+    hipLaunchKernelGGL(rs_final_kernel<P>, dim3(rows / 4), dim3(4 * FftThreads<P>::value), 0, st, a);
+  }
   return hipGetLastError();
 }
 
