@@ -222,29 +222,33 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_col_kernel(RsA
   fft_inverse_from_regs<P, TPF, DST_FN>(s, t, u, store);
 }
 
-// Per block 4 rows (one per thread group): the 4-row tiles of each component
-// are staged into LDS with whole-line loads, inverse-transformed in LDS, and
-// the first N samples stored; |E|^2 is accumulated over the components.
+// Per block 2 rows (one per thread group): the row halves of
+// each component's tiles are staged into LDS with 64 B loads (the block
+// holding the other two rows of a tile runs next on the same XCD and finds
+// the line in L2), inverse-transformed in LDS, and the first N samples stored;
+// |E|^2 is accumulated over the components.  Two 35 KB buffers per block let
+// two blocks share a CU.
 template <int P>
-__global__ __launch_bounds__(4 * FftThreads<P>::value) void rs_final_kernel(RsArgs a) {
+__global__ __launch_bounds__(2 * FftThreads<P>::value) void rs_final_kernel(RsArgs a) {
   constexpr int TPF = FftThreads<P>::value, LS = LdsLen<P>::value;
-  constexpr int NT = 4 * TPF;
+  constexpr int NT = 2 * TPF;
   constexpr int PPT = (P / 2 + TPF - 1) / TPF;
-  __shared__ float2 lds[4 * LS];
+  __shared__ float2 lds[2 * LS];
   const int tid = threadIdx.x, f = tid / TPF, t = tid % TPF;
   float2* s = lds + f * LS;
   const int nrg = (a.n + 3) >> 2;
-  const int rg = blockIdx.x;
-  const int row = rg * 4 + f;
+  const int pair = xcd_remap(blockIdx.x, gridDim.x);
+  const int rg = pair >> 1, r0 = (pair & 1) * 2;
+  const int row = rg * 4 + r0 + f;
   float inten[PPT];
 #pragma unroll
   for (int i = 0; i < PPT; ++i) inten[i] = 0.f;
   for (int c = 0; c < a.ncomp; ++c) {
     const float2* tc = a.t + (int64_t)c * nrg * 4 * P;
 #pragma unroll 4
-    for (int e = tid; e < 4 * P; e += NT) {
-      const int cgp = e >> 4, rr = (e >> 2) & 3, cc = e & 3;
-      lds[rr * LS + lpad(cgp * 4 + cc)] = tc[((int64_t)cgp * nrg + rg) * 16 + rr * 4 + cc];
+    for (int e = tid; e < 2 * P; e += NT) {
+      const int cgp = e >> 3, rr = (e >> 2) & 1, cc = e & 3;
+      lds[rr * LS + lpad(cgp * 4 + cc)] = tc[((int64_t)cgp * nrg + rg) * 16 + (r0 + rr) * 4 + cc];
     }
     __syncthreads();
     fft_lds<P, TPF, true>(s, t);
@@ -296,7 +300,7 @@ inline hipError_t rs_launch(const RsArgs& a, hipStream_t st) {
   }
   {
     KTimer t(K_RS_FINAL, st);
-    hipLaunchKernelGGL(rs_final_kernel<P>, dim3(rows / 4), dim3(4 * FftThreads<P>::value), 0, st, a);
+    hipLaunchKernelGGL(rs_final_kernel<P>, dim3(rows / 2), dim3(2 * FftThreads<P>::value), 0, st, a);
   }
   return hipGetLastError();
 }
