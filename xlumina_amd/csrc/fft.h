@@ -10,7 +10,7 @@
 //
 // Inter-pass twiddles come from a 4096-entry table e^{-2 pi i k/4096}
 // generated once per device in float64 (xl_fft_init_kernel).  A butterfly
-// loads w, w^2, w^4, w^8 and forms the other powers by complex products.
+// loads w and forms the other powers by complex products.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -87,27 +87,29 @@ template <bool INV> struct DFT<1, INV> {
   __device__ __forceinline__ static void run(float2*) {}
 };
 
-// v[r] *= w^r for r < R with w = e^{-+2 pi i base / TWN}: four table loads
-// (w, w^2, w^4, w^8), the other powers as products formed next to their use so
-// that only the four bases stay live.
+// v[r] *= w^r for r < R with w = e^{-+2 pi i base / TWN}: one table load (w);
+// w^2, w^4, w^8 by squaring and the other powers as products formed next to
+// their use, so that only the four bases stay live.  Loads cost more than the
+// products here (each is a scattered L1 access per lane); the squarings add
+// at most a few float32 ulps to w^8.
 template <int R, bool INV> __device__ __forceinline__ void apply_twiddles(float2* v, int base) {
   if constexpr (R >= 2) {
     const float2 w1 = g_tw[base];
     v[1] = twmul<INV>(v[1], w1);
     if constexpr (R >= 4) {
-      const float2 w2 = g_tw[2 * base];
+      const float2 w2 = cmul(w1, w1);
       v[2] = twmul<INV>(v[2], w2);
       const float2 w3 = cmul(w1, w2);
       v[3] = twmul<INV>(v[3], w3);
       if constexpr (R >= 8) {
-        const float2 w4 = g_tw[4 * base];
+        const float2 w4 = cmul(w2, w2);
         v[4] = twmul<INV>(v[4], w4);
         v[5] = twmul<INV>(v[5], cmul(w1, w4));
         v[6] = twmul<INV>(v[6], cmul(w2, w4));
         const float2 w7 = cmul(w3, w4);
         v[7] = twmul<INV>(v[7], w7);
         if constexpr (R >= 16) {
-          const float2 w8 = g_tw[8 * base];
+          const float2 w8 = cmul(w4, w4);
           v[8] = twmul<INV>(v[8], w8);
           v[9] = twmul<INV>(v[9], cmul(w1, w8));
           v[10] = twmul<INV>(v[10], cmul(w2, w8));
