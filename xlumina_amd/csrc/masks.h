@@ -35,9 +35,16 @@ __device__ __forceinline__ float2 c_mul(float2 a, float2 b) {
 }
 __device__ __forceinline__ float2 c_add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 c_scale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// sin and cos of an angle in radians through sincospif: the angle becomes
+// half-turns in float32 (one rounding of x / pi), which avoids the
+// large-argument reduction path of sincosf.  For |x| <= 2 pi the added phase
+// error is below 1e-6 rad.
+__device__ __forceinline__ void sincos_rad(float x, float* s, float* c) {
+  sincospif(x * 0.31830988618379067f, s, c);
+}
 __device__ __forceinline__ float2 c_expi(float p) {
   float s, c;
-  sincosf(p, &s, &c);
+  sincos_rad(p, &s, &c);
   return make_float2(c, s);
 }
 
@@ -46,8 +53,8 @@ __device__ __forceinline__ float2 c_expi(float p) {
 //   J11 = e^{-i eta/2} s^2 + e^{i eta/2} c^2.
 __device__ __forceinline__ void lcd_jones(float eta, float theta, float2& j00, float2& j01, float2& j11) {
   float sh, ch, st, ct;
-  sincosf(0.5f * eta, &sh, &ch);
-  sincosf(theta, &st, &ct);
+  sincos_rad(0.5f * eta, &sh, &ch);
+  sincos_rad(theta, &st, &ct);
   const float c2 = ct * ct, s2 = st * st, sc = st * ct;
   j00 = make_float2(ch * (c2 + s2), -sh * c2 + sh * s2);
   j11 = make_float2(ch * (s2 + c2), -sh * s2 + sh * c2);
@@ -76,7 +83,7 @@ __device__ __forceinline__ void apply_stage(const MaskStage& m, int64_t pix, flo
     }
     case MASK_LP: {
       float s, c;
-      sincosf(m.m0[pix], &s, &c);
+      sincos_rad(m.m0[pix], &s, &c);
       const float2 a = e0, b = e1;
       e0 = c_add(c_scale(a, c * c), c_scale(b, c * s));
       e1 = c_add(c_scale(a, c * s), c_scale(b, s * s));

@@ -53,7 +53,7 @@ __device__ __forceinline__ void hna_map(double X, double Y, double radius, doubl
   const double rho = sqrt(X * X + Y * Y);
   const float cp = (float)(X / rho), sp = (float)(Y / rho);
   float st, ct;
-  sincosf((float)(rho / f), &st, &ct);
+  sincos_rad((float)(rho / f), &st, &ct);
   const double u2 = (X * X + Y * Y) / (radius * radius);
   const float G = u2 < 1.0 ? (float)(1.0 / sqrt(fabs(1.0 - u2 * stm2))) : 0.f;
   const float w = sqrtf(fabsf(ct)) * G;
@@ -117,8 +117,8 @@ __device__ __forceinline__ void stage_backward(const MaskStage& m, int64_t pix, 
       const float eta = m.kind == MASK_LCD ? m.m0[pix] : (float)m.s0;
       const float th = m.kind == MASK_LCD ? m.m1[pix] : (float)m.s1;
       float sh, ch, st, ct;
-      sincosf(0.5f * eta, &sh, &ch);
-      sincosf(th, &st, &ct);
+      sincos_rad(0.5f * eta, &sh, &ch);
+      sincos_rad(th, &st, &ct);
       const float c2 = ct * ct, s2 = st * st, sc = st * ct;
       const float2 j00 = make_float2(ch * (c2 + s2), -sh * c2 + sh * s2);
       const float2 j11 = make_float2(ch * (s2 + c2), -sh * s2 + sh * c2);
@@ -145,7 +145,7 @@ __device__ __forceinline__ void stage_backward(const MaskStage& m, int64_t pix, 
     }
     case MASK_LP: {
       float s, c;
-      sincosf(m.m0[pix], &s, &c);
+      sincos_rad(m.m0[pix], &s, &c);
       if (gp0) {
         const float d00 = -2.f * s * c, d01 = c * c - s * s, d11 = 2.f * s * c;
         gp0[pix] = re_conj_mul(g0, c_add(c_scale(e0, d00), c_scale(e1, d01))) +
