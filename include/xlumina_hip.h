@@ -74,6 +74,9 @@ typedef struct xl_rs_params {
   xl_mask_stage mask[2];
   const double* z_dev;     /* optional device copy of z read by the kernels instead of `z`
                               (lets a captured HIP graph replay with a new distance) */
+  int32_t kernel_dz;       /* 1: convolve with dH/dz instead of H (derivative of the
+                              propagation with respect to z, for gradients in z) */
+  int32_t reserved2;
 } xl_rs_params;
 
 typedef struct xl_czt_params {

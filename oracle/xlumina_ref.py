@@ -96,6 +96,32 @@ def transfer_function_RS(z, Xext, Yext, k):
     return np.exp(-1j * k * r) * factor
 
 
+def transfer_function_RS_dz(z, Xext, Yext, k):
+    """d/dz of transfer_function_RS (the derivative JAX takes through
+    wave_optics.py:291-297 when z is a parameter), from the product rule with
+    dr/dz = z/r and s = +1 (z > 0) or -1:
+        dH/dz = (1/2pi) e^{i s k r} [ (r^2 - 2 z^2)/r^4 (1/r - i k) - z^2/r^5
+                                      + s (z^2/r^3)(1/r - i k)(i k) ].
+    """
+    r = np.sqrt(Xext ** 2 + Yext ** 2 + z ** 2)
+    s = 1.0 if z > 0 else -1.0
+    b = 1 / r - 1j * k
+    coef = (r ** 2 - 2 * z ** 2) / r ** 4 * b - z ** 2 / r ** 5 + s * (z ** 2 / r ** 3) * b * (1j * k)
+    return np.exp(1j * s * k * r) * coef / (2 * np.pi)
+
+
+def rs_propagation_dz(field, x, y, wavelength, z, dtype=np.complex128):
+    """d/dz of the RS propagation of a fixed field: the same zero-padded
+    convolution (wave_optics.py:281-289) with dH/dz in place of H."""
+    nx, ny, dx, dy, Xext, Yext = build_grid(np.asarray(x), np.asarray(y))
+    k = 2 * np.pi / wavelength
+    H = transfer_function_RS_dz(z, Xext, Yext, k).astype(dtype)
+    U = np.zeros((2 * ny - 1, 2 * nx - 1), dtype=dtype)
+    U[0:ny, 0:nx] = np.asarray(field, dtype=dtype)
+    out = _ifft2(_fft2(U) * _fft2(H)) * dx * dy
+    return out[ny - 1:, nx - 1:]
+
+
 def RS_propagation_jit(input_field, z, nx, ny, dx, dy, Xext, Yext, k, dtype=np.complex128):
     """wave_optics.py:281-289: zero-pad to (2ny-1, 2nx-1), fft2 product, slice."""
     H = transfer_function_RS(z, Xext, Yext, k).astype(dtype)

@@ -172,3 +172,70 @@ def test_rs_slm_grad_vs_fd(cuda_dev):
         p[i, j] += 1e-3
         fd.append((lp - lm) / 2e-3)
     assert rel_l2(g.cpu().numpy().ravel()[pix], np.array(fd)) < 2e-3
+
+
+def test_rs_kernel_dz_propagation_vs_oracle(cuda_dev):
+    """The dH/dz convolution (kernel_dz) against the float64 oracle's."""
+    n = 96
+    x, y = R.space(150.0, n)
+    rng = np.random.default_rng(21)
+    u = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    for z in (800.0, -600.0):
+        got = ops.rs_propagate(torch.from_numpy(u.astype(np.complex64))[None].to(cuda_dev), x, y, z, 0.65,
+                               kernel_dz=True)[0].cpu().numpy()
+        ref = R.rs_propagation_dz(u, x, y, 0.65, z)
+        assert rel_l2(got, ref) <= 1e-5, z
+
+
+def test_rs_grad_wrt_z_vs_fd(cuda_dev):
+    """d loss / d z through ScalarLight-style RS with a fused SLM, against a
+    central difference of the float64 oracle loss in z."""
+    n = 64
+    x, y = R.space(100.0, n)
+    rng = np.random.default_rng(22)
+    u = R.gaussian_beam(x, y, 0.65, (40.0, 40.0), 1.0)
+    phi = rng.uniform(-np.pi, np.pi, (n, n))
+    tgt = rng.random((n, n))
+    field = torch.from_numpy(u.astype(np.complex64))[None].to(cuda_dev)
+    z0 = 700.0
+    z = torch.tensor(z0, dtype=torch.float64, device=cuda_dev, requires_grad=True)
+    out = AG.rs(field, x, y, z, 0.65, masks=[MaskSpec(N.XL_MASK_SLM, torch.from_numpy(phi).float().to(cuda_dev))])
+    L = ((out[0].abs() ** 2 - torch.from_numpy(tgt).float().to(cuda_dev)) ** 2).mean()
+    (gz,) = torch.autograd.grad(L, z)
+
+    def oracle_loss(zz):
+        o, _ = R.rs_propagation(u * np.exp(1j * phi), x, y, 0.65, zz)
+        return float(np.mean((np.abs(o) ** 2 - tgt) ** 2))
+
+    eps = 1e-3
+    fd = (oracle_loss(z0 + eps) - oracle_loss(z0 - eps)) / (2 * eps)
+    assert abs(float(gz) - fd) <= 2e-3 * abs(fd)
+
+
+def test_vrs_grad_wrt_z_vs_fd(cuda_dev):
+    """VRS with an LCD and the constructed Ez (whose z-dependence enters through
+    r = sqrt(X^2 + Y^2 + z^2)) against a central difference of the oracle."""
+    n = 64
+    x, y = R.space(100.0, n)
+    rng = np.random.default_rng(23)
+    ex, ey, _ = R.polarized_gaussian_beam(x, y, 0.65, (40.0, 40.0), (1.0, 1j))
+    eta, th = rng.uniform(0, 2 * np.pi, (2, n, n))
+    w = rng.random((n, n))
+    src = torch.from_numpy(np.stack([ex, ey]).astype(np.complex64)).to(cuda_dev)
+    masks = [MaskSpec(N.XL_MASK_LCD, torch.from_numpy(eta).float().to(cuda_dev),
+                      torch.from_numpy(th).float().to(cuda_dev))]
+    z0 = 600.0
+    z = torch.tensor(z0, dtype=torch.float64, device=cuda_dev, requires_grad=True)
+    out = AG.rs(src, x, y, z, 0.65, masks=masks, ez_mode=N.XL_EZ_VRS, nin=2)
+    wt = torch.from_numpy(w).float().to(cuda_dev)
+    L = (wt * (out.abs() ** 2).sum(0)).sum()
+    (gz,) = torch.autograd.grad(L, z)
+
+    def oracle_loss(zz):
+        lx, ly, _ = R.LCD(ex, ey, None, eta, th)
+        (vx, vy, vz), _ = R.vrs_propagation(lx, ly, x, y, 0.65, zz)
+        return float(np.sum(w * (np.abs(vx) ** 2 + np.abs(vy) ** 2 + np.abs(vz) ** 2)))
+
+    eps = 1e-3
+    fd = (oracle_loss(z0 + eps) - oracle_loss(z0 - eps)) / (2 * eps)
+    assert abs(float(gz) - fd) <= 2e-3 * abs(fd)

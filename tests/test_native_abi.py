@@ -50,6 +50,7 @@ int main(void) {
          offsetof(xl_pre_params, z_dev), offsetof(xl_pre_params, gparam_batch_stride), sizeof(xl_mse_params));
   printf("%zu %zu %zu %zu\n", offsetof(xl_mse_params, scale), sizeof(xl_adam_params), offsetof(xl_adam_params, count),
          offsetof(xl_adam_params, weight_decay));
+  printf("%zu\n", offsetof(xl_rs_params, kernel_dz));
   return 0;
 }
 '''
@@ -70,10 +71,11 @@ int main(void) {
                                                   C.sizeof(N.MseParams)]
     assert [int(v) for v in lines[4].split()] == [N.MseParams.scale.offset, C.sizeof(N.AdamParams),
                                                   N.AdamParams.count.offset, N.AdamParams.weight_decay.offset]
+    assert int(lines[5]) == N.RsParams.kernel_dz.offset
 
 
 def test_host_side_queries(lib):
-    assert lib.xl_abi_version() == 2
+    assert lib.xl_abi_version() == 3
     assert lib.xl_max_fft_length() == 4096
     # RS workspace: ncomp * round4(n) * P + (P/2) * P + P complex64 (+ alignment slack)
     n, P = 1024, 2048

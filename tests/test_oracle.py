@@ -300,3 +300,30 @@ def test_golden_vectors_reproduce(golden):
             mx, my, _ = R.LCD(c["Ex"], c["Ey"], None, c["eta"].astype(np.float64), c["theta"].astype(np.float64))
             (px, py, pz), _ = R.vrs_propagation(mx, my, c["x"], c["y"], float(c["wavelength"]), float(c["z"]))
             assert rel_l2(np.stack([px, py, pz]), c["out"]) < 1e-12
+
+
+def test_rs_kernel_z_derivative_vs_central_difference():
+    """dH/dz (the derivative the reference's jax.grad takes through
+    wave_optics.py:291-297 when z is a parameter) against a central
+    difference of H in z, both signs of z."""
+    X = np.array([0.0, 13.7, -250.0, 1200.0, 3.3])
+    Y = np.array([5.0, -40.0, 300.0, 0.1, -700.0])
+    k = 2 * np.pi / 0.65
+    eps = 1e-4
+    for z in (3e4, 731.0, 50.0, -500.0):
+        d = R.transfer_function_RS_dz(z, X, Y, k)
+        fd = (R.transfer_function_RS(z + eps, X, Y, k) - R.transfer_function_RS(z - eps, X, Y, k)) / (2 * eps)
+        assert np.max(np.abs(fd - d) / np.abs(d)) < 1e-6
+
+
+def test_rs_propagation_z_derivative_is_linear_in_h():
+    """The propagation derivative is the convolution with dH/dz: it equals a
+    central difference of the propagated field in z."""
+    n = 24
+    x, y = R.space(60.0, n)
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    z, eps = 900.0, 1e-4
+    d = R.rs_propagation_dz(u, x, y, 0.65, z)
+    fd = (R.rs_propagation(u, x, y, 0.65, z + eps)[0] - R.rs_propagation(u, x, y, 0.65, z - eps)[0]) / (2 * eps)
+    assert np.linalg.norm(fd - d) / np.linalg.norm(d) < 1e-6

@@ -35,7 +35,7 @@ class RsParams(C.Structure):
                 ("out", C.c_void_p), ("out_comp_stride", C.c_int64), ("intensity", C.c_void_p),
                 ("n", C.c_int32), ("adjoint", C.c_int32), ("x", C.c_void_p), ("y", C.c_void_p),
                 ("z", C.c_double), ("k", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
-                ("mask", MaskStage * 2), ("z_dev", C.c_void_p)]
+                ("mask", MaskStage * 2), ("z_dev", C.c_void_p), ("kernel_dz", C.c_int32), ("reserved2", C.c_int32)]
 
 
 class CztParams(C.Structure):

@@ -74,7 +74,7 @@ class RSConfig:
 
 class _RSFunction(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, cfg: RSConfig, src, p00, p01, p10, p11):
+    def forward(ctx, cfg: RSConfig, src, p00, p01, p10, p11, z_t):
         params = (p00, p01, p10, p11)
         masks = _rebuild(cfg.mask_struct, params)
         out = ops.rs_propagate(src, cfg.x, cfg.y, cfg.z, cfg.wavelength, masks=masks, ez_mode=cfg.ez_mode,
@@ -82,6 +82,7 @@ class _RSFunction(torch.autograd.Function):
         ctx.cfg = cfg
         ctx.save_for_backward(src, *[p if p is not None else torch.empty(0) for p in params])
         ctx.has = [p is not None for p in params]
+        ctx.z_meta = None if z_t is None else (z_t.shape, z_t.dtype, z_t.device)
         return out
 
     @staticmethod
@@ -90,10 +91,15 @@ class _RSFunction(torch.autograd.Function):
         saved = ctx.saved_tensors
         src = saved[0]
         params = [saved[1 + i] if ctx.has[i] else None for i in range(4)]
-        g_pre = ops.rs_propagate(gout.contiguous(), cfg.x, cfg.y, cfg.z, cfg.wavelength, adjoint=True)
+        gout = gout.contiguous()
+        g_pre = ops.rs_propagate(gout, cfg.x, cfg.y, cfg.z, cfg.wavelength, adjoint=True)
         masks = _rebuild(cfg.mask_struct, params)
         need_params = ((ctx.needs_input_grad[2] and params[0] is not None, ctx.needs_input_grad[3] and params[1] is not None),
                        (ctx.needs_input_grad[4] and params[2] is not None, ctx.needs_input_grad[5] and params[3] is not None))
+        g_z = None
+        if ctx.needs_input_grad[6]:
+            shape, dtype, device = ctx.z_meta
+            g_z = _rs_z_grad(cfg, src, masks, gout, g_pre).to(device=device, dtype=dtype).reshape(shape)
         if not masks and cfg.ez_mode == N.XL_EZ_NONE:
             g_src, gp = g_pre, [[None, None], [None, None]]
         else:
@@ -106,15 +112,38 @@ class _RSFunction(torch.autograd.Function):
                 g_src = full
         else:
             g_src = None
-        return (None, g_src, *_param_grads(ctx, cfg.mask_struct, params, gp))
+        return (None, g_src, *_param_grads(ctx, cfg.mask_struct, params, gp), g_z)
 
 
-def rs(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Sequence[MaskSpec] = (),
+def _rs_z_grad(cfg: RSConfig, src, masks, gout, g_pre):
+    """dL/dz = Re<g, d out/dz>: the propagation of the pre-op'd field with
+    dH/dz, plus (VRS) the z-dependence of the constructed Ez_in =
+    (Ex X + Ey Y)/r, r = sqrt(X^2 + Y^2 + z^2) (vectorized_optics.py:262-267),
+    through its cotangent g_pre[2]."""
+    d = ops.rs_propagate(src, cfg.x, cfg.y, cfg.z, cfg.wavelength, masks=masks, ez_mode=cfg.ez_mode, nin=cfg.nin,
+                         kernel_dz=True)
+    g = torch.real(torch.vdot(d.reshape(-1).to(torch.complex128), gout.reshape(-1).to(torch.complex128)))
+    if cfg.ez_mode == N.XL_EZ_VRS:
+        dev = src.device
+        pre = ops.pre_forward(src, cfg.x, cfg.y, cfg.z, masks=masks, ncomp=2, nin=cfg.nin) if masks else src[:2]
+        X = ops.device_grid(cfg.x, dev)[None, :]
+        Y = ops.device_grid(cfg.y, dev)[:, None]
+        r = torch.sqrt(X * X + Y * Y + cfg.z * cfg.z)
+        dez = -(pre[0].to(torch.complex128) * X + pre[1].to(torch.complex128) * Y) * (cfg.z / r ** 3)
+        g = g + torch.real(torch.vdot(dez.reshape(-1), g_pre[2].reshape(-1).to(torch.complex128)))
+    return g
+
+
+def rs(src: torch.Tensor, x, y, z, wavelength: float, masks: Sequence[MaskSpec] = (),
        ez_mode: int = N.XL_EZ_NONE, nin: Optional[int] = None) -> torch.Tensor:
-    """Differentiable RS / VRS propagation of src (C, n, n)."""
+    """Differentiable RS / VRS propagation of src (C, n, n).  `z` may be a
+    float or a one-element tensor; with requires_grad the distance receives
+    dL/dz (convolution with the analytic dH/dz)."""
     nin = src.shape[0] if nin is None else nin
-    cfg = RSConfig(x, y, float(z), float(wavelength), ez_mode, nin, _mask_structure(masks))
-    return _RSFunction.apply(cfg, src, *_flat_params(masks))
+    z_t = z if isinstance(z, torch.Tensor) else None
+    zf = float(z.detach().reshape(-1)[0]) if z_t is not None else float(z)
+    cfg = RSConfig(x, y, zf, float(wavelength), ez_mode, nin, _mask_structure(masks))
+    return _RSFunction.apply(cfg, src, *_flat_params(masks), z_t)
 
 
 @dataclass

@@ -44,6 +44,7 @@ struct RsArgs {
   const double* z_dev;    // if set, z is read from device memory
   float scale;            // dx*dy/P^2
   int n, nin, ncomp, ez_mode, adjoint;
+  int kernel_dz;          // 1: convolve with dH/dz instead of H
   MaskArgs mask;
 };
 
@@ -63,6 +64,22 @@ __device__ __forceinline__ float2 rs_kernel_value(double X, double Y, double z, 
   const float2 e = expi_turns(z > 0 ? turns : -turns);
   const double f = 0.15915494309189535 * z * ir * ir;   // (1/2pi) z / r^2
   const float fr = (float)(f * ir), fi = (float)(-f * k);   // f * (1/r - i k)
+  return make_float2(fr * e.x - fi * e.y, fr * e.y + fi * e.x);
+}
+
+// dH/dz of rs_kernel_value (product rule, dr/dz = z/r, s = sign of the phase):
+//   (1/2pi) e^{i s k r} [ (r^2 - 3z^2)/r^5 + s k^2 z^2/r^3 + i k (s z^2 - (r^2 - 2 z^2))/r^4 ].
+__device__ __forceinline__ float2 rs_kernel_dz_value(double X, double Y, double z, double k) {
+  const double r2 = X * X + Y * Y + z * z;
+  const double r = sqrt(r2);
+  const double ir = 1.0 / r, ir2 = ir * ir;
+  const double s = z > 0 ? 1.0 : -1.0;
+  const double turns = k * r * 0.15915494309189535;
+  const float2 e = expi_turns(s * turns);
+  const double z2 = z * z;
+  const double cr = (r2 - 3.0 * z2) * ir2 * ir2 * ir + s * k * k * z2 * ir2 * ir;
+  const double ci = k * (s * z2 - (r2 - 2.0 * z2)) * ir2 * ir2;
+  const float fr = (float)(0.15915494309189535 * cr), fi = (float)(0.15915494309189535 * ci);
   return make_float2(fr * e.x - fi * e.y, fr * e.y + fi * e.x);
 }
 
@@ -95,7 +112,8 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_hrow_kernel(Rs
   for (int c = t; c <= P / 2; c += TPF) {
     double dx;
     float2 v = make_float2(0.f, 0.f);
-    if (rowok && lag_of(a.x, c, a.n, P, &dx)) v = rs_kernel_value(dx, dy, z, a.k);
+    if (rowok && lag_of(a.x, c, a.n, P, &dx))
+      v = a.kernel_dz ? rs_kernel_dz_value(dx, dy, z, a.k) : rs_kernel_value(dx, dy, z, a.k);
     s[lpad(c)] = v;
     if (c > 0 && c < P / 2) s[lpad(P - c)] = v;
   }

@@ -187,7 +187,7 @@ char* align256(char* p) { return (char*)(((uintptr_t)p + 255) & ~(uintptr_t)255)
 extern "C" {
 
 const char* xl_last_error(void) { return g_err.c_str(); }
-int xl_abi_version(void) { return 2; }
+int xl_abi_version(void) { return 3; }
 int xl_max_fft_length(void) { return xl::MAX_FFT; }
 int xl_init(void) { return ensure_init(); }
 
@@ -229,6 +229,7 @@ int xl_rs_propagate(const xl_rs_params* p, void* ws, size_t ws_bytes, void* stre
   a.z = p->z;
   a.k = p->k;
   a.z_dev = p->z_dev;
+  a.kernel_dz = p->kernel_dz ? 1 : 0;
   a.n = n;
   a.nin = p->nin;
   a.ncomp = ncomp;

@@ -175,7 +175,7 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
                  ez_mode: int = N.XL_EZ_NONE, nin: Optional[int] = None, intensity: bool = False,
                  adjoint: bool = False, out: Optional[torch.Tensor] = None,
                  workspace: Optional[torch.Tensor] = None, z_dev: Optional[torch.Tensor] = None,
-                 inten: Optional[torch.Tensor] = None):
+                 inten: Optional[torch.Tensor] = None, kernel_dz: bool = False):
     """RS_propagation_jit / VRS_propagation_jit (wave_optics.py:281-289,
     vectorized_optics.py:364-373) with fused masks and Ez construction.
 
@@ -183,6 +183,8 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
     Returns the (ncomp, n, n) field, plus |E|^2 if `intensity`.
     z_dev: optional float64 device scalar read by the kernels instead of `z`
     (a captured graph then replays with whatever distance it holds).
+    kernel_dz: convolve with dH/dz instead of H (the z-derivative of the
+    propagation of the pre-op'd field).
     """
     lib = N.load()
     src = _prep_field(src)
@@ -221,6 +223,7 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
     p.dx = float(xa[1] - xa[0])
     p.dy = float(ya[1] - ya[0])
     p.z_dev = _z_ptr(z_dev, dev)
+    p.kernel_dz = 1 if kernel_dz else 0
     keep = fill_masks(p.mask, masks, (n, n), dev, nin)
     N.check(lib.xl_rs_propagate(C.byref(p), workspace.data_ptr(), workspace.numel(), _stream(dev)))
     del keep

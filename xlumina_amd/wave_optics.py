@@ -119,10 +119,10 @@ class ScalarLight(_GridMixin):
         """wave_optics.py:173-196: Rayleigh-Sommerfeld to distance z -> (light, quality factor)."""
         tic = time.perf_counter()
         src, masks = self._pending()
-        out = AG.rs(src, self.x, self.y, float(z), self.wavelength, masks=masks)
+        out = AG.rs(src, self.x, self.y, z, self.wavelength, masks=masks)
         light = ScalarLight(self.x, self.y, self.wavelength, self.device)
         light._field = out[0]
-        qf = ops.quality_factor(self.x, self.y, self.wavelength, float(z))
+        qf = ops.quality_factor(self.x, self.y, self.wavelength, _zf(z))
         if VERBOSE:
             print(f"Time taken to perform one RS propagation (in seconds): {(time.perf_counter() - tic):.4f}")
         return light, qf
@@ -146,6 +146,10 @@ class ScalarLight(_GridMixin):
         if VERBOSE:
             print(f"Time taken to perform one CZT propagation (in seconds): {(time.perf_counter() - tic):.4f}")
         return light
+
+
+def _zf(z) -> float:
+    return float(z.detach().reshape(-1)[0]) if isinstance(z, torch.Tensor) else float(z)
 
 
 def minimum_z(x, y, wavelength, n=1, quality_factor=1) -> float:
