@@ -139,6 +139,27 @@ def test_czt_vs_oracle(n, mx, my, cuda_dev):
     assert rel_l2(H(out[0]), ref) < TOL
 
 
+@pytest.mark.parametrize("n,mx,my", [(1100, 200, 300), (700, 60, 100), (2000, 600, 700)])
+def test_czt_three_smooth_lengths(n, mx, my, cuda_dev):
+    """Bluestein lengths 3 * 2^k (768, 1536, 3072; chosen when shorter than the
+    next power of two) give the same operator as the oracle's power-of-two
+    length, forward and through the adjoint identity."""
+    rng = np.random.default_rng(n)
+    x, y = R.space(1200.0, n)
+    u = cfield(rng, n)
+    xo = np.linspace(-100, 100, mx)
+    yo = np.linspace(-80, 120, my)
+    g = ops.czt_geometry(x, y, xo, yo, 3e4, 0.635)
+    ref = R.czt(u, x, y, 0.635, 3e4, xo, yo)
+    out = ops.czt(T(u, cuda_dev), g, form="bluestein")
+    assert rel_l2(H(out[0]), ref) < TOL
+    c = torch.randn((1, my, mx), dtype=torch.complex64, device=cuda_dev)
+    back = ops.czt(c, g, adjoint=True, form="bluestein")
+    lhs = complex(torch.vdot(c.flatten().to(torch.complex128), out.flatten().to(torch.complex128)))
+    rhs = complex(torch.vdot(back.flatten().to(torch.complex128), T(u, cuda_dev).flatten().to(torch.complex128)))
+    assert abs(lhs - rhs) / abs(lhs) < 1e-5
+
+
 def test_czt_rejections(cuda_dev):
     x, y = R.space(100.0, 64)
     with pytest.raises(ValueError, match="power of two"):

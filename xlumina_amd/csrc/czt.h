@@ -95,12 +95,12 @@ struct CztSetupPair {
   CztSetupArgs ax[2];
 };
 
-template <int Q>
-__global__ __launch_bounds__(Q / 16) void czt_setup_kernel(CztSetupPair pair) {
-  const CztSetupArgs& a = pair.ax[blockIdx.x];
+// Tables of one axis on the first Q/16 threads of the block (GUARD: the block
+// is wider; the other threads only join the FFT's barriers).
+template <int Q, bool GUARD>
+__device__ __forceinline__ void czt_setup_body(const CztSetupArgs& a, float2* s, int t) {
   constexpr int TPF = Q / 16;
-  __shared__ float2 s[LdsLen<Q>::value];
-  const int t = threadIdx.x;
+  const bool act = !GUARD || t < TPF;
   const double th = a.th_turns;
   const double shift = 0.5 - 0.5 * a.m;
   // forward tables
@@ -118,22 +118,40 @@ __global__ __launch_bounds__(Q / 16) void czt_setup_kernel(CztSetupPair pair) {
     const double u = (double)(sidx - a.m + 1);
     return expi_turns(-th * 0.5 * u * u);
   };
-  if (!a.adjoint) {
-    for (int j = t; j < a.m; j += TPF) a.pre[j] = pre_f(j);
-    for (int k = t; k < a.M; k += TPF) a.post[k] = post_f(k);
-    for (int i = t; i < Q; i += TPF) s[lpad(i)] = g_f(i);
-  } else {
-    for (int k = t; k < a.M; k += TPF) a.pre[k] = conjf2(post_f(k));
-    for (int j = t; j < a.m; j += TPF) a.post[j] = conjf2(pre_f(j));
-    for (int i = t; i < Q; i += TPF) {
-      const int d = i < a.m ? i : i - Q;   // lag d in [-(Q-m), m)
-      s[lpad(i)] = (d > -a.M) ? conjf2(g_f(a.m - d)) : make_float2(0.f, 0.f);
+  if (act) {
+    if (!a.adjoint) {
+      for (int j = t; j < a.m; j += TPF) a.pre[j] = pre_f(j);
+      for (int k = t; k < a.M; k += TPF) a.post[k] = post_f(k);
+      for (int i = t; i < Q; i += TPF) s[lpad(i)] = g_f(i);
+    } else {
+      for (int k = t; k < a.M; k += TPF) a.pre[k] = conjf2(post_f(k));
+      for (int j = t; j < a.m; j += TPF) a.post[j] = conjf2(pre_f(j));
+      for (int i = t; i < Q; i += TPF) {
+        const int d = i < a.m ? i : i - Q;   // lag d in [-(Q-m), m)
+        s[lpad(i)] = (d > -a.M) ? conjf2(g_f(a.m - d)) : make_float2(0.f, 0.f);
+      }
     }
   }
   __syncthreads();
-  fft_lds<Q, TPF, false>(s, t);
+  fft_lds<Q, TPF, false, 0, GUARD>(s, t);
   const float iq = 1.0f / (float)Q;
-  for (int i = t; i < Q; i += TPF) a.gf[i] = s[lpad(i)] * iq;
+  if (act)
+    for (int i = t; i < Q; i += TPF) a.gf[i] = s[lpad(i)] * iq;
+}
+
+template <int Q>
+__global__ __launch_bounds__(Q / 16) void czt_setup_kernel(CztSetupPair pair) {
+  __shared__ float2 s[LdsLen<Q>::value];
+  czt_setup_body<Q, false>(pair.ax[blockIdx.x], s, threadIdx.x);
+}
+
+// Both axes in one launch when their lengths differ (block 0: QA, block 1: QB).
+template <int QA, int QB>
+__global__ __launch_bounds__((QA > QB ? QA : QB) / 16) void czt_setup2_kernel(CztSetupPair pair) {
+  constexpr int QM = QA > QB ? QA : QB;
+  __shared__ float2 s[LdsLen<QM>::value];
+  if (blockIdx.x == 0) czt_setup_body<QA, (QA < QM)>(pair.ax[0], s, threadIdx.x);
+  else czt_setup_body<QB, (QB < QM)>(pair.ax[1], s, threadIdx.x);
 }
 
 // Pre-op families the row kernel is specialised for.
@@ -250,7 +268,7 @@ __global__ __launch_bounds__(3 * Q / 16) void czt_rows_kernel(CztArgs a0) {
 // columns: the CPB groups of a block (and at CPB = 2 the neighbouring block,
 // kept on the same XCD) share it.
 template <int Q> struct CztColsShape {
-  static constexpr int CPB = Q >= 4096 ? 2 : 4;   // columns per block (more blocks at 4096)
+  static constexpr int CPB = Q >= 3072 ? 2 : 4;   // columns per block (more blocks at 3072 and 4096)
 };
 
 template <int Q>
@@ -324,6 +342,12 @@ template <int Q>
 inline void czt_setup_launch(const CztSetupPair& p, int nblocks, hipStream_t st) {
   KTimer t(K_CZT_SETUP, st);
   hipLaunchKernelGGL(czt_setup_kernel<Q>, dim3(nblocks), dim3(Q / 16), 0, st, p);
+}
+template <int QA, int QB>
+inline void czt_setup2_launch(const CztSetupPair& p, hipStream_t st) {
+  KTimer t(K_CZT_SETUP, st);
+  constexpr int QM = QA > QB ? QA : QB;
+  hipLaunchKernelGGL((czt_setup2_kernel<QA, QB>), dim3(2), dim3(QM / 16), 0, st, p);
 }
 
 }  // namespace xl

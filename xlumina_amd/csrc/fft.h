@@ -18,9 +18,11 @@
 namespace xl {
 
 constexpr int TWN = 4096;                 // twiddle table length == max FFT length
+constexpr int TWN3 = 3072;                // table for the lengths 3 * 2^k
 constexpr int MAX_FFT = 4096;
 
 __device__ float2 g_tw[TWN];              // e^{-2 pi i k / TWN}, float64-generated
+__device__ float2 g_tw3[TWN3];            // e^{-2 pi i k / TWN3}, float64-generated
 
 __global__ void xl_fft_init_kernel() {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -29,7 +31,50 @@ __global__ void xl_fft_init_kernel() {
     sincospi(-2.0 * (double)k / (double)TWN, &s, &c);
     g_tw[k] = make_float2((float)c, (float)s);
   }
+  if (k < TWN3) {
+    double s, c;
+    sincospi(-2.0 * (double)k / (double)TWN3, &s, &c);
+    g_tw3[k] = make_float2((float)c, (float)s);
+  }
 }
+
+// Lengths are 2^k (64..4096) or 3 * 2^k (768..3072); the latter start with one
+// radix-3 pass and take their twiddles from the 3072-entry table.
+template <int L> struct Tw {
+  static constexpr bool TRI = L % 3 == 0;
+  static constexpr int N = TRI ? TWN3 : TWN;
+  __device__ static float2 at(int i) { return TRI ? g_tw3[i] : g_tw[i]; }
+};
+
+constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x / 2); }
+
+// Pass sequence of an L-point FFT: forward [3,] 16, ..., 16, REM; the inverse
+// runs the same radices in reverse order, so the last forward pass and the
+// first inverse pass share one register layout.
+template <int L> struct FftPlan {
+  static constexpr bool TRI = L % 3 == 0;
+  static constexpr int L2 = TRI ? L / 3 : L;
+  static constexpr int LG = ilog2c(L2);
+  static constexpr int N16 = LG / 4;
+  static constexpr int REM = 1 << (LG % 4);
+  static constexpr int NP = (TRI ? 1 : 0) + N16 + (REM > 1 ? 1 : 0);
+  static constexpr int fwd_radix(int i) {
+    if (TRI) {
+      if (i == 0) return 3;
+      i -= 1;
+    }
+    return i < N16 ? 16 : REM;
+  }
+  static constexpr int radix(bool fwd, int i) { return fwd ? fwd_radix(i) : fwd_radix(NP - 1 - i); }
+  static constexpr int ns(bool fwd, int i) {
+    int p = 1;
+    for (int k = 0; k < i; ++k) p *= radix(fwd, k);
+    return p;
+  }
+  // register layout at the forward/inverse hand-off
+  static constexpr int RH = radix(true, NP - 1);
+  static constexpr int NBH = L / RH;
+};
 
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
 template <int L> struct LdsLen { static constexpr int value = L + L / 16; };
@@ -86,15 +131,31 @@ template <int R, bool INV> struct DFT {
 template <bool INV> struct DFT<1, INV> {
   __device__ __forceinline__ static void run(float2*) {}
 };
+// X1,2 = a - (b + c)/2 -+ i (sqrt(3)/2)(b - c) (signs swapped for the inverse).
+template <bool INV> struct DFT<3, INV> {
+  __device__ __forceinline__ static void run(float2* v) {
+    const float2 a = v[0], t1 = v[1] + v[2], d = v[1] - v[2];
+    const float2 t2 = a - t1 * 0.5f;
+    const float h = 0.86602540378443865f;
+    const float2 m = make_float2(-d.y * h, d.x * h);   // i h d
+    v[0] = a + t1;
+    v[1] = INV ? t2 + m : t2 - m;
+    v[2] = INV ? t2 - m : t2 + m;
+  }
+};
 
 // v[r] *= w^r for r < R with w = e^{-+2 pi i base / TWN}: one table load (w);
 // w^2, w^4, w^8 by squaring and the other powers as products formed next to
 // their use, so that only the four bases stay live.  Loads cost more than the
 // products here (each is a scattered L1 access per lane); the squarings add
 // at most a few float32 ulps to w^8.
-template <int R, bool INV> __device__ __forceinline__ void apply_twiddles(float2* v, int base) {
-  if constexpr (R >= 2) {
-    const float2 w1 = g_tw[base];
+template <int R, bool INV, int L> __device__ __forceinline__ void apply_twiddles(float2* v, int base) {
+  if constexpr (R == 3) {
+    const float2 w1 = Tw<L>::at(base);
+    v[1] = twmul<INV>(v[1], w1);
+    v[2] = twmul<INV>(v[2], cmul(w1, w1));
+  } else if constexpr (R >= 2) {
+    const float2 w1 = Tw<L>::at(base);
     v[1] = twmul<INV>(v[1], w1);
     if constexpr (R >= 4) {
       const float2 w2 = cmul(w1, w1);
@@ -125,18 +186,21 @@ template <int R, bool INV> __device__ __forceinline__ void apply_twiddles(float2
 }
 
 // One Stockham radix-R pass of an L-point FFT over TPF threads (thread t).
-template <int L, int R, int NS, int TPF, bool INV>
+// GUARD: the block has more than TPF threads; the others only join the
+// barriers.
+template <int L, int R, int NS, int TPF, bool INV, bool GUARD = false>
 __device__ __forceinline__ void stockham_pass(float2* s, int t) {
   constexpr int NB = L / R;
   constexpr int BPT = (NB + TPF - 1) / TPF;
   float2 v[BPT][R];
+  const bool act = !GUARD || t < TPF;
 #pragma unroll
   for (int b = 0; b < BPT; ++b) {
     const int j = t + b * TPF;
-    if (NB % TPF == 0 || j < NB) {
+    if (act && (NB % TPF == 0 || j < NB)) {
 #pragma unroll
       for (int r = 0; r < R; ++r) v[b][r] = s[lpad(j + r * NB)];
-      if constexpr (NS > 1) apply_twiddles<R, INV>(v[b], (j & (NS - 1)) * (TWN / (NS * R)));
+      if constexpr (NS > 1) apply_twiddles<R, INV, L>(v[b], (j % NS) * (Tw<L>::N / (NS * R)));
       DFT<R, INV>::run(v[b]);
     }
   }
@@ -144,8 +208,8 @@ __device__ __forceinline__ void stockham_pass(float2* s, int t) {
 #pragma unroll
   for (int b = 0; b < BPT; ++b) {
     const int j = t + b * TPF;
-    if (NB % TPF == 0 || j < NB) {
-      const int idxD = (j / NS) * NS * R + (j & (NS - 1));
+    if (act && (NB % TPF == 0 || j < NB)) {
+      const int idxD = (j / NS) * NS * R + (j % NS);
 #pragma unroll
       for (int r = 0; r < R; ++r) s[lpad(idxD + r * NS)] = v[b][r];
     }
@@ -153,13 +217,12 @@ __device__ __forceinline__ void stockham_pass(float2* s, int t) {
   __syncthreads();
 }
 
-template <int L, int TPF, bool INV, int NS = 1>
+template <int L, int TPF, bool INV, int I = 0, bool GUARD = false>
 __device__ __forceinline__ void fft_lds(float2* s, int t) {
-  if constexpr (NS < L) {
-    constexpr int rem = L / NS;
-    constexpr int R = rem >= 16 ? 16 : rem;
-    stockham_pass<L, R, NS, TPF, INV>(s, t);
-    fft_lds<L, TPF, INV, NS * R>(s, t);
+  using PL = FftPlan<L>;
+  if constexpr (I < PL::NP) {
+    stockham_pass<L, PL::radix(true, I), PL::ns(true, I), TPF, INV, GUARD>(s, t);
+    fft_lds<L, TPF, INV, I + 1, GUARD>(s, t);
   }
 }
 

@@ -3,8 +3,8 @@
 // Each pass is the radix-R Stockham step of fft.h, but its source can be LDS,
 // a caller functor (global loads, masks, on-the-fly generation) or registers,
 // and its sink LDS, a caller functor (global stores of only the samples that
-// are needed) or registers.  The forward chain uses radices 16,...,16,REM and
-// the inverse chain REM,16,...,16, so the last forward pass leaves element
+// are needed) or registers.  The forward chain uses radices [3,] 16,...,16,REM
+// and the inverse chain the same in reverse order, so the last forward pass leaves element
 // j + q*(L/R) in register v[b][q] (j = t + b*TPF) and the first inverse pass
 // reads exactly that layout: a frequency-domain product (transfer function,
 // chirp spectrum) is applied in registers between the two chains, with no
@@ -18,26 +18,6 @@
 #include "fft.h"
 
 namespace xl {
-
-constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x / 2); }
-
-template <int L> struct FftPlan {
-  static constexpr int LG = ilog2c(L);
-  static constexpr int N16 = LG / 4;
-  static constexpr int REM = 1 << (LG % 4);
-  static constexpr int NP = N16 + (REM > 1 ? 1 : 0);
-  static constexpr int radix(bool fwd, int i) {
-    return fwd ? (i < N16 ? 16 : REM) : ((REM > 1) ? (i == 0 ? REM : 16) : 16);
-  }
-  static constexpr int ns(bool fwd, int i) {
-    int p = 1;
-    for (int k = 0; k < i; ++k) p *= radix(fwd, k);
-    return p;
-  }
-  // register layout at the forward/inverse hand-off
-  static constexpr int RH = radix(true, NP - 1);
-  static constexpr int NBH = L / RH;
-};
 
 template <int BPT, int R> struct Regs {
   float2 v[BPT][R];
@@ -62,7 +42,7 @@ __device__ __forceinline__ void pass_io(float2* s, int t, Load& load, Store& sto
         else if constexpr (SRC == SRC_FN) v[b][r] = load(j + r * NB);
         else v[b][r] = reg.v[b][r];
       }
-      if constexpr (NS > 1) apply_twiddles<R, INV>(v[b], (j & (NS - 1)) * (TWN / (NS * R)));
+      if constexpr (NS > 1) apply_twiddles<R, INV, L>(v[b], (j % NS) * (Tw<L>::N / (NS * R)));
       DFT<R, INV>::run(v[b]);
     }
   }
@@ -72,7 +52,7 @@ __device__ __forceinline__ void pass_io(float2* s, int t, Load& load, Store& sto
   for (int b = 0; b < BPT; ++b) {
     const int j = t + b * TPF;
     if (NB % TPF == 0 || j < NB) {
-      const int idxD = (j / NS) * NS * R + (j & (NS - 1));
+      const int idxD = (j / NS) * NS * R + (j % NS);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if constexpr (DST == DST_LDS) s[lpad(idxD + r * NS)] = v[b][r];
@@ -89,8 +69,7 @@ struct NoIO {
   __device__ void operator()(int, float2) const {}
 };
 
-// Pass chain i..NP-1 of the forward (FWD = true, radices 16..REM) or inverse
-// (radices REM..16) plan.  The first pass reads SRC, the last writes DST, the
+// Pass chain i..NP-1 of the forward (FWD = true) or inverse plan (FftPlan).  The first pass reads SRC, the last writes DST, the
 // others go through LDS.  Every pass that writes LDS first waits for all
 // threads (so a chain may follow any earlier LDS use); a chain ending in
 // DST_FN / DST_REG leaves LDS free without a final barrier.
