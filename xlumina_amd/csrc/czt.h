@@ -57,6 +57,11 @@ struct CztArgs {
   int64_t in_bs, out_bs, t_bs, mask_bs;   // batch strides (blockIdx.y / .z selects the member)
   int rin4;            // rin rounded up to a multiple of 4
   int64_t t_cs;        // component stride of the intermediate (rin4 * lout4)
+  // Optional tables of the input-plane factor (rin x ax.lin) and the
+  // output-plane factor (ay.lout x ax.lout), shared by a batch: built once by
+  // czt_ftab_kernel instead of per member in float64.
+  const float2* ftab;
+  const float2* f0tab;
 };
 
 // Offset of (row, k) in the tiled intermediate: 4 rows x 4 outputs per 128-byte
@@ -206,10 +211,34 @@ __device__ __forceinline__ void czt_pre_values(const CztArgs& a, int row, int co
 }
 
 // Transfer function of the input plane (or its conjugate times cte for the adjoint).
-__device__ __forceinline__ float2 czt_f_value(const CztArgs& a, int row, int col) {
+__device__ __forceinline__ float2 czt_f_compute(const CztArgs& a, int row, int col) {
   float2 h = rs_kernel_value(a.px[col], a.py[row], a.z, a.k);
   if (a.pre_mode == PRE_CONJ_RS) h = c_mul(conjf2(h), a.cte);
   return h;
+}
+__device__ __forceinline__ float2 czt_f_value(const CztArgs& a, int row, int col) {
+  return a.ftab ? a.ftab[(int64_t)row * a.ax.lin + col] : czt_f_compute(a, row, col);
+}
+
+// Output-plane factor of element (ky, kx): F0 cte (forward), conj(F0)
+// (adjoint), cte, or 1.
+__device__ __forceinline__ float2 czt_post_factor(const CztArgs& a, int ky, int kx) {
+  if (a.post_mode == POST_RS) return c_mul(rs_kernel_value(a.qx[kx], a.qy[ky], a.z, a.k), a.cte);
+  if (a.post_mode == POST_CONJ_RS) return conjf2(rs_kernel_value(a.qx[kx], a.qy[ky], a.z, a.k));
+  if (a.post_mode == POST_CONST) return a.cte;
+  return make_float2(1.f, 0.f);
+}
+
+// Both factor tables of a batched RS-mode CZT (grid-stride).
+__global__ void czt_ftab_kernel(CztArgs a, float2* ftab, float2* f0tab) {
+  const int64_t n1 = (int64_t)a.rin * a.ax.lin, n2 = (int64_t)a.ay.lout * a.ax.lout;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n1 + n2; e += (int64_t)gridDim.x * blockDim.x) {
+    if (e < n1) ftab[e] = czt_f_compute(a, (int)(e / a.ax.lin), (int)(e % a.ax.lin));
+    else {
+      const int64_t f = e - n1;
+      f0tab[f] = czt_post_factor(a, (int)(f / a.ax.lout), (int)(f % a.ax.lout));
+    }
+  }
 }
 
 // One block per input row: component c runs its FFT on threads
@@ -301,9 +330,8 @@ __global__ __launch_bounds__(CztColsShape<Q>::CPB * FftThreads<Q>::value) void c
     const int ky = idx - woff;
     if (colok && ky >= 0 && ky < lout) {
       v = cmul(v, a.ay.post[ky]);
-      if (a.post_mode == POST_RS) v = c_mul(c_mul(v, rs_kernel_value(a.qx[kx], a.qy[ky], a.z, a.k)), a.cte);
-      else if (a.post_mode == POST_CONJ_RS) v = c_mul(v, conjf2(rs_kernel_value(a.qx[kx], a.qy[ky], a.z, a.k)));
-      else if (a.post_mode == POST_CONST) v = c_mul(v, a.cte);
+      if (a.f0tab) v = c_mul(v, a.f0tab[(int64_t)ky * lout_x + kx]);
+      else if (a.post_mode != POST_NONE) v = c_mul(v, czt_post_factor(a, ky, kx));
       oc[(int64_t)ky * lout_x + kx] = v;
     }
   };

@@ -98,6 +98,7 @@ __global__ __launch_bounds__(256) void adam_kernel(AdamArgs a) {
     const float mh = m * c1, vh = v * c2;
     p -= a.lr * (mh / (sqrtf(vh) + a.eps) + a.wd * p);
   };
+#pragma unroll 2
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (int64_t)gridDim.x * 256) {
     float4 p = p4[i], m = m4[i], v = v4[i];
     const float4 g = g4[i];

@@ -73,18 +73,21 @@ __global__ void pre_forward_kernel(PreArgs a0) {
   const int64_t npix = (int64_t)a.rows * a.cols;
   for (int64_t pix = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; pix < npix;
        pix += (int64_t)gridDim.x * blockDim.x) {
-    const int row = (int)(pix / a.cols), col = (int)(pix % a.cols);
     float2 e0 = a.in[pix];
     float2 e1 = a.nin > 1 ? a.in[a.in_cs + pix] : make_float2(0.f, 0.f);
     float2 e2 = a.nin > 2 ? a.in[2 * a.in_cs + pix] : make_float2(0.f, 0.f);
     apply_mask(a.mask, pix, e0, e1, e2);
     if (a.hna) {
+      const int row = (int)(pix / a.cols), col = (int)(pix % a.cols);
       float A[3][2];
       hna_map(a.x[col], a.y[row], a.hna_radius, a.hna_f, a.hna_stm2, A);
       for (int i = 0; i < 3; ++i) a.out[i * a.cs + pix] = c_add(c_scale(e0, A[i][0]), c_scale(e1, A[i][1]));
       continue;
     }
-    if (a.ez_mode != EZ_NONE) e2 = build_ez(a.ez_mode, e0, e1, a.x[col], a.y[row], a.z_dev ? *a.z_dev : a.z);
+    if (a.ez_mode != EZ_NONE) {
+      const int row = (int)(pix / a.cols), col = (int)(pix % a.cols);
+      e2 = build_ez(a.ez_mode, e0, e1, a.x[col], a.y[row], a.z_dev ? *a.z_dev : a.z);
+    }
     a.out[pix] = e0;
     if (a.ncomp > 1) a.out[a.cs + pix] = e1;
     if (a.ncomp > 2) a.out[2 * a.cs + pix] = e2;
@@ -168,24 +171,26 @@ __global__ void pre_backward_kernel(PreArgs a0) {
   const int64_t npix = (int64_t)a.rows * a.cols;
   for (int64_t pix = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; pix < npix;
        pix += (int64_t)gridDim.x * blockDim.x) {
-    const int row = (int)(pix / a.cols), col = (int)(pix % a.cols);
-    // forward recompute of the stage inputs
+    // forward recompute of the stage inputs (the second stage's input only
+    // when there is a second stage)
     float2 x0 = a.in[pix];
     float2 x1 = a.nin > 1 ? a.in[a.in_cs + pix] : make_float2(0.f, 0.f);
     float2 x2 = a.nin > 2 ? a.in[2 * a.in_cs + pix] : make_float2(0.f, 0.f);
     float2 m0 = x0, m1 = x1, m2 = x2;
-    apply_stage(a.mask.st[0], pix, m0, m1, m2);
+    if (a.mask.st[1].kind != MASK_NONE) apply_stage(a.mask.st[0], pix, m0, m1, m2);
     // cotangent of the masked (Ex', Ey', Ez')
     float2 g0 = a.gout[pix];
     float2 g1 = a.ncomp > 1 ? a.gout[a.cs + pix] : make_float2(0.f, 0.f);
     float2 g2 = a.ncomp > 2 ? a.gout[2 * a.cs + pix] : make_float2(0.f, 0.f);
     if (a.hna) {
+      const int row = (int)(pix / a.cols), col = (int)(pix % a.cols);
       float A[3][2];
       hna_map(a.x[col], a.y[row], a.hna_radius, a.hna_f, a.hna_stm2, A);
       const float2 h0 = c_add(c_add(c_scale(g0, A[0][0]), c_scale(g1, A[1][0])), c_scale(g2, A[2][0]));
       const float2 h1 = c_add(c_add(c_scale(g0, A[0][1]), c_scale(g1, A[1][1])), c_scale(g2, A[2][1]));
       g0 = h0; g1 = h1; g2 = make_float2(0.f, 0.f);
     } else if (a.ez_mode != EZ_NONE) {
+      const int row = (int)(pix / a.cols), col = (int)(pix % a.cols);
       const double X = a.x[col], Y = a.y[row], z = a.z_dev ? *a.z_dev : a.z;
       float cx, cy;
       if (a.ez_mode == EZ_HNA) {

@@ -145,8 +145,12 @@ int plan_axis(int m, int M, double f1, double f2, double Dm, bool allow3, AxisPl
   return XL_OK;
 }
 
+constexpr int kFtabBatch = 4;   // batch size from which the plane factors are tabulated
+
 struct CztPlan {
   AxisPlan x, y;       // forward axes (x: nx_in -> mx, y: ny_in -> my)
+  bool ftab;
+  size_t ftab_bytes;
   int ncomp, batch;
   int rin, lin_x, lout_x, lout_y;
   size_t tables_bytes, tiles_bytes;
@@ -170,6 +174,9 @@ int plan_czt(const xl_czt_params* p, CztPlan* cp) {
   }
   if (p->batch < 0 || p->batch > 65535) return fail(XL_EINVAL, "batch must be 0..65535");
   cp->batch = std::max(1, (int)p->batch);
+  // batches of RS-mode CZTs share tables of the plane factors (F, F0)
+  cp->ftab = cp->batch >= kFtabBatch && p->mode == XL_CZT_RS;
+  cp->ftab_bytes = cp->ftab ? ((size_t)cp->rin * cp->lin_x + (size_t)cp->lout_y * cp->lout_x) * sizeof(float2) : 0;
   cp->tables_bytes = (size_t)2 * 3 * xl::MAX_FFT * sizeof(float2);
   cp->tiles_bytes = (size_t)cp->batch * cp->ncomp * ((cp->rin + 3) / 4 * 4) * ((cp->lout_x + 3) / 4 * 4) * sizeof(float2);
   return XL_OK;
@@ -287,8 +294,10 @@ int xl_rs_propagate(const xl_rs_params* p, void* ws, size_t ws_bytes, void* stre
 size_t xl_czt_workspace_bytes(const xl_czt_params* p) {
   CztPlan cp;
   if (plan_czt(p, &cp)) return 0;
-  return cp.tables_bytes + cp.tiles_bytes + 512;
+  return cp.tables_bytes + cp.tiles_bytes + cp.ftab_bytes + 768;
 }
+
+static dim3 grid_for(int64_t npix, int batch = 1);
 
 int xl_czt(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream) {
   if (!device_ready()) return fail(XL_EINVAL, "xl_init() has not run on this device");
@@ -307,7 +316,7 @@ int xl_czt(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream) {
   if (p->adjoint && (p->mask[0].kind || p->mask[1].kind || p->ez_mode || p->nin != p->ncomp))
     return fail(XL_EINVAL, "the adjoint CZT takes ncomp cotangents and no masks / Ez (use xl_pre_backward)");
   if (int rc = check_mask(p->mask, p->nin)) return rc;
-  const size_t need = cp.tables_bytes + cp.tiles_bytes + 512;
+  const size_t need = cp.tables_bytes + cp.tiles_bytes + cp.ftab_bytes + 768;
   if (!ws || ws_bytes < need) return fail(XL_EWORKSPACE, "CZT workspace too small: need " + std::to_string(need));
   char* w = align256((char*)ws);
   float2* tab = (float2*)w;
@@ -386,6 +395,14 @@ int xl_czt(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream) {
     else { a.pre_mode = xl::PRE_NONE; a.post_mode = xl::POST_NONE; }
   }
   const int batch = std::max(1, (int)p->batch);
+  if (cp.ftab) {
+    float2* ft = (float2*)align256((char*)(tiles + cp.tiles_bytes / sizeof(float2)));
+    float2* f0 = ft + (size_t)cp.rin * cp.lin_x;
+    xl::KTimer t(xl::K_CZT_SETUP, st);
+    hipLaunchKernelGGL(xl::czt_ftab_kernel, grid_for(cp.ftab_bytes / sizeof(float2)), dim3(256), 0, st, a, ft, f0);
+    a.ftab = ft;
+    a.f0tab = f0;
+  }
   a.in_bs = batch > 1 ? p->in_batch_stride : 0;
   a.out_bs = batch > 1 ? p->out_batch_stride : 0;
   a.mask_bs = batch > 1 ? p->mask_batch_stride : 0;
@@ -399,7 +416,6 @@ int xl_czt(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream) {
   return hip_check(hipGetLastError(), "CZT launch");
 }
 
-static dim3 grid_for(int64_t npix, int batch = 1);
 
 // ---------------------------------------------------------------- zoom-GEMM CZT
 namespace {
