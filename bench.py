@@ -125,9 +125,12 @@ def main():
         secondary = run_secondary(dev, workloads, args)
 
     cpu = None
+    accuracy = None
     if not args.no_cpu_baseline:
         _progress("CPU baseline (float64 oracle, one chain)")
-        cpu = cpu_baseline(w)
+        cpu, ref = cpu_baseline(w)
+        accuracy = chain_accuracy(w, geom, ref)
+        _progress(f"accuracy vs float64: {accuracy}")
 
     line = {
         "metric": METRIC,
@@ -147,6 +150,7 @@ def main():
                    "form": "plain FP32 (LDS radix-16 FFT RS + Bluestein CZT)", "grid": n, "fft_len": P},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "accuracy": accuracy,
         "latency": _latency(secondary.get("serving_mix")),
         "secondary": secondary,
     }
@@ -235,6 +239,11 @@ def run_secondary(dev, workloads, args):
             ent["mfma_frac_of_2500"] = round(flops / (ms * 1e-3) / 2.5e15, 4)
         out[f"vczt_2048_to_512_{form}"] = ent
         _progress(f"vczt {form}: {ms:.4f} ms")
+    # the whole chain with the chosen CZT form (split-BF16 MFMA zoom-DFT)
+    ms = _time(lambda: w.run(g, czt_fn=lambda v, gg, **k: ops.czt(v, gg, form="gemm", **k)), max(10, args.steps), 3)
+    out["chain_2048_to_512_gemm_form"] = {"value": round(1000.0 / ms, 2), "unit": "VRS+VCZT chains/s",
+                                          "ms": round(ms, 4)}
+    _progress(f"chain with GEMM CZT: {ms:.4f} ms")
     # CZT form crossover: VCZT 2048^2 -> M^2 for small output windows
     cross = {}
     for m_out in (64, 128, 192, 256):
@@ -312,10 +321,24 @@ def cpu_baseline(w):
     t0 = time.perf_counter()
     mx, my, _ = R.LCD(Ex, Ey, None, eta, th)
     (vx, vy, _), _ = R.vrs_propagation(mx, my, w.x, w.y, w.wavelength, w.z_vrs)
-    R.vczt(vx, vy, w.x, w.y, w.wavelength, w.z_czt, w.xout, w.yout)
+    ref = R.vczt(vx, vy, w.x, w.y, w.wavelength, w.z_czt, w.xout, w.yout)
     dt = time.perf_counter() - t0
-    return {"value": round(1.0 / dt, 5), "unit": "VRS+VCZT chains/s", "cores": cores, "kind": "port",
-            "sample": f"1 chain at 2048^2 -> 512^2, complex128, scipy.fft workers={cores}, {dt:.1f} s"}
+    return ({"value": round(1.0 / dt, 5), "unit": "VRS+VCZT chains/s", "cores": cores, "kind": "port",
+             "sample": f"1 chain at 2048^2 -> 512^2, complex128, scipy.fft workers={cores}, {dt:.1f} s"},
+            np.stack(ref))
+
+
+def chain_accuracy(w, geom, ref):
+    """Relative L2 of the GPU chain output (both CZT forms) against the
+    float64 oracle chain of the same inputs."""
+    import numpy as np
+    from xlumina_amd import ops
+    out = {}
+    for form in ("bluestein", "gemm"):
+        _, got = w.run(geom, czt_fn=lambda v, g, **k: ops.czt(v, g, form=form, **k))
+        got = got.cpu().numpy().astype(np.complex128)
+        out[f"{form}_rel_l2_vs_float64"] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    return out
 
 
 if __name__ == "__main__":

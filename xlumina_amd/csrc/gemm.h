@@ -119,11 +119,11 @@ __device__ __forceinline__ float2 kgen_value(const KgenArgs& a, int r, int c) {
   return v;
 }
 
-// Two consecutive columns per thread (ld is a multiple of 32).
-__global__ void kgen_kernel(KgenArgs a) {
+// Two consecutive columns per thread (ld is a multiple of 32); blocks
+// [0, nblocks) of the caller's grid.
+__device__ __forceinline__ void kgen_body(const KgenArgs& a, int block, int nblocks) {
   const int64_t total2 = (int64_t)a.rows * a.dst.ld / 2;
-  for (int64_t e2 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e2 < total2;
-       e2 += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t e2 = block * (int64_t)blockDim.x + threadIdx.x; e2 < total2; e2 += (int64_t)nblocks * blockDim.x) {
     const int64_t e = e2 * 2;
     const int r = (int)(e / a.dst.ld), c0 = (int)(e % a.dst.ld);
     Split2 sp;
@@ -182,11 +182,11 @@ __device__ __forceinline__ void gemm_pre_values(const GemmPreArgs& a, int row, i
   }
 }
 
-// Two consecutive columns per thread (ld is a multiple of 32), 4-byte plane stores.
-__global__ void gemm_pre_kernel(GemmPreArgs a) {
+// Two consecutive columns per thread (ld is a multiple of 32), 4-byte plane
+// stores; blocks [0, nblocks) of the caller's grid.
+__device__ __forceinline__ void gemm_pre_body(const GemmPreArgs& a, int block, int nblocks) {
   const int64_t total2 = (int64_t)a.rowsp * a.dst.ld / 2;
-  for (int64_t e2 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e2 < total2;
-       e2 += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t e2 = block * (int64_t)blockDim.x + threadIdx.x; e2 < total2; e2 += (int64_t)nblocks * blockDim.x) {
     const int64_t e = e2 * 2;
     const int row = (int)(e / a.dst.ld), col0 = (int)(e % a.dst.ld);
     Split2 sp[3];
@@ -201,6 +201,16 @@ __global__ void gemm_pre_kernel(GemmPreArgs a) {
     for (int c = 0; c < 3; ++c)
       if (c < a.ncomp) store_split2(a.dst, (int64_t)c * a.rowsp * a.dst.ld + e, sp[c]);
   }
+}
+
+// Operand preparation in one launch: the two kernel matrices (float64 phase,
+// compute-bound) and the pre-op'd field planes (memory-bound) on disjoint
+// block ranges, so they overlap instead of running back to back.
+__global__ __launch_bounds__(256) void gemm_prep_kernel(KgenArgs k0, KgenArgs k1, GemmPreArgs pre, int nb0, int nb1) {
+  const int b = blockIdx.x;
+  if (b < nb0) kgen_body(k0, b, nb0);
+  else if (b < nb0 + nb1) kgen_body(k1, b - nb0, nb1);
+  else gemm_pre_body(pre, b - nb0 - nb1, gridDim.x - nb0 - nb1);
 }
 
 // ---------------------------------------------------------------------------

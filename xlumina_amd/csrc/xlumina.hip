@@ -557,26 +557,26 @@ int xl_czt_gemm(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream)
   hipStream_t st = (hipStream_t)stream;
   const int nc = p->ncomp;
 
-  // kernel matrices (x axis for stage 1, y axis for stage 2)
+  // kernel matrices (x axis for stage 1, y axis for stage 2) and the pre-op
+  // planes, prepared by one launch
   const AxisPlan* axp[2] = {&g.cz.x, &g.cz.y};
   xl::Planes* kd[2] = {&ka, &kb};
   const int krows[2] = {g.a1_rows, g.a2_rows};
+  xl::KgenArgs k[2];
+  int nbk[2];
   for (int i = 0; i < 2; ++i) {
-    xl::KgenArgs k;
-    k.dst = *kd[i];
-    k.m = axp[i]->m;
-    k.M = axp[i]->M;
-    k.rows = krows[i];
-    k.cols = (int)kd[i]->ld;
-    k.valid_cols = p->adjoint ? axp[i]->M : axp[i]->m;
-    k.adjoint = p->adjoint;
-    k.d1_dm = axp[i]->d1_dm;
-    k.dl_dm = axp[i]->dl_dm;
-    k.th_turns = axp[i]->th_turns;
-    xl::KTimer t(xl::K_GEMM_KGEN, st);
-    hipLaunchKernelGGL(xl::kgen_kernel, grid_for((int64_t)k.rows * k.cols / 2), dim3(256), 0, st, k);
+    k[i].dst = *kd[i];
+    k[i].m = axp[i]->m;
+    k[i].M = axp[i]->M;
+    k[i].rows = krows[i];
+    k[i].cols = (int)kd[i]->ld;
+    k[i].valid_cols = p->adjoint ? axp[i]->M : axp[i]->m;
+    k[i].adjoint = p->adjoint;
+    k[i].d1_dm = axp[i]->d1_dm;
+    k[i].dl_dm = axp[i]->dl_dm;
+    k[i].th_turns = axp[i]->th_turns;
+    nbk[i] = (int)grid_for((int64_t)k[i].rows * k[i].cols / 2).x;
   }
-  // pre-op planes
   {
     xl::GemmPreArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -605,8 +605,10 @@ int xl_czt_gemm(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream)
       a.cte = ctec;
       a.pre_mode = p->mode == XL_CZT_RS ? xl::PRE_CONJ_RS : (p->mode == XL_CZT_HNA ? xl::PRE_CONST : xl::PRE_NONE);
     }
+    const int nbp = (int)grid_for((int64_t)g.b_rowsp * g.b_ld / 2).x;
     xl::KTimer t(xl::K_GEMM_PRE, st);
-    hipLaunchKernelGGL(xl::gemm_pre_kernel, grid_for((int64_t)g.b_rowsp * g.b_ld / 2), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(xl::gemm_prep_kernel, dim3(nbk[0] + nbk[1] + nbp), dim3(256), 0, st, k[0], k[1], a, nbk[0],
+                       nbk[1]);
   }
   // stage 1: C1 = K1 G^T
   {
