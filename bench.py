@@ -10,7 +10,9 @@ Also reported (`secondary`): VCZT alone in both forms (FP32 Bluestein and
 split-BF16 MFMA), scalar RS 1024^2 (SLM fused, field + |E|^2)
 propagations/s, high-NA 2048^2 -> 256^2 forward+backward steps/s, and the
 serving loop on the mixed request stream (capacity, then Poisson arrivals at
-0.25-2x capacity with p50/p99 latency; summarised in `latency`).
+0.25-2x capacity with p50/p99 latency; summarised in `latency`), the chain
+with the split-BF16 GEMM CZT form, and (with the CPU baseline) the relative
+L2 of both forms' chain outputs against the float64 oracle chain.
 
 `roofline` covers the kernel with the largest share of the timed region,
 timed live with HIP events recorded on its launch stream; `cpu_baseline` is
@@ -179,12 +181,12 @@ def _progress(msg):
 
 
 def _latency(sweep):
-    """p50/p99 of the mixed request stream at half and at full measured capacity."""
+    """p50/p99 of the mixed request stream at 0.5, 0.75 and 1.0 of the measured capacity."""
     if not sweep:
         return None
     out = {"capacity_req_s": sweep["capacity"]["throughput_req_s"]}
     for r in sweep["rates"]:
-        if r["fraction_of_capacity"] in (0.5, 1.0):
+        if r["fraction_of_capacity"] in (0.5, 0.75, 1.0):
             out[f"at_{r['fraction_of_capacity']}x"] = {"p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
                                                       "throughput_req_s": r["throughput_req_s"]}
     return out
@@ -297,7 +299,7 @@ def run_secondary(dev, workloads, args):
     if args.serve_requests > 0:
         from xlumina_amd import serving as S
         srv = S.Server(dev)
-        sweep = S.rate_sweep(srv, count=args.serve_requests, fractions=(0.25, 0.5, 1.0, 2.0))
+        sweep = S.rate_sweep(srv, count=args.serve_requests, fractions=(0.25, 0.5, 0.75, 1.0, 2.0))
         sweep["workload"] = (f"{args.serve_requests} requests, n in 512/1024/2048 (p .5/.3/.2), scalar/vectorial "
                              "70/30, z ~ U[5, 100] mm, 20% gradient w.r.t. one phase mask, Poisson arrivals at "
                              "fractions of the measured capacity")

@@ -36,6 +36,9 @@ extern "C" {
 #define XL_MASK_LCD 3        /* retarder(eta=m0, theta=m1)       optical_elements.py:266-305 */
 #define XL_MASK_LCD_SCALAR 4 /* retarder(eta=s0, theta=s1)       optical_elements.py:266-305 */
 #define XL_MASK_LP 5         /* polariser alpha=m0, Ez -> 0      optical_elements.py:307-332 */
+#define XL_MASK_AMP 6        /* real amplitudes: Ex*m0 (and Ey*m1 when m1 is set); with a
+                                preceding XL_MASK_SSLM this is sSLM_with_amplitude
+                                optical_elements.py:155-168, 224-264 */
 
 /* construction of the propagated Ez from (Ex, Ey) (incoming Ez discarded) */
 #define XL_EZ_NONE 0

@@ -351,6 +351,21 @@ def sSLM(Ex, Ey, Ez, alpha, phi):
     return ox, oy, Ez
 
 
+def sSLM_with_amplitude(Ex, Ey, Ez, alpha, phi, a1, a2):
+    """optical_elements.py:155-168, 224-264: diag(a1 e^{i alpha}, a2 e^{i phi}), Ez kept."""
+    return a1 * np.exp(1j * alpha) * Ex, a2 * np.exp(1j * phi) * Ey, Ez
+
+
+def lens_mask(x, y, wavelength, radius, focal):
+    """optical_elements.py:678-703 with circular_mask :781-795."""
+    X, Y = np.meshgrid(x, y)
+    rx, ry = radius
+    fx, fy = focal
+    k = 2 * np.pi / wavelength
+    pupil = np.where((X ** 2 / rx ** 2 + Y ** 2 / ry ** 2) < 1, 1, 0)
+    return pupil * np.exp(-1j * k * (X ** 2 / (2 * fx) + Y ** 2 / (2 * fy)))
+
+
 def build_LCD_cell(eta, theta, shape):
     """toolbox.py:121-135: eta*ones -- scalars or per-pixel arrays both work."""
     return eta * np.ones((shape, shape)), theta * np.ones((shape, shape))

@@ -1,0 +1,99 @@
+"""Amplitude sSLM, lens and the reference's building blocks through the
+drop-in API (GPU), against float64 compositions of the oracle.
+
+Tolerances: fields <= 1e-5 relative L2; amplitude gradients vs central
+differences of the float64 oracle loss (eps = 1e-3) <= 2e-3 relative L2.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import xlumina_ref as R
+from tests.conftest import rel_l2
+from xlumina_amd import optical_elements as OE
+from xlumina_amd.vectorized_optics import VectorizedLight
+
+pytestmark = pytest.mark.gpu
+
+
+def light_from(ex, ey, x, y, dev):
+    L = VectorizedLight(x, y, 0.65, dev)
+    L.Ex = torch.from_numpy(ex.astype(np.complex64)).to(dev)
+    L.Ey = torch.from_numpy(ey.astype(np.complex64)).to(dev)
+    return L
+
+
+def test_amplitude_sslm_fused_into_vrs(cuda_dev):
+    n = 96
+    x, y = R.space(150.0, n)
+    rng = np.random.default_rng(40)
+    ex, ey = rng.standard_normal((2, n, n)) + 1j * rng.standard_normal((2, n, n))
+    a, p = rng.uniform(-np.pi, np.pi, (2, n, n))
+    a1, a2 = rng.uniform(0, 1, (2, n, n))
+    L = OE.sSLM_with_amplitude(light_from(ex, ey, x, y, cuda_dev), a, p, a1, a2)
+    out, _ = L.VRS_propagation(900.0)
+    sx, sy, _ = R.sSLM_with_amplitude(ex, ey, None, a, p, a1, a2)
+    (ox, oy, oz), _ = R.vrs_propagation(sx, sy, x, y, 0.65, 900.0)
+    got = np.stack([out.Ex.cpu().numpy(), out.Ey.cpu().numpy(), out.Ez.cpu().numpy()])
+    assert rel_l2(got, np.stack([ox, oy, oz])) <= 1e-5
+
+
+def test_building_blocks_vs_oracle(cuda_dev):
+    n = 96
+    x, y = R.space(150.0, n)
+    rng = np.random.default_rng(41)
+    ex, ey = rng.standard_normal((2, n, n)) + 1j * rng.standard_normal((2, n, n))
+    a, p = rng.uniform(-np.pi, np.pi, (2, n, n))
+    a1, a2 = rng.uniform(0, 1, (2, n, n))
+    eta, th = 0.8, 1.3
+    out = OE.bb_amplitude_and_phase_mod(light_from(ex, ey, x, y, cuda_dev), a, p, a1, a2, 700.0, eta, th)
+    out = OE.building_block(out, p, a, 500.0, th, eta)
+    got = np.stack([out.Ex.cpu().numpy(), out.Ey.cpu().numpy(), out.Ez.cpu().numpy()])
+    sx, sy, _ = R.sSLM_with_amplitude(ex, ey, None, a, p, a1, a2)
+    (vx, vy, vz), _ = R.vrs_propagation(sx, sy, x, y, 0.65, 700.0)
+    lx, ly, lz = R.LCD(vx, vy, vz, eta, th)
+    sx, sy, _ = R.sSLM(lx, ly, lz, p, a)
+    (vx, vy, vz), _ = R.vrs_propagation(sx, sy, x, y, 0.65, 500.0)
+    lx, ly, lz = R.LCD(vx, vy, vz, th, eta)
+    assert rel_l2(got, np.stack([lx, ly, lz])) <= 1e-5
+
+
+def test_amplitude_gradient_vs_fd(cuda_dev):
+    n = 48
+    x, y = R.space(100.0, n)
+    rng = np.random.default_rng(42)
+    ex, ey = rng.standard_normal((2, n, n)) + 1j * rng.standard_normal((2, n, n))
+    a, p = rng.uniform(-np.pi, np.pi, (2, n, n))
+    a10, a2 = rng.uniform(0.2, 1, (2, n, n))
+    w = rng.random((n, n))
+    A1 = torch.tensor(a10, dtype=torch.float32, device=cuda_dev, requires_grad=True)
+    out, _ = OE.sSLM_with_amplitude(light_from(ex, ey, x, y, cuda_dev), a, p, A1, a2).VRS_propagation(600.0)
+    wt = torch.from_numpy(w).float().to(cuda_dev)
+    L = (wt * (out.Ex.abs() ** 2 + out.Ey.abs() ** 2 + out.Ez.abs() ** 2)).sum()
+    (g,) = torch.autograd.grad(L, A1)
+
+    def oracle_loss(aa):
+        sx, sy, _ = R.sSLM_with_amplitude(ex, ey, None, a, p, aa, a2)
+        (vx, vy, vz), _ = R.vrs_propagation(sx, sy, x, y, 0.65, 600.0)
+        return float(np.sum(w * (np.abs(vx) ** 2 + np.abs(vy) ** 2 + np.abs(vz) ** 2)))
+
+    pix = rng.choice(n * n, 16, replace=False)
+    fd = []
+    for q in pix:
+        aa = a10.copy()
+        aa.flat[q] += 1e-3
+        lp = oracle_loss(aa)
+        aa.flat[q] -= 2e-3
+        fd.append((lp - oracle_loss(aa)) / 2e-3)
+    assert rel_l2(g.cpu().numpy().ravel()[pix], np.array(fd)) <= 2e-3
+
+
+def test_lens_vs_oracle(cuda_dev):
+    n = 128
+    x, y = R.space(400.0, n)
+    rng = np.random.default_rng(43)
+    ex, ey = rng.standard_normal((2, n, n)) + 1j * rng.standard_normal((2, n, n))
+    out, mask = OE.lens(light_from(ex, ey, x, y, cuda_dev), (350.0, 300.0), (2e4, 3e4))
+    ref = R.lens_mask(x, y, 0.65, (350.0, 300.0), (2e4, 3e4))
+    assert rel_l2(mask.cpu().numpy(), ref) <= 1e-6
+    assert rel_l2(out.Ex.cpu().numpy(), ex * ref) <= 1e-6 and float(out.Ez.abs().max()) == 0.0

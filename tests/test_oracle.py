@@ -327,3 +327,33 @@ def test_rs_propagation_z_derivative_is_linear_in_h():
     d = R.rs_propagation_dz(u, x, y, 0.65, z)
     fd = (R.rs_propagation(u, x, y, 0.65, z + eps)[0] - R.rs_propagation(u, x, y, 0.65, z - eps)[0]) / (2 * eps)
     assert np.linalg.norm(fd - d) / np.linalg.norm(d) < 1e-6
+
+
+def test_amplitude_sslm_reduces_to_sslm_and_scales():
+    rng = np.random.default_rng(30)
+    Ex, Ey = rng.standard_normal((2, 6, 6)) + 1j * rng.standard_normal((2, 6, 6))
+    a, p = rng.uniform(-np.pi, np.pi, (2, 6, 6))
+    one = np.ones((6, 6))
+    s1 = R.sSLM(Ex, Ey, None, a, p)
+    s2 = R.sSLM_with_amplitude(Ex, Ey, None, a, p, one, one)
+    assert np.allclose(s1[0], s2[0]) and np.allclose(s1[1], s2[1])
+    a1, a2 = rng.uniform(0, 1, (2, 6, 6))
+    s3 = R.sSLM_with_amplitude(Ex, Ey, None, a, p, a1, a2)
+    assert np.allclose(np.abs(s3[0]), a1 * np.abs(Ex)) and np.allclose(np.abs(s3[1]), a2 * np.abs(Ey))
+
+
+def test_lens_focuses_a_plane_wave():
+    """A plane wave through lens(f) and RS-propagated by f concentrates at the
+    axis: the on-axis intensity is many times the input's, and it is the
+    maximum of the plane (a sign error in the lens phase would diverge)."""
+    n = 256
+    x, y = R.space(400.0, n)
+    f = 20000.0
+    m = R.lens_mask(x, y, 0.65, (350.0, 350.0), (f, f))
+    assert abs(abs(m[n // 2, n // 2]) - 1.0) < 1e-12 and np.all(np.abs(m) <= 1.0 + 1e-12)
+    out, _ = R.rs_propagation(m, x, y, 0.65, f)
+    I = np.abs(out) ** 2
+    c = I[n // 2 - 1:n // 2 + 1, n // 2 - 1:n // 2 + 1].max()
+    assert c == I.max() and c > 50.0
+    diverging = R.rs_propagation(np.conj(m), x, y, 0.65, f)[0]
+    assert np.abs(diverging).max() ** 2 < 2.0

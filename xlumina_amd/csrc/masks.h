@@ -16,7 +16,7 @@
 
 namespace xl {
 
-enum { MASK_NONE = 0, MASK_SLM = 1, MASK_SSLM = 2, MASK_LCD = 3, MASK_LCD_SCALAR = 4, MASK_LP = 5 };
+enum { MASK_NONE = 0, MASK_SLM = 1, MASK_SSLM = 2, MASK_LCD = 3, MASK_LCD_SCALAR = 4, MASK_LP = 5, MASK_AMP = 6 };
 enum { EZ_NONE = 0, EZ_VRS = 1, EZ_VCZT = 2, EZ_HNA = 3 };
 
 struct MaskStage {
@@ -90,6 +90,10 @@ __device__ __forceinline__ void apply_stage(const MaskStage& m, int64_t pix, flo
       e2 = make_float2(0.f, 0.f);
       break;
     }
+    case MASK_AMP:
+      e0 = c_scale(e0, m.m0[pix]);
+      if (m.m1) e1 = c_scale(e1, m.m1[pix]);
+      break;
     default:
       break;
   }

@@ -161,6 +161,17 @@ __device__ __forceinline__ void stage_backward(const MaskStage& m, int64_t pix, 
       g2 = make_float2(0.f, 0.f);
       break;
     }
+    case MASK_AMP: {   // e' = A e: dA = Re(conj(g) e), g <- A g
+      const float a0 = m.m0[pix];
+      if (gp0) gp0[pix] = re_conj_mul(g0, e0);
+      g0 = c_scale(g0, a0);
+      if (m.m1) {
+        const float a1 = m.m1[pix];
+        if (gp1) gp1[pix] = re_conj_mul(g1, e1);
+        g1 = c_scale(g1, a1);
+      }
+      break;
+    }
     default:
       break;
   }

@@ -22,7 +22,6 @@ enum KernelId {
   K_PRE_FWD,
   K_PRE_BWD,
   K_TF,
-  K_GEMM_KGEN,
   K_GEMM_PRE,
   K_GEMM_Y,
   K_GEMM_X,
@@ -35,7 +34,7 @@ enum KernelId {
 inline const char* kernel_name(int id) {
   static const char* names[K_COUNT] = {"rs_hrow", "rs_urow", "rs_col", "rs_final", "czt_setup", "czt_rows",
                                        "czt_cols", "pre_forward", "pre_backward", "transfer_function",
-                                       "gemm_kernel_gen", "gemm_pre", "gemm_y", "gemm_x", "mse_intensity",
+                                       "gemm_pre", "gemm_y", "gemm_x", "mse_intensity",
                                        "adam", "rs_hcol"};
   return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }

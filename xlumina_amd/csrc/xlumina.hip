@@ -87,12 +87,14 @@ xl::MaskArgs to_mask(const xl_mask_stage* m) {
 int check_mask(const xl_mask_stage* m, int nin) {
   for (int i = 0; i < 2; ++i) {
     const int k = m[i].kind;
-    if (k < XL_MASK_NONE || k > XL_MASK_LP) return fail(XL_EINVAL, "unknown mask kind");
+    if (k < XL_MASK_NONE || k > XL_MASK_AMP) return fail(XL_EINVAL, "unknown mask kind");
     if (k == XL_MASK_NONE) continue;
-    if ((k == XL_MASK_SLM || k == XL_MASK_SSLM || k == XL_MASK_LCD || k == XL_MASK_LP) && !m[i].m0)
+    if ((k == XL_MASK_SLM || k == XL_MASK_SSLM || k == XL_MASK_LCD || k == XL_MASK_LP || k == XL_MASK_AMP) &&
+        !m[i].m0)
       return fail(XL_EINVAL, "mask parameter m0 is NULL");
     if ((k == XL_MASK_SSLM || k == XL_MASK_LCD) && !m[i].m1) return fail(XL_EINVAL, "mask parameter m1 is NULL");
-    if (k != XL_MASK_SLM && nin < 2) return fail(XL_EINVAL, "polarisation masks need Ex and Ey");
+    if (k == XL_MASK_AMP && m[i].m1 && nin < 2) return fail(XL_EINVAL, "an Ey amplitude needs Ex and Ey");
+    if (k != XL_MASK_SLM && k != XL_MASK_AMP && nin < 2) return fail(XL_EINVAL, "polarisation masks need Ex and Ey");
   }
   return XL_OK;
 }

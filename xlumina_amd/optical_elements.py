@@ -84,12 +84,27 @@ def jones_LCD(eta, theta):
     return jones_general_retarder(eta, theta, 0.0)
 
 
+def jones_sSLM_with_amplitude(alpha, phi, a1, a2):
+    """optical_elements.py:155-168: diag(a1 e^{i alpha}, a2 e^{i phi})."""
+    j = jones_sSLM(alpha, phi)
+    amp = torch.stack([torch.as_tensor(a1, dtype=torch.complex128), torch.as_tensor(a2, dtype=torch.complex128)])
+    return j * amp[:, None]
+
+
 # -- (3) polarisation devices ----------------------------------------------------------------
 
 def sSLM(input_field: VectorizedLight, alpha_array=None, phi_array=None) -> VectorizedLight:
     """optical_elements.py:186-222: diag(e^{i alpha}, e^{i phi}) per pixel, Ez kept."""
     return input_field._with_mask(MaskSpec(N.XL_MASK_SSLM, _param(alpha_array, input_field),
                                            _param(phi_array, input_field)))
+
+
+def sSLM_with_amplitude(input_field: VectorizedLight, alpha_array=None, phi_array=None, A1_array=None,
+                        A2_array=None) -> VectorizedLight:
+    """optical_elements.py:224-264: diag(A1 e^{i alpha}, A2 e^{i phi}) per pixel, Ez kept
+    (two fused stages: the phases, then the real amplitudes)."""
+    light = sSLM(input_field, alpha_array, phi_array)
+    return light._with_mask(MaskSpec(N.XL_MASK_AMP, _param(A1_array, input_field), _param(A2_array, input_field)))
 
 
 def LCD(input_field: VectorizedLight, eta, theta) -> VectorizedLight:
@@ -140,3 +155,45 @@ def VCZT_objective_lens(input_field: VectorizedLight, r: float, f: float, xout, 
         print("Time taken to perform one VCZT propagation through objective lens (in seconds):  "
               f"{(time.perf_counter() - tic):.4f}")
     return light
+
+
+# -- (4) lenses and building blocks ------------------------------------------------------------
+
+def lens(input_field, radius, focal):
+    """optical_elements.py:678-703: circular pupil of radius (rx, ry) times
+    exp(-i k (X^2 / (2 fx) + Y^2 / (2 fy))) -> (light, lens mask).
+
+    The phase reaches ~k R^2 / (2 f) (10^4 rad for a 1 mm pupil at f = 25 mm),
+    so it is formed in float64 on the device and reduced before the complex64
+    cast."""
+    rx, ry = radius
+    fx, fy = focal
+    dev = input_field.device
+    X = torch.as_tensor(np.asarray(input_field.x, dtype=np.float64), device=dev)[None, :]
+    Y = torch.as_tensor(np.asarray(input_field.y, dtype=np.float64), device=dev)[:, None]
+    pupil = ((X / rx) ** 2 + (Y / ry) ** 2 < 1.0).to(torch.float64)     # circular_mask, :781-795
+    turns = -(input_field.k / (2 * math.pi)) * (X ** 2 / (2 * fx) + Y ** 2 / (2 * fy))
+    turns = turns - torch.round(turns)
+    mask = torch.polar(pupil, 2 * math.pi * turns).to(torch.complex64)
+    if isinstance(input_field, VectorizedLight):
+        out = VectorizedLight(input_field.x, input_field.y, input_field.wavelength, dev)
+        E = input_field.E
+        out._E = torch.stack([E[0] * mask, E[1] * mask, torch.zeros_like(E[2])])
+    else:
+        out = ScalarLight(input_field.x, input_field.y, input_field.wavelength, dev)
+        out._field = input_field.field * mask
+    return out, mask
+
+
+def building_block(input_light: VectorizedLight, alpha, phi, z, eta, theta) -> VectorizedLight:
+    """optical_elements.py:919-939: sSLM(alpha, phi) -> VRS(z) -> LCD(eta, theta).
+    The sSLM is fused into the propagation and the LCD stays pending for the
+    next one; z may be a tensor with requires_grad."""
+    l_propagated, _ = sSLM(input_light, alpha, phi).VRS_propagation(z)
+    return LCD(l_propagated, eta, theta)
+
+
+def bb_amplitude_and_phase_mod(input_light: VectorizedLight, alpha, phi, amp1, amp2, z, eta, theta) -> VectorizedLight:
+    """optical_elements.py:894-917: sSLM_with_amplitude -> VRS(z) -> LCD(eta, theta)."""
+    l_propagated, _ = sSLM_with_amplitude(input_light, alpha, phi, amp1, amp2).VRS_propagation(z)
+    return LCD(l_propagated, eta, theta)

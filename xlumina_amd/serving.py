@@ -335,7 +335,7 @@ class Server:
         return rep
 
 
-def rate_sweep(server: Server, count: int = 10000, fractions=(0.25, 0.5, 1.0, 2.0), seed: int = 7) -> dict:
+def rate_sweep(server: Server, count: int = 10000, fractions=(0.25, 0.5, 0.75, 1.0, 2.0), seed: int = 7) -> dict:
     """Measures capacity (all requests at t = 0), then serves Poisson streams
     at each fraction of it; returns the summaries."""
     cap = server.run(make_requests(count, None, seed))
