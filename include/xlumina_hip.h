@@ -212,6 +212,96 @@ typedef struct xl_adam_params {
 } xl_adam_params;
 int xl_adam_step(const xl_adam_params* p, void* stream);
 
+/* Field losses against a complex64 target field (loss_functions.py:107-149),
+ * for a batch of fields: loss_b = scale * sum_p d_bp^2 / npix with
+ *   XL_LOSS_INTENSITY  d = |E|^2 - |T|^2          (MSE_Intensity :137-149)
+ *   XL_LOSS_AMPLITUDE  d = |E| - |T|              (MSE_Amplitude :107-120)
+ *   XL_LOSS_PHASE      d = angle(E) - angle(T)    (MSE_Phase     :122-135)
+ * The batch covers vMSE_* (:65-105, members = the Ex, Ey, Ez components) and
+ * mean_batch_MSE_Intensity (:51-63).  grad (optional) is the cotangent of the
+ * field, dL/dRe + i dL/dIm: 4 s d E, 2 s d E/|E|, 2 s d i E/|E|^2 (s = scale /
+ * npix; the last two are 0 where E = 0).  The target is a constant.
+ * Deterministic two-stage reduction. */
+#define XL_LOSS_INTENSITY 0
+#define XL_LOSS_AMPLITUDE 1
+#define XL_LOSS_PHASE 2
+typedef struct xl_field_loss_params {
+  const void* field;            /* batch x npix complex64 */
+  int64_t field_batch_stride;
+  const void* target;           /* complex64 target fields */
+  int64_t target_batch_stride;  /* 0: one target for every member */
+  void* grad;                   /* batch x npix complex64, or NULL; may alias field */
+  int64_t grad_batch_stride;
+  float* loss;                  /* batch float32 losses, or NULL */
+  int64_t npix;
+  int32_t batch;
+  int32_t mode;                 /* XL_LOSS_* */
+  float scale;
+  int32_t reserved;
+} xl_field_loss_params;
+size_t xl_field_loss_workspace_bytes(int64_t npix, int32_t batch);
+int xl_field_loss(const xl_field_loss_params* p, void* workspace, size_t workspace_bytes, void* stream);
+
+/* small_area_hybrid (loss_functions.py:24-40) of a batch of detected
+ * intensities D_b (vectorized_loss_hybrid :42-49): with S = sum D,
+ * I = D / (S + 1e-8) and mask m = [I > 0.7 max I],
+ * loss_b = scale * sum m / (sum m I + 1e-8).  grad (optional) = dloss/dD with
+ * the mask held constant (as jnp.where differentiates):
+ * -scale * (sum m) / (sum m I + 1e-8)^2 * (m_p - sum m I) / (S + 1e-8). */
+typedef struct xl_small_area_params {
+  const float* intensity;      /* batch x npix float32 */
+  int64_t batch_stride;
+  float* grad;                 /* batch x npix float32, or NULL */
+  int64_t grad_batch_stride;
+  float* loss;                 /* batch float32 */
+  int64_t npix;
+  int32_t batch;
+  float scale;
+} xl_small_area_params;
+size_t xl_small_area_workspace_bytes(int64_t npix, int32_t batch);
+int xl_small_area(const xl_small_area_params* p, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Two-port beam splitter on (Ex, Ey) with complex coefficients:
+ *   c = ca a + cb b,  d = da a + db b   (b = NULL: single input)
+ * which is BS_symmetric (optical_elements.py:334-392: ca = db = R, cb = da = iT
+ * with the 1% loss folded into R, T), BS_symmetric_SI (:394-460: b = NULL,
+ * port 1 swaps c and d) and BS (:462-513: ca = db = sqrt T,
+ * cb = sqrt R e^{i phase}, da = -sqrt R e^{-i phase}).  The adjoint is the
+ * same call with (a, b) = cotangents of (c, d) and coefficients
+ * (conj ca, conj da, conj cb, conj db).  c or d may be NULL; with
+ * zero_ez = 1 component 2 of each output is set to 0 (the reference builds c
+ * and d as new lights, Ez = 0). */
+typedef struct xl_bs_params {
+  const void* a;               /* 2 components (Ex, Ey) of npix complex64 */
+  const void* b;               /* or NULL */
+  int64_t a_comp_stride, b_comp_stride;
+  void* c;                     /* or NULL */
+  void* d;                     /* or NULL */
+  int64_t c_comp_stride, d_comp_stride;
+  float ca[2], cb[2], da[2], db[2];   /* (re, im) */
+  int64_t npix;
+  int32_t zero_ez;
+  int32_t reserved;
+} xl_bs_params;
+int xl_beam_splitter(const xl_bs_params* p, void* stream);
+
+/* Coefficient cotangents of a beam splitter: dots = the four complex sums
+ * over both components and all pixels of conj(gc) a, conj(gc) b, conj(gd) a,
+ * conj(gd) b as 8 float32 (re, im); a NULL gc, gd or b contributes 0.
+ * For a real parameter t of the coefficients, dL/dt = Re sum_k kappa_k'(t) S_k. */
+typedef struct xl_bs_dots_params {
+  const void* gc;
+  const void* gd;
+  int64_t gc_comp_stride, gd_comp_stride;
+  const void* a;
+  const void* b;
+  int64_t a_comp_stride, b_comp_stride;
+  float* dots;                 /* 8 float32 in device memory */
+  int64_t npix;
+} xl_bs_dots_params;
+size_t xl_bs_dots_workspace_bytes(int64_t npix);
+int xl_bs_dots(const xl_bs_dots_params* p, void* workspace, size_t workspace_bytes, void* stream);
+
 /* transfer_function_RS (wave_optics.py:291-297) on meshgrid(x, y): out is
  * ny x nx complex64. */
 int xl_transfer_function_rs(void* out, const double* x, const double* y, int32_t nx, int32_t ny, double z,

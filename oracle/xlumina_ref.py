@@ -509,3 +509,42 @@ def radial_beam(x, y, w0):
 def is_conserving_energy(src, prop):
     """toolbox.py:74-96 on tuples of component arrays."""
     return sum(np.sum(np.abs(c) ** 2) for c in prop) / sum(np.sum(np.abs(c) ** 2) for c in src)
+
+
+def _bs_RT(theta):
+    """optical_elements.py:376-383: T = |cos|, R = |sin|, both less 1% of T."""
+    T = np.abs(np.cos(theta))
+    R = np.abs(np.sin(theta))
+    noise = T * 0.01
+    return R - noise, T - noise
+
+
+def BS_symmetric(a, b, theta):
+    """optical_elements.py:334-392: a, b are (Ex, Ey) pairs; returns the
+    (Ex, Ey, Ez=0) triples of ports c and d."""
+    R, T = _bs_RT(theta)
+    (ax, ay), (bx, by) = a, b
+    z = np.zeros_like(ax)
+    c = (R * ax + 1j * T * bx, R * ay + 1j * T * by, z)
+    d = (1j * T * ax + R * bx, 1j * T * ay + R * by, z)
+    return c, d
+
+
+def BS_symmetric_SI(a, theta, port=0):
+    """optical_elements.py:394-460 (bs_port0 / bs_port1)."""
+    R, T = _bs_RT(theta)
+    ax, ay = a
+    z = np.zeros_like(ax)
+    if port == 0:
+        return (R * ax, R * ay, z), (1j * T * ax, 1j * T * ay, z)
+    return (1j * T * ax, 1j * T * ay, z), (R * ax, R * ay, z)
+
+
+def BS(a1, a2, R, T, phase):
+    """optical_elements.py:462-513: lossless splitter with a phase."""
+    (ax, ay), (bx, by) = a1, a2
+    z = np.zeros_like(ax)
+    st, sr = np.sqrt(T), np.sqrt(R)
+    b1 = (st * ax + sr * np.exp(1j * phase) * bx, st * ay + sr * np.exp(1j * phase) * by, z)
+    b2 = (-sr * np.exp(-1j * phase) * ax + st * bx, -sr * np.exp(-1j * phase) * ay + st * by, z)
+    return b1, b2

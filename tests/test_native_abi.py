@@ -75,7 +75,7 @@ int main(void) {
 
 
 def test_host_side_queries(lib):
-    assert lib.xl_abi_version() == 3
+    assert lib.xl_abi_version() == 4
     assert lib.xl_max_fft_length() == 4096
     # RS workspace: ncomp * round4(n) * P + (P/2) * P + P complex64 (+ alignment slack)
     n, P = 1024, 2048

@@ -23,6 +23,7 @@ class XlError(RuntimeError):
 XL_MASK_NONE, XL_MASK_SLM, XL_MASK_SSLM, XL_MASK_LCD, XL_MASK_LCD_SCALAR, XL_MASK_LP, XL_MASK_AMP = range(7)
 XL_EZ_NONE, XL_EZ_VRS, XL_EZ_VCZT, XL_EZ_HNA = range(4)
 XL_CZT_RS, XL_CZT_HNA, XL_CZT_PLAIN = range(3)
+XL_LOSS_INTENSITY, XL_LOSS_AMPLITUDE, XL_LOSS_PHASE = range(3)
 
 
 class MaskStage(C.Structure):
@@ -74,6 +75,32 @@ class AdamParams(C.Structure):
                 ("eps", C.c_float), ("weight_decay", C.c_float), ("reserved", C.c_int32)]
 
 
+class FieldLossParams(C.Structure):
+    _fields_ = [("field", C.c_void_p), ("field_batch_stride", C.c_int64), ("target", C.c_void_p),
+                ("target_batch_stride", C.c_int64), ("grad", C.c_void_p), ("grad_batch_stride", C.c_int64),
+                ("loss", C.c_void_p), ("npix", C.c_int64), ("batch", C.c_int32), ("mode", C.c_int32),
+                ("scale", C.c_float), ("reserved", C.c_int32)]
+
+
+class SmallAreaParams(C.Structure):
+    _fields_ = [("intensity", C.c_void_p), ("batch_stride", C.c_int64), ("grad", C.c_void_p),
+                ("grad_batch_stride", C.c_int64), ("loss", C.c_void_p), ("npix", C.c_int64), ("batch", C.c_int32),
+                ("scale", C.c_float)]
+
+
+class BsParams(C.Structure):
+    _fields_ = [("a", C.c_void_p), ("b", C.c_void_p), ("a_comp_stride", C.c_int64), ("b_comp_stride", C.c_int64),
+                ("c", C.c_void_p), ("d", C.c_void_p), ("c_comp_stride", C.c_int64), ("d_comp_stride", C.c_int64),
+                ("ca", C.c_float * 2), ("cb", C.c_float * 2), ("da", C.c_float * 2), ("db", C.c_float * 2),
+                ("npix", C.c_int64), ("zero_ez", C.c_int32), ("reserved", C.c_int32)]
+
+
+class BsDotsParams(C.Structure):
+    _fields_ = [("gc", C.c_void_p), ("gd", C.c_void_p), ("gc_comp_stride", C.c_int64),
+                ("gd_comp_stride", C.c_int64), ("a", C.c_void_p), ("b", C.c_void_p), ("a_comp_stride", C.c_int64),
+                ("b_comp_stride", C.c_int64), ("dots", C.c_void_p), ("npix", C.c_int64)]
+
+
 class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_int64), ("total_ms", C.c_double)]
 
@@ -97,6 +124,13 @@ SIGNATURES = {
     "xl_mse_workspace_bytes": (C.c_size_t, [C.c_int64, C.c_int32]),
     "xl_mse_intensity": (C.c_int, [C.POINTER(MseParams), C.c_void_p, C.c_size_t, C.c_void_p]),
     "xl_adam_step": (C.c_int, [C.POINTER(AdamParams), C.c_void_p]),
+    "xl_field_loss_workspace_bytes": (C.c_size_t, [C.c_int64, C.c_int32]),
+    "xl_field_loss": (C.c_int, [C.POINTER(FieldLossParams), C.c_void_p, C.c_size_t, C.c_void_p]),
+    "xl_small_area_workspace_bytes": (C.c_size_t, [C.c_int64, C.c_int32]),
+    "xl_small_area": (C.c_int, [C.POINTER(SmallAreaParams), C.c_void_p, C.c_size_t, C.c_void_p]),
+    "xl_beam_splitter": (C.c_int, [C.POINTER(BsParams), C.c_void_p]),
+    "xl_bs_dots_workspace_bytes": (C.c_size_t, [C.c_int64]),
+    "xl_bs_dots": (C.c_int, [C.POINTER(BsDotsParams), C.c_void_p, C.c_size_t, C.c_void_p]),
     "xl_timing_enable": (C.c_int, [C.c_int32, C.c_int32]),
     "xl_timing_collect": (C.c_int, [C.POINTER(KernelTime), C.c_int32]),
 }

@@ -119,4 +119,209 @@ __global__ void adam_tick_kernel(int32_t* count) {
   if (threadIdx.x == 0 && blockIdx.x == 0) count[0] = count[0] + 1;
 }
 
+// ---------------------------------------------------------------- field losses
+// MSE_Intensity / MSE_Amplitude / MSE_Phase (xlumina/loss_functions.py:107-149)
+// against a complex target field, batched over fields or components.
+enum { LOSS_INTENSITY = 0, LOSS_AMPLITUDE = 1, LOSS_PHASE = 2 };
+
+struct FieldLossArgs {
+  const float2* field;
+  int64_t fbs;
+  const float2* target;
+  int64_t tbs;
+  float2* grad;
+  int64_t gbs;
+  float* partial;      // batch x kMseBlocks
+  float* loss;
+  int64_t npix;
+  float gscale;        // 4 s (intensity) or 2 s, s = scale / npix
+  float lscale;        // s
+};
+
+// d of one pixel and the direction of its field cotangent (grad = gscale d dir)
+template <int MODE>
+__device__ __forceinline__ float loss_term(float2 v, float2 t, float2& dir) {
+  if constexpr (MODE == LOSS_INTENSITY) {
+    dir = v;
+    return v.x * v.x + v.y * v.y - (t.x * t.x + t.y * t.y);
+  } else if constexpr (MODE == LOSS_AMPLITUDE) {
+    const float av = sqrtf(v.x * v.x + v.y * v.y);
+    const float inv = av > 0.f ? 1.f / av : 0.f;
+    dir = make_float2(v.x * inv, v.y * inv);
+    return av - sqrtf(t.x * t.x + t.y * t.y);
+  } else {
+    const float a2 = v.x * v.x + v.y * v.y;
+    const float inv = a2 > 0.f ? 1.f / a2 : 0.f;
+    dir = make_float2(-v.y * inv, v.x * inv);   // i E / |E|^2 = d angle / d(Re + i Im)
+    return atan2f(v.y, v.x) - atan2f(t.y, t.x);
+  }
+}
+
+// grid (kMseBlocks, batch): partial[b][blk] = sum over the block's pixels of d^2.
+template <int MODE>
+__global__ __launch_bounds__(kMseThreads) void field_loss_kernel(FieldLossArgs a) {
+  const int b = blockIdx.y;
+  const float2* e = a.field + (int64_t)b * a.fbs;
+  const float2* tg = a.target + (int64_t)b * a.tbs;
+  float2* g = a.grad ? a.grad + (int64_t)b * a.gbs : nullptr;
+  float acc = 0.f;
+  for (int64_t p = (int64_t)blockIdx.x * kMseThreads + threadIdx.x; p < a.npix;
+       p += (int64_t)gridDim.x * kMseThreads) {
+    float2 dir;
+    const float d = loss_term<MODE>(e[p], tg[p], dir);
+    acc += d * d;
+    if (g) g[p] = make_float2(a.gscale * d * dir.x, a.gscale * d * dir.y);
+  }
+  __shared__ float red[kMseThreads / 64];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float s = 0.f;
+    for (int w = 0; w < kMseThreads / 64; ++w) s += red[w];
+    a.partial[(int64_t)b * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(kMseThreads) void field_loss_reduce_kernel(FieldLossArgs a, int nblk) {
+  const int b = blockIdx.x;
+  float acc = 0.f;
+  for (int i = threadIdx.x; i < nblk; i += kMseThreads) acc += a.partial[(int64_t)b * nblk + i];
+  __shared__ float red[kMseThreads / 64];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float s = 0.f;
+    for (int w = 0; w < kMseThreads / 64; ++w) s += red[w];
+    a.loss[b] = a.lscale * s;
+  }
+}
+
+// ---------------------------------------------------------------- small-area loss
+// small_area_hybrid (xlumina/loss_functions.py:24-40): per detector b with
+// S = sum D, I = D / (S + eps), m = [I > 0.7 max I]:
+// loss = scale * sum m / (sum m I + eps).
+constexpr float kSaFrac = 0.7f;   // epsilon of the reference (fraction of the peak)
+constexpr float kSaEps = 1e-8f;
+
+struct SmallAreaArgs {
+  const float* in;
+  int64_t ibs;
+  float* grad;
+  int64_t gbs;
+  float* partial;      // batch x kMseBlocks x 2
+  float* stats;        // batch x 4: S, max D, count, sum m I
+  float* loss;
+  int64_t npix;
+  float scale;
+};
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Block sums of two per-thread values (PHASE 0: sum D and max D; PHASE 1:
+// mask count and sum m I) into partial[b][blk][0..1].
+template <int PHASE>
+__global__ __launch_bounds__(kMseThreads) void small_area_kernel(SmallAreaArgs a) {
+  const int b = blockIdx.y;
+  const float* d = a.in + (int64_t)b * a.ibs;
+  float s0 = 0.f, s1 = PHASE == 0 ? -__builtin_huge_valf() : 0.f;
+  float den = 1.f, imax = 0.f;
+  if constexpr (PHASE == 1) {
+    den = a.stats[4 * b] + kSaEps;
+    imax = a.stats[4 * b + 1] / den;
+  }
+  for (int64_t p = (int64_t)blockIdx.x * kMseThreads + threadIdx.x; p < a.npix;
+       p += (int64_t)gridDim.x * kMseThreads) {
+    const float v = d[p];
+    if constexpr (PHASE == 0) {
+      s0 += v;
+      s1 = fmaxf(s1, v);
+    } else {
+      const float i = v / den;
+      if (i > kSaFrac * imax) {
+        s0 += 1.f;
+        s1 += i;
+      }
+    }
+  }
+  __shared__ float red[2][kMseThreads / 64];
+  s0 = wave_sum(s0);
+  s1 = PHASE == 0 ? wave_max(s1) : wave_sum(s1);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = s0;
+    red[1][threadIdx.x >> 6] = s1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float t0 = 0.f, t1 = PHASE == 0 ? -__builtin_huge_valf() : 0.f;
+    for (int w = 0; w < kMseThreads / 64; ++w) {
+      t0 += red[0][w];
+      t1 = PHASE == 0 ? fmaxf(t1, red[1][w]) : t1 + red[1][w];
+    }
+    float* o = a.partial + ((int64_t)b * gridDim.x + blockIdx.x) * 2;
+    o[0] = t0;
+    o[1] = t1;
+  }
+}
+
+// One block per detector: reduces the partials of PHASE into stats; after
+// phase 1 also writes the loss.  Counts are summed as integers (each block's
+// count is exact in float32).
+template <int PHASE>
+__global__ __launch_bounds__(kMseThreads) void small_area_reduce_kernel(SmallAreaArgs a, int nblk) {
+  const int b = blockIdx.x;
+  const float* part = a.partial + (int64_t)b * nblk * 2;
+  double s0 = 0.0;
+  float s1 = PHASE == 0 ? -__builtin_huge_valf() : 0.f;
+  for (int i = threadIdx.x; i < nblk; i += kMseThreads) {
+    s0 += PHASE == 0 ? (double)part[2 * i] : (double)(int64_t)part[2 * i];
+    s1 = PHASE == 0 ? fmaxf(s1, part[2 * i + 1]) : s1 + part[2 * i + 1];
+  }
+  __shared__ double r0[kMseThreads];
+  __shared__ float r1[kMseThreads];
+  r0[threadIdx.x] = s0;
+  r1[threadIdx.x] = s1;
+  __syncthreads();
+  for (int w = kMseThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      r0[threadIdx.x] += r0[threadIdx.x + w];
+      r1[threadIdx.x] = PHASE == 0 ? fmaxf(r1[threadIdx.x], r1[threadIdx.x + w]) : r1[threadIdx.x] + r1[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    float* st = a.stats + 4 * b;
+    if constexpr (PHASE == 0) {
+      st[0] = (float)r0[0];
+      st[1] = r1[0];
+    } else {
+      st[2] = (float)r0[0];
+      st[3] = r1[0];
+      if (a.loss) a.loss[b] = a.scale * (float)(r0[0] / ((double)r1[0] + kSaEps));
+    }
+  }
+}
+
+// dloss/dD with the mask held constant.
+__global__ __launch_bounds__(kMseThreads) void small_area_grad_kernel(SmallAreaArgs a) {
+  const int b = blockIdx.y;
+  const float* d = a.in + (int64_t)b * a.ibs;
+  float* g = a.grad + (int64_t)b * a.gbs;
+  const float* st = a.stats + 4 * b;
+  const float den = st[0] + kSaEps, imax = st[1] / den;
+  const float sm = st[3] + kSaEps;
+  const float coef = -a.scale * st[2] / (sm * sm) / den;
+  for (int64_t p = (int64_t)blockIdx.x * kMseThreads + threadIdx.x; p < a.npix;
+       p += (int64_t)gridDim.x * kMseThreads) {
+    const float i = d[p] / den;
+    const float m = i > kSaFrac * imax ? 1.f : 0.f;
+    g[p] = coef * (m - st[3]);
+  }
+}
+
 }  // namespace xl

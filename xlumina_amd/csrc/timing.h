@@ -28,6 +28,8 @@ enum KernelId {
   K_MSE,
   K_ADAM,
   K_RS_HCOL,
+  K_LOSS,
+  K_BS,
   K_COUNT
 };
 
@@ -35,7 +37,7 @@ inline const char* kernel_name(int id) {
   static const char* names[K_COUNT] = {"rs_hrow", "rs_urow", "rs_col", "rs_final", "czt_setup", "czt_rows",
                                        "czt_cols", "pre_forward", "pre_backward", "transfer_function",
                                        "gemm_pre", "gemm_y", "gemm_x", "mse_intensity",
-                                       "adam", "rs_hcol"};
+                                       "adam", "rs_hcol", "loss", "beam_splitter"};
   return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
 

@@ -19,6 +19,7 @@
 #include "gemm.h"
 #include "fft.h"
 #include "masks.h"
+#include "mix.h"
 #include "pre.h"
 #include "rs.h"
 #include "timing.h"
@@ -237,7 +238,7 @@ char* align256(char* p) { return (char*)(((uintptr_t)p + 255) & ~(uintptr_t)255)
 extern "C" {
 
 const char* xl_last_error(void) { return g_err.c_str(); }
-int xl_abi_version(void) { return 3; }
+int xl_abi_version(void) { return 4; }
 int xl_max_fft_length(void) { return xl::MAX_FFT; }
 int xl_init(void) { return ensure_init(); }
 
@@ -782,6 +783,135 @@ int xl_adam_step(const xl_adam_params* p, void* stream) {
   hipLaunchKernelGGL(xl::adam_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a);
   hipLaunchKernelGGL(xl::adam_tick_kernel, dim3(1), dim3(64), 0, st, p->count);
   return hip_check(hipGetLastError(), "Adam launch");
+}
+
+// ---------------------------------------------------------------- losses
+size_t xl_field_loss_workspace_bytes(int64_t npix, int32_t batch) {
+  if (npix < 1 || batch < 1) return 0;
+  return (size_t)batch * xl::kMseBlocks * sizeof(float) + 256;
+}
+
+int xl_field_loss(const xl_field_loss_params* p, void* ws, size_t ws_bytes, void* stream) {
+  if (!p) return fail(XL_EINVAL, "null params");
+  if (!p->field || !p->target || p->npix < 1 || p->batch < 1 || p->batch > 65535)
+    return fail(XL_EINVAL, "bad field-loss arguments");
+  if (p->mode < XL_LOSS_INTENSITY || p->mode > XL_LOSS_PHASE) return fail(XL_EINVAL, "unknown loss mode");
+  if (!p->grad && !p->loss) return fail(XL_EINVAL, "loss call asks for neither loss nor gradient");
+  const size_t need = xl_field_loss_workspace_bytes(p->npix, p->batch);
+  if (!ws || ws_bytes < need) return fail(XL_EWORKSPACE, "loss workspace too small: need " + std::to_string(need));
+  xl::FieldLossArgs a;
+  a.field = (const float2*)p->field;
+  a.fbs = p->field_batch_stride;
+  a.target = (const float2*)p->target;
+  a.tbs = p->target_batch_stride;
+  a.grad = (float2*)p->grad;
+  a.gbs = p->grad_batch_stride;
+  a.partial = (float*)align256((char*)ws);
+  a.loss = p->loss;
+  a.npix = p->npix;
+  const double s = p->scale / (double)p->npix;
+  a.gscale = (float)((p->mode == XL_LOSS_INTENSITY ? 4.0 : 2.0) * s);
+  a.lscale = (float)s;
+  hipStream_t st = (hipStream_t)stream;
+  xl::KTimer t(xl::K_LOSS, st);
+  const dim3 g(xl::kMseBlocks, p->batch), b(xl::kMseThreads);
+  if (p->mode == XL_LOSS_INTENSITY) hipLaunchKernelGGL(xl::field_loss_kernel<xl::LOSS_INTENSITY>, g, b, 0, st, a);
+  else if (p->mode == XL_LOSS_AMPLITUDE) hipLaunchKernelGGL(xl::field_loss_kernel<xl::LOSS_AMPLITUDE>, g, b, 0, st, a);
+  else hipLaunchKernelGGL(xl::field_loss_kernel<xl::LOSS_PHASE>, g, b, 0, st, a);
+  if (p->loss)
+    hipLaunchKernelGGL(xl::field_loss_reduce_kernel, dim3(p->batch), b, 0, st, a, xl::kMseBlocks);
+  return hip_check(hipGetLastError(), "loss launch");
+}
+
+size_t xl_small_area_workspace_bytes(int64_t npix, int32_t batch) {
+  if (npix < 1 || batch < 1) return 0;
+  return (size_t)batch * (xl::kMseBlocks * 2 + 4) * sizeof(float) + 512;
+}
+
+int xl_small_area(const xl_small_area_params* p, void* ws, size_t ws_bytes, void* stream) {
+  if (!p) return fail(XL_EINVAL, "null params");
+  if (!p->intensity || !p->loss || p->npix < 1 || p->batch < 1 || p->batch > 65535)
+    return fail(XL_EINVAL, "bad small-area arguments");
+  const size_t need = xl_small_area_workspace_bytes(p->npix, p->batch);
+  if (!ws || ws_bytes < need)
+    return fail(XL_EWORKSPACE, "small-area workspace too small: need " + std::to_string(need));
+  xl::SmallAreaArgs a;
+  a.in = p->intensity;
+  a.ibs = p->batch_stride;
+  a.grad = p->grad;
+  a.gbs = p->grad_batch_stride;
+  a.partial = (float*)align256((char*)ws);
+  a.stats = (float*)align256((char*)(a.partial + (size_t)p->batch * xl::kMseBlocks * 2));
+  a.loss = p->loss;
+  a.npix = p->npix;
+  a.scale = p->scale;
+  hipStream_t st = (hipStream_t)stream;
+  xl::KTimer t(xl::K_LOSS, st);
+  const dim3 g(xl::kMseBlocks, p->batch), b(xl::kMseThreads);
+  hipLaunchKernelGGL(xl::small_area_kernel<0>, g, b, 0, st, a);
+  hipLaunchKernelGGL(xl::small_area_reduce_kernel<0>, dim3(p->batch), b, 0, st, a, xl::kMseBlocks);
+  hipLaunchKernelGGL(xl::small_area_kernel<1>, g, b, 0, st, a);
+  hipLaunchKernelGGL(xl::small_area_reduce_kernel<1>, dim3(p->batch), b, 0, st, a, xl::kMseBlocks);
+  if (p->grad) hipLaunchKernelGGL(xl::small_area_grad_kernel, g, b, 0, st, a);
+  return hip_check(hipGetLastError(), "small-area launch");
+}
+
+// ---------------------------------------------------------------- beam splitters
+int xl_beam_splitter(const xl_bs_params* p, void* stream) {
+  if (!p) return fail(XL_EINVAL, "null params");
+  if (!p->a || p->npix < 1) return fail(XL_EINVAL, "beam splitter needs input a and npix >= 1");
+  if (!p->c && !p->d) return fail(XL_EINVAL, "beam splitter call has no output");
+  xl::BsArgs a;
+  a.a = (const float2*)p->a;
+  a.b = (const float2*)p->b;
+  a.acs = p->a_comp_stride;
+  a.bcs = p->b_comp_stride;
+  a.c = (float2*)p->c;
+  a.d = (float2*)p->d;
+  a.ccs = p->c_comp_stride;
+  a.dcs = p->d_comp_stride;
+  a.ca = make_float2(p->ca[0], p->ca[1]);
+  a.cb = make_float2(p->cb[0], p->cb[1]);
+  a.da = make_float2(p->da[0], p->da[1]);
+  a.db = make_float2(p->db[0], p->db[1]);
+  a.npix = p->npix;
+  a.zero_ez = p->zero_ez ? 1 : 0;
+  hipStream_t st = (hipStream_t)stream;
+  xl::KTimer t(xl::K_BS, st);
+  const int64_t total = (a.zero_ez ? 3 : 2) * p->npix;
+  const int64_t blocks = std::min<int64_t>((total + 255) / 256, 8192);
+  hipLaunchKernelGGL(xl::bs_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a);
+  return hip_check(hipGetLastError(), "beam splitter launch");
+}
+
+size_t xl_bs_dots_workspace_bytes(int64_t npix) {
+  if (npix < 1) return 0;
+  return (size_t)xl::kBsDotBlocks * 8 * sizeof(float) + 256;
+}
+
+int xl_bs_dots(const xl_bs_dots_params* p, void* ws, size_t ws_bytes, void* stream) {
+  if (!p) return fail(XL_EINVAL, "null params");
+  if (!p->a || !p->dots || p->npix < 1) return fail(XL_EINVAL, "bad beam-splitter dot arguments");
+  const size_t need = xl_bs_dots_workspace_bytes(p->npix);
+  if (!ws || ws_bytes < need)
+    return fail(XL_EWORKSPACE, "beam-splitter dot workspace too small: need " + std::to_string(need));
+  xl::BsDotsArgs a;
+  a.gc = (const float2*)p->gc;
+  a.gd = (const float2*)p->gd;
+  a.gccs = p->gc_comp_stride;
+  a.gdcs = p->gd_comp_stride;
+  a.a = (const float2*)p->a;
+  a.b = (const float2*)p->b;
+  a.acs = p->a_comp_stride;
+  a.bcs = p->b_comp_stride;
+  a.partial = (float*)align256((char*)ws);
+  a.dots = p->dots;
+  a.npix = p->npix;
+  hipStream_t st = (hipStream_t)stream;
+  xl::KTimer t(xl::K_BS, st);
+  hipLaunchKernelGGL(xl::bs_dots_kernel, dim3(xl::kBsDotBlocks), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(xl::bs_dots_reduce_kernel, dim3(1), dim3(64), 0, st, a, xl::kBsDotBlocks);
+  return hip_check(hipGetLastError(), "beam-splitter dot launch");
 }
 
 int xl_transfer_function_rs(void* out, const double* x, const double* y, int32_t nx, int32_t ny, double z, double k,

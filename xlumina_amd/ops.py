@@ -653,3 +653,124 @@ def adam_step(param: torch.Tensor, grad: torch.Tensor, m: torch.Tensor, v: torch
     p.count = count.data_ptr()
     p.lr, p.b1, p.b2, p.eps, p.weight_decay = lr, b1, b2, eps, weight_decay
     N.check(lib.xl_adam_step(C.byref(p), _stream(param.device)))
+
+
+def _gpu_c64(t: torch.Tensor, what: str) -> torch.Tensor:
+    if t.dtype != torch.complex64 or t.device.type != "cuda":
+        raise N.NativeUnavailable(f"{what} must be a complex64 tensor on a ROCm GPU")
+    return t.contiguous()
+
+
+def field_loss(field: torch.Tensor, target: torch.Tensor, mode: int, want_grad: bool = False,
+               scale: float = 1.0, grad: Optional[torch.Tensor] = None):
+    """MSE_Intensity / MSE_Amplitude / MSE_Phase (loss_functions.py:107-149)
+    of each member of `field` (B, rows, cols) against `target` (B, rows, cols)
+    or one shared (rows, cols) target.  Returns (loss (B,), cotangent or None)."""
+    lib = N.load()
+    field = _gpu_c64(field, "field")
+    dev = field.device
+    N.ensure_device(dev)
+    B = field.shape[0]
+    npix = field[0].numel()
+    tgt = _gpu_c64(target.to(dev), "target")
+    if tgt.numel() == npix:
+        tbs = 0
+    elif tgt.numel() == B * npix:
+        tbs = npix
+    else:
+        raise ValueError("target must be one field or one field per batch member")
+    loss = torch.empty(B, dtype=torch.float32, device=dev)
+    if want_grad and grad is None:
+        grad = torch.empty_like(field)
+    need = lib.xl_field_loss_workspace_bytes(npix, B)
+    ws = torch.empty(need, dtype=torch.uint8, device=dev)
+    p = N.FieldLossParams()
+    p.field, p.field_batch_stride = field.data_ptr(), npix
+    p.target, p.target_batch_stride = tgt.data_ptr(), tbs
+    p.grad = grad.data_ptr() if want_grad else None
+    p.grad_batch_stride = npix
+    p.loss = loss.data_ptr()
+    p.npix, p.batch, p.mode, p.scale = npix, B, mode, scale
+    N.check(lib.xl_field_loss(C.byref(p), ws.data_ptr(), ws.numel(), _stream(dev)))
+    return loss, (grad if want_grad else None)
+
+
+def small_area(intensity: torch.Tensor, want_grad: bool = False, scale: float = 1.0):
+    """small_area_hybrid (loss_functions.py:24-40) of each detector of
+    `intensity` (M, rows, cols) float32.  Returns (loss (M,), dloss/dI or None)."""
+    lib = N.load()
+    if intensity.dtype != torch.float32 or intensity.device.type != "cuda":
+        raise N.NativeUnavailable("intensity must be a float32 tensor on a ROCm GPU")
+    d = intensity.contiguous()
+    dev = d.device
+    N.ensure_device(dev)
+    M = d.shape[0]
+    npix = d[0].numel()
+    loss = torch.empty(M, dtype=torch.float32, device=dev)
+    grad = torch.empty_like(d) if want_grad else None
+    need = lib.xl_small_area_workspace_bytes(npix, M)
+    ws = torch.empty(need, dtype=torch.uint8, device=dev)
+    p = N.SmallAreaParams()
+    p.intensity, p.batch_stride = d.data_ptr(), npix
+    p.grad = _ptr(grad)
+    p.grad_batch_stride = npix
+    p.loss, p.npix, p.batch, p.scale = loss.data_ptr(), npix, M, scale
+    N.check(lib.xl_small_area(C.byref(p), ws.data_ptr(), ws.numel(), _stream(dev)))
+    return loss, grad
+
+
+def beam_splitter(a: torch.Tensor, b: Optional[torch.Tensor], coef, want_c: bool = True,
+                  want_d: bool = True, ncomp_out: int = 3):
+    """c = ca a + cb b, d = da a + db b on components 0, 1 of `a`, `b`
+    (k, rows, cols) complex64, k >= 2; coef = (ca, cb, da, db) complex
+    numbers.  Outputs are (ncomp_out, rows, cols) with component 2 (if any) 0."""
+    lib = N.load()
+    a = _gpu_c64(a, "a")
+    dev = a.device
+    N.ensure_device(dev)
+    if b is not None:
+        b = _gpu_c64(b.to(dev), "b")
+        if b.shape[1:] != a.shape[1:] or b.shape[0] < 2:
+            raise ValueError("beam-splitter inputs must have the same plane shape and >= 2 components")
+    if a.shape[0] < 2:
+        raise ValueError("beam-splitter inputs need Ex and Ey")
+    npix = a[0].numel()
+    shape = (ncomp_out,) + tuple(a.shape[1:])
+    c = torch.empty(shape, dtype=torch.complex64, device=dev) if want_c else None
+    d = torch.empty(shape, dtype=torch.complex64, device=dev) if want_d else None
+    p = N.BsParams()
+    p.a, p.a_comp_stride = a.data_ptr(), npix
+    p.b, p.b_comp_stride = _ptr(b), npix
+    p.c, p.d = _ptr(c), _ptr(d)
+    p.c_comp_stride = p.d_comp_stride = npix
+    for name, k in zip(("ca", "cb", "da", "db"), coef):
+        k = complex(k)
+        getattr(p, name)[0], getattr(p, name)[1] = k.real, k.imag
+    p.npix = npix
+    p.zero_ez = 1 if ncomp_out > 2 else 0
+    N.check(lib.xl_beam_splitter(C.byref(p), _stream(dev)))
+    return c, d
+
+
+def bs_dots(gc: Optional[torch.Tensor], gd: Optional[torch.Tensor], a: torch.Tensor,
+            b: Optional[torch.Tensor]) -> torch.Tensor:
+    """(4,) complex64 device tensor: sums over Ex, Ey and all pixels of
+    conj(gc) a, conj(gc) b, conj(gd) a, conj(gd) b (absent terms are 0)."""
+    lib = N.load()
+    a = _gpu_c64(a, "a")
+    dev = a.device
+    npix = a[0].numel()
+    gc = None if gc is None else _gpu_c64(gc, "gc")
+    gd = None if gd is None else _gpu_c64(gd, "gd")
+    b = None if b is None else _gpu_c64(b, "b")
+    dots = torch.empty(8, dtype=torch.float32, device=dev)
+    need = lib.xl_bs_dots_workspace_bytes(npix)
+    ws = torch.empty(need, dtype=torch.uint8, device=dev)
+    p = N.BsDotsParams()
+    p.gc, p.gd = _ptr(gc), _ptr(gd)
+    p.gc_comp_stride = p.gd_comp_stride = npix
+    p.a, p.b = a.data_ptr(), _ptr(b)
+    p.a_comp_stride = p.b_comp_stride = npix
+    p.dots, p.npix = dots.data_ptr(), npix
+    N.check(lib.xl_bs_dots(C.byref(p), ws.data_ptr(), ws.numel(), _stream(dev)))
+    return torch.view_as_complex(dots.view(4, 2))

@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import math
 import time
+from typing import Optional
 
 import numpy as np
 import torch
@@ -197,3 +198,94 @@ def bb_amplitude_and_phase_mod(input_light: VectorizedLight, alpha, phi, amp1, a
     """optical_elements.py:894-917: sSLM_with_amplitude -> VRS(z) -> LCD(eta, theta)."""
     l_propagated, _ = sSLM_with_amplitude(input_light, alpha, phi, amp1, amp2).VRS_propagation(z)
     return LCD(l_propagated, eta, theta)
+
+
+# -- beam splitters (optical_elements.py:334-513) --------------------------------------------
+
+class _BeamSplitter(torch.autograd.Function):
+    """c = ca a + cb b, d = da a + db b on (Ex, Ey) with coef = (ca, cb, da, db)
+    (a complex128 CPU tensor that may carry a gradient); Ez of c and d is 0.
+    Backward: the same kernel with the conjugate-transposed coefficients, and
+    the coefficient cotangents conj(sum conj(g_out) in) from xl_bs_dots."""
+
+    @staticmethod
+    def forward(ctx, a, b, coef):
+        k = [complex(v) for v in coef.detach().tolist()]
+        c, d = ops.beam_splitter(a.detach(), None if b is None else b.detach(), k)
+        ctx.k = k
+        ctx.has_b = b is not None
+        ctx.save_for_backward(a, b if b is not None else a)
+        return c, d
+
+    @staticmethod
+    def backward(ctx, gc, gd):
+        a, b = ctx.saved_tensors
+        b = b if ctx.has_b else None
+        ca, cb, da, db = ctx.k
+        gc = gc.contiguous() if gc is not None else None
+        gd = gd.contiguous() if gd is not None else None
+        z = torch.zeros_like(a) if gc is None or gd is None else None
+        gcc = gc if gc is not None else z
+        gdd = gd if gd is not None else z
+        ga = gb = gk = None
+        need_b = ctx.has_b and ctx.needs_input_grad[1]
+        if ctx.needs_input_grad[0] or need_b:
+            conj = lambda v: v.conjugate()
+            ga, gb = ops.beam_splitter(gcc, gdd, (conj(ca), conj(da), conj(cb), conj(db)), want_c=True, want_d=need_b,
+                                       ncomp_out=a.shape[0])
+        if ctx.needs_input_grad[2]:
+            s = ops.bs_dots(gc, gd, a.detach(), None if b is None else b.detach())
+            gk = s.to(device="cpu", dtype=torch.complex128).conj()
+        return (ga if ctx.needs_input_grad[0] else None), (gb if need_b else None), gk
+
+
+def _scalar(v):
+    if isinstance(v, torch.Tensor):
+        return v.to(device="cpu", dtype=torch.float64)
+    return torch.tensor(float(v), dtype=torch.float64)
+
+
+def _bs_RT(theta):
+    """optical_elements.py:376-383: T = |cos theta|, R = |sin theta|, both reduced by 1% of T."""
+    th = _scalar(theta)
+    T = torch.abs(torch.cos(th))
+    R = torch.abs(torch.sin(th))
+    noise = T * 0.01
+    return R - noise, T - noise
+
+
+def _bs_apply(a: VectorizedLight, b: Optional[VectorizedLight], coef):
+    out_c = VectorizedLight(a.x, a.y, a.wavelength, a.device)
+    out_d = VectorizedLight(a.x, a.y, a.wavelength, a.device)
+    out_c._E, out_d._E = _BeamSplitter.apply(a.E, None if b is None else b.E, coef)
+    return out_c, out_d
+
+
+def BS_symmetric(a: VectorizedLight, b: VectorizedLight, theta):
+    """optical_elements.py:334-392: lossy symmetric splitter [[R, iT], [iT, R]]
+    (R = |sin theta|, T = |cos theta| less 1% of T) -> lights at ports c, d."""
+    R, T = _bs_RT(theta)
+    zero = torch.zeros((), dtype=torch.float64)
+    r, t = torch.complex(R, zero), torch.complex(zero, T)
+    return _bs_apply(a, b, torch.stack([r, t, t, r]))
+
+
+def BS_symmetric_SI(a: VectorizedLight, theta, port=0):
+    """optical_elements.py:394-460: single-input splitter; port 0: c = R a,
+    d = iT a; port 1: c = iT a, d = R a."""
+    R, T = _bs_RT(theta)
+    zero = torch.zeros((), dtype=torch.float64)
+    r, t = torch.complex(R, zero), torch.complex(zero, T)
+    z = torch.complex(zero, zero)
+    coef = torch.stack([r, z, t, z]) if int(port) == 0 else torch.stack([t, z, r, z])
+    return _bs_apply(a, None, coef)
+
+
+def BS(a1: VectorizedLight, a2: VectorizedLight, R, T, phase):
+    """optical_elements.py:462-513: [[sqrt T, e^{i phase} sqrt R],
+    [-e^{-i phase} sqrt R, sqrt T]] -> lights at ports b1, b2."""
+    sR, sT, ph = torch.sqrt(_scalar(R)), torch.sqrt(_scalar(T)), _scalar(phase)
+    zero = torch.zeros((), dtype=torch.float64)
+    st = torch.complex(sT, zero)
+    e = torch.polar(torch.ones((), dtype=torch.float64), ph)
+    return _bs_apply(a1, a2, torch.stack([st, sR * e, -sR * e.conj(), st]))

@@ -35,3 +35,9 @@ def is_conserving_energy(light_source, propagated_light):
             return torch.sum(torch.abs(l.field) ** 2)
         return sum(torch.sum(torch.abs(c) ** 2) for c in (l.Ex, l.Ey, l.Ez))
     return total(propagated_light) / total(light_source)
+
+
+def softmin(args, beta=90):
+    """toolbox.py:98-103: -logsumexp(-beta args) / beta (differentiable min)."""
+    a = args if isinstance(args, torch.Tensor) else torch.as_tensor(np.asarray(args, dtype=np.float64))
+    return -torch.logsumexp(-beta * a.reshape(-1), dim=0) / beta
