@@ -159,3 +159,17 @@ def test_masks_fuse_into_propagation(grid512, cuda_dev):
     lx, ly, _ = R.LCD(sx, sy, None, 0.4, 1.1)
     (rx, ry, rz), _ = R.vrs_propagation(lx, ly, x, y, WL, 2000.0)
     assert rel_l2(H(fused.E), np.stack([rx, ry, rz])) < TOL
+
+
+def test_training_pair_loader_delivers_device_batches(cuda_dev, tmp_path):
+    from xlumina_amd import data as D
+    rng = np.random.default_rng(70)
+    xin = (rng.standard_normal((6, 32, 32)) + 1j * rng.standard_normal((6, 32, 32))).astype(np.complex64)
+    xout = rng.standard_normal((6, 32, 32)).astype(np.float32)
+    D.save_training_pairs(str(tmp_path / "pairs.npz"), xin, xout)
+    L = D.MultiHDF5DataLoader(str(tmp_path), batch_size=3, device=cuda_dev, seed=0)
+    a, b = next(L)
+    torch.cuda.synchronize()
+    assert a.device.type == "cuda" and a.shape == (3, 32, 32)
+    np.testing.assert_array_equal(a.cpu().numpy(), xin[L.last_indices])
+    np.testing.assert_array_equal(b.cpu().numpy().real, xout[L.last_indices])
