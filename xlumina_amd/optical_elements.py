@@ -240,9 +240,12 @@ class _BeamSplitter(torch.autograd.Function):
 
 
 def _scalar(v):
+    """A one-element parameter (float, 0-d or (1,) tensor) as a 0-d float64 CPU tensor."""
     if isinstance(v, torch.Tensor):
-        return v.to(device="cpu", dtype=torch.float64)
-    return torch.tensor(float(v), dtype=torch.float64)
+        if v.numel() != 1:
+            raise ValueError("beam-splitter parameters are scalars")
+        return v.to(device="cpu", dtype=torch.float64).reshape(())
+    return torch.tensor(float(np.asarray(v).reshape(-1)[0]), dtype=torch.float64)
 
 
 def _bs_RT(theta):
@@ -289,3 +292,75 @@ def BS(a1: VectorizedLight, a2: VectorizedLight, R, T, phase):
     st = torch.complex(sT, zero)
     e = torch.polar(torch.ones((), dtype=torch.float64), ph)
     return _bs_apply(a1, a2, torch.stack([st, sR * e, -sR * e.conj(), st]))
+
+
+# -- fluorescence and a pre-built table (optical_elements.py:941-962, 1503-1649) ----------------
+
+def fluorescence(i_ex, i_dep, beta=1):
+    """optical_elements.py:941-955: effective STED intensity
+    i_ex (1 - beta (1 - exp(-i_dep / (i_ex + 1e-8))))."""
+    return i_ex * (1 - beta * (1 - torch.exp(-(i_dep / (i_ex + 1e-8)))))
+
+
+def vectorized_fluorophores(i_ex, i_dep):
+    """optical_elements.py:957-962: fluorescence over a stack of detectors."""
+    return fluorescence(i_ex, i_dep)
+
+
+def _unit_angle(p):
+    return p * (2 * math.pi) - math.pi          # raw (0, 1) -> (-pi, pi)
+
+
+def _distance(p, offset):
+    return (torch.abs(p) * 100 + offset) * 1e4  # cm -> um
+
+
+def hybrid_setup_sharp_focus(ls1, ls2, ls3, ls4, ls5, ls6, parameters, fixed_params, distance_offset=8.9):
+    """optical_elements.py:1503-1649: the 3 x 3 table of three building blocks
+    and nine symmetric beam splitters, six high-NA detectors.
+
+    parameters (29 tensors): per block i = 1..3 [phase_i_1, phase_i_2 (N x N),
+    eta_i, theta_i, z_i_1, z_i_2], then bs1..bs9, z4, z5 -- raw values mapped
+    as the reference does (angles p 2pi - pi, distances (|p| 100 + offset) cm).
+    fixed_params = [r, f, xout, yout] of the objective lens.
+    Returns (intensities (6, My, Mx) = |Ez|^2 per detector, detector lights).
+    Every stage runs on the device; gradients reach all 29 parameters."""
+    r, f, xout, yout = fixed_params[:4]
+    P = [p if isinstance(p, torch.Tensor) else torch.as_tensor(np.asarray(p, dtype=np.float64)) for p in parameters]
+    blocks = []
+    for i in range(3):
+        ph1, ph2, eta, th, za, zb = P[6 * i: 6 * i + 6]
+        blocks.append((_unit_angle(ph1), _unit_angle(ph2), _unit_angle(eta), _unit_angle(th),
+                       _distance(za, distance_offset), _distance(zb, distance_offset)))
+    bs = [_unit_angle(p) for p in P[18:27]]
+    z4, z5 = _distance(P[27], distance_offset), _distance(P[28], distance_offset)
+    (p11, p12, e1, t1, z11, z12), (p21, p22, e2, t2, z21, z22), (p31, p32, e3, t3, z31, z32) = blocks
+
+    def vrs(light, z):
+        return light.VRS_propagation(z)[0]
+
+    def olens(light):
+        return VCZT_objective_lens(light, r, f, xout, yout)
+
+    c1, d1 = BS_symmetric(ls1, ls4, bs[0])
+    b2 = vrs(building_block(c1, p11, p12, z11, e1, t1), z12)
+    c2, d2 = BS_symmetric(ls2, b2, bs[1])
+    b3 = vrs(c2, z21 + z22)
+    c3, d3 = BS_symmetric(ls3, b3, bs[2])
+    det_1 = olens(vrs(c3, z31 + z32))
+    a5, a6 = vrs(d2, z4), vrs(d3, z4)
+    c4, d4 = BS_symmetric(d1, ls5, bs[3])
+    b5 = vrs(c4, z11 + z12)
+    c5, d5 = BS_symmetric(a5, b5, bs[4])
+    b6 = vrs(building_block(c5, p21, p22, z21, e2, t2), z22)
+    c6, d6 = BS_symmetric(a6, b6, bs[5])
+    det_2 = olens(vrs(c6, z31 + z32))
+    a8, a9 = vrs(d5, z5), vrs(d6, z5)
+    c7, d7 = BS_symmetric(d4, ls6, bs[6])
+    b8 = vrs(c7, z11 + z12)
+    c8, d8 = BS_symmetric(a8, b8, bs[7])
+    b9 = vrs(c8, z21 + z22)
+    c9, d9 = BS_symmetric(a9, b9, bs[8])
+    det_3 = olens(vrs(building_block(c9, p31, p32, z31, e3, t3), z32))
+    dets = [det_1, det_2, det_3, olens(d7), olens(d8), olens(d9)]
+    return torch.stack([d.Ez.abs() ** 2 for d in dets]), dets
