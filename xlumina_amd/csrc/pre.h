@@ -227,6 +227,45 @@ __global__ void pre_backward_kernel(PreArgs a0) {
   }
 }
 
+// Adjoint of a lone SLM on one component (the 4f optimiser's and the scalar
+// requests' mask): e' = e w, w = e^{i phi}, so dL/dphi = Re(conj(g) i e') and
+// the input cotangent is conj(w) g -- the arithmetic of stage_backward's
+// MASK_SLM case.  A pure stream (20 B in, up to 12 B out per pixel): each
+// thread issues the loads of kSlmUnroll pixels before any arithmetic, so that
+// enough bytes are in flight to run at HBM rate.
+constexpr int kSlmUnroll = 4;
+
+__global__ __launch_bounds__(256) void pre_slm_backward_kernel(PreArgs a0) {
+  const PreArgs a = pre_member(a0, blockIdx.y);
+  const int64_t npix = (int64_t)a.rows * a.cols;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const float* ph = a.mask.st[0].m0;
+  float* gp = a.gparam[0][0];
+  for (int64_t base = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; base < npix; base += kSlmUnroll * stride) {
+    float2 e[kSlmUnroll], g[kSlmUnroll];
+    float q[kSlmUnroll];
+#pragma unroll
+    for (int u = 0; u < kSlmUnroll; ++u) {
+      const int64_t pix = base + u * stride;
+      if (pix < npix) {
+        e[u] = a.in[pix];
+        g[u] = a.gout[pix];
+        q[u] = ph[pix];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kSlmUnroll; ++u) {
+      const int64_t pix = base + u * stride;
+      if (pix < npix) {
+        const float2 w = c_expi(q[u]);
+        const float2 o = c_mul(e[u], w);
+        if (gp) gp[pix] = re_conj_mul(g[u], make_float2(-o.y, o.x));
+        if (a.out) a.out[pix] = c_mul(g[u], make_float2(w.x, -w.y));
+      }
+    }
+  }
+}
+
 __global__ void transfer_function_kernel(float2* out, const double* x, const double* y, int nx, int ny, double z,
                                          double k) {
   const int64_t npix = (int64_t)nx * ny;

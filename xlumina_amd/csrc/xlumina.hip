@@ -722,8 +722,14 @@ int xl_pre_backward(const xl_pre_params* p, void* stream) {
   if (int rc = pre_common(p, &a)) return rc;
   if (!p->gout) return fail(XL_EINVAL, "null cotangent");
   xl::KTimer t(xl::K_PRE_BWD, (hipStream_t)stream);
-  hipLaunchKernelGGL(xl::pre_backward_kernel, grid_for((int64_t)p->rows * p->cols, p->batch), dim3(256), 0,
-                     (hipStream_t)stream, a);
+  const int64_t npix = (int64_t)p->rows * p->cols;
+  const bool lone_slm = !p->hna && p->ez_mode == XL_EZ_NONE && p->nin == 1 && p->ncomp == 1 &&
+                        p->mask[0].kind == XL_MASK_SLM && p->mask[1].kind == XL_MASK_NONE && !p->gparam[0][1];
+  if (lone_slm)
+    hipLaunchKernelGGL(xl::pre_slm_backward_kernel, grid_for((npix + xl::kSlmUnroll - 1) / xl::kSlmUnroll, p->batch),
+                       dim3(256), 0, (hipStream_t)stream, a);
+  else
+    hipLaunchKernelGGL(xl::pre_backward_kernel, grid_for(npix, p->batch), dim3(256), 0, (hipStream_t)stream, a);
   return hip_check(hipGetLastError(), "pre_backward launch");
 }
 
