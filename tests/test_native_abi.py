@@ -74,6 +74,36 @@ int main(void) {
     assert int(lines[5]) == N.RsParams.kernel_dz.offset
 
 
+STRUCTS = {"xl_mask_stage": "MaskStage", "xl_rs_params": "RsParams", "xl_czt_params": "CztParams",
+           "xl_pre_params": "PreParams", "xl_mse_params": "MseParams", "xl_adam_params": "AdamParams",
+           "xl_field_loss_params": "FieldLossParams", "xl_small_area_params": "SmallAreaParams",
+           "xl_bs_params": "BsParams", "xl_bs_dots_params": "BsDotsParams", "xl_kernel_time": "KernelTime"}
+
+
+def test_every_struct_field_matches_c():
+    """sizeof and the offset of every field, for every parameter struct of the header."""
+    src = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    assert set(re.findall(r"typedef struct (xl_\w+)", src)) == set(STRUCTS)
+    lines, expect = [], []
+    for cname, pname in STRUCTS.items():
+        cls = getattr(N, pname)
+        lines.append(f'  printf("%zu\\n", sizeof({cname}));')
+        expect.append(C.sizeof(cls))
+        for fname, *_ in cls._fields_:
+            cfield = {"inp": "in"}.get(fname, fname)   # `in` is a Python keyword
+            lines.append(f'  printf("%zu\\n", offsetof({cname}, {cfield}));')
+            expect.append(getattr(cls, fname).offset)
+    code = ("#include <stdio.h>\n#include <stddef.h>\n#include \"xlumina_hip.h\"\nint main(void) {\n"
+            + "\n".join(lines) + "\n  return 0;\n}\n")
+    with tempfile.TemporaryDirectory() as d:
+        c_src = os.path.join(d, "fields.c")
+        exe = os.path.join(d, "fields")
+        open(c_src, "w").write(code)
+        subprocess.run(["gcc", "-std=c99", "-I", os.path.dirname(HEADER), c_src, "-o", exe], check=True)
+        got = [int(v) for v in subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()]
+    assert got == expect
+
+
 def test_host_side_queries(lib):
     assert lib.xl_abi_version() == 4
     assert lib.xl_max_fft_length() == 4096
