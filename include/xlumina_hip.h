@@ -302,6 +302,21 @@ typedef struct xl_bs_dots_params {
 size_t xl_bs_dots_workspace_bytes(int64_t npix);
 int xl_bs_dots(const xl_bs_dots_params* p, void* workspace, size_t workspace_bytes, void* stream);
 
+/* The adjoint of one XL_MASK_SLM stage on one component (xl_pre_backward
+ * with nin = ncomp = 1) fused with the Adam / AdamW step of those phases: per
+ * pixel dL/dphi = Re(conj(g) i E e^{i phi}) stays in registers and the phases
+ * (mask[0].m0, which must be adam->param), m and v (laid out like the phases:
+ * same batch stride, mask_batch_stride) take xl_adam_step's update with
+ * t = count + 1.  count is NOT advanced: call xl_adam_tick once after the
+ * last update of a training step.  adam->grad is ignored; gparam[0][0], if
+ * set, still receives the phase cotangent and out the field cotangent.
+ * What optax.adam(w) after jax.grad of an SLM (optical_elements.py:87-103,
+ * experiments/four_f_optimizer.py:112) computes, without a gradient array in
+ * HBM. */
+int xl_slm_backward_adam(const xl_pre_params* p, const xl_adam_params* adam, void* stream);
+/* count += 1 on the stream (one thread). */
+int xl_adam_tick(int32_t* count, void* stream);
+
 /* transfer_function_RS (wave_optics.py:291-297) on meshgrid(x, y): out is
  * ny x nx complex64. */
 int xl_transfer_function_rs(void* out, const double* x, const double* y, int32_t nx, int32_t ny, double z,

@@ -155,3 +155,22 @@ def test_fourf_config_size_descends(cuda_dev):
     L = np_(opt.launch())
     assert np.isfinite(L).all()
     assert (L[-1] < L[0]).all(), (L[0][:4], L[-1][:4])
+
+
+def test_fused_mask_adjoint_adam_step_matches_unfused(cuda_dev):
+    """step() (mask adjoints fused with the Adam update) against
+    forward_backward() + xl_adam_step over three steps from one state."""
+    from xlumina_amd.fourf import FourFOptimizer
+    kw = dict(batch=3, n=96, half_width=300.0, f=5e3, w0=300.0, steps_per_launch=1, use_graph=False,
+              weight_decay=0.01)
+    fused, plain = FourFOptimizer(cuda_dev, **kw), FourFOptimizer(cuda_dev, **kw)
+    lf = torch.zeros(3, device=cuda_dev)
+    lp = torch.zeros(3, device=cuda_dev)
+    for _ in range(3):
+        fused.step(lf)
+        plain.step_unfused(lp)
+        torch.cuda.synchronize()
+        assert torch.allclose(lf, lp, rtol=1e-6, atol=0)
+    for a, b in ((fused.phi, plain.phi), (fused.m, plain.m), (fused.v, plain.v)):
+        assert float((a - b).abs().max()) <= 1e-6 * float(b.abs().max())
+    assert int(fused.count) == int(plain.count) == 3

@@ -124,6 +124,8 @@ SIGNATURES = {
     "xl_mse_workspace_bytes": (C.c_size_t, [C.c_int64, C.c_int32]),
     "xl_mse_intensity": (C.c_int, [C.POINTER(MseParams), C.c_void_p, C.c_size_t, C.c_void_p]),
     "xl_adam_step": (C.c_int, [C.POINTER(AdamParams), C.c_void_p]),
+    "xl_slm_backward_adam": (C.c_int, [C.POINTER(PreParams), C.POINTER(AdamParams), C.c_void_p]),
+    "xl_adam_tick": (C.c_int, [C.c_void_p, C.c_void_p]),
     "xl_field_loss_workspace_bytes": (C.c_size_t, [C.c_int64, C.c_int32]),
     "xl_field_loss": (C.c_int, [C.POINTER(FieldLossParams), C.c_void_p, C.c_size_t, C.c_void_p]),
     "xl_small_area_workspace_bytes": (C.c_size_t, [C.c_int64, C.c_int32]),

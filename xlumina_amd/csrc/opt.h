@@ -83,35 +83,44 @@ struct AdamArgs {
   int vec;             // all four arrays 16-byte aligned: float4 accesses
 };
 
-__global__ __launch_bounds__(256) void adam_kernel(AdamArgs a) {
+// Bias corrections 1 / (1 - b^t) of step t = count + 1, in float64.
+__device__ __forceinline__ void adam_factors(const AdamArgs& a, float& c1, float& c2) {
   const int t = *a.count + 1;
-  const float c1 = (float)(1.0 / (1.0 - pow((double)a.b1, (double)t)));
-  const float c2 = (float)(1.0 / (1.0 - pow((double)a.b2, (double)t)));
+  c1 = (float)(1.0 / (1.0 - pow((double)a.b1, (double)t)));
+  c2 = (float)(1.0 / (1.0 - pow((double)a.b2, (double)t)));
+}
+
+// One parameter's Adam / AdamW step.
+__device__ __forceinline__ void adam_update(const AdamArgs& a, float c1, float c2, float& p, float g, float& m,
+                                            float& v) {
+  m = a.b1 * m + (1.f - a.b1) * g;
+  v = a.b2 * v + (1.f - a.b2) * g * g;
+  const float mh = m * c1, vh = v * c2;
+  p -= a.lr * (mh / (sqrtf(vh) + a.eps) + a.wd * p);
+}
+
+__global__ __launch_bounds__(256) void adam_kernel(AdamArgs a) {
+  float c1, c2;
+  adam_factors(a, c1, c2);
   const int64_t n4 = a.vec ? a.n >> 2 : 0;
   float4* p4 = (float4*)a.p;
   const float4* g4 = (const float4*)a.g;
   float4* m4 = (float4*)a.m;
   float4* v4 = (float4*)a.v;
-  auto upd = [&](float& p, float g, float& m, float& v) {
-    m = a.b1 * m + (1.f - a.b1) * g;
-    v = a.b2 * v + (1.f - a.b2) * g * g;
-    const float mh = m * c1, vh = v * c2;
-    p -= a.lr * (mh / (sqrtf(vh) + a.eps) + a.wd * p);
-  };
 #pragma unroll 2
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (int64_t)gridDim.x * 256) {
     float4 p = p4[i], m = m4[i], v = v4[i];
     const float4 g = g4[i];
-    upd(p.x, g.x, m.x, v.x);
-    upd(p.y, g.y, m.y, v.y);
-    upd(p.z, g.z, m.z, v.z);
-    upd(p.w, g.w, m.w, v.w);
+    adam_update(a, c1, c2, p.x, g.x, m.x, v.x);
+    adam_update(a, c1, c2, p.y, g.y, m.y, v.y);
+    adam_update(a, c1, c2, p.z, g.z, m.z, v.z);
+    adam_update(a, c1, c2, p.w, g.w, m.w, v.w);
     p4[i] = p;
     m4[i] = m;
     v4[i] = v;
   }
   for (int64_t i = 4 * n4 + (int64_t)blockIdx.x * 256 + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * 256)
-    upd(a.p[i], a.g[i], a.m[i], a.v[i]);
+    adam_update(a, c1, c2, a.p[i], a.g[i], a.m[i], a.v[i]);
 }
 
 // Advances the step counter after every block of adam_kernel has read it.

@@ -9,6 +9,7 @@
 #pragma once
 #include "fft.h"
 #include "masks.h"
+#include "opt.h"
 #include "rs.h"
 
 namespace xl {
@@ -261,6 +262,55 @@ __global__ __launch_bounds__(256) void pre_slm_backward_kernel(PreArgs a0) {
         const float2 o = c_mul(e[u], w);
         if (gp) gp[pix] = re_conj_mul(g[u], make_float2(-o.y, o.x));
         if (a.out) a.out[pix] = c_mul(g[u], make_float2(w.x, -w.y));
+      }
+    }
+  }
+}
+
+// The same adjoint fused with the Adam / AdamW step of the phases it
+// differentiates (the mask's m0, updated in place; m and v laid out like the
+// phases, batch stride mask_bs): the phase cotangent stays in registers, so a
+// training step neither writes nor re-reads a gradient array.  The step uses
+// t = count + 1 and leaves count alone (adam_tick_kernel advances it once per
+// training step).  gparam[0][0], when set, still receives the cotangent.
+__global__ __launch_bounds__(256) void pre_slm_backward_adam_kernel(PreArgs a0, AdamArgs ad) {
+  const PreArgs a = pre_member(a0, blockIdx.y);
+  const int64_t npix = (int64_t)a.rows * a.cols;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  float* ph = const_cast<float*>(a.mask.st[0].m0);
+  float* mb = ad.m + blockIdx.y * a0.mask_bs;
+  float* vb = ad.v + blockIdx.y * a0.mask_bs;
+  float* gp = a.gparam[0][0];
+  float c1, c2;
+  adam_factors(ad, c1, c2);
+  for (int64_t base = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; base < npix; base += kSlmUnroll * stride) {
+    float2 e[kSlmUnroll], g[kSlmUnroll];
+    float q[kSlmUnroll], m[kSlmUnroll], v[kSlmUnroll];
+#pragma unroll
+    for (int u = 0; u < kSlmUnroll; ++u) {
+      const int64_t pix = base + u * stride;
+      if (pix < npix) {
+        e[u] = a.in[pix];
+        g[u] = a.gout[pix];
+        q[u] = ph[pix];
+        m[u] = mb[pix];
+        v[u] = vb[pix];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kSlmUnroll; ++u) {
+      const int64_t pix = base + u * stride;
+      if (pix < npix) {
+        const float2 w = c_expi(q[u]);
+        const float2 o = c_mul(e[u], w);
+        const float gq = re_conj_mul(g[u], make_float2(-o.y, o.x));
+        if (gp) gp[pix] = gq;
+        if (a.out) a.out[pix] = c_mul(g[u], make_float2(w.x, -w.y));
+        float pq = q[u];
+        adam_update(ad, c1, c2, pq, gq, m[u], v[u]);
+        ph[pix] = pq;
+        mb[pix] = m[u];
+        vb[pix] = v[u];
       }
     }
   }

@@ -733,6 +733,45 @@ int xl_pre_backward(const xl_pre_params* p, void* stream) {
   return hip_check(hipGetLastError(), "pre_backward launch");
 }
 
+int xl_slm_backward_adam(const xl_pre_params* p, const xl_adam_params* ad, void* stream) {
+  xl::PreArgs a;
+  if (int rc = pre_common(p, &a)) return rc;
+  if (!p->gout) return fail(XL_EINVAL, "null cotangent");
+  if (!ad) return fail(XL_EINVAL, "null Adam params");
+  if (p->hna || p->ez_mode != XL_EZ_NONE || p->nin != 1 || p->ncomp != 1 || p->mask[0].kind != XL_MASK_SLM ||
+      p->mask[1].kind != XL_MASK_NONE || p->gparam[0][1] || p->gparam[1][0] || p->gparam[1][1])
+    return fail(XL_EINVAL, "the fused SLM adjoint + Adam step takes one component and one XL_MASK_SLM stage");
+  if (ad->param != p->mask[0].m0) return fail(XL_EINVAL, "Adam param must be the SLM phase array (mask[0].m0)");
+  if (!ad->m || !ad->v || !ad->count) return fail(XL_EINVAL, "Adam m, v and count are required");
+  if (!(ad->b1 >= 0.f && ad->b1 < 1.f && ad->b2 >= 0.f && ad->b2 < 1.f)) return fail(XL_EINVAL, "Adam betas must be in [0, 1)");
+  xl::AdamArgs d;
+  d.p = ad->param;
+  d.g = nullptr;
+  d.m = ad->m;
+  d.v = ad->v;
+  d.n = (int64_t)p->rows * p->cols;
+  d.count = ad->count;
+  d.lr = ad->lr;
+  d.b1 = ad->b1;
+  d.b2 = ad->b2;
+  d.eps = ad->eps;
+  d.wd = ad->weight_decay;
+  d.vec = 0;
+  hipStream_t st = (hipStream_t)stream;
+  xl::KTimer t(xl::K_PRE_BWD, st);
+  const int64_t npix = (int64_t)p->rows * p->cols;
+  hipLaunchKernelGGL(xl::pre_slm_backward_adam_kernel,
+                     grid_for((npix + xl::kSlmUnroll - 1) / xl::kSlmUnroll, p->batch), dim3(256), 0, st, a, d);
+  return hip_check(hipGetLastError(), "fused SLM adjoint + Adam launch");
+}
+
+int xl_adam_tick(int32_t* count, void* stream) {
+  if (!count) return fail(XL_EINVAL, "null count");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(xl::adam_tick_kernel, dim3(1), dim3(64), 0, st, count);
+  return hip_check(hipGetLastError(), "Adam tick launch");
+}
+
 size_t xl_mse_workspace_bytes(int64_t npix, int32_t batch) {
   if (npix < 1 || batch < 1) return 0;
   return (size_t)batch * xl::kMseBlocks * sizeof(float) + 256;

@@ -15,10 +15,11 @@ Device work per step, every launch covering all B candidates:
   2. CZT(f) of a2 with SLM phi_b2 fused                                  -> a3
   3. loss and cotangent g3 = 4 (|a3|^2 - T) a3 / n^2 (in place)
   4. adjoint CZT(f) of g3 -> cotangent of the phi_b2-masked field
-  5. SLM adjoint: d loss / d phi_b2 and the cotangent of a2
+  5. SLM adjoint: d loss / d phi_b2 and the cotangent of a2, with the Adam
+     update of phi_b2 fused in (the phase gradient never reaches HBM)
   6. adjoint CZT(2f) -> cotangent of the phi_b1-masked field
-  7. SLM adjoint: d loss / d phi_b1
-  8. Adam over all 2 B n^2 phases.
+  7. SLM adjoint + Adam update of phi_b1
+  8. the Adam step counter advances.
 a1 does not depend on the phases and is computed once; a2 is the only
 activation kept for the backward pass.  `steps_per_launch` steps are captured
 into one HIP graph, so a launch is one graph replay.
@@ -120,6 +121,30 @@ class FourFOptimizer:
                                  gparams=[[self.grad[:, 0], None], [None, None]])
 
     def step(self, loss_out: torch.Tensor) -> None:
+        """One training step.  The mask adjoints run fused with the Adam update
+        of their phases (xl_slm_backward_adam), so no gradient array is written:
+        phi_b2 is updated as soon as its cotangent exists (nothing later in the
+        step reads it), then phi_b1; the step counter advances once at the end.
+        `self.grad` is filled only by forward_backward()."""
+        B, n = self.batch, self.n
+        ph1, ph2 = self.phi[:, 0], self.phi[:, 1]
+        ops.czt(self.a1, self.g_2f, masks=[MaskSpec(N.XL_MASK_SLM, ph1)], batch=B, out=self.a2,
+                workspace=self.ws)
+        ops.czt(self.a2, self.g_f, masks=[MaskSpec(N.XL_MASK_SLM, ph2)], batch=B, out=self.buf_a,
+                workspace=self.ws)
+        fa = self.buf_a.view(B, n, n)
+        ops.mse_intensity(fa, self.target, grad=fa, loss=loss_out, workspace=self.mse_ws)
+        ops.czt(self.buf_a, self.g_f, adjoint=True, batch=B, out=self.buf_b, workspace=self.ws)
+        hp = (self.lr, self.b1, self.b2, self.eps, self.wd)
+        ops.slm_backward_adam(self.a2, self.buf_b, self.x, self.y, B, ph2, self.m[:, 1], self.v[:, 1], self.count,
+                              *hp, out=self.buf_a)
+        ops.czt(self.buf_a, self.g_2f, adjoint=True, batch=B, out=self.buf_b, workspace=self.ws)
+        ops.slm_backward_adam(self.a1, self.buf_b, self.x, self.y, B, ph1, self.m[:, 0], self.v[:, 0], self.count,
+                              *hp)
+        ops.adam_tick(self.count)
+
+    def step_unfused(self, loss_out: torch.Tensor) -> None:
+        """The same step as forward_backward() + one xl_adam_step over all phases."""
         self.forward_backward(loss_out)
         ops.adam_step(self.phi, self.grad, self.m, self.v, self.count, self.lr, self.b1, self.b2, self.eps,
                       self.wd)

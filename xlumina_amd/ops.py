@@ -774,3 +774,57 @@ def bs_dots(gc: Optional[torch.Tensor], gd: Optional[torch.Tensor], a: torch.Ten
     p.dots, p.npix = dots.data_ptr(), npix
     N.check(lib.xl_bs_dots(C.byref(p), ws.data_ptr(), ws.numel(), _stream(dev)))
     return torch.view_as_complex(dots.view(4, 2))
+
+
+def slm_backward_adam(src: torch.Tensor, gout: torch.Tensor, x, y, batch: int, phase: torch.Tensor,
+                      m: torch.Tensor, v: torch.Tensor, count: torch.Tensor, lr: float, b1: float = 0.9,
+                      b2: float = 0.999, eps: float = 1e-8, weight_decay: float = 0.0,
+                      out: Optional[torch.Tensor] = None) -> Optional[torch.Tensor]:
+    """Adjoint of a lone SLM on one component fused with the Adam / AdamW step
+    of its phases (xl_slm_backward_adam).  src: (B, 1, rows, cols) or shared
+    (1, rows, cols); gout: (B, 1, rows, cols); phase, m, v: (B, rows, cols)
+    float32 views with one batch stride and contiguous rows (phase is updated
+    in place).  `count` is read, not advanced (adam_tick).  Returns the field
+    cotangent into `out` (B, 1, rows, cols) when given."""
+    lib = N.load()
+    dev = src.device
+    N.ensure_device(dev)
+    src, gout = src.contiguous(), gout.contiguous()
+    rows, cols = src.shape[-2:]
+    for t in (phase, m, v):
+        if (tuple(t.shape) != (batch, rows, cols) or t.dtype != torch.float32 or t.device != dev
+                or t.stride(2) != 1 or t.stride(1) != cols or t.stride(0) != phase.stride(0)):
+            raise ValueError("phase, m and v must be float32 (B, rows, cols) views with one batch stride")
+    if count.dtype != torch.int32 or count.numel() != 1 or count.device != dev:
+        raise ValueError("count must be a one-element int32 tensor on the field's device")
+    p = N.PreParams()
+    p.inp, p.in_comp_stride, p.nin, p.ncomp = src.data_ptr(), rows * cols, 1, 1
+    p.comp_stride = rows * cols
+    p.rows, p.cols = rows, cols
+    xd, yd = device_grid(x, dev), device_grid(y, dev)
+    p.x, p.y = xd.data_ptr(), yd.data_ptr()
+    p.gout = gout.data_ptr()
+    p.mask[0].kind = N.XL_MASK_SLM
+    p.mask[0].m0 = phase.data_ptr()
+    p.mask[1].kind = N.XL_MASK_NONE
+    p.batch = batch
+    p.mask_batch_stride = phase.stride(0)
+    p.in_batch_stride = 0 if src.dim() == 3 else rows * cols
+    p.gout_batch_stride = rows * cols
+    if out is not None:
+        if tuple(out.shape) != (batch, 1, rows, cols) or not out.is_contiguous():
+            raise ValueError("out must be a contiguous (B, 1, rows, cols) tensor")
+        p.out = out.data_ptr()
+        p.out_batch_stride = rows * cols
+    a = N.AdamParams()
+    a.param, a.grad, a.m, a.v = phase.data_ptr(), None, m.data_ptr(), v.data_ptr()
+    a.n = rows * cols
+    a.count = count.data_ptr()
+    a.lr, a.b1, a.b2, a.eps, a.weight_decay = lr, b1, b2, eps, weight_decay
+    N.check(lib.xl_slm_backward_adam(C.byref(p), C.byref(a), _stream(dev)))
+    return out
+
+
+def adam_tick(count: torch.Tensor) -> None:
+    """count += 1 on the current stream (after the last fused update of a step)."""
+    N.check(N.load().xl_adam_tick(count.data_ptr(), _stream(count.device)))
