@@ -68,8 +68,8 @@ def main():
                     help="also run a 1 GiB device-to-device copy (scale reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the propagation path shards "
-              "independent requests / candidates with no exchange, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); several GPUs would each run "
+              "whole requests / candidates with no exchange between them", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
