@@ -2,8 +2,10 @@
 
 This package restates, in float64 NumPy, the arithmetic of the XLuminA
 reference functions on the propagation path (RS / CZT / VRS / VCZT /
-high-NA objective lens / Jones masks / sources).  It is the checker that the
-HIP kernels are compared against and the CPU baseline that ``bench.py`` times.
+high-NA objective lens / Jones masks / sources / beam splitters, and in
+``losses_ref`` and ``fourf_ref`` the losses and the 4f training step).  The
+HIP kernels are compared against it, and ``bench.py`` times it as the CPU
+baseline.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
 leg may import it.  The product package ``xlumina_amd`` never does: its
@@ -20,6 +22,13 @@ Pinning (what checks this restatement independently of retyping it):
   * High-NA map: orthogonality of the Richards-Wolf 3x3 matrix and its
     phi=0 special case (rotation about y).
   * Sources: closed-form values (peak amplitude, 1/e radius, normalisation).
+  * dH/dz: central differences of the RS kernel in z.
+  * Losses: closed forms (a flat spot's small-area loss is the total
+    intensity; MSE of scaled / rotated fields; softmin limits) and central
+    differences of the hand-derived small-area gradient.
+  * Beam splitters: energy identities ([[R, iT], [iT, R]] scales the energy
+    by R^2 + T^2; BS is lossless for R + T = 1) and the port swap.
+  * 4f step: central differences of the loss, Adam closed forms.
 Parity unpinned: the z<0 branch of the RS kernel (the reference keeps the
 1/r - i k factor unconjugated there) is pinned only by this restatement.
 """
