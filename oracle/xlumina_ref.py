@@ -548,3 +548,9 @@ def BS(a1, a2, R, T, phase):
     b1 = (st * ax + sr * np.exp(1j * phase) * bx, st * ay + sr * np.exp(1j * phase) * by, z)
     b2 = (-sr * np.exp(-1j * phase) * ax + st * bx, -sr * np.exp(-1j * phase) * ay + st * by, z)
     return b1, b2
+
+
+def fluorescence(i_ex, i_dep, beta=1):
+    """optical_elements.py:941-955."""
+    eps = 1e-8
+    return i_ex * (1 - beta * (1 - np.exp(-(i_dep / (i_ex + eps)))))

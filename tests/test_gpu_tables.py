@@ -141,3 +141,99 @@ def test_sharp_focus_table_gradients(cuda_dev):
             fd = (lp - ref_loss(q)) / (2 * h)
             g = float(grads[idx].sum())
         assert abs(g - fd) <= 1e-3 * abs(fd) + 1e-7 * abs(lp), (idx, g, fd)
+
+
+WL_DEP, WL_EX = 0.65, 0.532
+
+
+def oracle_fluorophores(p, offset=8.9):
+    """optical_elements.py:1651-1853 composed from oracle functions."""
+    ang = lambda v: v * 2 * np.pi - np.pi
+    dist = lambda v: (np.abs(v) * 100 + offset) * 1e4
+    red = R.polarized_gaussian_beam(X, Y, WL_DEP, (1200.0, 1200.0), (1, 1))[:2]
+    grn = R.polarized_gaussian_beam(X, Y, WL_EX, (1200.0, 1200.0), (1, 1))[:2]
+    blk = []
+    for i in range(3):
+        q = p[6 * i: 6 * i + 6]
+        blk.append((ang(q[0]), ang(q[1]), float(ang(q[2])[0]), float(ang(q[3])[0]), float(dist(q[4])[0]),
+                    float(dist(q[5])[0])))
+    bs = [float(ang(v)[0]) for v in p[18:27]]
+    z4, z5 = float(dist(p[27])[0]), float(dist(p[28])[0])
+
+    def vrs(e, z, wl):
+        (ex, ey, _), _ = R.vrs_propagation(e[0], e[1], X, Y, wl, z)
+        return ex, ey
+
+    def bb(e, a, ph, z, eta, th, wl):
+        sx, sy, _ = R.sSLM(e[0], e[1], None, a, ph)
+        vx, vy = vrs((sx, sy), z, wl)
+        lx, ly, _ = R.LCD(vx, vy, None, eta, th)
+        return lx, ly
+
+    def bsym(a, b, th):
+        c, d = R.BS_symmetric(a[:2], b[:2], th)
+        return c[:2], d[:2]
+
+    def bsi(a, th, port):
+        c, d = R.BS_symmetric_SI(a[:2], th, port)
+        return c[:2], d[:2]
+
+    def ol(e, wl):
+        return R.vczt_objective_lens(e[0], e[1], X, Y, wl, RL, FL, XO, YO)
+
+    (p11, p12, e1, t1, z11, z12), (p21, p22, e2, t2, z21, z22), (p31, p32, e3, t3, z31, z32) = blk
+    R_, G_ = WL_DEP, WL_EX
+    c1r, d1r = bsi(red, bs[0], 0)
+    c1g, d1g = bsi(grn, bs[0], 1)
+    b2r, b2g = vrs(bb(c1r, p11, p12, z11, e1, t1, R_), z12, R_), vrs(bb(c1g, p11, p12, z11, e1, t1, G_), z12, G_)
+    c2r, d2r = bsi(b2r, bs[1], 1)
+    c2g, d2g = bsym(grn, b2g, bs[1])
+    b3r, b3g = vrs(c2r, z21 + z22, R_), vrs(c2g, z21 + z22, G_)
+    c3r, d3r = bsym(red, b3r, bs[2])
+    c3g, d3g = bsi(b3g, bs[2], 1)
+    det1r, det1g = ol(vrs(c3r, z31 + z32, R_), R_), ol(vrs(c3g, z31 + z32, G_), G_)
+    a5r, a5g, a6r, a6g = vrs(d2r, z4, R_), vrs(d2g, z4, G_), vrs(d3r, z4, R_), vrs(d3g, z4, G_)
+    c4r, d4r = bsym(d1r, red, bs[3])
+    c4g, d4g = bsi(d1g, bs[3], 0)
+    b5r, b5g = vrs(c4r, z11 + z12, R_), vrs(c4g, z11 + z12, G_)
+    c5r, d5r = bsym(a5r, b5r, bs[4])
+    c5g, d5g = bsym(a5g, b5g, bs[4])
+    b6r, b6g = vrs(bb(c5r, p21, p22, z21, e2, t2, R_), z22, R_), vrs(bb(c5g, p21, p22, z21, e2, t2, G_), z22, G_)
+    c6r, d6r = bsym(a6r, b6r, bs[5])
+    c6g, d6g = bsym(a6g, b6g, bs[5])
+    det2r, det2g = ol(vrs(c6r, z31 + z32, R_), R_), ol(vrs(c6g, z31 + z32, G_), G_)
+    a8r, a8g, a9r, a9g = vrs(d5r, z5, R_), vrs(d5g, z5, G_), vrs(d6r, z5, R_), vrs(d6g, z5, G_)
+    c7r, d7r = bsi(d4r, bs[6], 0)
+    c7g, d7g = bsym(d4g, grn, bs[6])
+    b8r, b8g = vrs(c7r, z11 + z12, R_), vrs(c7g, z11 + z12, G_)
+    c8r, d8r = bsym(a8r, b8r, bs[7])
+    c8g, d8g = bsym(a8g, b8g, bs[7])
+    b9r, b9g = vrs(c8r, z21 + z22, R_), vrs(c8g, z21 + z22, G_)
+    c9r, d9r = bsym(a9r, b9r, bs[8])
+    c9g, d9g = bsym(a9g, b9g, bs[8])
+    det3r = ol(vrs(bb(c9r, p31, p32, z31, e3, t3, R_), z32, R_), R_)
+    det3g = ol(vrs(bb(c9g, p31, p32, z31, e3, t3, G_), z32, G_), G_)
+    dep = [det1r, det2r, det3r, ol(d7r, R_), ol(d8r, R_), ol(d9r, R_)]
+    exc = [det1g, det2g, det3g, ol(d7g, G_), ol(d8g, G_), ol(d9g, G_)]
+    norm = lambda ds: np.stack([(np.abs(d[0]) ** 2 + np.abs(d[1]) ** 2) / np.sum(np.abs(d[0]) ** 2 + np.abs(d[1]) ** 2)
+                                for d in ds])
+    return R.fluorescence(norm(exc), norm(dep))
+
+
+def test_sted_table_vs_oracle(cuda_dev):
+    p = raw_parameters(83)
+    ls = []
+    for wl in (WL_DEP, WL_EX):
+        s = PolarizedLightSource(X, Y, wl, cuda_dev)
+        s.gaussian_beam(w0=(1200.0, 1200.0), jones_vector=(1, 1))
+        ls.append(s)
+    red, grn = ls
+    tp = [torch.tensor(v, device=cuda_dev, requires_grad=True) for v in p]
+    got = OE.hybrid_setup_fluorophores(red, grn, red, grn, red, grn, tp, [RL, FL, XO, YO])
+    ref = oracle_fluorophores(p)
+    assert got.shape == (6, M, M)
+    g = got.detach().cpu().numpy()
+    for d in range(6):
+        assert rel_l2(g[d], ref[d]) <= 2e-5, d
+    (gz,) = torch.autograd.grad(got.sum() * 1e3, [tp[27]])     # the whole chain is differentiable
+    assert torch.isfinite(gz).all()

@@ -326,14 +326,7 @@ def hybrid_setup_sharp_focus(ls1, ls2, ls3, ls4, ls5, ls6, parameters, fixed_par
     Returns (intensities (6, My, Mx) = |Ez|^2 per detector, detector lights).
     Every stage runs on the device; gradients reach all 29 parameters."""
     r, f, xout, yout = fixed_params[:4]
-    P = [p if isinstance(p, torch.Tensor) else torch.as_tensor(np.asarray(p, dtype=np.float64)) for p in parameters]
-    blocks = []
-    for i in range(3):
-        ph1, ph2, eta, th, za, zb = P[6 * i: 6 * i + 6]
-        blocks.append((_unit_angle(ph1), _unit_angle(ph2), _unit_angle(eta), _unit_angle(th),
-                       _distance(za, distance_offset), _distance(zb, distance_offset)))
-    bs = [_unit_angle(p) for p in P[18:27]]
-    z4, z5 = _distance(P[27], distance_offset), _distance(P[28], distance_offset)
+    blocks, bs, z4, z5 = _table_parameters(parameters, distance_offset)
     (p11, p12, e1, t1, z11, z12), (p21, p22, e2, t2, z21, z22), (p31, p32, e3, t3, z31, z32) = blocks
 
     def vrs(light, z):
@@ -364,3 +357,81 @@ def hybrid_setup_sharp_focus(ls1, ls2, ls3, ls4, ls5, ls6, parameters, fixed_par
     det_3 = olens(vrs(building_block(c9, p31, p32, z31, e3, t3), z32))
     dets = [det_1, det_2, det_3, olens(d7), olens(d8), olens(d9)]
     return torch.stack([d.Ez.abs() ** 2 for d in dets]), dets
+
+
+def _table_parameters(parameters, distance_offset):
+    """The 29 raw parameters of the 3 x 3 tables (optical_elements.py:1561-1596):
+    three blocks [phase_1, phase_2, eta, theta, z_1, z_2], nine splitter
+    angles, z4, z5, mapped to radians and microns."""
+    P = [p if isinstance(p, torch.Tensor) else torch.as_tensor(np.asarray(p, dtype=np.float64)) for p in parameters]
+    blocks = []
+    for i in range(3):
+        ph1, ph2, eta, th, za, zb = P[6 * i: 6 * i + 6]
+        blocks.append((_unit_angle(ph1), _unit_angle(ph2), _unit_angle(eta), _unit_angle(th),
+                       _distance(za, distance_offset), _distance(zb, distance_offset)))
+    bs = [_unit_angle(p) for p in P[18:27]]
+    return blocks, bs, _distance(P[27], distance_offset), _distance(P[28], distance_offset)
+
+
+def hybrid_setup_fluorophores(ls1, ls2, ls3, ls4, ls5, ls6, parameters, fixed_params, distance_offset=8.9):
+    """optical_elements.py:1651-1853: the 3 x 3 table run for a depletion
+    colour (sources ls1, ls3, ls5) and an excitation colour (ls2, ls4, ls6)
+    through the same elements, single-input splitters where only one colour
+    enters; six high-NA detectors; returns the STED effective intensity
+    fluorescence(I_ex / sum I_ex, I_dep / sum I_dep) per detector (6, My, Mx),
+    with I = |Ex|^2 + |Ey|^2.  Parameters as hybrid_setup_sharp_focus."""
+    r, f, xout, yout = fixed_params[:4]
+    blocks, bs, z4, z5 = _table_parameters(parameters, distance_offset)
+    (p11, p12, e1, t1, z11, z12), (p21, p22, e2, t2, z21, z22), (p31, p32, e3, t3, z31, z32) = blocks
+
+    def vrs(light, z):
+        return light.VRS_propagation(z)[0]
+
+    def olens(light):
+        return VCZT_objective_lens(light, r, f, xout, yout)
+
+    c1_r, d1_r = BS_symmetric_SI(ls1, bs[0], port=0)
+    c1_g, d1_g = BS_symmetric_SI(ls4, bs[0], port=1)
+    b2_r = vrs(building_block(c1_r, p11, p12, z11, e1, t1), z12)
+    b2_g = vrs(building_block(c1_g, p11, p12, z11, e1, t1), z12)
+    c2_r, d2_r = BS_symmetric_SI(b2_r, bs[1], port=1)
+    c2_g, d2_g = BS_symmetric(ls2, b2_g, bs[1])
+    b3_r, b3_g = vrs(c2_r, z21 + z22), vrs(c2_g, z21 + z22)
+    c3_r, d3_r = BS_symmetric(ls3, b3_r, bs[2])
+    c3_g, d3_g = BS_symmetric_SI(b3_g, bs[2], port=1)
+    det1_r, det1_g = olens(vrs(c3_r, z31 + z32)), olens(vrs(c3_g, z31 + z32))
+    a5_r, a5_g = vrs(d2_r, z4), vrs(d2_g, z4)
+    a6_r, a6_g = vrs(d3_r, z4), vrs(d3_g, z4)
+    c4_r, d4_r = BS_symmetric(d1_r, ls5, bs[3])
+    c4_g, d4_g = BS_symmetric_SI(d1_g, bs[3], port=0)
+    b5_r, b5_g = vrs(c4_r, z11 + z12), vrs(c4_g, z11 + z12)
+    c5_r, d5_r = BS_symmetric(a5_r, b5_r, bs[4])
+    c5_g, d5_g = BS_symmetric(a5_g, b5_g, bs[4])
+    b6_r = vrs(building_block(c5_r, p21, p22, z21, e2, t2), z22)
+    b6_g = vrs(building_block(c5_g, p21, p22, z21, e2, t2), z22)
+    c6_r, d6_r = BS_symmetric(a6_r, b6_r, bs[5])
+    c6_g, d6_g = BS_symmetric(a6_g, b6_g, bs[5])
+    det2_r, det2_g = olens(vrs(c6_r, z31 + z32)), olens(vrs(c6_g, z31 + z32))
+    a8_r, a8_g = vrs(d5_r, z5), vrs(d5_g, z5)
+    a9_r, a9_g = vrs(d6_r, z5), vrs(d6_g, z5)
+    c7_r, d7_r = BS_symmetric_SI(d4_r, bs[6], port=0)
+    c7_g, d7_g = BS_symmetric(d4_g, ls6, bs[6])
+    b8_r, b8_g = vrs(c7_r, z11 + z12), vrs(c7_g, z11 + z12)
+    c8_r, d8_r = BS_symmetric(a8_r, b8_r, bs[7])
+    c8_g, d8_g = BS_symmetric(a8_g, b8_g, bs[7])
+    b9_r, b9_g = vrs(c8_r, z21 + z22), vrs(c8_g, z21 + z22)
+    c9_r, d9_r = BS_symmetric(a9_r, b9_r, bs[8])
+    c9_g, d9_g = BS_symmetric(a9_g, b9_g, bs[8])
+    det3_r = olens(vrs(building_block(c9_r, p31, p32, z31, e3, t3), z32))
+    det3_g = olens(vrs(building_block(c9_g, p31, p32, z31, e3, t3), z32))
+    dep = [det1_r, det2_r, det3_r, olens(d7_r), olens(d8_r), olens(d9_r)]
+    exc = [det1_g, det2_g, det3_g, olens(d7_g), olens(d8_g), olens(d9_g)]
+
+    def normalised(dets):
+        out = []
+        for d in dets:
+            i = d.Ex.abs() ** 2 + d.Ey.abs() ** 2
+            out.append(i / i.sum())
+        return torch.stack(out)
+
+    return vectorized_fluorophores(normalised(exc), normalised(dep))
