@@ -282,6 +282,9 @@ typedef struct xl_bs_params {
   int64_t npix;
   int32_t zero_ez;
   int32_t reserved;
+  const float* coef;           /* optional device array (ca, cb, da, db as re, im pairs: 8 floats) read
+                                  by the kernel instead of the values above (no host round trip when
+                                  the coefficients are computed on the device) */
 } xl_bs_params;
 int xl_beam_splitter(const xl_bs_params* p, void* stream);
 

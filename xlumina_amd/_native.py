@@ -92,7 +92,7 @@ class BsParams(C.Structure):
     _fields_ = [("a", C.c_void_p), ("b", C.c_void_p), ("a_comp_stride", C.c_int64), ("b_comp_stride", C.c_int64),
                 ("c", C.c_void_p), ("d", C.c_void_p), ("c_comp_stride", C.c_int64), ("d_comp_stride", C.c_int64),
                 ("ca", C.c_float * 2), ("cb", C.c_float * 2), ("da", C.c_float * 2), ("db", C.c_float * 2),
-                ("npix", C.c_int64), ("zero_ez", C.c_int32), ("reserved", C.c_int32)]
+                ("npix", C.c_int64), ("zero_ez", C.c_int32), ("reserved", C.c_int32), ("coef", C.c_void_p)]
 
 
 class BsDotsParams(C.Structure):

@@ -70,6 +70,7 @@ class RSConfig:
     ez_mode: int
     nin: int
     mask_struct: tuple = field(default_factory=tuple)
+    z_dev: Optional[torch.Tensor] = None   # float64 device scalar read by the kernels instead of z
 
 
 class _RSFunction(torch.autograd.Function):
@@ -78,7 +79,7 @@ class _RSFunction(torch.autograd.Function):
         params = (p00, p01, p10, p11)
         masks = _rebuild(cfg.mask_struct, params)
         out = ops.rs_propagate(src, cfg.x, cfg.y, cfg.z, cfg.wavelength, masks=masks, ez_mode=cfg.ez_mode,
-                               nin=cfg.nin)
+                               nin=cfg.nin, z_dev=cfg.z_dev)
         ctx.cfg = cfg
         ctx.save_for_backward(src, *[p if p is not None else torch.empty(0) for p in params])
         ctx.has = [p is not None for p in params]
@@ -92,7 +93,7 @@ class _RSFunction(torch.autograd.Function):
         src = saved[0]
         params = [saved[1 + i] if ctx.has[i] else None for i in range(4)]
         gout = gout.contiguous()
-        g_pre = ops.rs_propagate(gout, cfg.x, cfg.y, cfg.z, cfg.wavelength, adjoint=True)
+        g_pre = ops.rs_propagate(gout, cfg.x, cfg.y, cfg.z, cfg.wavelength, adjoint=True, z_dev=cfg.z_dev)
         masks = _rebuild(cfg.mask_struct, params)
         need_params = ((ctx.needs_input_grad[2] and params[0] is not None, ctx.needs_input_grad[3] and params[1] is not None),
                        (ctx.needs_input_grad[4] and params[2] is not None, ctx.needs_input_grad[5] and params[3] is not None))
@@ -104,7 +105,8 @@ class _RSFunction(torch.autograd.Function):
             g_src, gp = g_pre, [[None, None], [None, None]]
         else:
             g_src, gp = ops.pre_backward(src, g_pre, cfg.x, cfg.y, cfg.z, masks=masks, ez_mode=cfg.ez_mode,
-                                         nin=cfg.nin, need_src=ctx.needs_input_grad[1], need_params=need_params)
+                                         nin=cfg.nin, need_src=ctx.needs_input_grad[1], need_params=need_params,
+                                         z_dev=cfg.z_dev)
         if g_src is not None and ctx.needs_input_grad[1]:
             if g_src.shape[0] != src.shape[0]:
                 full = torch.zeros_like(src)
@@ -121,16 +123,21 @@ def _rs_z_grad(cfg: RSConfig, src, masks, gout, g_pre):
     (Ex X + Ey Y)/r, r = sqrt(X^2 + Y^2 + z^2) (vectorized_optics.py:262-267),
     through its cotangent g_pre[2]."""
     d = ops.rs_propagate(src, cfg.x, cfg.y, cfg.z, cfg.wavelength, masks=masks, ez_mode=cfg.ez_mode, nin=cfg.nin,
-                         kernel_dz=True)
-    g = torch.real(torch.vdot(d.reshape(-1).to(torch.complex128), gout.reshape(-1).to(torch.complex128)))
+                         kernel_dz=True, z_dev=cfg.z_dev)
+    # complex64 inner products (tree reductions: relative error ~ log2(n) eps),
+    # accumulated as float64 scalars
+    g = torch.real(torch.vdot(d.reshape(-1), gout.reshape(-1))).double()
     if cfg.ez_mode == N.XL_EZ_VRS:
         dev = src.device
         pre = ops.pre_forward(src, cfg.x, cfg.y, cfg.z, masks=masks, ncomp=2, nin=cfg.nin) if masks else src[:2]
-        X = ops.device_grid(cfg.x, dev)[None, :]
-        Y = ops.device_grid(cfg.y, dev)[:, None]
-        r = torch.sqrt(X * X + Y * Y + cfg.z * cfg.z)
-        dez = -(pre[0].to(torch.complex128) * X + pre[1].to(torch.complex128) * Y) * (cfg.z / r ** 3)
-        g = g + torch.real(torch.vdot(dez.reshape(-1), g_pre[2].reshape(-1).to(torch.complex128)))
+        X = ops.device_grid(cfg.x, dev).float()[None, :]
+        Y = ops.device_grid(cfg.y, dev).float()[:, None]
+        z = cfg.z_dev.reshape(()).float() if cfg.z_dev is not None else float(cfg.z)
+        # d Ez_in / dz = -(Ex' X + Ey' Y) z / r^3: weight maps -X z/r^3, -Y z/r^3
+        w = -z * torch.rsqrt(X * X + Y * Y + z * z) ** 3
+        g2 = g_pre[2].conj()
+        q = torch.real(g2 * pre[0]) * (X * w) + torch.real(g2 * pre[1]) * (Y * w)
+        g = g + q.sum(dtype=torch.float64)
     return g
 
 
@@ -141,8 +148,14 @@ def rs(src: torch.Tensor, x, y, z, wavelength: float, masks: Sequence[MaskSpec] 
     dL/dz (convolution with the analytic dH/dz)."""
     nin = src.shape[0] if nin is None else nin
     z_t = z if isinstance(z, torch.Tensor) else None
-    zf = float(z.detach().reshape(-1)[0]) if z_t is not None else float(z)
-    cfg = RSConfig(x, y, zf, float(wavelength), ez_mode, nin, _mask_structure(masks))
+    if z_t is not None:
+        # the kernels read the distance from device memory: no host round trip
+        if z_t.numel() != 1:
+            raise ValueError("the propagation distance must be a single value")
+        z_dev = z_t.detach().to(device=src.device, dtype=torch.float64).reshape(1)
+        cfg = RSConfig(x, y, 0.0, float(wavelength), ez_mode, nin, _mask_structure(masks), z_dev)
+    else:
+        cfg = RSConfig(x, y, float(z), float(wavelength), ez_mode, nin, _mask_structure(masks))
     return _RSFunction.apply(cfg, src, *_flat_params(masks), z_t)
 
 

@@ -19,6 +19,7 @@ struct BsArgs {
   float2 ca, cb, da, db;
   int64_t npix;
   int zero_ez;
+  const float2* coef;  // optional device copy of (ca, cb, da, db)
 };
 
 __device__ __forceinline__ float2 bs_mac(float2 k, float2 v, float2 acc) {
@@ -30,6 +31,8 @@ __device__ __forceinline__ float2 bs_mac(float2 k, float2 v, float2 acc) {
 __global__ __launch_bounds__(256) void bs_kernel(BsArgs a) {
   const int64_t n2 = 2 * a.npix, total = a.zero_ez ? 3 * a.npix : n2;
   const float2 zero = make_float2(0.f, 0.f);
+  const float2 ca = a.coef ? a.coef[0] : a.ca, cb = a.coef ? a.coef[1] : a.cb;
+  const float2 da = a.coef ? a.coef[2] : a.da, db = a.coef ? a.coef[3] : a.db;
   for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
     const int comp = (int)(e / a.npix);
     const int64_t p = e - (int64_t)comp * a.npix;
@@ -40,8 +43,8 @@ __global__ __launch_bounds__(256) void bs_kernel(BsArgs a) {
     }
     const float2 va = a.a[comp * a.acs + p];
     const float2 vb = a.b ? a.b[comp * a.bcs + p] : zero;
-    if (a.c) a.c[comp * a.ccs + p] = bs_mac(a.cb, vb, bs_mac(a.ca, va, zero));
-    if (a.d) a.d[comp * a.dcs + p] = bs_mac(a.db, vb, bs_mac(a.da, va, zero));
+    if (a.c) a.c[comp * a.ccs + p] = bs_mac(cb, vb, bs_mac(ca, va, zero));
+    if (a.d) a.d[comp * a.dcs + p] = bs_mac(db, vb, bs_mac(da, va, zero));
   }
 }
 

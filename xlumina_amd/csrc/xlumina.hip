@@ -921,6 +921,7 @@ int xl_beam_splitter(const xl_bs_params* p, void* stream) {
   a.db = make_float2(p->db[0], p->db[1]);
   a.npix = p->npix;
   a.zero_ez = p->zero_ez ? 1 : 0;
+  a.coef = (const float2*)p->coef;
   hipStream_t st = (hipStream_t)stream;
   xl::KTimer t(xl::K_BS, st);
   const int64_t total = (a.zero_ez ? 3 : 2) * p->npix;

@@ -143,14 +143,33 @@ def _fill_masks_batched(dst, masks, shape, device, batch):
     return keep, (strides.pop() if strides else 0)
 
 
-def quality_factor(x, y, wavelength, z) -> float:
-    """wave_optics.py:187-191 (host float64, as the reference)."""
+_qf_cache: dict = {}
+
+
+def _qf_grid(x, y):
+    """(sqrt(dx^2 + dy^2), sqrt(max x^2 + max y^2)) of a grid pair, cached by content."""
     x = np.asarray(x, dtype=np.float64)
     y = np.asarray(y, dtype=np.float64)
-    dx = x[1] - x[0]
-    dy = y[1] - y[0]
-    dr_real = math.sqrt(dx ** 2 + dy ** 2)
-    rmax = math.sqrt(float((x ** 2).max()) + float((y ** 2).max()))
+    key = (x.tobytes(), y.tobytes())
+    hit = _qf_cache.get(key)
+    if hit is None:
+        dx = x[1] - x[0]
+        dy = y[1] - y[0]
+        hit = (math.sqrt(dx ** 2 + dy ** 2), math.sqrt(float((x ** 2).max()) + float((y ** 2).max())))
+        if len(_qf_cache) > 64:
+            _qf_cache.clear()
+        _qf_cache[key] = hit
+    return hit
+
+
+def quality_factor(x, y, wavelength, z):
+    """wave_optics.py:187-191 in float64: a float for a float z; for a tensor z
+    a tensor on z's device (computed there, so no host synchronisation)."""
+    dr_real, rmax = _qf_grid(x, y)
+    if isinstance(z, torch.Tensor):
+        zt = z.detach().to(torch.float64).reshape(())
+        dr_ideal = torch.sqrt(wavelength ** 2 + rmax ** 2 + 2 * wavelength * torch.sqrt(rmax ** 2 + zt ** 2)) - rmax
+        return dr_ideal / dr_real
     dr_ideal = math.sqrt(wavelength ** 2 + rmax ** 2 + 2 * wavelength * math.sqrt(rmax ** 2 + z ** 2)) - rmax
     return dr_ideal / dr_real
 
@@ -722,8 +741,10 @@ def small_area(intensity: torch.Tensor, want_grad: bool = False, scale: float = 
 def beam_splitter(a: torch.Tensor, b: Optional[torch.Tensor], coef, want_c: bool = True,
                   want_d: bool = True, ncomp_out: int = 3):
     """c = ca a + cb b, d = da a + db b on components 0, 1 of `a`, `b`
-    (k, rows, cols) complex64, k >= 2; coef = (ca, cb, da, db) complex
-    numbers.  Outputs are (ncomp_out, rows, cols) with component 2 (if any) 0."""
+    (k, rows, cols) complex64, k >= 2; coef = (ca, cb, da, db): complex
+    numbers, or a complex tensor of 4 on the GPU (then read by the kernel, with
+    no host synchronisation).  Outputs are (ncomp_out, rows, cols) with
+    component 2 (if any) 0."""
     lib = N.load()
     a = _gpu_c64(a, "a")
     dev = a.device
@@ -743,12 +764,21 @@ def beam_splitter(a: torch.Tensor, b: Optional[torch.Tensor], coef, want_c: bool
     p.b, p.b_comp_stride = _ptr(b), npix
     p.c, p.d = _ptr(c), _ptr(d)
     p.c_comp_stride = p.d_comp_stride = npix
-    for name, k in zip(("ca", "cb", "da", "db"), coef):
-        k = complex(k)
-        getattr(p, name)[0], getattr(p, name)[1] = k.real, k.imag
+    keep = None
+    if isinstance(coef, torch.Tensor) and coef.device.type == "cuda":
+        if coef.numel() != 4 or not coef.is_complex():
+            raise ValueError("device coefficients must be 4 complex numbers (ca, cb, da, db)")
+        keep = coef.to(device=dev, dtype=torch.complex64).contiguous()
+        p.coef = keep.data_ptr()       # read by the kernel: no host round trip
+    else:
+        vals = coef.tolist() if isinstance(coef, torch.Tensor) else coef
+        for name, k in zip(("ca", "cb", "da", "db"), vals):
+            k = complex(k)
+            getattr(p, name)[0], getattr(p, name)[1] = k.real, k.imag
     p.npix = npix
     p.zero_ez = 1 if ncomp_out > 2 else 0
     N.check(lib.xl_beam_splitter(C.byref(p), _stream(dev)))
+    del keep
     return c, d
 
 

@@ -204,13 +204,15 @@ def bb_amplitude_and_phase_mod(input_light: VectorizedLight, alpha, phi, amp1, a
 
 class _BeamSplitter(torch.autograd.Function):
     """c = ca a + cb b, d = da a + db b on (Ex, Ey) with coef = (ca, cb, da, db)
-    (a complex128 CPU tensor that may carry a gradient); Ez of c and d is 0.
+    (a complex128 tensor of 4 that may carry a gradient); Ez of c and d is 0.
     Backward: the same kernel with the conjugate-transposed coefficients, and
-    the coefficient cotangents conj(sum conj(g_out) in) from xl_bs_dots."""
+    the coefficient cotangents conj(sum conj(g_out) in) from xl_bs_dots.
+    Coefficients on the GPU are read by the kernels from device memory, so a
+    table of splitters runs without host synchronisation."""
 
     @staticmethod
     def forward(ctx, a, b, coef):
-        k = [complex(v) for v in coef.detach().tolist()]
+        k = coef.detach()
         c, d = ops.beam_splitter(a.detach(), None if b is None else b.detach(), k)
         ctx.k = k
         ctx.has_b = b is not None
@@ -221,7 +223,7 @@ class _BeamSplitter(torch.autograd.Function):
     def backward(ctx, gc, gd):
         a, b = ctx.saved_tensors
         b = b if ctx.has_b else None
-        ca, cb, da, db = ctx.k
+        k = ctx.k
         gc = gc.contiguous() if gc is not None else None
         gd = gd.contiguous() if gd is not None else None
         z = torch.zeros_like(a) if gc is None or gd is None else None
@@ -230,22 +232,22 @@ class _BeamSplitter(torch.autograd.Function):
         ga = gb = gk = None
         need_b = ctx.has_b and ctx.needs_input_grad[1]
         if ctx.needs_input_grad[0] or need_b:
-            conj = lambda v: v.conjugate()
-            ga, gb = ops.beam_splitter(gcc, gdd, (conj(ca), conj(da), conj(cb), conj(db)), want_c=True, want_d=need_b,
-                                       ncomp_out=a.shape[0])
+            kh = k.conj()[[0, 2, 1, 3]]      # (conj ca, conj da, conj cb, conj db)
+            ga, gb = ops.beam_splitter(gcc, gdd, kh, want_c=True, want_d=need_b, ncomp_out=a.shape[0])
         if ctx.needs_input_grad[2]:
             s = ops.bs_dots(gc, gd, a.detach(), None if b is None else b.detach())
-            gk = s.to(device="cpu", dtype=torch.complex128).conj()
+            gk = s.to(device=k.device, dtype=k.dtype).conj()
         return (ga if ctx.needs_input_grad[0] else None), (gb if need_b else None), gk
 
 
-def _scalar(v):
-    """A one-element parameter (float, 0-d or (1,) tensor) as a 0-d float64 CPU tensor."""
+def _scalar(v, device=None):
+    """A one-element parameter (float, 0-d or (1,) tensor) as a 0-d float64
+    tensor, on the tensor's own device (a float goes to `device`)."""
     if isinstance(v, torch.Tensor):
         if v.numel() != 1:
             raise ValueError("beam-splitter parameters are scalars")
-        return v.to(device="cpu", dtype=torch.float64).reshape(())
-    return torch.tensor(float(np.asarray(v).reshape(-1)[0]), dtype=torch.float64)
+        return v.to(dtype=torch.float64).reshape(())
+    return torch.tensor(float(np.asarray(v).reshape(-1)[0]), dtype=torch.float64, device=device)
 
 
 def _bs_RT(theta):
@@ -268,7 +270,7 @@ def BS_symmetric(a: VectorizedLight, b: VectorizedLight, theta):
     """optical_elements.py:334-392: lossy symmetric splitter [[R, iT], [iT, R]]
     (R = |sin theta|, T = |cos theta| less 1% of T) -> lights at ports c, d."""
     R, T = _bs_RT(theta)
-    zero = torch.zeros((), dtype=torch.float64)
+    zero = torch.zeros((), dtype=torch.float64, device=R.device)
     r, t = torch.complex(R, zero), torch.complex(zero, T)
     return _bs_apply(a, b, torch.stack([r, t, t, r]))
 
@@ -277,7 +279,7 @@ def BS_symmetric_SI(a: VectorizedLight, theta, port=0):
     """optical_elements.py:394-460: single-input splitter; port 0: c = R a,
     d = iT a; port 1: c = iT a, d = R a."""
     R, T = _bs_RT(theta)
-    zero = torch.zeros((), dtype=torch.float64)
+    zero = torch.zeros((), dtype=torch.float64, device=R.device)
     r, t = torch.complex(R, zero), torch.complex(zero, T)
     z = torch.complex(zero, zero)
     coef = torch.stack([r, z, t, z]) if int(port) == 0 else torch.stack([t, z, r, z])
@@ -287,10 +289,11 @@ def BS_symmetric_SI(a: VectorizedLight, theta, port=0):
 def BS(a1: VectorizedLight, a2: VectorizedLight, R, T, phase):
     """optical_elements.py:462-513: [[sqrt T, e^{i phase} sqrt R],
     [-e^{-i phase} sqrt R, sqrt T]] -> lights at ports b1, b2."""
-    sR, sT, ph = torch.sqrt(_scalar(R)), torch.sqrt(_scalar(T)), _scalar(phase)
-    zero = torch.zeros((), dtype=torch.float64)
+    dev = next((v.device for v in (R, T, phase) if isinstance(v, torch.Tensor)), None)
+    sR, sT, ph = torch.sqrt(_scalar(R, dev)), torch.sqrt(_scalar(T, dev)), _scalar(phase, dev)
+    zero = torch.zeros((), dtype=torch.float64, device=sT.device)
     st = torch.complex(sT, zero)
-    e = torch.polar(torch.ones((), dtype=torch.float64), ph)
+    e = torch.polar(torch.ones((), dtype=torch.float64, device=ph.device), ph)
     return _bs_apply(a1, a2, torch.stack([st, sR * e, -sR * e.conj(), st]))
 
 

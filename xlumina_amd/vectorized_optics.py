@@ -35,7 +35,7 @@ class VectorizedLight(_GridMixin):
         self.n = 1
         self.device = torch.device(device) if device is not None else default_device()
         self._X = self._Y = None
-        self._E = torch.zeros((3, len(self.x), len(self.y)), dtype=torch.complex64, device=self.device)
+        self._E: Optional[torch.Tensor] = None          # zeros, allocated on first read
         self._src: Optional[torch.Tensor] = None
         self._masks: Tuple[MaskSpec, ...] = ()
         self.info = "Vectorized light"
@@ -47,6 +47,8 @@ class VectorizedLight(_GridMixin):
         if self._masks:
             self._E = AG.pre(self._src, self.x, self.y, masks=self._masks, nin=self._src.shape[0], ncomp=3)
             self._src, self._masks = None, ()
+        elif self._E is None:
+            self._E = torch.zeros((3, len(self.x), len(self.y)), dtype=torch.complex64, device=self.device)
         return self._E
 
     @E.setter
@@ -66,7 +68,7 @@ class VectorizedLight(_GridMixin):
     def _pending(self):
         if self._masks:
             return self._src, self._masks
-        return self._E, ()
+        return self.E, ()
 
     def _with_mask(self, spec: MaskSpec) -> "VectorizedLight":
         src, masks = self._pending()
@@ -85,7 +87,7 @@ class VectorizedLight(_GridMixin):
         out = AG.rs(src, self.x, self.y, z, self.wavelength, masks=masks, ez_mode=N.XL_EZ_VRS, nin=2)
         light = VectorizedLight(self.x, self.y, self.wavelength, self.device)
         light._E = out
-        qf = ops.quality_factor(self.x, self.y, self.wavelength, _wo._zf(z))
+        qf = ops.quality_factor(self.x, self.y, self.wavelength, z)
         if _wo.VERBOSE:
             print(f"Time taken to perform one VRS propagation (in seconds): {(time.perf_counter() - tic):.4f}")
         return light, qf

@@ -82,7 +82,7 @@ class ScalarLight(_GridMixin):
         self.n = 1
         self.device = torch.device(device) if device is not None else default_device()
         self._X = self._Y = None
-        self._field = torch.zeros((len(self.x), len(self.y)), dtype=torch.complex64, device=self.device)
+        self._field: Optional[torch.Tensor] = None     # zeros, allocated on first read
         self._src: Optional[torch.Tensor] = None
         self._masks: Tuple[MaskSpec, ...] = ()
         self.info = "Wave optics light"
@@ -93,6 +93,8 @@ class ScalarLight(_GridMixin):
         if self._masks:
             self._field = AG.pre(self._src, self.x, self.y, masks=self._masks)[0]
             self._src, self._masks = None, ()
+        elif self._field is None:
+            self._field = torch.zeros((len(self.x), len(self.y)), dtype=torch.complex64, device=self.device)
         return self._field
 
     @field.setter
@@ -104,7 +106,8 @@ class ScalarLight(_GridMixin):
         """(source (1, n, n), mask chain) for a fused propagation."""
         if self._masks:
             return self._src, self._masks
-        return self._field.reshape((1,) + tuple(self._field.shape)), ()
+        f = self.field
+        return f.reshape((1,) + tuple(f.shape)), ()
 
     def _with_mask(self, spec: MaskSpec) -> "ScalarLight":
         src, masks = self._pending()
@@ -122,7 +125,7 @@ class ScalarLight(_GridMixin):
         out = AG.rs(src, self.x, self.y, z, self.wavelength, masks=masks)
         light = ScalarLight(self.x, self.y, self.wavelength, self.device)
         light._field = out[0]
-        qf = ops.quality_factor(self.x, self.y, self.wavelength, _zf(z))
+        qf = ops.quality_factor(self.x, self.y, self.wavelength, z)
         if VERBOSE:
             print(f"Time taken to perform one RS propagation (in seconds): {(time.perf_counter() - tic):.4f}")
         return light, qf
