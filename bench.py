@@ -282,10 +282,19 @@ def run_secondary(dev, workloads, args):
             L.backward()
 
         ms = _time(step, max(5, args.steps // 4), 2)
-        ops.set_czt_form(prev)
         out[f"high_na_fwd_bwd_2048_to_256_{form}"] = {"value": round(1000.0 / ms, 2), "unit": "fwd+bwd steps/s",
                                                       "ms": round(ms, 4)}
         _progress(f"high-NA fwd+bwd {form}: {ms:.4f} ms")
+        # the same step captured in a HIP graph (no launch or Python overhead per step)
+        a_g = h.alpha.clone().requires_grad_(True)
+        p_g = h.phi.clone().requires_grad_(True)
+        cap = workloads.CapturedStep(lambda: h.loss(a_g, p_g, geom), [a_g, p_g])
+        ms = _time(cap.replay, max(5, args.steps // 4), 2)
+        out[f"high_na_fwd_bwd_2048_to_256_{form}_graph"] = {"value": round(1000.0 / ms, 2),
+                                                            "unit": "fwd+bwd steps/s", "ms": round(ms, 4)}
+        del cap, a_g, p_g
+        ops.set_czt_form(prev)
+        _progress(f"high-NA fwd+bwd {form}, graph-captured: {ms:.4f} ms")
     # hybrid optical table of experiments/hybrid_sharp_optical_table.py: 1024^2, 635 nm, 3 building
     # blocks, 9 splitters, 6 high-NA detectors of 400^2 (NA 0.9); loss softmin(small-area losses)
     from xlumina_amd import loss_functions as LF

@@ -305,6 +305,31 @@ typedef struct xl_bs_dots_params {
 size_t xl_bs_dots_workspace_bytes(int64_t npix);
 int xl_bs_dots(const xl_bs_dots_params* p, void* workspace, size_t workspace_bytes, void* stream);
 
+/* Second moment of a focal spot, L = sum_p I_p r2_p / sum_p I_p with
+ * I = sum_c |E_c|^2 over ncomp components (the high-NA focusing loss of the
+ * optimisation benchmark; no counterpart function in the reference, whose
+ * optimisers build losses from jnp reductions), in one fused pass with a
+ * fixed-order reduction.  forward writes stats = (sum I, L) and loss (both
+ * device floats; loss may be NULL); backward writes the field cotangent
+ * grad_c = g 2 (r2 - L) E_c / sum I, with g read from device memory
+ * (`gscale`, the incoming gradient of L; NULL means 1), so neither direction
+ * synchronises with the host. */
+typedef struct xl_moment_params {
+  const void* field;           /* ncomp x npix complex64 */
+  int64_t comp_stride;
+  int32_t ncomp;
+  int32_t reserved;
+  const float* r2;             /* npix weights (u^2 + v^2 of the focal window) */
+  int64_t npix;
+  float* stats;                /* 2 floats: sum I, L */
+  float* loss;                 /* forward: L (device float), or NULL */
+  const float* gscale;         /* backward: incoming gradient of L (device float), or NULL */
+  void* grad;                  /* backward: ncomp x npix complex64 (comp_stride apart) */
+} xl_moment_params;
+size_t xl_moment_workspace_bytes(int64_t npix);
+int xl_moment_forward(const xl_moment_params* p, void* workspace, size_t workspace_bytes, void* stream);
+int xl_moment_backward(const xl_moment_params* p, void* stream);
+
 /* The adjoint of one XL_MASK_SLM stage on one component (xl_pre_backward
  * with nin = ncomp = 1) fused with the Adam / AdamW step of those phases: per
  * pixel dL/dphi = Re(conj(g) i E e^{i phi}) stays in registers and the phases

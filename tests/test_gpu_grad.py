@@ -239,3 +239,33 @@ def test_vrs_grad_wrt_z_vs_fd(cuda_dev):
     eps = 1e-3
     fd = (oracle_loss(z0 + eps) - oracle_loss(z0 - eps)) / (2 * eps)
     assert abs(float(gz) - fd) <= 2e-3 * abs(fd)
+
+
+@pytest.mark.parametrize("form", ["bluestein", "gemm"])
+def test_captured_high_na_step_matches_eager(cuda_dev, form):
+    """A HIP-graph capture of the high-NA forward + backward: every replay
+    (with parameter updates and eager steps in between) matches the eager step."""
+    from xlumina_amd import ops, workloads
+    h = workloads.high_na_2048(cuda_dev, n=256, m_out=64)
+    geom = h.geometry()
+    prev = ops.get_czt_form()
+    ops.set_czt_form(form)
+    try:
+        alpha = h.alpha.clone().requires_grad_(True)
+        phi = h.phi.clone().requires_grad_(True)
+        step = workloads.CapturedStep(lambda: h.loss(alpha, phi, geom), [alpha, phi])
+        for it in range(3):
+            L, (ga, gp) = step.replay()
+            L, ga, gp = float(L), ga.clone(), gp.clone()
+            a2 = alpha.detach().clone().requires_grad_(True)
+            p2 = phi.detach().clone().requires_grad_(True)
+            Le = h.loss(a2, p2, geom)
+            ea, ep = torch.autograd.grad(Le, (a2, p2))
+            torch.cuda.synchronize()
+            assert abs(L - float(Le)) <= 1e-6 * abs(float(Le)), (it, L, float(Le))
+            assert float((ga - ea).norm() / ea.norm()) <= 1e-6 and float((gp - ep).norm() / ep.norm()) <= 1e-6
+            with torch.no_grad():
+                alpha.add_(0.05 * ea.sign())
+                phi.sub_(0.03)
+    finally:
+        ops.set_czt_form(prev)

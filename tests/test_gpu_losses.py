@@ -153,3 +153,23 @@ def test_beam_splitter_gradients(cuda_dev):
     Lr = (torch.tensor(w1) * (cx.abs() ** 2 + cy.abs() ** 2)).sum() + (torch.tensor(w2) * dy.abs() ** 2).sum()
     ra, rb = torch.autograd.grad(Lr, (ta, tb))
     assert rel_l2(g_a.cpu().numpy(), ra.numpy()) <= 1e-5 and rel_l2(g_b.cpu().numpy(), rb.numpy()) <= 1e-5
+
+
+def test_focal_moment_kernel_and_gradient(cuda_dev):
+    """The fused second-moment loss against float64 torch (CPU) and its
+    autograd gradient, with a non-unit incoming gradient."""
+    from xlumina_amd import workloads
+    rng = np.random.default_rng(65)
+    E = cfield(rng, (3, 96, 80))
+    xo, yo = np.linspace(-2, 2, 80), np.linspace(-1.5, 1.5, 96)
+    Et = torch.tensor(E.astype(np.complex64), device=cuda_dev, requires_grad=True)
+    L = workloads.focal_second_moment(Et, xo, yo)
+    (g,) = torch.autograd.grad(2.5 * L, Et)
+    Ec = torch.tensor(E.astype(np.complex64).astype(np.complex128), requires_grad=True)
+    yy, xx = np.meshgrid(yo, xo, indexing="ij")
+    r2 = torch.tensor((xx ** 2 + yy ** 2).astype(np.float32).astype(np.float64))
+    I = (Ec.abs() ** 2).sum(0)
+    Lr = (I * r2).sum() / I.sum()
+    (gr,) = torch.autograd.grad(2.5 * Lr, Ec)
+    assert abs(float(L) - float(Lr)) <= 1e-6 * abs(float(Lr))
+    assert rel_l2(g.cpu().numpy(), gr.numpy()) <= 1e-6

@@ -77,7 +77,8 @@ int main(void) {
 STRUCTS = {"xl_mask_stage": "MaskStage", "xl_rs_params": "RsParams", "xl_czt_params": "CztParams",
            "xl_pre_params": "PreParams", "xl_mse_params": "MseParams", "xl_adam_params": "AdamParams",
            "xl_field_loss_params": "FieldLossParams", "xl_small_area_params": "SmallAreaParams",
-           "xl_bs_params": "BsParams", "xl_bs_dots_params": "BsDotsParams", "xl_kernel_time": "KernelTime"}
+           "xl_bs_params": "BsParams", "xl_bs_dots_params": "BsDotsParams", "xl_kernel_time": "KernelTime",
+           "xl_moment_params": "MomentParams"}
 
 
 def test_every_struct_field_matches_c():

@@ -101,6 +101,12 @@ class BsDotsParams(C.Structure):
                 ("b_comp_stride", C.c_int64), ("dots", C.c_void_p), ("npix", C.c_int64)]
 
 
+class MomentParams(C.Structure):
+    _fields_ = [("field", C.c_void_p), ("comp_stride", C.c_int64), ("ncomp", C.c_int32), ("reserved", C.c_int32),
+                ("r2", C.c_void_p), ("npix", C.c_int64), ("stats", C.c_void_p), ("loss", C.c_void_p),
+                ("gscale", C.c_void_p), ("grad", C.c_void_p)]
+
+
 class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_int64), ("total_ms", C.c_double)]
 
@@ -130,6 +136,9 @@ SIGNATURES = {
     "xl_field_loss": (C.c_int, [C.POINTER(FieldLossParams), C.c_void_p, C.c_size_t, C.c_void_p]),
     "xl_small_area_workspace_bytes": (C.c_size_t, [C.c_int64, C.c_int32]),
     "xl_small_area": (C.c_int, [C.POINTER(SmallAreaParams), C.c_void_p, C.c_size_t, C.c_void_p]),
+    "xl_moment_workspace_bytes": (C.c_size_t, [C.c_int64]),
+    "xl_moment_forward": (C.c_int, [C.POINTER(MomentParams), C.c_void_p, C.c_size_t, C.c_void_p]),
+    "xl_moment_backward": (C.c_int, [C.POINTER(MomentParams), C.c_void_p]),
     "xl_beam_splitter": (C.c_int, [C.POINTER(BsParams), C.c_void_p]),
     "xl_bs_dots_workspace_bytes": (C.c_size_t, [C.c_int64]),
     "xl_bs_dots": (C.c_int, [C.POINTER(BsDotsParams), C.c_void_p, C.c_size_t, C.c_void_p]),

@@ -333,4 +333,90 @@ __global__ __launch_bounds__(kMseThreads) void small_area_grad_kernel(SmallAreaA
   }
 }
 
+// ---------------------------------------------------------------- focal second moment
+struct MomentArgs {
+  const float2* e;
+  int64_t cs;
+  int nc;
+  const float* r2;
+  int64_t npix;
+  float* partial;      // kMseBlocks x 2
+  float* stats;        // sum I, L
+  float* loss;
+  const float* gscale;
+  float2* grad;
+};
+
+// grid kMseBlocks: per block sums of I and I r2 (fixed order).
+__global__ __launch_bounds__(kMseThreads) void moment_kernel(MomentArgs a) {
+  float s0 = 0.f, s1 = 0.f;
+  for (int64_t p = (int64_t)blockIdx.x * kMseThreads + threadIdx.x; p < a.npix;
+       p += (int64_t)gridDim.x * kMseThreads) {
+    float i = 0.f;
+    for (int c = 0; c < a.nc; ++c) {
+      const float2 v = a.e[c * a.cs + p];
+      i += v.x * v.x + v.y * v.y;
+    }
+    s0 += i;
+    s1 += i * a.r2[p];
+  }
+  __shared__ float red[2][kMseThreads / 64];
+  s0 = wave_sum(s0);
+  s1 = wave_sum(s1);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = s0;
+    red[1][threadIdx.x >> 6] = s1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float t0 = 0.f, t1 = 0.f;
+    for (int w = 0; w < kMseThreads / 64; ++w) {
+      t0 += red[0][w];
+      t1 += red[1][w];
+    }
+    a.partial[2 * blockIdx.x] = t0;
+    a.partial[2 * blockIdx.x + 1] = t1;
+  }
+}
+
+// One block: stats = (S, T / S) from the partials, in a fixed order (float64).
+__global__ __launch_bounds__(kMseThreads) void moment_reduce_kernel(MomentArgs a, int nblk) {
+  __shared__ double r0[kMseThreads], r1[kMseThreads];
+  double s0 = 0.0, s1 = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kMseThreads) {
+    s0 += a.partial[2 * i];
+    s1 += a.partial[2 * i + 1];
+  }
+  r0[threadIdx.x] = s0;
+  r1[threadIdx.x] = s1;
+  __syncthreads();
+  for (int w = kMseThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      r0[threadIdx.x] += r0[threadIdx.x + w];
+      r1[threadIdx.x] += r1[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const float L = (float)(r1[0] / r0[0]);
+    a.stats[0] = (float)r0[0];
+    a.stats[1] = L;
+    if (a.loss) a.loss[0] = L;
+  }
+}
+
+// grad_c = g 2 (r2 - L) E_c / S (grid-stride over pixels).
+__global__ __launch_bounds__(256) void moment_grad_kernel(MomentArgs a) {
+  const float g = a.gscale ? a.gscale[0] : 1.f;
+  const float S = a.stats[0], L = a.stats[1];
+  const float k = 2.f * g / S;
+  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < a.npix; p += (int64_t)gridDim.x * 256) {
+    const float w = k * (a.r2[p] - L);
+    for (int c = 0; c < a.nc; ++c) {
+      const float2 v = a.e[c * a.cs + p];
+      a.grad[c * a.cs + p] = make_float2(w * v.x, w * v.y);
+    }
+  }
+}
+
 }  // namespace xl

@@ -901,6 +901,52 @@ int xl_small_area(const xl_small_area_params* p, void* ws, size_t ws_bytes, void
   return hip_check(hipGetLastError(), "small-area launch");
 }
 
+// ---------------------------------------------------------------- focal second moment
+size_t xl_moment_workspace_bytes(int64_t npix) {
+  if (npix < 1) return 0;
+  return (size_t)xl::kMseBlocks * 2 * sizeof(float) + 256;
+}
+
+static int moment_args(const xl_moment_params* p, xl::MomentArgs* a) {
+  if (!p) return fail(XL_EINVAL, "null params");
+  if (!p->field || !p->r2 || !p->stats || p->npix < 1 || p->ncomp < 1 || p->ncomp > 3)
+    return fail(XL_EINVAL, "bad focal-moment arguments");
+  a->e = (const float2*)p->field;
+  a->cs = p->comp_stride;
+  a->nc = p->ncomp;
+  a->r2 = p->r2;
+  a->npix = p->npix;
+  a->partial = nullptr;
+  a->stats = p->stats;
+  a->loss = p->loss;
+  a->gscale = p->gscale;
+  a->grad = (float2*)p->grad;
+  return XL_OK;
+}
+
+int xl_moment_forward(const xl_moment_params* p, void* ws, size_t ws_bytes, void* stream) {
+  xl::MomentArgs a;
+  if (int rc = moment_args(p, &a)) return rc;
+  const size_t need = xl_moment_workspace_bytes(p->npix);
+  if (!ws || ws_bytes < need) return fail(XL_EWORKSPACE, "moment workspace too small: need " + std::to_string(need));
+  a.partial = (float*)align256((char*)ws);
+  hipStream_t st = (hipStream_t)stream;
+  xl::KTimer t(xl::K_LOSS, st);
+  hipLaunchKernelGGL(xl::moment_kernel, dim3(xl::kMseBlocks), dim3(xl::kMseThreads), 0, st, a);
+  hipLaunchKernelGGL(xl::moment_reduce_kernel, dim3(1), dim3(xl::kMseThreads), 0, st, a, xl::kMseBlocks);
+  return hip_check(hipGetLastError(), "moment launch");
+}
+
+int xl_moment_backward(const xl_moment_params* p, void* stream) {
+  xl::MomentArgs a;
+  if (int rc = moment_args(p, &a)) return rc;
+  if (!p->grad) return fail(XL_EINVAL, "null gradient output");
+  hipStream_t st = (hipStream_t)stream;
+  xl::KTimer t(xl::K_LOSS, st);
+  hipLaunchKernelGGL(xl::moment_grad_kernel, grid_for(p->npix), dim3(256), 0, st, a);
+  return hip_check(hipGetLastError(), "moment gradient launch");
+}
+
 // ---------------------------------------------------------------- beam splitters
 int xl_beam_splitter(const xl_bs_params* p, void* stream) {
   if (!p) return fail(XL_EINVAL, "null params");

@@ -858,3 +858,47 @@ def slm_backward_adam(src: torch.Tensor, gout: torch.Tensor, x, y, batch: int, p
 def adam_tick(count: torch.Tensor) -> None:
     """count += 1 on the current stream (after the last fused update of a step)."""
     N.check(N.load().xl_adam_tick(count.data_ptr(), _stream(count.device)))
+
+
+def focal_moment(E: torch.Tensor, r2: torch.Tensor):
+    """L = sum I r2 / sum I, I = sum_c |E_c|^2 (xl_moment_forward): returns
+    (L as a 0-d float32 tensor, stats (sum I, L)) with no host synchronisation."""
+    lib = N.load()
+    E = _gpu_c64(E, "field")
+    dev = E.device
+    N.ensure_device(dev)
+    nc = E.shape[0]
+    npix = E[0].numel()
+    r2 = r2.to(device=dev, dtype=torch.float32).contiguous()
+    if r2.numel() != npix:
+        raise ValueError("r2 must hold one weight per pixel")
+    stats = torch.empty(2, dtype=torch.float32, device=dev)
+    loss = torch.empty(1, dtype=torch.float32, device=dev)
+    ws = torch.empty(lib.xl_moment_workspace_bytes(npix), dtype=torch.uint8, device=dev)
+    p = N.MomentParams()
+    p.field, p.comp_stride, p.ncomp = E.data_ptr(), npix, nc
+    p.r2, p.npix = r2.data_ptr(), npix
+    p.stats, p.loss = stats.data_ptr(), loss.data_ptr()
+    N.check(lib.xl_moment_forward(C.byref(p), ws.data_ptr(), ws.numel(), _stream(dev)))
+    return loss.reshape(()), stats
+
+
+def focal_moment_grad(E: torch.Tensor, r2: torch.Tensor, stats: torch.Tensor,
+                      gscale: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Field cotangent of focal_moment scaled by the device scalar `gscale`."""
+    lib = N.load()
+    E = _gpu_c64(E, "field")
+    dev = E.device
+    nc = E.shape[0]
+    npix = E[0].numel()
+    r2 = r2.to(device=dev, dtype=torch.float32).contiguous()
+    g = None if gscale is None else gscale.to(device=dev, dtype=torch.float32).reshape(1).contiguous()
+    grad = torch.empty_like(E)
+    p = N.MomentParams()
+    p.field, p.comp_stride, p.ncomp = E.data_ptr(), npix, nc
+    p.r2, p.npix = r2.data_ptr(), npix
+    p.stats = stats.data_ptr()
+    p.gscale = _ptr(g)
+    p.grad = grad.data_ptr()
+    N.check(lib.xl_moment_backward(C.byref(p), _stream(dev)))
+    return grad

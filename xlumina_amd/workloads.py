@@ -134,11 +134,37 @@ class HighNA:
         return focal_second_moment(E, self.xout, self.yout)
 
 
+_r2_cache: dict = {}
+
+
+class _FocalMoment(torch.autograd.Function):
+    """sum I r2 / sum I through the fused native reduction and its gradient
+    kernel (the incoming gradient stays on the device)."""
+
+    @staticmethod
+    def forward(ctx, E, r2):
+        loss, stats = ops.focal_moment(E.detach(), r2)
+        ctx.save_for_backward(E, r2, stats)
+        return loss
+
+    @staticmethod
+    def backward(ctx, gl):
+        E, r2, stats = ctx.saved_tensors
+        return ops.focal_moment_grad(E.detach(), r2, stats, gl), None
+
+
 def focal_second_moment(E: torch.Tensor, xo, yo) -> torch.Tensor:
-    I = (E.real ** 2 + E.imag ** 2).sum(0)
-    yy, xx = np.meshgrid(np.asarray(yo), np.asarray(xo), indexing="ij")
-    r2 = torch.from_numpy((xx ** 2 + yy ** 2).astype(np.float32)).to(E.device)
-    return (I * r2).sum() / I.sum()
+    """sum I (u^2 + v^2) / sum I over the focal window, I = sum_c |E_c|^2.  The
+    u^2 + v^2 map is built once per window and device (a host-to-device copy
+    per step would stall the stream)."""
+    xo, yo = np.asarray(xo, dtype=np.float64), np.asarray(yo, dtype=np.float64)
+    key = (xo.tobytes(), yo.tobytes(), str(E.device))
+    r2 = _r2_cache.get(key)
+    if r2 is None:
+        yy, xx = np.meshgrid(yo, xo, indexing="ij")
+        r2 = torch.from_numpy((xx ** 2 + yy ** 2).astype(np.float32)).to(E.device)
+        _r2_cache[key] = r2
+    return _FocalMoment.apply(E.contiguous(), r2)
 
 
 def high_na_2048(device, n: int = 2048, m_out: int = 256) -> HighNA:
@@ -154,3 +180,44 @@ def high_na_2048(device, n: int = 2048, m_out: int = 256) -> HighNA:
     xo = np.linspace(-2.0, 2.0, m_out)
     return HighNA(x, y, 0.65, radius, f, xo, xo.copy(), _c64(E, device), _f32(a, device), _f32(p, device),
                   float(eta), float(th))
+
+
+class CapturedStep:
+    """One forward + backward of `loss_fn()` (a scalar) with respect to
+    `params`, captured into a HIP graph: `replay()` re-runs every kernel of the
+    step with no Python or launch overhead and returns (loss, grads) -- static
+    tensors overwritten by each replay.  The parameters are updated in place
+    between replays by the caller (their storage is what the graph reads).
+
+    Capture follows torch's whole-step recipe: warm-up iterations on a side
+    stream, the parameters' .grad cleared so that the captured backward
+    allocates the gradient buffers from the graph's pool (and a replay
+    overwrites them rather than accumulating)."""
+
+    def __init__(self, loss_fn, params, warmup: int = 3):
+        self.params = list(params)
+        dev = self.params[0].device
+        self.stream = torch.cuda.Stream(device=dev)
+        self.stream.wait_stream(torch.cuda.current_stream(dev))
+        with torch.cuda.stream(self.stream):
+            for _ in range(warmup):
+                for p in self.params:
+                    p.grad = None
+                loss_fn().backward()
+        torch.cuda.current_stream(dev).wait_stream(self.stream)
+        torch.cuda.synchronize(dev)
+        for p in self.params:
+            p.grad = None
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph, stream=self.stream):
+            self.loss = loss_fn()
+            self.loss.backward()
+        self.grads = [p.grad for p in self.params]
+        self.device = dev
+
+    def replay(self):
+        self.stream.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(self.stream):
+            self.graph.replay()
+        torch.cuda.current_stream(self.device).wait_stream(self.stream)
+        return self.loss, self.grads
