@@ -269,6 +269,23 @@ def run_secondary(dev, workloads, args):
     rs = workloads.scalar_rs_1024(dev)
     ms = _time(lambda: rs.run(), max(10, args.steps), 3)
     out["rs_1024_slm"] = {"value": round(1000.0 / ms, 2), "unit": "propagations/s", "ms": round(ms, 4)}
+    # the same propagation replayed from a captured HIP graph over static buffers
+    # (what the serving slots do): no per-call host work
+    o_buf = torch.empty((1, 1024, 1024), dtype=torch.complex64, device=dev)
+    ws = torch.empty(ops.rs_workspace_bytes(1024, 1), dtype=torch.uint8, device=dev)
+    rs.run(out=o_buf, workspace=ws)
+    side = torch.cuda.Stream(device=dev)
+    side.wait_stream(torch.cuda.current_stream(dev))
+    with torch.cuda.stream(side):
+        rs.run(out=o_buf, workspace=ws)
+    torch.cuda.current_stream(dev).wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=side):
+        rs.run(out=o_buf, workspace=ws)
+    ms = _time(graph.replay, max(10, args.steps), 3)
+    out["rs_1024_slm_graph"] = {"value": round(1000.0 / ms, 2), "unit": "propagations/s", "ms": round(ms, 4)}
+    del graph, o_buf, ws
     h = workloads.high_na_2048(dev)
     geom = h.geometry()
     alpha = h.alpha.clone().requires_grad_(True)
