@@ -162,6 +162,55 @@ __global__ __launch_bounds__(3 * (P / 16)) void rs_urow_kernel(RsArgs a) {
   fft_io<P, TPF, false, SRC_LDS, DST_FN>(lds + f * LS, t, none, store, unused);
 }
 
+// One input row per block of P/16 threads and one 35 KB buffer, so several
+// blocks share a CU.  Thread t's first-pass inputs are the columns
+// t + r P/16; the nonzero ones (columns < N <= P/2) are r < 8, so each thread
+// computes the masks and Ez of exactly those 8 pixels once, keeps all
+// components in registers, and the components are transformed one after the
+// other from there (no LDS staging pass).
+template <int P>
+__global__ __launch_bounds__(P / 16) void rs_urow1_kernel(RsArgs a) {
+  constexpr int TPF = P / 16, LS = LdsLen<P>::value;
+  constexpr int R0 = FftPlan<P>::radix(true, 0);
+  static_assert(R0 == 16 && P / R0 == TPF, "first pass: one radix-16 butterfly per thread");
+  constexpr int H = R0 / 2;   // inputs r < H hold the N <= P/2 pixels
+  __shared__ float2 lds[LS];
+  const int t = threadIdx.x;
+  const int row = xcd_remap(blockIdx.x, gridDim.x);
+  const int nrg = (a.n + 3) >> 2, rg = row >> 2, rr = row & 3;
+  const int nc = a.ncomp;
+  const double z = a.z_dev ? *a.z_dev : a.z;
+  float2 e[3][H];
+#pragma unroll
+  for (int r = 0; r < H; ++r) {
+    const int col = t + r * TPF;
+    float2 e0 = make_float2(0.f, 0.f), e1 = e0, e2 = e0;
+    if (row < a.n && col < a.n) {
+      const int64_t pix = (int64_t)row * a.n + col;
+      e0 = a.in[pix];
+      if (a.nin > 1) e1 = a.in[a.in_cs + pix];
+      if (a.nin > 2) e2 = a.in[2 * a.in_cs + pix];
+      apply_mask(a.mask, pix, e0, e1, e2);
+      if (a.ez_mode != EZ_NONE) e2 = build_ez(a.ez_mode, e0, e1, a.x[col], a.y[row], z);
+    }
+    e[0][r] = e0;
+    e[1][r] = e1;
+    e[2][r] = e2;
+  }
+  NoIO none;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (c >= nc) break;
+    Regs<1, R0> rin;
+#pragma unroll
+    for (int r = 0; r < R0; ++r) rin.v[0][r] = r < H ? e[c][r] : make_float2(0.f, 0.f);
+    float2* dst = a.t + (int64_t)c * nrg * 4 * P + rg * 16 + rr * 4;
+    auto store = [&](int col, float2 v) { dst[(int64_t)(col >> 2) * nrg * 16 + (col & 3)] = v; };
+    typename LastPass<P, TPF, true>::Type unused;
+    chain_io<P, TPF, false, true, 0, SRC_REG, DST_FN>(lds, t, none, store, rin, unused);
+  }
+}
+
 // Kernel spectrum.  Per column: the Hr column (rows mirrored, row P/2 zero)
 // is transformed along y and the spectrum, times dx dy / P^2 (conjugated for
 // the adjoint), replaces the Hr column in place for ky < P/2; ky = P/2 goes
@@ -293,13 +342,72 @@ __global__ __launch_bounds__(2 * FftThreads<P>::value) void rs_final_kernel(RsAr
   }
 }
 
+// One output row per block of P/16 threads and one 35 KB buffer, so four
+// blocks share a CU: the components are inverse-transformed one after the
+// other, each read straight from its tiles into the first pass (32 B of each
+// 128 B line; the blocks of the tile's other rows run next on the same XCD),
+// each last pass left in registers, where the first N samples are stored and
+// |E|^2 is accumulated over the components.
+template <int P>
+__global__ __launch_bounds__(P / 16) void rs_final1_kernel(RsArgs a) {
+  constexpr int TPF = P / 16, LS = LdsLen<P>::value;
+  using LP = LastPass<P, TPF, false>;
+  __shared__ float2 lds[LS];
+  const int t = threadIdx.x;
+  const int row = xcd_remap(blockIdx.x, gridDim.x);
+  if (row >= a.n) return;   // whole block
+  const int nrg = (a.n + 3) >> 2, rg = row >> 2, rr = row & 3;
+  // output columns < N <= P/2 are slots q < R/2 of the last pass
+  constexpr int QH = LP::R / 2;
+  float inten[LP::BPT][QH];
+#pragma unroll
+  for (int b = 0; b < LP::BPT; ++b)
+#pragma unroll
+    for (int q = 0; q < QH; ++q) inten[b][q] = 0.f;
+  NoIO none;
+  for (int c = 0; c < a.ncomp; ++c) {
+    const float2* tc = a.t + (int64_t)c * nrg * 4 * P + rg * 16 + rr * 4;
+    auto load = [&](int col) -> float2 { return tc[(int64_t)(col >> 2) * nrg * 16 + (col & 3)]; };
+    typename LP::Type o;
+    fft_io<P, TPF, true, SRC_FN, DST_REG>(lds, t, load, none, o);
+    float2* oc = a.out + (int64_t)c * a.out_cs + (int64_t)row * a.n;
+#pragma unroll
+    for (int b = 0; b < LP::BPT; ++b) {
+      if (!LP::valid(t, b)) continue;
+#pragma unroll
+      for (int q = 0; q < QH; ++q) {
+        const int col = LP::index(t, b, q);
+        if (col < a.n) {
+          const float2 v = o.v[b][q];
+          oc[col] = v;
+          inten[b][q] += v.x * v.x + v.y * v.y;
+        }
+      }
+    }
+  }
+  if (a.intensity) {
+#pragma unroll
+    for (int b = 0; b < LP::BPT; ++b) {
+      if (!LP::valid(t, b)) continue;
+#pragma unroll
+      for (int q = 0; q < QH; ++q) {
+        const int col = LP::index(t, b, q);
+        if (col < a.n) a.intensity[(int64_t)row * a.n + col] = inten[b][q];
+      }
+    }
+  }
+}
+
 template <int P>
 inline hipError_t rs_launch(const RsArgs& a, hipStream_t st) {
   constexpr int CPB = Units<P>::CPB;
   const dim3 blk(CPB * (P / 16));
   const int rows = ((a.n + 3) >> 2) * 4;
   { KTimer t(K_RS_HROW, st); hipLaunchKernelGGL(rs_hrow_kernel<P>, dim3(P / 2 / CPB), blk, 0, st, a); }
-  {
+  if constexpr (P >= 1024) {
+    KTimer t(K_RS_UROW, st);
+    hipLaunchKernelGGL(rs_urow1_kernel<P>, dim3(rows), dim3(P / 16), 0, st, a);
+  } else {
     KTimer t(K_RS_UROW, st);
     static bool attr_set = false;   // up to three 4096-point buffers of dynamic LDS
     if (!attr_set) {
@@ -318,7 +426,10 @@ inline hipError_t rs_launch(const RsArgs& a, hipStream_t st) {
   }
   {
     KTimer t(K_RS_FINAL, st);
-    hipLaunchKernelGGL(rs_final_kernel<P>, dim3(rows / 2), dim3(2 * FftThreads<P>::value), 0, st, a);
+    if constexpr (P >= 1024)
+      hipLaunchKernelGGL(rs_final1_kernel<P>, dim3(a.n), dim3(P / 16), 0, st, a);
+    else
+      hipLaunchKernelGGL(rs_final_kernel<P>, dim3(rows / 2), dim3(2 * FftThreads<P>::value), 0, st, a);
   }
   return hipGetLastError();
 }
