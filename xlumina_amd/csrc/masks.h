@@ -104,12 +104,20 @@ __device__ __forceinline__ void apply_mask(const MaskArgs& m, int64_t pix, float
   apply_stage(m.st[1], pix, e0, e1, e2);
 }
 
+// The direction cosines X/r, Y/r (and z/r) are formed in float32 with one
+// reciprocal square root: they multiply float32 fields, and their relative
+// error (a few float32 ulps) stays far below the field's own rounding budget.
 __device__ __forceinline__ float2 build_ez(int mode, float2 ex, float2 ey, double X, double Y, double z) {
   if (mode == EZ_VRS || mode == EZ_VCZT) {
-    const double r = sqrt(X * X + Y * Y + z * z);
-    double cx = X / r, cy = Y / r;
-    if (mode == EZ_VCZT) { cx *= z / r; cy *= z / r; }
-    return c_add(c_scale(ex, (float)cx), c_scale(ey, (float)cy));
+    const float xf = (float)X, yf = (float)Y, zf = (float)z;
+    const float ir = rsqrtf(fmaf(xf, xf, fmaf(yf, yf, zf * zf)));
+    float cx = xf * ir, cy = yf * ir;
+    if (mode == EZ_VCZT) {
+      const float cz = zf * ir;
+      cx *= cz;
+      cy *= cz;
+    }
+    return c_add(c_scale(ex, cx), c_scale(ey, cy));
   }
   // EZ_HNA: (Ex X + Ey Y) / rho; rho = 0 gives NaN as in the reference
   const double rho = sqrt(X * X + Y * Y);
