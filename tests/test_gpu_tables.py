@@ -237,3 +237,71 @@ def test_sted_table_vs_oracle(cuda_dev):
         assert rel_l2(g[d], ref[d]) <= 2e-5, d
     (gz,) = torch.autograd.grad(got.sum() * 1e3, [tp[27]])     # the whole chain is differentiable
     assert torch.isfinite(gz).all()
+
+
+def oracle_fixed_slms(p, fixed, offset=8.9):
+    """optical_elements.py:1347-1501 composed from oracle functions."""
+    ang = lambda v: float(v[0]) * 2 * np.pi - np.pi
+    dist = lambda v: (abs(float(v[0])) * 100 + offset) * 1e4
+    src = R.polarized_gaussian_beam(X, Y, WL, (1200.0, 1200.0), (1, 1))[:2]
+    z11, z12, z21, z22, z31, z32, z41, z42 = [dist(v) for v in p[0:8]]
+    bs = [ang(v) for v in p[8:17]]
+    e1, t1, e2, t2, e3, t3, e4, t4 = [ang(v) for v in p[17:25]]
+    ph11, ph12, ph2, ph3, ph41, ph42 = fixed
+
+    def vrs(e, z):
+        (ex, ey, _), _ = R.vrs_propagation(e[0], e[1], X, Y, WL, z)
+        return ex, ey
+
+    def bb(e, a, ph, z, eta, th):
+        sx, sy, _ = R.sSLM(e[0], e[1], None, a, ph)
+        vx, vy = vrs((sx, sy), z)
+        lx, ly, _ = R.LCD(vx, vy, None, eta, th)
+        return lx, ly
+
+    def bsym(a, b, th):
+        c, d = R.BS_symmetric(a[:2], b[:2], th)
+        return c[:2], d[:2]
+
+    def ol(e):
+        return R.vczt_objective_lens(e[0], e[1], X, Y, WL, RL, FL, XO, YO)
+
+    c1, d1 = bsym(src, src, bs[0])
+    b2 = vrs(bb(c1, ph11, ph12, z11, e1, t1), z12)
+    c2, d2 = bsym(src, b2, bs[1])
+    b3 = vrs(c2, z21 + z22)
+    c3, d3 = bsym(src, b3, bs[2])
+    det1 = ol(c3)
+    a5 = vrs(d2, z31 + z32)
+    a6 = vrs(bb(d3, ph3, ph3, z31, e3, t3), z32)
+    c4, d4 = bsym(d1, src, bs[3])
+    b5 = vrs(c4, z11 + z12)
+    c5, d5 = bsym(a5, b5, bs[4])
+    b6 = vrs(bb(c5, ph2, ph2, z21, e2, t2), z22)
+    c6, d6 = bsym(a6, b6, bs[5])
+    det2 = ol(c6)
+    a7 = vrs(bb(d4, ph41, ph42, z41, e4, t4), z42)
+    a8, a9 = vrs(d5, z41 + z42), vrs(d6, z41 + z42)
+    c7, d7 = bsym(a7, src, bs[6])
+    b8 = vrs(c7, z11 + z12)
+    c8, d8 = bsym(a8, b8, bs[7])
+    b9 = vrs(c8, z21 + z22)
+    c9, d9 = bsym(a9, b9, bs[8])
+    det3 = ol(c9)
+    dets = [det1, det2, det3, ol(d7), ol(d8), ol(d9)]
+    return np.stack([np.abs(d[2]) ** 2 for d in dets])
+
+
+def test_fixed_slm_table_vs_oracle(cuda_dev):
+    rng = np.random.default_rng(84)
+    p = [rng.uniform(0, 1, 1) for _ in range(25)]
+    fixed = [rng.uniform(-np.pi, np.pi, (N, N)) for _ in range(6)]
+    ls = PolarizedLightSource(X, Y, WL, cuda_dev)
+    ls.gaussian_beam(w0=(1200.0, 1200.0), jones_vector=(1, 1))
+    tp = [torch.tensor(v, device=cuda_dev) for v in p]
+    got, dets = OE.hybrid_setup_fixed_slms(ls, ls, ls, ls, ls, ls, tp, [RL, FL, XO, YO] + fixed)
+    ref = oracle_fixed_slms(p, fixed)
+    g = got.cpu().numpy()
+    assert g.shape == (6, M, M) and len(dets) == 6
+    for d in range(6):
+        assert rel_l2(g[d], ref[d]) <= 2e-5, d

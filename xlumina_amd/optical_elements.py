@@ -438,3 +438,49 @@ def hybrid_setup_fluorophores(ls1, ls2, ls3, ls4, ls5, ls6, parameters, fixed_pa
         return torch.stack(out)
 
     return vectorized_fluorophores(normalised(exc), normalised(dep))
+
+
+def hybrid_setup_fixed_slms(ls1, ls2, ls3, ls4, ls5, ls6, parameters, fixed_params, distance_offset=8.9):
+    """optical_elements.py:1347-1501: the 3 x 3 table with four building
+    blocks whose phase masks are fixed (fixed_params[4:10]: phase1_1,
+    phase1_2, phase2 (both masks), phase3 (both masks), phase4_1, phase4_2,
+    in radians) and 25 trained raw parameters: eight distances z1_1..z4_2,
+    nine splitter angles, four (eta, theta) pairs.  Returns (intensities
+    (6, My, Mx) = |Ez|^2 per detector, detector lights)."""
+    r, f, xout, yout = fixed_params[:4]
+    ph11, ph12, ph2, ph3, ph41, ph42 = fixed_params[4:10]
+    P = [p if isinstance(p, torch.Tensor) else torch.as_tensor(np.asarray(p, dtype=np.float64)) for p in parameters]
+    z11, z12, z21, z22, z31, z32, z41, z42 = [_distance(p, distance_offset) for p in P[0:8]]
+    bs = [_unit_angle(p) for p in P[8:17]]
+    e1, t1, e2, t2, e3, t3, e4, t4 = [_unit_angle(p) for p in P[17:25]]
+
+    def vrs(light, z):
+        return light.VRS_propagation(z)[0]
+
+    def olens(light):
+        return VCZT_objective_lens(light, r, f, xout, yout)
+
+    c1, d1 = BS_symmetric(ls1, ls4, bs[0])
+    b2 = vrs(building_block(c1, ph11, ph12, z11, e1, t1), z12)
+    c2, d2 = BS_symmetric(ls2, b2, bs[1])
+    b3 = vrs(c2, z21 + z22)
+    c3, d3 = BS_symmetric(ls3, b3, bs[2])
+    det_1 = olens(c3)
+    a5 = vrs(d2, z31 + z32)
+    a6 = vrs(building_block(d3, ph3, ph3, z31, e3, t3), z32)
+    c4, d4 = BS_symmetric(d1, ls5, bs[3])
+    b5 = vrs(c4, z11 + z12)
+    c5, d5 = BS_symmetric(a5, b5, bs[4])
+    b6 = vrs(building_block(c5, ph2, ph2, z21, e2, t2), z22)
+    c6, d6 = BS_symmetric(a6, b6, bs[5])
+    det_2 = olens(c6)
+    a7 = vrs(building_block(d4, ph41, ph42, z41, e4, t4), z42)
+    a8, a9 = vrs(d5, z41 + z42), vrs(d6, z41 + z42)
+    c7, d7 = BS_symmetric(a7, ls6, bs[6])
+    b8 = vrs(c7, z11 + z12)
+    c8, d8 = BS_symmetric(a8, b8, bs[7])
+    b9 = vrs(c8, z21 + z22)
+    c9, d9 = BS_symmetric(a9, b9, bs[8])
+    det_3 = olens(c9)
+    dets = [det_1, det_2, det_3, olens(d7), olens(d8), olens(d9)]
+    return torch.stack([d.Ez.abs() ** 2 for d in dets]), dets
