@@ -305,3 +305,82 @@ def test_fixed_slm_table_vs_oracle(cuda_dev):
     assert g.shape == (6, M, M) and len(dets) == 6
     for d in range(6):
         assert rel_l2(g[d], ref[d]) <= 2e-5, d
+
+
+def oracle_robust(p, noise, offset=15.0):
+    """optical_elements.py:964-1117 composed from oracle functions."""
+    n_d, n_s, n_w, n_a = noise
+    ang = lambda v: v * 2 * np.pi - np.pi
+    dist = lambda v: (np.abs(float(v[0])) * 100 + offset) * 1e4
+    src = R.polarized_gaussian_beam(X, Y, WL, (1200.0, 1200.0), (1, 1))[:2]
+    blk, zs = [], []
+    for i in range(3):
+        q = p[8 * i: 8 * i + 8]
+        blk.append((ang(q[0]) + n_s[2 * i], ang(q[1]) + n_s[2 * i + 1], q[2] + n_a[2 * i], q[3] + n_a[2 * i + 1],
+                    float(ang(q[4])[0]) + n_w[2 * i], float(ang(q[5])[0]) + n_w[2 * i + 1]))
+        zs.append((dist(q[6]) + n_d[2 * i], dist(q[7]) + n_d[2 * i + 1]))
+    bs = [float(ang(v)[0]) for v in p[24:33]]
+    z4, z5 = dist(p[33]) + n_d[6], dist(p[34]) + n_d[7]
+
+    def vrs(e, z):
+        (ex, ey, _), _ = R.vrs_propagation(e[0], e[1], X, Y, WL, z)
+        return ex, ey
+
+    def bb(e, i):
+        ph1, ph2, a1, a2, eta, th = blk[i]
+        sx, sy, _ = R.sSLM_with_amplitude(e[0], e[1], None, ph1, ph2, a1, a2)
+        vx, vy = vrs((sx, sy), zs[i][0])
+        lx, ly, _ = R.LCD(vx, vy, None, eta, th)
+        return lx, ly
+
+    def bsym(a, b, th):
+        c, d = R.BS_symmetric(a[:2], b[:2], th)
+        return c[:2], d[:2]
+
+    def ol(e):
+        return R.vczt_objective_lens(e[0], e[1], X, Y, WL, RL, FL, XO, YO)
+
+    (z11, z12), (z21, z22), (z31, z32) = zs
+    c1, d1 = bsym(src, src, bs[0])
+    b2 = vrs(bb(c1, 0), z12)
+    c2, d2 = bsym(src, b2, bs[1])
+    b3 = vrs(c2, z21 + z22)
+    c3, d3 = bsym(src, b3, bs[2])
+    det1 = ol(vrs(c3, z31 + z32))
+    a5, a6 = vrs(d2, z4), vrs(d3, z4)
+    c4, d4 = bsym(d1, src, bs[3])
+    b5 = vrs(c4, z11 + z12)
+    c5, d5 = bsym(a5, b5, bs[4])
+    b6 = vrs(bb(c5, 1), z22)
+    c6, d6 = bsym(a6, b6, bs[5])
+    det2 = ol(vrs(c6, z31 + z32))
+    a8, a9 = vrs(d5, z5), vrs(d6, z5)
+    c7, d7 = bsym(d4, src, bs[6])
+    b8 = vrs(c7, z11 + z12)
+    c8, d8 = bsym(a8, b8, bs[7])
+    b9 = vrs(c8, z21 + z22)
+    c9, d9 = bsym(a9, b9, bs[8])
+    det3 = ol(vrs(bb(c9, 2), z32))
+    dets = [det1, det2, det3, ol(d7), ol(d8), ol(d9)]
+    return np.stack([np.abs(d[2]) ** 2 for d in dets])
+
+
+def test_robust_discovery_table_vs_oracle(cuda_dev):
+    rng = np.random.default_rng(85)
+    p = []
+    for _ in range(3):
+        p += [rng.uniform(0, 1, (N, N)), rng.uniform(0, 1, (N, N)), rng.uniform(0.3, 1, (N, N)),
+              rng.uniform(0.3, 1, (N, N))] + [rng.uniform(0, 1, 1) for _ in range(4)]
+    p += [rng.uniform(0, 1, 1) for _ in range(11)]
+    noise = (rng.uniform(-50, 50, 8), [rng.uniform(-0.05, 0.05, (N, N)) for _ in range(6)],
+             rng.uniform(-0.05, 0.05, 6), [rng.uniform(-0.05, 0.05, (N, N)) for _ in range(6)])
+    ls = PolarizedLightSource(X, Y, WL, cuda_dev)
+    ls.gaussian_beam(w0=(1200.0, 1200.0), jones_vector=(1, 1))
+    tp = [torch.tensor(v, device=cuda_dev) for v in p]
+    n_d, n_s, n_w, n_a = noise
+    got = OE.robust_discovery(ls, ls, ls, ls, ls, ls, tp, [RL, FL, XO, YO], list(n_d), n_s, list(n_w), n_a)
+    ref = oracle_robust(p, noise)
+    g = got.cpu().numpy()
+    assert g.shape == (6, M, M)
+    for d in range(6):
+        assert rel_l2(g[d], ref[d]) <= 2e-5, d

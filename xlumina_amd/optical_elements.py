@@ -318,6 +318,41 @@ def _distance(p, offset):
     return (torch.abs(p) * 100 + offset) * 1e4  # cm -> um
 
 
+def _table_3x3(ls, block, zs, bs, z4, z5, lens):
+    """Topology shared by hybrid_setup_sharp_focus and robust_discovery
+    (optical_elements.py:1603-1649, 1063-1110): sources ls[0..5], block(light, i)
+    the i-th building block (which includes its first propagation), zs[i] =
+    (z_i1, z_i2), nine splitter angles, the mid-space distances z4, z5 and the
+    objective lens map.  Returns the six detector lights."""
+    ls1, ls2, ls3, ls4, ls5, ls6 = ls
+    (z11, z12), (z21, z22), (z31, z32) = zs
+
+    def vrs(light, z):
+        return light.VRS_propagation(z)[0]
+
+    c1, d1 = BS_symmetric(ls1, ls4, bs[0])
+    b2 = vrs(block(c1, 0), z12)
+    c2, d2 = BS_symmetric(ls2, b2, bs[1])
+    b3 = vrs(c2, z21 + z22)
+    c3, d3 = BS_symmetric(ls3, b3, bs[2])
+    det_1 = lens(vrs(c3, z31 + z32))
+    a5, a6 = vrs(d2, z4), vrs(d3, z4)
+    c4, d4 = BS_symmetric(d1, ls5, bs[3])
+    b5 = vrs(c4, z11 + z12)
+    c5, d5 = BS_symmetric(a5, b5, bs[4])
+    b6 = vrs(block(c5, 1), z22)
+    c6, d6 = BS_symmetric(a6, b6, bs[5])
+    det_2 = lens(vrs(c6, z31 + z32))
+    a8, a9 = vrs(d5, z5), vrs(d6, z5)
+    c7, d7 = BS_symmetric(d4, ls6, bs[6])
+    b8 = vrs(c7, z11 + z12)
+    c8, d8 = BS_symmetric(a8, b8, bs[7])
+    b9 = vrs(c8, z21 + z22)
+    c9, d9 = BS_symmetric(a9, b9, bs[8])
+    det_3 = lens(vrs(block(c9, 2), z32))
+    return [det_1, det_2, det_3, lens(d7), lens(d8), lens(d9)]
+
+
 def hybrid_setup_sharp_focus(ls1, ls2, ls3, ls4, ls5, ls6, parameters, fixed_params, distance_offset=8.9):
     """optical_elements.py:1503-1649: the 3 x 3 table of three building blocks
     and nine symmetric beam splitters, six high-NA detectors.
@@ -330,36 +365,53 @@ def hybrid_setup_sharp_focus(ls1, ls2, ls3, ls4, ls5, ls6, parameters, fixed_par
     Every stage runs on the device; gradients reach all 29 parameters."""
     r, f, xout, yout = fixed_params[:4]
     blocks, bs, z4, z5 = _table_parameters(parameters, distance_offset)
-    (p11, p12, e1, t1, z11, z12), (p21, p22, e2, t2, z21, z22), (p31, p32, e3, t3, z31, z32) = blocks
 
-    def vrs(light, z):
-        return light.VRS_propagation(z)[0]
+    def block(light, i):
+        ph1, ph2, eta, th, za, _ = blocks[i]
+        return building_block(light, ph1, ph2, za, eta, th)
 
-    def olens(light):
-        return VCZT_objective_lens(light, r, f, xout, yout)
-
-    c1, d1 = BS_symmetric(ls1, ls4, bs[0])
-    b2 = vrs(building_block(c1, p11, p12, z11, e1, t1), z12)
-    c2, d2 = BS_symmetric(ls2, b2, bs[1])
-    b3 = vrs(c2, z21 + z22)
-    c3, d3 = BS_symmetric(ls3, b3, bs[2])
-    det_1 = olens(vrs(c3, z31 + z32))
-    a5, a6 = vrs(d2, z4), vrs(d3, z4)
-    c4, d4 = BS_symmetric(d1, ls5, bs[3])
-    b5 = vrs(c4, z11 + z12)
-    c5, d5 = BS_symmetric(a5, b5, bs[4])
-    b6 = vrs(building_block(c5, p21, p22, z21, e2, t2), z22)
-    c6, d6 = BS_symmetric(a6, b6, bs[5])
-    det_2 = olens(vrs(c6, z31 + z32))
-    a8, a9 = vrs(d5, z5), vrs(d6, z5)
-    c7, d7 = BS_symmetric(d4, ls6, bs[6])
-    b8 = vrs(c7, z11 + z12)
-    c8, d8 = BS_symmetric(a8, b8, bs[7])
-    b9 = vrs(c8, z21 + z22)
-    c9, d9 = BS_symmetric(a9, b9, bs[8])
-    det_3 = olens(vrs(building_block(c9, p31, p32, z31, e3, t3), z32))
-    dets = [det_1, det_2, det_3, olens(d7), olens(d8), olens(d9)]
+    dets = _table_3x3((ls1, ls2, ls3, ls4, ls5, ls6), block, [(b[4], b[5]) for b in blocks], bs, z4, z5,
+                      lambda light: VCZT_objective_lens(light, r, f, xout, yout))
     return torch.stack([d.Ez.abs() ** 2 for d in dets]), dets
+
+
+def robust_discovery(ls1, ls2, ls3, ls4, ls5, ls6, parameters, fixed_params, noise_distances, noise_slms, noise_wps,
+                     noise_amps, distance_offset=15):
+    """optical_elements.py:964-1117: the sharp-focus table with amplitude-and-
+    phase building blocks (bb_amplitude_and_phase_mod) and additive noise on
+    every physical parameter: noise_slms (6 phase masks), noise_amps (6
+    amplitude masks), noise_wps (6 retarder angles), noise_distances (8:
+    z1_1, z1_2, z2_1, z2_2, z3_1, z3_2, z4, z5).  parameters (35): per block
+    [phase_1, phase_2, A_1, A_2, eta, theta, z_1, z_2], then bs1..bs9, z4, z5.
+    Returns the intensities (6, My, Mx) = |Ez|^2 per detector."""
+    r, f, xout, yout = fixed_params[:4]
+    P = [p if isinstance(p, torch.Tensor) else torch.as_tensor(np.asarray(p, dtype=np.float64)) for p in parameters]
+    dev = P[0].device
+
+    def on(v):   # noise as a tensor next to the parameters
+        return v.to(dev) if isinstance(v, torch.Tensor) else torch.as_tensor(np.asarray(v, dtype=np.float64), device=dev)
+
+    noise_slms, noise_amps = [on(v) for v in noise_slms], [on(v) for v in noise_amps]
+    noise_wps, noise_distances = [on(v) for v in noise_wps], [on(v) for v in noise_distances]
+    blocks, zs = [], []
+    for i in range(3):
+        q = P[8 * i: 8 * i + 8]
+        blocks.append((_unit_angle(q[0]) + noise_slms[2 * i], _unit_angle(q[1]) + noise_slms[2 * i + 1],
+                       q[2] + noise_amps[2 * i], q[3] + noise_amps[2 * i + 1],
+                       _unit_angle(q[4]) + noise_wps[2 * i], _unit_angle(q[5]) + noise_wps[2 * i + 1]))
+        zs.append((_distance(q[6], distance_offset) + noise_distances[2 * i],
+                   _distance(q[7], distance_offset) + noise_distances[2 * i + 1]))
+    bs = [_unit_angle(p) for p in P[24:33]]
+    z4 = _distance(P[33], distance_offset) + noise_distances[6]
+    z5 = _distance(P[34], distance_offset) + noise_distances[7]
+
+    def block(light, i):
+        ph1, ph2, a1, a2, eta, th = blocks[i]
+        return bb_amplitude_and_phase_mod(light, ph1, ph2, a1, a2, zs[i][0], eta, th)
+
+    dets = _table_3x3((ls1, ls2, ls3, ls4, ls5, ls6), block, zs, bs, z4, z5,
+                      lambda light: VCZT_objective_lens(light, r, f, xout, yout))
+    return torch.stack([d.Ez.abs() ** 2 for d in dets])
 
 
 def _table_parameters(parameters, distance_offset):
