@@ -384,3 +384,203 @@ def test_robust_discovery_table_vs_oracle(cuda_dev):
     assert g.shape == (6, M, M)
     for d in range(6):
         assert rel_l2(g[d], ref[d]) <= 2e-5, d
+
+
+def oracle_fixed_sted(p, fixed, offset=8.9):
+    """optical_elements.py:1119-1345 composed from oracle functions, one colour
+    at a time (depletion: ls1, ls3, ls5; excitation: ls2, ls4, ls6)."""
+    ang = lambda v: float(v[0]) * 2 * np.pi - np.pi
+    dist = lambda v: (abs(float(v[0])) * 100 + offset) * 1e4
+    z11, z12, z21, z22, z31, z32, z41, z42 = [dist(v) for v in p[0:8]]
+    bs = [ang(v) for v in p[8:17]]
+    e1, t1, e2, t2, e3, t3, e4, t4 = [ang(v) for v in p[17:25]]
+    ph11, ph12, ph2, ph3, ph41, ph42 = fixed
+
+    def colour(wl, dep):
+        src = R.polarized_gaussian_beam(X, Y, wl, (1200.0, 1200.0), (1, 1))[:2]
+
+        def vrs(e, z):
+            (ex, ey, _), _ = R.vrs_propagation(e[0], e[1], X, Y, wl, z)
+            return ex, ey
+
+        def bb(e, a, ph, z, eta, th):
+            sx, sy, _ = R.sSLM(e[0], e[1], None, a, ph)
+            lx, ly, _ = R.LCD(*vrs((sx, sy), z), None, eta, th)
+            return lx, ly
+
+        def bsym(a, b, th):
+            c, d = R.BS_symmetric(a[:2], b[:2], th)
+            return c[:2], d[:2]
+
+        def bsi(a, th, port):
+            c, d = R.BS_symmetric_SI(a[:2], th, port)
+            return c[:2], d[:2]
+
+        def ol(e):
+            return R.vczt_objective_lens(e[0], e[1], X, Y, wl, RL, FL, XO, YO)
+
+        c1, d1 = bsi(src, bs[0], 0 if dep else 1)
+        b2 = vrs(bb(c1, ph11, ph12, z11, e1, t1), z12)
+        c2, d2 = bsi(b2, bs[1], 1) if dep else bsym(src, b2, bs[1])
+        b3 = vrs(c2, z21 + z22)
+        c3, d3 = bsym(src, b3, bs[2]) if dep else bsi(b3, bs[2], 1)
+        det1 = ol(c3)
+        a5 = vrs(d2, z31 + z32)
+        a6 = vrs(bb(d3, ph3, ph3, z31, e3, t3), z32)
+        c4, d4 = bsym(d1, src, bs[3]) if dep else bsi(d1, bs[3], 0)
+        b5 = vrs(c4, z11 + z12)
+        c5, d5 = bsym(a5, b5, bs[4])
+        b6 = vrs(bb(c5, ph2, ph2, z21, e2, t2), z22)
+        c6, d6 = bsym(a6, b6, bs[5])
+        det2 = ol(c6)
+        a7 = vrs(bb(d4, ph41, ph42, z41, e4, t4), z42)
+        a8, a9 = vrs(d5, z41 + z42), vrs(d6, z41 + z42)
+        c7, d7 = bsi(a7, bs[6], 0) if dep else bsym(a7, src, bs[6])
+        b8 = vrs(c7, z11 + z12)
+        c8, d8 = bsym(a8, b8, bs[7])
+        b9 = vrs(c8, z21 + z22)
+        c9, d9 = bsym(a9, b9, bs[8])
+        det3 = ol(c9)
+        i = [np.abs(d[0]) ** 2 + np.abs(d[1]) ** 2 for d in (det1, det2, det3, ol(d7), ol(d8), ol(d9))]
+        return np.stack([v / v.sum() for v in i])
+
+    return R.fluorescence(colour(WL_EX, False), colour(WL_DEP, True))
+
+
+def two_colour_sources(dev):
+    out = []
+    for wl in (WL_DEP, WL_EX):
+        s = PolarizedLightSource(X, Y, wl, dev)
+        s.gaussian_beam(w0=(1200.0, 1200.0), jones_vector=(1, 1))
+        out.append(s)
+    return out
+
+
+def test_fixed_slm_sted_table_vs_oracle(cuda_dev):
+    rng = np.random.default_rng(86)
+    p = [rng.uniform(0, 1, 1) for _ in range(25)]
+    fixed = [rng.uniform(-np.pi, np.pi, (N, N)) for _ in range(6)]
+    red, grn = two_colour_sources(cuda_dev)
+    tp = [torch.tensor(v, device=cuda_dev, requires_grad=True) for v in p]
+    got = OE.hybrid_setup_fixed_slms_fluorophores(red, grn, red, grn, red, grn, tp, [RL, FL, XO, YO] + fixed)
+    ref = oracle_fixed_sted(p, fixed)
+    g = got.detach().cpu().numpy()
+    assert g.shape == (6, M, M)
+    for d in range(6):
+        assert rel_l2(g[d], ref[d]) <= 2e-5, d
+    (gz,) = torch.autograd.grad(got.sum() * 1e3, [tp[3]])
+    assert torch.isfinite(gz).all()
+
+
+def oracle_six_by_six(p, fixed, offset=7.6):
+    """optical_elements.py:1855-2125 composed from oracle functions, splitter by
+    splitter as the reference lists them."""
+    ang = lambda v: float(v[0]) * 2 * np.pi - np.pi
+    dist = lambda v: (abs(float(v[0])) * 100 + offset) * 1e4
+    z1, z2, z3, z4, z5, z6, z7, z8, z9, z10, z11, z12 = [dist(v) for v in p[0:12]]
+    bs = [ang(v) for v in p[12:48]]
+    e1, t1, e2, t2 = [ang(v) for v in p[48:52]]
+    ph11, ph12, ph21, ph22 = fixed
+    s = R.polarized_gaussian_beam(X, Y, WL, (1200.0, 1200.0), (1, 1))[:2]
+
+    def v(e, z):
+        (ex, ey, _), _ = R.vrs_propagation(e[0], e[1], X, Y, WL, z)
+        return ex, ey
+
+    def bb(e, a, ph, z, eta, th):
+        sx, sy, _ = R.sSLM(e[0], e[1], None, a, ph)
+        lx, ly, _ = R.LCD(*v((sx, sy), z), None, eta, th)
+        return lx, ly
+
+    def B(a, b, k):
+        c, d = R.BS_symmetric(a[:2], b[:2], bs[k - 1])
+        return c[:2], d[:2]
+
+    def ol(e):
+        return R.vczt_objective_lens(e[0], e[1], X, Y, WL, RL, FL, XO, YO)
+
+    c1, d1 = B(s, s, 1); b2 = v(c1, z1); a7 = v(d1, z8)                           # noqa: E702
+    c2, d2 = B(s, b2, 2); b3 = v(c2, z2 + z3); a8 = v(d2, z8)                     # noqa: E702
+    c3, d3 = B(s, b3, 3); b4 = v(c3, z4); a9 = v(d3, z8)                          # noqa: E702
+    c4, d4 = B(s, b4, 4); b5 = v(c4, z5 + z6); a10 = v(d4, z8)                    # noqa: E702
+    c5, d5 = B(s, b5, 5); b6 = v(c5, z7); a11 = v(d5, z8)                         # noqa: E702
+    c6, d6 = B(s, b6, 6); a12 = v(d6, z8); det1 = ol(c6)                          # noqa: E702
+    c7, d7 = B(a7, s, 7); b8 = v(c7, z1); a13 = v(d7, z9)                         # noqa: E702
+    c8, d8 = B(a8, b8, 8); b9 = v(bb(c8, ph11, ph12, z2, e1, t1), z3); a14 = v(d8, z9)  # noqa: E702
+    c9, d9 = B(a9, b9, 9); b10 = v(c9, z4); a15 = v(d9, z9)                       # noqa: E702
+    c10, d10 = B(a10, b10, 10); b11 = v(c10, z5 + z6); a16 = v(d10, z9)           # noqa: E702
+    c11, d11 = B(a11, b11, 11); b12 = v(c11, z7); a17 = v(d11, z9)                # noqa: E702
+    c12, d12 = B(a12, b12, 12); a18 = v(d12, z9); det2 = ol(c12)                  # noqa: E702
+    c13, d13 = B(a13, s, 13); b14 = v(c13, z1); a19 = v(d13, z10)                 # noqa: E702
+    c14, d14 = B(a14, b14, 14); b15 = v(c14, z2 + z3); a20 = v(d14, z10)          # noqa: E702
+    c15, d15 = B(a15, b15, 15); b16 = v(c15, z4); a21 = v(d15, z10)               # noqa: E702
+    c16, d16 = B(a16, b16, 16); b17 = v(c16, z5 + z6); a22 = v(d16, z10)          # noqa: E702
+    c17, d17 = B(a17, b17, 17); b18 = v(c17, z7); a23 = v(d17, z10)               # noqa: E702
+    c18, d18 = B(a18, b18, 18); a24 = v(d18, z10); det3 = ol(c18)                 # noqa: E702
+    c19, d19 = B(a19, s, 19); b20 = v(c19, z1); a25 = v(d19, z11)                 # noqa: E702
+    c20, d20 = B(a20, b20, 20); b21 = v(c20, z2 + z3); a26 = v(d20, z11)          # noqa: E702
+    c21, d21 = B(a21, b21, 21); b22 = v(c21, z4); a27 = v(d21, z11)               # noqa: E702
+    c22, d22 = B(a22, b22, 22); b23 = v(bb(c22, ph21, ph22, z5, e2, t2), z6); a28 = v(d22, z11)  # noqa: E702
+    c23, d23 = B(a23, b23, 23); b24 = v(c23, z7); a29 = v(d23, z11)               # noqa: E702
+    c24, d24 = B(a24, b24, 24); a30 = v(d24, z11); det4 = ol(c24)                 # noqa: E702
+    c25, d25 = B(a25, s, 25); b26 = v(c25, z1); a31 = v(d25, z12)                 # noqa: E702
+    c26, d26 = B(a26, b26, 26); b27 = v(c26, z2 + z3); a32 = v(d26, z12)          # noqa: E702
+    c27, d27 = B(a27, b27, 27); b28 = v(c27, z4); a33 = v(d27, z12)               # noqa: E702
+    c28, d28 = B(a28, b28, 28); b29 = v(c28, z5 + z6); a34 = v(d28, z12)          # noqa: E702
+    c29, d29 = B(a29, b29, 29); b30 = v(c29, z7); a35 = v(d29, z12)               # noqa: E702
+    c30, d30 = B(a30, b30, 30); a36 = v(d30, z12); det5 = ol(c30)                 # noqa: E702
+    c31, d31 = B(a31, s, 31); b32 = v(c31, z1); det7 = ol(c31)                    # noqa: E702
+    c32, d32 = B(a32, b32, 32); b33 = v(c32, z2 + z3); det8 = ol(c32)             # noqa: E702
+    c33, d33 = B(a33, b33, 33); b34 = v(c33, z4); det9 = ol(c33)                  # noqa: E702
+    c34, d34 = B(a34, b34, 34); b35 = v(c34, z5 + z6); det10 = ol(c34)            # noqa: E702
+    c35, d35 = B(a35, b35, 35); b36 = v(c35, z7); det11 = ol(c35)                 # noqa: E702
+    c36, d36 = B(a36, b36, 36); det6 = ol(c36); det12 = ol(d36)                   # noqa: E702
+    dets = [det1, det2, det3, det4, det5, det6, det7, det8, det9, det10, det11, det12]
+    return np.stack([np.abs(d[2]) ** 2 for d in dets])
+
+
+def test_six_by_six_table_vs_oracle(cuda_dev):
+    rng = np.random.default_rng(87)
+    p = [rng.uniform(0, 1, 1) for _ in range(52)]
+    fixed = [rng.uniform(-np.pi, np.pi, (N, N)) for _ in range(4)]
+    ls = PolarizedLightSource(X, Y, WL, cuda_dev)
+    ls.gaussian_beam(w0=(1200.0, 1200.0), jones_vector=(1, 1))
+    tp = [torch.tensor(v, device=cuda_dev, requires_grad=True) for v in p]
+    got = OE.six_times_six_ansatz(*([ls] * 12), tp, [RL, FL, XO, YO] + fixed)
+    ref = oracle_six_by_six(p, fixed)
+    g = got.detach().cpu().numpy()
+    assert g.shape == (12, M, M)
+    for d in range(12):
+        assert rel_l2(g[d], ref[d]) <= 2e-5, d
+    gz = torch.autograd.grad(got.sum() * 1e3, [tp[7], tp[20], tp[48]])   # z8, bs9, eta1
+    assert all(torch.isfinite(v).all() for v in gz)
+
+
+def test_noise_aware_batch_vs_oracle(cuda_dev):
+    """Two tables of robust_discovery with shake_setup noise (batch over the
+    noise axis, noisy_optimization.ipynb cell 7) against the oracle table per
+    noise draw; the experiment's batch loss (cell 11) on top."""
+    rng = np.random.default_rng(88)
+    p = []
+    for _ in range(3):
+        p += [rng.uniform(0, 1, (N, N)), rng.uniform(0, 1, (N, N)), rng.uniform(0.3, 1, (N, N)),
+              rng.uniform(0.3, 1, (N, N))] + [rng.uniform(0, 1, 1) for _ in range(4)]
+    p += [rng.uniform(0, 1, 1) for _ in range(11)]
+    NS = {'n_tables': 2, 'number of distances': 8, 'number of sSLM': 3, 'number of wps': 3,
+          'noise_level': 'low', 'discretize': False}
+    nd, nsl, nw, na, _, _ = OE.batch_shake_setup([5, 6], N, NS, device=cuda_dev)
+    ls = PolarizedLightSource(X, Y, WL, cuda_dev)
+    ls.gaussian_beam(w0=(1200.0, 1200.0), jones_vector=(1, 1))
+    tp = [torch.tensor(v, device=cuda_dev) for v in p]
+    got = OE.batch_robust_discovery(ls, ls, ls, ls, ls, ls, tp, [RL, FL, XO, YO], nd, nsl, nw, na)
+    assert got.shape == (2, 6, M, M)
+    g = got.cpu().numpy()
+    refs = []
+    for t in range(2):
+        noise = (nd[t].cpu().numpy(), list(nsl[t].cpu().numpy()), nw[t].cpu().numpy(), list(na[t].cpu().numpy()))
+        refs.append(oracle_robust(p, noise))
+        for d in range(6):
+            assert rel_l2(g[t, d], refs[t][d]) <= 2e-5, (t, d)
+    loss = softmin(torch.stack([LF.vectorized_loss_hybrid(got[t]) for t in range(2)]).mean(0, keepdim=True))
+    ref_loss = LR.softmin(np.mean([LR.vectorized_loss_hybrid(r) for r in refs], axis=0, keepdims=True))
+    assert abs(float(loss) - float(ref_loss)) <= 1e-4 * abs(float(ref_loss))

@@ -318,10 +318,36 @@ def _distance(p, offset):
     return (torch.abs(p) * 100 + offset) * 1e4  # cm -> um
 
 
+def _bs_pair(a, b, theta):
+    """BS_symmetric(a, b); where one input port carries no light (a source the
+    colour of a two-colour table does not have) the single-input splitter fed
+    at the other port, as optical_elements.py:1741-1828 writes it."""
+    if a is None:
+        return BS_symmetric_SI(b, theta, port=1)
+    if b is None:
+        return BS_symmetric_SI(a, theta, port=0)
+    return BS_symmetric(a, b, theta)
+
+
+def _as_tensors(parameters):
+    return [p if isinstance(p, torch.Tensor) else torch.as_tensor(np.asarray(p, dtype=np.float64))
+            for p in parameters]
+
+
+def _normalised_intensity(dets):
+    """(6, My, Mx) of (|Ex|^2 + |Ey|^2) / sum per detector."""
+    out = []
+    for d in dets:
+        i = d.Ex.abs() ** 2 + d.Ey.abs() ** 2
+        out.append(i / i.sum())
+    return torch.stack(out)
+
+
 def _table_3x3(ls, block, zs, bs, z4, z5, lens):
-    """Topology shared by hybrid_setup_sharp_focus and robust_discovery
-    (optical_elements.py:1603-1649, 1063-1110): sources ls[0..5], block(light, i)
-    the i-th building block (which includes its first propagation), zs[i] =
+    """Topology shared by hybrid_setup_sharp_focus, robust_discovery and
+    hybrid_setup_fluorophores (optical_elements.py:1594-1634, 1062-1102,
+    1741-1828): sources ls[0..5] (None where a colour has no source), block(light,
+    i) the i-th building block (which includes its first propagation), zs[i] =
     (z_i1, z_i2), nine splitter angles, the mid-space distances z4, z5 and the
     objective lens map.  Returns the six detector lights."""
     ls1, ls2, ls3, ls4, ls5, ls6 = ls
@@ -330,21 +356,21 @@ def _table_3x3(ls, block, zs, bs, z4, z5, lens):
     def vrs(light, z):
         return light.VRS_propagation(z)[0]
 
-    c1, d1 = BS_symmetric(ls1, ls4, bs[0])
+    c1, d1 = _bs_pair(ls1, ls4, bs[0])
     b2 = vrs(block(c1, 0), z12)
-    c2, d2 = BS_symmetric(ls2, b2, bs[1])
+    c2, d2 = _bs_pair(ls2, b2, bs[1])
     b3 = vrs(c2, z21 + z22)
-    c3, d3 = BS_symmetric(ls3, b3, bs[2])
+    c3, d3 = _bs_pair(ls3, b3, bs[2])
     det_1 = lens(vrs(c3, z31 + z32))
     a5, a6 = vrs(d2, z4), vrs(d3, z4)
-    c4, d4 = BS_symmetric(d1, ls5, bs[3])
+    c4, d4 = _bs_pair(d1, ls5, bs[3])
     b5 = vrs(c4, z11 + z12)
     c5, d5 = BS_symmetric(a5, b5, bs[4])
     b6 = vrs(block(c5, 1), z22)
     c6, d6 = BS_symmetric(a6, b6, bs[5])
     det_2 = lens(vrs(c6, z31 + z32))
     a8, a9 = vrs(d5, z5), vrs(d6, z5)
-    c7, d7 = BS_symmetric(d4, ls6, bs[6])
+    c7, d7 = _bs_pair(d4, ls6, bs[6])
     b8 = vrs(c7, z11 + z12)
     c8, d8 = BS_symmetric(a8, b8, bs[7])
     b9 = vrs(c8, z21 + z22)
@@ -385,7 +411,7 @@ def robust_discovery(ls1, ls2, ls3, ls4, ls5, ls6, parameters, fixed_params, noi
     [phase_1, phase_2, A_1, A_2, eta, theta, z_1, z_2], then bs1..bs9, z4, z5.
     Returns the intensities (6, My, Mx) = |Ez|^2 per detector."""
     r, f, xout, yout = fixed_params[:4]
-    P = [p if isinstance(p, torch.Tensor) else torch.as_tensor(np.asarray(p, dtype=np.float64)) for p in parameters]
+    P = _as_tensors(parameters)
     dev = P[0].device
 
     def on(v):   # noise as a tensor next to the parameters
@@ -437,59 +463,74 @@ def hybrid_setup_fluorophores(ls1, ls2, ls3, ls4, ls5, ls6, parameters, fixed_pa
     with I = |Ex|^2 + |Ey|^2.  Parameters as hybrid_setup_sharp_focus."""
     r, f, xout, yout = fixed_params[:4]
     blocks, bs, z4, z5 = _table_parameters(parameters, distance_offset)
-    (p11, p12, e1, t1, z11, z12), (p21, p22, e2, t2, z21, z22), (p31, p32, e3, t3, z31, z32) = blocks
+    zs = [(b[4], b[5]) for b in blocks]
 
-    def vrs(light, z):
-        return light.VRS_propagation(z)[0]
+    def block(light, i):
+        ph1, ph2, eta, th, za, _ = blocks[i]
+        return building_block(light, ph1, ph2, za, eta, th)
 
     def olens(light):
         return VCZT_objective_lens(light, r, f, xout, yout)
 
-    c1_r, d1_r = BS_symmetric_SI(ls1, bs[0], port=0)
-    c1_g, d1_g = BS_symmetric_SI(ls4, bs[0], port=1)
-    b2_r = vrs(building_block(c1_r, p11, p12, z11, e1, t1), z12)
-    b2_g = vrs(building_block(c1_g, p11, p12, z11, e1, t1), z12)
-    c2_r, d2_r = BS_symmetric_SI(b2_r, bs[1], port=1)
-    c2_g, d2_g = BS_symmetric(ls2, b2_g, bs[1])
-    b3_r, b3_g = vrs(c2_r, z21 + z22), vrs(c2_g, z21 + z22)
-    c3_r, d3_r = BS_symmetric(ls3, b3_r, bs[2])
-    c3_g, d3_g = BS_symmetric_SI(b3_g, bs[2], port=1)
-    det1_r, det1_g = olens(vrs(c3_r, z31 + z32)), olens(vrs(c3_g, z31 + z32))
-    a5_r, a5_g = vrs(d2_r, z4), vrs(d2_g, z4)
-    a6_r, a6_g = vrs(d3_r, z4), vrs(d3_g, z4)
-    c4_r, d4_r = BS_symmetric(d1_r, ls5, bs[3])
-    c4_g, d4_g = BS_symmetric_SI(d1_g, bs[3], port=0)
-    b5_r, b5_g = vrs(c4_r, z11 + z12), vrs(c4_g, z11 + z12)
-    c5_r, d5_r = BS_symmetric(a5_r, b5_r, bs[4])
-    c5_g, d5_g = BS_symmetric(a5_g, b5_g, bs[4])
-    b6_r = vrs(building_block(c5_r, p21, p22, z21, e2, t2), z22)
-    b6_g = vrs(building_block(c5_g, p21, p22, z21, e2, t2), z22)
-    c6_r, d6_r = BS_symmetric(a6_r, b6_r, bs[5])
-    c6_g, d6_g = BS_symmetric(a6_g, b6_g, bs[5])
-    det2_r, det2_g = olens(vrs(c6_r, z31 + z32)), olens(vrs(c6_g, z31 + z32))
-    a8_r, a8_g = vrs(d5_r, z5), vrs(d5_g, z5)
-    a9_r, a9_g = vrs(d6_r, z5), vrs(d6_g, z5)
-    c7_r, d7_r = BS_symmetric_SI(d4_r, bs[6], port=0)
-    c7_g, d7_g = BS_symmetric(d4_g, ls6, bs[6])
-    b8_r, b8_g = vrs(c7_r, z11 + z12), vrs(c7_g, z11 + z12)
-    c8_r, d8_r = BS_symmetric(a8_r, b8_r, bs[7])
-    c8_g, d8_g = BS_symmetric(a8_g, b8_g, bs[7])
-    b9_r, b9_g = vrs(c8_r, z21 + z22), vrs(c8_g, z21 + z22)
-    c9_r, d9_r = BS_symmetric(a9_r, b9_r, bs[8])
-    c9_g, d9_g = BS_symmetric(a9_g, b9_g, bs[8])
-    det3_r = olens(vrs(building_block(c9_r, p31, p32, z31, e3, t3), z32))
-    det3_g = olens(vrs(building_block(c9_g, p31, p32, z31, e3, t3), z32))
-    dep = [det1_r, det2_r, det3_r, olens(d7_r), olens(d8_r), olens(d9_r)]
-    exc = [det1_g, det2_g, det3_g, olens(d7_g), olens(d8_g), olens(d9_g)]
+    dep = _table_3x3((ls1, None, ls3, None, ls5, None), block, zs, bs, z4, z5, olens)
+    exc = _table_3x3((None, ls2, None, ls4, None, ls6), block, zs, bs, z4, z5, olens)
+    return vectorized_fluorophores(_normalised_intensity(exc), _normalised_intensity(dep))
 
-    def normalised(dets):
-        out = []
-        for d in dets:
-            i = d.Ex.abs() ** 2 + d.Ey.abs() ** 2
-            out.append(i / i.sum())
-        return torch.stack(out)
 
-    return vectorized_fluorophores(normalised(exc), normalised(dep))
+def _fixed_parameters(parameters, fixed_params, distance_offset):
+    """The 25 raw parameters of the fixed-mask tables (optical_elements.py:1174-1215,
+    1405-1446): eight distances z1_1..z4_2, nine splitter angles, four (eta,
+    theta) pairs; the fixed masks fixed_params[4:10] are phase1_1, phase1_2,
+    phase2 (both masks), phase3 (both masks), phase4_1, phase4_2.  Returns the
+    per-block (phase_1, phase_2, eta, theta, z_1, z_2) and the splitter angles."""
+    ph11, ph12, ph2, ph3, ph41, ph42 = fixed_params[4:10]
+    P = _as_tensors(parameters)
+    z = [_distance(p, distance_offset) for p in P[0:8]]
+    bs = [_unit_angle(p) for p in P[8:17]]
+    wp = [_unit_angle(p) for p in P[17:25]]
+    masks = ((ph11, ph12), (ph2, ph2), (ph3, ph3), (ph41, ph42))
+    blocks = [(masks[i][0], masks[i][1], wp[2 * i], wp[2 * i + 1], z[2 * i], z[2 * i + 1]) for i in range(4)]
+    return blocks, bs
+
+
+def _table_fixed(ls, blocks, bs, lens):
+    """Topology of the fixed-mask tables (optical_elements.py:1448-1486,
+    1217-1321): four building blocks placed on the first row, the second row,
+    above the third splitter column and below the first; sources ls[0..5]
+    (None where a colour has no source).  Returns the six detector lights."""
+    ls1, ls2, ls3, ls4, ls5, ls6 = ls
+    (z11, z12), (z21, z22), (z31, z32), (z41, z42) = [(b[4], b[5]) for b in blocks]
+
+    def vrs(light, z):
+        return light.VRS_propagation(z)[0]
+
+    def block(light, i):
+        ph1, ph2, eta, th, za, zb = blocks[i]
+        return vrs(building_block(light, ph1, ph2, za, eta, th), zb)
+
+    c1, d1 = _bs_pair(ls1, ls4, bs[0])
+    b2 = block(c1, 0)
+    c2, d2 = _bs_pair(ls2, b2, bs[1])
+    b3 = vrs(c2, z21 + z22)
+    c3, d3 = _bs_pair(ls3, b3, bs[2])
+    det_1 = lens(c3)
+    a5 = vrs(d2, z31 + z32)
+    a6 = block(d3, 2)
+    c4, d4 = _bs_pair(d1, ls5, bs[3])
+    b5 = vrs(c4, z11 + z12)
+    c5, d5 = BS_symmetric(a5, b5, bs[4])
+    b6 = block(c5, 1)
+    c6, d6 = BS_symmetric(a6, b6, bs[5])
+    det_2 = lens(c6)
+    a7 = block(d4, 3)
+    a8, a9 = vrs(d5, z41 + z42), vrs(d6, z41 + z42)
+    c7, d7 = _bs_pair(a7, ls6, bs[6])
+    b8 = vrs(c7, z11 + z12)
+    c8, d8 = BS_symmetric(a8, b8, bs[7])
+    b9 = vrs(c8, z21 + z22)
+    c9, d9 = BS_symmetric(a9, b9, bs[8])
+    det_3 = lens(c9)
+    return [det_1, det_2, det_3, lens(d7), lens(d8), lens(d9)]
 
 
 def hybrid_setup_fixed_slms(ls1, ls2, ls3, ls4, ls5, ls6, parameters, fixed_params, distance_offset=8.9):
@@ -500,39 +541,170 @@ def hybrid_setup_fixed_slms(ls1, ls2, ls3, ls4, ls5, ls6, parameters, fixed_para
     nine splitter angles, four (eta, theta) pairs.  Returns (intensities
     (6, My, Mx) = |Ez|^2 per detector, detector lights)."""
     r, f, xout, yout = fixed_params[:4]
-    ph11, ph12, ph2, ph3, ph41, ph42 = fixed_params[4:10]
-    P = [p if isinstance(p, torch.Tensor) else torch.as_tensor(np.asarray(p, dtype=np.float64)) for p in parameters]
-    z11, z12, z21, z22, z31, z32, z41, z42 = [_distance(p, distance_offset) for p in P[0:8]]
-    bs = [_unit_angle(p) for p in P[8:17]]
-    e1, t1, e2, t2, e3, t3, e4, t4 = [_unit_angle(p) for p in P[17:25]]
+    blocks, bs = _fixed_parameters(parameters, fixed_params, distance_offset)
+    dets = _table_fixed((ls1, ls2, ls3, ls4, ls5, ls6), blocks, bs,
+                        lambda light: VCZT_objective_lens(light, r, f, xout, yout))
+    return torch.stack([d.Ez.abs() ** 2 for d in dets]), dets
 
-    def vrs(light, z):
-        return light.VRS_propagation(z)[0]
+
+def hybrid_setup_fixed_slms_fluorophores(ls1, ls2, ls3, ls4, ls5, ls6, parameters, fixed_params,
+                                         distance_offset=8.9):
+    """optical_elements.py:1119-1345: the fixed-mask table run for a depletion
+    colour (ls1, ls3, ls5) and an excitation colour (ls2, ls4, ls6), single-input
+    splitters where only one colour enters; returns the STED effective
+    intensity fluorescence(I_ex / sum I_ex, I_dep / sum I_dep) per detector
+    (6, My, Mx), I = |Ex|^2 + |Ey|^2.  Parameters as hybrid_setup_fixed_slms."""
+    r, f, xout, yout = fixed_params[:4]
+    blocks, bs = _fixed_parameters(parameters, fixed_params, distance_offset)
 
     def olens(light):
         return VCZT_objective_lens(light, r, f, xout, yout)
 
-    c1, d1 = BS_symmetric(ls1, ls4, bs[0])
-    b2 = vrs(building_block(c1, ph11, ph12, z11, e1, t1), z12)
-    c2, d2 = BS_symmetric(ls2, b2, bs[1])
-    b3 = vrs(c2, z21 + z22)
-    c3, d3 = BS_symmetric(ls3, b3, bs[2])
-    det_1 = olens(c3)
-    a5 = vrs(d2, z31 + z32)
-    a6 = vrs(building_block(d3, ph3, ph3, z31, e3, t3), z32)
-    c4, d4 = BS_symmetric(d1, ls5, bs[3])
-    b5 = vrs(c4, z11 + z12)
-    c5, d5 = BS_symmetric(a5, b5, bs[4])
-    b6 = vrs(building_block(c5, ph2, ph2, z21, e2, t2), z22)
-    c6, d6 = BS_symmetric(a6, b6, bs[5])
-    det_2 = olens(c6)
-    a7 = vrs(building_block(d4, ph41, ph42, z41, e4, t4), z42)
-    a8, a9 = vrs(d5, z41 + z42), vrs(d6, z41 + z42)
-    c7, d7 = BS_symmetric(a7, ls6, bs[6])
-    b8 = vrs(c7, z11 + z12)
-    c8, d8 = BS_symmetric(a8, b8, bs[7])
-    b9 = vrs(c8, z21 + z22)
-    c9, d9 = BS_symmetric(a9, b9, bs[8])
-    det_3 = olens(c9)
-    dets = [det_1, det_2, det_3, olens(d7), olens(d8), olens(d9)]
-    return torch.stack([d.Ez.abs() ** 2 for d in dets]), dets
+    dep = _table_fixed((ls1, None, ls3, None, ls5, None), blocks, bs, olens)
+    exc = _table_fixed((None, ls2, None, ls4, None, ls6), blocks, bs, olens)
+    return vectorized_fluorophores(_normalised_intensity(exc), _normalised_intensity(dep))
+
+
+def six_times_six_ansatz(ls1, ls2, ls3, ls4, ls5, ls6, ls7, ls8, ls9, ls10, ls11, ls12, parameters, fixed_params,
+                         distance_offset=7.6):
+    """optical_elements.py:1855-2125: a 6 x 6 grid of 36 symmetric splitters fed
+    by ls1..ls6 from the top and ls7..ls12 from the left, two building blocks
+    with fixed masks (fixed_params[4:8] = phase1_1, phase1_2, phase2_1,
+    phase2_2) and 12 high-NA detectors.
+
+    parameters (52 raw): z1..z12, bs1..bs36 (row-major), eta1, theta1, eta2,
+    theta2.  Light leaving a splitter to the right travels z1, z2 + z3, z4,
+    z5 + z6, z7 after columns 1..5; light leaving downwards travels z8..z12
+    after rows 1..5.  Block 1 sits after row 2, column 2 (z2 | z3), block 2
+    after row 4, column 4 (z5 | z6).  Detectors 1..5 take the last column of
+    rows 1..5, detectors 7..11 the bottom row's splitters 1..5 (whose light
+    also travels on), detectors 6 and 12 the two outputs of the last splitter.
+    Returns the intensities (12, My, Mx) = |Ez|^2 per detector."""
+    r, f, xout, yout = fixed_params[:4]
+    ph11, ph12, ph21, ph22 = fixed_params[4:8]
+    P = _as_tensors(parameters)
+    z = [_distance(p, distance_offset) for p in P[0:12]]
+    bs = [_unit_angle(p) for p in P[12:48]]
+    e1, t1, e2, t2 = [_unit_angle(p) for p in P[48:52]]
+    across = [z[0], z[1] + z[2], z[3], z[4] + z[5], z[6]]
+    down = z[7:12]
+
+    def vrs(light, d):
+        return light.VRS_propagation(d)[0]
+
+    def olens(light):
+        return VCZT_objective_lens(light, r, f, xout, yout)
+
+    def right(light, i, j):
+        if (i, j) == (1, 1):
+            return vrs(building_block(light, ph11, ph12, z[1], e1, t1), z[2])
+        if (i, j) == (3, 3):
+            return vrs(building_block(light, ph21, ph22, z[4], e2, t2), z[5])
+        return vrs(light, across[j])
+
+    dets = [None] * 12
+    above = [ls1, ls2, ls3, ls4, ls5, ls6]
+    for i, left in enumerate((ls7, ls8, ls9, ls10, ls11, ls12)):
+        below, b = [], left
+        for j in range(6):
+            c, d = BS_symmetric(above[j], b, bs[6 * i + j])
+            if i < 5:
+                below.append(vrs(d, down[i]))
+            elif j < 5:
+                dets[6 + j] = olens(c)
+            else:
+                dets[11] = olens(d)
+            if j < 5:
+                b = right(c, i, j)
+            else:
+                dets[i] = olens(c)
+        above = below
+    return torch.stack([d.Ez.abs() ** 2 for d in dets])
+
+
+# -- noise for robust (noise-aware) tables (optical_elements.py:2127-2300) ---------------------
+
+_NOISE_LEVELS = {            # (distance min, max) in microns, (phase min, max) in radians
+    'low': ((10.0, 50.0), (0.01, 0.05)),
+    'mild': ((50.0, 500.0), (0.05, 0.5)),
+    'high': ((500.0, 1000.0), (0.5, 1.0)),
+    'all': ((10.0, 1000.0), (0.01, 1.0)),
+}
+
+
+def _noise_bounds(NS):
+    level = NS['noise_level']
+    if level == 'tunable':
+        return tuple(NS['misalignment']), tuple(NS['phase_noise'])
+    if level not in _NOISE_LEVELS:
+        raise ValueError(f"noise_level must be one of {sorted(_NOISE_LEVELS) + ['tunable']}, got {level!r}")
+    return _NOISE_LEVELS[level]
+
+
+def shake_setup(key, resolution: int, NS: dict, device=None):
+    """optical_elements.py:2127-2217: noise for every physical variable of an
+    optical table, drawn with torch's generators (the reference draws from
+    JAX's threefry streams, so the values differ; the distributions are the
+    same).
+
+    key: an integer seed.  NS: 'number of distances', 'number of sSLM',
+    'number of wps', 'noise_level' ('low', 'mild', 'high', 'all' or 'tunable'
+    with 'misalignment' and 'phase_noise' (min, max)), 'discretize'.
+    Returns (noise_distances (n_d,), noise_slms (2 n_slm, res, res), noise_wps
+    (2 n_wp,), noise_amps (2 n_slm, res, res), key0, key): distances uniform in
+    [min_d, max_d) with no sign; the others a random sign times a uniform
+    magnitude in [min_phase, max_phase).  As in the reference, the SLM phase
+    and amplitude noise share their magnitudes pixel by pixel (both come from
+    one key) with independent signs, and with 'discretize' the sign is cast to
+    uint8, so a negative sign becomes 255 for the SLM and amplitude noise.
+    key0 is the seed for the next call.  All arrays are float64 on `device`."""
+    (d_lo, d_hi), (p_lo, p_hi) = _noise_bounds(NS)
+    dev = torch.device(device) if device is not None else torch.device('cpu')
+    seeds = torch.randint(0, 2 ** 62, (5,), generator=torch.Generator().manual_seed(int(key)))
+    key0, k1, k2, k3, k4 = (int(s) for s in seeds)
+
+    def gen(seed):
+        return torch.Generator(device=dev).manual_seed(seed)
+
+    def uniform(shape, lo, hi, g):
+        return torch.rand(shape, dtype=torch.float64, device=dev, generator=g) * (hi - lo) + lo
+
+    def sign(shape, g, wrap):
+        s = torch.randint(0, 2, shape, device=dev, generator=g).to(torch.float64) * 2 - 1
+        return torch.where(s < 0, torch.full_like(s, 255.0), s) if wrap else s
+
+    n_slm = 2 * int(NS['number of sSLM'])
+    n_wp = 2 * int(NS['number of wps'])
+    wrap = bool(NS['discretize'])
+    masks = (n_slm, resolution, resolution)
+    g2 = gen(k2)
+    slm_sign = sign(masks, g2, wrap)
+    magnitude = uniform(masks, p_lo, p_hi, g2)
+    distances = uniform((int(NS['number of distances']),), d_lo, d_hi, gen(k1))
+    amps = sign(masks, gen(k4), wrap) * magnitude
+    slms = slm_sign * magnitude
+    g3 = gen(k3)
+    wps = sign((n_wp,), g3, False) * uniform((n_wp,), p_lo, p_hi, g3)
+    return distances, slms, wps, amps, key0, int(key)
+
+
+def batch_shake_setup(keys, resolution: int, NS: dict, device=None):
+    """The per-table noise of a noise-aware batch (noisy_optimization.ipynb,
+    cells 20-25: shake_setup vmapped over one key per table): stacks of
+    (T, n_d), (T, 2 n_slm, res, res), (T, 2 n_wp), (T, 2 n_slm, res, res), the
+    next keys and the keys used."""
+    out = [shake_setup(k, resolution, NS, device) for k in keys]
+    d, s, w, a = (torch.stack([o[i] for o in out]) for i in range(4))
+    return d, s, w, a, [o[4] for o in out], [o[5] for o in out]
+
+
+def batch_robust_discovery(ls1, ls2, ls3, ls4, ls5, ls6, parameters, fixed_params, noise_distances, noise_slms,
+                           noise_wps, noise_amps, distance_offset=15):
+    """robust_discovery over a batch of tables that share sources and
+    parameters and differ in their noise (noisy_optimization.ipynb, cell 7:
+    vmap over axis 0 of the four noise stacks).  Every table runs its own
+    propagation distances, so the tables run one after another on the device.
+    Returns (T, 6, My, Mx)."""
+    return torch.stack([robust_discovery(ls1, ls2, ls3, ls4, ls5, ls6, parameters, fixed_params, noise_distances[t],
+                                         noise_slms[t], noise_wps[t], noise_amps[t], distance_offset)
+                        for t in range(len(noise_distances))])
