@@ -366,6 +366,77 @@ def lens_mask(x, y, wavelength, radius, focal):
     return pupil * np.exp(-1j * k * (X ** 2 / (2 * fx) + Y ** 2 / (2 * fy)))
 
 
+def rotate_mask(X, Y, angle, origin=None):
+    """toolbox.py:269-289."""
+    if origin is None:
+        x0, y0 = (X[-1] + X[0]) / 2, (Y[-1] + Y[0]) / 2
+    else:
+        x0, y0 = origin
+    Xrot = (X - x0) * np.cos(angle) + (Y - y0) * np.sin(angle)
+    Yrot = -(X - x0) * np.sin(angle) + (Y - y0) * np.cos(angle)
+    return Xrot, Yrot
+
+
+def cylindrical_lens_mask(x, y, wavelength, focal_length, refractive_index=1.5, angle=0.0):
+    """optical_elements.py:705-744."""
+    X, Y = np.meshgrid(x, y)
+    k = 2 * np.pi / wavelength
+    Xrot = X * np.cos(angle) + Y * np.sin(angle)
+    R = focal_length * (refractive_index - 1)
+    with np.errstate(invalid="ignore"):
+        thickness = R - np.sqrt(R ** 2 - Xrot ** 2)
+    phase = k * (refractive_index - 1) * (Xrot ** 2 / (2 * focal_length) + thickness)
+    return np.exp(-1j * phase)
+
+
+def axicon_mask(x, y, wavelength, alpha, n=1.5):
+    """optical_elements.py:746-779."""
+    X, Y = np.meshgrid(x, y)
+    r = np.sqrt(X ** 2 + Y ** 2)
+    return np.exp(-1j * (2 * np.pi / wavelength) * r * (n - 1) * np.sin(alpha) * np.tan(alpha))
+
+
+def circular_mask(X, Y, r):
+    """optical_elements.py:781-795."""
+    rx, ry = r
+    return np.where((X ** 2 / rx ** 2 + Y ** 2 / ry ** 2) < 1, 1, 0)
+
+
+def triangular_mask(X, Y, r, angle, m, height):
+    """optical_elements.py:797-817."""
+    x0, y0 = r
+    Xrot, Yrot = rotate_mask(X, Y, angle * (np.pi / 180), r)
+    Yedge = -m * np.abs(Xrot - x0) + y0
+    return np.where((Yrot < Yedge) & (Yrot > y0 - height), 1, 0)
+
+
+def rectangular_mask(X, Y, center, width, height, angle):
+    """optical_elements.py:819-836."""
+    Xrot, Yrot = rotate_mask(X, Y, angle * (np.pi / 180), center)
+    return np.where((Xrot < (width / 2)) & (Xrot > (-width / 2)) & (Yrot < (height / 2)) & (Yrot > (-height / 2)), 1, 0)
+
+
+def annular_aperture(di, do, X, Y):
+    """optical_elements.py:838-854."""
+    di, do = di / 2, do / 2
+    stop = np.where(((X ** 2 + Y ** 2) / di ** 2) < 1, 0, 1)
+    ring = np.where(((X ** 2 + Y ** 2) / do ** 2) < 1, 1, 0)
+    return stop * ring
+
+
+def forked_grating(X, Y, center, angle, period, l, kind=""):
+    """optical_elements.py:856-892 (None for a kind other than 'Amplitude'/'Phase')."""
+    Xrot, Yrot = rotate_mask(X, Y, angle * (np.pi / 180), center)
+    theta = np.arctan2(Xrot, Yrot)
+    g = np.angle(np.exp(1j * np.cos(l * theta - 2 * np.pi / period * Xrot)))
+    binary = np.where(g < 0, 0, 1)
+    if kind == "Amplitude":
+        return binary
+    if kind == "Phase":
+        return np.exp(1j * np.pi * binary)
+    return None
+
+
 def build_LCD_cell(eta, theta, shape):
     """toolbox.py:121-135: eta*ones -- scalars or per-pixel arrays both work."""
     return eta * np.ones((shape, shape)), theta * np.ones((shape, shape))

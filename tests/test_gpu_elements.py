@@ -1,4 +1,5 @@
-"""Amplitude sSLM, lens and the reference's building blocks through the
+"""Amplitude sSLM, lenses (spherical, cylindrical, axicon), an aperture and the
+reference's building blocks through the
 drop-in API (GPU), against float64 compositions of the oracle.
 
 Tolerances: fields <= 1e-5 relative L2; amplitude gradients vs central
@@ -97,3 +98,25 @@ def test_lens_vs_oracle(cuda_dev):
     ref = R.lens_mask(x, y, 0.65, (350.0, 300.0), (2e4, 3e4))
     assert rel_l2(mask.cpu().numpy(), ref) <= 1e-6
     assert rel_l2(out.Ex.cpu().numpy(), ex * ref) <= 1e-6 and float(out.Ez.abs().max()) == 0.0
+
+
+def test_cylindrical_and_axicon_then_vrs(cuda_dev):
+    """Phase elements built on the device, then a propagation, against the
+    oracle masks applied to the same fields and the float64 VRS."""
+    n = 128
+    x, y = R.space(400.0, n)
+    rng = np.random.default_rng(44)
+    ex, ey = rng.standard_normal((2, n, n)) + 1j * rng.standard_normal((2, n, n))
+    cyl, m = OE.cylindrical_lens(light_from(ex, ey, x, y, cuda_dev), 3e4, 1.5, 0.7)
+    ref = R.cylindrical_lens_mask(x, y, 0.65, 3e4, 1.5, 0.7)
+    assert rel_l2(m.cpu().numpy(), ref) <= 1e-6
+    ax, am = OE.axicon_lens(cyl, 0.05)
+    aref = R.axicon_mask(x, y, 0.65, 0.05)
+    assert rel_l2(am.cpu().numpy(), aref) <= 1e-6
+    out, _ = ax.VRS_propagation(1500.0)
+    (ox, oy, oz), _ = R.vrs_propagation(ex * ref * aref, ey * ref * aref, x, y, 0.65, 1500.0)
+    got = np.stack([out.Ex.cpu().numpy(), out.Ey.cpu().numpy(), out.Ez.cpu().numpy()])
+    assert rel_l2(got, np.stack([ox, oy, oz])) <= 1e-5
+    ring = OE.annular_aperture(200.0, 600.0, cyl.X, cyl.Y)
+    assert ring.device.type == "cuda"
+    assert np.array_equal(ring.cpu().numpy(), R.annular_aperture(200.0, 600.0, *np.meshgrid(x, y)))

@@ -1,0 +1,119 @@
+"""Lens and aperture elements (optical_elements.py:678-892, toolbox.py:269-289):
+the oracle pinned by closed forms (pixel counts of known areas, exact set
+identities, paraxial phases, symmetry), then the device-side elements against
+the oracle on the same grids (bit-exact for the 0/1 masks away from edges
+that rounding could move; <= 1e-6 for the complex64 phase masks)."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import xlumina_ref as R
+from xlumina_amd import optical_elements as OE
+from xlumina_amd.toolbox import rotate_mask, space
+from xlumina_amd.vectorized_optics import PolarizedLightSource
+from xlumina_amd.wave_optics import LightSource
+
+x, y = space(100.0, 201)          # dx = 1 um, samples on integers
+X, Y = np.meshgrid(x, y)
+dx = x[1] - x[0]
+
+
+# -- oracle pins ------------------------------------------------------------------------
+
+def test_oracle_rectangle_counts_and_rotation():
+    m = R.rectangular_mask(X, Y, (0.0, 0.0), 40.5, 20.5, 0.0)
+    assert m.sum() == 41 * 21                                   # |x| < 20.25, |y| < 10.25 on integers
+    m90 = R.rectangular_mask(X, Y, (0.0, 0.0), 40.5, 20.5, 90.0)
+    assert m90.sum() == 21 * 41 and np.array_equal(m90, m.T)
+    shifted = R.rectangular_mask(X, Y, (10.0, -5.0), 40.5, 20.5, 0.0)
+    assert np.array_equal(np.roll(np.roll(m, 10, axis=1), -5, axis=0), shifted)
+
+
+def test_oracle_circle_annulus_triangle_areas():
+    c = R.circular_mask(X, Y, (50.3, 50.3))
+    assert abs(c.sum() - math.pi * 50.3 ** 2 / dx ** 2) < 2 * math.pi * 50.3   # boundary pixels only
+    a = R.annular_aperture(40.0, 100.6, X, Y)
+    assert np.array_equal(a, R.circular_mask(X, Y, (50.3, 50.3)) - R.circular_mask(X, Y, (20.0, 20.0)))
+    t = R.triangular_mask(X, Y, (0.0, 0.0), 0.0, 2.0, 60.3)     # y < -2|x| and y > -60.3
+    assert abs(t.sum() - 60.3 ** 2 / 2.0) < 2 * 60.3 * 1.2
+    assert np.array_equal(t, t[:, ::-1])                        # symmetric in x
+
+
+def test_oracle_forked_grating_reduces_to_stripes():
+    g = R.forked_grating(X, Y, (0.0, 0.0), 0.0, 20.3, 0, "Amplitude")     # zeros of the cosine off the grid
+    stripes = np.where(np.cos(2 * np.pi * X / 20.3) < 0, 0, 1)
+    assert np.array_equal(g, stripes)
+    p = R.forked_grating(X, Y, (0.0, 0.0), 0.0, 20.0, 1, "Phase")
+    assert set(np.round(np.unique(p.real), 12)) == {-1.0, 1.0}
+    assert R.forked_grating(X, Y, (0.0, 0.0), 0.0, 20.0, 1) is None
+
+
+def test_oracle_cylindrical_and_axicon_phases():
+    wl, f = 0.6, 5000.0
+    k = 2 * np.pi / wl
+    m = R.cylindrical_lens_mask(x, y, wl, f)
+    ph = -np.unwrap(np.angle(m[100, :]))                        # row y = 0; no dependence on y
+    ph -= ph[100]
+    paraxial = k * 1.5 * x ** 2 / (2 * f)                        # k (n-1)(x^2/2f + x^2/2R), R = f (n-1)
+    assert np.max(np.abs(ph - paraxial)) < 1e-3 * np.max(paraxial) + 1e-9
+    assert np.allclose(m, m[::-1, :])
+    m90 = R.cylindrical_lens_mask(x, y, wl, f, angle=np.pi / 2)
+    assert np.max(np.abs(m90 - m.T)) < 1e-9
+    a = R.axicon_mask(x, y, wl, 0.1)
+    r = np.sqrt(X ** 2 + Y ** 2)
+    assert np.max(np.abs(np.angle(a * np.exp(1j * k * r * 0.5 * np.sin(0.1) * np.tan(0.1))))) < 1e-9
+    small = R.cylindrical_lens_mask(x, y, wl, 100.0)            # R = 50: NaN beyond |x| >= 50
+    assert np.isnan(small[:, 0]).all() and np.isfinite(small[:, 100]).all()
+
+
+# -- product vs oracle --------------------------------------------------------------------
+
+@pytest.mark.parametrize("args", [((0.0, 0.0), 40.5, 20.5, 0.0), ((7.0, -3.0), 30.2, 50.7, 33.0)])
+def test_rectangular_mask(args):
+    assert np.array_equal(OE.rectangular_mask(X, Y, *args).numpy(), R.rectangular_mask(X, Y, *args))
+
+
+@pytest.mark.parametrize("args", [((0.0, 0.0), 0.0, 2.0, 60.3), ((5.0, 12.0), 17.0, 1.3, 40.2)])
+def test_triangular_mask(args):
+    assert np.array_equal(OE.triangular_mask(X, Y, *args).numpy(), R.triangular_mask(X, Y, *args))
+
+
+def test_circular_and_annular():
+    assert np.array_equal(OE.circular_mask(X, Y, (50.3, 30.7)).numpy(), R.circular_mask(X, Y, (50.3, 30.7)))
+    assert np.array_equal(OE.annular_aperture(40.3, 100.6, X, Y).numpy(), R.annular_aperture(40.3, 100.6, X, Y))
+
+
+@pytest.mark.parametrize("kind", ["Amplitude", "Phase", ""])
+def test_forked_grating(kind):
+    got = OE.forked_grating(X, Y, (3.0, -2.0), 25.0, 17.0, 3, kind)
+    ref = R.forked_grating(X, Y, (3.0, -2.0), 25.0, 17.0, 3, kind)
+    if ref is None:
+        assert got is None
+    else:
+        assert np.allclose(got.numpy(), ref, atol=1e-12)
+
+
+def test_rotate_mask_default_origin():
+    Xr, Yr = rotate_mask(torch.tensor(X), torch.tensor(Y), 0.3)
+    Xo, Yo = R.rotate_mask(X, Y, 0.3)
+    assert np.allclose(Xr.numpy(), Xo, atol=1e-12) and np.allclose(Yr.numpy(), Yo, atol=1e-12)
+
+
+def test_lens_elements_on_lights():
+    wl = 0.6
+    ls = LightSource(x, y, wl, "cpu")
+    ls.gaussian_beam(w0=(40.0, 40.0), E0=1)
+    field = ls.field.numpy()
+    out, m = OE.cylindrical_lens(ls, 5000.0, 1.5, 0.4)
+    ref = R.cylindrical_lens_mask(x, y, wl, 5000.0, 1.5, 0.4)
+    assert np.max(np.abs(m.numpy() - ref)) < 1e-6
+    assert np.max(np.abs(out.field.numpy() - field * ref)) < 1e-6 * np.max(np.abs(field))
+    out, m = OE.axicon_lens(ls, 0.2)
+    assert np.max(np.abs(m.numpy() - R.axicon_mask(x, y, wl, 0.2))) < 1e-6
+    vl = PolarizedLightSource(x, y, wl, "cpu")
+    vl.gaussian_beam(w0=(40.0, 40.0), jones_vector=(1, 1))
+    out, m = OE.cylindrical_lens(vl, 5000.0)
+    assert float(out.Ez.abs().max()) == 0.0                    # a new light from Ex, Ey only
+    assert np.max(np.abs(out.Ex.numpy() - vl.Ex.numpy() * R.cylindrical_lens_mask(x, y, wl, 5000.0))) < 1e-6

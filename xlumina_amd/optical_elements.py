@@ -160,6 +160,45 @@ def VCZT_objective_lens(input_field: VectorizedLight, r: float, f: float, xout, 
 
 # -- (4) lenses and building blocks ------------------------------------------------------------
 
+def _grid(A, like=None):
+    if isinstance(A, torch.Tensor):
+        return A.to(torch.float64)
+    return torch.as_tensor(np.asarray(A, dtype=np.float64), device=None if like is None else like.device)
+
+
+def _mesh(light):
+    """(X (1, Nx), Y (Ny, 1)) float64 on the light's device, broadcasting to the grid."""
+    dev = light.device
+    X = torch.as_tensor(np.asarray(light.x, dtype=np.float64), device=dev)[None, :]
+    Y = torch.as_tensor(np.asarray(light.y, dtype=np.float64), device=dev)[:, None]
+    return X, Y
+
+
+def _phase_mask(phase_turns, amplitude=None):
+    """exp(2 pi i t) as complex64 from float64 turns t, reduced to [-1/2, 1/2]
+    first: lens phases reach 10^4 rad, beyond float32's reach."""
+    t = phase_turns - torch.round(phase_turns)
+    amp = torch.ones_like(t) if amplitude is None else amplitude.to(t.dtype).expand_as(t)
+    return torch.polar(amp, 2 * math.pi * t).to(torch.complex64)
+
+
+def _masked(input_field, mask):
+    """The light times a complex mask; a vectorized light's Ez is dropped, as
+    every mask element of optical_elements.py:678-779 builds a new light
+    from Ex and Ey only."""
+    dev = input_field.device
+    if isinstance(input_field, VectorizedLight):
+        out = VectorizedLight(input_field.x, input_field.y, input_field.wavelength, dev)
+        E = input_field.E
+        out._E = torch.stack([E[0] * mask, E[1] * mask, torch.zeros_like(E[2])])
+    elif isinstance(input_field, ScalarLight):
+        out = ScalarLight(input_field.x, input_field.y, input_field.wavelength, dev)
+        out._field = input_field.field * mask
+    else:
+        raise ValueError("Invalid input. Please use ScalarLight or VectorizedLight object.")
+    return out
+
+
 def lens(input_field, radius, focal):
     """optical_elements.py:678-703: circular pupil of radius (rx, ry) times
     exp(-i k (X^2 / (2 fx) + Y^2 / (2 fy))) -> (light, lens mask).
@@ -167,23 +206,97 @@ def lens(input_field, radius, focal):
     The phase reaches ~k R^2 / (2 f) (10^4 rad for a 1 mm pupil at f = 25 mm),
     so it is formed in float64 on the device and reduced before the complex64
     cast."""
-    rx, ry = radius
     fx, fy = focal
-    dev = input_field.device
-    X = torch.as_tensor(np.asarray(input_field.x, dtype=np.float64), device=dev)[None, :]
-    Y = torch.as_tensor(np.asarray(input_field.y, dtype=np.float64), device=dev)[:, None]
-    pupil = ((X / rx) ** 2 + (Y / ry) ** 2 < 1.0).to(torch.float64)     # circular_mask, :781-795
+    X, Y = _mesh(input_field)
+    pupil = circular_mask(X, Y, radius).to(torch.float64)
     turns = -(input_field.k / (2 * math.pi)) * (X ** 2 / (2 * fx) + Y ** 2 / (2 * fy))
-    turns = turns - torch.round(turns)
-    mask = torch.polar(pupil, 2 * math.pi * turns).to(torch.complex64)
-    if isinstance(input_field, VectorizedLight):
-        out = VectorizedLight(input_field.x, input_field.y, input_field.wavelength, dev)
-        E = input_field.E
-        out._E = torch.stack([E[0] * mask, E[1] * mask, torch.zeros_like(E[2])])
-    else:
-        out = ScalarLight(input_field.x, input_field.y, input_field.wavelength, dev)
-        out._field = input_field.field * mask
-    return out, mask
+    mask = _phase_mask(turns, pupil)
+    return _masked(input_field, mask), mask
+
+
+def cylindrical_lens(input_field, focal_length, refractive_index=1.5, angle=0):
+    """optical_elements.py:705-744: plano-convex cylindrical lens with its phase
+    along the rotated x axis: R = f (n - 1), thickness R - sqrt(R^2 - Xrot^2),
+    phase k (n - 1) (Xrot^2 / (2 f) + thickness) -> (light, exp(-i phase)).
+    Outside |Xrot| < R the square root is NaN, as in the reference."""
+    X, Y = _mesh(input_field)
+    a = torch.as_tensor(angle, dtype=torch.float64, device=X.device)
+    Xrot = X * torch.cos(a) + Y * torch.sin(a)
+    R = focal_length * (refractive_index - 1)
+    thickness = R - torch.sqrt(R ** 2 - Xrot ** 2)
+    phase = input_field.k * (refractive_index - 1) * (Xrot ** 2 / (2 * focal_length) + thickness)
+    mask = _phase_mask(-phase / (2 * math.pi))
+    return _masked(input_field, mask), mask
+
+
+def axicon_lens(input_field, alpha, n=1.5):
+    """optical_elements.py:746-779: phase k r (n - 1) sin(alpha) tan(alpha),
+    r = sqrt(X^2 + Y^2) -> (light, exp(-i phase)) (a Bessel-beam generator)."""
+    X, Y = _mesh(input_field)
+    r = torch.sqrt(X ** 2 + Y ** 2)
+    phase = input_field.k * r * (n - 1) * math.sin(alpha) * math.tan(alpha)
+    mask = _phase_mask(-phase / (2 * math.pi))
+    return _masked(input_field, mask), mask
+
+
+# -- amplitude masks (optical_elements.py:781-892) ------------------------------------------------
+
+def circular_mask(X, Y, r):
+    """optical_elements.py:781-795: 1 inside X^2 / rx^2 + Y^2 / ry^2 < 1, else 0."""
+    rx, ry = r
+    X = _grid(X)
+    Y = _grid(Y, X)
+    return torch.where(X ** 2 / rx ** 2 + Y ** 2 / ry ** 2 < 1, 1, 0)
+
+
+def triangular_mask(X, Y, r, angle, m, height):
+    """optical_elements.py:797-817: triangle with top corner r = (x0, y0), edge
+    slope m and the given height, rotated by `angle` degrees about r; as in the
+    reference, the corner is subtracted again after the rotation about it."""
+    from .toolbox import rotate_mask
+    x0, y0 = r
+    X = _grid(X)
+    Xrot, Yrot = rotate_mask(X, _grid(Y, X), angle * (math.pi / 180), r)
+    edge = -m * torch.abs(Xrot - x0) + y0
+    return torch.where((Yrot < edge) & (Yrot > y0 - height), 1, 0)
+
+
+def rectangular_mask(X, Y, center, width, height, angle):
+    """optical_elements.py:819-836: rectangle of width x height about `center`,
+    rotated by `angle` degrees."""
+    from .toolbox import rotate_mask
+    X = _grid(X)
+    Xrot, Yrot = rotate_mask(X, _grid(Y, X), angle * (math.pi / 180), center)
+    return torch.where((Xrot < width / 2) & (Xrot > -width / 2) & (Yrot < height / 2) & (Yrot > -height / 2), 1, 0)
+
+
+def annular_aperture(di, do, X, Y):
+    """optical_elements.py:838-854: ring between the inner and outer diameters
+    di and do (radii di / 2, do / 2)."""
+    X = _grid(X)
+    Y = _grid(Y, X)
+    r2 = X ** 2 + Y ** 2
+    stop = torch.where(r2 / (di / 2) ** 2 < 1, 0, 1)
+    ring = torch.where(r2 / (do / 2) ** 2 < 1, 1, 0)
+    return stop * ring
+
+
+def forked_grating(X, Y, center, angle, period, l, kind=''):
+    """optical_elements.py:856-892: binary forked grating, 0 where
+    cos(l atan2(Xrot, Yrot) - 2 pi Xrot / period) < 0 and 1 elsewhere, rotated
+    by `angle` degrees about `center` ('Amplitude': the 0/1 mask; 'Phase':
+    exp(i pi mask); any other kind returns None, as in the reference).  The
+    reference thresholds angle(exp(i c)), which equals c for |c| <= 1."""
+    from .toolbox import rotate_mask
+    X = _grid(X)
+    Xrot, Yrot = rotate_mask(X, _grid(Y, X), angle * (math.pi / 180), center)
+    c = torch.cos(l * torch.atan2(Xrot, Yrot) - 2 * math.pi / period * Xrot)
+    binary = torch.where(c < 0, 0, 1)
+    if kind == 'Amplitude':
+        return binary
+    if kind == 'Phase':
+        return torch.polar(torch.ones_like(c), math.pi * binary.to(torch.float64))
+    return None
 
 
 def building_block(input_light: VectorizedLight, alpha, phi, z, eta, theta) -> VectorizedLight:

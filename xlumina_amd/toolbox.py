@@ -41,3 +41,18 @@ def softmin(args, beta=90):
     """toolbox.py:98-103: -logsumexp(-beta args) / beta (differentiable min)."""
     a = args if isinstance(args, torch.Tensor) else torch.as_tensor(np.asarray(args, dtype=np.float64))
     return -torch.logsumexp(-beta * a.reshape(-1), dim=0) / beta
+
+
+def rotate_mask(X, Y, angle, origin=None):
+    """toolbox.py:269-289: the frame (X, Y) rotated by `angle` (radians) about
+    `origin`; with no origin, about ((X[-1] + X[0]) / 2, (Y[-1] + Y[0]) / 2),
+    which for meshgrids is the mean of the first and last rows, as written."""
+    X = X if isinstance(X, torch.Tensor) else torch.as_tensor(np.asarray(X, dtype=np.float64))
+    Y = Y if isinstance(Y, torch.Tensor) else torch.as_tensor(np.asarray(Y, dtype=np.float64), device=X.device)
+    if origin is None:
+        x0, y0 = (X[-1] + X[0]) / 2, (Y[-1] + Y[0]) / 2
+    else:
+        x0, y0 = origin
+    a = torch.as_tensor(angle, dtype=torch.float64, device=X.device)
+    c, s = torch.cos(a), torch.sin(a)
+    return (X - x0) * c + (Y - y0) * s, -(X - x0) * s + (Y - y0) * c
