@@ -117,3 +117,19 @@ def test_lens_elements_on_lights():
     out, m = OE.cylindrical_lens(vl, 5000.0)
     assert float(out.Ez.abs().max()) == 0.0                    # a new light from Ex, Ey only
     assert np.max(np.abs(out.Ex.numpy() - vl.Ex.numpy() * R.cylindrical_lens_mask(x, y, wl, 5000.0))) < 1e-6
+
+
+def test_scalar_light_apertures():
+    """wave_optics.py:96-171 on a light: the field times each mask (the
+    triangle's angle in radians on the light, degrees in optical_elements)."""
+    ls = LightSource(x, y, 0.6, "cpu")
+    ls.gaussian_beam(w0=(40.0, 40.0), E0=1)
+    f = ls.field.numpy()
+    checks = [
+        (ls.apply_circular_mask((50.3, 30.7)), R.circular_mask(X, Y, (50.3, 30.7))),
+        (ls.apply_triangular_mask((5.0, 12.0), 0.3, 1.3, 40.2), R.triangular_mask(X, Y, (5.0, 12.0), 0.3 * 180 / np.pi, 1.3, 40.2)),
+        (ls.apply_rectangular_mask((7.0, -3.0), 30.2, 50.7, 33.0), R.rectangular_mask(X, Y, (7.0, -3.0), 30.2, 50.7, 33.0)),
+        (ls.apply_annular_aperture(40.3, 100.6), R.annular_aperture(40.3, 100.6, X, Y)),
+    ]
+    for out, m in checks:
+        assert np.array_equal(out.field.numpy(), (f * m).astype(np.complex64))

@@ -249,16 +249,20 @@ def circular_mask(X, Y, r):
     return torch.where(X ** 2 / rx ** 2 + Y ** 2 / ry ** 2 < 1, 1, 0)
 
 
+def _triangle(X, Y, r, angle, m, height):
+    from .toolbox import rotate_mask
+    x0, y0 = r
+    X = _grid(X)
+    Xrot, Yrot = rotate_mask(X, _grid(Y, X), angle, r)
+    edge = -m * torch.abs(Xrot - x0) + y0
+    return torch.where((Yrot < edge) & (Yrot > y0 - height), 1, 0)
+
+
 def triangular_mask(X, Y, r, angle, m, height):
     """optical_elements.py:797-817: triangle with top corner r = (x0, y0), edge
     slope m and the given height, rotated by `angle` degrees about r; as in the
     reference, the corner is subtracted again after the rotation about it."""
-    from .toolbox import rotate_mask
-    x0, y0 = r
-    X = _grid(X)
-    Xrot, Yrot = rotate_mask(X, _grid(Y, X), angle * (math.pi / 180), r)
-    edge = -m * torch.abs(Xrot - x0) + y0
-    return torch.where((Yrot < edge) & (Yrot > y0 - height), 1, 0)
+    return _triangle(X, Y, r, angle * (math.pi / 180), m, height)
 
 
 def rectangular_mask(X, Y, center, width, height, angle):

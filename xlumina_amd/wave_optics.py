@@ -117,6 +117,33 @@ class ScalarLight(_GridMixin):
         out._src, out._masks = src, tuple(masks) + (spec,)
         return out
 
+    # -- aperture masks (wave_optics.py:96-171) ------------------------------------
+    def _times(self, mask) -> "ScalarLight":
+        out = ScalarLight(self.x, self.y, self.wavelength, self.device)
+        out._field = self.field * mask.to(self.device)
+        return out
+
+    def apply_circular_mask(self, r):
+        """wave_optics.py:96-109: field times the ellipse X^2/rx^2 + Y^2/ry^2 < 1."""
+        from .optical_elements import circular_mask
+        return self._times(circular_mask(self.X, self.Y, r))
+
+    def apply_triangular_mask(self, r, angle, m, height):
+        """wave_optics.py:111-133: field times the triangle of optical_elements'
+        triangular_mask, with `angle` in radians here (degrees there)."""
+        from .optical_elements import _triangle
+        return self._times(_triangle(self.X, self.Y, r, angle, m, height))
+
+    def apply_rectangular_mask(self, center, width, height, angle):
+        """wave_optics.py:135-153: field times the rectangle (angle in degrees)."""
+        from .optical_elements import rectangular_mask
+        return self._times(rectangular_mask(self.X, self.Y, center, width, height, angle))
+
+    def apply_annular_aperture(self, di, do):
+        """wave_optics.py:155-171: field times the ring between diameters di and do."""
+        from .optical_elements import annular_aperture
+        return self._times(annular_aperture(di, do, self.X, self.Y))
+
     # -- propagation ------------------------------------------------------------
     def RS_propagation(self, z):
         """wave_optics.py:173-196: Rayleigh-Sommerfeld to distance z -> (light, quality factor)."""
