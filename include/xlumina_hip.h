@@ -330,6 +330,35 @@ size_t xl_moment_workspace_bytes(int64_t npix);
 int xl_moment_forward(const xl_moment_params* p, void* workspace, size_t workspace_bytes, void* stream);
 int xl_moment_backward(const xl_moment_params* p, void* stream);
 
+/* Distance gradient of one RS / VRS propagation (the reduction half of
+ * rs_propagation_dz, wave_optics.py:173-196 / vectorized_optics.py:244-284
+ * differentiated in z): out[0] = sum_c Re(conj(d_c) g_c) over `ncomp` planes
+ * of the dz-propagated field d and the output cotangent g, plus, when `pre`
+ * is set (VRS), sum Re(conj(gez) Ex') X w + Re(conj(gez) Ey') Y w with
+ * w = -z / (X^2 + Y^2 + z^2)^{3/2}: the z-dependence of the constructed
+ * Ez_in = (Ex' X + Ey' Y)/r (vectorized_optics.py:258-261), where pre holds
+ * (Ex', Ey') and gez the cotangent of Ez_in.  float64 result on the device,
+ * summed in a fixed order. */
+typedef struct xl_dz_dot_params {
+  const void* d;               /* ncomp x rows x cols complex64, comp_stride apart */
+  const void* g;               /* same layout as d */
+  int64_t comp_stride;
+  int32_t ncomp;
+  int32_t rows;
+  int32_t cols;
+  int32_t reserved;
+  const void* pre;             /* 2 planes complex64 (pre_comp_stride apart), or NULL */
+  int64_t pre_comp_stride;
+  const void* gez;             /* rows x cols complex64 (with pre) */
+  const double* x;             /* cols grid (with pre) */
+  const double* y;             /* rows grid (with pre) */
+  double z;
+  const double* z_dev;         /* device copy of z read instead of `z`, or NULL */
+  double* out;                 /* one float64 on the device */
+} xl_dz_dot_params;
+size_t xl_dz_dot_workspace_bytes(void);
+int xl_dz_dot(const xl_dz_dot_params* p, void* workspace, size_t workspace_bytes, void* stream);
+
 /* The adjoint of one XL_MASK_SLM stage on one component (xl_pre_backward
  * with nin = ncomp = 1) fused with the Adam / AdamW step of those phases: per
  * pixel dL/dphi = Re(conj(g) i E e^{i phi}) stays in registers and the phases

@@ -78,7 +78,7 @@ STRUCTS = {"xl_mask_stage": "MaskStage", "xl_rs_params": "RsParams", "xl_czt_par
            "xl_pre_params": "PreParams", "xl_mse_params": "MseParams", "xl_adam_params": "AdamParams",
            "xl_field_loss_params": "FieldLossParams", "xl_small_area_params": "SmallAreaParams",
            "xl_bs_params": "BsParams", "xl_bs_dots_params": "BsDotsParams", "xl_kernel_time": "KernelTime",
-           "xl_moment_params": "MomentParams"}
+           "xl_moment_params": "MomentParams", "xl_dz_dot_params": "DzDotParams"}
 
 
 def test_every_struct_field_matches_c():
@@ -106,7 +106,7 @@ def test_every_struct_field_matches_c():
 
 
 def test_host_side_queries(lib):
-    assert lib.xl_abi_version() == 4
+    assert lib.xl_abi_version() == 5
     assert lib.xl_max_fft_length() == 4096
     # RS workspace: ncomp * round4(n) * P + (P/2) * P + P complex64 (+ alignment slack)
     n, P = 1024, 2048

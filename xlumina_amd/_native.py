@@ -107,6 +107,13 @@ class MomentParams(C.Structure):
                 ("gscale", C.c_void_p), ("grad", C.c_void_p)]
 
 
+class DzDotParams(C.Structure):
+    _fields_ = [("d", C.c_void_p), ("g", C.c_void_p), ("comp_stride", C.c_int64), ("ncomp", C.c_int32),
+                ("rows", C.c_int32), ("cols", C.c_int32), ("reserved", C.c_int32), ("pre", C.c_void_p),
+                ("pre_comp_stride", C.c_int64), ("gez", C.c_void_p), ("x", C.c_void_p), ("y", C.c_void_p),
+                ("z", C.c_double), ("z_dev", C.c_void_p), ("out", C.c_void_p)]
+
+
 class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_int64), ("total_ms", C.c_double)]
 
@@ -139,6 +146,8 @@ SIGNATURES = {
     "xl_moment_workspace_bytes": (C.c_size_t, [C.c_int64]),
     "xl_moment_forward": (C.c_int, [C.POINTER(MomentParams), C.c_void_p, C.c_size_t, C.c_void_p]),
     "xl_moment_backward": (C.c_int, [C.POINTER(MomentParams), C.c_void_p]),
+    "xl_dz_dot_workspace_bytes": (C.c_size_t, []),
+    "xl_dz_dot": (C.c_int, [C.POINTER(DzDotParams), C.c_void_p, C.c_size_t, C.c_void_p]),
     "xl_beam_splitter": (C.c_int, [C.POINTER(BsParams), C.c_void_p]),
     "xl_bs_dots_workspace_bytes": (C.c_size_t, [C.c_int64]),
     "xl_bs_dots": (C.c_int, [C.POINTER(BsDotsParams), C.c_void_p, C.c_size_t, C.c_void_p]),

@@ -124,21 +124,11 @@ def _rs_z_grad(cfg: RSConfig, src, masks, gout, g_pre):
     through its cotangent g_pre[2]."""
     d = ops.rs_propagate(src, cfg.x, cfg.y, cfg.z, cfg.wavelength, masks=masks, ez_mode=cfg.ez_mode, nin=cfg.nin,
                          kernel_dz=True, z_dev=cfg.z_dev)
-    # complex64 inner products (tree reductions: relative error ~ log2(n) eps),
-    # accumulated as float64 scalars
-    g = torch.real(torch.vdot(d.reshape(-1), gout.reshape(-1))).double()
-    if cfg.ez_mode == N.XL_EZ_VRS:
-        dev = src.device
-        pre = ops.pre_forward(src, cfg.x, cfg.y, cfg.z, masks=masks, ncomp=2, nin=cfg.nin) if masks else src[:2]
-        X = ops.device_grid(cfg.x, dev).float()[None, :]
-        Y = ops.device_grid(cfg.y, dev).float()[:, None]
-        z = cfg.z_dev.reshape(()).float() if cfg.z_dev is not None else float(cfg.z)
-        # d Ez_in / dz = -(Ex' X + Ey' Y) z / r^3: weight maps -X z/r^3, -Y z/r^3
-        w = -z * torch.rsqrt(X * X + Y * Y + z * z) ** 3
-        g2 = g_pre[2].conj()
-        q = torch.real(g2 * pre[0]) * (X * w) + torch.real(g2 * pre[1]) * (Y * w)
-        g = g + q.sum(dtype=torch.float64)
-    return g
+    if cfg.ez_mode != N.XL_EZ_VRS:
+        return ops.dz_dot(d, gout)
+    # d Ez_in / dz = -(Ex' X + Ey' Y) z / r^3, through Ez_in's cotangent g_pre[2]
+    pre = ops.pre_forward(src, cfg.x, cfg.y, cfg.z, masks=masks, ncomp=2, nin=cfg.nin) if masks else src[:2]
+    return ops.dz_dot(d, gout, pre=pre, gez=g_pre[2], x=cfg.x, y=cfg.y, z=cfg.z, z_dev=cfg.z_dev)
 
 
 def rs(src: torch.Tensor, x, y, z, wavelength: float, masks: Sequence[MaskSpec] = (),

@@ -102,13 +102,29 @@ __global__ __launch_bounds__(256) void bs_dots_kernel(BsDotsArgs a) {
   }
 }
 
-// One block of 64 threads: dots[k] = sum over blocks of partial[.][k] (fixed order).
-__global__ __launch_bounds__(64) void bs_dots_reduce_kernel(BsDotsArgs a, int nblk) {
-  if (threadIdx.x < 8) {
-    float acc = 0.f;
-    for (int i = 0; i < nblk; ++i) acc += a.partial[i * 8 + threadIdx.x];
-    a.dots[threadIdx.x] = acc;
+// One block of 256 threads: dots[k] = sum over blocks of partial[.][k].  Each
+// thread adds a fixed stride of the partials, then a fixed-order tree in LDS:
+// the sums are deterministic and the loads are issued in parallel.
+__global__ __launch_bounds__(256) void bs_dots_reduce_kernel(BsDotsArgs a, int nblk) {
+  __shared__ float red[8][256];
+  float acc[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+  for (int i = threadIdx.x; i < nblk; i += 256) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] += a.partial[i * 8 + k];
   }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 8) a.dots[threadIdx.x] = red[threadIdx.x][0];
 }
 
 }  // namespace xl

@@ -419,4 +419,76 @@ __global__ __launch_bounds__(256) void moment_grad_kernel(MomentArgs a) {
   }
 }
 
+
+// --- distance gradient of a propagation --------------------------------------
+// dL/dz = sum_c Re(conj(d_c) g_c) over the dz-propagated field d and the output
+// cotangent g, plus (VRS) the z-dependence of the constructed Ez_in =
+// (Ex' X + Ey' Y)/r, r^2 = X^2 + Y^2 + z^2: sum Re(conj(gez) Ex') X w +
+// Re(conj(gez) Ey') Y w with w = -z / r^3 and gez the cotangent of Ez_in.
+// One grid-stride pass with per-pixel products in float32 and float64 sums,
+// block partials, then a fixed-order tree: the result does not depend on the
+// launch.
+constexpr int kDzBlocks = 512;
+constexpr int kDzThreads = 256;
+
+struct DzDotArgs {
+  const float2* d;
+  const float2* g;
+  int64_t cs;
+  int nc;
+  const float2* pre;     // Ex', Ey' (pre_cs apart), or null
+  int64_t pre_cs;
+  const float2* gez;
+  const double* x;
+  const double* y;
+  int rows, cols;
+  double z;
+  const double* z_dev;
+  double* partial;       // kDzBlocks
+  double* out;
+};
+
+__device__ __forceinline__ double block_sum_f64(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kDzThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  return red[0];
+}
+
+__global__ __launch_bounds__(kDzThreads) void dz_dot_kernel(DzDotArgs a) {
+  __shared__ double red[kDzThreads];
+  const int64_t npix = (int64_t)a.rows * a.cols;
+  const float z = (float)(a.z_dev ? a.z_dev[0] : a.z);
+  double acc = 0.0;
+  for (int64_t p = (int64_t)blockIdx.x * kDzThreads + threadIdx.x; p < npix; p += (int64_t)gridDim.x * kDzThreads) {
+    float s = 0.f;
+    for (int c = 0; c < a.nc; ++c) {
+      const float2 dv = a.d[c * a.cs + p], gv = a.g[c * a.cs + p];
+      s += dv.x * gv.x + dv.y * gv.y;
+    }
+    if (a.pre) {
+      const int i = (int)(p / a.cols), j = (int)(p - (int64_t)i * a.cols);
+      const float X = (float)a.x[j], Y = (float)a.y[i];
+      const float ir = rsqrtf(X * X + Y * Y + z * z);
+      const float w = -z * ir * ir * ir;
+      const float2 gz = a.gez[p], px = a.pre[p], py = a.pre[a.pre_cs + p];
+      s += (gz.x * px.x + gz.y * px.y) * (X * w) + (gz.x * py.x + gz.y * py.y) * (Y * w);
+    }
+    acc += (double)s;
+  }
+  const double t = block_sum_f64(acc, red);
+  if (threadIdx.x == 0) a.partial[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kDzThreads) void dz_dot_reduce_kernel(DzDotArgs a, int nblk) {
+  __shared__ double red[kDzThreads];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kDzThreads) acc += a.partial[i];
+  const double t = block_sum_f64(acc, red);
+  if (threadIdx.x == 0) a.out[0] = t;
+}
+
 }  // namespace xl

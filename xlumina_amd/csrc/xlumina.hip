@@ -238,7 +238,7 @@ char* align256(char* p) { return (char*)(((uintptr_t)p + 255) & ~(uintptr_t)255)
 extern "C" {
 
 const char* xl_last_error(void) { return g_err.c_str(); }
-int xl_abi_version(void) { return 4; }
+int xl_abi_version(void) { return 5; }
 int xl_max_fft_length(void) { return xl::MAX_FFT; }
 int xl_init(void) { return ensure_init(); }
 
@@ -947,6 +947,39 @@ int xl_moment_backward(const xl_moment_params* p, void* stream) {
   return hip_check(hipGetLastError(), "moment gradient launch");
 }
 
+// ---------------------------------------------------------------- distance gradient
+size_t xl_dz_dot_workspace_bytes(void) { return (size_t)xl::kDzBlocks * sizeof(double) + 256; }
+
+int xl_dz_dot(const xl_dz_dot_params* p, void* ws, size_t ws_bytes, void* stream) {
+  if (!p) return fail(XL_EINVAL, "null params");
+  if (!p->d || !p->g || !p->out || p->rows < 1 || p->cols < 1 || p->ncomp < 1 || p->ncomp > 3)
+    return fail(XL_EINVAL, "bad distance-gradient arguments");
+  if (p->pre && (!p->gez || !p->x || !p->y)) return fail(XL_EINVAL, "the Ez_in term needs gez, x and y");
+  const size_t need = xl_dz_dot_workspace_bytes();
+  if (!ws || ws_bytes < need) return fail(XL_EWORKSPACE, "dz workspace too small: need " + std::to_string(need));
+  xl::DzDotArgs a;
+  a.d = (const float2*)p->d;
+  a.g = (const float2*)p->g;
+  a.cs = p->comp_stride;
+  a.nc = p->ncomp;
+  a.pre = (const float2*)p->pre;
+  a.pre_cs = p->pre_comp_stride;
+  a.gez = (const float2*)p->gez;
+  a.x = p->x;
+  a.y = p->y;
+  a.rows = p->rows;
+  a.cols = p->cols;
+  a.z = p->z;
+  a.z_dev = p->z_dev;
+  a.partial = (double*)align256((char*)ws);
+  a.out = p->out;
+  hipStream_t st = (hipStream_t)stream;
+  xl::KTimer t(xl::K_LOSS, st);
+  hipLaunchKernelGGL(xl::dz_dot_kernel, dim3(xl::kDzBlocks), dim3(xl::kDzThreads), 0, st, a);
+  hipLaunchKernelGGL(xl::dz_dot_reduce_kernel, dim3(1), dim3(xl::kDzThreads), 0, st, a, xl::kDzBlocks);
+  return hip_check(hipGetLastError(), "dz dot launch");
+}
+
 // ---------------------------------------------------------------- beam splitters
 int xl_beam_splitter(const xl_bs_params* p, void* stream) {
   if (!p) return fail(XL_EINVAL, "null params");
@@ -1002,7 +1035,7 @@ int xl_bs_dots(const xl_bs_dots_params* p, void* ws, size_t ws_bytes, void* stre
   hipStream_t st = (hipStream_t)stream;
   xl::KTimer t(xl::K_BS, st);
   hipLaunchKernelGGL(xl::bs_dots_kernel, dim3(xl::kBsDotBlocks), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(xl::bs_dots_reduce_kernel, dim3(1), dim3(64), 0, st, a, xl::kBsDotBlocks);
+  hipLaunchKernelGGL(xl::bs_dots_reduce_kernel, dim3(1), dim3(256), 0, st, a, xl::kBsDotBlocks);
   return hip_check(hipGetLastError(), "beam-splitter dot launch");
 }
 

@@ -782,6 +782,46 @@ def beam_splitter(a: torch.Tensor, b: Optional[torch.Tensor], coef, want_c: bool
     return c, d
 
 
+def dz_dot(d: torch.Tensor, gout: torch.Tensor, pre: Optional[torch.Tensor] = None,
+           gez: Optional[torch.Tensor] = None, x=None, y=None, z: float = 0.0,
+           z_dev: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """0-d float64 device tensor: sum_c Re(conj(d_c) gout_c) over the planes
+    of d and gout (C, rows, cols), plus, with `pre` (Ex', Ey') and `gez` (the
+    cotangent of the VRS Ez_in), sum Re(conj(gez) Ex') X w + Re(conj(gez) Ey') Y w,
+    w = -z / (X^2 + Y^2 + z^2)^{3/2} (xl_dz_dot: one pass, fixed-order sums)."""
+    lib = N.load()
+    d = _gpu_c64(d, "d")
+    g = _gpu_c64(gout, "gout")
+    dev = d.device
+    N.ensure_device(dev)
+    if d.shape != g.shape or d.dim() < 2:
+        raise ValueError("d and gout must have the same (..., rows, cols) shape")
+    rows, cols = d.shape[-2:]
+    d3, g3 = d.reshape(-1, rows, cols), g.reshape(-1, rows, cols)
+    if d3.shape[0] > 3:
+        raise ValueError("at most 3 planes")
+    p = N.DzDotParams()
+    p.d, p.g = d3.data_ptr(), g3.data_ptr()
+    p.comp_stride, p.ncomp, p.rows, p.cols = rows * cols, d3.shape[0], rows, cols
+    if pre is not None:
+        pre = _gpu_c64(pre, "pre")
+        gez = _gpu_c64(gez, "gez")
+        if pre.shape[0] < 2 or tuple(pre.shape[-2:]) != (rows, cols) or tuple(gez.shape[-2:]) != (rows, cols):
+            raise ValueError("pre must hold (Ex', Ey') and gez one plane of the field's shape")
+        xd, yd = device_grid(x, dev), device_grid(y, dev)
+        if xd.numel() != cols or yd.numel() != rows:
+            raise ValueError("grid lengths do not match the field")
+        p.pre, p.pre_comp_stride, p.gez = pre.data_ptr(), rows * cols, gez.data_ptr()
+        p.x, p.y = xd.data_ptr(), yd.data_ptr()
+    p.z = float(z)
+    p.z_dev = _z_ptr(z_dev, dev)
+    out = torch.empty((), dtype=torch.float64, device=dev)
+    ws = torch.empty(lib.xl_dz_dot_workspace_bytes(), dtype=torch.uint8, device=dev)
+    p.out = out.data_ptr()
+    N.check(lib.xl_dz_dot(C.byref(p), ws.data_ptr(), ws.numel(), _stream(dev)))
+    return out
+
+
 def bs_dots(gc: Optional[torch.Tensor], gd: Optional[torch.Tensor], a: torch.Tensor,
             b: Optional[torch.Tensor]) -> torch.Tensor:
     """(4,) complex64 device tensor: sums over Ex, Ey and all pixels of

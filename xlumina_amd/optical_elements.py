@@ -349,7 +349,9 @@ class _BeamSplitter(torch.autograd.Function):
         ga = gb = gk = None
         need_b = ctx.has_b and ctx.needs_input_grad[1]
         if ctx.needs_input_grad[0] or need_b:
-            kh = k.conj()[[0, 2, 1, 3]]      # (conj ca, conj da, conj cb, conj db)
+            # (conj ca, conj da, conj cb, conj db); basic indexing keeps the step free of
+            # host-to-device index uploads, so it can be captured in a HIP graph
+            kh = torch.stack([k[0], k[2], k[1], k[3]]).conj()
             ga, gb = ops.beam_splitter(gcc, gdd, kh, want_c=True, want_d=need_b, ncomp_out=a.shape[0])
         if ctx.needs_input_grad[2]:
             s = ops.bs_dots(gc, gd, a.detach(), None if b is None else b.detach())
