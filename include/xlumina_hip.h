@@ -281,17 +281,25 @@ typedef struct xl_bs_params {
   float ca[2], cb[2], da[2], db[2];   /* (re, im) */
   int64_t npix;
   int32_t zero_ez;
-  int32_t reserved;
+  int32_t theta_mode;          /* 0: the coefficients above (or coef); 1: BS_symmetric (R, iT, iT, R),
+                                  2: BS_symmetric_SI port 0 (R, 0, iT, 0), 3: port 1 (iT, 0, R, 0), with
+                                  R = |sin t| - 0.01 |cos t|, T = 0.99 |cos t| formed on the device from
+                                  theta (optical_elements.py:376-383, 394-460) */
   const float* coef;           /* optional device array (ca, cb, da, db as re, im pairs: 8 floats) read
                                   by the kernel instead of the values above (no host round trip when
                                   the coefficients are computed on the device) */
+  const double* theta;         /* theta_mode > 0: the splitter angle t (one float64 on the device) */
+  int32_t adjoint;             /* theta_mode > 0: apply the conjugate transpose, i.e. the cotangents of
+                                  (a, b) from those of (c, d) passed as a and b */
+  int32_t reserved;
 } xl_bs_params;
 int xl_beam_splitter(const xl_bs_params* p, void* stream);
 
 /* Coefficient cotangents of a beam splitter: dots = the four complex sums
  * over both components and all pixels of conj(gc) a, conj(gc) b, conj(gd) a,
  * conj(gd) b as 8 float32 (re, im); a NULL gc, gd or b contributes 0.
- * For a real parameter t of the coefficients, dL/dt = Re sum_k kappa_k'(t) S_k. */
+ * For a real parameter t of the coefficients, dL/dt = Re sum_k kappa_k'(t) S_k; for the
+ * splitters of theta_mode 1-3 the kernel writes that derivative itself. */
 typedef struct xl_bs_dots_params {
   const void* gc;
   const void* gd;
@@ -299,8 +307,12 @@ typedef struct xl_bs_dots_params {
   const void* a;
   const void* b;
   int64_t a_comp_stride, b_comp_stride;
-  float* dots;                 /* 8 float32 in device memory */
+  float* dots;                 /* 8 float32 in device memory, or NULL */
   int64_t npix;
+  const double* theta;         /* with theta_mode > 0 (as in xl_bs_params): the splitter angle */
+  int32_t theta_mode;
+  int32_t reserved;
+  double* dtheta;              /* with theta_mode > 0: dL/dt = Re sum_k S_k dkappa_k/dt (one float64) */
 } xl_bs_dots_params;
 size_t xl_bs_dots_workspace_bytes(int64_t npix);
 int xl_bs_dots(const xl_bs_dots_params* p, void* workspace, size_t workspace_bytes, void* stream);
@@ -329,6 +341,13 @@ typedef struct xl_moment_params {
 size_t xl_moment_workspace_bytes(int64_t npix);
 int xl_moment_forward(const xl_moment_params* p, void* workspace, size_t workspace_bytes, void* stream);
 int xl_moment_backward(const xl_moment_params* p, void* stream);
+
+/* Quality factor of a propagation to a distance held on the device
+ * (wave_optics.py:187-191): out = (sqrt(wl^2 + rmax^2 + 2 wl sqrt(rmax^2 + z^2))
+ * - rmax) / dr_real with z = z_dev[0]; rmax and dr_real as the reference
+ * derives them from the grid.  One float64 written on the stream. */
+int xl_quality_factor(const double* z_dev, double rmax, double dr_real, double wavelength, double* out,
+                      void* stream);
 
 /* Distance gradient of one RS / VRS propagation (the reduction half of
  * rs_propagation_dz, wave_optics.py:173-196 / vectorized_optics.py:244-284

@@ -173,3 +173,37 @@ def test_focal_moment_kernel_and_gradient(cuda_dev):
     (gr,) = torch.autograd.grad(2.5 * Lr, Ec)
     assert abs(float(L) - float(Lr)) <= 1e-6 * abs(float(Lr))
     assert rel_l2(g.cpu().numpy(), gr.numpy()) <= 1e-6
+
+
+@pytest.mark.parametrize("port", [0, 1])
+@pytest.mark.parametrize("theta", [0.7, 2.6, -1.9])
+def test_single_input_splitter_gradients(cuda_dev, port, theta):
+    """dL/dtheta of BS_symmetric_SI, formed by the dot-reduce kernel from the
+    four coefficient sums, against central differences of the oracle splitter
+    (theta in each quadrant's sign pattern of sin and cos)."""
+    rng = np.random.default_rng(65 + port)
+    (la, ea), _ = lights(rng, cuda_dev)
+    w1, w2 = rng.random((48, 48)), rng.random((48, 48))
+    wt1, wt2 = (torch.tensor(w, dtype=torch.float32, device=cuda_dev) for w in (w1, w2))
+    th = torch.tensor(theta, dtype=torch.float64, device=cuda_dev, requires_grad=True)
+    c, d = OE.BS_symmetric_SI(la, th, port=port)
+    L = (wt1 * (c.Ex.abs() ** 2 + c.Ey.abs() ** 2)).sum() + (wt2 * (d.Ex * d.Ey.conj()).real).sum()
+    (g,) = torch.autograd.grad(L, (th,))
+
+    def ref_loss(t):
+        rc, rd = R.BS_symmetric_SI(ea[:2], t, port)
+        return float(np.sum(w1 * (np.abs(rc[0]) ** 2 + np.abs(rc[1]) ** 2)) + np.sum(w2 * (rd[0] * np.conj(rd[1])).real))
+
+    fd = (ref_loss(theta + 1e-6) - ref_loss(theta - 1e-6)) / 2e-6
+    assert abs(float(g) - fd) <= 1e-4 * abs(fd) + 1e-6 * abs(ref_loss(theta)), (float(g), fd)
+
+
+def test_quality_factor_from_device_distance(cuda_dev):
+    """ops.quality_factor with z on the device (one launch) equals the float64
+    host formula (wave_optics.py:187-191)."""
+    from xlumina_amd import ops
+    x, y = R.space(500.0, 64)
+    for z in (150.0, 8.9e4, 2.3e6):
+        got = ops.quality_factor(x, y, 0.635, torch.tensor([z], dtype=torch.float64, device=cuda_dev))
+        assert got.device.type == "cuda"
+        assert abs(float(got) - ops.quality_factor(x, y, 0.635, z)) <= 1e-12 * ops.quality_factor(x, y, 0.635, z)

@@ -584,3 +584,34 @@ def test_noise_aware_batch_vs_oracle(cuda_dev):
     loss = softmin(torch.stack([LF.vectorized_loss_hybrid(got[t]) for t in range(2)]).mean(0, keepdim=True))
     ref_loss = LR.softmin(np.mean([LR.vectorized_loss_hybrid(r) for r in refs], axis=0, keepdims=True))
     assert abs(float(loss) - float(ref_loss)) <= 1e-4 * abs(float(ref_loss))
+
+
+def test_captured_table_step_matches_eager(cuda_dev):
+    """The whole sharp-focus table step (forward, loss, backward) captured in
+    a HIP graph: three replays, with parameter updates and eager steps between
+    them, against the eager step on the same parameters."""
+    from xlumina_amd import workloads
+    p = raw_parameters(89)
+    tp = [torch.tensor(v, device=cuda_dev, requires_grad=True) for v in p]
+    ls = PolarizedLightSource(X, Y, WL, cuda_dev)
+    ls.gaussian_beam(w0=(1200.0, 1200.0), jones_vector=(1, 1))
+
+    def loss():
+        inten, _ = OE.hybrid_setup_sharp_focus(ls, ls, ls, ls, ls, ls, tp, [RL, FL, XO, YO])
+        return softmin(LF.vectorized_loss_hybrid(inten))
+
+    cap = workloads.CapturedStep(loss, tp)
+    gen = torch.Generator(device=cuda_dev).manual_seed(5)
+    for _ in range(3):
+        Lg, gg = cap.replay()
+        Lg, gg = float(Lg.detach()), [g.clone() for g in gg]
+        for q in tp:
+            q.grad = None
+        Le = loss()
+        Le.backward()
+        assert abs(Lg - float(Le.detach())) <= 1e-6 * abs(float(Le.detach()))
+        for a, b in zip(gg, (q.grad for q in tp)):
+            assert float((a - b).norm()) <= 1e-6 * float(b.norm()) + 1e-30
+        with torch.no_grad():
+            for q in tp:
+                q.add_(0.01 * torch.randn(q.shape, generator=gen, device=cuda_dev, dtype=q.dtype))

@@ -92,13 +92,15 @@ class BsParams(C.Structure):
     _fields_ = [("a", C.c_void_p), ("b", C.c_void_p), ("a_comp_stride", C.c_int64), ("b_comp_stride", C.c_int64),
                 ("c", C.c_void_p), ("d", C.c_void_p), ("c_comp_stride", C.c_int64), ("d_comp_stride", C.c_int64),
                 ("ca", C.c_float * 2), ("cb", C.c_float * 2), ("da", C.c_float * 2), ("db", C.c_float * 2),
-                ("npix", C.c_int64), ("zero_ez", C.c_int32), ("reserved", C.c_int32), ("coef", C.c_void_p)]
+                ("npix", C.c_int64), ("zero_ez", C.c_int32), ("theta_mode", C.c_int32), ("coef", C.c_void_p),
+                ("theta", C.c_void_p), ("adjoint", C.c_int32), ("reserved", C.c_int32)]
 
 
 class BsDotsParams(C.Structure):
     _fields_ = [("gc", C.c_void_p), ("gd", C.c_void_p), ("gc_comp_stride", C.c_int64),
                 ("gd_comp_stride", C.c_int64), ("a", C.c_void_p), ("b", C.c_void_p), ("a_comp_stride", C.c_int64),
-                ("b_comp_stride", C.c_int64), ("dots", C.c_void_p), ("npix", C.c_int64)]
+                ("b_comp_stride", C.c_int64), ("dots", C.c_void_p), ("npix", C.c_int64), ("theta", C.c_void_p),
+                ("theta_mode", C.c_int32), ("reserved", C.c_int32), ("dtheta", C.c_void_p)]
 
 
 class MomentParams(C.Structure):
@@ -146,6 +148,7 @@ SIGNATURES = {
     "xl_moment_workspace_bytes": (C.c_size_t, [C.c_int64]),
     "xl_moment_forward": (C.c_int, [C.POINTER(MomentParams), C.c_void_p, C.c_size_t, C.c_void_p]),
     "xl_moment_backward": (C.c_int, [C.POINTER(MomentParams), C.c_void_p]),
+    "xl_quality_factor": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p]),
     "xl_dz_dot_workspace_bytes": (C.c_size_t, []),
     "xl_dz_dot": (C.c_int, [C.POINTER(DzDotParams), C.c_void_p, C.c_size_t, C.c_void_p]),
     "xl_beam_splitter": (C.c_int, [C.POINTER(BsParams), C.c_void_p]),

@@ -20,7 +20,39 @@ struct BsArgs {
   int64_t npix;
   int zero_ez;
   const float2* coef;  // optional device copy of (ca, cb, da, db)
+  const double* theta; // theta_mode > 0: splitter angle
+  int theta_mode;      // 1 symmetric, 2 / 3 single input at port 0 / 1
+  int adjoint;
 };
+
+// Lossy splitter of angle t: R = |sin t| - 0.01 |cos t|, T = 0.99 |cos t|
+// (optical_elements.py:376-383) and its derivatives, in float64.
+struct SplitterRT {
+  double R, T, dR, dT;
+  __device__ explicit SplitterRT(double t) {
+    const double c = cos(t), s = sin(t);
+    const double sc = (c > 0.0) - (c < 0.0), ss = (s > 0.0) - (s < 0.0);
+    const double T0 = fabs(c), noise = T0 * 0.01;
+    R = fabs(s) - noise;
+    T = T0 - noise;
+    dR = ss * c + 0.01 * sc * s;
+    dT = -0.99 * sc * s;
+  }
+};
+
+// (ca, cb, da, db) of the splitter modes: 1 (R, iT, iT, R), 2 (R, 0, iT, 0),
+// 3 (iT, 0, R, 0) (optical_elements.py:334-460).
+__device__ __forceinline__ void splitter_coef(int mode, double t, float2& ca, float2& cb, float2& da, float2& db) {
+  const SplitterRT k(t);
+  const float2 r = make_float2((float)k.R, 0.f), it = make_float2(0.f, (float)k.T), z = make_float2(0.f, 0.f);
+  if (mode == 1) {
+    ca = r; cb = it; da = it; db = r;
+  } else if (mode == 2) {
+    ca = r; cb = z; da = it; db = z;
+  } else {
+    ca = it; cb = z; da = r; db = z;
+  }
+}
 
 __device__ __forceinline__ float2 bs_mac(float2 k, float2 v, float2 acc) {
   return make_float2(acc.x + k.x * v.x - k.y * v.y, acc.y + k.x * v.y + k.y * v.x);
@@ -31,8 +63,18 @@ __device__ __forceinline__ float2 bs_mac(float2 k, float2 v, float2 acc) {
 __global__ __launch_bounds__(256) void bs_kernel(BsArgs a) {
   const int64_t n2 = 2 * a.npix, total = a.zero_ez ? 3 * a.npix : n2;
   const float2 zero = make_float2(0.f, 0.f);
-  const float2 ca = a.coef ? a.coef[0] : a.ca, cb = a.coef ? a.coef[1] : a.cb;
-  const float2 da = a.coef ? a.coef[2] : a.da, db = a.coef ? a.coef[3] : a.db;
+  float2 ca = a.coef ? a.coef[0] : a.ca, cb = a.coef ? a.coef[1] : a.cb;
+  float2 da = a.coef ? a.coef[2] : a.da, db = a.coef ? a.coef[3] : a.db;
+  if (a.theta_mode) {
+    splitter_coef(a.theta_mode, a.theta[0], ca, cb, da, db);
+    if (a.adjoint) {   // (conj ca, conj da, conj cb, conj db)
+      const float2 t = cb;
+      ca = make_float2(ca.x, -ca.y);
+      cb = make_float2(da.x, -da.y);
+      da = make_float2(t.x, -t.y);
+      db = make_float2(db.x, -db.y);
+    }
+  }
   for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
     const int comp = (int)(e / a.npix);
     const int64_t p = e - (int64_t)comp * a.npix;
@@ -48,7 +90,7 @@ __global__ __launch_bounds__(256) void bs_kernel(BsArgs a) {
   }
 }
 
-constexpr int kBsDotBlocks = 256;
+constexpr int kBsDotBlocks = 2048;
 
 struct BsDotsArgs {
   const float2* gc;
@@ -60,6 +102,9 @@ struct BsDotsArgs {
   float* partial;      // kBsDotBlocks x 8
   float* dots;
   int64_t npix;
+  const double* theta;
+  int theta_mode;
+  double* dtheta;
 };
 
 // conj(u) v accumulated into acc
@@ -73,6 +118,7 @@ __device__ __forceinline__ void cdot_acc(float2 u, float2 v, float& re, float& i
 __global__ __launch_bounds__(256) void bs_dots_kernel(BsDotsArgs a) {
   float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   const float2 zero = make_float2(0.f, 0.f);
+#pragma unroll 2
   for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < 2 * a.npix; e += (int64_t)gridDim.x * 256) {
     const int comp = e >= a.npix ? 1 : 0;
     const int64_t p = e - comp * a.npix;
@@ -124,7 +170,18 @@ __global__ __launch_bounds__(256) void bs_dots_reduce_kernel(BsDotsArgs a, int n
     }
     __syncthreads();
   }
-  if (threadIdx.x < 8) a.dots[threadIdx.x] = red[threadIdx.x][0];
+  if (a.dots && threadIdx.x < 8) a.dots[threadIdx.x] = red[threadIdx.x][0];
+  if (a.theta_mode && a.dtheta && threadIdx.x == 0) {
+    // dL/dt = Re sum_k S_k dkappa_k/dt with S = (conj(gc) a, conj(gc) b, conj(gd) a, conj(gd) b)
+    const SplitterRT k(a.theta[0]);
+    const double re0 = red[0][0], im0 = red[1][0], im1 = red[3][0];
+    const double re2 = red[4][0], im2 = red[5][0], re3 = red[6][0];
+    double g;
+    if (a.theta_mode == 1) g = k.dR * (re0 + re3) - k.dT * (im1 + im2);
+    else if (a.theta_mode == 2) g = k.dR * re0 - k.dT * im2;
+    else g = -k.dT * im0 + k.dR * re2;
+    a.dtheta[0] = g;
+  }
 }
 
 }  // namespace xl

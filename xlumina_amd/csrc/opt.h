@@ -428,7 +428,7 @@ __global__ __launch_bounds__(256) void moment_grad_kernel(MomentArgs a) {
 // One grid-stride pass with per-pixel products in float32 and float64 sums,
 // block partials, then a fixed-order tree: the result does not depend on the
 // launch.
-constexpr int kDzBlocks = 512;
+constexpr int kDzBlocks = 2048;
 constexpr int kDzThreads = 256;
 
 struct DzDotArgs {
@@ -489,6 +489,16 @@ __global__ __launch_bounds__(kDzThreads) void dz_dot_reduce_kernel(DzDotArgs a, 
   for (int i = threadIdx.x; i < nblk; i += kDzThreads) acc += a.partial[i];
   const double t = block_sum_f64(acc, red);
   if (threadIdx.x == 0) a.out[0] = t;
+}
+
+// Quality factor of a propagation (wave_optics.py:187-191) with z read from
+// device memory: one thread, so a propagation to a distance held on the
+// device reports it without a host round trip.
+__global__ void quality_factor_kernel(const double* z, double rmax, double dr_real, double wl, double* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const double zz = z[0];
+    out[0] = (sqrt(wl * wl + rmax * rmax + 2.0 * wl * sqrt(rmax * rmax + zz * zz)) - rmax) / dr_real;
+  }
 }
 
 }  // namespace xl

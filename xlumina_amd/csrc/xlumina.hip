@@ -947,6 +947,14 @@ int xl_moment_backward(const xl_moment_params* p, void* stream) {
   return hip_check(hipGetLastError(), "moment gradient launch");
 }
 
+int xl_quality_factor(const double* z_dev, double rmax, double dr_real, double wavelength, double* out,
+                      void* stream) {
+  if (!z_dev || !out || !(dr_real > 0.0)) return fail(XL_EINVAL, "bad quality-factor arguments");
+  hipLaunchKernelGGL(xl::quality_factor_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, z_dev, rmax, dr_real,
+                     wavelength, out);
+  return hip_check(hipGetLastError(), "quality factor launch");
+}
+
 // ---------------------------------------------------------------- distance gradient
 size_t xl_dz_dot_workspace_bytes(void) { return (size_t)xl::kDzBlocks * sizeof(double) + 256; }
 
@@ -1001,6 +1009,11 @@ int xl_beam_splitter(const xl_bs_params* p, void* stream) {
   a.npix = p->npix;
   a.zero_ez = p->zero_ez ? 1 : 0;
   a.coef = (const float2*)p->coef;
+  if (p->theta_mode < 0 || p->theta_mode > 3) return fail(XL_EINVAL, "theta_mode must be 0..3");
+  if (p->theta_mode && !p->theta) return fail(XL_EINVAL, "theta_mode needs the device angle theta");
+  a.theta = p->theta;
+  a.theta_mode = p->theta_mode;
+  a.adjoint = p->adjoint ? 1 : 0;
   hipStream_t st = (hipStream_t)stream;
   xl::KTimer t(xl::K_BS, st);
   const int64_t total = (a.zero_ez ? 3 : 2) * p->npix;
@@ -1016,7 +1029,9 @@ size_t xl_bs_dots_workspace_bytes(int64_t npix) {
 
 int xl_bs_dots(const xl_bs_dots_params* p, void* ws, size_t ws_bytes, void* stream) {
   if (!p) return fail(XL_EINVAL, "null params");
-  if (!p->a || !p->dots || p->npix < 1) return fail(XL_EINVAL, "bad beam-splitter dot arguments");
+  if (!p->a || p->npix < 1 || (!p->dots && !p->dtheta)) return fail(XL_EINVAL, "bad beam-splitter dot arguments");
+  if (p->theta_mode < 0 || p->theta_mode > 3 || (p->theta_mode && (!p->theta || !p->dtheta)))
+    return fail(XL_EINVAL, "theta_mode 1..3 needs theta and dtheta");
   const size_t need = xl_bs_dots_workspace_bytes(p->npix);
   if (!ws || ws_bytes < need)
     return fail(XL_EWORKSPACE, "beam-splitter dot workspace too small: need " + std::to_string(need));
@@ -1032,6 +1047,9 @@ int xl_bs_dots(const xl_bs_dots_params* p, void* ws, size_t ws_bytes, void* stre
   a.partial = (float*)align256((char*)ws);
   a.dots = p->dots;
   a.npix = p->npix;
+  a.theta = p->theta;
+  a.theta_mode = p->theta_mode;
+  a.dtheta = p->dtheta;
   hipStream_t st = (hipStream_t)stream;
   xl::KTimer t(xl::K_BS, st);
   hipLaunchKernelGGL(xl::bs_dots_kernel, dim3(xl::kBsDotBlocks), dim3(256), 0, st, a);

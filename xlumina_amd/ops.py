@@ -168,6 +168,11 @@ def quality_factor(x, y, wavelength, z):
     dr_real, rmax = _qf_grid(x, y)
     if isinstance(z, torch.Tensor):
         zt = z.detach().to(torch.float64).reshape(())
+        if zt.device.type == "cuda":       # one launch (xl_quality_factor)
+            out = torch.empty((), dtype=torch.float64, device=zt.device)
+            N.check(N.load().xl_quality_factor(zt.contiguous().data_ptr(), float(rmax), float(dr_real),
+                                               float(wavelength), out.data_ptr(), _stream(zt.device)))
+            return out
         dr_ideal = torch.sqrt(wavelength ** 2 + rmax ** 2 + 2 * wavelength * torch.sqrt(rmax ** 2 + zt ** 2)) - rmax
         return dr_ideal / dr_real
     dr_ideal = math.sqrt(wavelength ** 2 + rmax ** 2 + 2 * wavelength * math.sqrt(rmax ** 2 + z ** 2)) - rmax
@@ -738,12 +743,24 @@ def small_area(intensity: torch.Tensor, want_grad: bool = False, scale: float = 
     return loss, grad
 
 
+def _theta_ptr(theta: Optional[torch.Tensor], dev):
+    if theta is None:
+        return None
+    if theta.dtype != torch.float64 or theta.device != dev or theta.numel() != 1:
+        raise ValueError("theta must be a one-element float64 tensor on the field's device")
+    return theta.data_ptr()
+
+
 def beam_splitter(a: torch.Tensor, b: Optional[torch.Tensor], coef, want_c: bool = True,
-                  want_d: bool = True, ncomp_out: int = 3):
+                  want_d: bool = True, ncomp_out: int = 3, theta: Optional[torch.Tensor] = None,
+                  theta_mode: int = 0, adjoint: bool = False):
     """c = ca a + cb b, d = da a + db b on components 0, 1 of `a`, `b`
     (k, rows, cols) complex64, k >= 2; coef = (ca, cb, da, db): complex
     numbers, or a complex tensor of 4 on the GPU (then read by the kernel, with
-    no host synchronisation).  Outputs are (ncomp_out, rows, cols) with
+    no host synchronisation).  With theta_mode 1 (symmetric), 2 / 3 (single
+    input at port 0 / 1) the kernel forms the coefficients from the float64
+    device angle `theta` instead (coef is ignored), and `adjoint` applies
+    their conjugate transpose.  Outputs are (ncomp_out, rows, cols) with
     component 2 (if any) 0."""
     lib = N.load()
     a = _gpu_c64(a, "a")
@@ -765,7 +782,9 @@ def beam_splitter(a: torch.Tensor, b: Optional[torch.Tensor], coef, want_c: bool
     p.c, p.d = _ptr(c), _ptr(d)
     p.c_comp_stride = p.d_comp_stride = npix
     keep = None
-    if isinstance(coef, torch.Tensor) and coef.device.type == "cuda":
+    if theta_mode:
+        p.theta_mode, p.theta, p.adjoint = int(theta_mode), _theta_ptr(theta, dev), 1 if adjoint else 0
+    elif isinstance(coef, torch.Tensor) and coef.device.type == "cuda":
         if coef.numel() != 4 or not coef.is_complex():
             raise ValueError("device coefficients must be 4 complex numbers (ca, cb, da, db)")
         keep = coef.to(device=dev, dtype=torch.complex64).contiguous()
@@ -823,9 +842,11 @@ def dz_dot(d: torch.Tensor, gout: torch.Tensor, pre: Optional[torch.Tensor] = No
 
 
 def bs_dots(gc: Optional[torch.Tensor], gd: Optional[torch.Tensor], a: torch.Tensor,
-            b: Optional[torch.Tensor]) -> torch.Tensor:
+            b: Optional[torch.Tensor], theta: Optional[torch.Tensor] = None, theta_mode: int = 0) -> torch.Tensor:
     """(4,) complex64 device tensor: sums over Ex, Ey and all pixels of
-    conj(gc) a, conj(gc) b, conj(gd) a, conj(gd) b (absent terms are 0)."""
+    conj(gc) a, conj(gc) b, conj(gd) a, conj(gd) b (absent terms are 0).
+    With theta_mode 1-3 (the splitters of beam_splitter), the 0-d float64
+    device tensor dL/dtheta instead."""
     lib = N.load()
     a = _gpu_c64(a, "a")
     dev = a.device
@@ -833,7 +854,7 @@ def bs_dots(gc: Optional[torch.Tensor], gd: Optional[torch.Tensor], a: torch.Ten
     gc = None if gc is None else _gpu_c64(gc, "gc")
     gd = None if gd is None else _gpu_c64(gd, "gd")
     b = None if b is None else _gpu_c64(b, "b")
-    dots = torch.empty(8, dtype=torch.float32, device=dev)
+    dots = None if theta_mode else torch.empty(8, dtype=torch.float32, device=dev)
     need = lib.xl_bs_dots_workspace_bytes(npix)
     ws = torch.empty(need, dtype=torch.uint8, device=dev)
     p = N.BsDotsParams()
@@ -841,7 +862,12 @@ def bs_dots(gc: Optional[torch.Tensor], gd: Optional[torch.Tensor], a: torch.Ten
     p.gc_comp_stride = p.gd_comp_stride = npix
     p.a, p.b = a.data_ptr(), _ptr(b)
     p.a_comp_stride = p.b_comp_stride = npix
-    p.dots, p.npix = dots.data_ptr(), npix
+    p.dots, p.npix = _ptr(dots), npix
+    if theta_mode:
+        dtheta = torch.empty((), dtype=torch.float64, device=dev)
+        p.theta, p.theta_mode, p.dtheta = _theta_ptr(theta, dev), int(theta_mode), dtheta.data_ptr()
+        N.check(lib.xl_bs_dots(C.byref(p), ws.data_ptr(), ws.numel(), _stream(dev)))
+        return dtheta
     N.check(lib.xl_bs_dots(C.byref(p), ws.data_ptr(), ws.numel(), _stream(dev)))
     return torch.view_as_complex(dots.view(4, 2))
 

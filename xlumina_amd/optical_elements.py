@@ -359,6 +359,51 @@ class _BeamSplitter(torch.autograd.Function):
         return (ga if ctx.needs_input_grad[0] else None), (gb if need_b else None), gk
 
 
+class _SplitterTheta(torch.autograd.Function):
+    """The lossy symmetric / single-input splitters with their coefficients
+    formed by the kernels from the float64 device angle theta (theta_mode 1:
+    symmetric, 2 / 3: one input at port 0 / 1).  Backward: the same kernel
+    with the conjugate-transposed coefficients for the field cotangents and
+    xl_bs_dots writing dL/dtheta, so a splitter adds no scalar torch kernels
+    to a step."""
+
+    @staticmethod
+    def forward(ctx, a, b, theta, mode):
+        th = theta.detach()
+        c, d = ops.beam_splitter(a.detach(), None if b is None else b.detach(), None, theta=th, theta_mode=mode)
+        ctx.mode, ctx.has_b = mode, b is not None
+        ctx.save_for_backward(a, b if b is not None else a, th)
+        return c, d
+
+    @staticmethod
+    def backward(ctx, gc, gd):
+        a, b, th = ctx.saved_tensors
+        b = b if ctx.has_b else None
+        gc = gc.contiguous() if gc is not None else None
+        gd = gd.contiguous() if gd is not None else None
+        z = torch.zeros_like(a) if gc is None or gd is None else None
+        ga = gb = gt = None
+        need_b = ctx.has_b and ctx.needs_input_grad[1]
+        if ctx.needs_input_grad[0] or need_b:
+            ga, gb = ops.beam_splitter(gc if gc is not None else z, gd if gd is not None else z, None,
+                                       want_c=True, want_d=need_b, ncomp_out=a.shape[0], theta=th,
+                                       theta_mode=ctx.mode, adjoint=True)
+        if ctx.needs_input_grad[2]:
+            gt = ops.bs_dots(gc, gd, a.detach(), None if b is None else b.detach(), theta=th,
+                             theta_mode=ctx.mode).reshape(th.shape)
+        return (ga if ctx.needs_input_grad[0] else None), (gb if need_b else None), gt, None
+
+
+def _splitter(a: VectorizedLight, b: Optional[VectorizedLight], theta, mode):
+    th = _scalar(theta, a.device)
+    if th.device != a.device:
+        th = th.to(a.device)
+    out_c = VectorizedLight(a.x, a.y, a.wavelength, a.device)
+    out_d = VectorizedLight(a.x, a.y, a.wavelength, a.device)
+    out_c._E, out_d._E = _SplitterTheta.apply(a.E, None if b is None else b.E, th.reshape(1), mode)
+    return out_c, out_d
+
+
 def _scalar(v, device=None):
     """A one-element parameter (float, 0-d or (1,) tensor) as a 0-d float64
     tensor, on the tensor's own device (a float goes to `device`)."""
@@ -367,15 +412,6 @@ def _scalar(v, device=None):
             raise ValueError("beam-splitter parameters are scalars")
         return v.to(dtype=torch.float64).reshape(())
     return torch.tensor(float(np.asarray(v).reshape(-1)[0]), dtype=torch.float64, device=device)
-
-
-def _bs_RT(theta):
-    """optical_elements.py:376-383: T = |cos theta|, R = |sin theta|, both reduced by 1% of T."""
-    th = _scalar(theta)
-    T = torch.abs(torch.cos(th))
-    R = torch.abs(torch.sin(th))
-    noise = T * 0.01
-    return R - noise, T - noise
 
 
 def _bs_apply(a: VectorizedLight, b: Optional[VectorizedLight], coef):
@@ -388,21 +424,13 @@ def _bs_apply(a: VectorizedLight, b: Optional[VectorizedLight], coef):
 def BS_symmetric(a: VectorizedLight, b: VectorizedLight, theta):
     """optical_elements.py:334-392: lossy symmetric splitter [[R, iT], [iT, R]]
     (R = |sin theta|, T = |cos theta| less 1% of T) -> lights at ports c, d."""
-    R, T = _bs_RT(theta)
-    zero = torch.zeros((), dtype=torch.float64, device=R.device)
-    r, t = torch.complex(R, zero), torch.complex(zero, T)
-    return _bs_apply(a, b, torch.stack([r, t, t, r]))
+    return _splitter(a, b, theta, 1)
 
 
 def BS_symmetric_SI(a: VectorizedLight, theta, port=0):
     """optical_elements.py:394-460: single-input splitter; port 0: c = R a,
     d = iT a; port 1: c = iT a, d = R a."""
-    R, T = _bs_RT(theta)
-    zero = torch.zeros((), dtype=torch.float64, device=R.device)
-    r, t = torch.complex(R, zero), torch.complex(zero, T)
-    z = torch.complex(zero, zero)
-    coef = torch.stack([r, z, t, z]) if int(port) == 0 else torch.stack([t, z, r, z])
-    return _bs_apply(a, None, coef)
+    return _splitter(a, None, theta, 2 if int(port) == 0 else 3)
 
 
 def BS(a1: VectorizedLight, a2: VectorizedLight, R, T, phase):
