@@ -314,42 +314,18 @@ def run_secondary(dev, workloads, args):
         _progress(f"high-NA fwd+bwd {form}, graph-captured: {ms:.4f} ms")
     # hybrid optical table of experiments/hybrid_sharp_optical_table.py: 1024^2, 635 nm, 3 building
     # blocks, 9 splitters, 6 high-NA detectors of 400^2 (NA 0.9); loss softmin(small-area losses)
-    from xlumina_amd import loss_functions as LF
-    from xlumina_amd import optical_elements as OE
-    from xlumina_amd.toolbox import softmin, space
-    from xlumina_amd.vectorized_optics import PolarizedLightSource
-    xs, ys = space(2500.0, 1024)
-    src = PolarizedLightSource(xs, ys, 0.635, dev)
-    src.gaussian_beam(w0=(1200.0, 1200.0), jones_vector=(1, 1))
-    xo, yo = space(10.0, 400)
-    fixed = [1800.0, 1800.0 / 0.9, xo, yo]
-    rng = np.random.default_rng(9)
-    raw = []
-    for _ in range(3):
-        raw += [rng.uniform(0, 1, (1024, 1024)), rng.uniform(0, 1, (1024, 1024))] + [rng.uniform(0, 1, 1)
-                                                                                     for _ in range(4)]
-    raw += [rng.uniform(0, 1, 1) for _ in range(11)]
-    tp = [torch.tensor(v, device=dev, requires_grad=True) for v in raw]
-
-    def table_step():
-        inten, _ = OE.hybrid_setup_sharp_focus(src, src, src, src, src, src, tp, fixed)
-        L = softmin(LF.vectorized_loss_hybrid(inten))
-        L.backward()
-
-    ms = _time(table_step, max(3, args.steps // 20), 1)
+    table = workloads.SharpFocusTable(dev, n=1024, m=400)
+    tp = table.params
+    ms = _time(table.step, max(3, args.steps // 20), 1)
     out["hybrid_sharp_focus_table_1024_fwd_bwd"] = {"value": round(1000.0 / ms, 2), "unit": "fwd+bwd steps/s",
                                                     "ms": round(ms, 3), "propagations_per_step": "16 VRS + 6 high-NA VCZT forward, their adjoints backward"}
     _progress(f"hybrid table fwd+bwd: {ms:.3f} ms")
-
-    def table_loss():
-        inten, _ = OE.hybrid_setup_sharp_focus(src, src, src, src, src, src, tp, fixed)
-        return softmin(LF.vectorized_loss_hybrid(inten))
-
+    table_loss = table.loss
     cap = workloads.CapturedStep(table_loss, tp)       # the same step in one HIP graph
     ms = _time(cap.replay, max(3, args.steps // 20), 1)
     out["hybrid_sharp_focus_table_1024_fwd_bwd_graph"] = {"value": round(1000.0 / ms, 2),
                                                           "unit": "fwd+bwd steps/s", "ms": round(ms, 3)}
-    del cap, tp
+    del cap, tp, table
     torch.cuda.empty_cache()
     _progress(f"hybrid table fwd+bwd, graph-captured: {ms:.3f} ms")
     if args.fourf_launches > 0:

@@ -79,7 +79,12 @@ typedef struct xl_rs_params {
                               (lets a captured HIP graph replay with a new distance) */
   int32_t kernel_dz;       /* 1: convolve with dH/dz instead of H (derivative of the
                               propagation with respect to z, for gradients in z) */
-  int32_t reserved2;
+  int32_t hhat_mode;       /* 0: the kernel spectrum is built in the workspace; 1: built into `hhat`
+                              and kept there; 2: read from `hhat` as a mode-1 call with the same n,
+                              x, y, z, k, dx, dy left it, skipping the two kernel passes (an adjoint
+                              call conjugates it on read) -- a gradient step's adjoint propagation
+                              reuses its forward's spectrum.  Modes 1 and 2 need kernel_dz = 0. */
+  void* hhat;              /* modes 1, 2: xl_rs_hhat_bytes(n) bytes of device memory */
 } xl_rs_params;
 
 typedef struct xl_czt_params {
@@ -154,6 +159,7 @@ int xl_init(void);
 /* RS_propagation_jit (wave_optics.py:281-289), VRS_propagation_jit
  * (vectorized_optics.py:364-373) with Jones masks / Ez construction fused. */
 size_t xl_rs_workspace_bytes(int32_t n, int32_t ncomp);
+size_t xl_rs_hhat_bytes(int32_t n);
 int xl_rs_propagate(const xl_rs_params* p, void* workspace, size_t workspace_bytes, void* stream);
 
 /* CZT_jit (wave_optics.py:333-357), VCZT_jit (vectorized_optics.py:375-384),

@@ -302,3 +302,29 @@ def test_dz_dot_vs_torch_reference(cuda_dev):
     got_dev = ops.dz_dot(d, g, pre=pre, gez=gez, x=x, y=y, z=0.0, z_dev=zt)
     assert float(got_dev) == float(got_vrs)
     assert float(ops.dz_dot(d, g)) == float(got)                 # fixed-order sums: repeatable bit for bit
+
+
+@pytest.mark.parametrize("n,ez", [(64, N.XL_EZ_NONE), (96, N.XL_EZ_VRS)])
+def test_stored_kernel_spectrum_reuse(cuda_dev, n, ez):
+    """A forward that keeps the kernel spectrum in a caller buffer (hhat_mode 1)
+    and an adjoint that reuses it (mode 2, conjugated on read) give exactly
+    the results of building the spectrum in the workspace each time."""
+    rng = np.random.default_rng(92)
+    x, y = R.space(200.0, n)
+    nin = 1 if ez == N.XL_EZ_NONE else 2
+    src = torch.tensor(rng.standard_normal((nin, n, n)) + 1j * rng.standard_normal((nin, n, n)),
+                       dtype=torch.complex64, device=cuda_dev)
+    ncomp = 3 if ez == N.XL_EZ_VRS else 1
+    g = torch.tensor(rng.standard_normal((ncomp, n, n)) + 1j * rng.standard_normal((ncomp, n, n)),
+                     dtype=torch.complex64, device=cuda_dev)
+    hhat = torch.empty(ops.rs_hhat_bytes(n), dtype=torch.uint8, device=cuda_dev)
+    fwd0 = ops.rs_propagate(src, x, y, 700.0, 0.6, ez_mode=ez)
+    fwd1 = ops.rs_propagate(src, x, y, 700.0, 0.6, ez_mode=ez, hhat=hhat, hhat_mode=1)
+    assert torch.equal(fwd0, fwd1)
+    adj0 = ops.rs_propagate(g, x, y, 700.0, 0.6, adjoint=True)
+    adj2 = ops.rs_propagate(g, x, y, 700.0, 0.6, adjoint=True, hhat=hhat, hhat_mode=2)
+    assert torch.equal(adj0, adj2)
+    fwd2 = ops.rs_propagate(src, x, y, 700.0, 0.6, ez_mode=ez, hhat=hhat, hhat_mode=2)
+    assert torch.equal(fwd0, fwd2)
+    with pytest.raises(Exception):
+        ops.rs_propagate(src, x, y, 700.0, 0.6, ez_mode=ez, hhat=hhat, hhat_mode=1, kernel_dz=True)

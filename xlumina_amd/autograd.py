@@ -78,10 +78,15 @@ class _RSFunction(torch.autograd.Function):
     def forward(ctx, cfg: RSConfig, src, p00, p01, p10, p11, z_t):
         params = (p00, p01, p10, p11)
         masks = _rebuild(cfg.mask_struct, params)
+        # a step that will run the adjoint keeps the kernel spectrum for it
+        hhat = None
+        if any(ctx.needs_input_grad[1:]):
+            hhat = torch.empty(ops.rs_hhat_bytes(src.shape[-1]), dtype=torch.uint8, device=src.device)
         out = ops.rs_propagate(src, cfg.x, cfg.y, cfg.z, cfg.wavelength, masks=masks, ez_mode=cfg.ez_mode,
-                               nin=cfg.nin, z_dev=cfg.z_dev)
+                               nin=cfg.nin, z_dev=cfg.z_dev, hhat=hhat, hhat_mode=1 if hhat is not None else 0)
         ctx.cfg = cfg
-        ctx.save_for_backward(src, *[p if p is not None else torch.empty(0) for p in params])
+        ctx.save_for_backward(src, *[p if p is not None else torch.empty(0) for p in params],
+                              hhat if hhat is not None else torch.empty(0))
         ctx.has = [p is not None for p in params]
         ctx.z_meta = None if z_t is None else (z_t.shape, z_t.dtype, z_t.device)
         return out
@@ -92,8 +97,10 @@ class _RSFunction(torch.autograd.Function):
         saved = ctx.saved_tensors
         src = saved[0]
         params = [saved[1 + i] if ctx.has[i] else None for i in range(4)]
+        hhat = saved[5] if saved[5].numel() else None
         gout = gout.contiguous()
-        g_pre = ops.rs_propagate(gout, cfg.x, cfg.y, cfg.z, cfg.wavelength, adjoint=True, z_dev=cfg.z_dev)
+        g_pre = ops.rs_propagate(gout, cfg.x, cfg.y, cfg.z, cfg.wavelength, adjoint=True, z_dev=cfg.z_dev,
+                                 hhat=hhat, hhat_mode=2 if hhat is not None else 0)
         masks = _rebuild(cfg.mask_struct, params)
         need_params = ((ctx.needs_input_grad[2] and params[0] is not None, ctx.needs_input_grad[3] and params[1] is not None),
                        (ctx.needs_input_grad[4] and params[2] is not None, ctx.needs_input_grad[5] and params[3] is not None))

@@ -65,15 +65,21 @@ __global__ __launch_bounds__(256) void bs_kernel(BsArgs a) {
   const float2 zero = make_float2(0.f, 0.f);
   float2 ca = a.coef ? a.coef[0] : a.ca, cb = a.coef ? a.coef[1] : a.cb;
   float2 da = a.coef ? a.coef[2] : a.da, db = a.coef ? a.coef[3] : a.db;
-  if (a.theta_mode) {
-    splitter_coef(a.theta_mode, a.theta[0], ca, cb, da, db);
-    if (a.adjoint) {   // (conj ca, conj da, conj cb, conj db)
-      const float2 t = cb;
-      ca = make_float2(ca.x, -ca.y);
-      cb = make_float2(da.x, -da.y);
-      da = make_float2(t.x, -t.y);
-      db = make_float2(db.x, -db.y);
+  if (a.theta_mode) {   // formed once per block in float64, shared through LDS
+    __shared__ float2 k4[4];
+    if (threadIdx.x == 0) {
+      splitter_coef(a.theta_mode, a.theta[0], ca, cb, da, db);
+      if (a.adjoint) {   // (conj ca, conj da, conj cb, conj db)
+        const float2 t = cb;
+        ca = make_float2(ca.x, -ca.y);
+        cb = make_float2(da.x, -da.y);
+        da = make_float2(t.x, -t.y);
+        db = make_float2(db.x, -db.y);
+      }
+      k4[0] = ca; k4[1] = cb; k4[2] = da; k4[3] = db;
     }
+    __syncthreads();
+    ca = k4[0]; cb = k4[1]; da = k4[2]; db = k4[3];
   }
   for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
     const int comp = (int)(e / a.npix);

@@ -45,6 +45,8 @@ struct RsArgs {
   float scale;            // dx*dy/P^2
   int n, nin, ncomp, ez_mode, adjoint;
   int kernel_dz;          // 1: convolve with dH/dz instead of H
+  int hstored;            // the spectrum lives in a caller buffer, unconjugated (adjoints conjugate on read)
+  int hreuse;             // the spectrum is already there: skip rs_hrow / rs_hcol
   MaskArgs mask;
 };
 
@@ -238,7 +240,7 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_hcol_kernel(Rs
     for (int q = 0; q < HO::R; ++q) {
       const int ky = HO::index(t, b, q);
       float2 h = u.v[b][q] * a.scale;
-      if (a.adjoint) h = conjf2(h);
+      if (a.adjoint && !a.hstored) h = conjf2(h);
       if (ky < P / 2) hc[(ky >> 2) * 16 + (ky & 3) * 4] = h;
       else if (ky == P / 2) a.hmid[col] = h;
     }
@@ -272,6 +274,7 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_col_kernel(RsA
   };
   typename HO::Type u;
   fft_forward_to_regs<P, TPF, SRC_FN>(s, t, load, u);
+  const float hs = (a.adjoint && a.hstored) ? -1.f : 1.f;   // stored spectrum: conjugate for the adjoint
   if (active) {
 #pragma unroll
     for (int b = 0; b < HO::BPT; ++b)
@@ -279,7 +282,8 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_col_kernel(RsA
       for (int q = 0; q < HO::R; ++q) {
         const int ky = HO::index(t, b, q);
         const int kk = ky <= P / 2 ? ky : P - ky;
-        const float2 h = kk == P / 2 ? a.hmid[col] : hc[(kk >> 2) * 16 + (kk & 3) * 4];
+        float2 h = kk == P / 2 ? a.hmid[col] : hc[(kk >> 2) * 16 + (kk & 3) * 4];
+        h.y *= hs;
         u.v[b][q] = cmul(u.v[b][q], h);
       }
   }
@@ -403,7 +407,7 @@ inline hipError_t rs_launch(const RsArgs& a, hipStream_t st) {
   constexpr int CPB = Units<P>::CPB;
   const dim3 blk(CPB * (P / 16));
   const int rows = ((a.n + 3) >> 2) * 4;
-  { KTimer t(K_RS_HROW, st); hipLaunchKernelGGL(rs_hrow_kernel<P>, dim3(P / 2 / CPB), blk, 0, st, a); }
+  if (!a.hreuse) { KTimer t(K_RS_HROW, st); hipLaunchKernelGGL(rs_hrow_kernel<P>, dim3(P / 2 / CPB), blk, 0, st, a); }
   if constexpr (P >= 1024) {
     KTimer t(K_RS_UROW, st);
     hipLaunchKernelGGL(rs_urow1_kernel<P>, dim3(rows), dim3(P / 16), 0, st, a);
@@ -418,7 +422,7 @@ inline hipError_t rs_launch(const RsArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(rs_urow_kernel<P>, dim3(rows), dim3(a.ncomp * (P / 16)), 
                        (size_t)a.ncomp * LdsLen<P>::value * sizeof(float2), st, a);
   }
-  { KTimer t(K_RS_HCOL, st); hipLaunchKernelGGL(rs_hcol_kernel<P>, dim3(P / CPB), blk, 0, st, a); }
+  if (!a.hreuse) { KTimer t(K_RS_HCOL, st); hipLaunchKernelGGL(rs_hcol_kernel<P>, dim3(P / CPB), blk, 0, st, a); }
   {
     KTimer t(K_RS_COL, st);
     const int units = P * a.ncomp;

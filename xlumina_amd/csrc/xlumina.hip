@@ -242,6 +242,12 @@ int xl_abi_version(void) { return 5; }
 int xl_max_fft_length(void) { return xl::MAX_FFT; }
 int xl_init(void) { return ensure_init(); }
 
+size_t xl_rs_hhat_bytes(int32_t n) {
+  if (n < 2) return 0;
+  const int64_t P = rs_fft_len(n);
+  return (size_t)((P / 2) * P + P) * sizeof(float2);
+}
+
 size_t xl_rs_workspace_bytes(int32_t n, int32_t ncomp) {
   if (n < 2 || ncomp < 1 || ncomp > 3) return 0;
   return rs_ws(n, ncomp);
@@ -261,6 +267,9 @@ int xl_rs_propagate(const xl_rs_params* p, void* ws, size_t ws_bytes, void* stre
   if (int rc = check_mask(p->mask, p->nin)) return rc;
   if (p->adjoint && (p->mask[0].kind || p->mask[1].kind || p->ez_mode))
     return fail(XL_EINVAL, "the adjoint convolution takes no masks / Ez construction (use xl_pre_backward)");
+  if (p->hhat_mode < 0 || p->hhat_mode > 2) return fail(XL_EINVAL, "hhat_mode must be 0, 1 or 2");
+  if (p->hhat_mode && (!p->hhat || p->kernel_dz))
+    return fail(XL_EINVAL, "hhat_mode 1 / 2 needs the hhat buffer and kernel_dz = 0");
   const int ncomp = p->ez_mode == XL_EZ_VRS ? 3 : p->nin;
   const size_t need = rs_ws(n, ncomp);
   if (!ws || ws_bytes < need) return fail(XL_EWORKSPACE, "RS workspace too small: need " + std::to_string(need));
@@ -273,8 +282,10 @@ int xl_rs_propagate(const xl_rs_params* p, void* ws, size_t ws_bytes, void* stre
   a.out_cs = p->out_comp_stride;
   a.intensity = p->intensity;
   a.t = (float2*)w;
-  a.hr = a.t + ncomp * nr4 * P;
+  a.hr = p->hhat_mode ? (float2*)p->hhat : a.t + ncomp * nr4 * P;
   a.hmid = a.hr + (int64_t)(P / 2) * P;
+  a.hstored = p->hhat_mode ? 1 : 0;
+  a.hreuse = p->hhat_mode == 2 ? 1 : 0;
   a.x = p->x;
   a.y = p->y;
   a.z = p->z;

@@ -195,11 +195,17 @@ def rs_workspace_bytes(n: int, ncomp: int) -> int:
     return int(N.load().xl_rs_workspace_bytes(n, ncomp))
 
 
+def rs_hhat_bytes(n: int) -> int:
+    """Bytes of the kernel-spectrum buffer of an n x n RS propagation (hhat)."""
+    return int(N.load().xl_rs_hhat_bytes(n))
+
+
 def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Sequence[MaskSpec] = (),
                  ez_mode: int = N.XL_EZ_NONE, nin: Optional[int] = None, intensity: bool = False,
                  adjoint: bool = False, out: Optional[torch.Tensor] = None,
                  workspace: Optional[torch.Tensor] = None, z_dev: Optional[torch.Tensor] = None,
-                 inten: Optional[torch.Tensor] = None, kernel_dz: bool = False):
+                 inten: Optional[torch.Tensor] = None, kernel_dz: bool = False,
+                 hhat: Optional[torch.Tensor] = None, hhat_mode: int = 0):
     """RS_propagation_jit / VRS_propagation_jit (wave_optics.py:281-289,
     vectorized_optics.py:364-373) with fused masks and Ez construction.
 
@@ -209,6 +215,9 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
     (a captured graph then replays with whatever distance it holds).
     kernel_dz: convolve with dH/dz instead of H (the z-derivative of the
     propagation of the pre-op'd field).
+    hhat, hhat_mode: 1 builds the kernel spectrum into `hhat` (a device buffer
+    of rs_hhat_bytes(n)) and keeps it; 2 reuses what a mode-1 call of the same
+    geometry and distance left there (an adjoint call conjugates it).
     """
     lib = N.load()
     src = _prep_field(src)
@@ -248,6 +257,10 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
     p.dy = float(ya[1] - ya[0])
     p.z_dev = _z_ptr(z_dev, dev)
     p.kernel_dz = 1 if kernel_dz else 0
+    if hhat_mode:
+        if hhat is None or hhat.device != dev or hhat.numel() * hhat.element_size() < rs_hhat_bytes(n):
+            raise ValueError("hhat must be a device buffer of rs_hhat_bytes(n) bytes on the field's device")
+        p.hhat_mode, p.hhat = int(hhat_mode), hhat.data_ptr()
     keep = fill_masks(p.mask, masks, (n, n), dev, nin)
     N.check(lib.xl_rs_propagate(C.byref(p), workspace.data_ptr(), workspace.numel(), _stream(dev)))
     del keep

@@ -11,6 +11,9 @@ benchmark, the tests and the serving generator).
                    650 nm, Gaussian w0 1.2 mm Jones (1, 1), seeded sSLM phases
                    U[-pi, pi] and a constant LCD, focused by VCZT_objective_lens
                    to a 256^2 window of 4 um; loss = intensity second moment
+  SharpFocusTable  the hybrid sharp-focus optical table (1024^2 -> six 400^2
+                   high-NA detectors) with seeded raw parameters, forward +
+                   backward of its softmin small-area loss
 Lengths in microns.
 """
 from __future__ import annotations
@@ -221,3 +224,43 @@ class CapturedStep:
             self.graph.replay()
         torch.cuda.current_stream(self.device).wait_stream(self.stream)
         return self.loss, self.grads
+
+
+class SharpFocusTable:
+    """The optical table of experiments/hybrid_sharp_optical_table.py at n^2
+    (635 nm, 2500 um half-width, three building blocks, nine splitters, six
+    high-NA detectors of m^2 at NA 0.9) with seeded random raw parameters;
+    the loss is softmin of the small-area losses of the six detectors
+    (hybrid_sharp_optical_table.py:50-58).  step() runs one forward +
+    backward."""
+
+    def __init__(self, dev=None, n: int = 1024, m: int = 400, seed: int = 9):
+        from . import loss_functions as LF
+        from . import optical_elements as OE
+        from .toolbox import softmin, space
+        from .vectorized_optics import PolarizedLightSource
+        dev = torch.device(dev) if dev is not None else torch.device("cuda", torch.cuda.current_device())
+        xs, ys = space(2500.0, n)
+        self.src = PolarizedLightSource(xs, ys, 0.635, dev)
+        self.src.gaussian_beam(w0=(1200.0, 1200.0), jones_vector=(1, 1))
+        xo, yo = space(10.0, m)
+        self.fixed = [1800.0, 1800.0 / 0.9, xo, yo]
+        rng = np.random.default_rng(seed)
+        raw = []
+        for _ in range(3):
+            raw += [rng.uniform(0, 1, (n, n)), rng.uniform(0, 1, (n, n))] + [rng.uniform(0, 1, 1) for _ in range(4)]
+        raw += [rng.uniform(0, 1, 1) for _ in range(11)]
+        self.params = [torch.tensor(v, device=dev, requires_grad=True) for v in raw]
+        self._OE, self._LF, self._softmin = OE, LF, softmin
+
+    def loss(self):
+        s = self.src
+        inten, _ = self._OE.hybrid_setup_sharp_focus(s, s, s, s, s, s, self.params, self.fixed)
+        return self._softmin(self._LF.vectorized_loss_hybrid(inten))
+
+    def step(self):
+        for p in self.params:
+            p.grad = None
+        L = self.loss()
+        L.backward()
+        return L
