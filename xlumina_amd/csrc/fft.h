@@ -131,6 +131,40 @@ template <int R, bool INV> struct DFT {
 template <bool INV> struct DFT<1, INV> {
   __device__ __forceinline__ static void run(float2*) {}
 };
+// DFT of R points whose upper half v[R/2..R-1] is zero (a zero-padded
+// input): the bottom radix-2 stage of the recursion is a copy.
+template <int R, bool INV> struct DFTHalf {
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 e[R / 2], o[R / 2];
+#pragma unroll
+    for (int i = 0; i < R / 2; ++i) { e[i] = v[2 * i]; o[i] = v[2 * i + 1]; }
+    DFTHalf<R / 2, INV>::run(e);
+    DFTHalf<R / 2, INV>::run(o);
+#pragma unroll
+    for (int k = 0; k < R / 2; ++k) {
+      const float2 t = rot16<INV>(o[k], k * (16 / R));
+      v[k] = e[k] + t;
+      v[k + R / 2] = e[k] - t;
+    }
+  }
+};
+template <bool INV> struct DFTHalf<2, INV> {
+  __device__ __forceinline__ static void run(float2* v) { v[1] = v[0]; }
+};
+// DFT of R points of which only the lower half of the outputs is kept
+// (a cropped output): the top radix-2 stage forms v[0..R/2-1] only.
+template <int R, bool INV> struct DFTLow {
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 e[R / 2], o[R / 2];
+#pragma unroll
+    for (int i = 0; i < R / 2; ++i) { e[i] = v[2 * i]; o[i] = v[2 * i + 1]; }
+    DFT<R / 2, INV>::run(e);
+    DFT<R / 2, INV>::run(o);
+#pragma unroll
+    for (int k = 0; k < R / 2; ++k) v[k] = e[k] + rot16<INV>(o[k], k * (16 / R));
+  }
+};
+
 // X1,2 = a - (b + c)/2 -+ i (sqrt(3)/2)(b - c) (signs swapped for the inverse).
 template <bool INV> struct DFT<3, INV> {
   __device__ __forceinline__ static void run(float2* v) {

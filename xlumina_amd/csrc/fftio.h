@@ -27,10 +27,17 @@ enum { SRC_LDS = 0, SRC_FN = 1, SRC_REG = 2 };
 enum { DST_LDS = 0, DST_FN = 1, DST_REG = 2 };
 
 // One pass.  SRC_REG / DST_REG use `reg` (shape [BPT][R] of this pass).
-template <int L, int R, int NS, int TPF, bool INV, int SRC, int DST, class Load, class Store, class RegT>
+// ZHI: the inputs r >= R/2 are zero (a first pass over a zero-padded upper
+// half: indices >= L/2); they are neither loaded nor transformed.  LOW: only
+// the outputs r < R/2 are formed and written (a last pass whose indices
+// >= L/2 are cropped away).
+template <int L, int R, int NS, int TPF, bool INV, int SRC, int DST, bool ZHI = false, bool LOW = false, class Load,
+          class Store, class RegT>
 __device__ __forceinline__ void pass_io(float2* s, int t, Load& load, Store& store, RegT& reg) {
   constexpr int NB = L / R;
   constexpr int BPT = (NB + TPF - 1) / TPF;
+  static_assert(!(ZHI && LOW), "a pass is either the first or the last of a chain here");
+  static_assert(!ZHI || NS == 1, "zero upper inputs only on the first pass");
   float2 v[BPT][R];
 #pragma unroll
   for (int b = 0; b < BPT; ++b) {
@@ -38,12 +45,15 @@ __device__ __forceinline__ void pass_io(float2* s, int t, Load& load, Store& sto
     if (NB % TPF == 0 || j < NB) {
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        if constexpr (SRC == SRC_LDS) v[b][r] = s[lpad(j + r * NB)];
+        if (ZHI && r >= R / 2) v[b][r] = make_float2(0.f, 0.f);
+        else if constexpr (SRC == SRC_LDS) v[b][r] = s[lpad(j + r * NB)];
         else if constexpr (SRC == SRC_FN) v[b][r] = load(j + r * NB);
         else v[b][r] = reg.v[b][r];
       }
       if constexpr (NS > 1) apply_twiddles<R, INV, L>(v[b], (j % NS) * (Tw<L>::N / (NS * R)));
-      DFT<R, INV>::run(v[b]);
+      if constexpr (ZHI) DFTHalf<R, INV>::run(v[b]);
+      else if constexpr (LOW) DFTLow<R, INV>::run(v[b]);
+      else DFT<R, INV>::run(v[b]);
     }
   }
   // LDS may still be read by other threads (this pass or the previous phase)
@@ -54,7 +64,7 @@ __device__ __forceinline__ void pass_io(float2* s, int t, Load& load, Store& sto
     if (NB % TPF == 0 || j < NB) {
       const int idxD = (j / NS) * NS * R + (j % NS);
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
+      for (int r = 0; r < (LOW ? R / 2 : R); ++r) {
         if constexpr (DST == DST_LDS) s[lpad(idxD + r * NS)] = v[b][r];
         else if constexpr (DST == DST_FN) store(idxD + r * NS, v[b][r]);
         else reg.v[b][r] = v[b][r];
@@ -73,22 +83,28 @@ struct NoIO {
 // others go through LDS.  Every pass that writes LDS first waits for all
 // threads (so a chain may follow any earlier LDS use); a chain ending in
 // DST_FN / DST_REG leaves LDS free without a final barrier.
-template <int L, int TPF, bool INV, bool FWD, int I, int SRC, int DST, class Load, class Store, class RegIn,
-          class RegOut>
+// ZHI / LOW (see pass_io) apply to the first / last pass of the chain; both
+// need a plan that starts (ZHI) or ends (LOW) with an unstrided radix-16 pass
+// over the L/2 split, i.e. a power-of-two L.
+template <int L, int TPF, bool INV, bool FWD, int I, int SRC, int DST, bool ZHI = false, bool LOW = false,
+          class Load, class Store, class RegIn, class RegOut>
 __device__ __forceinline__ void chain_io(float2* s, int t, Load& load, Store& store, RegIn& rin, RegOut& rout) {
   using P = FftPlan<L>;
   constexpr int R = P::radix(FWD, I), NS = P::ns(FWD, I);
   constexpr bool first = I == 0, last = I == P::NP - 1;
   constexpr int S = first ? SRC : SRC_LDS;
   constexpr int D = last ? DST : DST_LDS;
+  static_assert(!(ZHI || LOW) || (!P::TRI && P::NP >= 2), "zero-half passes need a power-of-two plan");
+  static_assert(!(ZHI && first) || R == 16, "ZHI needs a radix-16 first pass");
+  static_assert(!(LOW && last) || (R == 16 && NS * R == L), "LOW needs an unstrided radix-16 last pass");
   if constexpr (first) {
-    pass_io<L, R, NS, TPF, INV, S, D>(s, t, load, store, rin);
+    pass_io<L, R, NS, TPF, INV, S, D, ZHI, false>(s, t, load, store, rin);
   } else if constexpr (last) {
-    pass_io<L, R, NS, TPF, INV, S, D>(s, t, load, store, rout);
+    pass_io<L, R, NS, TPF, INV, S, D, false, LOW>(s, t, load, store, rout);
   } else {
     pass_io<L, R, NS, TPF, INV, S, D>(s, t, load, store, rout);
   }
-  if constexpr (!last) chain_io<L, TPF, INV, FWD, I + 1, SRC, DST>(s, t, load, store, rin, rout);
+  if constexpr (!last) chain_io<L, TPF, INV, FWD, I + 1, SRC, DST, ZHI, LOW>(s, t, load, store, rin, rout);
 }
 
 template <int L, int TPF> struct HandOff {
@@ -103,20 +119,20 @@ template <int L, int TPF> struct HandOff {
 };
 
 // forward FFT: SRC -> registers (hand-off layout)
-template <int L, int TPF, int SRC, class Load>
+template <int L, int TPF, int SRC, bool ZHI = false, class Load>
 __device__ __forceinline__ void fft_forward_to_regs(float2* s, int t, Load& load,
                                                     typename HandOff<L, TPF>::Type& out) {
   NoIO none;
   typename HandOff<L, TPF>::Type dummy;
-  chain_io<L, TPF, false, true, 0, SRC, DST_REG>(s, t, load, none, dummy, out);
+  chain_io<L, TPF, false, true, 0, SRC, DST_REG, ZHI, false>(s, t, load, none, dummy, out);
 }
 
 // inverse FFT: registers (hand-off layout) -> DST
-template <int L, int TPF, int DST, class Store>
+template <int L, int TPF, int DST, bool LOW = false, class Store>
 __device__ __forceinline__ void fft_inverse_from_regs(float2* s, int t, typename HandOff<L, TPF>::Type& in,
                                                       Store& store) {
   NoIO none;
-  chain_io<L, TPF, true, false, 0, SRC_REG, DST>(s, t, none, store, in, in);
+  chain_io<L, TPF, true, false, 0, SRC_REG, DST, false, LOW>(s, t, none, store, in, in);
 }
 
 // Output layout of the last pass of a plan: slot (b, q) holds index j + q*NS,

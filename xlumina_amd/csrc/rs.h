@@ -209,7 +209,7 @@ __global__ __launch_bounds__(P / 16) void rs_urow1_kernel(RsArgs a) {
     float2* dst = a.t + (int64_t)c * nrg * 4 * P + rg * 16 + rr * 4;
     auto store = [&](int col, float2 v) { dst[(int64_t)(col >> 2) * nrg * 16 + (col & 3)] = v; };
     typename LastPass<P, TPF, true>::Type unused;
-    chain_io<P, TPF, false, true, 0, SRC_REG, DST_FN>(lds, t, none, store, rin, unused);
+    chain_io<P, TPF, false, true, 0, SRC_REG, DST_FN, true, false>(lds, t, none, store, rin, unused);
   }
 }
 
@@ -273,7 +273,10 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_col_kernel(RsA
     return (active && r < nr) ? tc[(r >> 2) * 16 + (r & 3) * 4] : make_float2(0.f, 0.f);
   };
   typename HO::Type u;
-  fft_forward_to_regs<P, TPF, SRC_FN>(s, t, load, u);
+  // rows >= N <= P/2 are zero padding, and the rows kept are < N: the first
+  // forward pass skips the upper half of its inputs, the last inverse pass
+  // the upper half of its outputs
+  fft_forward_to_regs<P, TPF, SRC_FN, true>(s, t, load, u);
   const float hs = (a.adjoint && a.hstored) ? -1.f : 1.f;   // stored spectrum: conjugate for the adjoint
   if (active) {
 #pragma unroll
@@ -290,7 +293,7 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_col_kernel(RsA
   auto store = [&](int r, float2 v) {
     if (active && r < nr) tc[(r >> 2) * 16 + (r & 3) * 4] = v;
   };
-  fft_inverse_from_regs<P, TPF, DST_FN>(s, t, u, store);
+  fft_inverse_from_regs<P, TPF, DST_FN, true>(s, t, u, store);
 }
 
 // Per block 2 rows (one per thread group): the row halves of
