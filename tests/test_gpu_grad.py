@@ -328,3 +328,73 @@ def test_stored_kernel_spectrum_reuse(cuda_dev, n, ez):
     assert torch.equal(fwd0, fwd2)
     with pytest.raises(Exception):
         ops.rs_propagate(src, x, y, 700.0, 0.6, ez_mode=ez, hhat=hhat, hhat_mode=1, kernel_dz=True)
+
+
+@pytest.mark.parametrize("form", ["bluestein", "gemm"])
+def test_czt_grad_wrt_z_vs_fd(cuda_dev, form):
+    """dL/dz of ScalarLight.CZT after an SLM (the distance enters the
+    transfer functions on both grids, the constant z dx dy lambda and both
+    axes' kernels through Dm = lambda z / dx) against central differences of
+    the float64 oracle CZT."""
+    from xlumina_amd import optical_elements as OE
+    from xlumina_amd.wave_optics import ScalarLight
+    n, m = 64, 40
+    x, y = R.space(150.0, n)
+    xo, yo = R.space(40.0, m)
+    rng = np.random.default_rng(93)
+    f = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    ph = rng.uniform(-np.pi, np.pi, (n, n))
+    w = rng.random((m, m))
+    z0 = 2500.0
+    prev = ops.get_czt_form()
+    ops.set_czt_form(form)
+    try:
+        light = ScalarLight(x, y, 0.6, cuda_dev)
+        light.field = f.astype(np.complex64)
+        zt = torch.tensor([z0], dtype=torch.float64, device=cuda_dev, requires_grad=True)
+        lit, _ = OE.SLM(light, ph, n)
+        out = lit.CZT(zt, xo, yo)
+        L = (torch.tensor(w, dtype=torch.float32, device=cuda_dev) * out.field.abs() ** 2).sum()
+        (gz,) = torch.autograd.grad(L, (zt,))
+    finally:
+        ops.set_czt_form(prev)
+
+    def ref(z):
+        return float(np.sum(w * np.abs(R.czt(f * np.exp(1j * ph), x, y, 0.6, z, xo, yo)) ** 2))
+
+    h = 0.05
+    fd = (ref(z0 + h) - ref(z0 - h)) / (2 * h)
+    assert abs(float(gz) - fd) <= 2e-3 * abs(fd), (float(gz), fd)
+
+
+def test_vczt_grad_wrt_z_vs_fd(cuda_dev):
+    """dL/dz of VectorizedLight.VCZT after a per-pixel LCD, including the
+    z-dependence of the constructed Ez_in = (Ex X + Ey Y) z / r^2, against
+    central differences of the float64 oracle."""
+    from xlumina_amd import optical_elements as OE
+    from xlumina_amd.vectorized_optics import VectorizedLight
+    n, m = 64, 40
+    x, y = R.space(150.0, n)
+    xo, yo = R.space(40.0, m)
+    rng = np.random.default_rng(94)
+    ex, ey = rng.standard_normal((2, n, n)) + 1j * rng.standard_normal((2, n, n))
+    eta, th = rng.uniform(0, 2 * np.pi, (2, n, n))
+    w = rng.random((3, m, m))
+    z0 = 900.0
+    light = VectorizedLight(x, y, 0.6, cuda_dev)
+    light.Ex = torch.tensor(ex.astype(np.complex64), device=cuda_dev)
+    light.Ey = torch.tensor(ey.astype(np.complex64), device=cuda_dev)
+    zt = torch.tensor(z0, dtype=torch.float64, device=cuda_dev, requires_grad=True)
+    out = OE.LCD(light, eta, th).VCZT(zt, xo, yo)
+    E = torch.stack([out.Ex, out.Ey, out.Ez])
+    L = (torch.tensor(w, dtype=torch.float32, device=cuda_dev) * E.abs() ** 2).sum()
+    (gz,) = torch.autograd.grad(L, (zt,))
+
+    def ref(z):
+        lx, ly, _ = R.LCD(ex, ey, None, eta, th)
+        o = R.vczt(lx, ly, x, y, 0.6, z, xo, yo)
+        return float(sum(np.sum(w[c] * np.abs(o[c]) ** 2) for c in range(3)))
+
+    h = 0.02
+    fd = (ref(z0 + h) - ref(z0 - h)) / (2 * h)
+    assert abs(float(gz) - fd) <= 2e-3 * abs(fd), (float(gz), fd)

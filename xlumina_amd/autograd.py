@@ -13,9 +13,11 @@ recomputed in the backward pass, so a chain keeps one field per stage alive.
 """
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass, field
 from typing import Optional, Sequence, Tuple
 
+import numpy as np
 import torch
 
 from . import _native as N
@@ -168,13 +170,16 @@ class CZTConfig:
 
 class _CZTFunction(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, cfg: CZTConfig, src, p00, p01, p10, p11):
+    def forward(ctx, cfg: CZTConfig, src, p00, p01, p10, p11, z_t):
         params = (p00, p01, p10, p11)
         masks = _rebuild(cfg.mask_struct, params)
         out = ops.czt(src, cfg.geom, masks=masks, ez_mode=cfg.ez_mode, nin=cfg.nin, mode=cfg.mode, form=cfg.form)
         ctx.cfg = cfg
-        ctx.save_for_backward(src, *[p if p is not None else torch.empty(0) for p in params])
+        keep_out = z_t is not None and ctx.needs_input_grad[6]
+        ctx.save_for_backward(src, *[p if p is not None else torch.empty(0) for p in params],
+                              out if keep_out else torch.empty(0))
         ctx.has = [p is not None for p in params]
+        ctx.z_meta = None if z_t is None else (z_t.shape, z_t.dtype, z_t.device)
         return out
 
     @staticmethod
@@ -187,6 +192,11 @@ class _CZTFunction(torch.autograd.Function):
         g_pre = ops.czt(gout.contiguous(), g, adjoint=True, mode=cfg.mode, form=cfg.form)
         masks = _rebuild(cfg.mask_struct, params)
         hna = cfg.mode == N.XL_CZT_HNA
+        g_z = None
+        if ctx.z_meta is not None and ctx.needs_input_grad[6]:
+            shape, dtype, device = ctx.z_meta
+            g_z = _czt_z_grad(cfg, src, masks, gout.contiguous(), g_pre, saved[5]).to(device=device, dtype=dtype)
+            g_z = g_z.reshape(shape)
         need_params = ((ctx.needs_input_grad[2] and params[0] is not None, ctx.needs_input_grad[3] and params[1] is not None),
                        (ctx.needs_input_grad[4] and params[2] is not None, ctx.needs_input_grad[5] and params[3] is not None))
         if not masks and cfg.ez_mode == N.XL_EZ_NONE and not hna:
@@ -202,16 +212,100 @@ class _CZTFunction(torch.autograd.Function):
                 g_src = full
         else:
             g_src = None
-        return (None, g_src, *_param_grads(ctx, cfg.mask_struct, params, gp))
+        return (None, g_src, *_param_grads(ctx, cfg.mask_struct, params, gp), g_z)
+
+
+def _dlnh_dz(X, Y, z: float, k: float):
+    """d/dz log h for h = transfer_function_RS (wave_optics.py:291-297):
+    h = (1/2pi) (z/r^2)(1/r - i k) e^{i s k r}, s = sign(z), r^2 = X^2 + Y^2 + z^2."""
+    r2 = X * X + Y * Y + z * z
+    r = torch.sqrt(r2)
+    s = 1.0 if z > 0 else -1.0
+    return 1.0 / z - 2.0 * z / r2 - (z / (r2 * r)) / (1.0 / r - 1j * k) + 1j * s * k * z / r
+
+
+def _czt_axis_weights(out_grid, m: int, dm: float, dev):
+    """The z-derivative of the CZT kernel of one axis (the dense form of
+    Bluestein_method, wave_optics.py:412-460): with W = out[-1] - out[0],
+    Delta = W / M and D1 = C + Dm (C = out[0] + W / (2M)), the kernel phase is
+    psi(k, j) - 2 pi j' with psi = (p_k - j' q_k), p_k = 2 pi Delta (k + 1/2 -
+    (m-1)/2) / Dm, q_k = 2 pi (C + (k+1) Delta) / Dm, j' = j - (m-1)/2, and
+    Dm = lambda z / dx makes d phase / dz = -psi / z."""
+    o = np.asarray(out_grid, dtype=np.float64)
+    M = len(o)
+    W = o[-1] - o[0]
+    delta = W / M
+    C = o[0] + W / (2 * M)
+    k = np.arange(M, dtype=np.float64)
+    p = 2 * np.pi * delta * (k + 0.5 - (m - 1) / 2) / dm
+    q = 2 * np.pi * (C + (k + 1) * delta) / dm
+    jp = np.arange(m, dtype=np.float64) - (m - 1) / 2
+    t = lambda a: torch.from_numpy(a).to(dev)
+    return t(p), t(q), t(jp)
+
+
+def _czt_z_grad(cfg: CZTConfig, src, masks, gout, g_pre, out):
+    """dL/dz of CZT / VCZT (wave_optics.py:333-357, vectorized_optics.py:321-361),
+    out = c F0 (K_y (E' F) K_x^T) with c = z dx dy lambda, F, F0 the transfer
+    function on the input / output grids, K the per-axis kernels of Dm =
+    lambda z / dx and E' the pre-op'd input (for VCZT with Ez_in = (Ex X +
+    Ey Y) z / r^2):  Re<g, out>/z  +  Re<g, (dF0/F0) out>  +  the two kernel
+    terms Re<g, (-i/z)(p (.) out - q (.) CZT(j' E'))> per axis (output rows /
+    columns weighted by p, q; the input by j')  +  Re<CZT^H g, (dF/F) E'>  +
+    (VCZT) Re<g_pre[2], dEz_in/dz>.  Sums in float64."""
+    g = cfg.geom
+    if cfg.mode != N.XL_CZT_RS:
+        raise ValueError("distance gradients exist for the RS-mode CZT only")
+    dev = src.device
+    z, k = g.z, 2 * math.pi / g.wavelength
+    ncomp = out.shape[0]
+    if masks or cfg.ez_mode != N.XL_EZ_NONE:
+        pre = ops.pre_forward(src, g.x, g.y, z, masks=masks, ez_mode=cfg.ez_mode, ncomp=ncomp, nin=cfg.nin)
+    else:
+        pre = src[:ncomp]
+
+    def rdot(a, b):
+        return torch.real(torch.sum(a.conj().to(torch.complex128) * b.to(torch.complex128)))
+
+    py, qy, jy = _czt_axis_weights(g.yout, len(g.y), g.dm, dev)
+    px, qx, jx = _czt_axis_weights(g.xout, len(g.x), g.dm, dev)
+    pre_y = (pre * jy.to(torch.float32)[None, :, None]).contiguous()
+    pre_x = (pre * jx.to(torch.float32)[None, None, :]).contiguous()
+    wy = ops.czt(pre_y, g, nin=ncomp, mode=N.XL_CZT_RS, form=cfg.form)
+    wx = ops.czt(pre_x, g, nin=ncomp, mode=N.XL_CZT_RS, form=cfg.form)
+    o = out.to(torch.complex128)
+    total = rdot(gout, o) / z
+    Xo = torch.from_numpy(np.asarray(g.xout, dtype=np.float64)).to(dev)[None, :]
+    Yo = torch.from_numpy(np.asarray(g.yout, dtype=np.float64)).to(dev)[:, None]
+    total = total + rdot(gout, o * _dlnh_dz(Xo, Yo, z, k))
+    total = total + rdot(gout, (-1j / z) * (py[None, :, None] * o - qy[None, :, None] * wy.to(torch.complex128)))
+    total = total + rdot(gout, (-1j / z) * (px[None, None, :] * o - qx[None, None, :] * wx.to(torch.complex128)))
+    Xi = torch.from_numpy(np.asarray(g.x, dtype=np.float64)).to(dev)[None, :]
+    Yi = torch.from_numpy(np.asarray(g.y, dtype=np.float64)).to(dev)[:, None]
+    # Re<CZT^H g, (dF/F) E'>: dF/F holds i s k z / r ~ i k, whose product with
+    # sum conj(g_pre) E' cancels only in the sum (adjointness makes that sum
+    # <g, out>).  The constant i s k is taken out and applied to <g, out>, so the
+    # per-pixel float32 rounding of g_pre meets weights of k (z/r - 1) only.
+    s_k = (1.0 if z > 0 else -1.0) * k
+    total = total + rdot(g_pre[:ncomp], pre.to(torch.complex128) * (_dlnh_dz(Xi, Yi, z, k) - 1j * s_k))
+    total = total + torch.real(1j * s_k * torch.sum(gout.conj().to(torch.complex128) * o))
+    if cfg.ez_mode == N.XL_EZ_VCZT:
+        r2 = Xi * Xi + Yi * Yi + z * z
+        dez = (pre[0].to(torch.complex128) * Xi + pre[1].to(torch.complex128) * Yi) * ((r2 - 2 * z * z) / (r2 * r2))
+        total = total + rdot(g_pre[2], dez)
+    return total
 
 
 def czt(src: torch.Tensor, geom: ops.CztGeometry, masks: Sequence[MaskSpec] = (), ez_mode: int = N.XL_EZ_NONE,
-        nin: Optional[int] = None, mode: Optional[int] = None, form: Optional[str] = None) -> torch.Tensor:
-    """Differentiable CZT / VCZT / high-NA VCZT of src (C, ny, nx); `form` as ops.czt."""
+        nin: Optional[int] = None, mode: Optional[int] = None, form: Optional[str] = None,
+        z: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Differentiable CZT / VCZT / high-NA VCZT of src (C, ny, nx); `form` as
+    ops.czt.  `z`: the distance as a tensor (geom holds its value) when dL/dz
+    is wanted (RS-mode CZT only)."""
     nin = src.shape[0] if nin is None else nin
     mode = geom.mode if mode is None else mode
     cfg = CZTConfig(geom, ez_mode, nin, mode, _mask_structure(masks), ops.resolve_czt_form(geom, form))
-    return _CZTFunction.apply(cfg, src, *_flat_params(masks))
+    return _CZTFunction.apply(cfg, src, *_flat_params(masks), z)
 
 
 @dataclass

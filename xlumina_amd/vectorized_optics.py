@@ -20,7 +20,7 @@ from . import _native as N
 from . import autograd as AG
 from . import ops
 from .ops import MaskSpec
-from .wave_optics import _GridMixin, _field_tensor, _grid, default_device, gaussian_envelope, minimum_z
+from .wave_optics import _GridMixin, _field_tensor, _grid, _zf, default_device, gaussian_envelope, minimum_z
 from . import wave_optics as _wo
 
 
@@ -104,8 +104,9 @@ class VectorizedLight(_GridMixin):
         xout = self.x if xout is None else _grid(xout)
         yout = self.y if yout is None else _grid(yout)
         src, masks = self._pending()
-        geom = ops.czt_geometry(self.x, self.y, xout, yout, float(z), self.wavelength)
-        out = AG.czt(src, geom, masks=masks, ez_mode=N.XL_EZ_VCZT, nin=2)
+        zt = z if isinstance(z, torch.Tensor) and z.requires_grad else None    # dL/dz wanted
+        geom = ops.czt_geometry(self.x, self.y, xout, yout, _zf(z), self.wavelength)
+        out = AG.czt(src, geom, masks=masks, ez_mode=N.XL_EZ_VCZT, nin=2, z=zt)
         light = VectorizedLight(xout, yout, self.wavelength, self.device)
         light._E = out
         if _wo.VERBOSE:

@@ -169,8 +169,9 @@ class ScalarLight(_GridMixin):
         xout = self.x if xout is None else _grid(xout)
         yout = self.y if yout is None else _grid(yout)
         src, masks = self._pending()
-        geom = ops.czt_geometry(self.x, self.y, xout, yout, float(z), self.wavelength)
-        out = AG.czt(src, geom, masks=masks)
+        zt = z if isinstance(z, torch.Tensor) and z.requires_grad else None    # dL/dz wanted
+        geom = ops.czt_geometry(self.x, self.y, xout, yout, _zf(z), self.wavelength)
+        out = AG.czt(src, geom, masks=masks, z=zt)
         light = ScalarLight(xout, yout, self.wavelength, self.device)
         light._field = out[0]
         if VERBOSE:
