@@ -78,7 +78,8 @@ typedef struct xl_rs_params {
   const double* z_dev;     /* optional device copy of z read by the kernels instead of `z`
                               (lets a captured HIP graph replay with a new distance) */
   int32_t kernel_dz;       /* 1: convolve with dH/dz instead of H (derivative of the
-                              propagation with respect to z, for gradients in z) */
+                              propagation with respect to z, for gradients in z); 2: with
+                              dH/dk, k = 2 pi / lambda (for gradients in the wavelength) */
   int32_t hhat_mode;       /* 0: the kernel spectrum is built in the workspace; 1: built into `hhat`
                               and kept there; 2: read from `hhat` as a mode-1 call with the same n,
                               x, y, z, k, dx, dy left it, skipping the two kernel passes (an adjoint

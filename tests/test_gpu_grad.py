@@ -398,3 +398,42 @@ def test_vczt_grad_wrt_z_vs_fd(cuda_dev):
     h = 0.02
     fd = (ref(z0 + h) - ref(z0 - h)) / (2 * h)
     assert abs(float(gz) - fd) <= 2e-3 * abs(fd), (float(gz), fd)
+
+
+@pytest.mark.parametrize("vectorial", [False, True])
+def test_rs_grad_wrt_wavelength_vs_fd(cuda_dev, vectorial):
+    """dL/dlambda of RS / VRS (after a per-pixel mask): the convolution with
+    dH/dk times dk/dlambda = -k/lambda, against central differences of the
+    float64 oracle in the wavelength."""
+    n = 64
+    x, y = R.space(120.0, n)
+    rng = np.random.default_rng(95)
+    ex, ey = rng.standard_normal((2, n, n)) + 1j * rng.standard_normal((2, n, n))
+    a, b = rng.uniform(-np.pi, np.pi, (2, n, n))
+    w = rng.random((3 if vectorial else 1, n, n))
+    wl0, z = 0.6, 600.0
+    wl = torch.tensor([wl0], dtype=torch.float64, device=cuda_dev, requires_grad=True)
+    if vectorial:
+        src = torch.tensor(np.stack([ex, ey]).astype(np.complex64), device=cuda_dev)
+        masks = (MaskSpec(N.XL_MASK_LCD, torch.tensor(a, dtype=torch.float32, device=cuda_dev),
+                          torch.tensor(b, dtype=torch.float32, device=cuda_dev)),)
+        out = AG.rs(src, x, y, z, wl, masks=masks, ez_mode=N.XL_EZ_VRS)
+    else:
+        src = torch.tensor(ex[None].astype(np.complex64), device=cuda_dev)
+        masks = (MaskSpec(N.XL_MASK_SLM, torch.tensor(a, dtype=torch.float32, device=cuda_dev)),)
+        out = AG.rs(src, x, y, z, wl, masks=masks)
+    L = (torch.tensor(w, dtype=torch.float32, device=cuda_dev) * out.abs() ** 2).sum()
+    (g,) = torch.autograd.grad(L, (wl,))
+
+    def ref(lam):
+        if vectorial:
+            lx, ly, _ = R.LCD(ex, ey, None, a, b)
+            (ox, oy, oz), _ = R.vrs_propagation(lx, ly, x, y, lam, z)
+            o = np.stack([ox, oy, oz])
+        else:
+            o = R.rs_propagation(ex * np.exp(1j * a), x, y, lam, z)[0][None]
+        return float(np.sum(w * np.abs(o) ** 2))
+
+    h = 1e-6
+    fd = (ref(wl0 + h) - ref(wl0 - h)) / (2 * h)
+    assert abs(float(g) - fd) <= 2e-3 * abs(fd), (float(g), fd)

@@ -77,7 +77,7 @@ class RSConfig:
 
 class _RSFunction(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, cfg: RSConfig, src, p00, p01, p10, p11, z_t):
+    def forward(ctx, cfg: RSConfig, src, p00, p01, p10, p11, z_t, wl_t):
         params = (p00, p01, p10, p11)
         masks = _rebuild(cfg.mask_struct, params)
         # a step that will run the adjoint keeps the kernel spectrum for it
@@ -91,6 +91,7 @@ class _RSFunction(torch.autograd.Function):
                               hhat if hhat is not None else torch.empty(0))
         ctx.has = [p is not None for p in params]
         ctx.z_meta = None if z_t is None else (z_t.shape, z_t.dtype, z_t.device)
+        ctx.wl_meta = None if wl_t is None else (wl_t.shape, wl_t.dtype, wl_t.device)
         return out
 
     @staticmethod
@@ -110,6 +111,14 @@ class _RSFunction(torch.autograd.Function):
         if ctx.needs_input_grad[6]:
             shape, dtype, device = ctx.z_meta
             g_z = _rs_z_grad(cfg, src, masks, gout, g_pre).to(device=device, dtype=dtype).reshape(shape)
+        g_wl = None
+        if ctx.needs_input_grad[7]:
+            # dL/dlambda = -(k/lambda) Re<g, conv(E', dH/dk)>; Ez_in does not depend on lambda
+            shape, dtype, device = ctx.wl_meta
+            d = ops.rs_propagate(src, cfg.x, cfg.y, cfg.z, cfg.wavelength, masks=masks, ez_mode=cfg.ez_mode,
+                                 nin=cfg.nin, kernel_dz=2, z_dev=cfg.z_dev)
+            k = 2 * math.pi / cfg.wavelength
+            g_wl = (-(k / cfg.wavelength) * ops.dz_dot(d, gout)).to(device=device, dtype=dtype).reshape(shape)
         if not masks and cfg.ez_mode == N.XL_EZ_NONE:
             g_src, gp = g_pre, [[None, None], [None, None]]
         else:
@@ -123,7 +132,7 @@ class _RSFunction(torch.autograd.Function):
                 g_src = full
         else:
             g_src = None
-        return (None, g_src, *_param_grads(ctx, cfg.mask_struct, params, gp), g_z)
+        return (None, g_src, *_param_grads(ctx, cfg.mask_struct, params, gp), g_z, g_wl)
 
 
 def _rs_z_grad(cfg: RSConfig, src, masks, gout, g_pre):
@@ -144,8 +153,12 @@ def rs(src: torch.Tensor, x, y, z, wavelength: float, masks: Sequence[MaskSpec] 
        ez_mode: int = N.XL_EZ_NONE, nin: Optional[int] = None) -> torch.Tensor:
     """Differentiable RS / VRS propagation of src (C, n, n).  `z` may be a
     float or a one-element tensor; with requires_grad the distance receives
-    dL/dz (convolution with the analytic dH/dz)."""
+    dL/dz (convolution with the analytic dH/dz).  A one-element `wavelength`
+    tensor with requires_grad receives dL/dlambda (convolution with dH/dk)."""
     nin = src.shape[0] if nin is None else nin
+    wl_t = wavelength if isinstance(wavelength, torch.Tensor) and wavelength.requires_grad else None
+    if isinstance(wavelength, torch.Tensor):
+        wavelength = float(wavelength.detach().reshape(-1)[0])
     z_t = z if isinstance(z, torch.Tensor) else None
     if z_t is not None:
         # the kernels read the distance from device memory: no host round trip
@@ -155,7 +168,7 @@ def rs(src: torch.Tensor, x, y, z, wavelength: float, masks: Sequence[MaskSpec] 
         cfg = RSConfig(x, y, 0.0, float(wavelength), ez_mode, nin, _mask_structure(masks), z_dev)
     else:
         cfg = RSConfig(x, y, float(z), float(wavelength), ez_mode, nin, _mask_structure(masks))
-    return _RSFunction.apply(cfg, src, *_flat_params(masks), z_t)
+    return _RSFunction.apply(cfg, src, *_flat_params(masks), z_t, wl_t)
 
 
 @dataclass

@@ -44,7 +44,7 @@ struct RsArgs {
   const double* z_dev;    // if set, z is read from device memory
   float scale;            // dx*dy/P^2
   int n, nin, ncomp, ez_mode, adjoint;
-  int kernel_dz;          // 1: convolve with dH/dz instead of H
+  int kernel_dz;          // 1: convolve with dH/dz instead of H, 2: with dH/dk
   int hstored;            // the spectrum lives in a caller buffer, unconjugated (adjoints conjugate on read)
   int hreuse;             // the spectrum is already there: skip rs_hrow / rs_hcol
   MaskArgs mask;
@@ -85,6 +85,19 @@ __device__ __forceinline__ float2 rs_kernel_dz_value(double X, double Y, double 
   return make_float2(fr * e.x - fi * e.y, fr * e.y + fi * e.x);
 }
 
+// dH/dk of rs_kernel_value (k = 2 pi / lambda; r and z do not depend on k):
+//   (1/2pi)(z/r^2) e^{i s k r} [ s k r + i (s - 1) ].
+__device__ __forceinline__ float2 rs_kernel_dk_value(double X, double Y, double z, double k) {
+  const double r2 = X * X + Y * Y + z * z;
+  const double r = sqrt(r2);
+  const double s = z > 0 ? 1.0 : -1.0;
+  const double turns = k * r * 0.15915494309189535;
+  const float2 e = expi_turns(s * turns);
+  const double f = 0.15915494309189535 * z / r2;
+  const float fr = (float)(f * s * k * r), fi = (float)(f * (s - 1.0));
+  return make_float2(fr * e.x - fi * e.y, fr * e.y + fi * e.x);
+}
+
 // Hr tile index of (row r, column c): P/8 row groups
 __device__ __forceinline__ int64_t hr_idx(int r, int c, int P) {
   return ((int64_t)(c >> 2) * (P / 8) + (r >> 2)) * 16 + (r & 3) * 4 + (c & 3);
@@ -115,7 +128,9 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_hrow_kernel(Rs
     double dx;
     float2 v = make_float2(0.f, 0.f);
     if (rowok && lag_of(a.x, c, a.n, P, &dx))
-      v = a.kernel_dz ? rs_kernel_dz_value(dx, dy, z, a.k) : rs_kernel_value(dx, dy, z, a.k);
+      v = a.kernel_dz == 1   ? rs_kernel_dz_value(dx, dy, z, a.k)
+          : a.kernel_dz == 2 ? rs_kernel_dk_value(dx, dy, z, a.k)
+                             : rs_kernel_value(dx, dy, z, a.k);
     s[lpad(c)] = v;
     if (c > 0 && c < P / 2) s[lpad(P - c)] = v;
   }

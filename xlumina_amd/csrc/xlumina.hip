@@ -291,7 +291,8 @@ int xl_rs_propagate(const xl_rs_params* p, void* ws, size_t ws_bytes, void* stre
   a.z = p->z;
   a.k = p->k;
   a.z_dev = p->z_dev;
-  a.kernel_dz = p->kernel_dz ? 1 : 0;
+  if (p->kernel_dz < 0 || p->kernel_dz > 2) return fail(XL_EINVAL, "kernel_dz must be 0, 1 (dH/dz) or 2 (dH/dk)");
+  a.kernel_dz = p->kernel_dz;
   a.n = n;
   a.nin = p->nin;
   a.ncomp = ncomp;

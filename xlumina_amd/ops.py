@@ -204,7 +204,7 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
                  ez_mode: int = N.XL_EZ_NONE, nin: Optional[int] = None, intensity: bool = False,
                  adjoint: bool = False, out: Optional[torch.Tensor] = None,
                  workspace: Optional[torch.Tensor] = None, z_dev: Optional[torch.Tensor] = None,
-                 inten: Optional[torch.Tensor] = None, kernel_dz: bool = False,
+                 inten: Optional[torch.Tensor] = None, kernel_dz: int = 0,
                  hhat: Optional[torch.Tensor] = None, hhat_mode: int = 0):
     """RS_propagation_jit / VRS_propagation_jit (wave_optics.py:281-289,
     vectorized_optics.py:364-373) with fused masks and Ez construction.
@@ -213,8 +213,8 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
     Returns the (ncomp, n, n) field, plus |E|^2 if `intensity`.
     z_dev: optional float64 device scalar read by the kernels instead of `z`
     (a captured graph then replays with whatever distance it holds).
-    kernel_dz: convolve with dH/dz instead of H (the z-derivative of the
-    propagation of the pre-op'd field).
+    kernel_dz: 1 (or True) convolves with dH/dz instead of H (the z-derivative
+    of the propagation of the pre-op'd field), 2 with dH/dk (k = 2 pi/lambda).
     hhat, hhat_mode: 1 builds the kernel spectrum into `hhat` (a device buffer
     of rs_hhat_bytes(n)) and keeps it; 2 reuses what a mode-1 call of the same
     geometry and distance left there (an adjoint call conjugates it).
@@ -256,7 +256,7 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
     p.dx = float(xa[1] - xa[0])
     p.dy = float(ya[1] - ya[0])
     p.z_dev = _z_ptr(z_dev, dev)
-    p.kernel_dz = 1 if kernel_dz else 0
+    p.kernel_dz = int(kernel_dz)
     if hhat_mode:
         if hhat is None or hhat.device != dev or hhat.numel() * hhat.element_size() < rs_hhat_bytes(n):
             raise ValueError("hhat must be a device buffer of rs_hhat_bytes(n) bytes on the field's device")
