@@ -80,7 +80,7 @@ def test_golden_vczt_and_hna(golden, cuda_dev):
 
 # -- oracle at moderate and workload sizes --------------------------------------------------
 
-@pytest.mark.parametrize("n,z", [(37, 400.0), (250, 2000.0), (1000, 5e4), (1024, -3e4)])
+@pytest.mark.parametrize("n,z", [(20, 150.0), (37, 400.0), (250, 2000.0), (300, 1500.0), (1000, 5e4), (1024, -3e4)])
 def test_rs_vs_oracle(n, z, cuda_dev):
     rng = np.random.default_rng(n)
     x, y = R.space(1200.0 * n / 1024, n)
@@ -90,6 +90,23 @@ def test_rs_vs_oracle(n, z, cuda_dev):
     out, inten = ops.rs_propagate(T(u, cuda_dev), x, y, z, 0.65, intensity=True)
     assert rel_l2(H(out[0]), ref) < TOL
     assert rel_l2(inten.cpu().numpy(), np.abs(ref) ** 2) < 2 * TOL
+
+
+@pytest.mark.parametrize("n", [20, 300])
+def test_vrs_lcd_vs_oracle_small_and_1024_fft(n, cuda_dev):
+    """VRS with a per-pixel LCD at FFT lengths 64 and 1024 (the smallest plan
+    and the first that runs the one-row-per-block row passes)."""
+    rng = np.random.default_rng(500 + n)
+    x, y = R.space(2.0 * n, n)
+    ex, ey = rng.standard_normal((2, n, n)) + 1j * rng.standard_normal((2, n, n))
+    eta, th = rng.uniform(0, 2 * np.pi, (2, n, n))
+    lx, ly, _ = R.LCD(ex, ey, None, eta, th)
+    (ox, oy, oz), _ = R.vrs_propagation(lx, ly, x, y, 0.6, 3.0 * n)
+    src = torch.tensor(np.stack([ex, ey]).astype(np.complex64), device=cuda_dev)
+    m = ops.MaskSpec(N.XL_MASK_LCD, torch.tensor(eta, dtype=torch.float32, device=cuda_dev),
+                     torch.tensor(th, dtype=torch.float32, device=cuda_dev))
+    out = ops.rs_propagate(src, x, y, 3.0 * n, 0.6, masks=[m], ez_mode=N.XL_EZ_VRS, nin=2)
+    assert rel_l2(out.cpu().numpy(), np.stack([ox, oy, oz])) < TOL
 
 
 def test_config1_slm_fused(cuda_dev):
