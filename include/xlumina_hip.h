@@ -114,7 +114,11 @@ typedef struct xl_czt_params {
   /* Batch of independent problems sharing this geometry, run by the same
    * launches (Bluestein form only).  batch <= 1: one problem. */
   int32_t batch;
-  int32_t reserved2;
+  int32_t tables_mode;         /* Bluestein form: 0 stages the axis tables (chirps and their spectra) in
+                                  this call; 1 stages them into the workspace and returns (no other
+                                  launch, in / out untouched); 2 uses the tables a mode-1 call with the
+                                  same parameters staged in this workspace -- so a caller can stage
+                                  them on another stream while the input is still being produced */
   int64_t in_batch_stride;    /* complex elements between problems' inputs (0: one shared input) */
   int64_t out_batch_stride;   /* complex elements between problems' outputs */
   int64_t mask_batch_stride;  /* float elements between problems' per-pixel mask parameters (0: shared) */

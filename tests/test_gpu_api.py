@@ -173,3 +173,30 @@ def test_training_pair_loader_delivers_device_batches(cuda_dev, tmp_path):
     assert a.device.type == "cuda" and a.shape == (3, 32, 32)
     np.testing.assert_array_equal(a.cpu().numpy(), xin[L.last_indices])
     np.testing.assert_array_equal(b.cpu().numpy().real, xout[L.last_indices])
+
+
+def test_staged_czt_tables_and_overlapped_chain(cuda_dev):
+    """CZT axis tables staged by one call (tables_mode 1, here also on another
+    stream) and used by the next (mode 2) give exactly the plain call's output."""
+    import torch
+    from xlumina_amd import _native as N
+    from xlumina_amd import ops, workloads
+    w = workloads.vrs_vczt_2048(cuda_dev, n=256, m_out=64)
+    g = w.geometry()
+    vrs, ref = w.run(g, czt_fn=lambda v, gg, **kw: ops.czt(v, gg, form="bluestein", **kw))
+    ws = torch.empty(ops.czt_workspace_bytes(g, ncomp=3, nin=2, ez_mode=N.XL_EZ_VCZT), dtype=torch.uint8,
+                     device=cuda_dev)
+    out = torch.empty_like(ref)
+    ops.czt(vrs, g, ez_mode=N.XL_EZ_VCZT, nin=2, out=out, workspace=ws, form="bluestein", tables_mode=1)
+    assert float(out.abs().sum()) >= 0.0                # staging leaves `out` alone (any values)
+    ops.czt(vrs, g, ez_mode=N.XL_EZ_VCZT, nin=2, out=out, workspace=ws, form="bluestein", tables_mode=2)
+    assert torch.equal(out, ref)
+    side = torch.cuda.Stream(device=cuda_dev)          # staged on another stream, then joined
+    for _ in range(3):                                   # the workspace is reused call after call
+        out2 = torch.empty_like(ref)
+        side.wait_stream(torch.cuda.current_stream(cuda_dev))
+        with torch.cuda.stream(side):
+            ops.czt(vrs, g, ez_mode=N.XL_EZ_VCZT, nin=2, out=out2, workspace=ws, form="bluestein", tables_mode=1)
+        torch.cuda.current_stream(cuda_dev).wait_stream(side)
+        ops.czt(vrs, g, ez_mode=N.XL_EZ_VCZT, nin=2, out=out2, workspace=ws, form="bluestein", tables_mode=2)
+        assert torch.equal(out2, ref)

@@ -48,7 +48,7 @@ class CztParams(C.Structure):
                 ("z", C.c_double), ("wavelength", C.c_double), ("k", C.c_double), ("dm", C.c_double),
                 ("fy1", C.c_double), ("fy2", C.c_double), ("fx1", C.c_double), ("fx2", C.c_double),
                 ("cte_re", C.c_double), ("cte_im", C.c_double), ("hna_radius", C.c_double), ("hna_f", C.c_double),
-                ("mask", MaskStage * 2), ("batch", C.c_int32), ("reserved2", C.c_int32),
+                ("mask", MaskStage * 2), ("batch", C.c_int32), ("tables_mode", C.c_int32),
                 ("in_batch_stride", C.c_int64), ("out_batch_stride", C.c_int64), ("mask_batch_stride", C.c_int64)]
 
 

@@ -366,18 +366,22 @@ int xl_czt(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream) {
     ax.lout = p->adjoint ? ap.m : ap.M;
     ax.woff = p->adjoint ? 0 : ap.m;
   }
-  if (cp.x.q == cp.y.q) {
-    if (int rc = dispatch_czt_q(cp.x.q, [&](auto c) { xl::czt_setup_launch<decltype(c)::value>(pair, 2, st); }))
-      return rc;
-  } else if (setup_pair_launch(cp.x.q, cp.y.q, pair, st)) {
-    // both axes in one launch
-  } else {
-    for (int i = 0; i < 2; ++i) {
-      xl::CztSetupPair one;
-      one.ax[0] = pair.ax[i];
-      if (int rc = dispatch_czt_q(axp[i]->q, [&](auto c) { xl::czt_setup_launch<decltype(c)::value>(one, 1, st); }))
+  if (p->tables_mode < 0 || p->tables_mode > 2) return fail(XL_EINVAL, "tables_mode must be 0, 1 or 2");
+  if (p->tables_mode != 2) {
+    if (cp.x.q == cp.y.q) {
+      if (int rc = dispatch_czt_q(cp.x.q, [&](auto c) { xl::czt_setup_launch<decltype(c)::value>(pair, 2, st); }))
         return rc;
+    } else if (setup_pair_launch(cp.x.q, cp.y.q, pair, st)) {
+      // both axes in one launch
+    } else {
+      for (int i = 0; i < 2; ++i) {
+        xl::CztSetupPair one;
+        one.ax[0] = pair.ax[i];
+        if (int rc = dispatch_czt_q(axp[i]->q, [&](auto c) { xl::czt_setup_launch<decltype(c)::value>(one, 1, st); }))
+          return rc;
+      }
     }
+    if (p->tables_mode == 1) return hip_check(hipGetLastError(), "CZT table staging");
   }
   a.in = (const float2*)p->in;
   a.in_cs = p->in_comp_stride;

@@ -375,7 +375,7 @@ def resolve_czt_form(g: "CztGeometry", form: Optional[str] = None) -> str:
 def czt(src: torch.Tensor, g: CztGeometry, masks: Sequence[MaskSpec] = (), ez_mode: int = N.XL_EZ_NONE,
         nin: Optional[int] = None, ncomp: Optional[int] = None, adjoint: bool = False, mode: Optional[int] = None,
         out: Optional[torch.Tensor] = None, workspace: Optional[torch.Tensor] = None,
-        form: Optional[str] = None, batch: Optional[int] = None) -> torch.Tensor:
+        form: Optional[str] = None, batch: Optional[int] = None, tables_mode: int = 0) -> torch.Tensor:
     """CZT_jit / VCZT_jit / CZT_for_high_NA_jit / VCZT_objective_lens core.
 
     Forward: src (C, ny, nx) -> (ncomp, my, mx).  Adjoint: src (ncomp, my, mx)
@@ -384,6 +384,9 @@ def czt(src: torch.Tensor, g: CztGeometry, masks: Sequence[MaskSpec] = (), ez_mo
     `batch`: B independent problems of this geometry in the same launches
     (Bluestein form): src (B, C, ...) or one shared (C, ...), mask parameters
     (B, rows, cols) views or shared, output (B, ncomp, ...).
+    `tables_mode` (Bluestein form, with a caller `workspace`): 1 only stages the
+    axis tables into the workspace (src and out are not touched), 2 uses the
+    tables a mode-1 call with the same arguments staged there.
     """
     if batch is not None:
         return _czt_batched(src, g, masks, ez_mode, nin, ncomp, adjoint, mode, out, workspace, batch)
@@ -412,6 +415,11 @@ def czt(src: torch.Tensor, g: CztGeometry, masks: Sequence[MaskSpec] = (), ez_mo
     need = wsfn(C.byref(p))
     if need == 0:
         N.check(runfn(C.byref(p), None, 0, _stream(dev)))   # raises with the plan error
+    if tables_mode:
+        if use_gemm or workspace is None or workspace.numel() < need:
+            raise ValueError("staged CZT tables need the Bluestein form and a caller workspace of "
+                             "czt_workspace_bytes")
+        p.tables_mode = int(tables_mode)
     if workspace is None or workspace.numel() < need:
         workspace = torch.empty(need, dtype=torch.uint8, device=dev)
     N.check(runfn(C.byref(p), workspace.data_ptr(), workspace.numel(), _stream(dev)))
