@@ -58,11 +58,11 @@ __device__ __forceinline__ float2 bs_mac(float2 k, float2 v, float2 acc) {
   return make_float2(acc.x + k.x * v.x - k.y * v.y, acc.y + k.x * v.y + k.y * v.x);
 }
 
-// Grid-stride over the 2 npix samples of (Ex, Ey); with zero_ez also over the
-// npix samples of component 2 of each output.
+// blockIdx.y = component (0, 1: Ex, Ey; 2 with zero_ez: the zeroed Ez of both
+// outputs), grid-stride over its npix samples along x.
 __global__ __launch_bounds__(256) void bs_kernel(BsArgs a) {
-  const int64_t n2 = 2 * a.npix, total = a.zero_ez ? 3 * a.npix : n2;
   const float2 zero = make_float2(0.f, 0.f);
+  const int comp = blockIdx.y;
   float2 ca = a.coef ? a.coef[0] : a.ca, cb = a.coef ? a.coef[1] : a.cb;
   float2 da = a.coef ? a.coef[2] : a.da, db = a.coef ? a.coef[3] : a.db;
   if (a.theta_mode) {   // formed once per block in float64, shared through LDS
@@ -81,18 +81,22 @@ __global__ __launch_bounds__(256) void bs_kernel(BsArgs a) {
     __syncthreads();
     ca = k4[0]; cb = k4[1]; da = k4[2]; db = k4[3];
   }
-  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
-    const int comp = (int)(e / a.npix);
-    const int64_t p = e - (int64_t)comp * a.npix;
-    if (comp == 2) {
+  if (comp == 2) {
+    for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < a.npix; p += (int64_t)gridDim.x * 256) {
       if (a.c) a.c[2 * a.ccs + p] = zero;
       if (a.d) a.d[2 * a.dcs + p] = zero;
-      continue;
     }
-    const float2 va = a.a[comp * a.acs + p];
-    const float2 vb = a.b ? a.b[comp * a.bcs + p] : zero;
-    if (a.c) a.c[comp * a.ccs + p] = bs_mac(cb, vb, bs_mac(ca, va, zero));
-    if (a.d) a.d[comp * a.dcs + p] = bs_mac(db, vb, bs_mac(da, va, zero));
+    return;
+  }
+  const float2* ap = a.a + comp * a.acs;
+  const float2* bp = a.b ? a.b + comp * a.bcs : nullptr;
+  float2* cp = a.c ? a.c + comp * a.ccs : nullptr;
+  float2* dp = a.d ? a.d + comp * a.dcs : nullptr;
+  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < a.npix; p += (int64_t)gridDim.x * 256) {
+    const float2 va = ap[p];
+    const float2 vb = bp ? bp[p] : zero;
+    if (cp) cp[p] = bs_mac(cb, vb, bs_mac(ca, va, zero));
+    if (dp) dp[p] = bs_mac(db, vb, bs_mac(da, va, zero));
   }
 }
 

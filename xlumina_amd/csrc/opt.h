@@ -458,19 +458,21 @@ __device__ __forceinline__ double block_sum_f64(double v, double* red) {
   return red[0];
 }
 
+// Rows are spread over the blocks (grid-stride), columns over the threads,
+// so a pixel's row and column need no division.
 __global__ __launch_bounds__(kDzThreads) void dz_dot_kernel(DzDotArgs a) {
   __shared__ double red[kDzThreads];
-  const int64_t npix = (int64_t)a.rows * a.cols;
   const float z = (float)(a.z_dev ? a.z_dev[0] : a.z);
   double acc = 0.0;
-  for (int64_t p = (int64_t)blockIdx.x * kDzThreads + threadIdx.x; p < npix; p += (int64_t)gridDim.x * kDzThreads) {
+  for (int i = blockIdx.x; i < a.rows; i += gridDim.x)
+  for (int j = threadIdx.x; j < a.cols; j += kDzThreads) {
+    const int64_t p = (int64_t)i * a.cols + j;
     float s = 0.f;
     for (int c = 0; c < a.nc; ++c) {
       const float2 dv = a.d[c * a.cs + p], gv = a.g[c * a.cs + p];
       s += dv.x * gv.x + dv.y * gv.y;
     }
     if (a.pre) {
-      const int i = (int)(p / a.cols), j = (int)(p - (int64_t)i * a.cols);
       const float X = (float)a.x[j], Y = (float)a.y[i];
       const float ir = rsqrtf(X * X + Y * Y + z * z);
       const float w = -z * ir * ir * ir;

@@ -1032,9 +1032,8 @@ int xl_beam_splitter(const xl_bs_params* p, void* stream) {
   a.adjoint = p->adjoint ? 1 : 0;
   hipStream_t st = (hipStream_t)stream;
   xl::KTimer t(xl::K_BS, st);
-  const int64_t total = (a.zero_ez ? 3 : 2) * p->npix;
-  const int64_t blocks = std::min<int64_t>((total + 255) / 256, 8192);
-  hipLaunchKernelGGL(xl::bs_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a);
+  const int64_t blocks = std::min<int64_t>((p->npix + 255) / 256, 4096);
+  hipLaunchKernelGGL(xl::bs_kernel, dim3((unsigned)blocks, a.zero_ez ? 3 : 2), dim3(256), 0, st, a);
   return hip_check(hipGetLastError(), "beam splitter launch");
 }
 
