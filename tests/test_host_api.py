@@ -93,3 +93,49 @@ def test_product_path_fails_loudly_without_gpu():
         l.RS_propagation(100.0)
     with pytest.raises(N.NativeUnavailable):
         l.CZT(1000.0)
+
+
+# -- argument validation before any pointer reaches a kernel (CPU: the checks are host logic) --
+
+def test_rs_argument_checks_reject_bad_buffers():
+    cpu = torch.device("cpu")
+    n = 8
+    good_out = torch.empty((3, n, n), dtype=torch.complex64)
+    good_i = torch.empty((n, n), dtype=torch.float32)
+    ops.check_rs_args((2, n, n), 2, 3, n, n, good_out, good_i, cpu)      # accepted
+    bad = [
+        dict(nin=3),                                                   # more components than the source holds
+        dict(nin=0),
+        dict(nx=n + 1),                                                # grid length differs from the field
+        dict(ny=n - 1),
+        dict(out=torch.empty((2, n, n), dtype=torch.complex64)),       # too few planes
+        dict(out=torch.empty((3, n, n), dtype=torch.complex128)),      # wrong dtype
+        dict(out=torch.empty((3, n, 2 * n), dtype=torch.complex64)[..., ::2]),   # not contiguous
+        dict(inten=torch.empty((n, n), dtype=torch.float64)),
+        dict(inten=torch.empty((n, n + 1), dtype=torch.float32)),
+    ]
+    for kw in bad:
+        args = dict(nin=2, nx=n, ny=n, out=good_out, inten=good_i)
+        args.update(kw)
+        with pytest.raises(ValueError):
+            ops.check_rs_args((2, n, n), args["nin"], 3, args["nx"], args["ny"], args["out"], args["inten"], cpu)
+    with pytest.raises(ValueError):       # a buffer on another device than the field
+        ops.check_rs_args((2, n, n), 2, 3, n, n, good_out, good_i, torch.device("meta"))
+
+
+def test_czt_argument_checks_reject_bad_buffers():
+    cpu = torch.device("cpu")
+    out = torch.empty((3, 5, 6), dtype=torch.complex64)
+    ops.check_czt_args((2, 8, 9), 2, 3, (8, 9), (5, 6), out, cpu)
+    with pytest.raises(ValueError):
+        ops.check_czt_args((2, 8, 9), 2, 3, (9, 8), (5, 6), out, cpu)      # input plane mismatch
+    with pytest.raises(ValueError):
+        ops.check_czt_args((2, 8, 9), 3, 3, (8, 9), (5, 6), out, cpu)      # nin above the source's components
+    with pytest.raises(ValueError):
+        ops.check_czt_args((2, 8, 9), 2, 3, (8, 9), (6, 5), out, cpu)      # output plane mismatch
+    with pytest.raises(ValueError):
+        ops.check_czt_args((2, 8, 9), 2, 2, (8, 9), (5, 6), out, cpu)      # component count mismatch
+    with pytest.raises(ValueError):
+        ops.check_czt_args((2, 8, 9), 2, 3, (8, 9), (5, 6), out.to(torch.complex128), cpu)
+    with pytest.raises(ValueError):
+        ops.check_workspace(torch.empty(16, dtype=torch.uint8), torch.device("meta"))

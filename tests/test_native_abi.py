@@ -141,3 +141,46 @@ def test_entry_points_refuse_without_init(lib):
     p.n = 64
     rc = lib.xl_rs_propagate(C.byref(p), None, 0, None)
     assert rc != 0
+
+
+FAKE = 1 << 20     # a non-null address the host checks never dereference (no kernel is launched)
+
+
+def test_gemm_entry_rejects_tables_mode(lib):
+    """xl_czt_gemm has no staged-table mode: tables_mode != 0 is an error
+    (checked on the host, before the initialisation check)."""
+    p = _czt_params(256, 64)
+    p.inp = p.out = p.x = p.y = p.xout = p.yout = FAKE
+    p.tables_mode = 1
+    assert lib.xl_czt_gemm(C.byref(p), None, 0, None) == N.XL_EINVAL
+    assert b"tables_mode" in lib.xl_last_error()
+
+
+def _slm_pre(batch, mask_stride, gparam_stride=0):
+    p = N.PreParams()
+    p.inp = p.x = p.y = p.gout = FAKE
+    p.nin = p.ncomp = 1
+    p.rows = p.cols = 8
+    p.comp_stride = p.in_comp_stride = 64
+    p.mask[0].kind = N.XL_MASK_SLM
+    p.mask[0].m0 = FAKE
+    p.batch = batch
+    p.mask_batch_stride = mask_stride
+    p.gparam_batch_stride = gparam_stride
+    return p
+
+
+def test_batched_adjoints_reject_shared_outputs(lib):
+    """Members of a batch must not share the arrays a kernel writes: a fused
+    Adam step on one shared phase array, or one shared parameter-gradient
+    array, would race (host checks; nothing is launched)."""
+    a = N.AdamParams()
+    a.param = a.m = a.v = FAKE
+    a.count = FAKE
+    a.lr, a.b1, a.b2, a.eps = 0.05, 0.9, 0.999, 1e-8
+    assert lib.xl_slm_backward_adam(C.byref(_slm_pre(4, 0)), C.byref(a), None) == N.XL_EINVAL
+    assert b"race" in lib.xl_last_error()
+    p = _slm_pre(4, 64)
+    p.gparam[0][0] = FAKE
+    assert lib.xl_pre_backward(C.byref(p), None) == N.XL_EINVAL
+    assert b"race" in lib.xl_last_error()

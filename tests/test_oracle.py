@@ -230,6 +230,37 @@ def test_jones_identities():
     assert np.allclose(S, np.diag(np.exp(1j * np.array([0.3, -1.1]))))
 
 
+def _rot2(t):
+    return np.array([[np.cos(t), -np.sin(t)], [np.sin(t), np.cos(t)]])
+
+
+def _retarder_axis_residual(jones_lcd):
+    """Largest deviation of a retarder J(eta, theta) from the one whose fast
+    axis is (cos theta, sin theta) (optical_elements.py:123-139, theta = angle
+    of the fast axis w.r.t. horizontal): J (c, s) = e^{-i eta/2} (c, s),
+    J (-s, c) = e^{+i eta/2} (-s, c), and J = R(theta) diag(e^{-i eta/2},
+    e^{i eta/2}) R(-theta).  Eigenphases alone cannot see the axis direction."""
+    worst = 0.0
+    for eta, th in np.random.default_rng(11).uniform(0, 2 * np.pi, (16, 2)):
+        J = np.asarray(jones_lcd(eta, th), dtype=np.complex128)
+        fast = np.array([np.cos(th), np.sin(th)])
+        slow = np.array([-np.sin(th), np.cos(th)])
+        built = _rot2(th) @ np.diag(np.exp(np.array([-0.5j, 0.5j]) * eta)) @ _rot2(-th)
+        worst = max(worst, np.abs(J @ fast - np.exp(-0.5j * eta) * fast).max(),
+                    np.abs(J @ slow - np.exp(0.5j * eta) * slow).max(), np.abs(J - built).max())
+    return worst
+
+
+def test_retarder_fast_axis_orientation():
+    """Pins the sign of theta in the retarder (the per-pixel LCD of the
+    2048^2 VRS workload): the oracle's matrix has its fast axis along
+    (cos theta, sin theta).  The same check applied to the oracle with theta's
+    sign flipped (the mistake that unitarity and the eigenphases both miss)
+    fails, which the second assertion demonstrates on the CPU."""
+    assert _retarder_axis_residual(R.jones_LCD) < 1e-12
+    assert _retarder_axis_residual(lambda eta, th: R.jones_LCD(eta, -th)) > 0.1
+
+
 # -- high-NA -----------------------------------------------------------------------------------
 
 def test_richards_wolf_matrix_is_rotation():
@@ -257,6 +288,69 @@ def test_high_na_lens_norm_and_constants():
     ez = (Ex * X + Ey * Y) / rho
     inten_in = np.abs(Ex) ** 2 + np.abs(Ey) ** 2 + np.abs(ez) ** 2
     assert np.allclose(np.sum(np.abs(out) ** 2, 0), w ** 2 * inten_in)
+
+
+def _rz(p):
+    return np.array([[np.cos(p), -np.sin(p), 0.0], [np.sin(p), np.cos(p), 0.0], [0.0, 0.0, 1.0]])
+
+
+def _ry(t):
+    return np.array([[np.cos(t), 0.0, np.sin(t)], [0.0, 1.0, 0.0], [-np.sin(t), 0.0, np.cos(t)]])
+
+
+def _hna_independent(Ex, Ey, x, y, radius, f):
+    """The lens map built pixel by pixel from rotations: E_f = sqrt|cos t| G
+    Rz(phi) Ry(-t) Rz(-phi) (Ex, Ey, Ez_in), phi the polar angle of (X, Y)
+    taken from the complex number X + iY, t = rho / f, Ez_in = (Ex X + Ey Y) /
+    rho, G and sin(theta_max) as optical_elements.py:528-545 define them
+    (the rows at :573-581 are this product written out)."""
+    stm = radius / np.hypot(radius, f)
+    n = len(x)
+    out = np.zeros((3, n, n), dtype=np.complex128)
+    for i in range(n):
+        for j in range(n):
+            X, Y = x[j], y[i]
+            rho = abs(complex(X, Y))
+            phi = np.angle(complex(X, Y))
+            t = rho / f
+            G = 1.0 / np.sqrt(abs(1 - (rho / radius) ** 2 * stm ** 2)) if rho < radius else 0.0
+            e = np.array([Ex[i, j], Ey[i, j], (Ex[i, j] * X + Ey[i, j] * Y) / rho])
+            out[:, i, j] = np.sqrt(abs(np.cos(t))) * G * (_rz(phi) @ _ry(-t) @ _rz(-phi) @ e)
+    return out
+
+
+def _hna_orientation_residual(n=12):
+    x, y = R.space(1800.0, n)
+    rng = np.random.default_rng(12)
+    Ex = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    Ey = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    got, _ = R.high_NA_objective_lens(Ex, Ey, x, y, 1800.0, 2000.0)
+    ref = _hna_independent(Ex, Ey, x, y, 1800.0, 2000.0)
+    # azimuthal polarisation (-sin phi, cos phi) has Ez_in = 0 and is left
+    # unchanged by the map apart from the factor sqrt|cos t| G
+    X, Y = np.meshgrid(x, y)
+    rho, ph = np.hypot(X, Y), np.arctan2(Y, X)
+    g = np.exp(-rho ** 2 / 900.0 ** 2)
+    az, _ = R.high_NA_objective_lens(-np.sin(ph) * g + 0j, np.cos(ph) * g + 0j, x, y, 1800.0, 2000.0)
+    G = np.where(rho < 1800.0, 1 / np.sqrt(np.abs(1 - (rho / 1800.0) ** 2 * (1800.0 / np.hypot(1800.0, 2000.0)) ** 2)),
+                 0.0)
+    w = np.sqrt(np.abs(np.cos(rho / 2000.0))) * G
+    az_ref = np.stack([-np.sin(ph) * g * w, np.cos(ph) * g * w, np.zeros_like(g)])
+    return max(rel_l2(got, ref), rel_l2(az, az_ref))
+
+
+def test_high_na_phi_orientation(monkeypatch):
+    """Pins the orientation of phi in the Richards-Wolf map (the high-NA
+    focusing workload): the oracle equals the rotation product Rz(phi) Ry(-t)
+    Rz(-phi) built per pixel with phi taken independently, and passes an
+    azimuthally polarised beam through unchanged apart from sqrt|cos t| G.
+    Orthogonality, det = 1 and the phi = 0 case cannot see the sign of
+    sin(phi); the second half plants that mistake in the oracle (sin(phi)
+    replaced by -sin(phi)) and shows that this check fails on it."""
+    assert _hna_orientation_residual() < 1e-12
+    orig = R.richards_wolf_matrix
+    monkeypatch.setattr(R, "richards_wolf_matrix", lambda c_t, s_t, c_p, s_p: orig(c_t, s_t, c_p, -s_p))
+    assert _hna_orientation_residual() > 0.05
 
 
 def test_hna_vczt_equals_dense_form(golden):

@@ -20,6 +20,7 @@ class XlError(RuntimeError):
     pass
 
 
+XL_OK, XL_EINVAL, XL_EHIP, XL_EWORKSPACE = range(4)
 XL_MASK_NONE, XL_MASK_SLM, XL_MASK_SSLM, XL_MASK_LCD, XL_MASK_LCD_SCALAR, XL_MASK_LP, XL_MASK_AMP = range(7)
 XL_EZ_NONE, XL_EZ_VRS, XL_EZ_VCZT, XL_EZ_HNA = range(4)
 XL_CZT_RS, XL_CZT_HNA, XL_CZT_PLAIN = range(3)

@@ -343,15 +343,6 @@ inline void czt_rows_launch(const CztArgs& a, int batch, hipStream_t st) {
   KTimer t(K_CZT_ROWS, st);
   const dim3 g(a.rin, batch), b(a.ncomp * Q / 16);
   const size_t lds = (size_t)a.ncomp * LdsLen<Q>::value * sizeof(float2);
-  static bool attr_set = false;   // allow up to three 4096-point buffers of dynamic LDS
-  if (!attr_set) {
-    const int mx = 3 * LdsLen<Q>::value * (int)sizeof(float2);
-    (void)hipFuncSetAttribute((const void*)czt_rows_kernel<Q, PREK_F>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
-    (void)hipFuncSetAttribute((const void*)czt_rows_kernel<Q, PREK_HNA>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
-    (void)hipFuncSetAttribute((const void*)czt_rows_kernel<Q, PREK_SIMPLE>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              mx);
-    attr_set = true;
-  }
   if (a.pre_mode == PRE_RS || a.pre_mode == PRE_CONJ_RS)
     hipLaunchKernelGGL((czt_rows_kernel<Q, PREK_F>), g, b, lds, st, a);
   else if (a.pre_mode == PRE_HNA)
@@ -359,6 +350,28 @@ inline void czt_rows_launch(const CztArgs& a, int batch, hipStream_t st) {
   else
     hipLaunchKernelGGL((czt_rows_kernel<Q, PREK_SIMPLE>), g, b, lds, st, a);
 }
+// Dynamic-LDS limit of the row kernel (three Q-point buffers), set once per
+// device by the library's initialisation for every Bluestein length.
+template <int Q> inline hipError_t czt_set_attribute() {
+  const int mx = 3 * LdsLen<Q>::value * (int)sizeof(float2);
+  hipError_t e = hipSuccess;
+  for (const void* f : {(const void*)czt_rows_kernel<Q, PREK_F>, (const void*)czt_rows_kernel<Q, PREK_HNA>,
+                        (const void*)czt_rows_kernel<Q, PREK_SIMPLE>}) {
+    const hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+    if (r != hipSuccess) e = r;
+  }
+  return e;
+}
+inline hipError_t czt_set_attributes() {
+  hipError_t e = hipSuccess;
+  for (hipError_t r : {czt_set_attribute<64>(), czt_set_attribute<128>(), czt_set_attribute<256>(),
+                       czt_set_attribute<512>(), czt_set_attribute<1024>(), czt_set_attribute<2048>(),
+                       czt_set_attribute<4096>(), czt_set_attribute<768>(), czt_set_attribute<1536>(),
+                       czt_set_attribute<3072>()})
+    if (r != hipSuccess) e = r;
+  return e;
+}
+
 template <int Q>
 inline void czt_cols_launch(const CztArgs& a, int batch, hipStream_t st) {
   KTimer t(K_CZT_COLS, st);

@@ -420,6 +420,20 @@ __global__ __launch_bounds__(P / 16) void rs_final1_kernel(RsArgs a) {
   }
 }
 
+// Dynamic-LDS limits of the multi-buffer row kernels (up to three buffers of
+// P points), set once per device by the library's initialisation.
+template <int P> inline hipError_t rs_set_attribute() {
+  return hipFuncSetAttribute((const void*)rs_urow_kernel<P>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             3 * LdsLen<P>::value * (int)sizeof(float2));
+}
+inline hipError_t rs_set_attributes() {
+  hipError_t e = hipSuccess;
+  for (hipError_t r : {rs_set_attribute<64>(), rs_set_attribute<128>(), rs_set_attribute<256>(),
+                       rs_set_attribute<512>()})
+    if (r != hipSuccess) e = r;
+  return e;
+}
+
 template <int P>
 inline hipError_t rs_launch(const RsArgs& a, hipStream_t st) {
   constexpr int CPB = Units<P>::CPB;
@@ -431,12 +445,6 @@ inline hipError_t rs_launch(const RsArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(rs_urow1_kernel<P>, dim3(rows), dim3(P / 16), 0, st, a);
   } else {
     KTimer t(K_RS_UROW, st);
-    static bool attr_set = false;   // up to three 4096-point buffers of dynamic LDS
-    if (!attr_set) {
-      (void)hipFuncSetAttribute((const void*)rs_urow_kernel<P>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                3 * LdsLen<P>::value * (int)sizeof(float2));
-      attr_set = true;
-    }
     hipLaunchKernelGGL(rs_urow_kernel<P>, dim3(rows), dim3(a.ncomp * (P / 16)), 
                        (size_t)a.ncomp * LdsLen<P>::value * sizeof(float2), st, a);
   }

@@ -6,6 +6,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <stdint.h>
+
+#include <atomic>
 #include <mutex>
 #include <vector>
 
@@ -43,7 +46,8 @@ inline const char* kernel_name(int id) {
 
 struct TimingState {
   std::mutex mu;
-  bool on = false;
+  std::atomic<bool> on{false};      // read without the lock on every launch
+  uint64_t gen = 0;                 // bumped whenever the pool is rebuilt or emptied
   std::vector<hipEvent_t> pool;     // 2 events per record
   std::vector<int> ids;             // kernel id per used record
   size_t used = 0;                  // records in use
@@ -56,23 +60,30 @@ inline TimingState& timing_state() {
 }
 
 // Records a start event on construction and a stop event on destruction.
+// Both take the state's lock; a record whose pool was rebuilt or collected in
+// between (another host thread calling xl_timing_enable / collect) is dropped.
 class KTimer {
  public:
   KTimer(int id, hipStream_t st) : st_(st) {
     TimingState& s = timing_state();
-    if (!s.on) return;
+    if (!s.on.load(std::memory_order_relaxed)) return;
     std::lock_guard<std::mutex> lk(s.mu);
+    if (!s.on.load(std::memory_order_relaxed)) return;
     if (2 * (s.used + 1) > s.pool.size()) {
       ++s.dropped;
       return;
     }
     slot_ = (long)s.used++;
+    gen_ = s.gen;
     s.ids[slot_] = id;
     (void)hipEventRecord(s.pool[2 * slot_], st_);
   }
   ~KTimer() {
     if (slot_ < 0) return;
-    (void)hipEventRecord(timing_state().pool[2 * slot_ + 1], st_);
+    TimingState& s = timing_state();
+    std::lock_guard<std::mutex> lk(s.mu);
+    if (s.gen != gen_ || (size_t)slot_ >= s.used) return;
+    (void)hipEventRecord(s.pool[2 * slot_ + 1], st_);
   }
   KTimer(const KTimer&) = delete;
   KTimer& operator=(const KTimer&) = delete;
@@ -80,6 +91,7 @@ class KTimer {
  private:
   hipStream_t st_;
   long slot_ = -1;
+  uint64_t gen_ = 0;
 };
 
 }  // namespace xl

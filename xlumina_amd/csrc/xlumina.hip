@@ -56,6 +56,11 @@ int ensure_init() {
   if (rc) return rc;
   rc = hip_check(hipDeviceSynchronize(), "twiddle init");
   if (rc) return rc;
+  // launch attributes are per device: set them here, once, under the lock
+  rc = hip_check(xl::rs_set_attributes(), "RS launch attributes");
+  if (rc) return rc;
+  rc = hip_check(xl::czt_set_attributes(), "CZT launch attributes");
+  if (rc) return rc;
   g_init_done[dev] = true;
   return XL_OK;
 }
@@ -543,10 +548,10 @@ size_t xl_czt_gemm_workspace_bytes(const xl_czt_params* p) {
 }
 
 int xl_czt_gemm(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream) {
-  if (!device_ready()) return fail(XL_EINVAL, "xl_init() has not run on this device");
   GemmPlan g;
   if (int rc = plan_gemm(p, &g)) return rc;
   if (p->batch > 1) return fail(XL_EINVAL, "batched CZT runs in the Bluestein form (xl_czt)");
+  if (p->tables_mode != 0) return fail(XL_EINVAL, "tables_mode is a Bluestein-form option (xl_czt); it must be 0 here");
   if (!p->in || !p->out || !p->x || !p->y || !p->xout || !p->yout) return fail(XL_EINVAL, "null pointer argument");
   if (p->nin < 1 || p->nin > 3 || p->ncomp < 1 || p->ncomp > 3) return fail(XL_EINVAL, "nin / ncomp must be 1..3");
   if (p->mode < XL_CZT_RS || p->mode > XL_CZT_PLAIN) return fail(XL_EINVAL, "unknown CZT mode");
@@ -562,6 +567,7 @@ int xl_czt_gemm(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream)
   if (int rc = check_mask(p->mask, p->nin)) return rc;
   const size_t need = gemm_ws(g);
   if (!ws || ws_bytes < need) return fail(XL_EWORKSPACE, "zoom-GEMM workspace too small: need " + std::to_string(need));
+  if (!device_ready()) return fail(XL_EINVAL, "xl_init() has not run on this device");
   auto al = [](size_t b) { return (b + 255) / 256 * 256; };
   char* w = align256((char*)ws);
   xl::Planes ka = planes_at(w, g.a1_rows, g.a1_ld);
@@ -737,6 +743,10 @@ int xl_pre_backward(const xl_pre_params* p, void* stream) {
   xl::PreArgs a;
   if (int rc = pre_common(p, &a)) return rc;
   if (!p->gout) return fail(XL_EINVAL, "null cotangent");
+  const bool any_gparam = p->gparam[0][0] || p->gparam[0][1] || p->gparam[1][0] || p->gparam[1][1];
+  if (p->batch > 1 && any_gparam && p->gparam_batch_stride < (int64_t)p->rows * p->cols)
+    return fail(XL_EINVAL, "batched mask-parameter gradients need one array per member (gparam_batch_stride >= "
+                           "rows * cols): members writing one array would race");
   xl::KTimer t(xl::K_PRE_BWD, (hipStream_t)stream);
   const int64_t npix = (int64_t)p->rows * p->cols;
   const bool lone_slm = !p->hna && p->ez_mode == XL_EZ_NONE && p->nin == 1 && p->ncomp == 1 &&
@@ -759,6 +769,9 @@ int xl_slm_backward_adam(const xl_pre_params* p, const xl_adam_params* ad, void*
     return fail(XL_EINVAL, "the fused SLM adjoint + Adam step takes one component and one XL_MASK_SLM stage");
   if (ad->param != p->mask[0].m0) return fail(XL_EINVAL, "Adam param must be the SLM phase array (mask[0].m0)");
   if (!ad->m || !ad->v || !ad->count) return fail(XL_EINVAL, "Adam m, v and count are required");
+  if (p->batch > 1 && p->mask_batch_stride < (int64_t)p->rows * p->cols)
+    return fail(XL_EINVAL, "a batched fused Adam step needs one phase array per member (mask_batch_stride >= "
+                           "rows * cols): members sharing phases would race on them");
   if (!(ad->b1 >= 0.f && ad->b1 < 1.f && ad->b2 >= 0.f && ad->b2 < 1.f)) return fail(XL_EINVAL, "Adam betas must be in [0, 1)");
   xl::AdamArgs d;
   d.p = ad->param;
@@ -1090,6 +1103,7 @@ int xl_timing_enable(int32_t enable, int32_t capacity) {
   s.used = 0;
   s.dropped = 0;
   s.on = false;
+  ++s.gen;
   if (!enable) return XL_OK;
   if (capacity < 1 || capacity > (1 << 20)) return fail(XL_EINVAL, "timing capacity must be 1..2^20");
   s.pool.resize(2 * (size_t)capacity);
@@ -1114,6 +1128,7 @@ int xl_timing_collect(xl_kernel_time* out, int32_t max_entries) {
     cnt[s.ids[i]] += 1;
   }
   s.used = 0;
+  ++s.gen;
   int n = 0;
   for (int k = 0; k < xl::K_COUNT; ++k) {
     if (!cnt[k]) continue;

@@ -179,6 +179,39 @@ def quality_factor(x, y, wavelength, z):
     return dr_ideal / dr_real
 
 
+def check_buffer(t, shape, dtype, dev, name: str) -> None:
+    """Raises ValueError unless `t` is a contiguous `dtype` tensor of `shape`
+    on `dev`: the kernels write whole planes through its pointer."""
+    if (not isinstance(t, torch.Tensor) or t.dtype != dtype or t.device != torch.device(dev)
+            or tuple(t.shape) != tuple(shape) or not t.is_contiguous()):
+        got = (f"{tuple(t.shape)} {t.dtype} on {t.device}" if isinstance(t, torch.Tensor) else type(t).__name__)
+        raise ValueError(f"{name} must be a contiguous {dtype} tensor of shape {tuple(shape)} on {dev} (got {got})")
+
+
+def check_workspace(ws, dev) -> None:
+    if ws is not None and (not isinstance(ws, torch.Tensor) or ws.device != torch.device(dev)
+                           or not ws.is_contiguous()):
+        raise ValueError(f"workspace must be a contiguous device buffer on {dev}")
+
+
+def check_rs_args(src_shape, nin: int, ncomp: int, nx: int, ny: int, out, inten, dev) -> None:
+    """Shape checks of rs_propagate before any pointer reaches a kernel: the
+    component count read, the grid lengths and the caller's output buffers."""
+    n = src_shape[-1]
+    if not 1 <= nin <= src_shape[0]:
+        raise ValueError(f"nin = {nin} but the source holds {src_shape[0]} component(s)")
+    if nx != n or ny != n:
+        raise ValueError(f"grid lengths ({nx}, {ny}) do not match the {n} x {n} field")
+    if out is not None:
+        check_buffer(out, (ncomp, n, n), torch.complex64, dev, "out")
+    if inten is not None:
+        check_buffer(inten, (n, n), torch.float32, dev, "inten")
+
+
+def _grid_len(g) -> int:
+    return int(g.numel()) if isinstance(g, torch.Tensor) else len(np.asarray(g).ravel())
+
+
 def _z_ptr(z_dev: Optional[torch.Tensor], dev):
     if z_dev is None:
         return None
@@ -227,6 +260,8 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
         raise ValueError("RS propagation needs a square grid (the reference builds one lag vector for both axes)")
     nin = src.shape[0] if nin is None else nin
     ncomp = 3 if ez_mode == N.XL_EZ_VRS else nin
+    check_rs_args(tuple(src.shape), nin, ncomp, _grid_len(x), _grid_len(y), out, inten if intensity else None, dev)
+    check_workspace(workspace, dev)
     xd, yd = device_grid(x, dev), device_grid(y, dev)
     if out is None:
         out = torch.empty((ncomp, n, n), dtype=torch.complex64, device=dev)
@@ -400,8 +435,8 @@ def czt(src: torch.Tensor, g: CztGeometry, masks: Sequence[MaskSpec] = (), ez_mo
     p, keep = _czt_params(g, dev, ncomp, nin, adjoint, ez_mode, mode)
     rows_in, cols_in = (p.ny_in, p.nx_in) if not adjoint else (p.my, p.mx)
     rows_out, cols_out = (p.my, p.mx) if not adjoint else (p.ny_in, p.nx_in)
-    if tuple(src.shape[-2:]) != (rows_in, cols_in):
-        raise ValueError(f"CZT input is {tuple(src.shape[-2:])}, expected {(rows_in, cols_in)}")
+    check_czt_args(tuple(src.shape), nin, ncomp, (rows_in, cols_in), (rows_out, cols_out), out, dev)
+    check_workspace(workspace, dev)
     if out is None:
         out = torch.empty((ncomp, rows_out, cols_out), dtype=torch.complex64, device=dev)
     p.inp = src.data_ptr()
@@ -427,6 +462,18 @@ def czt(src: torch.Tensor, g: CztGeometry, masks: Sequence[MaskSpec] = (), ez_mo
     return out
 
 
+def check_czt_args(src_shape, nin: int, ncomp: int, plane_in, plane_out, out, dev) -> None:
+    """Shape checks of a single CZT: input plane, components read, output buffer."""
+    if tuple(src_shape[-2:]) != tuple(plane_in):
+        raise ValueError(f"CZT input is {tuple(src_shape[-2:])}, expected {tuple(plane_in)}")
+    if not 1 <= nin <= src_shape[0]:
+        raise ValueError(f"nin = {nin} but the source holds {src_shape[0]} component(s)")
+    if not 1 <= ncomp <= 3:
+        raise ValueError(f"ncomp must be 1..3 (got {ncomp})")
+    if out is not None:
+        check_buffer(out, (ncomp,) + tuple(plane_out), torch.complex64, dev, "out")
+
+
 def _czt_batched(src, g, masks, ez_mode, nin, ncomp, adjoint, mode, out, workspace, batch):
     lib = N.load()
     if not isinstance(src, torch.Tensor) or src.device.type != "cuda" or src.dtype != torch.complex64:
@@ -447,10 +494,12 @@ def _czt_batched(src, g, masks, ez_mode, nin, ncomp, adjoint, mode, out, workspa
     rows_out, cols_out = (p.my, p.mx) if not adjoint else (p.ny_in, p.nx_in)
     if tuple(src.shape[-2:]) != (rows_in, cols_in):
         raise ValueError(f"CZT input is {tuple(src.shape[-2:])}, expected {(rows_in, cols_in)}")
+    if not 1 <= nin <= cdim:
+        raise ValueError(f"nin = {nin} but the source holds {cdim} component(s)")
+    check_workspace(workspace, dev)
     if out is None:
         out = torch.empty((batch, ncomp, rows_out, cols_out), dtype=torch.complex64, device=dev)
-    if tuple(out.shape) != (batch, ncomp, rows_out, cols_out) or not out.is_contiguous():
-        raise ValueError("batched CZT output must be a contiguous (B, ncomp, rows, cols) tensor")
+    check_buffer(out, (batch, ncomp, rows_out, cols_out), torch.complex64, dev, "batched CZT output")
     p.inp = src.data_ptr()
     p.in_comp_stride = rows_in * cols_in
     p.out = out.data_ptr()
@@ -624,9 +673,10 @@ def pre_backward_batched(src: torch.Tensor, gout: torch.Tensor, x, y, batch: int
             if need_params[i][j]:
                 t = gparams[i][j] if gparams is not None and gparams[i][j] is not None else \
                     torch.empty((batch, rows, cols), dtype=torch.float32, device=dev)
-                if (tuple(t.shape) != (batch, rows, cols) or t.dtype != torch.float32 or t.stride(2) != 1
-                        or t.stride(1) != cols):
-                    raise ValueError("parameter-gradient outputs must be float32 (B, rows, cols) with contiguous rows")
+                if (tuple(t.shape) != (batch, rows, cols) or t.dtype != torch.float32 or t.device != dev
+                        or t.stride(2) != 1 or t.stride(1) != cols or (batch > 1 and t.stride(0) < rows * cols)):
+                    raise ValueError("parameter-gradient outputs must be float32 (B, rows, cols) on the field's "
+                                     "device with contiguous rows and one array per member")
                 strides.add(t.stride(0))
                 gp[i][j] = t
                 p.gparam[i][j] = t.data_ptr()
@@ -912,6 +962,9 @@ def slm_backward_adam(src: torch.Tensor, gout: torch.Tensor, x, y, batch: int, p
         if (tuple(t.shape) != (batch, rows, cols) or t.dtype != torch.float32 or t.device != dev
                 or t.stride(2) != 1 or t.stride(1) != cols or t.stride(0) != phase.stride(0)):
             raise ValueError("phase, m and v must be float32 (B, rows, cols) views with one batch stride")
+    if batch > 1 and phase.stride(0) < rows * cols:
+        raise ValueError("the fused Adam step needs one phase array per batch member (members sharing phases "
+                         "would race on them)")
     if count.dtype != torch.int32 or count.numel() != 1 or count.device != dev:
         raise ValueError("count must be a one-element int32 tensor on the field's device")
     p = N.PreParams()
