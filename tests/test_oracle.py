@@ -451,3 +451,19 @@ def test_lens_focuses_a_plane_wave():
     assert c == I.max() and c > 50.0
     diverging = R.rs_propagation(np.conj(m), x, y, 0.65, f)[0]
     assert np.abs(diverging).max() ** 2 < 2.0
+
+
+def test_plane_wave_phase_gradient():
+    """The oracle's tilted plane wave has the closed-form phase step
+    k sin(theta) (cos(phi) dx, sin(phi) dy) between neighbouring pixels and the
+    constant phase k z0 cos(theta) at the origin (wave_optics.py:525-542)."""
+    x, y = R.space(300.0, 33)
+    wl, th, ph, z0 = 0.633, 0.03, 0.8, 5.0
+    k = 2 * np.pi / wl
+    pw = R.plane_wave(x, y, wl, 1.5, th, ph, z0)
+    dx, dy = x[1] - x[0], y[1] - y[0]
+    assert np.allclose(pw[:, 1:] / pw[:, :-1], np.exp(1j * k * np.sin(th) * np.cos(ph) * dx))
+    assert np.allclose(pw[1:, :] / pw[:-1, :], np.exp(1j * k * np.sin(th) * np.sin(ph) * dy))
+    assert np.allclose(pw[16, 16], 1.5 * np.exp(1j * k * z0 * np.cos(th)))     # x = y = 0 at the centre
+    ex, ey, ez = R.polarized_plane_wave(x, y, wl, (3, 4), th, ph, z0)
+    assert np.allclose(ex, 0.6 * pw / 1.5) and np.allclose(ey, 0.8 * pw / 1.5) and not ez.any()
