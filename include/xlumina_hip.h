@@ -167,6 +167,20 @@ size_t xl_rs_workspace_bytes(int32_t n, int32_t ncomp);
 size_t xl_rs_hhat_bytes(int32_t n);
 int xl_rs_propagate(const xl_rs_params* p, void* workspace, size_t workspace_bytes, void* stream);
 
+/* No counterpart in the reference.  xl_rs_propagate without a workspace
+ * argument, for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * kernels use a library-owned scratch array of XL_RS_NOALLOC_SCRATCH_BYTES
+ * bytes, one per device, held from the device's xl_init until the process
+ * ends.  Grids above XL_RS_NOALLOC_MAX_N per side return XL_EINVAL.  The call
+ * allocates nothing.  Every call on one device shares that device's array:
+ * calls that overlap on one device (other streams, other host threads) race
+ * and must be serialised by the caller.  The hhat buffer of hhat_mode 1 / 2
+ * is still the caller's. */
+#define XL_RS_NOALLOC_MAX_N 1024
+#define XL_RS_NOALLOC_SCRATCH_BYTES 67125504 /* xl_rs_workspace_bytes(1024, 3) */
+int xl_rs_propagate_noalloc(const xl_rs_params* p, void* stream);
+
 /* CZT_jit (wave_optics.py:333-357), VCZT_jit (vectorized_optics.py:375-384),
  * CZT_for_high_NA_jit (wave_optics.py:359-371), VCZT_objective_lens
  * (optical_elements.py:600-638); Bluestein FP32 form.  Returns XL_EINVAL for

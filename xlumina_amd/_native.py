@@ -25,6 +25,8 @@ XL_MASK_NONE, XL_MASK_SLM, XL_MASK_SSLM, XL_MASK_LCD, XL_MASK_LCD_SCALAR, XL_MAS
 XL_EZ_NONE, XL_EZ_VRS, XL_EZ_VCZT, XL_EZ_HNA = range(4)
 XL_CZT_RS, XL_CZT_HNA, XL_CZT_PLAIN = range(3)
 XL_LOSS_INTENSITY, XL_LOSS_AMPLITUDE, XL_LOSS_PHASE = range(3)
+XL_RS_NOALLOC_MAX_N = 1024
+XL_RS_NOALLOC_SCRATCH_BYTES = 67125504
 
 
 class MaskStage(C.Structure):
@@ -130,6 +132,7 @@ SIGNATURES = {
     "xl_rs_workspace_bytes": (C.c_size_t, [C.c_int32, C.c_int32]),
     "xl_rs_hhat_bytes": (C.c_size_t, [C.c_int32]),
     "xl_rs_propagate": (C.c_int, [C.POINTER(RsParams), C.c_void_p, C.c_size_t, C.c_void_p]),
+    "xl_rs_propagate_noalloc": (C.c_int, [C.POINTER(RsParams), C.c_void_p]),
     "xl_czt_workspace_bytes": (C.c_size_t, [C.POINTER(CztParams)]),
     "xl_czt": (C.c_int, [C.POINTER(CztParams), C.c_void_p, C.c_size_t, C.c_void_p]),
     "xl_czt_gemm_workspace_bytes": (C.c_size_t, [C.POINTER(CztParams)]),

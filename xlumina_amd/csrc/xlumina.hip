@@ -42,6 +42,13 @@ constexpr int kMaxDevices = 64;
 std::mutex g_init_mu;
 bool g_init_done[kMaxDevices] = {};
 
+// Scratch of xl_rs_propagate_noalloc: one copy per device in the code
+// object's zero-initialised data.  In host code `scratch` names a host-side
+// stand-in, so the device address of each device's copy is looked up once at
+// initialisation (under g_init_mu) and kept in g_scratch.
+static __device__ __align__(256) char scratch[XL_RS_NOALLOC_SCRATCH_BYTES];
+void* g_scratch[kMaxDevices] = {};
+
 int ensure_init() {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
@@ -60,6 +67,8 @@ int ensure_init() {
   rc = hip_check(xl::rs_set_attributes(), "RS launch attributes");
   if (rc) return rc;
   rc = hip_check(xl::czt_set_attributes(), "CZT launch attributes");
+  if (rc) return rc;
+  rc = hip_check(hipGetSymbolAddress(&g_scratch[dev], HIP_SYMBOL(scratch)), "scratch address");
   if (rc) return rc;
   g_init_done[dev] = true;
   return XL_OK;
@@ -309,6 +318,28 @@ int xl_rs_propagate(const xl_rs_params* p, void* ws, size_t ws_bytes, void* stre
   int rc = dispatch_q(P, [&](auto c) { (void)xl::rs_launch<decltype(c)::value>(a, st); });
   if (rc) return rc;
   return hip_check(hipGetLastError(), "RS launch");
+}
+
+int xl_rs_propagate_noalloc(const xl_rs_params* p, void* stream) {
+  static_assert(XL_RS_NOALLOC_MAX_N == 1024, "the scratch size below is the n = 1024 workspace");
+  static_assert(XL_RS_NOALLOC_SCRATCH_BYTES >= (3 * 1024 * 2048 + 1024 * 2048 + 2048) * 8 + 256,
+                "scratch smaller than the n = 1024, 3-component workspace");
+  if (!p) return fail(XL_EINVAL, "null params");
+  if (p->n > XL_RS_NOALLOC_MAX_N)
+    return fail(XL_EINVAL, "xl_rs_propagate_noalloc takes grids up to " + std::to_string(XL_RS_NOALLOC_MAX_N) +
+                               " per side (its scratch holds " + std::to_string(XL_RS_NOALLOC_SCRATCH_BYTES) +
+                               " bytes); use xl_rs_propagate with a workspace for n = " + std::to_string(p->n));
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return fail(XL_EHIP, "hipGetDevice failed");
+  void* ws = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_init_mu);
+    if (!g_init_done[dev]) return fail(XL_EINVAL, "xl_init() has not run on this device");
+    ws = g_scratch[dev];
+  }
+  // every kernel writes the scratch before reading it (zero-padded rows and
+  // columns included), so nothing in it has to start at zero
+  return xl_rs_propagate(p, ws, XL_RS_NOALLOC_SCRATCH_BYTES, stream);
 }
 
 size_t xl_czt_workspace_bytes(const xl_czt_params* p) {

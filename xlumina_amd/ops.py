@@ -238,7 +238,7 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
                  adjoint: bool = False, out: Optional[torch.Tensor] = None,
                  workspace: Optional[torch.Tensor] = None, z_dev: Optional[torch.Tensor] = None,
                  inten: Optional[torch.Tensor] = None, kernel_dz: int = 0,
-                 hhat: Optional[torch.Tensor] = None, hhat_mode: int = 0):
+                 hhat: Optional[torch.Tensor] = None, hhat_mode: int = 0, noalloc: bool = False):
     """RS_propagation_jit / VRS_propagation_jit (wave_optics.py:281-289,
     vectorized_optics.py:364-373) with fused masks and Ez construction.
 
@@ -251,6 +251,10 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
     hhat, hhat_mode: 1 builds the kernel spectrum into `hhat` (a device buffer
     of rs_hhat_bytes(n)) and keeps it; 2 reuses what a mode-1 call of the same
     geometry and distance left there (an adjoint call conjugates it).
+    noalloc: run on the library's own per-device scratch
+    (xl_rs_propagate_noalloc, n <= 1024) instead of a workspace; with `out`
+    (and `inten`) given the call allocates nothing.  Such calls share one
+    scratch per device, so they must not overlap on a device.
     """
     lib = N.load()
     src = _prep_field(src)
@@ -269,9 +273,15 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
         inten = torch.empty((n, n), dtype=torch.float32, device=dev)
     if not intensity:
         inten = None
-    need = lib.xl_rs_workspace_bytes(n, ncomp)
-    if workspace is None or workspace.numel() < need:
-        workspace = torch.empty(need, dtype=torch.uint8, device=dev)
+    if noalloc:
+        if workspace is not None:
+            raise ValueError("noalloc runs on the library's scratch; pass no workspace")
+        if n > N.XL_RS_NOALLOC_MAX_N:
+            raise ValueError(f"noalloc propagation takes grids up to {N.XL_RS_NOALLOC_MAX_N} per side, got {n}")
+    else:
+        need = lib.xl_rs_workspace_bytes(n, ncomp)
+        if workspace is None or workspace.numel() < need:
+            workspace = torch.empty(need, dtype=torch.uint8, device=dev)
     p = N.RsParams()
     p.inp = src.data_ptr()
     p.in_comp_stride = n * n
@@ -297,7 +307,10 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
             raise ValueError("hhat must be a device buffer of rs_hhat_bytes(n) bytes on the field's device")
         p.hhat_mode, p.hhat = int(hhat_mode), hhat.data_ptr()
     keep = fill_masks(p.mask, masks, (n, n), dev, nin)
-    N.check(lib.xl_rs_propagate(C.byref(p), workspace.data_ptr(), workspace.numel(), _stream(dev)))
+    if noalloc:
+        N.check(lib.xl_rs_propagate_noalloc(C.byref(p), _stream(dev)))
+    else:
+        N.check(lib.xl_rs_propagate(C.byref(p), workspace.data_ptr(), workspace.numel(), _stream(dev)))
     del keep
     return (out, inten) if intensity else out
 
