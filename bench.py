@@ -131,16 +131,18 @@ def main():
 
     _progress(f"headline: {value:.1f} chains/s ({ms:.4f} ms/step)")
     secondary = {}
+    checks = {}
     if not args.no_secondary:
-        secondary = run_secondary(dev, workloads, args)
+        secondary = run_secondary(dev, workloads, args, checks)
 
     cpu = None
     accuracy = None
     if not args.no_cpu_baseline:
-        _progress("CPU baseline (float64 oracle, one chain)")
+        _progress("CPU baseline (float64 oracle chains, cold + warm, complex128 and complex64)")
         cpu, ref = cpu_baseline(w)
         accuracy = chain_accuracy(w, geom, ref)
         _progress(f"accuracy vs float64: {accuracy}")
+        secondary_accuracy(secondary, checks)
 
     line = {
         "metric": METRIC,
@@ -229,7 +231,9 @@ def _time(fn, steps, warmup):
     return s.elapsed_time(e) / steps
 
 
-def run_secondary(dev, workloads, args):
+def run_secondary(dev, workloads, args, checks):
+    """Timings of the other workloads; the GPU outputs the float64 checks
+    compare are kept in `checks` (host arrays, taken outside the timed regions)."""
     import numpy as np
     import torch
     from xlumina_amd import _native as N
@@ -269,6 +273,10 @@ def run_secondary(dev, workloads, args):
     rs = workloads.scalar_rs_1024(dev)
     ms = _time(lambda: rs.run(), max(10, args.steps), 3)
     out["rs_1024_slm"] = {"value": round(1000.0 / ms, 2), "unit": "propagations/s", "ms": round(ms, 4)}
+    got, _ = rs.run()
+    checks["rs_1024_slm"] = {"field": got[0].cpu().numpy(), "src": rs.field[0].cpu().numpy(),
+                             "phase": rs.phase.cpu().numpy(), "x": rs.x, "y": rs.y, "wavelength": rs.wavelength,
+                             "z": rs.z}
     # the same propagation replayed from a captured HIP graph over static buffers
     # (what the serving slots do): no per-call host work
     o_buf = torch.empty((1, 1024, 1024), dtype=torch.complex64, device=dev)
@@ -288,6 +296,9 @@ def run_secondary(dev, workloads, args):
     del graph, o_buf, ws
     h = workloads.high_na_2048(dev)
     geom = h.geometry()
+    checks["high_na"] = {"h": h, "fields": {
+        form: ops.czt(h.field, geom, masks=h.masks(), nin=2, form=form).cpu().numpy()
+        for form in ("bluestein", "gemm")}}
     alpha = h.alpha.clone().requires_grad_(True)
     phi = h.phi.clone().requires_grad_(True)
     for form in ("bluestein", "gemm"):
@@ -331,6 +342,8 @@ def run_secondary(dev, workloads, args):
     if args.fourf_launches > 0:
         from xlumina_amd import fourf
         opt = fourf.FourFOptimizer(dev, batch=64, n=1024, steps_per_launch=100)
+        checks["fourf"] = {"opt": opt, "intensity0": opt.forward_intensity()[0].cpu().numpy(),
+                           "phases0": opt.phi[0].cpu().numpy(), "einit": opt.e_in.cpu().numpy()}
         first = opt.launch().clone()          # captures the 100-step graph, then runs it
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -343,6 +356,7 @@ def run_secondary(dev, workloads, args):
             "value": round(opt.candidate_steps_per_launch() / dt, 1), "unit": "candidate-steps/s",
             "ms_per_launch": round(dt * 1e3, 2), "steps_per_launch": 100, "candidates": 64,
             "mean_loss_step0": float(first[0].mean()), "mean_loss_last": float(last[-1].mean())}
+        checks["fourf"]["opt"] = None
         del opt
         torch.cuda.empty_cache()
         _progress(f"4f batched optimisation: {dt * 1e3:.1f} ms per 100-step launch")
@@ -355,12 +369,42 @@ def run_secondary(dev, workloads, args):
                              "fractions of the measured capacity")
         sweep["hbm_pool_bytes"] = srv.used_bytes
         out["serving_mix"] = sweep
+        # one scalar and one vectorial 512^2 request, kept for the float64 check
+        probe = [S.Request(0, 512, False, 37000.0, False, 123456789), S.Request(1, 512, True, 52000.0, False, 987654321)]
+        srv.run(probe, keep_results=True)
+        checks["serving"] = {"requests": probe, "intensities": [srv.results[r.rid][0].cpu().numpy() for r in probe],
+                             "phases": [S.request_phases(r, dev).cpu().numpy() for r in probe],
+                             "shared": srv.shared(512), "wavelength": srv.wavelength}
         _progress(f"serving: capacity {sweep['capacity']['throughput_req_s']} req/s")
     return out
 
 
-def cpu_baseline(w):
-    """One VRS + VCZT chain of the same workload through the float64 oracle."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _stats(samples):
+    import numpy as np
+    a = np.asarray(samples, dtype=np.float64)
+    return {"n": int(a.size), "mean_s": round(float(a.mean()), 4), "p50_s": round(float(np.percentile(a, 50)), 4),
+            "p99_s": round(float(np.percentile(a, 99)), 4)}
+
+
+def cpu_baseline(w, budget_s=24.0, max_warm=4):
+    """VRS + VCZT chains of the same workload through the oracle (the
+    reference algorithm restated in NumPy/SciPy: 2N-1 padding, per-call
+    kernel FFT, two-pass Bluestein) on the host cores, following the method of
+    examples/scalar_xlumina.py:31-55 within a time budget: one cold chain,
+    then warm chains alternating complex128 (the reference's precision) and
+    complex64 until the budget or max_warm of each is reached; mean / p50 /
+    p99 per precision.  `value` is 1 / mean of the warm complex128 chains."""
     import numpy as np
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import xlumina_ref as R
@@ -370,14 +414,103 @@ def cpu_baseline(w):
     Ey = w.field[1].cpu().numpy().astype(np.complex128)
     eta = w.eta.cpu().numpy().astype(np.float64)
     th = w.theta.cpu().numpy().astype(np.float64)
+
+    def chain(dtype):
+        mx, my, _ = R.LCD(Ex, Ey, None, eta, th)
+        (vx, vy, _), _ = R.vrs_propagation(mx, my, w.x, w.y, w.wavelength, w.z_vrs, dtype=dtype)
+        return R.vczt(vx, vy, w.x, w.y, w.wavelength, w.z_czt, w.xout, w.yout, dtype=dtype)
+
     t0 = time.perf_counter()
-    mx, my, _ = R.LCD(Ex, Ey, None, eta, th)
-    (vx, vy, _), _ = R.vrs_propagation(mx, my, w.x, w.y, w.wavelength, w.z_vrs)
-    ref = R.vczt(vx, vy, w.x, w.y, w.wavelength, w.z_czt, w.xout, w.yout)
-    dt = time.perf_counter() - t0
-    return ({"value": round(1.0 / dt, 5), "unit": "VRS+VCZT chains/s", "cores": cores, "kind": "port",
-             "sample": f"1 chain at 2048^2 -> 512^2, complex128, scipy.fft workers={cores}, {dt:.1f} s"},
+    ref = chain(np.complex128)
+    cold = time.perf_counter() - t0
+    warm = {"complex128": [], "complex64": []}
+    start = time.perf_counter()
+    while min(len(v) for v in warm.values()) < max_warm and time.perf_counter() - start < budget_s:
+        for name, dt in (("complex128", np.complex128), ("complex64", np.complex64)):
+            t1 = time.perf_counter()
+            chain(dt)
+            warm[name].append(time.perf_counter() - t1)
+    mean128 = float(np.mean(warm["complex128"]))
+    return ({"value": round(1.0 / mean128, 5), "unit": "VRS+VCZT chains/s", "cores": cores, "kind": "port",
+             "cpu_model": _cpu_model(), "cold_s": round(cold, 3),
+             "warm_complex128": _stats(warm["complex128"]), "warm_complex64": _stats(warm["complex64"]),
+             "complex64_value": round(1.0 / float(np.mean(warm["complex64"])), 5),
+             "sample": (f"2048^2 -> 512^2 chains: 1 cold complex128, then {len(warm['complex128'])} warm complex128 + "
+                        f"{len(warm['complex64'])} warm complex64 (about {budget_s:.0f} s budget), "
+                        f"scipy.fft workers={cores}")},
             np.stack(ref))
+
+
+def _rel(a, b):
+    import numpy as np
+    a = np.asarray(a, dtype=np.complex128)
+    b = np.asarray(b, dtype=np.complex128)
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def secondary_accuracy(secondary, checks):
+    """Relative L2 of every secondary workload's output against the float64
+    oracle on the same inputs (the checker side of the CPU leg): the 1024^2 RS
+    field, the high-NA focal field in both CZT forms, the 4f detector
+    intensity of candidate 0 before its first update, and the intensities of
+    one scalar and one vectorial 512^2 served request."""
+    import numpy as np
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import xlumina_ref as R
+    c = checks.get("rs_1024_slm")
+    if c and "rs_1024_slm" in secondary:
+        u, _ = R.slm(c["src"].astype(np.complex128), c["phase"].astype(np.float64))
+        ref, _ = R.rs_propagation(u, c["x"], c["y"], c["wavelength"], c["z"])
+        e = _rel(c["field"], ref)
+        secondary["rs_1024_slm"]["rel_l2_vs_float64"] = e
+        if "rs_1024_slm_graph" in secondary:
+            secondary["rs_1024_slm_graph"]["rel_l2_vs_float64"] = e
+    c = checks.get("high_na")
+    if c:
+        h = c["h"]
+        Ex, Ey = (h.field[i].cpu().numpy().astype(np.complex128) for i in range(2))
+        a64 = h.alpha.cpu().numpy().astype(np.float64)
+        p64 = h.phi.cpu().numpy().astype(np.float64)
+        ex, ey, _ = R.sSLM(Ex, Ey, None, a64, p64)
+        ex, ey, _ = R.LCD(ex, ey, None, h.eta, h.theta)
+        ref = R.vczt_objective_lens(ex, ey, h.x, h.y, h.wavelength, h.radius, h.f, h.xout, h.yout)
+        for form, got in c["fields"].items():
+            e = _rel(got, ref)
+            for key in (f"high_na_fwd_bwd_2048_to_256_{form}", f"high_na_fwd_bwd_2048_to_256_{form}_graph"):
+                if key in secondary:
+                    secondary[key]["rel_l2_vs_float64"] = e
+                    secondary[key]["rel_l2_of"] = "forward focal field (3 x 256^2)"
+    c = checks.get("fourf")
+    if c and "fourf_batched_opt_64x1024" in secondary:
+        from xlumina_amd import fourf
+        x, y = fourf.space(1200.0, 1024)
+        ph = c["phases0"].astype(np.float64)
+        e = c["einit"].astype(np.complex128)
+        a1 = R.czt(e, x, y, 0.65, 25e3)
+        a2 = R.czt(R.slm(a1, ph[0])[0], x, y, 0.65, 50e3)
+        a3 = R.czt(R.slm(a2, ph[1])[0], x, y, 0.65, 25e3)
+        secondary["fourf_batched_opt_64x1024"]["rel_l2_vs_float64"] = _rel(c["intensity0"], np.abs(a3) ** 2)
+        secondary["fourf_batched_opt_64x1024"]["rel_l2_of"] = "detector intensity of candidate 0 before step 1"
+    c = checks.get("serving")
+    if c and "serving_mix" in secondary:
+        sh, worst = c["shared"], 0.0
+        for r, got, ph in zip(c["requests"], c["intensities"], c["phases"]):
+            ph = ph.astype(np.float64)
+            if r.vectorial:
+                bx, by = (sh.beam_v[i].cpu().numpy().astype(np.complex128) for i in range(2))
+                ex, ey, _ = R.sSLM(bx, by, None, ph[0], ph[1])
+                (vx, vy, vz), _ = R.vrs_propagation(ex, ey, sh.x, sh.y, c["wavelength"], r.z)
+                ref = np.abs(vx) ** 2 + np.abs(vy) ** 2 + np.abs(vz) ** 2
+            else:
+                u, _ = R.slm(sh.beam_s[0].cpu().numpy().astype(np.complex128), ph[0])
+                E, _ = R.rs_propagation(u, sh.x, sh.y, c["wavelength"], r.z)
+                ref = np.abs(E) ** 2
+            worst = max(worst, _rel(got, ref))
+        secondary["serving_mix"]["rel_l2_vs_float64"] = worst
+        secondary["serving_mix"]["rel_l2_of"] = "|E|^2 of one scalar and one vectorial 512^2 request (worst)"
+    _progress("secondary accuracy vs float64: " + ", ".join(
+        f"{k} {v['rel_l2_vs_float64']:.2e}" for k, v in secondary.items()
+        if isinstance(v, dict) and "rel_l2_vs_float64" in v))
 
 
 def chain_accuracy(w, geom, ref):

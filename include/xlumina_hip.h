@@ -86,6 +86,19 @@ typedef struct xl_rs_params {
                               call conjugates it on read) -- a gradient step's adjoint propagation
                               reuses its forward's spectrum.  Modes 1 and 2 need kernel_dz = 0. */
   void* hhat;              /* modes 1, 2: xl_rs_hhat_bytes(n) bytes of device memory */
+  /* Batch of independent problems of this geometry run by the same launches
+   * (batch <= 1: one problem).  Member b reads in + b*in_batch_stride and the
+   * mask parameters + b*mask_batch_stride (0: shared by the batch), writes
+   * out + b*out_batch_stride and intensity + b*intensity_batch_stride, and,
+   * when z_dev is set, propagates over z_dev[b].  The workspace holds one
+   * block per member (xl_rs_batch_workspace_bytes); with hhat_mode 1 / 2,
+   * hhat holds one spectrum per member (batch * xl_rs_hhat_bytes(n) bytes). */
+  int32_t batch;
+  int32_t reserved2;
+  int64_t in_batch_stride;        /* complex elements (0: one input shared by the batch) */
+  int64_t out_batch_stride;       /* complex elements, >= ncomp n^2 */
+  int64_t intensity_batch_stride; /* float elements, >= n^2 when intensity is set */
+  int64_t mask_batch_stride;      /* float elements (0: shared mask parameters) */
 } xl_rs_params;
 
 typedef struct xl_czt_params {
@@ -144,7 +157,8 @@ typedef struct xl_pre_params {
   double z;
   double hna_radius, hna_f;
   xl_mask_stage mask[2];
-  const double* z_dev;     /* optional device copy of z (see xl_rs_params) */
+  const double* z_dev;     /* optional device copy of z (see xl_rs_params); with batch > 1 one
+                              distance per member, z_dev[b] */
   int32_t batch;           /* independent problems per launch (<= 1: one) */
   int32_t reserved;
   int64_t in_batch_stride;     /* complex elements (0: one input shared by the batch) */
@@ -164,6 +178,7 @@ int xl_init(void);
 /* RS_propagation_jit (wave_optics.py:281-289), VRS_propagation_jit
  * (vectorized_optics.py:364-373) with Jones masks / Ez construction fused. */
 size_t xl_rs_workspace_bytes(int32_t n, int32_t ncomp);
+size_t xl_rs_batch_workspace_bytes(int32_t n, int32_t ncomp, int32_t batch);
 size_t xl_rs_hhat_bytes(int32_t n);
 int xl_rs_propagate(const xl_rs_params* p, void* workspace, size_t workspace_bytes, void* stream);
 

@@ -204,22 +204,24 @@ def check_czt_lengths(m, M_out):
             "slice is one sample short and raises)")
 
 
-def compute_fft(x, D1, D2, Dm, m, n, mp, M_out, np2):
-    """wave_optics.py:385-410: chirp premultiply, FFT conv with 1/h, length np2."""
+def compute_fft(x, D1, D2, Dm, m, n, mp, M_out, np2, dtype=np.complex128):
+    """wave_optics.py:385-410: chirp premultiply, FFT conv with 1/h, length np2.
+
+    The chirp tables are formed in float64 and cast to ``dtype`` for the FFTs."""
     A = np.exp(1j * 2 * np.pi * D1 / Dm)
     W = np.exp(-1j * 2 * np.pi * (D2 - D1) / (M_out * Dm))
     h = np.arange(-m + 1, max(M_out - 1, m - 1) + 1)
     h = W ** (h ** 2 / 2)
     h_sliced = h[:mp + 1]
-    ft = _fft(1 / h_sliced, np2)
-    b = A ** (-(np.arange(m))) * h[np.arange(m - 1, 2 * m - 1)]
+    ft = _fft((1 / h_sliced).astype(dtype), np2)
+    b = (A ** (-(np.arange(m))) * h[np.arange(m - 1, 2 * m - 1)]).astype(dtype)
     tmp = np.tile(b, (n, 1)).T
-    b = _fft(x * tmp, np2, axis=0)
+    b = _fft(x.astype(dtype, copy=False) * tmp, np2, axis=0)
     b = _ifft(b * np.tile(ft, (n, 1)).T, axis=0)
     return b, h
 
 
-def bluestein_method(x, f1, f2, Dm, M_out):
+def bluestein_method(x, f1, f2, Dm, M_out, dtype=np.complex128):
     """wave_optics.py:412-460: 1-D CZT along axis 0; returns the TRANSPOSE (n, M_out)."""
     m, n = x.shape
     check_czt_lengths(m, M_out)
@@ -227,23 +229,24 @@ def bluestein_method(x, f1, f2, Dm, M_out):
     D2 = f2 + (M_out * Dm + f2 - f1) / (2 * M_out)
     mp = m + M_out - 1
     np2 = compute_np2(mp)
-    b, h = compute_fft(x, D1, D2, Dm, m, n, mp, M_out, np2)
-    b = b[m:mp + 1, 0:n].T * np.tile(h[m - 1:mp], (n, 1))
+    b, h = compute_fft(x, D1, D2, Dm, m, n, mp, M_out, np2, dtype)
+    b = b[m:mp + 1, 0:n].T * np.tile(h[m - 1:mp].astype(dtype), (n, 1))
     l = np.linspace(0, M_out - 1, M_out)
     l = l / M_out * (D2 - D1) + D1
     M_shift = -m / 2
-    M_shift = np.tile(np.exp(-1j * 2 * np.pi * l * (M_shift + 1 / 2) / Dm), (n, 1))
+    M_shift = np.tile(np.exp(-1j * 2 * np.pi * l * (M_shift + 1 / 2) / Dm).astype(dtype), (n, 1))
     return b * M_shift
 
 
-def CZT_jit(field, z, wavelength, k, nx, ny, dx, dy, Xout, Yout, X, Y, Dm, fy_1, fy_2, fx_1, fx_2):
+def CZT_jit(field, z, wavelength, k, nx, ny, dx, dy, Xout, Yout, X, Y, Dm, fy_1, fy_2, fx_1, fx_2,
+            dtype=np.complex128):
     """wave_optics.py:333-357: F0 * Bx(By(E*F)) * z*dx*dy*lambda."""
-    F0 = transfer_function_RS(z, Xout, Yout, k)
-    F = transfer_function_RS(z, X, Y, k)
+    F0 = transfer_function_RS(z, Xout, Yout, k).astype(dtype)
+    F = transfer_function_RS(z, X, Y, k).astype(dtype)
     field = field * F
-    U = bluestein_method(field, fy_1, fy_2, Dm, ny)
-    U = bluestein_method(U, fx_1, fx_2, Dm, nx)
-    return F0 * U * z * dx * dy * wavelength
+    U = bluestein_method(field, fy_1, fy_2, Dm, ny, dtype)
+    U = bluestein_method(U, fx_1, fx_2, Dm, nx, dtype)
+    return F0 * U * (z * dx * dy * wavelength)
 
 
 def CZT_for_high_NA_jit(field, nx, ny, Dm, fy_1, fy_2, fx_1, fx_2):
@@ -285,7 +288,7 @@ def vrs_propagation(Ex, Ey, x, y, wavelength, z, dtype=np.complex128):
     return tuple(out), quality_factor(x, y, wavelength, z)
 
 
-def vczt(Ex, Ey, x, y, wavelength, z, xout=None, yout=None):
+def vczt(Ex, Ey, x, y, wavelength, z, xout=None, yout=None, dtype=np.complex128):
     """VectorizedLight.VCZT, vectorized_optics.py:321-361.
 
     Ez_in = (Ex X/r + Ey Y/r) z/r (differs from VRS by z/r); incoming Ez discarded.
@@ -299,8 +302,8 @@ def vczt(Ex, Ey, x, y, wavelength, z, xout=None, yout=None):
     Ez = (Ex * X / r + Ey * Y / r) * z / r
     k = 2 * np.pi / wavelength
     nx, ny, dx, dy, Xout, Yout, Dm, fy_1, fy_2, fx_1, fx_2 = build_CZT_grid(z, wavelength, x, y, xout, yout)
-    return tuple(CZT_jit(np.asarray(c, dtype=np.complex128), z, wavelength, k, nx, ny, dx, dy, Xout, Yout,
-                         X, Y, Dm, fy_1, fy_2, fx_1, fx_2) for c in (Ex, Ey, Ez))
+    return tuple(CZT_jit(np.asarray(c, dtype=dtype), z, wavelength, k, nx, ny, dx, dy, Xout, Yout,
+                         X, Y, Dm, fy_1, fy_2, fx_1, fx_2, dtype) for c in (Ex, Ey, Ez))
 
 
 # ----------------------------------------------------------------------------

@@ -437,3 +437,46 @@ def test_rs_grad_wrt_wavelength_vs_fd(cuda_dev, vectorial):
     h = 1e-6
     fd = (ref(wl0 + h) - ref(wl0 - h)) / (2 * h)
     assert abs(float(g) - fd) <= 2e-3 * abs(fd), (float(g), fd)
+
+
+@pytest.mark.parametrize("form,vectorial", [("bluestein", False), ("gemm", False), ("bluestein", True)])
+def test_czt_grad_wrt_wavelength_vs_fd(cuda_dev, form, vectorial):
+    """dL/dlambda of CZT / VCZT after a per-pixel mask: lambda enters the
+    transfer functions on both grids (k = 2 pi / lambda), the constant
+    z dx dy lambda and both axes' kernels through Dm = lambda z / dx (not the
+    VCZT Ez_in); against central differences of the float64 oracle in the
+    wavelength (step 3e-8 um: the phase rate in lambda is ~k z / lambda, so a
+    wider step is dominated by truncation)."""
+    n, m = 64, 40
+    x, y = R.space(150.0, n)
+    xo, yo = R.space(40.0, m)
+    rng = np.random.default_rng(96)
+    ex, ey = rng.standard_normal((2, n, n)) + 1j * rng.standard_normal((2, n, n))
+    a, b = rng.uniform(-np.pi, np.pi, (2, n, n))
+    w = rng.random((3 if vectorial else 1, m, m))
+    wl0, z = 0.6, 2500.0
+    wl = torch.tensor([wl0], dtype=torch.float64, device=cuda_dev, requires_grad=True)
+    geom = ops.czt_geometry(x, y, xo, yo, z, wl0)
+    if vectorial:
+        src = torch.tensor(np.stack([ex, ey]).astype(np.complex64), device=cuda_dev)
+        masks = (MaskSpec(N.XL_MASK_LCD, torch.tensor(a, dtype=torch.float32, device=cuda_dev),
+                          torch.tensor(b, dtype=torch.float32, device=cuda_dev)),)
+        out = AG.czt(src, geom, masks=masks, ez_mode=N.XL_EZ_VCZT, nin=2, form=form, wavelength=wl)
+    else:
+        src = torch.tensor(ex[None].astype(np.complex64), device=cuda_dev)
+        masks = (MaskSpec(N.XL_MASK_SLM, torch.tensor(a, dtype=torch.float32, device=cuda_dev)),)
+        out = AG.czt(src, geom, masks=masks, form=form, wavelength=wl)
+    L = (torch.tensor(w, dtype=torch.float32, device=cuda_dev) * out.abs() ** 2).sum()
+    (g,) = torch.autograd.grad(L, (wl,))
+
+    def ref(lam):
+        if vectorial:
+            lx, ly, _ = R.LCD(ex, ey, None, a, b)
+            o = np.stack(R.vczt(lx, ly, x, y, lam, z, xo, yo))
+        else:
+            o = R.czt(ex * np.exp(1j * a), x, y, lam, z, xo, yo)[None]
+        return float(np.sum(w * np.abs(o) ** 2))
+
+    h = 3e-8
+    fd = (ref(wl0 + h) - ref(wl0 - h)) / (2 * h)
+    assert abs(float(g) - fd) <= 2e-3 * abs(fd), (float(g), fd)

@@ -39,7 +39,10 @@ class RsParams(C.Structure):
                 ("out", C.c_void_p), ("out_comp_stride", C.c_int64), ("intensity", C.c_void_p),
                 ("n", C.c_int32), ("adjoint", C.c_int32), ("x", C.c_void_p), ("y", C.c_void_p),
                 ("z", C.c_double), ("k", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
-                ("mask", MaskStage * 2), ("z_dev", C.c_void_p), ("kernel_dz", C.c_int32), ("hhat_mode", C.c_int32), ("hhat", C.c_void_p)]
+                ("mask", MaskStage * 2), ("z_dev", C.c_void_p), ("kernel_dz", C.c_int32), ("hhat_mode", C.c_int32),
+                ("hhat", C.c_void_p), ("batch", C.c_int32), ("reserved2", C.c_int32), ("in_batch_stride", C.c_int64),
+                ("out_batch_stride", C.c_int64), ("intensity_batch_stride", C.c_int64),
+                ("mask_batch_stride", C.c_int64)]
 
 
 class CztParams(C.Structure):
@@ -130,6 +133,7 @@ SIGNATURES = {
     "xl_max_fft_length": (C.c_int, []),
     "xl_init": (C.c_int, []),
     "xl_rs_workspace_bytes": (C.c_size_t, [C.c_int32, C.c_int32]),
+    "xl_rs_batch_workspace_bytes": (C.c_size_t, [C.c_int32, C.c_int32, C.c_int32]),
     "xl_rs_hhat_bytes": (C.c_size_t, [C.c_int32]),
     "xl_rs_propagate": (C.c_int, [C.POINTER(RsParams), C.c_void_p, C.c_size_t, C.c_void_p]),
     "xl_rs_propagate_noalloc": (C.c_int, [C.POINTER(RsParams), C.c_void_p]),

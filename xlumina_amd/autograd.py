@@ -183,16 +183,17 @@ class CZTConfig:
 
 class _CZTFunction(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, cfg: CZTConfig, src, p00, p01, p10, p11, z_t):
+    def forward(ctx, cfg: CZTConfig, src, p00, p01, p10, p11, z_t, wl_t):
         params = (p00, p01, p10, p11)
         masks = _rebuild(cfg.mask_struct, params)
         out = ops.czt(src, cfg.geom, masks=masks, ez_mode=cfg.ez_mode, nin=cfg.nin, mode=cfg.mode, form=cfg.form)
         ctx.cfg = cfg
-        keep_out = z_t is not None and ctx.needs_input_grad[6]
+        keep_out = (z_t is not None and ctx.needs_input_grad[6]) or (wl_t is not None and ctx.needs_input_grad[7])
         ctx.save_for_backward(src, *[p if p is not None else torch.empty(0) for p in params],
                               out if keep_out else torch.empty(0))
         ctx.has = [p is not None for p in params]
         ctx.z_meta = None if z_t is None else (z_t.shape, z_t.dtype, z_t.device)
+        ctx.wl_meta = None if wl_t is None else (wl_t.shape, wl_t.dtype, wl_t.device)
         return out
 
     @staticmethod
@@ -210,6 +211,11 @@ class _CZTFunction(torch.autograd.Function):
             shape, dtype, device = ctx.z_meta
             g_z = _czt_z_grad(cfg, src, masks, gout.contiguous(), g_pre, saved[5]).to(device=device, dtype=dtype)
             g_z = g_z.reshape(shape)
+        g_wl = None
+        if ctx.wl_meta is not None and ctx.needs_input_grad[7]:
+            shape, dtype, device = ctx.wl_meta
+            g_wl = _czt_z_grad(cfg, src, masks, gout.contiguous(), g_pre, saved[5], wrt="wavelength")
+            g_wl = g_wl.to(device=device, dtype=dtype).reshape(shape)
         need_params = ((ctx.needs_input_grad[2] and params[0] is not None, ctx.needs_input_grad[3] and params[1] is not None),
                        (ctx.needs_input_grad[4] and params[2] is not None, ctx.needs_input_grad[5] and params[3] is not None))
         if not masks and cfg.ez_mode == N.XL_EZ_NONE and not hna:
@@ -225,7 +231,7 @@ class _CZTFunction(torch.autograd.Function):
                 g_src = full
         else:
             g_src = None
-        return (None, g_src, *_param_grads(ctx, cfg.mask_struct, params, gp), g_z)
+        return (None, g_src, *_param_grads(ctx, cfg.mask_struct, params, gp), g_z, g_wl)
 
 
 def _dlnh_dz(X, Y, z: float, k: float):
@@ -257,20 +263,33 @@ def _czt_axis_weights(out_grid, m: int, dm: float, dev):
     return t(p), t(q), t(jp)
 
 
-def _czt_z_grad(cfg: CZTConfig, src, masks, gout, g_pre, out):
-    """dL/dz of CZT / VCZT (wave_optics.py:333-357, vectorized_optics.py:321-361),
+def _dlnh_dk(X, Y, z: float, k: float):
+    """d/dk log h for h = transfer_function_RS (wave_optics.py:291-297):
+    -i / (1/r - i k) + i s r, s = sign(z)."""
+    r = torch.sqrt(X * X + Y * Y + z * z)
+    s = 1.0 if z > 0 else -1.0
+    return -1j / (1.0 / r - 1j * k) + 1j * s * r
+
+
+def _czt_z_grad(cfg: CZTConfig, src, masks, gout, g_pre, out, wrt: str = "z"):
+    """dL/dz (wrt = "z") or dL/dlambda (wrt = "wavelength") of CZT / VCZT
+    (wave_optics.py:333-357, vectorized_optics.py:321-361),
     out = c F0 (K_y (E' F) K_x^T) with c = z dx dy lambda, F, F0 the transfer
     function on the input / output grids, K the per-axis kernels of Dm =
     lambda z / dx and E' the pre-op'd input (for VCZT with Ez_in = (Ex X +
-    Ey Y) z / r^2):  Re<g, out>/z  +  Re<g, (dF0/F0) out>  +  the two kernel
-    terms Re<g, (-i/z)(p (.) out - q (.) CZT(j' E'))> per axis (output rows /
-    columns weighted by p, q; the input by j')  +  Re<CZT^H g, (dF/F) E'>  +
-    (VCZT) Re<g_pre[2], dEz_in/dz>.  Sums in float64."""
+    Ey Y) z / r^2).  With t the parameter:  Re<g, out>/t  +  Re<g, (dF0/F0) out>
+    +  the two kernel terms Re<g, (-i/t)(p (.) out - q (.) CZT(j' E'))> per axis
+    (output rows / columns weighted by p, q; the input by j'; the kernels
+    depend on t only through Dm, which is proportional to both z and lambda)
+    +  Re<CZT^H g, (dF/F) E'>  +  (VCZT, t = z) Re<g_pre[2], dEz_in/dz>.
+    For lambda, d log F / d lambda = (d log h / dk)(-k / lambda).  Sums in
+    float64."""
     g = cfg.geom
     if cfg.mode != N.XL_CZT_RS:
-        raise ValueError("distance gradients exist for the RS-mode CZT only")
+        raise ValueError("distance and wavelength gradients exist for the RS-mode CZT only")
     dev = src.device
     z, k = g.z, 2 * math.pi / g.wavelength
+    t = z if wrt == "z" else g.wavelength
     ncomp = out.shape[0]
     if masks or cfg.ez_mode != N.XL_EZ_NONE:
         pre = ops.pre_forward(src, g.x, g.y, z, masks=masks, ez_mode=cfg.ez_mode, ncomp=ncomp, nin=cfg.nin)
@@ -287,22 +306,28 @@ def _czt_z_grad(cfg: CZTConfig, src, masks, gout, g_pre, out):
     wy = ops.czt(pre_y, g, nin=ncomp, mode=N.XL_CZT_RS, form=cfg.form)
     wx = ops.czt(pre_x, g, nin=ncomp, mode=N.XL_CZT_RS, form=cfg.form)
     o = out.to(torch.complex128)
-    total = rdot(gout, o) / z
+    total = rdot(gout, o) / t
     Xo = torch.from_numpy(np.asarray(g.xout, dtype=np.float64)).to(dev)[None, :]
     Yo = torch.from_numpy(np.asarray(g.yout, dtype=np.float64)).to(dev)[:, None]
-    total = total + rdot(gout, o * _dlnh_dz(Xo, Yo, z, k))
-    total = total + rdot(gout, (-1j / z) * (py[None, :, None] * o - qy[None, :, None] * wy.to(torch.complex128)))
-    total = total + rdot(gout, (-1j / z) * (px[None, None, :] * o - qx[None, None, :] * wx.to(torch.complex128)))
+    if wrt == "z":
+        dln = lambda X, Y: _dlnh_dz(X, Y, z, k)
+    else:
+        dln = lambda X, Y: _dlnh_dk(X, Y, z, k) * (-k / t)
+    total = total + rdot(gout, o * dln(Xo, Yo))
+    total = total + rdot(gout, (-1j / t) * (py[None, :, None] * o - qy[None, :, None] * wy.to(torch.complex128)))
+    total = total + rdot(gout, (-1j / t) * (px[None, None, :] * o - qx[None, None, :] * wx.to(torch.complex128)))
     Xi = torch.from_numpy(np.asarray(g.x, dtype=np.float64)).to(dev)[None, :]
     Yi = torch.from_numpy(np.asarray(g.y, dtype=np.float64)).to(dev)[:, None]
-    # Re<CZT^H g, (dF/F) E'>: dF/F holds i s k z / r ~ i k, whose product with
-    # sum conj(g_pre) E' cancels only in the sum (adjointness makes that sum
-    # <g, out>).  The constant i s k is taken out and applied to <g, out>, so the
-    # per-pixel float32 rounding of g_pre meets weights of k (z/r - 1) only.
-    s_k = (1.0 if z > 0 else -1.0) * k
-    total = total + rdot(g_pre[:ncomp], pre.to(torch.complex128) * (_dlnh_dz(Xi, Yi, z, k) - 1j * s_k))
-    total = total + torch.real(1j * s_k * torch.sum(gout.conj().to(torch.complex128) * o))
-    if cfg.ez_mode == N.XL_EZ_VCZT:
+    # Re<CZT^H g, (dF/F) E'>: dF/F holds a large near-constant phase rate (i s k
+    # z / r ~ i k in z; -i s k r / lambda ~ -i s k z / lambda in lambda), whose
+    # product with sum conj(g_pre) E' cancels only in the sum (adjointness makes
+    # that sum <g, out>).  The constant is taken out and applied to <g, out>, so
+    # the per-pixel float32 rounding of g_pre meets the varying remainder only.
+    s = 1.0 if z > 0 else -1.0
+    c0 = 1j * s * k if wrt == "z" else -1j * s * k * z / t
+    total = total + rdot(g_pre[:ncomp], pre.to(torch.complex128) * (dln(Xi, Yi) - c0))
+    total = total + torch.real(c0 * torch.sum(gout.conj().to(torch.complex128) * o))
+    if cfg.ez_mode == N.XL_EZ_VCZT and wrt == "z":
         r2 = Xi * Xi + Yi * Yi + z * z
         dez = (pre[0].to(torch.complex128) * Xi + pre[1].to(torch.complex128) * Yi) * ((r2 - 2 * z * z) / (r2 * r2))
         total = total + rdot(g_pre[2], dez)
@@ -311,14 +336,17 @@ def _czt_z_grad(cfg: CZTConfig, src, masks, gout, g_pre, out):
 
 def czt(src: torch.Tensor, geom: ops.CztGeometry, masks: Sequence[MaskSpec] = (), ez_mode: int = N.XL_EZ_NONE,
         nin: Optional[int] = None, mode: Optional[int] = None, form: Optional[str] = None,
-        z: Optional[torch.Tensor] = None) -> torch.Tensor:
+        z: Optional[torch.Tensor] = None, wavelength: Optional[torch.Tensor] = None) -> torch.Tensor:
     """Differentiable CZT / VCZT / high-NA VCZT of src (C, ny, nx); `form` as
-    ops.czt.  `z`: the distance as a tensor (geom holds its value) when dL/dz
-    is wanted (RS-mode CZT only)."""
+    ops.czt.  `z` / `wavelength`: the distance / wavelength as one-element
+    tensors (geom holds their values) when dL/dz / dL/dlambda is wanted
+    (RS-mode CZT only)."""
     nin = src.shape[0] if nin is None else nin
     mode = geom.mode if mode is None else mode
+    if wavelength is not None and abs(float(wavelength.detach().reshape(-1)[0]) - geom.wavelength) > 1e-12 * geom.wavelength:
+        raise ValueError("the wavelength tensor must hold the geometry's wavelength")
     cfg = CZTConfig(geom, ez_mode, nin, mode, _mask_structure(masks), ops.resolve_czt_form(geom, form))
-    return _CZTFunction.apply(cfg, src, *_flat_params(masks), z)
+    return _CZTFunction.apply(cfg, src, *_flat_params(masks), z, wavelength)
 
 
 @dataclass

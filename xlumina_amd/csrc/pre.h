@@ -35,6 +35,7 @@ struct PreArgs {
 __device__ __forceinline__ PreArgs pre_member(const PreArgs& a0, int b) {
   PreArgs a = a0;
   a.in += b * a0.in_bs;
+  if (a.z_dev) a.z_dev += b;   // one distance per member (b = 0 without a batch)
   if (a.gout) a.gout += b * a0.gout_bs;
   if (a.out) a.out += b * a0.out_bs;
 #pragma unroll

@@ -48,7 +48,31 @@ struct RsArgs {
   int hstored;            // the spectrum lives in a caller buffer, unconjugated (adjoints conjugate on read)
   int hreuse;             // the spectrum is already there: skip rs_hrow / rs_hcol
   MaskArgs mask;
+  // Batch of independent problems of one geometry (blockIdx.y = member): element strides
+  // between the members' inputs, outputs, intensities, mask parameters, tiles and kernel
+  // spectra; with z_dev set, member b reads z_dev[b].
+  int batch;
+  int64_t in_bs, out_bs, inten_bs, mask_bs, t_bs, hr_bs;
 };
+
+// The arguments of batch member b (pointers offset by the batch strides).
+__device__ __forceinline__ RsArgs rs_member(const RsArgs& a0, int b) {
+  if (a0.batch <= 1) return a0;
+  RsArgs a = a0;
+  a.in += b * a0.in_bs;
+  a.out += b * a0.out_bs;
+  if (a.intensity) a.intensity += b * a0.inten_bs;
+  a.t += b * a0.t_bs;
+  a.hr += b * a0.hr_bs;
+  a.hmid += b * a0.hr_bs;
+  if (a.z_dev) a.z_dev += b;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    if (a.mask.st[i].m0) a.mask.st[i].m0 += b * a0.mask_bs;
+    if (a.mask.st[i].m1) a.mask.st[i].m1 += b * a0.mask_bs;
+  }
+  return a;
+}
 
 // lag value of circular slot s on a P-point axis with N samples (0 => unused)
 __device__ __forceinline__ bool lag_of(const double* g, int s, int n, int P, double* d) {
@@ -115,7 +139,8 @@ template <int P> struct Units {
 // One kernel row per group: H of rows 0..P/2-1 generated in float64 phase
 // (the array is even in x: columns above P/2 mirror), row FFT, stored as Hr.
 template <int P>
-__global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_hrow_kernel(RsArgs a) {
+__global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_hrow_kernel(RsArgs a0) {
+  const RsArgs a = rs_member(a0, blockIdx.y);
   constexpr int TPF = P / 16, LS = LdsLen<P>::value, CPB = Units<P>::CPB;
   __shared__ float2 lds[CPB * LS];
   const int g = threadIdx.x / TPF, t = threadIdx.x % TPF;
@@ -146,7 +171,8 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_hrow_kernel(Rs
 // [c*TPF, (c+1)*TPF) in LDS buffer c.  Rows 4g..4g+3 run back to back on one
 // XCD, so the four 32 B row slices of each 128 B tile meet in the same L2.
 template <int P>
-__global__ __launch_bounds__(3 * (P / 16)) void rs_urow_kernel(RsArgs a) {
+__global__ __launch_bounds__(3 * (P / 16)) void rs_urow_kernel(RsArgs a0) {
+  const RsArgs a = rs_member(a0, blockIdx.y);
   constexpr int TPF = P / 16, LS = LdsLen<P>::value;
   extern __shared__ float2 lds[];
   const int tid = threadIdx.x, nthr = blockDim.x;
@@ -186,7 +212,8 @@ __global__ __launch_bounds__(3 * (P / 16)) void rs_urow_kernel(RsArgs a) {
 // components in registers, and the components are transformed one after the
 // other from there (no LDS staging pass).
 template <int P>
-__global__ __launch_bounds__(P / 16) void rs_urow1_kernel(RsArgs a) {
+__global__ __launch_bounds__(P / 16) void rs_urow1_kernel(RsArgs a0) {
+  const RsArgs a = rs_member(a0, blockIdx.y);
   constexpr int TPF = P / 16, LS = LdsLen<P>::value;
   constexpr int R0 = FftPlan<P>::radix(true, 0);
   static_assert(R0 == 16 && P / R0 == TPF, "first pass: one radix-16 butterfly per thread");
@@ -234,7 +261,8 @@ __global__ __launch_bounds__(P / 16) void rs_urow1_kernel(RsArgs a) {
 // to hmid[col].  The array is even in y, so ky > P/2 is not stored
 // (Hhat(P - ky) = Hhat(ky)).
 template <int P>
-__global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_hcol_kernel(RsArgs a) {
+__global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_hcol_kernel(RsArgs a0) {
+  const RsArgs a = rs_member(a0, blockIdx.y);
   constexpr int TPF = P / 16, LS = LdsLen<P>::value, CPB = Units<P>::CPB;
   using HO = HandOff<P, TPF>;
   __shared__ float2 lds[CPB * LS];
@@ -270,7 +298,8 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_hcol_kernel(Rs
 // units of a column and of its tile neighbours run back to back on one XCD,
 // so the Hhat column and the 128 B tile lines are fetched once into its L2.
 template <int P>
-__global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_col_kernel(RsArgs a) {
+__global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_col_kernel(RsArgs a0) {
+  const RsArgs a = rs_member(a0, blockIdx.y);
   constexpr int TPF = P / 16, LS = LdsLen<P>::value, CPB = Units<P>::CPB;
   using HO = HandOff<P, TPF>;
   __shared__ float2 lds[CPB * LS];
@@ -318,7 +347,8 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_col_kernel(RsA
 // |E|^2 is accumulated over the components.  Two 35 KB buffers per block let
 // two blocks share a CU.
 template <int P>
-__global__ __launch_bounds__(2 * FftThreads<P>::value) void rs_final_kernel(RsArgs a) {
+__global__ __launch_bounds__(2 * FftThreads<P>::value) void rs_final_kernel(RsArgs a0) {
+  const RsArgs a = rs_member(a0, blockIdx.y);
   constexpr int TPF = FftThreads<P>::value, LS = LdsLen<P>::value;
   constexpr int NT = 2 * TPF;
   constexpr int PPT = (P / 2 + TPF - 1) / TPF;
@@ -371,7 +401,8 @@ __global__ __launch_bounds__(2 * FftThreads<P>::value) void rs_final_kernel(RsAr
 // each last pass left in registers, where the first N samples are stored and
 // |E|^2 is accumulated over the components.
 template <int P>
-__global__ __launch_bounds__(P / 16) void rs_final1_kernel(RsArgs a) {
+__global__ __launch_bounds__(P / 16) void rs_final1_kernel(RsArgs a0) {
+  const RsArgs a = rs_member(a0, blockIdx.y);
   constexpr int TPF = P / 16, LS = LdsLen<P>::value;
   using LP = LastPass<P, TPF, false>;
   __shared__ float2 lds[LS];
@@ -390,8 +421,9 @@ __global__ __launch_bounds__(P / 16) void rs_final1_kernel(RsArgs a) {
   for (int c = 0; c < a.ncomp; ++c) {
     const float2* tc = a.t + (int64_t)c * nrg * 4 * P + rg * 16 + rr * 4;
     auto load = [&](int col) -> float2 { return tc[(int64_t)(col >> 2) * nrg * 16 + (col & 3)]; };
-    typename LP::Type o;
-    fft_io<P, TPF, true, SRC_FN, DST_REG>(lds, t, load, none, o);
+    typename LP::Type o, unused;
+    // the kept columns are < N <= P/2: the last pass forms the lower half only
+    chain_io<P, TPF, true, false, 0, SRC_FN, DST_REG, false, true>(lds, t, load, none, unused, o);
     float2* oc = a.out + (int64_t)c * a.out_cs + (int64_t)row * a.n;
 #pragma unroll
     for (int b = 0; b < LP::BPT; ++b) {
@@ -439,27 +471,34 @@ inline hipError_t rs_launch(const RsArgs& a, hipStream_t st) {
   constexpr int CPB = Units<P>::CPB;
   const dim3 blk(CPB * (P / 16));
   const int rows = ((a.n + 3) >> 2) * 4;
-  if (!a.hreuse) { KTimer t(K_RS_HROW, st); hipLaunchKernelGGL(rs_hrow_kernel<P>, dim3(P / 2 / CPB), blk, 0, st, a); }
+  const int nb = a.batch > 1 ? a.batch : 1;
+  if (!a.hreuse) {
+    KTimer t(K_RS_HROW, st);
+    hipLaunchKernelGGL(rs_hrow_kernel<P>, dim3(P / 2 / CPB, nb), blk, 0, st, a);
+  }
   if constexpr (P >= 1024) {
     KTimer t(K_RS_UROW, st);
-    hipLaunchKernelGGL(rs_urow1_kernel<P>, dim3(rows), dim3(P / 16), 0, st, a);
+    hipLaunchKernelGGL(rs_urow1_kernel<P>, dim3(rows, nb), dim3(P / 16), 0, st, a);
   } else {
     KTimer t(K_RS_UROW, st);
-    hipLaunchKernelGGL(rs_urow_kernel<P>, dim3(rows), dim3(a.ncomp * (P / 16)), 
+    hipLaunchKernelGGL(rs_urow_kernel<P>, dim3(rows, nb), dim3(a.ncomp * (P / 16)),
                        (size_t)a.ncomp * LdsLen<P>::value * sizeof(float2), st, a);
   }
-  if (!a.hreuse) { KTimer t(K_RS_HCOL, st); hipLaunchKernelGGL(rs_hcol_kernel<P>, dim3(P / CPB), blk, 0, st, a); }
+  if (!a.hreuse) {
+    KTimer t(K_RS_HCOL, st);
+    hipLaunchKernelGGL(rs_hcol_kernel<P>, dim3(P / CPB, nb), blk, 0, st, a);
+  }
   {
     KTimer t(K_RS_COL, st);
     const int units = P * a.ncomp;
-    hipLaunchKernelGGL(rs_col_kernel<P>, dim3((units + CPB - 1) / CPB), blk, 0, st, a);
+    hipLaunchKernelGGL(rs_col_kernel<P>, dim3((units + CPB - 1) / CPB, nb), blk, 0, st, a);
   }
   {
     KTimer t(K_RS_FINAL, st);
     if constexpr (P >= 1024)
-      hipLaunchKernelGGL(rs_final1_kernel<P>, dim3(a.n), dim3(P / 16), 0, st, a);
+      hipLaunchKernelGGL(rs_final1_kernel<P>, dim3(a.n, nb), dim3(P / 16), 0, st, a);
     else
-      hipLaunchKernelGGL(rs_final_kernel<P>, dim3(rows / 2), dim3(2 * FftThreads<P>::value), 0, st, a);
+      hipLaunchKernelGGL(rs_final_kernel<P>, dim3(rows / 2, nb), dim3(2 * FftThreads<P>::value), 0, st, a);
   }
   return hipGetLastError();
 }

@@ -117,10 +117,15 @@ int check_mask(const xl_mask_stage* m, int nin) {
 // ---------------------------------------------------------------- RS
 int rs_fft_len(int n) { return std::max(64, next_pow2(2 * (int64_t)n - 1)); }
 
-size_t rs_ws(int n, int ncomp) {
+// complex64 elements of one problem's workspace block: tiles + kernel spectrum
+int64_t rs_ws_elems(int n, int ncomp) {
   const int64_t P = rs_fft_len(n);
   const int64_t nr4 = ((n + 3) / 4) * 4;
-  return (size_t)(ncomp * nr4 * P + (P / 2) * P + P) * sizeof(float2) + 256;
+  return ncomp * nr4 * P + (P / 2) * P + P;
+}
+
+size_t rs_ws(int n, int ncomp, int batch = 1) {
+  return (size_t)std::max(1, batch) * (size_t)rs_ws_elems(n, ncomp) * sizeof(float2) + 256;
 }
 
 // ---------------------------------------------------------------- CZT
@@ -252,7 +257,7 @@ char* align256(char* p) { return (char*)(((uintptr_t)p + 255) & ~(uintptr_t)255)
 extern "C" {
 
 const char* xl_last_error(void) { return g_err.c_str(); }
-int xl_abi_version(void) { return 5; }
+int xl_abi_version(void) { return 6; }
 int xl_max_fft_length(void) { return xl::MAX_FFT; }
 int xl_init(void) { return ensure_init(); }
 
@@ -265,6 +270,11 @@ size_t xl_rs_hhat_bytes(int32_t n) {
 size_t xl_rs_workspace_bytes(int32_t n, int32_t ncomp) {
   if (n < 2 || ncomp < 1 || ncomp > 3) return 0;
   return rs_ws(n, ncomp);
+}
+
+size_t xl_rs_batch_workspace_bytes(int32_t n, int32_t ncomp, int32_t batch) {
+  if (n < 2 || ncomp < 1 || ncomp > 3 || batch < 0 || batch > 65535) return 0;
+  return rs_ws(n, ncomp, batch);
 }
 
 int xl_rs_propagate(const xl_rs_params* p, void* ws, size_t ws_bytes, void* stream) {
@@ -285,7 +295,21 @@ int xl_rs_propagate(const xl_rs_params* p, void* ws, size_t ws_bytes, void* stre
   if (p->hhat_mode && (!p->hhat || p->kernel_dz))
     return fail(XL_EINVAL, "hhat_mode 1 / 2 needs the hhat buffer and kernel_dz = 0");
   const int ncomp = p->ez_mode == XL_EZ_VRS ? 3 : p->nin;
-  const size_t need = rs_ws(n, ncomp);
+  if (p->batch < 0 || p->batch > 65535) return fail(XL_EINVAL, "batch must be 0..65535");
+  const int batch = std::max(1, (int)p->batch);
+  const int64_t n2 = (int64_t)n * n;
+  if (batch > 1) {
+    if (p->in_batch_stride < 0 || p->mask_batch_stride < 0)
+      return fail(XL_EINVAL, "input / mask batch strides must be >= 0 (0 shares one array)");
+    if (p->mask_batch_stride > 0 && p->mask_batch_stride < n2)
+      return fail(XL_EINVAL, "per-member mask parameters must not overlap (mask_batch_stride >= n^2)");
+    // members write their outputs concurrently: overlapping output blocks would race
+    if (p->out_batch_stride < (int64_t)ncomp * n2)
+      return fail(XL_EINVAL, "out_batch_stride must be >= ncomp n^2 (members write disjoint outputs)");
+    if (p->intensity && p->intensity_batch_stride < n2)
+      return fail(XL_EINVAL, "intensity_batch_stride must be >= n^2 (members write disjoint intensities)");
+  }
+  const size_t need = rs_ws(n, ncomp, batch);
   if (!ws || ws_bytes < need) return fail(XL_EWORKSPACE, "RS workspace too small: need " + std::to_string(need));
   char* w = align256((char*)ws);
   const int64_t nr4 = ((n + 3) / 4) * 4;
@@ -314,6 +338,13 @@ int xl_rs_propagate(const xl_rs_params* p, void* ws, size_t ws_bytes, void* stre
   a.adjoint = p->adjoint;
   a.mask = to_mask(p->mask);
   a.scale = (float)(p->dx * p->dy / ((double)P * (double)P));
+  a.batch = batch;
+  a.in_bs = batch > 1 ? p->in_batch_stride : 0;
+  a.out_bs = batch > 1 ? p->out_batch_stride : 0;
+  a.inten_bs = batch > 1 ? p->intensity_batch_stride : 0;
+  a.mask_bs = batch > 1 ? p->mask_batch_stride : 0;
+  a.t_bs = rs_ws_elems(n, ncomp);
+  a.hr_bs = p->hhat_mode ? (int64_t)(P / 2) * P + P : a.t_bs;   // hhat holds one spectrum per member
   hipStream_t st = (hipStream_t)stream;
   int rc = dispatch_q(P, [&](auto c) { (void)xl::rs_launch<decltype(c)::value>(a, st); });
   if (rc) return rc;
@@ -337,6 +368,10 @@ int xl_rs_propagate_noalloc(const xl_rs_params* p, void* stream) {
     if (!g_init_done[dev]) return fail(XL_EINVAL, "xl_init() has not run on this device");
     ws = g_scratch[dev];
   }
+  const int ncomp = p->ez_mode == XL_EZ_VRS ? 3 : std::max(1, (int)p->nin);
+  if (p->n >= 2 && ncomp <= 3 && rs_ws(p->n, ncomp, std::max(1, (int)p->batch)) > XL_RS_NOALLOC_SCRATCH_BYTES)
+    return fail(XL_EINVAL, "this batch of " + std::to_string(p->n) + "^2 problems needs more than the " +
+                               std::to_string(XL_RS_NOALLOC_SCRATCH_BYTES) + "-byte scratch of xl_rs_propagate_noalloc");
   // every kernel writes the scratch before reading it (zero-padded rows and
   // columns included), so nothing in it has to start at zero
   return xl_rs_propagate(p, ws, XL_RS_NOALLOC_SCRATCH_BYTES, stream);

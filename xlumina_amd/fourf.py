@@ -74,8 +74,8 @@ class FourFOptimizer:
         self.lr, self.b1, self.b2, self.eps, self.wd = lr, b1, b2, eps, weight_decay
         self.g_f = ops.czt_geometry(self.x, self.y, self.x, self.y, f, wavelength)
         self.g_2f = ops.czt_geometry(self.x, self.y, self.x, self.y, 2 * f, wavelength)
-        e_in = torch.from_numpy(gaussian_field(self.x, self.y, wavelength, w0).astype(np.complex64)).to(dev)
-        self.a1 = ops.czt(e_in[None], self.g_f, form="bluestein")          # (1, n, n), shared
+        self.e_in = torch.from_numpy(gaussian_field(self.x, self.y, wavelength, w0).astype(np.complex64)).to(dev)
+        self.a1 = ops.czt(self.e_in[None], self.g_f, form="bluestein")     # (1, n, n), shared
         ph = seeded_phases(batch, n) if phases is None else np.asarray(phases, dtype=np.float32)
         if ph.shape != (batch, 2, n, n):
             raise ValueError(f"phases must be {(batch, 2, n, n)}")
@@ -119,6 +119,18 @@ class FourFOptimizer:
         ops.pre_backward_batched(self.a1, self.buf_b, self.x, self.y, B, masks=[MaskSpec(N.XL_MASK_SLM, ph1)],
                                  need_src=False, need_params=((True, False), (False, False)),
                                  gparams=[[self.grad[:, 0], None], [None, None]])
+
+    def forward_intensity(self) -> torch.Tensor:
+        """Detector intensities (B, n, n) of the current phases, through the
+        same batched launches as a training step's forward half."""
+        B, n = self.batch, self.n
+        ph1, ph2 = self.phi[:, 0], self.phi[:, 1]
+        a2 = torch.empty_like(self.a2)
+        a3 = torch.empty_like(self.buf_a)
+        ops.czt(self.a1, self.g_2f, masks=[MaskSpec(N.XL_MASK_SLM, ph1)], batch=B, out=a2, workspace=self.ws)
+        ops.czt(a2, self.g_f, masks=[MaskSpec(N.XL_MASK_SLM, ph2)], batch=B, out=a3, workspace=self.ws)
+        a3 = a3.view(B, n, n)
+        return a3.real ** 2 + a3.imag ** 2
 
     def step(self, loss_out: torch.Tensor) -> None:
         """One training step.  The mask adjoints run fused with the Adam update

@@ -315,6 +315,97 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
     return (out, inten) if intensity else out
 
 
+def rs_batch_workspace_bytes(n: int, ncomp: int, batch: int) -> int:
+    return int(N.load().xl_rs_batch_workspace_bytes(n, ncomp, batch))
+
+
+def rs_propagate_batched(src: torch.Tensor, x, y, wavelength: float, z_dev: torch.Tensor,
+                         masks: Sequence[MaskSpec] = (), ez_mode: int = N.XL_EZ_NONE, nin: Optional[int] = None,
+                         intensity: bool = False, adjoint: bool = False, batch: Optional[int] = None,
+                         out: Optional[torch.Tensor] = None, inten: Optional[torch.Tensor] = None,
+                         workspace: Optional[torch.Tensor] = None, hhat: Optional[torch.Tensor] = None,
+                         hhat_mode: int = 0):
+    """B independent RS / VRS propagations of one grid in the same launches
+    (rs_propagate over a batch, each member with its own distance).
+
+    src: (B, C, n, n) complex64, or one (C, n, n) source shared by the batch
+    (then `batch` gives B); z_dev: (B,) float64 device distances, member b
+    propagates over z_dev[b]; mask parameters: (B, n, n) float32 views with
+    contiguous rows sharing one batch stride, or (n, n) shared by the batch;
+    hhat / hhat_mode as in rs_propagate with B spectra (B x rs_hhat_bytes(n)).
+    Returns out (B, ncomp, n, n), plus inten (B, n, n) if `intensity`.
+    """
+    lib = N.load()
+    if not isinstance(src, torch.Tensor) or src.device.type != "cuda" or src.dtype != torch.complex64:
+        raise N.NativeUnavailable("fields must be complex64 tensors on a ROCm GPU")
+    dev = src.device
+    N.ensure_device(dev)
+    src = src.contiguous()
+    shared = src.dim() == 3
+    if src.dim() not in (3, 4):
+        raise ValueError("src must be (B, C, n, n) or a shared (C, n, n)")
+    if batch is None:
+        if shared:
+            raise ValueError("a source shared by the batch needs batch=")
+        batch = src.shape[0]
+    if batch < 1 or (not shared and src.shape[0] != batch):
+        raise ValueError(f"source holds {src.shape[0]} members, batch is {batch}")
+    n = src.shape[-1]
+    if src.shape[-2] != n:
+        raise ValueError("RS propagation needs a square grid (the reference builds one lag vector for both axes)")
+    ncin = src.shape[-3]
+    nin = ncin if nin is None else nin
+    ncomp = 3 if ez_mode == N.XL_EZ_VRS else nin
+    check_rs_args((ncin, n, n), nin, ncomp, _grid_len(x), _grid_len(y), None, None, dev)
+    if (not isinstance(z_dev, torch.Tensor) or z_dev.dtype != torch.float64 or z_dev.device != dev
+            or z_dev.numel() != batch or not z_dev.is_contiguous()):
+        raise ValueError(f"z_dev must be a contiguous float64 tensor of {batch} distances on the field's device")
+    if out is None:
+        out = torch.empty((batch, ncomp, n, n), dtype=torch.complex64, device=dev)
+    check_buffer(out, (batch, ncomp, n, n), torch.complex64, dev, "out")
+    if intensity:
+        if inten is None:
+            inten = torch.empty((batch, n, n), dtype=torch.float32, device=dev)
+        check_buffer(inten, (batch, n, n), torch.float32, dev, "inten")
+    else:
+        inten = None
+    check_workspace(workspace, dev)
+    need = rs_batch_workspace_bytes(n, ncomp, batch)
+    if workspace is None or workspace.numel() < need:
+        workspace = torch.empty(need, dtype=torch.uint8, device=dev)
+    xd, yd = device_grid(x, dev), device_grid(y, dev)
+    p = N.RsParams()
+    p.inp = src.data_ptr()
+    p.in_comp_stride = n * n
+    p.nin = nin
+    p.ez_mode = ez_mode
+    p.out = out.data_ptr()
+    p.out_comp_stride = n * n
+    p.intensity = _ptr(inten) or None
+    p.n = n
+    p.adjoint = 1 if adjoint else 0
+    p.x, p.y = xd.data_ptr(), yd.data_ptr()
+    p.k = 2 * math.pi / wavelength
+    xa = np.asarray(x, dtype=np.float64) if not isinstance(x, torch.Tensor) else x.detach().cpu().double().numpy()
+    ya = np.asarray(y, dtype=np.float64) if not isinstance(y, torch.Tensor) else y.detach().cpu().double().numpy()
+    p.dx = float(xa[1] - xa[0])
+    p.dy = float(ya[1] - ya[0])
+    p.z = 0.0
+    p.z_dev = z_dev.data_ptr()
+    if hhat_mode:
+        if hhat is None or hhat.device != dev or hhat.numel() * hhat.element_size() < batch * rs_hhat_bytes(n):
+            raise ValueError("hhat must be a device buffer of batch * rs_hhat_bytes(n) bytes on the field's device")
+        p.hhat_mode, p.hhat = int(hhat_mode), hhat.data_ptr()
+    keep, p.mask_batch_stride = fill_masks(p.mask, masks, (n, n), dev, nin, batch=batch)
+    p.batch = batch
+    p.in_batch_stride = 0 if shared else ncin * n * n
+    p.out_batch_stride = ncomp * n * n
+    p.intensity_batch_stride = n * n
+    N.check(lib.xl_rs_propagate(C.byref(p), workspace.data_ptr(), workspace.numel(), _stream(dev)))
+    del keep
+    return (out, inten) if intensity else out
+
+
 # ---------------------------------------------------------------------------
 # CZT / VCZT / high-NA
 # ---------------------------------------------------------------------------
@@ -638,12 +729,14 @@ def transfer_function_rs(x, y, z: float, wavelength: float, device) -> torch.Ten
 def pre_backward_batched(src: torch.Tensor, gout: torch.Tensor, x, y, batch: int, z: float = 0.0,
                          masks: Sequence[MaskSpec] = (), ez_mode: int = N.XL_EZ_NONE, nin: Optional[int] = None,
                          need_src: bool = True, need_params=((False, False), (False, False)),
-                         gparams=None, out: Optional[torch.Tensor] = None):
+                         gparams=None, out: Optional[torch.Tensor] = None, z_dev: Optional[torch.Tensor] = None):
     """pre_backward over a batch in one launch.  src: (B, nin, rows, cols) or one
     shared (nin, rows, cols); gout: (B, ncomp, rows, cols); mask parameters
     (B, rows, cols) views or shared.  Returns (grad of src (B, nin, ...) or
     None, [[g00, g01], [g10, g11]] each (B, rows, cols) or None); `gparams`
-    may hold preallocated (B, rows, cols) views sharing one batch stride."""
+    may hold preallocated (B, rows, cols) views sharing one batch stride.
+    z_dev: optional (B,) float64 device distances of the Ez construction, one
+    per member (instead of `z`)."""
     lib = N.load()
     for t in (src, gout):
         if not isinstance(t, torch.Tensor) or t.device.type != "cuda" or t.dtype != torch.complex64:
@@ -698,6 +791,11 @@ def pre_backward_batched(src: torch.Tensor, gout: torch.Tensor, x, y, batch: int
     if len(strides) > 1:
         raise ValueError("parameter-gradient outputs must share one batch stride")
     p.gparam_batch_stride = strides.pop() if strides else 0
+    if z_dev is not None:
+        if (z_dev.dtype != torch.float64 or z_dev.device != dev or z_dev.numel() != batch
+                or not z_dev.is_contiguous()):
+            raise ValueError(f"z_dev must be a contiguous float64 tensor of {batch} distances on the field's device")
+        p.z_dev = z_dev.data_ptr()
     N.check(lib.xl_pre_backward(C.byref(p), _stream(dev)))
     del keep
     return gsrc, gp
