@@ -66,6 +66,9 @@ def main():
                     help="requests per serving run (0 skips the serving sweep)")
     ap.add_argument("--reference-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (scale reference for HBM byte counters)")
+    ap.add_argument("--workload", choices=("chain", "high_na", "fourf", "rs1024"), default="chain",
+                    help="chain (default): the full benchmark; the others time one secondary workload alone "
+                         "(for profiler runs) and print its JSON entry")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); several GPUs would each run "
@@ -81,6 +84,9 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     N.ensure_device(dev)
+    if args.workload != "chain":
+        print(json.dumps(run_single(dev, workloads, args)), flush=True)
+        return
 
     w = workloads.vrs_vczt_2048(dev)
     geom = w.geometry()
@@ -395,6 +401,48 @@ def _stats(samples):
     a = np.asarray(samples, dtype=np.float64)
     return {"n": int(a.size), "mean_s": round(float(a.mean()), 4), "p50_s": round(float(np.percentile(a, 50)), 4),
             "p99_s": round(float(np.percentile(a, 99)), 4)}
+
+
+def run_single(dev, workloads, args):
+    """One secondary workload alone, `args.steps` timed steps after
+    `args.warmup` (the profiler runs of the committed summaries use this)."""
+    from xlumina_amd import ops
+    if args.workload == "high_na":
+        h = workloads.high_na_2048(dev)
+        geom = h.geometry()
+        alpha = h.alpha.clone().requires_grad_(True)
+        phi = h.phi.clone().requires_grad_(True)
+        out = {"workload": "high_na_2048_to_256 forward + backward (second-moment loss)"}
+        for form in ("bluestein", "gemm"):
+            ops.set_czt_form(form)
+
+            def step():
+                L = h.loss(alpha, phi, geom)
+                L.backward()
+
+            ms = _time(step, args.steps, args.warmup)
+            out[form] = {"ms": round(ms, 4), "steps_s": round(1000.0 / ms, 2),
+                         "kernels_ms": _kernel_split(step, max(1, min(args.steps, 10)))}
+        ops.set_czt_form("auto")
+        return out
+    if args.workload == "rs1024":
+        rs = workloads.scalar_rs_1024(dev)
+        ms = _time(lambda: rs.run(), args.steps, args.warmup)
+        return {"workload": "scalar RS 1024^2 with SLM, field + |E|^2", "ms": round(ms, 4),
+                "kernels_ms": _kernel_split(lambda: rs.run(), max(1, min(args.steps, 10)))}
+    from xlumina_amd import fourf
+    opt = fourf.FourFOptimizer(dev, batch=64, n=1024, steps_per_launch=100, use_graph=False)
+    for _ in range(args.warmup):
+        opt.step(opt.losses[0])
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        opt.step(opt.losses[0])
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    return {"workload": "4f optimisation 64 x 1024^2, eager steps", "ms_per_step": round(dt * 1e3, 3),
+            "kernels_ms": _kernel_split(lambda: opt.step(opt.losses[0]), max(1, min(args.steps, 3)))}
 
 
 def cpu_baseline(w, budget_s=24.0, max_warm=4):

@@ -438,6 +438,15 @@ int xl_adam_tick(int32_t* count, void* stream);
 int xl_transfer_function_rs(void* out, const double* x, const double* y, int32_t nx, int32_t ny, double z,
                             double k, void* stream);
 
+/* Seeded uniform phase masks (no counterpart in the reference, whose phases
+ * come from the caller; used for synthetic request streams): for each member
+ * b < batch with enable[b] != 0 (enable NULL: every member),
+ * out[b * batch_stride + i] = 2 pi u - pi for i < count, u in [0, 1) from a
+ * counter-based hash (splitmix64) of seeds[b] and i.  Seeds and flags are
+ * device arrays, so a captured HIP graph regenerates a batch from new seeds. */
+int xl_uniform_phases(float* out, const uint64_t* seeds, const int32_t* enable, int64_t count, int32_t batch,
+                      int64_t batch_stride, void* stream);
+
 /* Opt-in per-kernel timing: HIP events recorded on the launch stream around
  * every kernel launch of the calls that follow (capacity = launches kept).
  * xl_timing_collect waits on them, writes per-kernel totals (up to

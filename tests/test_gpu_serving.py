@@ -21,6 +21,7 @@ from oracle import xlumina_ref as R
 from tests.conftest import rel_l2
 from xlumina_amd import _native as N
 from xlumina_amd import autograd as AG
+from xlumina_amd import ops
 from xlumina_amd import serving as S
 from xlumina_amd.ops import MaskSpec
 
@@ -176,3 +177,26 @@ def test_poisson_stream_report(server):
     assert server.used_bytes <= server.hbm_budget
     d = rep.summary()
     assert d["offered_rate_req_s"] == 2000.0 and d["throughput_req_s"] > 0
+
+
+def test_seeded_phase_masks(cuda_dev):
+    """xl_uniform_phases: U[-pi, pi) statistics, the same masks for the same
+    seed in any batch position, different masks for different seeds, and
+    members whose flag is 0 left untouched."""
+    n = 256
+    out = torch.full((3, 2, n, n), 7.0, dtype=torch.float32, device=cuda_dev)
+    seeds = torch.tensor([11, 12, 11], dtype=torch.int64, device=cuda_dev)
+    enable = torch.tensor([1, 1, 0], dtype=torch.int32, device=cuda_dev)
+    ops.uniform_phases(out, seeds, enable)
+    a = out.cpu().numpy()
+    assert np.all(a[2] == 7.0)                                   # disabled member untouched
+    v = a[:2].ravel()
+    assert v.min() >= -math.pi and v.max() < math.pi
+    assert abs(v.mean()) < 0.02 and abs(v.var() - math.pi ** 2 / 3) < 0.03     # about 5 standard errors
+    hist = np.histogram(v, bins=16, range=(-math.pi, math.pi))[0] / v.size
+    assert np.abs(hist - 1 / 16).max() < 0.002
+    again = torch.empty((1, 2, n, n), dtype=torch.float32, device=cuda_dev)
+    ops.uniform_phases(again, seeds[2:3])                        # seed 11 alone
+    assert np.array_equal(again.cpu().numpy()[0], a[0])
+    assert not np.array_equal(a[0], a[1])
+    assert abs(np.corrcoef(a[0].ravel(), a[1].ravel())[0, 1]) < 0.015

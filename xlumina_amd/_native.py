@@ -163,6 +163,8 @@ SIGNATURES = {
     "xl_beam_splitter": (C.c_int, [C.POINTER(BsParams), C.c_void_p]),
     "xl_bs_dots_workspace_bytes": (C.c_size_t, [C.c_int64]),
     "xl_bs_dots": (C.c_int, [C.POINTER(BsDotsParams), C.c_void_p, C.c_size_t, C.c_void_p]),
+    "xl_uniform_phases": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int64,
+                                    C.c_void_p]),
     "xl_timing_enable": (C.c_int, [C.c_int32, C.c_int32]),
     "xl_timing_collect": (C.c_int, [C.POINTER(KernelTime), C.c_int32]),
 }

@@ -52,12 +52,15 @@ __device__ __forceinline__ PreArgs pre_member(const PreArgs& a0, int b) {
 // Richards-Wolf lens factors (optical_elements.py:528-590): the 3x2 real map
 // (Ex', Ey') -> lens output, with Ez = (Ex' X + Ey' Y)/rho folded in.
 __device__ __forceinline__ void hna_map(double X, double Y, double radius, double f, double stm2, float A[3][2]) {
-  const double rho = sqrt(X * X + Y * Y);
-  const float cp = (float)(X / rho), sp = (float)(Y / rho);
+  // rho = 0 (odd N) gives cp = sp = NaN through 0 * inf, as X / rho does in the reference
+  const double rho2 = X * X + Y * Y;
+  const double irho = rsqrt_f64(rho2);
+  const double rho = rho2 * irho;
+  const float cp = (float)(X * irho), sp = (float)(Y * irho);
   float st, ct;
   sincos_rad((float)(rho / f), &st, &ct);
-  const double u2 = (X * X + Y * Y) / (radius * radius);
-  const float G = u2 < 1.0 ? (float)(1.0 / sqrt(fabs(1.0 - u2 * stm2))) : 0.f;
+  const double u2 = rho2 / (radius * radius);
+  const float G = u2 < 1.0 ? (float)rsqrt_f64(fabs(1.0 - u2 * stm2)) : 0.f;
   const float w = sqrtf(fabsf(ct)) * G;
   const float R[3][3] = {{ct * cp * cp + sp * sp, ct * cp * sp - sp * cp, -cp * st},
                          {sp * ct * cp - cp * sp, ct * sp * sp + cp * cp, -sp * st},

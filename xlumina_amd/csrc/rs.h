@@ -81,11 +81,24 @@ __device__ __forceinline__ bool lag_of(const double* g, int s, int n, int P, dou
   return false;
 }
 
+// 1/sqrt(x) in float64 for x > 0: the hardware estimate (v_rsq_f64) refined by
+// two Newton steps y <- y (3/2 - x y^2 / 2), which take any estimate within
+// 2^-20 to the last bits of float64; the kernel values need r = x / sqrt(x) to
+// ~1e-14 relative, for phases k r of up to ~1e6 rad.  One estimate and six
+// fused multiply-adds replace a correctly rounded square root and a division.
+__device__ __forceinline__ double rsqrt_f64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  y = y * fma(-hx * y, y, 1.5);
+  y = y * fma(-hx * y, y, 1.5);
+  return y;
+}
+
 // h(x, y; z) of wave_optics.py:291-297 with the phase k r reduced in float64.
 __device__ __forceinline__ float2 rs_kernel_value(double X, double Y, double z, double k) {
   const double r2 = X * X + Y * Y + z * z;
-  const double r = sqrt(r2);
-  const double ir = 1.0 / r;
+  const double ir = rsqrt_f64(r2);
+  const double r = r2 * ir;
   const double turns = k * r * 0.15915494309189535;   // k r / (2 pi)
   const float2 e = expi_turns(z > 0 ? turns : -turns);
   const double f = 0.15915494309189535 * z * ir * ir;   // (1/2pi) z / r^2
@@ -97,8 +110,8 @@ __device__ __forceinline__ float2 rs_kernel_value(double X, double Y, double z, 
 //   (1/2pi) e^{i s k r} [ (r^2 - 3z^2)/r^5 + s k^2 z^2/r^3 + i k (s z^2 - (r^2 - 2 z^2))/r^4 ].
 __device__ __forceinline__ float2 rs_kernel_dz_value(double X, double Y, double z, double k) {
   const double r2 = X * X + Y * Y + z * z;
-  const double r = sqrt(r2);
-  const double ir = 1.0 / r, ir2 = ir * ir;
+  const double ir = rsqrt_f64(r2), ir2 = ir * ir;
+  const double r = r2 * ir;
   const double s = z > 0 ? 1.0 : -1.0;
   const double turns = k * r * 0.15915494309189535;
   const float2 e = expi_turns(s * turns);
@@ -113,11 +126,12 @@ __device__ __forceinline__ float2 rs_kernel_dz_value(double X, double Y, double 
 //   (1/2pi)(z/r^2) e^{i s k r} [ s k r + i (s - 1) ].
 __device__ __forceinline__ float2 rs_kernel_dk_value(double X, double Y, double z, double k) {
   const double r2 = X * X + Y * Y + z * z;
-  const double r = sqrt(r2);
+  const double ir = rsqrt_f64(r2);
+  const double r = r2 * ir;
   const double s = z > 0 ? 1.0 : -1.0;
   const double turns = k * r * 0.15915494309189535;
   const float2 e = expi_turns(s * turns);
-  const double f = 0.15915494309189535 * z / r2;
+  const double f = 0.15915494309189535 * z * ir * ir;
   const float fr = (float)(f * s * k * r), fi = (float)(f * (s - 1.0));
   return make_float2(fr * e.x - fi * e.y, fr * e.y + fi * e.x);
 }

@@ -21,6 +21,7 @@
 #include "masks.h"
 #include "mix.h"
 #include "pre.h"
+#include "rng.h"
 #include "rs.h"
 #include "timing.h"
 
@@ -528,21 +529,31 @@ int r32(int v) { return (v + 31) / 32 * 32; }
 // that grows with the split count: a split is charged 0.1 + 0.01 per split
 // against the filled fraction, scaled down when the partials are small
 // (under 32 MiB).
+#ifndef XL_GEMM2_WEIGHT
+#define XL_GEMM2_WEIGHT 0.0
+#endif
+
+// wn = 0 selects cgemm2_kernel (128 x 128 tiles, 2 x 2 MFMA tiles per wave,
+// one block per CU), weighted by XL_GEMM2_WEIGHT against the filled fraction
+// of the cgemm_kernel shapes (0 keeps it off).
 void choose_gemm_shape(int M, int N, int K, int* wn_out, int* splits_out) {
   constexpr int kCUs = 256;
-  const int tm = (M + xl::GBM - 1) / xl::GBM;
   const int ksteps = K / xl::GBK;
   const int smax = std::max(1, std::min(16, ksteps / 4));
   double best = -1.0;
-  for (int wn = 4; wn >= 2; --wn) {
-    const int tn = (N + 32 * wn - 1) / (32 * wn);
+  for (int wn = 4; wn >= 0; --wn) {
+    if (wn == 1 || (wn == 0 && !(XL_GEMM2_WEIGHT > 0.0))) continue;
+    const int bm = wn ? xl::GBM : xl::G2BM, bnw = wn ? 32 * wn : xl::G2BN;
+    const int tm = (M + bm - 1) / bm;
+    const int tn = (N + bnw - 1) / bnw;
     const int slots = kCUs * (wn == 2 ? 2 : 1);
     for (int sp = 1; sp <= smax; ++sp) {
       const int64_t blocks = (int64_t)tm * tn * sp;
       const int64_t rounds = (blocks + slots - 1) / slots;
       const double part = (double)sp * M * N * sizeof(float2);
       const double charge = sp > 1 ? (0.1 + 0.01 * sp) * std::min(1.0, part / (32.0 * 1048576.0)) : 0.0;
-      const double score = (double)blocks / (double)(rounds * slots) - charge;
+      double score = (double)blocks / (double)(rounds * slots) - charge;
+      if (wn == 0) score *= XL_GEMM2_WEIGHT;
       if (score > best + 1e-9) {
         best = score;
         *wn_out = wn;
@@ -590,6 +601,13 @@ size_t gemm_ws(const GemmPlan& g) {
 // Launches C = A B^T with the stage's tile width and split-K factor (then the
 // partial reduction when split).
 void launch_cgemm(xl::GemmArgs& a, int wn, int splits, hipStream_t st) {
+  if (wn == 0) {
+    const dim3 grid((a.N + xl::G2BN - 1) / xl::G2BN, (a.M + xl::G2BM - 1) / xl::G2BM, splits);
+    hipLaunchKernelGGL(xl::cgemm2_kernel, grid, dim3(256), 0, st, a);
+    if (splits > 1)
+      hipLaunchKernelGGL(xl::cgemm_reduce_kernel, grid_for((int64_t)a.M * a.N), dim3(256), 0, st, a, splits);
+    return;
+  }
   const dim3 grid((a.N + 32 * wn - 1) / (32 * wn), (a.M + xl::GBM - 1) / xl::GBM, splits);
   if (wn == 4) hipLaunchKernelGGL(xl::cgemm_kernel<4>, grid, dim3(1024), 0, st, a);
   else if (wn == 3) hipLaunchKernelGGL(xl::cgemm_kernel<3>, grid, dim3(768), 0, st, a);
@@ -1149,6 +1167,22 @@ int xl_bs_dots(const xl_bs_dots_params* p, void* ws, size_t ws_bytes, void* stre
   hipLaunchKernelGGL(xl::bs_dots_kernel, dim3(xl::kBsDotBlocks), dim3(256), 0, st, a);
   hipLaunchKernelGGL(xl::bs_dots_reduce_kernel, dim3(1), dim3(256), 0, st, a, xl::kBsDotBlocks);
   return hip_check(hipGetLastError(), "beam-splitter dot launch");
+}
+
+int xl_uniform_phases(float* out, const uint64_t* seeds, const int32_t* enable, int64_t count, int32_t batch,
+                      int64_t batch_stride, void* stream) {
+  if (!out || !seeds) return fail(XL_EINVAL, "null pointer argument");
+  if (count < 1 || batch < 1 || batch > 65535) return fail(XL_EINVAL, "bad phase-mask count / batch");
+  if (batch > 1 && batch_stride < count)
+    return fail(XL_EINVAL, "batch_stride must be >= count (members write disjoint masks)");
+  xl::PhaseArgs a;
+  a.out = out;
+  a.seeds = seeds;
+  a.enable = enable;
+  a.count = count;
+  a.bs = batch > 1 ? batch_stride : 0;
+  hipLaunchKernelGGL(xl::uniform_phase_kernel, grid_for(count, batch), dim3(256), 0, (hipStream_t)stream, a);
+  return hip_check(hipGetLastError(), "phase launch");
 }
 
 int xl_transfer_function_rs(void* out, const double* x, const double* y, int32_t nx, int32_t ny, double z, double k,

@@ -715,6 +715,27 @@ def pre_backward(src: torch.Tensor, gout: torch.Tensor, x, y, z: float = 0.0, ma
     return gsrc, gp
 
 
+def uniform_phases(out: torch.Tensor, seeds: torch.Tensor, enable: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Seeded uniform phase masks in [-pi, pi) (xl_uniform_phases): out is a
+    (B, ...) float32 device tensor, member b filled from seeds[b] (a (B,)
+    int64 device tensor) where enable[b] != 0 (a (B,) int32 device tensor, or
+    None for every member).  Counter-based, so a captured graph regenerates
+    new masks from new seeds."""
+    lib = N.load()
+    if out.dtype != torch.float32 or out.device.type != "cuda" or out.dim() < 2 or not out.is_contiguous():
+        raise ValueError("out must be a contiguous float32 (B, ...) device tensor")
+    dev = out.device
+    N.ensure_device(dev)
+    B = out.shape[0]
+    for t, dt, name in ((seeds, torch.int64, "seeds"), (enable, torch.int32, "enable")):
+        if t is not None and (t.dtype != dt or t.device != dev or t.numel() != B or not t.is_contiguous()):
+            raise ValueError(f"{name} must be a contiguous {dt} tensor of {B} values on the output's device")
+    count = out[0].numel()
+    N.check(lib.xl_uniform_phases(out.data_ptr(), seeds.data_ptr(), _ptr(enable) or None, count, B, count,
+                                  _stream(dev)))
+    return out
+
+
 def transfer_function_rs(x, y, z: float, wavelength: float, device) -> torch.Tensor:
     """transfer_function_RS (wave_optics.py:291-297) on meshgrid(x, y)."""
     lib = N.load()

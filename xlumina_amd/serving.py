@@ -67,8 +67,6 @@ from . import ops
 from .ops import MaskSpec
 from .toolbox import space
 
-TWO_PI = 2.0 * math.pi
-
 OK, NONFINITE, REJECTED, CANCELLED = "ok", "nonfinite", "rejected", "cancelled"
 
 
@@ -116,23 +114,28 @@ def make_requests(count: int, rate: Optional[float] = None, seed: int = 7,
             for i in range(count)]
 
 
+def _client_phases(r: Request) -> np.ndarray:
+    nin = 2 if r.vectorial else 1
+    ph = np.asarray(r.phases, dtype=np.float32)
+    if ph.shape != (nin, r.n, r.n):
+        raise ValueError(f"request {r.rid}: phases must be {(nin, r.n, r.n)}, got {ph.shape}")
+    return ph
+
+
 def request_phases(r: Request, device, out: Optional[torch.Tensor] = None) -> torch.Tensor:
     """The request's phase masks: the client's (`r.phases`), else U[-pi, pi)
-    from a generator seeded with r.seed on `device`; (1, n, n) for a scalar
-    request (SLM), (2, n, n) for a vectorial one (sSLM alpha, phi).  Written
-    into `out` when given."""
+    generated on the device from r.seed (ops.uniform_phases, the generator the
+    slots run inside their graphs); (1, n, n) for a scalar request (SLM),
+    (2, n, n) for a vectorial one (sSLM alpha, phi).  Written into `out` when
+    given."""
     nin = 2 if r.vectorial else 1
     if out is None:
         out = torch.empty((nin, r.n, r.n), dtype=torch.float32, device=device)
     if r.phases is not None:
-        ph = np.asarray(r.phases, dtype=np.float32)
-        if ph.shape != (nin, r.n, r.n):
-            raise ValueError(f"request {r.rid}: phases must be {(nin, r.n, r.n)}, got {ph.shape}")
-        return out.copy_(torch.from_numpy(ph))
-    gen = torch.Generator(device=device)
-    gen.manual_seed(r.seed)
-    torch.rand(out.shape, generator=gen, device=device, out=out)
-    return out.mul_(TWO_PI).sub_(math.pi)
+        return out.copy_(torch.from_numpy(_client_phases(r)))
+    seeds = torch.tensor([r.seed], dtype=torch.int64, device=device)
+    ops.uniform_phases(out.view(1, -1), seeds)
+    return out
 
 
 @dataclass
@@ -170,8 +173,15 @@ class _Slot:
         self.src = sh.beam_v if b.vectorial else sh.beam_s
         B = capacity
         self.params = torch.zeros((B, self.nin, n, n), dtype=torch.float32, device=dev)
-        self.zd = torch.full((B,), 1e4, dtype=torch.float64, device=dev)
-        self.zd_host = torch.empty((B,), dtype=torch.float64).pin_memory()
+        # per-member control words, one host-to-device copy per batch: distance (float64
+        # bits), phase seed, and whether the graph generates the member's phases
+        self.ctl_host = torch.zeros((3, B), dtype=torch.int64).pin_memory()
+        self.ctl_host[0].view(torch.float64).fill_(1e4)
+        self.ctl_np = self.ctl_host.numpy()                 # host writes through numpy (no per-element torch calls)
+        self.ctl = self.ctl_host.to(dev)
+        self.zd = self.ctl[0].view(torch.float64)
+        self.seeds = self.ctl[1]
+        self.gen_flags = self.ctl[2].view(torch.int32)[:B]   # int32 flags in the first half of row 2
         self.ok = torch.ones((B,), dtype=torch.bool, device=dev)
         self.ok_host = torch.ones((B,), dtype=torch.bool).pin_memory()
         self.out = torch.empty((B, self.ncomp, n, n), dtype=torch.complex64, device=dev)
@@ -213,10 +223,12 @@ class _Slot:
         return [MaskSpec(N.XL_MASK_SLM, p[:, 0])], N.XL_EZ_NONE
 
     def program(self, k: int):
-        """The bucket's kernel sequence over the first k members of this slot."""
+        """The bucket's kernel sequence over the first k members of this slot:
+        seeded phase masks, propagation (+ adjoint and mask gradient)."""
         sh = self.sh
         masks, ez = self._masks(k)
         grad = self.bucket.grad
+        ops.uniform_phases(self.params[:k], self.seeds[:k], self.gen_flags[:k])
         ops.rs_propagate_batched(self.src, sh.x, sh.y, self.wavelength, self.zd[:k], masks=masks, ez_mode=ez,
                                  nin=self.nin, intensity=True, batch=k, out=self.out[:k], inten=self.inten[:k],
                                  workspace=self.ws, hhat=self.hhat, hhat_mode=1 if grad else 0)
@@ -233,18 +245,24 @@ class _Slot:
         runs the batch on the slot stream; returns the kept results (or None)."""
         k = len(reqs)
         self.requests = list(reqs)
+        ctl = self.ctl_np
+        zh, flags = ctl[0].view(np.float64), ctl[2].view(np.int32)
         with torch.cuda.stream(self.stream):
             for i, r in enumerate(reqs):
-                request_phases(r, self.params.device, out=self.params[i])
-                self.zd_host[i] = r.z
-            self.zd[:k].copy_(self.zd_host[:k], non_blocking=True)
+                zh[i] = r.z
+                ctl[1, i] = r.seed
+                flags[i] = 0 if r.phases is not None else 1
+                if r.phases is not None:
+                    self.params[i].copy_(torch.from_numpy(_client_phases(r)))
+            self.ctl.copy_(self.ctl_host, non_blocking=True)
             if k in self.graphs:
                 self.graphs[k].replay()
             else:
                 self.program(k)
-            fin = torch.isfinite(self.inten[:k]).flatten(1).all(1)
-            if self.gmask is not None:
-                fin &= torch.isfinite(self.gmask[:k]).flatten(1).all(1)
+            # a non-finite input (phase or distance) reaches the member's intensity, and
+            # max propagates NaN.  Two stages (k x 256 partial maxima, then k): a
+            # one-stage reduction runs one block per member over n^2 values
+            fin = torch.isfinite(self.inten[:k].view(k, 256, -1).amax(2).amax(1))
             self.ok_host[:k].copy_(fin, non_blocking=True)
             results = None
             if keep:
@@ -396,11 +414,17 @@ class Server:
         return s
 
     def prewarm(self, buckets) -> None:
-        """Creates (and captures) one slot per admissible bucket while the
-        budget allows, so that few requests pay for a capture."""
-        for b in sorted(buckets, key=lambda b: (b.n, b.vectorial, b.grad)):
-            if self.admissible(b) and not self.pools.get(b):
+        """Creates (and captures) the slots of the admissible buckets while
+        the budget allows -- first one per bucket, then the rest -- so that no
+        request pays for a capture unless the budget forces slots to be
+        released and rebuilt."""
+        order = sorted((b for b in buckets if self.admissible(b)), key=lambda b: (b.n, b.vectorial, b.grad))
+        for b in order:
+            if not self.pools.get(b):
                 self._grow(b, {}, evict=False)
+        for b in order:
+            while self._grow(b, {}, evict=False) is not None:
+                pass
         if not self.fake:
             torch.cuda.synchronize(self.device)
 
