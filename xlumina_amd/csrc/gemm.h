@@ -359,120 +359,6 @@ __global__ __launch_bounds__(256 * WN) void cgemm_kernel(GemmArgs a) {
   }
 }
 
-// Block tile 128 x 128 complex on 4 waves (2 x 2), each wave 64 x 64 as 2 x 2
-// MFMA tiles of 32 x 32: every A fragment read from LDS feeds two tiles and so
-// does every B fragment, so a K step reads half the LDS bytes per MFMA of
-// cgemm_kernel.  Same operands, split-K, epilogue and accumulation order per
-// output element (hi*hi, hi*lo, lo*hi terms per K slice) as cgemm_kernel.
-constexpr int G2BM = 128, G2BN = 128;
-__global__ __launch_bounds__(256) void cgemm2_kernel(GemmArgs a) {
-  constexpr int NT = 256;
-  constexpr int CH_A = 4 * G2BM * (GBK / 8);
-  constexpr int CH = CH_A + 4 * G2BN * (GBK / 8);
-  constexpr int PER = CH / NT;
-  static_assert(CH % NT == 0, "whole chunks per thread");
-  __shared__ __attribute__((aligned(16))) uint16_t smA[4][G2BM][GLDK];
-  __shared__ __attribute__((aligned(16))) uint16_t smB[4][G2BN][GLDK];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int wm = w & 1, wn = w >> 1;
-  const int bm = blockIdx.y * G2BM, bn = blockIdx.x * G2BN;
-  const int ksteps = a.K / GBK;
-  const int kbeg = (int)((int64_t)ksteps * blockIdx.z / gridDim.z);
-  const int kend = (int)((int64_t)ksteps * (blockIdx.z + 1) / gridDim.z);
-  uint4 pre[PER];
-  auto gload = [&](int k0) {
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int ch = tid + i * NT;
-      const bool isA = ch < CH_A;
-      const int c2 = isA ? ch : ch - CH_A;
-      const int rows = isA ? G2BM : G2BN;
-      const int pl = c2 / (4 * rows), r = (c2 >> 2) % rows, kc = c2 & 3;
-      const GemmOperand& o = isA ? a.A : a.B;
-      const int gr = (isA ? bm : bn) + r;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (gr < o.rows) v = *reinterpret_cast<const uint4*>(o.p + pl * o.ps + gop_off(o, gr) + k0 + kc * 8);
-      pre[i] = v;
-    }
-  };
-  auto sstore = [&]() {
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int ch = tid + i * NT;
-      if (ch < CH_A) {
-        const int pl = ch / (4 * G2BM), r = (ch >> 2) % G2BM, kc = ch & 3;
-        *reinterpret_cast<uint4*>(&smA[pl][r][kc * 8]) = pre[i];
-      } else {
-        const int c2 = ch - CH_A;
-        const int pl = c2 / (4 * G2BN), r = (c2 >> 2) % G2BN, kc = c2 & 3;
-        *reinterpret_cast<uint4*>(&smB[pl][r][kc * 8]) = pre[i];
-      }
-    }
-  };
-  f32x16 crr[2][2] = {}, cii[2][2] = {}, cri[2][2] = {};
-  gload(kbeg * GBK);
-  for (int ks = kbeg; ks < kend; ++ks) {
-    __syncthreads();
-    sstore();
-    __syncthreads();
-    if (ks + 1 < kend) gload((ks + 1) * GBK);
-#pragma unroll
-    for (int kk = 0; kk < GBK; kk += 16) {
-      const int kq = kk + 8 * (lane >> 5);
-      bf16x8 ah[2][2], al[2][2], bh[2][2], bl[2][2];   // [tile][re, im]
-#pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int ra = wm * 64 + t * 32 + (lane & 31), rb = wn * 64 + t * 32 + (lane & 31);
-        ah[t][0] = *reinterpret_cast<const bf16x8*>(&smA[0][ra][kq]);
-        al[t][0] = *reinterpret_cast<const bf16x8*>(&smA[1][ra][kq]);
-        ah[t][1] = *reinterpret_cast<const bf16x8*>(&smA[2][ra][kq]);
-        al[t][1] = *reinterpret_cast<const bf16x8*>(&smA[3][ra][kq]);
-        bh[t][0] = *reinterpret_cast<const bf16x8*>(&smB[0][rb][kq]);
-        bl[t][0] = *reinterpret_cast<const bf16x8*>(&smB[1][rb][kq]);
-        bh[t][1] = *reinterpret_cast<const bf16x8*>(&smB[2][rb][kq]);
-        bl[t][1] = *reinterpret_cast<const bf16x8*>(&smB[3][rb][kq]);
-      }
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          f32x16& rr = crr[i][j];
-          f32x16& ii = cii[i][j];
-          f32x16& ri = cri[i][j];
-          rr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i][0], bh[j][0], rr, 0, 0, 0);
-          rr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i][0], bl[j][0], rr, 0, 0, 0);
-          rr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i][0], bh[j][0], rr, 0, 0, 0);
-          ii = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i][1], bh[j][1], ii, 0, 0, 0);
-          ii = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i][1], bl[j][1], ii, 0, 0, 0);
-          ii = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i][1], bh[j][1], ii, 0, 0, 0);
-          ri = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i][0], bh[j][1], ri, 0, 0, 0);
-          ri = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i][0], bl[j][1], ri, 0, 0, 0);
-          ri = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i][1], bh[j][0], ri, 0, 0, 0);
-          ri = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i][1], bl[j][0], ri, 0, 0, 0);
-          ri = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i][0], bh[j][1], ri, 0, 0, 0);
-          ri = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i][1], bh[j][0], ri, 0, 0, 0);
-        }
-    }
-  }
-  const bool split = gridDim.z > 1;
-  float2* part = a.partial + (int64_t)blockIdx.z * a.M * a.N;
-#pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    const int n = bn + wn * 64 + j * 32 + (lane & 31);
-    if (n >= a.N) continue;
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int m = bm + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        if (m >= a.M) continue;
-        const float2 v = make_float2(crr[i][j][r] - cii[i][j][r], cri[i][j][r]);
-        if (split) part[(int64_t)m * a.N + n] = v;
-        else gemm_store(a, m, n, v);
-      }
-  }
-}
-
 // Sums the split-K partials in a fixed order and writes the output.
 __global__ void cgemm_reduce_kernel(GemmArgs a, int splits) {
   const int64_t total = (int64_t)a.M * a.N;

@@ -529,31 +529,21 @@ int r32(int v) { return (v + 31) / 32 * 32; }
 // that grows with the split count: a split is charged 0.1 + 0.01 per split
 // against the filled fraction, scaled down when the partials are small
 // (under 32 MiB).
-#ifndef XL_GEMM2_WEIGHT
-#define XL_GEMM2_WEIGHT 0.0
-#endif
-
-// wn = 0 selects cgemm2_kernel (128 x 128 tiles, 2 x 2 MFMA tiles per wave,
-// one block per CU), weighted by XL_GEMM2_WEIGHT against the filled fraction
-// of the cgemm_kernel shapes (0 keeps it off).
 void choose_gemm_shape(int M, int N, int K, int* wn_out, int* splits_out) {
   constexpr int kCUs = 256;
+  const int tm = (M + xl::GBM - 1) / xl::GBM;
   const int ksteps = K / xl::GBK;
   const int smax = std::max(1, std::min(16, ksteps / 4));
   double best = -1.0;
-  for (int wn = 4; wn >= 0; --wn) {
-    if (wn == 1 || (wn == 0 && !(XL_GEMM2_WEIGHT > 0.0))) continue;
-    const int bm = wn ? xl::GBM : xl::G2BM, bnw = wn ? 32 * wn : xl::G2BN;
-    const int tm = (M + bm - 1) / bm;
-    const int tn = (N + bnw - 1) / bnw;
+  for (int wn = 4; wn >= 2; --wn) {
+    const int tn = (N + 32 * wn - 1) / (32 * wn);
     const int slots = kCUs * (wn == 2 ? 2 : 1);
     for (int sp = 1; sp <= smax; ++sp) {
       const int64_t blocks = (int64_t)tm * tn * sp;
       const int64_t rounds = (blocks + slots - 1) / slots;
       const double part = (double)sp * M * N * sizeof(float2);
       const double charge = sp > 1 ? (0.1 + 0.01 * sp) * std::min(1.0, part / (32.0 * 1048576.0)) : 0.0;
-      double score = (double)blocks / (double)(rounds * slots) - charge;
-      if (wn == 0) score *= XL_GEMM2_WEIGHT;
+      const double score = (double)blocks / (double)(rounds * slots) - charge;
       if (score > best + 1e-9) {
         best = score;
         *wn_out = wn;
@@ -601,13 +591,6 @@ size_t gemm_ws(const GemmPlan& g) {
 // Launches C = A B^T with the stage's tile width and split-K factor (then the
 // partial reduction when split).
 void launch_cgemm(xl::GemmArgs& a, int wn, int splits, hipStream_t st) {
-  if (wn == 0) {
-    const dim3 grid((a.N + xl::G2BN - 1) / xl::G2BN, (a.M + xl::G2BM - 1) / xl::G2BM, splits);
-    hipLaunchKernelGGL(xl::cgemm2_kernel, grid, dim3(256), 0, st, a);
-    if (splits > 1)
-      hipLaunchKernelGGL(xl::cgemm_reduce_kernel, grid_for((int64_t)a.M * a.N), dim3(256), 0, st, a, splits);
-    return;
-  }
   const dim3 grid((a.N + 32 * wn - 1) / (32 * wn), (a.M + xl::GBM - 1) / xl::GBM, splits);
   if (wn == 4) hipLaunchKernelGGL(xl::cgemm_kernel<4>, grid, dim3(1024), 0, st, a);
   else if (wn == 3) hipLaunchKernelGGL(xl::cgemm_kernel<3>, grid, dim3(768), 0, st, a);
