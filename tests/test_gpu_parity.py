@@ -177,6 +177,31 @@ def test_czt_three_smooth_lengths(n, mx, my, cuda_dev):
     assert abs(lhs - rhs) / abs(lhs) < 1e-5
 
 
+@pytest.mark.parametrize("n,my", [(512, 257), (1024, 513), (2048, 1025)])
+def test_czt_length_equal_to_support(n, my, cuda_dev):
+    """Column axis whose Bluestein length 3 * 2^k equals m + M - 1 exactly (no
+    spare sample on the circle: every lag of the kernel is used once), with a
+    row axis in the zero-half form (input and output in half a power-of-two
+    length); forward against the oracle, adjoint through the inner-product
+    identity."""
+    rng = np.random.default_rng(my)
+    x, y = R.space(1200.0, n)
+    u = cfield(rng, n)
+    xo = np.linspace(-100, 100, 100)
+    yo = np.linspace(-80, 120, my)
+    g = ops.czt_geometry(x, y, xo, yo, 3e4, 0.635)
+    ref = R.czt(u, x, y, 0.635, 3e4, xo, yo)
+    out = ops.czt(T(u, cuda_dev), g, form="bluestein")
+    assert rel_l2(H(out[0]), ref) < TOL
+    # the last output row is the one a layout one sample short would miss
+    assert rel_l2(H(out[0])[-1], ref[-1]) < TOL
+    c = torch.randn((1, my, 100), dtype=torch.complex64, device=cuda_dev)
+    back = ops.czt(c, g, adjoint=True, form="bluestein")
+    lhs = complex(torch.vdot(c.flatten().to(torch.complex128), out.flatten().to(torch.complex128)))
+    rhs = complex(torch.vdot(back.flatten().to(torch.complex128), T(u, cuda_dev).flatten().to(torch.complex128)))
+    assert abs(lhs - rhs) / abs(lhs) < 1e-5
+
+
 def test_czt_rejections(cuda_dev):
     x, y = R.space(100.0, 64)
     with pytest.raises(ValueError, match="power of two"):

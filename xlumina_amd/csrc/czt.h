@@ -5,13 +5,20 @@
 // CZT_jit (:333-357), CZT_for_high_NA_jit (:359-371), VCZT_jit
 // (vectorized_optics.py:375-384), VCZT_objective_lens
 // (optical_elements.py:600-638).  Per axis the reference computes
-//   out_k = post_k * sum_j g[woff + k - j] * pre_j * x_j
-// by a length-Q circular convolution (Q = 2^ceil(log2(m + M - 1))), with
+//   out_k = post_k * sum_j g[m + k - j] * pre_j * x_j,   k < M, j < m,
+// by a length-np2 circular convolution, with
 //   pre_j  = A^{-j} W^{j^2/2},  g_s = W^{-(s-m+1)^2/2} (s < m+M; the reference's
-//   table lacks s = m+M-1 when M >= m), post_k = W^{k^2/2} e^{-i2pi l_k(1/2-m/2)/Dm},
-// woff = m.  The tables are generated on the device with float64 phases
-// (czt_setup_kernel); the FFT of g is stored pre-divided by Q.  The adjoint axis
-// uses pre' = conj(post), post' = conj(pre), g'[d mod Q] = conj(g[m-d]), woff = 0.
+//   table lacks s = m+M-1 when M >= m), post_k = W^{k^2/2} e^{-i2pi l_k(1/2-m/2)/Dm}.
+// That is a Toeplitz product out_k = post_k sum_j gamma(k - j) pre_j x_j with the
+// lag kernel gamma(d) = g[m + d]; the adjoint axis has pre' = conj(post),
+// post' = conj(pre), gamma'(d) = conj(g[m - d]).  Here the kernel is laid out
+// on the circle of length Q >= m + M - 1 by lag (lag d at index d mod Q), so
+// the outputs are the first M samples of the inverse transform whatever the
+// direction.  When the input and the output each fit in half the (power-of-
+// two) length, the first forward pass skips the zero upper half of its
+// inputs and the last inverse pass forms only the lower half of its outputs.
+// The tables are generated on the device with float64 phases
+// (czt_setup_kernel); the FFT of the kernel is stored pre-divided by Q.
 //
 // The 2-D transform is separable (K_y E K_x^T), so the row pass runs along x on
 // the row-major input (contiguous loads) and writes a 4x4-tiled intermediate
@@ -30,7 +37,7 @@ enum { PRE_NONE = 0, PRE_RS = 1, PRE_HNA = 2, PRE_CONJ_RS = 3, PRE_CONST = 4 };
 enum { POST_NONE = 0, POST_RS = 1, POST_CONST = 2, POST_CONJ_RS = 3 };
 
 struct CztAxis {
-  int lin, lout, woff, q;
+  int lin, lout, zl, q;   // zl: zero-half first / last passes (lin, lout <= q/2, q a power of two)
   const float2* pre;   // lin
   const float2* gf;    // q, FFT of the kernel divided by q
   const float2* post;  // lout
@@ -127,14 +134,17 @@ __device__ __forceinline__ void czt_setup_body(const CztSetupArgs& a, float2* s,
     if (!a.adjoint) {
       for (int j = t; j < a.m; j += TPF) a.pre[j] = pre_f(j);
       for (int k = t; k < a.M; k += TPF) a.post[k] = post_f(k);
-      for (int i = t; i < Q; i += TPF) s[lpad(i)] = g_f(i);
     } else {
       for (int k = t; k < a.M; k += TPF) a.pre[k] = conjf2(post_f(k));
       for (int j = t; j < a.m; j += TPF) a.post[j] = conjf2(pre_f(j));
-      for (int i = t; i < Q; i += TPF) {
-        const int d = i < a.m ? i : i - Q;   // lag d in [-(Q-m), m)
-        s[lpad(i)] = (d > -a.M) ? conjf2(g_f(a.m - d)) : make_float2(0.f, 0.f);
-      }
+    }
+    // lag d = k - j in [-(lin-1), lout-1] at index d mod Q (Q >= lin + lout - 1)
+    const int lin = a.adjoint ? a.M : a.m, lout = a.adjoint ? a.m : a.M;
+    for (int i = t; i < Q; i += TPF) {
+      const int d = i < lout ? i : (i > Q - lin ? i - Q : INT_MIN);
+      float2 v = make_float2(0.f, 0.f);
+      if (d != INT_MIN) v = a.adjoint ? conjf2(g_f(a.m - d)) : g_f(a.m + d);
+      s[lpad(i)] = v;
     }
   }
   __syncthreads();
@@ -247,9 +257,18 @@ __global__ void czt_ftab_kernel(CztArgs a, float2* ftab, float2* f0tab) {
 // thread for all components.  The chirp spectrum is applied in registers
 // between the forward and inverse chains and only the output window is
 // stored, into the tiled intermediate (consecutive rows run on one XCD, so
-// the four rows of a tile meet in one L2).
-template <int Q, int K>
-__global__ __launch_bounds__(3 * Q / 16) void czt_rows_kernel(CztArgs a0) {
+// the four rows of a tile meet in one L2).  ZL: the input and the output fit
+// in half the length (zero-half first pass, lower-half last pass).
+template <int Q, bool ZL> struct CztRowsShape {
+  // At 1024 and 2048 the three buffers (26 / 52 KB) leave room for 6 / 3
+  // blocks of 3 / 6 waves per CU, which a SIMD holds at 5 waves (<= 96 VGPRs);
+  // the zero-half forms fit that without spilling.
+  static constexpr int WAVES = (ZL && (Q == 1024 || Q == 2048)) ? 5 : 1;
+};
+
+template <int Q, int K, bool ZL>
+__global__ __launch_bounds__(3 * Q / 16)
+__attribute__((amdgpu_waves_per_eu(CztRowsShape<Q, ZL>::WAVES))) void czt_rows_kernel(CztArgs a0) {
   const CztArgs a = czt_member(a0, blockIdx.y);
   constexpr int TPF = Q / 16, LS = LdsLen<Q>::value;
   using HO = HandOff<Q, TPF>;
@@ -257,9 +276,9 @@ __global__ __launch_bounds__(3 * Q / 16) void czt_rows_kernel(CztArgs a0) {
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int f = tid / TPF, t = tid % TPF;
   const int row = xcd_remap(blockIdx.x, gridDim.x);
-  const int lin = a.ax.lin, lout = a.ax.lout, woff = a.ax.woff;
+  const int lin = a.ax.lin, lout = a.ax.lout;
   const int nc = a.ncomp;
-  for (int col = tid; col < Q; col += nthr) {
+  for (int col = tid; col < (ZL ? Q / 2 : Q); col += nthr) {
     float2 v[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
     if (col < lin) {
       float2 fv = make_float2(1.f, 0.f);
@@ -277,17 +296,16 @@ __global__ __launch_bounds__(3 * Q / 16) void czt_rows_kernel(CztArgs a0) {
   float2* s = lds + f * LS;
   typename HO::Type u;
   NoIO none;
-  fft_forward_to_regs<Q, TPF, SRC_LDS>(s, t, none, u);
+  fft_forward_to_regs<Q, TPF, SRC_LDS, ZL>(s, t, none, u);
 #pragma unroll
   for (int b = 0; b < HO::BPT; ++b)
 #pragma unroll
     for (int q = 0; q < HO::R; ++q) u.v[b][q] = cmul(u.v[b][q], a.ax.gf[HO::index(t, b, q)]);
   float2* dst = a.t + (int64_t)f * a.t_cs;
-  auto store = [&](int idx, float2 v) {
-    const int k = idx - woff;
-    if (k >= 0 && k < lout) dst[czt_tidx(a.rin4, row, k)] = cmul(v, a.ax.post[k]);
+  auto store = [&](int k, float2 v) {
+    if (k < lout) dst[czt_tidx(a.rin4, row, k)] = cmul(v, a.ax.post[k]);
   };
-  fft_inverse_from_regs<Q, TPF, DST_FN>(s, t, u, store);
+  fft_inverse_from_regs<Q, TPF, DST_FN, ZL>(s, t, u, store);
 }
 
 // Per block CPB output columns of the x axis (one per thread group) of one
@@ -295,13 +313,17 @@ __global__ __launch_bounds__(3 * Q / 16) void czt_rows_kernel(CztArgs a0) {
 // first pass, the chirp spectrum applied in registers, and only the output
 // window stored with the output-plane factor.  A 128 B tile line holds four
 // columns: the CPB groups of a block (and at CPB = 2 the neighbouring block,
-// kept on the same XCD) share it.
+// kept on the same XCD) share it.  ZL as in the row kernel.
 template <int Q> struct CztColsShape {
   static constexpr int CPB = Q >= 3072 ? 2 : 4;   // columns per block (more blocks at 3072 and 4096)
+  // At 3072 three blocks (52 KB of LDS, 6 waves each) fit a CU when a SIMD
+  // holds 5 waves, i.e. at most 96 VGPRs; unbounded, the compiler takes 112.
+  static constexpr int WAVES = Q == 3072 ? 5 : 1;
 };
 
-template <int Q>
-__global__ __launch_bounds__(CztColsShape<Q>::CPB * FftThreads<Q>::value) void czt_cols_kernel(CztArgs a0) {
+template <int Q, bool ZL>
+__global__ __launch_bounds__(CztColsShape<Q>::CPB * FftThreads<Q>::value)
+__attribute__((amdgpu_waves_per_eu(CztColsShape<Q>::WAVES))) void czt_cols_kernel(CztArgs a0) {
   const CztArgs a = czt_member(a0, blockIdx.z);
   constexpr int TPF = FftThreads<Q>::value, LS = LdsLen<Q>::value;
   constexpr int CPB = CztColsShape<Q>::CPB;
@@ -314,50 +336,66 @@ __global__ __launch_bounds__(CztColsShape<Q>::CPB * FftThreads<Q>::value) void c
   const int lout_x = a.ax.lout;
   const int kx = og * CPB + f;
   const bool colok = kx < lout_x;
-  const int lin = a.ay.lin, lout = a.ay.lout, woff = a.ay.woff;
+  const int lin = a.ay.lin, lout = a.ay.lout;
   const float2* src = a.t + (int64_t)c * a.t_cs;
   auto load = [&](int r) -> float2 {
     return (colok && r < lin) ? cmul(src[czt_tidx(a.rin4, r, kx)], a.ay.pre[r]) : make_float2(0.f, 0.f);
   };
   typename HO::Type u;
-  fft_forward_to_regs<Q, TPF, SRC_FN>(s, t, load, u);
+  fft_forward_to_regs<Q, TPF, SRC_FN, ZL>(s, t, load, u);
 #pragma unroll
   for (int b = 0; b < HO::BPT; ++b)
 #pragma unroll
     for (int q = 0; q < HO::R; ++q) u.v[b][q] = cmul(u.v[b][q], a.ay.gf[HO::index(t, b, q)]);
   float2* oc = a.out + (int64_t)c * a.out_cs;
-  auto store = [&](int idx, float2 v) {
-    const int ky = idx - woff;
-    if (colok && ky >= 0 && ky < lout) {
+  auto store = [&](int ky, float2 v) {
+    if (colok && ky < lout) {
       v = cmul(v, a.ay.post[ky]);
       if (a.f0tab) v = c_mul(v, a.f0tab[(int64_t)ky * lout_x + kx]);
       else if (a.post_mode != POST_NONE) v = c_mul(v, czt_post_factor(a, ky, kx));
       oc[(int64_t)ky * lout_x + kx] = v;
     }
   };
-  fft_inverse_from_regs<Q, TPF, DST_FN>(s, t, u, store);
+  fft_inverse_from_regs<Q, TPF, DST_FN, ZL>(s, t, u, store);
 }
 
-template <int Q>
-inline void czt_rows_launch(const CztArgs& a, int batch, hipStream_t st) {
-  KTimer t(K_CZT_ROWS, st);
+// Zero-half forms exist for the power-of-two lengths.
+template <int Q> struct CztZl { static constexpr bool value = (Q & (Q - 1)) == 0; };
+
+template <int Q, bool ZL>
+inline void czt_rows_launch_zl(const CztArgs& a, int batch, hipStream_t st) {
   const dim3 g(a.rin, batch), b(a.ncomp * Q / 16);
   const size_t lds = (size_t)a.ncomp * LdsLen<Q>::value * sizeof(float2);
   if (a.pre_mode == PRE_RS || a.pre_mode == PRE_CONJ_RS)
-    hipLaunchKernelGGL((czt_rows_kernel<Q, PREK_F>), g, b, lds, st, a);
+    hipLaunchKernelGGL((czt_rows_kernel<Q, PREK_F, ZL>), g, b, lds, st, a);
   else if (a.pre_mode == PRE_HNA)
-    hipLaunchKernelGGL((czt_rows_kernel<Q, PREK_HNA>), g, b, lds, st, a);
+    hipLaunchKernelGGL((czt_rows_kernel<Q, PREK_HNA, ZL>), g, b, lds, st, a);
   else
-    hipLaunchKernelGGL((czt_rows_kernel<Q, PREK_SIMPLE>), g, b, lds, st, a);
+    hipLaunchKernelGGL((czt_rows_kernel<Q, PREK_SIMPLE, ZL>), g, b, lds, st, a);
+}
+template <int Q>
+inline void czt_rows_launch(const CztArgs& a, int batch, hipStream_t st) {
+  KTimer t(K_CZT_ROWS, st);
+  if constexpr (CztZl<Q>::value)
+    if (a.ax.zl) return czt_rows_launch_zl<Q, true>(a, batch, st);
+  czt_rows_launch_zl<Q, false>(a, batch, st);
 }
 // Dynamic-LDS limit of the row kernel (three Q-point buffers), set once per
 // device by the library's initialisation for every Bluestein length.
-template <int Q> inline hipError_t czt_set_attribute() {
+template <int Q, bool ZL> inline hipError_t czt_set_attribute_zl() {
   const int mx = 3 * LdsLen<Q>::value * (int)sizeof(float2);
   hipError_t e = hipSuccess;
-  for (const void* f : {(const void*)czt_rows_kernel<Q, PREK_F>, (const void*)czt_rows_kernel<Q, PREK_HNA>,
-                        (const void*)czt_rows_kernel<Q, PREK_SIMPLE>}) {
+  for (const void* f : {(const void*)czt_rows_kernel<Q, PREK_F, ZL>, (const void*)czt_rows_kernel<Q, PREK_HNA, ZL>,
+                        (const void*)czt_rows_kernel<Q, PREK_SIMPLE, ZL>}) {
     const hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+    if (r != hipSuccess) e = r;
+  }
+  return e;
+}
+template <int Q> inline hipError_t czt_set_attribute() {
+  hipError_t e = czt_set_attribute_zl<Q, false>();
+  if constexpr (CztZl<Q>::value) {
+    const hipError_t r = czt_set_attribute_zl<Q, true>();
     if (r != hipSuccess) e = r;
   }
   return e;
@@ -376,8 +414,13 @@ template <int Q>
 inline void czt_cols_launch(const CztArgs& a, int batch, hipStream_t st) {
   KTimer t(K_CZT_COLS, st);
   constexpr int CPB = CztColsShape<Q>::CPB;
-  hipLaunchKernelGGL(czt_cols_kernel<Q>, dim3((a.ax.lout + CPB - 1) / CPB, a.ncomp, batch),
-                     dim3(CPB * FftThreads<Q>::value), 0, st, a);
+  const dim3 g((a.ax.lout + CPB - 1) / CPB, a.ncomp, batch), b(CPB * FftThreads<Q>::value);
+  if constexpr (CztZl<Q>::value)
+    if (a.ay.zl) {
+      hipLaunchKernelGGL((czt_cols_kernel<Q, true>), g, b, 0, st, a);
+      return;
+    }
+  hipLaunchKernelGGL((czt_cols_kernel<Q, false>), g, b, 0, st, a);
 }
 template <int Q>
 inline void czt_setup_launch(const CztSetupPair& p, int nblocks, hipStream_t st) {

@@ -436,7 +436,8 @@ int xl_czt(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream) {
     ax.post = s.post;
     ax.lin = p->adjoint ? ap.M : ap.m;
     ax.lout = p->adjoint ? ap.m : ap.M;
-    ax.woff = p->adjoint ? 0 : ap.m;
+    // zero-half passes: power-of-two length with the input and the output in its lower half
+    ax.zl = (ap.q & (ap.q - 1)) == 0 && 2 * std::max(ap.m, ap.M) <= ap.q;
   }
   if (p->tables_mode < 0 || p->tables_mode > 2) return fail(XL_EINVAL, "tables_mode must be 0, 1 or 2");
   if (p->tables_mode != 2) {
