@@ -309,16 +309,23 @@ __attribute__((amdgpu_waves_per_eu(CztRowsShape<Q, ZL>::WAVES))) void czt_rows_k
 }
 
 // Per block CPB output columns of the x axis (one per thread group) of one
-// component: the column of the tiled intermediate is loaded straight into the
-// first pass, the chirp spectrum applied in registers, and only the output
-// window stored with the output-plane factor.  A 128 B tile line holds four
-// columns: the CPB groups of a block (and at CPB = 2 the neighbouring block,
-// kept on the same XCD) share it.  ZL as in the row kernel.
+// component: the columns of the tiled intermediate feed the first pass, the
+// chirp spectrum is applied in registers, and only the output window is
+// stored with the output-plane factor.  A 128 B tile line holds four columns:
+// the CPB groups of a block (and at CPB = 2 the neighbouring block, kept on
+// the same XCD) share it.  Column-wise, each lane's 8 B access is a line of
+// its own (64 lines per wave for the output rows); staged through LDS, a wave
+// stores CPB-column row segments and loads whole tile lines.  ZL as in the
+// row kernel.
 template <int Q> struct CztColsShape {
   static constexpr int CPB = Q >= 3072 ? 2 : 4;   // columns per block (more blocks at 3072 and 4096)
   // At 3072 three blocks (52 KB of LDS, 6 waves each) fit a CU when a SIMD
   // holds 5 waves, i.e. at most 96 VGPRs; unbounded, the compiler takes 112.
   static constexpr int WAVES = Q == 3072 ? 5 : 1;
+  // Power-of-two lengths stage the tile loads and the output rows through
+  // LDS; the 3 * 2^k lengths read and write columns directly (staged, 3072
+  // no longer fits the 5-wave bound and measured slower).
+  static constexpr bool STAGE = (Q & (Q - 1)) == 0;
 };
 
 template <int Q, bool ZL>
@@ -338,25 +345,70 @@ __attribute__((amdgpu_waves_per_eu(CztColsShape<Q>::WAVES))) void czt_cols_kerne
   const bool colok = kx < lout_x;
   const int lin = a.ay.lin, lout = a.ay.lout;
   const float2* src = a.t + (int64_t)c * a.t_cs;
-  auto load = [&](int r) -> float2 {
-    return (colok && r < lin) ? cmul(src[czt_tidx(a.rin4, r, kx)], a.ay.pre[r]) : make_float2(0.f, 0.f);
-  };
   typename HO::Type u;
-  fft_forward_to_regs<Q, TPF, SRC_FN, ZL>(s, t, load, u);
+  NoIO none;
+  constexpr int NT = CPB * TPF;
+  const int kx0 = og * CPB;
+  if constexpr (CztColsShape<Q>::STAGE) {
+    // The block's CPB columns lie in one 4-column tile group: they are staged
+    // into LDS with lane-contiguous loads (column fastest) and the first pass
+    // reads them from there.
+    for (int e = threadIdx.x; e < (ZL ? Q / 2 : Q) * CPB; e += NT) {
+      const int r = e / CPB, g = e % CPB;
+      float2 v = make_float2(0.f, 0.f);
+      if (r < lin && kx0 + g < lout_x) v = cmul(src[czt_tidx(a.rin4, r, kx0 + g)], a.ay.pre[r]);
+      lds[g * LS + lpad(r)] = v;
+    }
+    __syncthreads();
+    fft_forward_to_regs<Q, TPF, SRC_LDS, ZL>(s, t, none, u);
+  } else {
+    auto load = [&](int r) -> float2 {
+      return (colok && r < lin) ? cmul(src[czt_tidx(a.rin4, r, kx)], a.ay.pre[r]) : make_float2(0.f, 0.f);
+    };
+    fft_forward_to_regs<Q, TPF, SRC_FN, ZL>(s, t, load, u);
+  }
 #pragma unroll
   for (int b = 0; b < HO::BPT; ++b)
 #pragma unroll
     for (int q = 0; q < HO::R; ++q) u.v[b][q] = cmul(u.v[b][q], a.ay.gf[HO::index(t, b, q)]);
   float2* oc = a.out + (int64_t)c * a.out_cs;
-  auto store = [&](int ky, float2 v) {
-    if (colok && ky < lout) {
-      v = cmul(v, a.ay.post[ky]);
-      if (a.f0tab) v = c_mul(v, a.f0tab[(int64_t)ky * lout_x + kx]);
-      else if (a.post_mode != POST_NONE) v = c_mul(v, czt_post_factor(a, ky, kx));
-      oc[(int64_t)ky * lout_x + kx] = v;
+  if constexpr (CztColsShape<Q>::STAGE) {
+    // Last pass into registers, the outputs transposed through LDS to
+    // [ky][column] and stored row by row (CPB consecutive columns per row).
+    using LP = LastPass<Q, TPF, false>;
+    typename LP::Type o;
+    chain_io<Q, TPF, true, false, 0, SRC_REG, DST_REG, false, ZL>(s, t, none, none, u, o);
+    __syncthreads();   // the last pass of other waves may still read LDS
+#pragma unroll
+    for (int b = 0; b < LP::BPT; ++b) {
+      if (!LP::valid(t, b)) continue;
+#pragma unroll
+      for (int q = 0; q < (ZL ? LP::R / 2 : LP::R); ++q) {
+        const int ky = LP::index(t, b, q);
+        if (ky < lout) lds[ky * CPB + f] = cmul(o.v[b][q], a.ay.post[ky]);
+      }
     }
-  };
-  fft_inverse_from_regs<Q, TPF, DST_FN, ZL>(s, t, u, store);
+    __syncthreads();
+    for (int e = threadIdx.x; e < lout * CPB; e += NT) {
+      const int ky = e / CPB, kx2 = kx0 + e % CPB;
+      if (kx2 < lout_x) {
+        float2 v = lds[e];
+        if (a.f0tab) v = c_mul(v, a.f0tab[(int64_t)ky * lout_x + kx2]);
+        else if (a.post_mode != POST_NONE) v = c_mul(v, czt_post_factor(a, ky, kx2));
+        oc[(int64_t)ky * lout_x + kx2] = v;
+      }
+    }
+  } else {
+    auto store = [&](int ky, float2 v) {
+      if (colok && ky < lout) {
+        v = cmul(v, a.ay.post[ky]);
+        if (a.f0tab) v = c_mul(v, a.f0tab[(int64_t)ky * lout_x + kx]);
+        else if (a.post_mode != POST_NONE) v = c_mul(v, czt_post_factor(a, ky, kx));
+        oc[(int64_t)ky * lout_x + kx] = v;
+      }
+    };
+    fft_inverse_from_regs<Q, TPF, DST_FN, ZL>(s, t, u, store);
+  }
 }
 
 // Zero-half forms exist for the power-of-two lengths.

@@ -36,8 +36,9 @@ struct RsArgs {
   int64_t out_cs;
   float* intensity;       // optional N x N, sum_c |out_c|^2
   float2* t;              // tiles, ncomp x NR4*4 x P
-  float2* hr;             // tiles, (P/2) x P: Hr after the row pass, Hhat (ky < P/2) after rs_hcol
-  float2* hmid;           // P: Hhat at ky = P/2
+  float2* hr;             // tiles, (P/2) x (P/2 + 1) of a (P/2) x P array: Hr after the row pass,
+                          // Hhat (ky < P/2) after rs_hcol; columns kx > P/2 mirror (Hhat is even in kx)
+  float2* hmid;           // P/2 + 1: Hhat at ky = P/2
   const double* x;
   const double* y;
   double z, k;
@@ -151,7 +152,8 @@ template <int P> struct Units {
 };
 
 // One kernel row per group: H of rows 0..P/2-1 generated in float64 phase
-// (the array is even in x: columns above P/2 mirror), row FFT, stored as Hr.
+// (the array is even in x: columns above P/2 mirror), row FFT, stored as Hr
+// for kx <= P/2 (the transform of an even row is even).
 template <int P>
 __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_hrow_kernel(RsArgs a0) {
   const RsArgs a = rs_member(a0, blockIdx.y);
@@ -175,7 +177,9 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_hrow_kernel(Rs
   }
   __syncthreads();
   NoIO none;
-  auto store = [&](int c, float2 v) { a.hr[hr_idx(r, c, P)] = v; };
+  auto store = [&](int c, float2 v) {
+    if (c <= P / 2) a.hr[hr_idx(r, c, P)] = v;
+  };
   typename LastPass<P, TPF, true>::Type unused;
   fft_io<P, TPF, false, SRC_LDS, DST_FN>(s, t, none, store, unused);
 }
@@ -269,11 +273,12 @@ __global__ __launch_bounds__(P / 16) void rs_urow1_kernel(RsArgs a0) {
   }
 }
 
-// Kernel spectrum.  Per column: the Hr column (rows mirrored, row P/2 zero)
-// is transformed along y and the spectrum, times dx dy / P^2 (conjugated for
-// the adjoint), replaces the Hr column in place for ky < P/2; ky = P/2 goes
-// to hmid[col].  The array is even in y, so ky > P/2 is not stored
-// (Hhat(P - ky) = Hhat(ky)).
+// Kernel spectrum.  Per column kx <= P/2: the Hr column (rows mirrored, row
+// P/2 zero) is transformed along y and the spectrum, times dx dy / P^2
+// (conjugated for the adjoint), replaces the Hr column in place for ky < P/2;
+// ky = P/2 goes to hmid[col].  The array is even in y and in x, so ky > P/2
+// and kx > P/2 are not formed (Hhat(P - ky, kx) = Hhat(ky, P - kx) = Hhat(ky, kx)).
+// Blocks beyond the P/2 + 1 columns (grid padding for the XCD remap) idle.
 template <int P>
 __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_hcol_kernel(RsArgs a0) {
   const RsArgs a = rs_member(a0, blockIdx.y);
@@ -283,11 +288,13 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_hcol_kernel(Rs
   const int g = threadIdx.x / TPF, t = threadIdx.x % TPF;
   float2* s = lds + g * LS;
   const int col = xcd_remap(blockIdx.x, gridDim.x) * CPB + g;
+  const bool act = col <= P / 2;
+  if (CPB == 1 && !act) return;   // whole block
   const int cg = col >> 2, cc = col & 3;
   float2* hc = a.hr + (int64_t)cg * (P / 8) * 16 + cc;
   auto load = [&](int r) -> float2 {
     const int rr = r <= P / 2 ? r : P - r;
-    return rr == P / 2 ? make_float2(0.f, 0.f) : hc[(rr >> 2) * 16 + (rr & 3) * 4];
+    return (!act || rr == P / 2) ? make_float2(0.f, 0.f) : hc[(rr >> 2) * 16 + (rr & 3) * 4];
   };
   typename HO::Type u;
   fft_forward_to_regs<P, TPF, SRC_FN>(s, t, load, u);
@@ -298,6 +305,7 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_hcol_kernel(Rs
       const int ky = HO::index(t, b, q);
       float2 h = u.v[b][q] * a.scale;
       if (a.adjoint && !a.hstored) h = conjf2(h);
+      if (!act) continue;
       if (ky < P / 2) hc[(ky >> 2) * 16 + (ky & 3) * 4] = h;
       else if (ky == P / 2) a.hmid[col] = h;
     }
@@ -325,7 +333,8 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_col_kernel(RsA
   const int nrg = (a.n + 3) >> 2;
   const int nr = nrg * 4;
   const int cg = col >> 2, cc = col & 3;
-  const float2* hc = a.hr + (int64_t)cg * (P / 8) * 16 + cc;
+  const int hcol = col <= P / 2 ? col : P - col;   // Hhat is even in kx
+  const float2* hc = a.hr + (int64_t)(hcol >> 2) * (P / 8) * 16 + (hcol & 3);
   float2* tc = a.t + (int64_t)comp * nrg * 4 * P + (int64_t)cg * nrg * 16 + cc;
   auto load = [&](int r) -> float2 {
     return (active && r < nr) ? tc[(r >> 2) * 16 + (r & 3) * 4] : make_float2(0.f, 0.f);
@@ -343,7 +352,7 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_col_kernel(RsA
       for (int q = 0; q < HO::R; ++q) {
         const int ky = HO::index(t, b, q);
         const int kk = ky <= P / 2 ? ky : P - ky;
-        float2 h = kk == P / 2 ? a.hmid[col] : hc[(kk >> 2) * 16 + (kk & 3) * 4];
+        float2 h = kk == P / 2 ? a.hmid[hcol] : hc[(kk >> 2) * 16 + (kk & 3) * 4];
         h.y *= hs;
         u.v[b][q] = cmul(u.v[b][q], h);
       }
@@ -500,7 +509,8 @@ inline hipError_t rs_launch(const RsArgs& a, hipStream_t st) {
   }
   if (!a.hreuse) {
     KTimer t(K_RS_HCOL, st);
-    hipLaunchKernelGGL(rs_hcol_kernel<P>, dim3(P / CPB, nb), blk, 0, st, a);
+    const int ncb = (P / 2 + 1 + CPB - 1) / CPB;   // columns kx <= P/2, grid padded to whole XCD rounds
+    hipLaunchKernelGGL(rs_hcol_kernel<P>, dim3((ncb + 7) / 8 * 8, nb), blk, 0, st, a);
   }
   {
     KTimer t(K_RS_COL, st);
