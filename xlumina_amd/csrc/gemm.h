@@ -156,11 +156,8 @@ struct GemmPreArgs {
 __device__ __forceinline__ void gemm_pre_values(const GemmPreArgs& a, int row, int col, float2 (&v)[3]) {
   v[0] = v[1] = v[2] = make_float2(0.f, 0.f);
   if (row >= a.rows || col >= a.cols) return;
-  const int64_t pix = (int64_t)row * a.cols + col;
-  float2 e0 = a.in[pix];
-  float2 e1 = a.nin > 1 ? a.in[a.in_cs + pix] : make_float2(0.f, 0.f);
-  float2 e2 = a.nin > 2 ? a.in[2 * a.in_cs + pix] : make_float2(0.f, 0.f);
-  apply_mask(a.mask, pix, e0, e1, e2);
+  float2 e0, e1, e2;
+  load_masked(a.in, a.in_cs, a.nin, a.mask, (int64_t)row * a.cols + col, e0, e1, e2);
   if (a.pre_mode == 2 /* PRE_HNA */) {
     float A[3][2];
     hna_map(a.px[col], a.py[row], a.hna_radius, a.hna_f, a.hna_stm2, A);
@@ -183,7 +180,8 @@ __device__ __forceinline__ void gemm_pre_values(const GemmPreArgs& a, int row, i
 }
 
 // Two consecutive columns per thread (ld is a multiple of 32), 4-byte plane
-// stores; blocks [0, nblocks) of the caller's grid.
+// stores; blocks [0, nblocks) of the caller's grid.  (Issuing both pixels'
+// loads ahead of their pre-ops measured slower here.)
 __device__ __forceinline__ void gemm_pre_body(const GemmPreArgs& a, int block, int nblocks) {
   const int64_t total2 = (int64_t)a.rowsp * a.dst.ld / 2;
   for (int64_t e2 = block * (int64_t)blockDim.x + threadIdx.x; e2 < total2; e2 += (int64_t)nblocks * blockDim.x) {

@@ -61,20 +61,32 @@ __device__ __forceinline__ void lcd_jones(float eta, float theta, float2& j00, f
   j01 = make_float2(0.f, -2.f * sh * sc);
 }
 
-__device__ __forceinline__ void apply_stage(const MaskStage& m, int64_t pix, float2& e0, float2& e1, float2& e2) {
+// Per-pixel parameters a stage reads (m0 / m1 arrays).
+__device__ __forceinline__ bool stage_m0(const MaskStage& m) {
+  return m.kind == MASK_SLM || m.kind == MASK_SSLM || m.kind == MASK_LCD || m.kind == MASK_LP || m.kind == MASK_AMP;
+}
+__device__ __forceinline__ bool stage_m1(const MaskStage& m) {
+  return m.kind == MASK_SSLM || m.kind == MASK_LCD || (m.kind == MASK_AMP && m.m1);
+}
+
+// One stage; p0() / p1() give the stage's per-pixel parameters (m0[pix],
+// m1[pix]) and are called only by the kinds that have them, so a direct
+// caller loads inside the case and a caller that loaded ahead passes values.
+template <class P0, class P1>
+__device__ __forceinline__ void stage_apply(const MaskStage& m, P0 p0, P1 p1, float2& e0, float2& e1, float2& e2) {
   switch (m.kind) {
     case MASK_SLM:
-      e0 = c_mul(e0, c_expi(m.m0[pix]));
+      e0 = c_mul(e0, c_expi(p0()));
       break;
     case MASK_SSLM:
-      e0 = c_mul(e0, c_expi(m.m0[pix]));
-      e1 = c_mul(e1, c_expi(m.m1[pix]));
+      e0 = c_mul(e0, c_expi(p0()));
+      e1 = c_mul(e1, c_expi(p1()));
       break;
     case MASK_LCD:
     case MASK_LCD_SCALAR: {
       float2 j00, j01, j11;
-      const float eta = m.kind == MASK_LCD ? m.m0[pix] : (float)m.s0;
-      const float th = m.kind == MASK_LCD ? m.m1[pix] : (float)m.s1;
+      const float eta = m.kind == MASK_LCD ? p0() : (float)m.s0;
+      const float th = m.kind == MASK_LCD ? p1() : (float)m.s1;
       lcd_jones(eta, th, j00, j01, j11);
       const float2 a = e0, b = e1;
       e0 = c_add(c_mul(j00, a), c_mul(j01, b));
@@ -83,7 +95,7 @@ __device__ __forceinline__ void apply_stage(const MaskStage& m, int64_t pix, flo
     }
     case MASK_LP: {
       float s, c;
-      sincos_rad(m.m0[pix], &s, &c);
+      sincos_rad(p0(), &s, &c);
       const float2 a = e0, b = e1;
       e0 = c_add(c_scale(a, c * c), c_scale(b, c * s));
       e1 = c_add(c_scale(a, c * s), c_scale(b, s * s));
@@ -91,17 +103,73 @@ __device__ __forceinline__ void apply_stage(const MaskStage& m, int64_t pix, flo
       break;
     }
     case MASK_AMP:
-      e0 = c_scale(e0, m.m0[pix]);
-      if (m.m1) e1 = c_scale(e1, m.m1[pix]);
+      e0 = c_scale(e0, p0());
+      if (m.m1) e1 = c_scale(e1, p1());
       break;
     default:
       break;
   }
 }
 
+__device__ __forceinline__ void apply_stage(const MaskStage& m, int64_t pix, float2& e0, float2& e1, float2& e2) {
+  stage_apply(m, [&] { return m.m0[pix]; }, [&] { return m.m1[pix]; }, e0, e1, e2);
+}
+
+// A stage whose parameters were loaded ahead (v0 = m0[pix], v1 = m1[pix]).
+__device__ __forceinline__ void apply_stage_vals(const MaskStage& m, float v0, float v1, float2& e0, float2& e1,
+                                                 float2& e2) {
+  stage_apply(m, [&] { return v0; }, [&] { return v1; }, e0, e1, e2);
+}
+
 __device__ __forceinline__ void apply_mask(const MaskArgs& m, int64_t pix, float2& e0, float2& e1, float2& e2) {
   apply_stage(m.st[0], pix, e0, e1, e2);
   apply_stage(m.st[1], pix, e0, e1, e2);
+}
+
+// The pre-op inputs of one pixel (field components and per-pixel mask
+// parameters), loaded apart from their use so that the loads of several
+// pixels issue back to back and are in flight together, instead of one
+// pixel's round trip at a time.  Every load is unconditional: an input that
+// is absent is read from the pixel's own first component (the same cache
+// line, no extra memory traffic) and ignored, since a conditional load would
+// need a copy at the branch join that waits for it.
+struct PixRaw {
+  float2 e[3];
+  float v[2][2];   // [stage][m0, m1]
+};
+
+__device__ __forceinline__ void pix_load(const float2* in, int64_t cs, int nin, const MaskArgs& m, int64_t pix,
+                                         PixRaw& r) {
+  const float2* p0 = in + pix;
+  const float* own = reinterpret_cast<const float*>(p0);
+  r.e[0] = *p0;
+  r.e[1] = *(nin > 1 ? p0 + cs : p0);
+  r.e[2] = *(nin > 2 ? p0 + 2 * cs : p0);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    r.v[i][0] = *(stage_m0(m.st[i]) ? m.st[i].m0 + pix : own);
+    r.v[i][1] = *(stage_m1(m.st[i]) ? m.st[i].m1 + pix : own);
+  }
+}
+
+// The masked field of pixel pix, loaded and masked in one go (for loops that
+// gain nothing from batching the loads).
+__device__ __forceinline__ void load_masked(const float2* in, int64_t cs, int nin, const MaskArgs& m, int64_t pix,
+                                            float2& e0, float2& e1, float2& e2) {
+  e0 = in[pix];
+  e1 = nin > 1 ? in[cs + pix] : make_float2(0.f, 0.f);
+  e2 = nin > 2 ? in[2 * cs + pix] : make_float2(0.f, 0.f);
+  apply_mask(m, pix, e0, e1, e2);
+}
+
+// The masked components of a loaded pixel (as apply_mask on the loaded field).
+__device__ __forceinline__ void pix_masked(const MaskArgs& m, int nin, const PixRaw& r, float2& e0, float2& e1,
+                                           float2& e2) {
+  e0 = r.e[0];
+  e1 = nin > 1 ? r.e[1] : make_float2(0.f, 0.f);
+  e2 = nin > 2 ? r.e[2] : make_float2(0.f, 0.f);
+  apply_stage_vals(m.st[0], r.v[0][0], r.v[0][1], e0, e1, e2);
+  apply_stage_vals(m.st[1], r.v[1][0], r.v[1][1], e0, e1, e2);
 }
 
 // The direction cosines X/r, Y/r (and z/r) are formed in float32 with one

@@ -242,18 +242,26 @@ __global__ __launch_bounds__(P / 16) void rs_urow1_kernel(RsArgs a0) {
   const int nrg = (a.n + 3) >> 2, rg = row >> 2, rr = row & 3;
   const int nc = a.ncomp;
   const double z = a.z_dev ? *a.z_dev : a.z;
+  // all loads of the 8 pixels first (in flight together), then the masks and Ez
+  const bool rowok = row < a.n, ez = a.ez_mode != EZ_NONE;
+  PixRaw raw[H];
+  double xc[H];
+#pragma unroll
+  for (int r = 0; r < H; ++r) {
+    const int col = t + r * TPF;
+    const bool ok = rowok && col < a.n;
+    pix_load(a.in, a.in_cs, a.nin, a.mask, ok ? (int64_t)row * a.n + col : 0, raw[r]);
+    xc[r] = ez ? a.x[ok ? col : 0] : 0.0;
+  }
+  const double yr = ez ? a.y[rowok ? row : 0] : 0.0;
   float2 e[3][H];
 #pragma unroll
   for (int r = 0; r < H; ++r) {
     const int col = t + r * TPF;
     float2 e0 = make_float2(0.f, 0.f), e1 = e0, e2 = e0;
-    if (row < a.n && col < a.n) {
-      const int64_t pix = (int64_t)row * a.n + col;
-      e0 = a.in[pix];
-      if (a.nin > 1) e1 = a.in[a.in_cs + pix];
-      if (a.nin > 2) e2 = a.in[2 * a.in_cs + pix];
-      apply_mask(a.mask, pix, e0, e1, e2);
-      if (a.ez_mode != EZ_NONE) e2 = build_ez(a.ez_mode, e0, e1, a.x[col], a.y[row], z);
+    if (rowok && col < a.n) {
+      pix_masked(a.mask, a.nin, raw[r], e0, e1, e2);
+      if (ez) e2 = build_ez(a.ez_mode, e0, e1, xc[r], yr, z);
     }
     e[0][r] = e0;
     e[1][r] = e1;
