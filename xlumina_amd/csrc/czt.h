@@ -107,12 +107,13 @@ struct CztSetupPair {
   CztSetupArgs ax[2];
 };
 
-// Tables of one axis on the first Q/16 threads of the block (GUARD: the block
-// is wider; the other threads only join the FFT's barriers).
-template <int Q, bool GUARD>
+// Tables of one axis on a block of NT threads: the chirps and the kernel
+// (float64 phases, the bulk of the work) on all NT threads, the kernel's FFT
+// on the first Q/16 (the others only join its barriers).
+template <int Q, int NT>
 __device__ __forceinline__ void czt_setup_body(const CztSetupArgs& a, float2* s, int t) {
   constexpr int TPF = Q / 16;
-  const bool act = !GUARD || t < TPF;
+  static_assert(NT >= TPF, "the FFT needs Q/16 threads");
   const double th = a.th_turns;
   const double shift = 0.5 - 0.5 * a.m;
   // forward tables
@@ -130,43 +131,44 @@ __device__ __forceinline__ void czt_setup_body(const CztSetupArgs& a, float2* s,
     const double u = (double)(sidx - a.m + 1);
     return expi_turns(-th * 0.5 * u * u);
   };
-  if (act) {
-    if (!a.adjoint) {
-      for (int j = t; j < a.m; j += TPF) a.pre[j] = pre_f(j);
-      for (int k = t; k < a.M; k += TPF) a.post[k] = post_f(k);
-    } else {
-      for (int k = t; k < a.M; k += TPF) a.pre[k] = conjf2(post_f(k));
-      for (int j = t; j < a.m; j += TPF) a.post[j] = conjf2(pre_f(j));
-    }
-    // lag d = k - j in [-(lin-1), lout-1] at index d mod Q (Q >= lin + lout - 1)
-    const int lin = a.adjoint ? a.M : a.m, lout = a.adjoint ? a.m : a.M;
-    for (int i = t; i < Q; i += TPF) {
-      const int d = i < lout ? i : (i > Q - lin ? i - Q : INT_MIN);
-      float2 v = make_float2(0.f, 0.f);
-      if (d != INT_MIN) v = a.adjoint ? conjf2(g_f(a.m - d)) : g_f(a.m + d);
-      s[lpad(i)] = v;
-    }
+  if (!a.adjoint) {
+    for (int j = t; j < a.m; j += NT) a.pre[j] = pre_f(j);
+    for (int k = t; k < a.M; k += NT) a.post[k] = post_f(k);
+  } else {
+    for (int k = t; k < a.M; k += NT) a.pre[k] = conjf2(post_f(k));
+    for (int j = t; j < a.m; j += NT) a.post[j] = conjf2(pre_f(j));
+  }
+  // lag d = k - j in [-(lin-1), lout-1] at index d mod Q (Q >= lin + lout - 1)
+  const int lin = a.adjoint ? a.M : a.m, lout = a.adjoint ? a.m : a.M;
+  for (int i = t; i < Q; i += NT) {
+    const int d = i < lout ? i : (i > Q - lin ? i - Q : INT_MIN);
+    float2 v = make_float2(0.f, 0.f);
+    if (d != INT_MIN) v = a.adjoint ? conjf2(g_f(a.m - d)) : g_f(a.m + d);
+    s[lpad(i)] = v;
   }
   __syncthreads();
-  fft_lds<Q, TPF, false, 0, GUARD>(s, t);
+  fft_lds<Q, TPF, false, 0, (NT > TPF)>(s, t);
   const float iq = 1.0f / (float)Q;
-  if (act)
-    for (int i = t; i < Q; i += TPF) a.gf[i] = s[lpad(i)] * iq;
+  for (int i = t; i < Q; i += NT) a.gf[i] = s[lpad(i)] * iq;
 }
 
+// Threads per set-up block: the float64 table work spreads over 1024 threads
+// (the block runs alone on its CU; one or two such blocks make up the launch).
+constexpr int CZT_SETUP_THREADS = 1024;
+
 template <int Q>
-__global__ __launch_bounds__(Q / 16) void czt_setup_kernel(CztSetupPair pair) {
+__global__ __launch_bounds__(CZT_SETUP_THREADS) void czt_setup_kernel(CztSetupPair pair) {
   __shared__ float2 s[LdsLen<Q>::value];
-  czt_setup_body<Q, false>(pair.ax[blockIdx.x], s, threadIdx.x);
+  czt_setup_body<Q, CZT_SETUP_THREADS>(pair.ax[blockIdx.x], s, threadIdx.x);
 }
 
 // Both axes in one launch when their lengths differ (block 0: QA, block 1: QB).
 template <int QA, int QB>
-__global__ __launch_bounds__((QA > QB ? QA : QB) / 16) void czt_setup2_kernel(CztSetupPair pair) {
+__global__ __launch_bounds__(CZT_SETUP_THREADS) void czt_setup2_kernel(CztSetupPair pair) {
   constexpr int QM = QA > QB ? QA : QB;
   __shared__ float2 s[LdsLen<QM>::value];
-  if (blockIdx.x == 0) czt_setup_body<QA, (QA < QM)>(pair.ax[0], s, threadIdx.x);
-  else czt_setup_body<QB, (QB < QM)>(pair.ax[1], s, threadIdx.x);
+  if (blockIdx.x == 0) czt_setup_body<QA, CZT_SETUP_THREADS>(pair.ax[0], s, threadIdx.x);
+  else czt_setup_body<QB, CZT_SETUP_THREADS>(pair.ax[1], s, threadIdx.x);
 }
 
 // Pre-op families the row kernel is specialised for.
@@ -479,13 +481,12 @@ inline void czt_cols_launch(const CztArgs& a, int batch, hipStream_t st) {
 template <int Q>
 inline void czt_setup_launch(const CztSetupPair& p, int nblocks, hipStream_t st) {
   KTimer t(K_CZT_SETUP, st);
-  hipLaunchKernelGGL(czt_setup_kernel<Q>, dim3(nblocks), dim3(Q / 16), 0, st, p);
+  hipLaunchKernelGGL(czt_setup_kernel<Q>, dim3(nblocks), dim3(CZT_SETUP_THREADS), 0, st, p);
 }
 template <int QA, int QB>
 inline void czt_setup2_launch(const CztSetupPair& p, hipStream_t st) {
   KTimer t(K_CZT_SETUP, st);
-  constexpr int QM = QA > QB ? QA : QB;
-  hipLaunchKernelGGL((czt_setup2_kernel<QA, QB>), dim3(2), dim3(QM / 16), 0, st, p);
+  hipLaunchKernelGGL((czt_setup2_kernel<QA, QB>), dim3(2), dim3(CZT_SETUP_THREADS), 0, st, p);
 }
 
 }  // namespace xl
