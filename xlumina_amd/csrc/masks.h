@@ -126,29 +126,44 @@ __device__ __forceinline__ void apply_mask(const MaskArgs& m, int64_t pix, float
   apply_stage(m.st[1], pix, e0, e1, e2);
 }
 
-// The pre-op inputs of one pixel (field components and per-pixel mask
-// parameters), loaded apart from their use so that the loads of several
-// pixels issue back to back and are in flight together, instead of one
-// pixel's round trip at a time.  Every load is unconditional: an input that
-// is absent is read from the pixel's own first component (the same cache
-// line, no extra memory traffic) and ignored, since a conditional load would
-// need a copy at the branch join that waits for it.
+// The pre-op inputs of U pixels (field components and per-pixel mask
+// parameters), loaded apart from their use so that the loads of all U pixels
+// issue back to back and are in flight together, instead of one pixel's round
+// trip at a time.  Each input kind is loaded for all U pixels under one
+// uniform branch (an absent input issues no load and reads as zero).
 struct PixRaw {
   float2 e[3];
   float v[2][2];   // [stage][m0, m1]
 };
 
-__device__ __forceinline__ void pix_load(const float2* in, int64_t cs, int nin, const MaskArgs& m, int64_t pix,
-                                         PixRaw& r) {
-  const float2* p0 = in + pix;
-  const float* own = reinterpret_cast<const float*>(p0);
-  r.e[0] = *p0;
-  r.e[1] = *(nin > 1 ? p0 + cs : p0);
-  r.e[2] = *(nin > 2 ? p0 + 2 * cs : p0);
+template <int U>
+__device__ __forceinline__ void pix_load_n(const float2* in, int64_t cs, int nin, const MaskArgs& m,
+                                           const int64_t (&pix)[U], PixRaw (&r)[U]) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    r[u].e[1] = r[u].e[2] = make_float2(0.f, 0.f);
+    r[u].v[0][0] = r[u].v[0][1] = r[u].v[1][0] = r[u].v[1][1] = 0.f;
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) r[u].e[0] = in[pix[u]];
+  if (nin > 1) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) r[u].e[1] = in[cs + pix[u]];
+  }
+  if (nin > 2) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) r[u].e[2] = in[2 * cs + pix[u]];
+  }
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
-    r.v[i][0] = *(stage_m0(m.st[i]) ? m.st[i].m0 + pix : own);
-    r.v[i][1] = *(stage_m1(m.st[i]) ? m.st[i].m1 + pix : own);
+    if (stage_m0(m.st[i])) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) r[u].v[i][0] = m.st[i].m0[pix[u]];
+    }
+    if (stage_m1(m.st[i])) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) r[u].v[i][1] = m.st[i].m1[pix[u]];
+    }
   }
 }
 
@@ -163,11 +178,10 @@ __device__ __forceinline__ void load_masked(const float2* in, int64_t cs, int ni
 }
 
 // The masked components of a loaded pixel (as apply_mask on the loaded field).
-__device__ __forceinline__ void pix_masked(const MaskArgs& m, int nin, const PixRaw& r, float2& e0, float2& e1,
-                                           float2& e2) {
+__device__ __forceinline__ void pix_masked(const MaskArgs& m, const PixRaw& r, float2& e0, float2& e1, float2& e2) {
   e0 = r.e[0];
-  e1 = nin > 1 ? r.e[1] : make_float2(0.f, 0.f);
-  e2 = nin > 2 ? r.e[2] : make_float2(0.f, 0.f);
+  e1 = r.e[1];
+  e2 = r.e[2];
   apply_stage_vals(m.st[0], r.v[0][0], r.v[0][1], e0, e1, e2);
   apply_stage_vals(m.st[1], r.v[1][0], r.v[1][1], e0, e1, e2);
 }

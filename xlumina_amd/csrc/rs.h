@@ -245,14 +245,16 @@ __global__ __launch_bounds__(P / 16) void rs_urow1_kernel(RsArgs a0) {
   // all loads of the 8 pixels first (in flight together), then the masks and Ez
   const bool rowok = row < a.n, ez = a.ez_mode != EZ_NONE;
   PixRaw raw[H];
+  int64_t pix[H];
   double xc[H];
 #pragma unroll
   for (int r = 0; r < H; ++r) {
     const int col = t + r * TPF;
     const bool ok = rowok && col < a.n;
-    pix_load(a.in, a.in_cs, a.nin, a.mask, ok ? (int64_t)row * a.n + col : 0, raw[r]);
+    pix[r] = ok ? (int64_t)row * a.n + col : 0;
     xc[r] = ez ? a.x[ok ? col : 0] : 0.0;
   }
+  pix_load_n<H>(a.in, a.in_cs, a.nin, a.mask, pix, raw);
   const double yr = ez ? a.y[rowok ? row : 0] : 0.0;
   float2 e[3][H];
 #pragma unroll
@@ -260,7 +262,7 @@ __global__ __launch_bounds__(P / 16) void rs_urow1_kernel(RsArgs a0) {
     const int col = t + r * TPF;
     float2 e0 = make_float2(0.f, 0.f), e1 = e0, e2 = e0;
     if (rowok && col < a.n) {
-      pix_masked(a.mask, a.nin, raw[r], e0, e1, e2);
+      pix_masked(a.mask, raw[r], e0, e1, e2);
       if (ez) e2 = build_ez(a.ez_mode, e0, e1, xc[r], yr, z);
     }
     e[0][r] = e0;
