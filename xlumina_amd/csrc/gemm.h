@@ -152,23 +152,21 @@ struct GemmPreArgs {
   MaskArgs mask;
 };
 
-// Pre-op values of pixel (row, col) for every component (zero in the padding).
-__device__ __forceinline__ void gemm_pre_values(const GemmPreArgs& a, int row, int col, float2 (&v)[3]) {
-  v[0] = v[1] = v[2] = make_float2(0.f, 0.f);
-  if (row >= a.rows || col >= a.cols) return;
-  float2 e0, e1, e2;
-  load_masked(a.in, a.in_cs, a.nin, a.mask, (int64_t)row * a.cols + col, e0, e1, e2);
+// Pre-op values of a pixel's masked field at input-plane coordinates (X, Y)
+// for every component.
+__device__ __forceinline__ void gemm_pre_compute(const GemmPreArgs& a, float2 e0, float2 e1, float2 e2, double X,
+                                                 double Y, float2 (&v)[3]) {
   if (a.pre_mode == 2 /* PRE_HNA */) {
     float A[3][2];
-    hna_map(a.px[col], a.py[row], a.hna_radius, a.hna_f, a.hna_stm2, A);
+    hna_map(X, Y, a.hna_radius, a.hna_f, a.hna_stm2, A);
 #pragma unroll
     for (int c = 0; c < 3; ++c) v[c] = c_add(c_scale(e0, A[c][0]), c_scale(e1, A[c][1]));
   } else {
     v[0] = e0;
     v[1] = e1;
-    v[2] = (a.ez_mode != EZ_NONE) ? build_ez(a.ez_mode, e0, e1, a.px[col], a.py[row], a.z) : e2;
+    v[2] = (a.ez_mode != EZ_NONE) ? build_ez(a.ez_mode, e0, e1, X, Y, a.z) : e2;
     if (a.pre_mode == 1 /* PRE_RS */ || a.pre_mode == 3 /* PRE_CONJ_RS */) {
-      float2 h = rs_kernel_value(a.px[col], a.py[row], a.z, a.k);
+      float2 h = rs_kernel_value(X, Y, a.z, a.k);
       if (a.pre_mode == 3) h = c_mul(conjf2(h), a.cte);
 #pragma unroll
       for (int c = 0; c < 3; ++c) v[c] = c_mul(v[c], h);
@@ -180,18 +178,39 @@ __device__ __forceinline__ void gemm_pre_values(const GemmPreArgs& a, int row, i
 }
 
 // Two consecutive columns per thread (ld is a multiple of 32), 4-byte plane
-// stores; blocks [0, nblocks) of the caller's grid.  (Issuing both pixels'
-// loads ahead of their pre-ops measured slower here.)
+// stores; blocks [0, nblocks) of the caller's grid.  Both pixels' loads are
+// grouped ahead of their pre-ops.
 __device__ __forceinline__ void gemm_pre_body(const GemmPreArgs& a, int block, int nblocks) {
   const int64_t total2 = (int64_t)a.rowsp * a.dst.ld / 2;
+  const bool grid = (a.pre_mode >= 1 && a.pre_mode <= 3) || a.ez_mode != EZ_NONE;   // RS, HNA, conj RS, or Ez
   for (int64_t e2 = block * (int64_t)blockDim.x + threadIdx.x; e2 < total2; e2 += (int64_t)nblocks * blockDim.x) {
     const int64_t e = e2 * 2;
     const int row = (int)(e / a.dst.ld), col0 = (int)(e % a.dst.ld);
+    const bool rowok = row < a.rows;
+    bool ok[2];
+    int64_t pix[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      ok[i] = rowok && col0 + i < a.cols;
+      pix[i] = ok[i] ? (int64_t)row * a.cols + col0 + i : 0;
+    }
+    PixRaw raw[2];
+    pix_load_n<2>(a.in, a.in_cs, a.nin, a.mask, pix, raw);
+    double X[2] = {0.0, 0.0}, Y = 0.0;
+    if (grid) {
+      X[0] = a.px[ok[0] ? col0 : 0];
+      X[1] = a.px[ok[1] ? col0 + 1 : 0];
+      Y = a.py[rowok ? row : 0];
+    }
     Split2 sp[3];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      float2 v[3];
-      gemm_pre_values(a, row, col0 + i, v);
+      float2 v[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+      if (ok[i]) {
+        float2 e0, e1, e2;
+        pix_masked(a.mask, raw[i], e0, e1, e2);
+        gemm_pre_compute(a, e0, e1, e2, X[i], Y, v);
+      }
 #pragma unroll
       for (int c = 0; c < 3; ++c) split2_put(sp[c], i, v[c]);
     }
