@@ -399,7 +399,9 @@ __attribute__((amdgpu_waves_per_eu(CztColsShape<Q>::WAVES))) void czt_cols_kerne
     auto load = [&](int r) -> float2 {
       return (colok && r < lin) ? cmul(src[czt_tidx(a.rin4, r, kx)], a.ay.pre[r]) : make_float2(0.f, 0.f);
     };
-    fft_forward_to_regs<Q, TPF, SRC_FN, ZL>(s, t, load, u);
+    // (twiddle bases loaded at each pass: prefetched, they cost the 3 * 2^k
+    // forms their 5-wave bound)
+    fft_forward_to_regs<Q, TPF, SRC_FN, ZL, false>(s, t, load, u);
   }
 #pragma unroll
   for (int b = 0; b < HO::BPT; ++b)
@@ -441,7 +443,7 @@ __attribute__((amdgpu_waves_per_eu(CztColsShape<Q>::WAVES))) void czt_cols_kerne
         oc[(int64_t)ky * lout_x + kx] = v;
       }
     };
-    fft_inverse_from_regs<Q, TPF, DST_FN, ZL>(s, t, u, store);
+    fft_inverse_from_regs<Q, TPF, DST_FN, ZL, false>(s, t, u, store);
   }
 }
 

@@ -178,18 +178,16 @@ template <bool INV> struct DFT<3, INV> {
   }
 };
 
-// v[r] *= w^r for r < R with w = e^{-+2 pi i base / TWN}: one table load (w);
+// v[r] *= w^r for r < R (w1 = w, e^{-+2 pi i base / TWN}, one table entry);
 // w^2, w^4, w^8 by squaring and the other powers as products formed next to
 // their use, so that only the four bases stay live.  Loads cost more than the
 // products here (each is a scattered L1 access per lane); the squarings add
 // at most a few float32 ulps to w^8.
-template <int R, bool INV, int L> __device__ __forceinline__ void apply_twiddles(float2* v, int base) {
+template <int R, bool INV> __device__ __forceinline__ void apply_twiddles_w(float2* v, const float2 w1) {
   if constexpr (R == 3) {
-    const float2 w1 = Tw<L>::at(base);
     v[1] = twmul<INV>(v[1], w1);
     v[2] = twmul<INV>(v[2], cmul(w1, w1));
   } else if constexpr (R >= 2) {
-    const float2 w1 = Tw<L>::at(base);
     v[1] = twmul<INV>(v[1], w1);
     if constexpr (R >= 4) {
       const float2 w2 = cmul(w1, w1);
@@ -217,6 +215,10 @@ template <int R, bool INV, int L> __device__ __forceinline__ void apply_twiddles
       }
     }
   }
+}
+
+template <int R, bool INV, int L> __device__ __forceinline__ void apply_twiddles(float2* v, int base) {
+  apply_twiddles_w<R, INV>(v, Tw<L>::at(base));
 }
 
 // One Stockham radix-R pass of an L-point FFT over TPF threads (thread t).

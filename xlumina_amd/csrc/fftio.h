@@ -31,9 +31,11 @@ enum { DST_LDS = 0, DST_FN = 1, DST_REG = 2 };
 // half: indices >= L/2); they are neither loaded nor transformed.  LOW: only
 // the outputs r < R/2 are formed and written (a last pass whose indices
 // >= L/2 are cropped away).
-template <int L, int R, int NS, int TPF, bool INV, int SRC, int DST, bool ZHI = false, bool LOW = false, class Load,
-          class Store, class RegT>
-__device__ __forceinline__ void pass_io(float2* s, int t, Load& load, Store& store, RegT& reg) {
+// wb: the pass's twiddle bases, one per butterfly slot b, when PREW (loaded
+// ahead by the caller); otherwise the pass loads them itself (NS > 1).
+template <int L, int R, int NS, int TPF, bool INV, int SRC, int DST, bool ZHI = false, bool LOW = false,
+          bool PREW = true, class Load, class Store, class RegT>
+__device__ __forceinline__ void pass_io(float2* s, int t, Load& load, Store& store, RegT& reg, const float2* wb) {
   constexpr int NB = L / R;
   constexpr int BPT = (NB + TPF - 1) / TPF;
   static_assert(!(ZHI && LOW), "a pass is either the first or the last of a chain here");
@@ -50,7 +52,10 @@ __device__ __forceinline__ void pass_io(float2* s, int t, Load& load, Store& sto
         else if constexpr (SRC == SRC_FN) v[b][r] = load(j + r * NB);
         else v[b][r] = reg.v[b][r];
       }
-      if constexpr (NS > 1) apply_twiddles<R, INV, L>(v[b], (j % NS) * (Tw<L>::N / (NS * R)));
+      if constexpr (NS > 1) {
+        if constexpr (PREW) apply_twiddles_w<R, INV>(v[b], wb[b]);
+        else apply_twiddles<R, INV, L>(v[b], (j % NS) * (Tw<L>::N / (NS * R)));
+      }
       if constexpr (ZHI) DFTHalf<R, INV>::run(v[b]);
       else if constexpr (LOW) DFTLow<R, INV>::run(v[b]);
       else DFT<R, INV>::run(v[b]);
@@ -79,6 +84,33 @@ struct NoIO {
   __device__ void operator()(int, float2) const {}
 };
 
+// Twiddle bases of every pass of a chain (the table entry w of pass I for
+// butterfly slot b of thread t), loaded together when the chain starts: a
+// pass's twiddles then need no memory access behind its barrier.
+template <int L, int TPF, bool FWD> struct TwSet {
+  using P = FftPlan<L>;
+  static constexpr int bpt(int I) { return (L / P::radix(FWD, I) + TPF - 1) / TPF; }
+  static constexpr int off(int I) {
+    int o = 0;
+    for (int k = 1; k < I; ++k) o += bpt(k);
+    return o;
+  }
+  static constexpr int N = off(P::NP) > 0 ? off(P::NP) : 1;
+  float2 w[N];
+  template <int I = 1> __device__ __forceinline__ void load(int t) {
+    if constexpr (I < P::NP) {
+      constexpr int R = P::radix(FWD, I), NS = P::ns(FWD, I), NB = L / R;
+#pragma unroll
+      for (int b = 0; b < bpt(I); ++b) {
+        const int j = t + b * TPF;
+        w[off(I) + b] = (NB % TPF == 0 || j < NB) ? Tw<L>::at((j % NS) * (Tw<L>::N / (NS * R))) : make_float2(1.f, 0.f);
+      }
+      load<I + 1>(t);
+    }
+  }
+  template <int I> __device__ __forceinline__ const float2* at() const { return w + (I >= 1 ? off(I) : 0); }
+};
+
 // Pass chain i..NP-1 of the forward (FWD = true) or inverse plan (FftPlan).  The first pass reads SRC, the last writes DST, the
 // others go through LDS.  Every pass that writes LDS first waits for all
 // threads (so a chain may follow any earlier LDS use); a chain ending in
@@ -86,9 +118,10 @@ struct NoIO {
 // ZHI / LOW (see pass_io) apply to the first / last pass of the chain; both
 // need a plan that starts (ZHI) or ends (LOW) with an unstrided radix-16 pass
 // over the L/2 split, i.e. a power-of-two L.
-template <int L, int TPF, bool INV, bool FWD, int I, int SRC, int DST, bool ZHI = false, bool LOW = false,
-          class Load, class Store, class RegIn, class RegOut>
-__device__ __forceinline__ void chain_io(float2* s, int t, Load& load, Store& store, RegIn& rin, RegOut& rout) {
+template <int L, int TPF, bool INV, bool FWD, int I, int SRC, int DST, bool ZHI, bool LOW, bool TWPRE, class Load,
+          class Store, class RegIn, class RegOut>
+__device__ __forceinline__ void chain_pass(float2* s, int t, Load& load, Store& store, RegIn& rin, RegOut& rout,
+                                           TwSet<L, TPF, FWD>& tw) {
   using P = FftPlan<L>;
   constexpr int R = P::radix(FWD, I), NS = P::ns(FWD, I);
   constexpr bool first = I == 0, last = I == P::NP - 1;
@@ -97,14 +130,30 @@ __device__ __forceinline__ void chain_io(float2* s, int t, Load& load, Store& st
   static_assert(!(ZHI || LOW) || (!P::TRI && P::NP >= 2), "zero-half passes need a power-of-two plan");
   static_assert(!(ZHI && first) || R == 16, "ZHI needs a radix-16 first pass");
   static_assert(!(LOW && last) || (R == 16 && NS * R == L), "LOW needs an unstrided radix-16 last pass");
+  // TWPRE: the bases were loaded when the chain started; otherwise each pass
+  // loads its own
+  const float2* wb = tw.template at<I>();
   if constexpr (first) {
-    pass_io<L, R, NS, TPF, INV, S, D, ZHI, false>(s, t, load, store, rin);
+    pass_io<L, R, NS, TPF, INV, S, D, ZHI, false, TWPRE>(s, t, load, store, rin, wb);
   } else if constexpr (last) {
-    pass_io<L, R, NS, TPF, INV, S, D, false, LOW>(s, t, load, store, rout);
+    pass_io<L, R, NS, TPF, INV, S, D, false, LOW, TWPRE>(s, t, load, store, rout, wb);
   } else {
-    pass_io<L, R, NS, TPF, INV, S, D>(s, t, load, store, rout);
+    pass_io<L, R, NS, TPF, INV, S, D, false, false, TWPRE>(s, t, load, store, rout, wb);
   }
-  if constexpr (!last) chain_io<L, TPF, INV, FWD, I + 1, SRC, DST, ZHI, LOW>(s, t, load, store, rin, rout);
+  if constexpr (!last)
+    chain_pass<L, TPF, INV, FWD, I + 1, SRC, DST, ZHI, LOW, TWPRE>(s, t, load, store, rin, rout, tw);
+}
+
+// The whole chain (I = 0).  TWPRE: the twiddle bases of all passes are loaded
+// first (a few registers per pass: the kernels that cannot spare them keep
+// loading each pass's bases at the pass).
+template <int L, int TPF, bool INV, bool FWD, int I, int SRC, int DST, bool ZHI = false, bool LOW = false,
+          bool TWPRE = true, class Load, class Store, class RegIn, class RegOut>
+__device__ __forceinline__ void chain_io(float2* s, int t, Load& load, Store& store, RegIn& rin, RegOut& rout) {
+  static_assert(I == 0, "a chain starts at its first pass");
+  TwSet<L, TPF, FWD> tw;
+  if constexpr (TWPRE) tw.load(t);
+  chain_pass<L, TPF, INV, FWD, 0, SRC, DST, ZHI, LOW, TWPRE>(s, t, load, store, rin, rout, tw);
 }
 
 template <int L, int TPF> struct HandOff {
@@ -119,20 +168,20 @@ template <int L, int TPF> struct HandOff {
 };
 
 // forward FFT: SRC -> registers (hand-off layout)
-template <int L, int TPF, int SRC, bool ZHI = false, class Load>
+template <int L, int TPF, int SRC, bool ZHI = false, bool TWPRE = true, class Load>
 __device__ __forceinline__ void fft_forward_to_regs(float2* s, int t, Load& load,
                                                     typename HandOff<L, TPF>::Type& out) {
   NoIO none;
   typename HandOff<L, TPF>::Type dummy;
-  chain_io<L, TPF, false, true, 0, SRC, DST_REG, ZHI, false>(s, t, load, none, dummy, out);
+  chain_io<L, TPF, false, true, 0, SRC, DST_REG, ZHI, false, TWPRE>(s, t, load, none, dummy, out);
 }
 
 // inverse FFT: registers (hand-off layout) -> DST
-template <int L, int TPF, int DST, bool LOW = false, class Store>
+template <int L, int TPF, int DST, bool LOW = false, bool TWPRE = true, class Store>
 __device__ __forceinline__ void fft_inverse_from_regs(float2* s, int t, typename HandOff<L, TPF>::Type& in,
                                                       Store& store) {
   NoIO none;
-  chain_io<L, TPF, true, false, 0, SRC_REG, DST, false, LOW>(s, t, none, store, in, in);
+  chain_io<L, TPF, true, false, 0, SRC_REG, DST, false, LOW, TWPRE>(s, t, none, store, in, in);
 }
 
 // Output layout of the last pass of a plan: slot (b, q) holds index j + q*NS,
