@@ -286,9 +286,13 @@ __global__ __launch_bounds__(256 * WN) void cgemm_kernel(GemmArgs a) {
   constexpr int CH_A = 4 * GBM * (GBK / 8);   // chunks of 8 bf16 per K step: A planes
   constexpr int CH = CH_A + 4 * BN * (GBK / 8);
   constexpr int PER = (CH + NT - 1) / NT;
-  // [plane][row][k] bf16, single buffer; the next K step is prefetched into registers
-  __shared__ __attribute__((aligned(16))) uint16_t smA[4][GBM][GLDK];
-  __shared__ __attribute__((aligned(16))) uint16_t smB[4][BN][GLDK];
+  // [buffer][plane][row][k] bf16, two buffers: step ks computes on one while
+  // the registers prefetched for step ks + 1 are stored into the other, so one
+  // barrier per step suffices.  The block runs alone on its CU either way (its
+  // register use allows at most four waves per SIMD, i.e. one block of 4 x WN
+  // waves), so the second buffer costs no occupancy.
+  __shared__ __attribute__((aligned(16))) uint16_t smA[2][4][GBM][GLDK];
+  __shared__ __attribute__((aligned(16))) uint16_t smB[2][4][BN][GLDK];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w & 3, wn = w >> 2;
   const int bm = blockIdx.y * GBM, bn = blockIdx.x * BN;
@@ -313,40 +317,35 @@ __global__ __launch_bounds__(256 * WN) void cgemm_kernel(GemmArgs a) {
       pre[i] = v;
     }
   };
-  auto sstore = [&]() {
+  auto sstore = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int ch = tid + i * NT;
       if (CH % NT == 0 || ch < CH) {
         if (ch < CH_A) {
           const int pl = ch / (4 * GBM), r = (ch >> 2) % GBM, kc = ch & 3;
-          *reinterpret_cast<uint4*>(&smA[pl][r][kc * 8]) = pre[i];
+          *reinterpret_cast<uint4*>(&smA[buf][pl][r][kc * 8]) = pre[i];
         } else {
           const int c2 = ch - CH_A;
           const int pl = c2 / (4 * BN), r = (c2 >> 2) % BN, kc = c2 & 3;
-          *reinterpret_cast<uint4*>(&smB[pl][r][kc * 8]) = pre[i];
+          *reinterpret_cast<uint4*>(&smB[buf][pl][r][kc * 8]) = pre[i];
         }
       }
     }
   };
   f32x16 crr = {}, cii = {}, cri = {};
-  gload(kbeg * GBK);
-  for (int ks = kbeg; ks < kend; ++ks) {
-    __syncthreads();
-    sstore();
-    __syncthreads();
-    if (ks + 1 < kend) gload((ks + 1) * GBK);
+  auto compute = [&](int buf) {
 #pragma unroll
     for (int kk = 0; kk < GBK; kk += 16) {
       const int ra = wm * 32 + (lane & 31), rb = wn * 32 + (lane & 31), kq = kk + 8 * (lane >> 5);
-      const bf16x8 arh = *reinterpret_cast<const bf16x8*>(&smA[0][ra][kq]);
-      const bf16x8 arl = *reinterpret_cast<const bf16x8*>(&smA[1][ra][kq]);
-      const bf16x8 aih = *reinterpret_cast<const bf16x8*>(&smA[2][ra][kq]);
-      const bf16x8 ail = *reinterpret_cast<const bf16x8*>(&smA[3][ra][kq]);
-      const bf16x8 brh = *reinterpret_cast<const bf16x8*>(&smB[0][rb][kq]);
-      const bf16x8 brl = *reinterpret_cast<const bf16x8*>(&smB[1][rb][kq]);
-      const bf16x8 bih = *reinterpret_cast<const bf16x8*>(&smB[2][rb][kq]);
-      const bf16x8 bil = *reinterpret_cast<const bf16x8*>(&smB[3][rb][kq]);
+      const bf16x8 arh = *reinterpret_cast<const bf16x8*>(&smA[buf][0][ra][kq]);
+      const bf16x8 arl = *reinterpret_cast<const bf16x8*>(&smA[buf][1][ra][kq]);
+      const bf16x8 aih = *reinterpret_cast<const bf16x8*>(&smA[buf][2][ra][kq]);
+      const bf16x8 ail = *reinterpret_cast<const bf16x8*>(&smA[buf][3][ra][kq]);
+      const bf16x8 brh = *reinterpret_cast<const bf16x8*>(&smB[buf][0][rb][kq]);
+      const bf16x8 brl = *reinterpret_cast<const bf16x8*>(&smB[buf][1][rb][kq]);
+      const bf16x8 bih = *reinterpret_cast<const bf16x8*>(&smB[buf][2][rb][kq]);
+      const bf16x8 bil = *reinterpret_cast<const bf16x8*>(&smB[buf][3][rb][kq]);
       crr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arl, brh, crr, 0, 0, 0);
       crr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, brl, crr, 0, 0, 0);
       crr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, brh, crr, 0, 0, 0);
@@ -360,6 +359,22 @@ __global__ __launch_bounds__(256 * WN) void cgemm_kernel(GemmArgs a) {
       cri = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, bih, cri, 0, 0, 0);
       cri = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aih, brh, cri, 0, 0, 0);
     }
+  };
+  if (kbeg < kend) {
+    gload(kbeg * GBK);
+    sstore(0);
+    __syncthreads();
+    if (kbeg + 1 < kend) gload((kbeg + 1) * GBK);
+  }
+  for (int ks = kbeg; ks < kend; ++ks) {
+    const int cur = (ks - kbeg) & 1;
+    compute(cur);
+    // the other buffer was last read in step ks - 1, before the barrier that ended it
+    if (ks + 1 < kend) {
+      sstore(cur ^ 1);
+      if (ks + 2 < kend) gload((ks + 2) * GBK);
+    }
+    __syncthreads();
   }
   // epilogue: lane owns column n = lane % 32 and rows 8*(r/4) + 4*(lane/32) + r%4
   const int n = bn + wn * 32 + (lane & 31);
