@@ -51,11 +51,29 @@ extern "C" {
 #define XL_CZT_HNA 1   /* cte * K_y (RW-lens(E)) K_x^T            optical_elements.py:600-638 */
 #define XL_CZT_PLAIN 2 /* K_y E K_x^T                             wave_optics.py:359-371   */
 
+/* Storage forms of the mask parameters m0 / m1 (xl_mask_stage.forms: the
+ * form of m0 in bits 0-2, of m1 in bits 3-5), so that a caller whose
+ * parameters are float64 raw values -- the reference's optimisers keep them
+ * so and map them with p * 2 pi - pi (optical_elements.py:1556-1559) -- passes
+ * them without a cast, a rescaling or an expansion to a per-pixel array:
+ *   0                                 float32 per pixel (const float*)
+ *   XL_PF_F64                         float64 per pixel (const double*)
+ *   XL_PF_F64 | XL_PF_SCALAR          one float64 on the device for every pixel
+ *   ... | XL_PF_UNIT_ANGLE            the value used is 2 pi p - pi of the stored p
+ * The float64 forms need batch <= 1.  xl_pre_backward writes the gradient of
+ * the stored p: float32 per pixel for form 0, float64 per pixel for XL_PF_F64
+ * (gparam then points to doubles), and for a scalar form the sum over the
+ * pixels at element 0 of a buffer of xl_pre_scalar_grad_elems(rows, cols)
+ * doubles. */
+#define XL_PF_F64 1
+#define XL_PF_SCALAR 2
+#define XL_PF_UNIT_ANGLE 4
+
 typedef struct xl_mask_stage {
   int32_t kind;
-  int32_t reserved;
-  const float* m0; /* per-pixel parameter 0 (phase / eta / alpha) or NULL */
-  const float* m1; /* per-pixel parameter 1 (phi / theta) or NULL        */
+  int32_t forms;   /* storage forms of m0 (bits 0-2) and m1 (bits 3-5), XL_PF_*  */
+  const float* m0; /* parameter 0 (phase / eta / alpha) or NULL              */
+  const float* m1; /* parameter 1 (phi / theta) or NULL                      */
   double s0, s1;   /* constant eta, theta for XL_MASK_LCD_SCALAR           */
 } xl_mask_stage;
 
@@ -99,6 +117,24 @@ typedef struct xl_rs_params {
   int64_t out_batch_stride;       /* complex elements, >= ncomp n^2 */
   int64_t intensity_batch_stride; /* float elements, >= n^2 when intensity is set */
   int64_t mask_batch_stride;      /* float elements (0: shared mask parameters) */
+  /* Gradients in z and in k = 2 pi / lambda without a derivative propagation
+   * (one problem, kernel_dz = 0): by Parseval, Re<g, conv(E', dH)> =
+   * sum_c sum_k Re(conj(G_c(k)) dH(k) U_c(k)) over the P x P spectra, where U_c
+   * is the spectrum of the padded, masked input component c and G_c that of
+   * the output cotangent.
+   *   spec_mode 1 (forward call): U_c is kept in `spec`
+   *     (xl_rs_spec_bytes(n, ncomp) bytes of device memory);
+   *   spec_mode 2 (the adjoint call of the same geometry and distance, its
+   *     input the output cotangent): the derivative spectra named by
+   *     spec_grads (bit 0: dH/dz, bit 1: dH/dk) are built in `spec` and
+   *     grad_out[0] = Re<g, conv(E', dH/dz)>, grad_out[1] = Re<g, conv(E', dH/dk)>
+   *     (float64, fixed summation order) are written for the bits set.
+   * The Ez_in construction's own dependence on z is not included (xl_dz_dot
+   * with ncomp = 0 adds it). */
+  int32_t spec_mode;       /* 0, 1 or 2 */
+  int32_t spec_grads;      /* spec_mode 2: bit 0 z, bit 1 k */
+  void* spec;
+  double* grad_out;        /* spec_mode 2: 2 float64 on the device */
 } xl_rs_params;
 
 typedef struct xl_czt_params {
@@ -148,7 +184,7 @@ typedef struct xl_pre_params {
   const void* gout;        /* backward: cotangent of the pre-op output         */
   void* out;               /* forward: pre-op output; backward: grad of inputs */
   int64_t comp_stride;
-  float* gparam[2][2];     /* backward: per-pixel grads of mask[s].m0 / m1, or NULL */
+  float* gparam[2][2];     /* backward: grads of mask[s].m0 / m1 in their storage forms, or NULL */
   int32_t rows, cols;
   int32_t ez_mode;         /* XL_EZ_*                                          */
   int32_t hna;             /* 1: high-NA lens map (optical_elements.py:555-594) */
@@ -180,6 +216,11 @@ int xl_init(void);
 size_t xl_rs_workspace_bytes(int32_t n, int32_t ncomp);
 size_t xl_rs_batch_workspace_bytes(int32_t n, int32_t ncomp, int32_t batch);
 size_t xl_rs_hhat_bytes(int32_t n);
+/* Bytes of the `spec` buffer of spec_mode 1 / 2 (the input spectra, two
+ * derivative spectra and per-column partial sums); 0 for bad arguments.
+ * The gradients in z / lambda of RS_propagation (wave_optics.py:281-289
+ * under jax.grad) without a second propagation. */
+size_t xl_rs_spec_bytes(int32_t n, int32_t ncomp);
 int xl_rs_propagate(const xl_rs_params* p, void* workspace, size_t workspace_bytes, void* stream);
 
 /* No counterpart in the reference.  xl_rs_propagate without a workspace
@@ -214,6 +255,10 @@ int xl_czt_gemm(const xl_czt_params* p, void* workspace, size_t workspace_bytes,
  * (optical_elements.py:87-332), high_NA_objective_lens (:515-594). */
 int xl_pre_forward(const xl_pre_params* p, void* stream);
 int xl_pre_backward(const xl_pre_params* p, void* stream);
+/* doubles of the gradient buffer of a scalar-form mask parameter in
+ * xl_pre_backward over rows x cols pixels: the sum at element 0, one partial
+ * sum per launch block after it (see XL_PF_SCALAR). */
+size_t xl_pre_scalar_grad_elems(int32_t rows, int32_t cols);
 
 /* MSE_Intensity (loss_functions.py:137-149) of a batch of detector fields
  * against one target intensity |T|^2, and its cotangent with respect to the
@@ -397,7 +442,8 @@ int xl_quality_factor(const double* z_dev, double rmax, double dr_real, double w
  * w = -z / (X^2 + Y^2 + z^2)^{3/2}: the z-dependence of the constructed
  * Ez_in = (Ex' X + Ey' Y)/r (vectorized_optics.py:258-261), where pre holds
  * (Ex', Ey') and gez the cotangent of Ez_in.  float64 result on the device,
- * summed in a fixed order. */
+ * summed in a fixed order.  ncomp = 0 (d and g unused) with `pre` set gives
+ * the Ez_in term alone. */
 typedef struct xl_dz_dot_params {
   const void* d;               /* ncomp x rows x cols complex64, comp_stride apart */
   const void* g;               /* same layout as d */

@@ -400,6 +400,71 @@ def test_vczt_grad_wrt_z_vs_fd(cuda_dev):
     assert abs(float(gz) - fd) <= 2e-3 * abs(fd), (float(gz), fd)
 
 
+@pytest.mark.parametrize("n,vectorial", [(20, False), (64, True), (300, False), (1024, True), (2048, False)])
+def test_rs_spectral_grads_match_derivative_propagation(cuda_dev, n, vectorial):
+    """The z and k gradients of the adjoint's column pass (Parseval over the
+    kept input spectra, spec_mode 1 / 2) against the derivative propagation
+    (kernel_dz 1 / 2) dotted with the same cotangent by xl_dz_dot; the forward
+    and adjoint fields are unchanged by keeping / dotting the spectra.  Sizes
+    cover P = 64 (16 columns per block) up to 4096; z > 0 and z < 0."""
+    x, y = R.space(3.0 * n, n)
+    rng = np.random.default_rng(1000 + n)
+
+    def c64(*shape):
+        return torch.tensor(rng.standard_normal(shape) + 1j * rng.standard_normal(shape), dtype=torch.complex64,
+                            device=cuda_dev)
+
+    nin = 2 if vectorial else 1
+    ncomp = 3 if vectorial else 1
+    src = c64(nin, n, n)
+    p0 = torch.tensor(rng.uniform(-np.pi, np.pi, (n, n)), dtype=torch.float32, device=cuda_dev)
+    p1 = torch.tensor(rng.uniform(-np.pi, np.pi, (n, n)), dtype=torch.float32, device=cuda_dev)
+    masks = [MaskSpec(N.XL_MASK_LCD, p0, p1)] if vectorial else [MaskSpec(N.XL_MASK_SLM, p0)]
+    ez = N.XL_EZ_VRS if vectorial else N.XL_EZ_NONE
+    g = c64(ncomp, n, n)
+    for z in (40.0 * n, -25.0 * n):
+        kw = dict(masks=masks, ez_mode=ez, nin=nin)
+        hhat = torch.empty(ops.rs_hhat_bytes(n), dtype=torch.uint8, device=cuda_dev)
+        spec = torch.empty(ops.rs_spec_bytes(n, ncomp), dtype=torch.uint8, device=cuda_dev)
+        out = ops.rs_propagate(src, x, y, z, 0.6, hhat=hhat, hhat_mode=1, spec=spec, spec_mode=1, **kw)
+        plain = ops.rs_propagate(src, x, y, z, 0.6, **kw)
+        assert float((out - plain).norm() / plain.norm()) <= 1e-6
+        grads = torch.full((2,), float("nan"), dtype=torch.float64, device=cuda_dev)
+        gp = ops.rs_propagate(g, x, y, z, 0.6, adjoint=True, hhat=hhat, hhat_mode=2, spec=spec, spec_mode=2,
+                              spec_grads=3, grad_out=grads)
+        plain = ops.rs_propagate(g, x, y, z, 0.6, adjoint=True, hhat=hhat, hhat_mode=2)
+        assert float((gp - plain).norm() / plain.norm()) <= 1e-6
+        for i, kd in enumerate((1, 2)):
+            d = ops.rs_propagate(src, x, y, z, 0.6, kernel_dz=kd, **kw)
+            ref = float(ops.dz_dot(d, g))
+            scale = float(torch.linalg.vector_norm(d) * torch.linalg.vector_norm(g))
+            assert abs(float(grads[i]) - ref) <= 2e-5 * scale, (n, z, kd, float(grads[i]), ref, scale)
+        # one derivative at a time leaves the other slot alone
+        one = torch.full((2,), 7.0, dtype=torch.float64, device=cuda_dev)
+        ops.rs_propagate(g, x, y, z, 0.6, adjoint=True, hhat=hhat, hhat_mode=2, spec=spec, spec_mode=2,
+                         spec_grads=2, grad_out=one)
+        assert float(one[0]) == 7.0 and float(one[1]) == float(grads[1])
+
+
+def test_dz_dot_ez_term_alone(cuda_dev):
+    """xl_dz_dot with no propagated planes (the VRS Ez_in term of the
+    distance gradient alone) equals the full sum minus its field term."""
+    rng = np.random.default_rng(97)
+    rows, cols = 70, 90
+    x, y = np.linspace(-200.0, 200.0, cols), np.linspace(-150.0, 150.0, rows)
+
+    def c64(*shape):
+        return torch.tensor(rng.standard_normal(shape) + 1j * rng.standard_normal(shape), dtype=torch.complex64,
+                            device=cuda_dev)
+
+    d, g, pre, gez = c64(3, rows, cols), c64(3, rows, cols), c64(2, rows, cols), c64(rows, cols)
+    full = float(ops.dz_dot(d, g, pre=pre, gez=gez, x=x, y=y, z=500.0))
+    alone = float(ops.dz_dot(None, None, pre=pre, gez=gez, x=x, y=y, z=500.0))
+    field = float(ops.dz_dot(d, g))
+    scale = float(torch.linalg.vector_norm(d) * torch.linalg.vector_norm(g)) + abs(alone)
+    assert abs(alone + field - full) <= 1e-6 * scale
+
+
 @pytest.mark.parametrize("vectorial", [False, True])
 def test_rs_grad_wrt_wavelength_vs_fd(cuda_dev, vectorial):
     """dL/dlambda of RS / VRS (after a per-pixel mask): the convolution with

@@ -1,0 +1,175 @@
+"""Mask parameters in their float64 storage forms against the same masks given
+as float32 arrays mapped on the host (GPU).
+
+Forms: float64 per pixel, one float64 value for every pixel, and raw angles p
+mapped to 2 pi p - pi inside the kernels (the reference's parameter map,
+optical_elements.py:1556-1559).  Both routes see the same float32 angles
+(the map is formed in float64 and rounded once in both), so fields and field
+cotangents agree to <= 1e-6 relative L2; parameter gradients are those of the
+stored values -- 2 pi times the angle's, summed over the pixels for a
+one-value parameter -- and agree to <= 1e-5 relative.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from xlumina_amd import _native as N
+from xlumina_amd import autograd as AG
+from xlumina_amd import ops
+from xlumina_amd.ops import MaskSpec
+
+pytestmark = pytest.mark.gpu
+
+TWO_PI = 2 * math.pi
+
+
+def mapped(p):
+    return (p * TWO_PI - math.pi).float()
+
+
+def rel(a, b):
+    return float((a - b).norm() / b.norm())
+
+
+def field(rng, c, n, dev):
+    return torch.tensor(rng.standard_normal((c, n, n)) + 1j * rng.standard_normal((c, n, n)),
+                        dtype=torch.complex64, device=dev)
+
+
+@pytest.mark.parametrize("kind", [N.XL_MASK_SLM, N.XL_MASK_SSLM, N.XL_MASK_LCD, N.XL_MASK_LP])
+@pytest.mark.parametrize("one_value", [False, True])
+def test_pre_op_forms_match_float32(cuda_dev, kind, one_value):
+    """pre_forward / pre_backward with raw float64 angles (per pixel or one
+    value) against float32 angles mapped on the host."""
+    rng = np.random.default_rng(300 + kind + 10 * one_value)
+    n = 72
+    x = np.linspace(-100.0, 100.0, n)
+    nin = 1 if kind == N.XL_MASK_SLM else 2
+    src = field(rng, nin, n, cuda_dev)
+    shape = (1,) if one_value else (n, n)
+    raw = [torch.tensor(rng.random(shape), dtype=torch.float64, device=cuda_dev) for _ in range(2)]
+    two = kind in (N.XL_MASK_SSLM, N.XL_MASK_LCD)
+    m64 = [MaskSpec(kind, raw[0], raw[1] if two else None, unit=3 if two else 1)]
+    m32 = [MaskSpec(kind, mapped(raw[0]).expand(n, n), mapped(raw[1]).expand(n, n) if two else None)]
+    out64 = ops.pre_forward(src, x, x, masks=m64, nin=nin)
+    out32 = ops.pre_forward(src, x, x, masks=m32, nin=nin)
+    assert rel(out64, out32) <= 1e-6
+    g = field(rng, out32.shape[0], n, cuda_dev)
+    need = ((True, two), (False, False))
+    gs64, gp64 = ops.pre_backward(src, g, x, x, masks=m64, nin=nin, need_params=need)
+    gs32, gp32 = ops.pre_backward(src, g, x, x, masks=m32, nin=nin, need_params=need)
+    assert rel(gs64, gs32) <= 1e-6
+    for j in range(2 if two else 1):
+        assert gp64[0][j].dtype == torch.float64
+        want = TWO_PI * gp32[0][j].double()
+        if one_value:
+            want = want.sum().reshape(1)
+            assert abs(float(gp64[0][j][0]) - float(want[0])) <= 1e-5 * float(gp32[0][j].double().abs().sum() * TWO_PI)
+        else:
+            assert rel(gp64[0][j], want) <= 1e-5
+
+
+def test_float64_per_pixel_without_map(cuda_dev):
+    """An amplitude mask given as float64 per pixel (no angle map): the same
+    field and a float64 gradient equal to the float32 route's."""
+    rng = np.random.default_rng(310)
+    n = 64
+    x = np.linspace(-90.0, 90.0, n)
+    src = field(rng, 2, n, cuda_dev)
+    a = [torch.tensor(rng.random((n, n)), dtype=torch.float64, device=cuda_dev) for _ in range(2)]
+    m64 = [MaskSpec(N.XL_MASK_AMP, a[0], a[1])]
+    m32 = [MaskSpec(N.XL_MASK_AMP, a[0].float(), a[1].float())]
+    assert rel(ops.pre_forward(src, x, x, masks=m64), ops.pre_forward(src, x, x, masks=m32)) <= 1e-6
+    g = field(rng, 2, n, cuda_dev)
+    _, gp64 = ops.pre_backward(src, g, x, x, masks=m64, need_params=((True, True), (False, False)))
+    _, gp32 = ops.pre_backward(src, g, x, x, masks=m32, need_params=((True, True), (False, False)))
+    for j in range(2):
+        assert gp64[0][j].dtype == torch.float64 and rel(gp64[0][j], gp32[0][j].double()) <= 1e-6
+
+
+@pytest.mark.parametrize("n", [64, 300])
+def test_vrs_autograd_raw_angles(cuda_dev, n):
+    """VRS after an sSLM with raw per-pixel angles and an LCD with raw
+    one-value angles, through autograd: output and the raw parameters'
+    gradients equal those of the host-mapped float32 route (whose torch map
+    carries the 2 pi and the pixel sum)."""
+    rng = np.random.default_rng(320 + n)
+    x = np.linspace(-4.0 * n, 4.0 * n, n)
+    src = field(rng, 2, n, cuda_dev)
+    leaves = [torch.tensor(rng.random(s), dtype=torch.float64, device=cuda_dev, requires_grad=True)
+              for s in ((n, n), (n, n), (1,), (1,))]
+    w = torch.tensor(rng.random((3, n, n)), dtype=torch.float32, device=cuda_dev)
+
+    def run(raw):
+        if raw:
+            masks = [MaskSpec(N.XL_MASK_SSLM, leaves[0], leaves[1], unit=3),
+                     MaskSpec(N.XL_MASK_LCD, leaves[2], leaves[3], unit=3)]
+        else:
+            masks = [MaskSpec(N.XL_MASK_SSLM, mapped(leaves[0]), mapped(leaves[1])),
+                     MaskSpec(N.XL_MASK_LCD, mapped(leaves[2]).expand(n, n), mapped(leaves[3]).expand(n, n))]
+        out = AG.rs(src, x, x, 30.0 * n, 0.6, masks=masks, ez_mode=N.XL_EZ_VRS, nin=2)
+        L = (w * out.abs() ** 2).sum()
+        return out.detach(), torch.autograd.grad(L, leaves)
+
+    o64, g64 = run(True)
+    o32, g32 = run(False)
+    assert rel(o64, o32) <= 1e-6
+    for a, b in zip(g64, g32):
+        assert a.shape == b.shape and a.dtype == torch.float64
+        assert rel(a, b) <= 1e-5
+
+
+@pytest.mark.parametrize("form", ["bluestein", "gemm"])
+def test_czt_raw_angles(cuda_dev, form):
+    """VCZT after an sSLM with raw float64 angles, Bluestein and zoom-GEMM
+    forms: the same field and raw-angle gradients as the float32 route."""
+    rng = np.random.default_rng(330)
+    n, m = 128, 48
+    x = np.linspace(-300.0, 300.0, n)
+    xo = np.linspace(-40.0, 40.0, m)
+    geom = ops.czt_geometry(x, x, xo, xo, 5000.0, 0.6)
+    src = field(rng, 2, n, cuda_dev)
+    leaves = [torch.tensor(rng.random((n, n)), dtype=torch.float64, device=cuda_dev, requires_grad=True)
+              for _ in range(2)]
+    w = torch.tensor(rng.random((3, m, m)), dtype=torch.float32, device=cuda_dev)
+
+    def run(raw):
+        if raw:
+            masks = [MaskSpec(N.XL_MASK_SSLM, leaves[0], leaves[1], unit=3)]
+        else:
+            masks = [MaskSpec(N.XL_MASK_SSLM, mapped(leaves[0]), mapped(leaves[1]))]
+        out = AG.czt(src, geom, masks=masks, ez_mode=N.XL_EZ_VCZT, nin=2, form=form)
+        L = (w * out.abs() ** 2).sum()
+        return out.detach(), torch.autograd.grad(L, leaves)
+
+    o64, g64 = run(True)
+    o32, g32 = run(False)
+    assert rel(o64, o32) <= 1e-6
+    for a, b in zip(g64, g32):
+        assert rel(a, b) <= 1e-5
+
+
+def test_float64_forms_refused_in_batches(cuda_dev):
+    """A batched launch reads per-member float32 parameters: the library
+    refuses float64 forms there (no launch)."""
+    lib = N.load()
+    n = 64
+    src = torch.zeros((2, 1, n, n), dtype=torch.complex64, device=cuda_dev)
+    out = torch.empty_like(src)
+    ph = torch.zeros(1, dtype=torch.float64, device=cuda_dev)
+    xd = ops.device_grid(np.linspace(-1.0, 1.0, n), cuda_dev)
+    p = N.RsParams()
+    p.inp, p.out, p.in_comp_stride, p.out_comp_stride = src.data_ptr(), out.data_ptr(), n * n, n * n
+    p.nin, p.n, p.x, p.y, p.z, p.k, p.dx, p.dy = 1, n, xd.data_ptr(), xd.data_ptr(), 100.0, 10.0, 0.03, 0.03
+    p.mask[0].kind, p.mask[0].forms, p.mask[0].m0 = N.XL_MASK_SLM, N.XL_PF_F64 | N.XL_PF_SCALAR, ph.data_ptr()
+    p.batch, p.in_batch_stride, p.out_batch_stride = 2, n * n, n * n
+    ws = torch.empty(int(lib.xl_rs_batch_workspace_bytes(n, 1, 2)), dtype=torch.uint8, device=cuda_dev)
+    rc = lib.xl_rs_propagate(C.byref(p), ws.data_ptr(), ws.numel(), ops._stream(cuda_dev))
+    assert rc == N.XL_EINVAL and b"batch" in lib.xl_last_error()
+    p.mask[0].forms = N.XL_PF_SCALAR   # a scalar form needs XL_PF_F64
+    p.batch = 1
+    rc = lib.xl_rs_propagate(C.byref(p), ws.data_ptr(), ws.numel(), ops._stream(cuda_dev))
+    assert rc == N.XL_EINVAL and b"XL_PF_F64" in lib.xl_last_error()

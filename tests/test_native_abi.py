@@ -106,13 +106,22 @@ def test_every_struct_field_matches_c():
 
 
 def test_host_side_queries(lib):
-    assert lib.xl_abi_version() == 6
+    assert lib.xl_abi_version() == 7
     assert lib.xl_max_fft_length() == 4096
     # RS workspace: ncomp * round4(n) * P + (P/2) * P + P complex64 (+ alignment slack)
     n, P = 1024, 2048
     assert lib.xl_rs_workspace_bytes(n, 1) == (n * P + (P // 2) * P + P) * 8 + 256
     assert lib.xl_rs_workspace_bytes(1, 1) == 0
     assert lib.xl_rs_workspace_bytes(4096, 1) > 0   # rejected later by the propagate call (P > 4096)
+    # spectrum buffer of the z / lambda gradients: ncomp P^2 input spectra, two
+    # kernel spectra, two float64 partials per (column, component) unit
+    hh = ((P // 2) * P + P) * 8
+    for nc in (1, 3):
+        assert lib.xl_rs_spec_bytes(n, nc) == nc * P * P * 8 + 2 * hh + 2 * nc * P * 8 + 256
+    # P = 64 (every region already a multiple of 256 bytes): 192 units in 12 blocks of 16
+    assert lib.xl_rs_spec_bytes(20, 3) == 3 * 64 * 64 * 8 + 2 * (32 * 64 + 64) * 8 + 2 * 192 * 8 + 256
+    assert lib.xl_rs_spec_bytes(20, 1) == 64 * 64 * 8 + 2 * (32 * 64 + 64) * 8 + 2 * 64 * 8 + 256
+    assert lib.xl_rs_spec_bytes(1, 1) == 0 and lib.xl_rs_spec_bytes(64, 4) == 0 and lib.xl_rs_spec_bytes(4096, 1) == 0
 
 
 def _czt_params(m, M):

@@ -23,6 +23,7 @@ class XlError(RuntimeError):
 XL_OK, XL_EINVAL, XL_EHIP, XL_EWORKSPACE = range(4)
 XL_MASK_NONE, XL_MASK_SLM, XL_MASK_SSLM, XL_MASK_LCD, XL_MASK_LCD_SCALAR, XL_MASK_LP, XL_MASK_AMP = range(7)
 XL_EZ_NONE, XL_EZ_VRS, XL_EZ_VCZT, XL_EZ_HNA = range(4)
+XL_PF_F64, XL_PF_SCALAR, XL_PF_UNIT_ANGLE = 1, 2, 4
 XL_CZT_RS, XL_CZT_HNA, XL_CZT_PLAIN = range(3)
 XL_LOSS_INTENSITY, XL_LOSS_AMPLITUDE, XL_LOSS_PHASE = range(3)
 XL_RS_NOALLOC_MAX_N = 1024
@@ -30,7 +31,7 @@ XL_RS_NOALLOC_SCRATCH_BYTES = 67125504
 
 
 class MaskStage(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("m0", C.c_void_p), ("m1", C.c_void_p),
+    _fields_ = [("kind", C.c_int32), ("forms", C.c_int32), ("m0", C.c_void_p), ("m1", C.c_void_p),
                 ("s0", C.c_double), ("s1", C.c_double)]
 
 
@@ -42,7 +43,8 @@ class RsParams(C.Structure):
                 ("mask", MaskStage * 2), ("z_dev", C.c_void_p), ("kernel_dz", C.c_int32), ("hhat_mode", C.c_int32),
                 ("hhat", C.c_void_p), ("batch", C.c_int32), ("reserved2", C.c_int32), ("in_batch_stride", C.c_int64),
                 ("out_batch_stride", C.c_int64), ("intensity_batch_stride", C.c_int64),
-                ("mask_batch_stride", C.c_int64)]
+                ("mask_batch_stride", C.c_int64), ("spec_mode", C.c_int32), ("spec_grads", C.c_int32),
+                ("spec", C.c_void_p), ("grad_out", C.c_void_p)]
 
 
 class CztParams(C.Structure):
@@ -135,6 +137,8 @@ SIGNATURES = {
     "xl_rs_workspace_bytes": (C.c_size_t, [C.c_int32, C.c_int32]),
     "xl_rs_batch_workspace_bytes": (C.c_size_t, [C.c_int32, C.c_int32, C.c_int32]),
     "xl_rs_hhat_bytes": (C.c_size_t, [C.c_int32]),
+    "xl_rs_spec_bytes": (C.c_size_t, [C.c_int32, C.c_int32]),
+    "xl_pre_scalar_grad_elems": (C.c_size_t, [C.c_int32, C.c_int32]),
     "xl_rs_propagate": (C.c_int, [C.POINTER(RsParams), C.c_void_p, C.c_size_t, C.c_void_p]),
     "xl_rs_propagate_noalloc": (C.c_int, [C.POINTER(RsParams), C.c_void_p]),
     "xl_czt_workspace_bytes": (C.c_size_t, [C.POINTER(CztParams)]),

@@ -27,13 +27,13 @@ from .ops import MaskSpec
 
 def _mask_structure(masks: Sequence[MaskSpec]):
     """Static part of a mask chain (kinds and scalar values)."""
-    return tuple((m.kind, m.s0, m.s1) for m in masks)
+    return tuple((m.kind, m.s0, m.s1, m.unit) for m in masks)
 
 
 def _rebuild(struct, tensors):
     out = []
-    for i, (kind, s0, s1) in enumerate(struct):
-        out.append(MaskSpec(kind, tensors[2 * i], tensors[2 * i + 1], s0, s1))
+    for i, (kind, s0, s1, unit) in enumerate(struct):
+        out.append(MaskSpec(kind, tensors[2 * i], tensors[2 * i + 1], s0, s1, unit))
     return out
 
 
@@ -42,6 +42,17 @@ def _flat_params(masks: Sequence[MaskSpec]):
     for i, m in enumerate(masks[:2]):
         flat[2 * i], flat[2 * i + 1] = m.p0, m.p1
     return flat
+
+
+def _full_grad(ctx, src, nin):
+    """A complex64 buffer of src's shape for the adjoint of a pre-op that
+    reads only src[:nin]: xl_pre_backward fills its first nin planes and the
+    rest become zeros (no zero-filled copy of the whole field); None where
+    src is not a contiguous complex64 (C > nin, rows, cols) field."""
+    if (not ctx.needs_input_grad[1] or src.dim() != 3 or src.shape[0] <= nin or src.dtype != torch.complex64
+            or not src.is_contiguous()):
+        return None
+    return torch.empty(src.shape, dtype=torch.complex64, device=src.device)
 
 
 def _param_grads(ctx, struct, params, gp):
@@ -58,7 +69,10 @@ def _param_grads(ctx, struct, params, gp):
                 outs.append(None)
                 continue
             if g.shape != p.shape:
-                g = g.sum() if p.dim() == 0 else g.sum_to_size(p.shape)
+                if g.numel() == p.numel():     # a one-value parameter's gradient, summed by the kernels
+                    g = g.reshape(p.shape)
+                else:
+                    g = g.sum() if p.dim() == 0 else g.sum_to_size(p.shape)
             outs.append(g.to(p.dtype))
     return outs
 
@@ -80,15 +94,22 @@ class _RSFunction(torch.autograd.Function):
     def forward(ctx, cfg: RSConfig, src, p00, p01, p10, p11, z_t, wl_t):
         params = (p00, p01, p10, p11)
         masks = _rebuild(cfg.mask_struct, params)
-        # a step that will run the adjoint keeps the kernel spectrum for it
-        hhat = None
+        # a step that will run the adjoint keeps the kernel spectrum for it, and,
+        # when z or lambda receive gradients, the input spectra they are dotted with
+        n = src.shape[-1]
+        hhat = spec = None
         if any(ctx.needs_input_grad[1:]):
-            hhat = torch.empty(ops.rs_hhat_bytes(src.shape[-1]), dtype=torch.uint8, device=src.device)
+            hhat = torch.empty(ops.rs_hhat_bytes(n), dtype=torch.uint8, device=src.device)
+        if ctx.needs_input_grad[6] or ctx.needs_input_grad[7]:
+            ncomp = 3 if cfg.ez_mode == N.XL_EZ_VRS else cfg.nin
+            spec = torch.empty(ops.rs_spec_bytes(n, ncomp), dtype=torch.uint8, device=src.device)
         out = ops.rs_propagate(src, cfg.x, cfg.y, cfg.z, cfg.wavelength, masks=masks, ez_mode=cfg.ez_mode,
-                               nin=cfg.nin, z_dev=cfg.z_dev, hhat=hhat, hhat_mode=1 if hhat is not None else 0)
+                               nin=cfg.nin, z_dev=cfg.z_dev, hhat=hhat, hhat_mode=1 if hhat is not None else 0,
+                               spec=spec, spec_mode=1 if spec is not None else 0)
         ctx.cfg = cfg
         ctx.save_for_backward(src, *[p if p is not None else torch.empty(0) for p in params],
-                              hhat if hhat is not None else torch.empty(0))
+                              hhat if hhat is not None else torch.empty(0),
+                              spec if spec is not None else torch.empty(0))
         ctx.has = [p is not None for p in params]
         ctx.z_meta = None if z_t is None else (z_t.shape, z_t.dtype, z_t.device)
         ctx.wl_meta = None if wl_t is None else (wl_t.shape, wl_t.dtype, wl_t.device)
@@ -101,30 +122,38 @@ class _RSFunction(torch.autograd.Function):
         src = saved[0]
         params = [saved[1 + i] if ctx.has[i] else None for i in range(4)]
         hhat = saved[5] if saved[5].numel() else None
+        spec = saved[6] if saved[6].numel() else None
         gout = gout.contiguous()
-        g_pre = ops.rs_propagate(gout, cfg.x, cfg.y, cfg.z, cfg.wavelength, adjoint=True, z_dev=cfg.z_dev,
-                                 hhat=hhat, hhat_mode=2 if hhat is not None else 0)
+        want_z, want_wl = ctx.needs_input_grad[6], ctx.needs_input_grad[7]
+        grads = None
+        if spec is not None:
+            # Re<g, conv(E', dH/dz)> and Re<g, conv(E', dH/dk)> by Parseval in the adjoint's column pass
+            grads = torch.empty(2, dtype=torch.float64, device=gout.device)
+            g_pre = ops.rs_propagate(gout, cfg.x, cfg.y, cfg.z, cfg.wavelength, adjoint=True, z_dev=cfg.z_dev,
+                                     hhat=hhat, hhat_mode=2 if hhat is not None else 0, spec=spec, spec_mode=2,
+                                     spec_grads=(1 if want_z else 0) | (2 if want_wl else 0), grad_out=grads)
+        else:
+            g_pre = ops.rs_propagate(gout, cfg.x, cfg.y, cfg.z, cfg.wavelength, adjoint=True, z_dev=cfg.z_dev,
+                                     hhat=hhat, hhat_mode=2 if hhat is not None else 0)
         masks = _rebuild(cfg.mask_struct, params)
         need_params = ((ctx.needs_input_grad[2] and params[0] is not None, ctx.needs_input_grad[3] and params[1] is not None),
                        (ctx.needs_input_grad[4] and params[2] is not None, ctx.needs_input_grad[5] and params[3] is not None))
         g_z = None
-        if ctx.needs_input_grad[6]:
+        if want_z:
             shape, dtype, device = ctx.z_meta
-            g_z = _rs_z_grad(cfg, src, masks, gout, g_pre).to(device=device, dtype=dtype).reshape(shape)
+            g_z = _rs_z_grad(cfg, src, masks, grads[0], g_pre).to(device=device, dtype=dtype).reshape(shape)
         g_wl = None
-        if ctx.needs_input_grad[7]:
+        if want_wl:
             # dL/dlambda = -(k/lambda) Re<g, conv(E', dH/dk)>; Ez_in does not depend on lambda
             shape, dtype, device = ctx.wl_meta
-            d = ops.rs_propagate(src, cfg.x, cfg.y, cfg.z, cfg.wavelength, masks=masks, ez_mode=cfg.ez_mode,
-                                 nin=cfg.nin, kernel_dz=2, z_dev=cfg.z_dev)
             k = 2 * math.pi / cfg.wavelength
-            g_wl = (-(k / cfg.wavelength) * ops.dz_dot(d, gout)).to(device=device, dtype=dtype).reshape(shape)
+            g_wl = (-(k / cfg.wavelength) * grads[1]).to(device=device, dtype=dtype).reshape(shape)
         if not masks and cfg.ez_mode == N.XL_EZ_NONE:
             g_src, gp = g_pre, [[None, None], [None, None]]
         else:
             g_src, gp = ops.pre_backward(src, g_pre, cfg.x, cfg.y, cfg.z, masks=masks, ez_mode=cfg.ez_mode,
                                          nin=cfg.nin, need_src=ctx.needs_input_grad[1], need_params=need_params,
-                                         z_dev=cfg.z_dev)
+                                         z_dev=cfg.z_dev, out=_full_grad(ctx, src, cfg.nin))
         if g_src is not None and ctx.needs_input_grad[1]:
             if g_src.shape[0] != src.shape[0]:
                 full = torch.zeros_like(src)
@@ -135,18 +164,17 @@ class _RSFunction(torch.autograd.Function):
         return (None, g_src, *_param_grads(ctx, cfg.mask_struct, params, gp), g_z, g_wl)
 
 
-def _rs_z_grad(cfg: RSConfig, src, masks, gout, g_pre):
-    """dL/dz = Re<g, d out/dz>: the propagation of the pre-op'd field with
-    dH/dz, plus (VRS) the z-dependence of the constructed Ez_in =
-    (Ex X + Ey Y)/r, r = sqrt(X^2 + Y^2 + z^2) (vectorized_optics.py:262-267),
-    through its cotangent g_pre[2]."""
-    d = ops.rs_propagate(src, cfg.x, cfg.y, cfg.z, cfg.wavelength, masks=masks, ez_mode=cfg.ez_mode, nin=cfg.nin,
-                         kernel_dz=True, z_dev=cfg.z_dev)
+def _rs_z_grad(cfg: RSConfig, src, masks, g_prop, g_pre):
+    """dL/dz = Re<g, d out/dz>: g_prop = Re<g, conv(E', dH/dz)> (the
+    propagation's own dependence on z, from the adjoint's column pass) plus,
+    for VRS, the z-dependence of the constructed Ez_in = (Ex X + Ey Y)/r,
+    r = sqrt(X^2 + Y^2 + z^2) (vectorized_optics.py:262-267), through its
+    cotangent g_pre[2]."""
     if cfg.ez_mode != N.XL_EZ_VRS:
-        return ops.dz_dot(d, gout)
+        return g_prop
     # d Ez_in / dz = -(Ex' X + Ey' Y) z / r^3, through Ez_in's cotangent g_pre[2]
     pre = ops.pre_forward(src, cfg.x, cfg.y, cfg.z, masks=masks, ncomp=2, nin=cfg.nin) if masks else src[:2]
-    return ops.dz_dot(d, gout, pre=pre, gez=g_pre[2], x=cfg.x, y=cfg.y, z=cfg.z, z_dev=cfg.z_dev)
+    return g_prop + ops.dz_dot(None, None, pre=pre, gez=g_pre[2], x=cfg.x, y=cfg.y, z=cfg.z, z_dev=cfg.z_dev)
 
 
 def rs(src: torch.Tensor, x, y, z, wavelength: float, masks: Sequence[MaskSpec] = (),
@@ -223,7 +251,7 @@ class _CZTFunction(torch.autograd.Function):
         else:
             g_src, gp = ops.pre_backward(src, g_pre, g.x, g.y, g.z, masks=masks, ez_mode=cfg.ez_mode, hna=hna,
                                          nin=cfg.nin, need_src=ctx.needs_input_grad[1], need_params=need_params,
-                                         hna_radius=g.hna_radius, hna_f=g.hna_f)
+                                         hna_radius=g.hna_radius, hna_f=g.hna_f, out=_full_grad(ctx, src, cfg.nin))
         if g_src is not None and ctx.needs_input_grad[1]:
             if g_src.shape[0] != src.shape[0]:
                 full = torch.zeros_like(src)

@@ -19,12 +19,37 @@ namespace xl {
 enum { MASK_NONE = 0, MASK_SLM = 1, MASK_SSLM = 2, MASK_LCD = 3, MASK_LCD_SCALAR = 4, MASK_LP = 5, MASK_AMP = 6 };
 enum { EZ_NONE = 0, EZ_VRS = 1, EZ_VCZT = 2, EZ_HNA = 3 };
 
+// Storage forms of a stage parameter (3 bits each, m0 in bits 0-2, m1 in
+// bits 3-5 of MaskStage::pf): 0 float32 per pixel; PF_F64 float64 per pixel;
+// PF_F64 | PF_SCALAR one float64 shared by every pixel; PF_UNIT (with PF_F64)
+// the value used is 2 pi p - pi of the stored p, formed in float64 as the
+// reference's parameter map is (optical_elements.py:1556-1559).
+enum { PF_F64 = 1, PF_SCALAR = 2, PF_UNIT = 4 };
+
 struct MaskStage {
   int kind;
-  const float* m0;   // phase / eta / alpha, per pixel
-  const float* m1;   // phi / theta, per pixel
+  int pf;            // storage forms of m0 / m1
+  const float* m0;   // phase / eta / alpha (a const double* in the float64 forms)
+  const float* m1;   // phi / theta
   double s0, s1;     // constant eta, theta (MASK_LCD_SCALAR)
 };
+
+__device__ __forceinline__ int param_form(const MaskStage& m, int j) { return (m.pf >> (3 * j)) & 7; }
+
+// The value of a float64-stored parameter p (two roundings, as p * 2pi - pi
+// in float64 without contraction, then one to float32).
+__device__ __forceinline__ float param_value(double p, int f) {
+  if (f & PF_UNIT) p = __dsub_rn(__dmul_rn(p, 6.283185307179586), 3.141592653589793);
+  return (float)p;
+}
+
+// Parameter j (0: m0, 1: m1) of stage m at pixel pix.
+__device__ __forceinline__ float mask_param(const MaskStage& m, int j, int64_t pix) {
+  const float* p = j ? m.m1 : m.m0;
+  const int f = param_form(m, j);
+  if (!(f & PF_F64)) return p[pix];
+  return param_value(reinterpret_cast<const double*>(p)[(f & PF_SCALAR) ? 0 : pix], f);
+}
 // Up to two masks applied in order (e.g. sSLM then LCD).
 struct MaskArgs {
   MaskStage st[2];
@@ -112,7 +137,7 @@ __device__ __forceinline__ void stage_apply(const MaskStage& m, P0 p0, P1 p1, fl
 }
 
 __device__ __forceinline__ void apply_stage(const MaskStage& m, int64_t pix, float2& e0, float2& e1, float2& e2) {
-  stage_apply(m, [&] { return m.m0[pix]; }, [&] { return m.m1[pix]; }, e0, e1, e2);
+  stage_apply(m, [&] { return mask_param(m, 0, pix); }, [&] { return mask_param(m, 1, pix); }, e0, e1, e2);
 }
 
 // A stage whose parameters were loaded ahead (v0 = m0[pix], v1 = m1[pix]).
@@ -155,16 +180,27 @@ __device__ __forceinline__ void pix_load_n(const float2* in, int64_t cs, int nin
     for (int u = 0; u < U; ++u) r[u].e[2] = in[2 * cs + pix[u]];
   }
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    if (stage_m0(m.st[i])) {
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int u = 0; u < U; ++u) r[u].v[i][0] = m.st[i].m0[pix[u]];
-    }
-    if (stage_m1(m.st[i])) {
+    for (int j = 0; j < 2; ++j) {
+      if (!(j ? stage_m1(m.st[i]) : stage_m0(m.st[i]))) continue;
+      const float* p = j ? m.st[i].m1 : m.st[i].m0;
+      const int f = param_form(m.st[i], j);
+      if (!(f & PF_F64)) {
 #pragma unroll
-      for (int u = 0; u < U; ++u) r[u].v[i][1] = m.st[i].m1[pix[u]];
+        for (int u = 0; u < U; ++u) r[u].v[i][j] = p[pix[u]];
+      } else if (f & PF_SCALAR) {
+        const float v = param_value(reinterpret_cast<const double*>(p)[0], f);
+#pragma unroll
+        for (int u = 0; u < U; ++u) r[u].v[i][j] = v;
+      } else {
+        double d[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) d[u] = reinterpret_cast<const double*>(p)[pix[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) r[u].v[i][j] = param_value(d[u], f);
+      }
     }
-  }
 }
 
 // The masked field of pixel pix, loaded and masked in one go (for loops that

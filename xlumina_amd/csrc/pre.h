@@ -21,7 +21,9 @@ struct PreArgs {
   const float2* gout;
   float2* out;
   int64_t cs;
-  float* gparam[2][2];
+  float* gparam[2][2];     // float32 per pixel, or a double* in the float64 parameter forms
+  int scal;                // bit 2i+j: gparam[i][j] is a scalar-form gradient (one launch row, blocks <= its
+                           // buffer): per-block sums at element 1 + block, reduced into element 0
   int rows, cols, ez_mode, hna;
   const double* x;
   const double* y;
@@ -100,30 +102,31 @@ __global__ void pre_forward_kernel(PreArgs a0) {
 }
 
 // adjoint of one mask stage: e_in -> (stage) -> e_out; g holds the cotangent of
-// e_out on entry and of e_in on exit; parameter cotangents go to gp0 / gp1.
+// e_out on entry and of e_in on exit; the cotangents of the parameter values
+// are formed in d0 / d1 where want0 / want1 ask for them.
 __device__ __forceinline__ void stage_backward(const MaskStage& m, int64_t pix, float2 e0, float2 e1, float2& g0,
-                                               float2& g1, float2& g2, float* gp0, float* gp1) {
+                                               float2& g1, float2& g2, bool want0, bool want1, float& d0, float& d1) {
   switch (m.kind) {
     case MASK_SLM: {
-      const float2 w = c_expi(m.m0[pix]);
+      const float2 w = c_expi(mask_param(m, 0, pix));
       const float2 o = c_mul(e0, w);
-      if (gp0) gp0[pix] = re_conj_mul(g0, make_float2(-o.y, o.x));   // d/dphi = i o
+      if (want0) d0 = re_conj_mul(g0, make_float2(-o.y, o.x));   // d/dphi = i o
       g0 = c_mul(g0, make_float2(w.x, -w.y));
       break;
     }
     case MASK_SSLM: {
-      const float2 w0 = c_expi(m.m0[pix]), w1 = c_expi(m.m1[pix]);
+      const float2 w0 = c_expi(mask_param(m, 0, pix)), w1 = c_expi(mask_param(m, 1, pix));
       const float2 o0 = c_mul(e0, w0), o1 = c_mul(e1, w1);
-      if (gp0) gp0[pix] = re_conj_mul(g0, make_float2(-o0.y, o0.x));
-      if (gp1) gp1[pix] = re_conj_mul(g1, make_float2(-o1.y, o1.x));
+      if (want0) d0 = re_conj_mul(g0, make_float2(-o0.y, o0.x));
+      if (want1) d1 = re_conj_mul(g1, make_float2(-o1.y, o1.x));
       g0 = c_mul(g0, make_float2(w0.x, -w0.y));
       g1 = c_mul(g1, make_float2(w1.x, -w1.y));
       break;
     }
     case MASK_LCD:
     case MASK_LCD_SCALAR: {
-      const float eta = m.kind == MASK_LCD ? m.m0[pix] : (float)m.s0;
-      const float th = m.kind == MASK_LCD ? m.m1[pix] : (float)m.s1;
+      const float eta = m.kind == MASK_LCD ? mask_param(m, 0, pix) : (float)m.s0;
+      const float th = m.kind == MASK_LCD ? mask_param(m, 1, pix) : (float)m.s1;
       float sh, ch, st, ct;
       sincos_rad(0.5f * eta, &sh, &ch);
       sincos_rad(th, &st, &ct);
@@ -139,11 +142,11 @@ __device__ __forceinline__ void stage_backward(const MaskStage& m, int64_t pix, 
       const float2 b00 = make_float2(ch * (-2.f * sc + 2.f * sc), 4.f * sh * sc);
       const float2 b11 = make_float2(ch * (2.f * sc - 2.f * sc), -4.f * sh * sc);
       const float2 b01 = make_float2(0.f, -2.f * sh * (c2 - s2));
-      if (gp0)
-        gp0[pix] = re_conj_mul(g0, c_add(c_mul(a00, e0), c_mul(a01, e1))) +
+      if (want0)
+        d0 = re_conj_mul(g0, c_add(c_mul(a00, e0), c_mul(a01, e1))) +
                    re_conj_mul(g1, c_add(c_mul(a01, e0), c_mul(a11, e1)));
-      if (gp1)
-        gp1[pix] = re_conj_mul(g0, c_add(c_mul(b00, e0), c_mul(b01, e1))) +
+      if (want1)
+        d1 = re_conj_mul(g0, c_add(c_mul(b00, e0), c_mul(b01, e1))) +
                    re_conj_mul(g1, c_add(c_mul(b01, e0), c_mul(b11, e1)));
       const float2 n0 = c_add(c_mul(conjf2(j00), g0), c_mul(conjf2(j01), g1));
       const float2 n1 = c_add(c_mul(conjf2(j01), g0), c_mul(conjf2(j11), g1));
@@ -153,10 +156,10 @@ __device__ __forceinline__ void stage_backward(const MaskStage& m, int64_t pix, 
     }
     case MASK_LP: {
       float s, c;
-      sincos_rad(m.m0[pix], &s, &c);
-      if (gp0) {
+      sincos_rad(mask_param(m, 0, pix), &s, &c);
+      if (want0) {
         const float d00 = -2.f * s * c, d01 = c * c - s * s, d11 = 2.f * s * c;
-        gp0[pix] = re_conj_mul(g0, c_add(c_scale(e0, d00), c_scale(e1, d01))) +
+        d0 = re_conj_mul(g0, c_add(c_scale(e0, d00), c_scale(e1, d01))) +
                    re_conj_mul(g1, c_add(c_scale(e0, d01), c_scale(e1, d11)));
       }
       const float2 n0 = c_add(c_scale(g0, c * c), c_scale(g1, c * s));
@@ -167,12 +170,12 @@ __device__ __forceinline__ void stage_backward(const MaskStage& m, int64_t pix, 
       break;
     }
     case MASK_AMP: {   // e' = A e: dA = Re(conj(g) e), g <- A g
-      const float a0 = m.m0[pix];
-      if (gp0) gp0[pix] = re_conj_mul(g0, e0);
+      const float a0 = mask_param(m, 0, pix);
+      if (want0) d0 = re_conj_mul(g0, e0);
       g0 = c_scale(g0, a0);
       if (m.m1) {
-        const float a1 = m.m1[pix];
-        if (gp1) gp1[pix] = re_conj_mul(g1, e1);
+        const float a1 = mask_param(m, 1, pix);
+        if (want1) d1 = re_conj_mul(g1, e1);
         g1 = c_scale(g1, a1);
       }
       break;
@@ -182,9 +185,10 @@ __device__ __forceinline__ void stage_backward(const MaskStage& m, int64_t pix, 
   }
 }
 
-__global__ void pre_backward_kernel(PreArgs a0) {
+__global__ __launch_bounds__(256) void pre_backward_kernel(PreArgs a0) {
   const PreArgs a = pre_member(a0, blockIdx.y);
   const int64_t npix = (int64_t)a.rows * a.cols;
+  double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
   for (int64_t pix = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; pix < npix;
        pix += (int64_t)gridDim.x * blockDim.x) {
     // forward recompute of the stage inputs (the second stage's input only
@@ -222,13 +226,73 @@ __global__ void pre_backward_kernel(PreArgs a0) {
       g1 = c_add(g1, c_scale(g2, cy));
       g2 = make_float2(0.f, 0.f);
     }
-    stage_backward(a.mask.st[1], pix, m0, m1, g0, g1, g2, a.gparam[1][0], a.gparam[1][1]);
-    stage_backward(a.mask.st[0], pix, x0, x1, g0, g1, g2, a.gparam[0][0], a.gparam[0][1]);
+    float d[2][2] = {{0.f, 0.f}, {0.f, 0.f}};
+    stage_backward(a.mask.st[1], pix, m0, m1, g0, g1, g2, a.gparam[1][0], a.gparam[1][1], d[1][0], d[1][1]);
+    stage_backward(a.mask.st[0], pix, x0, x1, g0, g1, g2, a.gparam[0][0], a.gparam[0][1], d[0][0], d[0][1]);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        if (!a.gparam[i][j]) continue;
+        const int f = param_form(a.mask.st[i], j);
+        if (!(f & PF_F64))
+          a.gparam[i][j][pix] = d[i][j];
+        else if (f & PF_SCALAR)
+          acc[i][j] += (double)d[i][j];
+        else
+          reinterpret_cast<double*>(a.gparam[i][j])[pix] = (f & PF_UNIT) ? 6.283185307179586 * d[i][j] : d[i][j];
+      }
     if (a.out) {
       a.out[pix] = g0;
       if (a.nin > 1) a.out[a.in_cs + pix] = g1;
       if (a.nin > 2) a.out[2 * a.in_cs + pix] = g2;
     }
+  }
+  // scalar-form parameters: this block's sum (times 2 pi for PF_UNIT) at
+  // element 1 + block of the parameter's buffer; pgrad_reduce_kernel adds them
+  if (a.scal) {
+    __shared__ double red[4][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        if (!(a.scal >> (2 * i + j) & 1)) continue;
+        double v = acc[i][j];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if ((threadIdx.x & 63) == 0) red[2 * i + j][threadIdx.x >> 6] = v;
+      }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          if (!(a.scal >> (2 * i + j) & 1)) continue;
+          const double v = red[2 * i + j][0] + red[2 * i + j][1] + red[2 * i + j][2] + red[2 * i + j][3];
+          const double sc = (param_form(a.mask.st[i], j) & PF_UNIT) ? 6.283185307179586 : 1.0;
+          reinterpret_cast<double*>(a.gparam[i][j])[1 + blockIdx.x] = sc * v;
+        }
+    }
+  }
+}
+
+// buf[i][0] = sum of buf[i][1 .. n] for the scalar-form parameter gradients
+// (block i; null pointers skip), fixed order.
+__global__ __launch_bounds__(1024) void pgrad_reduce_kernel(double* b0, double* b1, double* b2, double* b3, int n) {
+  double* b = blockIdx.x == 0 ? b0 : blockIdx.x == 1 ? b1 : blockIdx.x == 2 ? b2 : b3;
+  if (!b) return;   // whole block
+  __shared__ double red[16];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) s += b[1 + i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = 0.0;
+    for (int w = 0; w < 16; ++w) v += red[w];
+    b[0] = v;
   }
 }
 

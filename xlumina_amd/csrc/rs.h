@@ -54,6 +54,16 @@ struct RsArgs {
   // spectra; with z_dev set, member b reads z_dev[b].
   int batch;
   int64_t in_bs, out_bs, inten_bs, mask_bs, t_bs, hr_bs;
+  // Gradients by Parseval (one problem).  spec_mode 1: rs_col keeps the input
+  // column spectra in `spec`, unit-major in the hand-off register order
+  // (unit u, slot j, thread t at u P + j TPF + t).  spec_mode 2: rs_col adds
+  // Re(conj(G) dg[i] U) over its column into gpart[i][unit] for each
+  // derivative spectrum dg[i] (Hhat layout) that is set.
+  int spec_mode;
+  float2* spec;
+  float2* dg[2];
+  double* gpart[2];
+  double* grad_out;       // spec_mode 2: out[i] = sum of gpart[i]
 };
 
 // The arguments of batch member b (pointers offset by the batch strides).
@@ -321,6 +331,42 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_hcol_kernel(Rs
     }
 }
 
+// Sum of v over the TPF threads of each group (groups of TPF < 64 share a
+// wave, aligned to TPF lanes), fixed order; every thread of the block calls
+// it.  `red` holds CPB * (TPF / 64) values when TPF > 64.
+template <int TPF>
+__device__ __forceinline__ double group_sum_f64(double v, double* red, int g, int t) {
+  constexpr int W = TPF < 64 ? TPF : 64;
+#pragma unroll
+  for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if constexpr (TPF > 64) {
+    constexpr int NW = TPF / 64;
+    if ((t & 63) == 0) red[g * NW + (t >> 6)] = v;
+    __syncthreads();
+    v = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) v += red[g * NW + w];
+  }
+  return v;
+}
+
+// spec_mode 2 of the column pass, after its multiply loop has accumulated
+// acc[i] = sum over the thread's ky of Re(conj(G) dg[i] U) (G the spectrum in
+// registers, the cotangent's; U the kept input spectrum; dg[i] the derivative
+// spectrum, even in kx and ky like Hhat): the unit's sums, fixed order, into
+// gpart[i][unit] for i in the bit set GB.
+template <int TPF, int CPB, int GB>
+__device__ __forceinline__ void rs_col_dot_sums(const RsArgs& a, const double (&acc)[2], int unit, int g, int t) {
+  constexpr int NW = TPF > 64 ? TPF / 64 : 1;
+  __shared__ double red[2][CPB * NW];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    if (!(GB >> i & 1)) continue;
+    const double v = group_sum_f64<TPF>(acc[i], red[i], g, t);
+    if (t == 0) a.gpart[i][unit] = v;
+  }
+}
+
 // Column pass.  One (column, component) unit per thread group: the T column
 // (rows 0..N-1, zero padded) is transformed along y straight from the tiles,
 // multiplied in registers by the kernel spectrum of its column (read from
@@ -329,7 +375,10 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_hcol_kernel(Rs
 // LDS), so four blocks share a CU and their pass barriers interleave.  The
 // units of a column and of its tile neighbours run back to back on one XCD,
 // so the Hhat column and the 128 B tile lines are fetched once into its L2.
-template <int P>
+// SPEC 1 also keeps the column's spectrum; SPEC = 2 + GB, GB in 1..3, adds
+// its gradient dots for the derivative spectra in the bit set GB (spec_mode 2
+// of RsArgs).
+template <int P, int SPEC>
 __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_col_kernel(RsArgs a0) {
   const RsArgs a = rs_member(a0, blockIdx.y);
   constexpr int TPF = P / 16, LS = LdsLen<P>::value, CPB = Units<P>::CPB;
@@ -350,20 +399,56 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_col_kernel(RsA
     return (active && r < nr) ? tc[(r >> 2) * 16 + (r & 3) * 4] : make_float2(0.f, 0.f);
   };
   typename HO::Type u;
+  constexpr int GB = SPEC >= 3 ? SPEC - 2 : 0;
+  // the gradient dots (SPEC >= 3) read the kept input spectrum U: up to
+  // P = 2048 its loads issue before the forward transform and complete behind
+  // it (at 4096 they would not fit beside the transform's registers)
+  constexpr bool UPRE = GB != 0 && P <= 2048;
+  typename HO::Type uk;
+  if constexpr (UPRE) {
+    const float2* sp = a.spec + (int64_t)unit * P + t;
+#pragma unroll
+    for (int b = 0; b < HO::BPT; ++b)
+#pragma unroll
+      for (int q = 0; q < HO::R; ++q) uk.v[b][q] = active ? sp[(b * HO::R + q) * TPF] : make_float2(0.f, 0.f);
+  }
   // rows >= N <= P/2 are zero padding, and the rows kept are < N: the first
   // forward pass skips the upper half of its inputs, the last inverse pass
   // the upper half of its outputs
   fft_forward_to_regs<P, TPF, SRC_FN, true>(s, t, load, u);
+  if constexpr (SPEC == 1) {
+    if (active) {
+      float2* sp = a.spec + (int64_t)unit * P + t;
+#pragma unroll
+      for (int b = 0; b < HO::BPT; ++b)
+#pragma unroll
+        for (int q = 0; q < HO::R; ++q) sp[(b * HO::R + q) * TPF] = u.v[b][q];
+    }
+  }
   const float hs = (a.adjoint && a.hstored) ? -1.f : 1.f;   // stored spectrum: conjugate for the adjoint
+  double acc[2] = {0.0, 0.0};
   if (active) {
+    // dg is read in the same loop as the kernel spectrum, at the same Hhat offsets
 #pragma unroll
     for (int b = 0; b < HO::BPT; ++b)
 #pragma unroll
       for (int q = 0; q < HO::R; ++q) {
         const int ky = HO::index(t, b, q);
         const int kk = ky <= P / 2 ? ky : P - ky;
+        const int64_t ho = (int64_t)(hcol >> 2) * (P / 8) * 16 + (hcol & 3) + (kk >> 2) * 16 + (kk & 3) * 4;
         float2 h = kk == P / 2 ? a.hmid[hcol] : hc[(kk >> 2) * 16 + (kk & 3) * 4];
         h.y *= hs;
+        if constexpr (GB != 0) {
+          const float2 uv = UPRE ? uk.v[b][q] : a.spec[(int64_t)unit * P + t + (b * HO::R + q) * TPF];
+          const float2 gv = u.v[b][q];
+          const int64_t di = kk == P / 2 ? (int64_t)(P / 2) * P + hcol : ho;
+#pragma unroll
+          for (int i = 0; i < 2; ++i) {
+            if (!(GB >> i & 1)) continue;
+            const float2 w = cmul(a.dg[i][di], uv);
+            acc[i] += (double)(gv.x * w.x + gv.y * w.y);
+          }
+        }
         u.v[b][q] = cmul(u.v[b][q], h);
       }
   }
@@ -371,6 +456,7 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_col_kernel(RsA
     if (active && r < nr) tc[(r >> 2) * 16 + (r & 3) * 4] = v;
   };
   fft_inverse_from_regs<P, TPF, DST_FN, true>(s, t, u, store);
+  if constexpr (GB != 0) rs_col_dot_sums<TPF, CPB, GB>(a, acc, unit, g, t);
 }
 
 // Per block 2 rows (one per thread group): the row halves of
@@ -485,6 +571,33 @@ __global__ __launch_bounds__(P / 16) void rs_final1_kernel(RsArgs a0) {
   }
 }
 
+// out[i] = sum of the n per-unit partials of gradient i (block i of 1024
+// threads; a null partial array skips its block), fixed order.
+__global__ __launch_bounds__(1024) void rs_grad_reduce_kernel(const double* p0, const double* p1, int n,
+                                                              double* out) {
+  const double* p = blockIdx.x == 0 ? p0 : p1;
+  if (!p) return;   // whole block
+  __shared__ double red[16];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) s += p[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = 0.0;
+    for (int w = 0; w < 16; ++w) v += red[w];
+    out[blockIdx.x] = v;
+  }
+}
+
+// Partial sums written by rs_col_kernel<P, 2> for ncomp components (its
+// padded unit count).
+template <int P> inline int rs_col_units(int ncomp) {
+  constexpr int CPB = Units<P>::CPB;
+  return (P * ncomp + CPB - 1) / CPB * CPB;
+}
+
 // Dynamic-LDS limits of the multi-buffer row kernels (up to three buffers of
 // P points), set once per device by the library's initialisation.
 template <int P> inline hipError_t rs_set_attribute() {
@@ -509,6 +622,22 @@ inline hipError_t rs_launch(const RsArgs& a, hipStream_t st) {
     KTimer t(K_RS_HROW, st);
     hipLaunchKernelGGL(rs_hrow_kernel<P>, dim3(P / 2 / CPB, nb), blk, 0, st, a);
   }
+  const int ncb = (P / 2 + 1 + CPB - 1) / CPB;   // columns kx <= P/2, grid padded to whole XCD rounds
+  if (a.spec_mode == 2) {
+    // derivative spectra (dH/dz, dH/dk) with the spectrum passes, unconjugated
+    for (int i = 0; i < 2; ++i) {
+      if (!a.dg[i]) continue;
+      RsArgs d = a;
+      d.hr = a.dg[i];
+      d.hmid = a.dg[i] + (int64_t)(P / 2) * P;
+      d.kernel_dz = i + 1;
+      d.adjoint = 0;
+      d.hstored = 1;
+      KTimer t(K_RS_HROW, st);
+      hipLaunchKernelGGL(rs_hrow_kernel<P>, dim3(P / 2 / CPB, nb), blk, 0, st, d);
+      hipLaunchKernelGGL(rs_hcol_kernel<P>, dim3((ncb + 7) / 8 * 8, nb), blk, 0, st, d);
+    }
+  }
   if constexpr (P >= 1024) {
     KTimer t(K_RS_UROW, st);
     hipLaunchKernelGGL(rs_urow1_kernel<P>, dim3(rows, nb), dim3(P / 16), 0, st, a);
@@ -519,13 +648,27 @@ inline hipError_t rs_launch(const RsArgs& a, hipStream_t st) {
   }
   if (!a.hreuse) {
     KTimer t(K_RS_HCOL, st);
-    const int ncb = (P / 2 + 1 + CPB - 1) / CPB;   // columns kx <= P/2, grid padded to whole XCD rounds
     hipLaunchKernelGGL(rs_hcol_kernel<P>, dim3((ncb + 7) / 8 * 8, nb), blk, 0, st, a);
   }
   {
     KTimer t(K_RS_COL, st);
-    const int units = P * a.ncomp;
-    hipLaunchKernelGGL(rs_col_kernel<P>, dim3((units + CPB - 1) / CPB, nb), blk, 0, st, a);
+    const dim3 grid(rs_col_units<P>(a.ncomp) / CPB, nb);
+    const int gb = (a.dg[0] ? 1 : 0) | (a.dg[1] ? 2 : 0);
+    if (a.spec_mode == 1)
+      hipLaunchKernelGGL((rs_col_kernel<P, 1>), grid, blk, 0, st, a);
+    else if (a.spec_mode == 2 && gb == 1)
+      hipLaunchKernelGGL((rs_col_kernel<P, 3>), grid, blk, 0, st, a);
+    else if (a.spec_mode == 2 && gb == 2)
+      hipLaunchKernelGGL((rs_col_kernel<P, 4>), grid, blk, 0, st, a);
+    else if (a.spec_mode == 2 && gb == 3)
+      hipLaunchKernelGGL((rs_col_kernel<P, 5>), grid, blk, 0, st, a);
+    else
+      hipLaunchKernelGGL((rs_col_kernel<P, 0>), grid, blk, 0, st, a);
+  }
+  if (a.spec_mode == 2) {
+    KTimer t(K_LOSS, st);
+    hipLaunchKernelGGL(rs_grad_reduce_kernel, dim3(2), dim3(1024), 0, st, a.gpart[0], a.gpart[1],
+                       rs_col_units<P>(a.ncomp), a.grad_out);
   }
   {
     KTimer t(K_RS_FINAL, st);

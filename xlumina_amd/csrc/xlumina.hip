@@ -92,6 +92,7 @@ xl::MaskArgs to_mask(const xl_mask_stage* m) {
   xl::MaskArgs a;
   for (int i = 0; i < 2; ++i) {
     a.st[i].kind = m[i].kind;
+    a.st[i].pf = m[i].forms;
     a.st[i].m0 = m[i].m0;
     a.st[i].m1 = m[i].m1;
     a.st[i].s0 = m[i].s0;
@@ -100,10 +101,17 @@ xl::MaskArgs to_mask(const xl_mask_stage* m) {
   return a;
 }
 
-int check_mask(const xl_mask_stage* m, int nin) {
+int check_mask(const xl_mask_stage* m, int nin, int batch = 1) {
   for (int i = 0; i < 2; ++i) {
     const int k = m[i].kind;
     if (k < XL_MASK_NONE || k > XL_MASK_AMP) return fail(XL_EINVAL, "unknown mask kind");
+    const int pf = m[i].forms;
+    if (pf & ~63) return fail(XL_EINVAL, "unknown mask parameter form bits");
+    for (int j = 0; j < 2; ++j) {
+      const int f = pf >> (3 * j) & 7;
+      if (f && !(f & XL_PF_F64)) return fail(XL_EINVAL, "XL_PF_SCALAR / XL_PF_UNIT_ANGLE need XL_PF_F64");
+    }
+    if (pf && batch > 1) return fail(XL_EINVAL, "float64 mask parameter forms need batch <= 1");
     if (k == XL_MASK_NONE) continue;
     if ((k == XL_MASK_SLM || k == XL_MASK_SSLM || k == XL_MASK_LCD || k == XL_MASK_LP || k == XL_MASK_AMP) &&
         !m[i].m0)
@@ -127,6 +135,29 @@ int64_t rs_ws_elems(int n, int ncomp) {
 
 size_t rs_ws(int n, int ncomp, int batch = 1) {
   return (size_t)std::max(1, batch) * (size_t)rs_ws_elems(n, ncomp) * sizeof(float2) + 256;
+}
+
+// Byte offsets in the spec buffer of spec_mode 1 / 2 (from its 256-aligned
+// start): the ncomp P x P input spectra, two derivative spectra in the Hhat
+// layout, two arrays of per-unit partial sums; [5] is the total.
+struct RsSpecLayout {
+  int64_t off[6];
+  int units;   // padded (column, component) units of the column pass
+};
+RsSpecLayout rs_spec_layout(int n, int ncomp) {
+  const int64_t P = rs_fft_len(n);
+  const int tpf = (int)(P / 16), cpb = tpf >= 64 ? 1 : 64 / tpf;   // Units<P>::CPB
+  RsSpecLayout l;
+  l.units = (int)((P * ncomp + cpb - 1) / cpb * cpb);
+  auto up = [](int64_t b) { return (b + 255) / 256 * 256; };
+  const int64_t hh = ((P / 2) * P + P) * (int64_t)sizeof(float2);
+  l.off[0] = 0;
+  l.off[1] = up(ncomp * P * P * (int64_t)sizeof(float2));
+  l.off[2] = l.off[1] + up(hh);
+  l.off[3] = l.off[2] + up(hh);
+  l.off[4] = l.off[3] + up((int64_t)l.units * 8);
+  l.off[5] = l.off[4] + up((int64_t)l.units * 8);
+  return l;
 }
 
 // ---------------------------------------------------------------- CZT
@@ -258,7 +289,7 @@ char* align256(char* p) { return (char*)(((uintptr_t)p + 255) & ~(uintptr_t)255)
 extern "C" {
 
 const char* xl_last_error(void) { return g_err.c_str(); }
-int xl_abi_version(void) { return 6; }
+int xl_abi_version(void) { return 7; }
 int xl_max_fft_length(void) { return xl::MAX_FFT; }
 int xl_init(void) { return ensure_init(); }
 
@@ -266,6 +297,11 @@ size_t xl_rs_hhat_bytes(int32_t n) {
   if (n < 2) return 0;
   const int64_t P = rs_fft_len(n);
   return (size_t)((P / 2) * P + P) * sizeof(float2);
+}
+
+size_t xl_rs_spec_bytes(int32_t n, int32_t ncomp) {
+  if (n < 2 || ncomp < 1 || ncomp > 3 || rs_fft_len(n) > xl::MAX_FFT) return 0;
+  return (size_t)rs_spec_layout(n, ncomp).off[5] + 256;
 }
 
 size_t xl_rs_workspace_bytes(int32_t n, int32_t ncomp) {
@@ -289,7 +325,7 @@ int xl_rs_propagate(const xl_rs_params* p, void* ws, size_t ws_bytes, void* stre
   if (p->ez_mode != XL_EZ_NONE && p->ez_mode != XL_EZ_VRS) return fail(XL_EINVAL, "RS supports XL_EZ_NONE / XL_EZ_VRS");
   if (p->ez_mode == XL_EZ_VRS && p->nin < 2) return fail(XL_EINVAL, "Ez construction needs Ex and Ey");
   if (!p->in || !p->out || !p->x || !p->y) return fail(XL_EINVAL, "null pointer argument");
-  if (int rc = check_mask(p->mask, p->nin)) return rc;
+  if (int rc = check_mask(p->mask, p->nin, p->batch)) return rc;
   if (p->adjoint && (p->mask[0].kind || p->mask[1].kind || p->ez_mode))
     return fail(XL_EINVAL, "the adjoint convolution takes no masks / Ez construction (use xl_pre_backward)");
   if (p->hhat_mode < 0 || p->hhat_mode > 2) return fail(XL_EINVAL, "hhat_mode must be 0, 1 or 2");
@@ -297,6 +333,11 @@ int xl_rs_propagate(const xl_rs_params* p, void* ws, size_t ws_bytes, void* stre
     return fail(XL_EINVAL, "hhat_mode 1 / 2 needs the hhat buffer and kernel_dz = 0");
   const int ncomp = p->ez_mode == XL_EZ_VRS ? 3 : p->nin;
   if (p->batch < 0 || p->batch > 65535) return fail(XL_EINVAL, "batch must be 0..65535");
+  if (p->spec_mode < 0 || p->spec_mode > 2) return fail(XL_EINVAL, "spec_mode must be 0, 1 or 2");
+  if (p->spec_mode && (!p->spec || p->batch > 1 || p->kernel_dz))
+    return fail(XL_EINVAL, "spec_mode 1 / 2 needs the spec buffer, one problem and kernel_dz = 0");
+  if (p->spec_mode == 2 && (p->spec_grads < 1 || p->spec_grads > 3 || !p->grad_out))
+    return fail(XL_EINVAL, "spec_mode 2 needs spec_grads 1..3 and grad_out");
   const int batch = std::max(1, (int)p->batch);
   const int64_t n2 = (int64_t)n * n;
   if (batch > 1) {
@@ -346,6 +387,21 @@ int xl_rs_propagate(const xl_rs_params* p, void* ws, size_t ws_bytes, void* stre
   a.mask_bs = batch > 1 ? p->mask_batch_stride : 0;
   a.t_bs = rs_ws_elems(n, ncomp);
   a.hr_bs = p->hhat_mode ? (int64_t)(P / 2) * P + P : a.t_bs;   // hhat holds one spectrum per member
+  a.spec_mode = p->spec_mode;
+  a.spec = nullptr;
+  a.dg[0] = a.dg[1] = nullptr;
+  a.gpart[0] = a.gpart[1] = nullptr;
+  a.grad_out = p->grad_out;
+  if (p->spec_mode) {
+    const RsSpecLayout l = rs_spec_layout(n, ncomp);
+    char* sb = align256((char*)p->spec);
+    a.spec = (float2*)sb;
+    for (int i = 0; i < 2; ++i)
+      if (p->spec_mode == 2 && (p->spec_grads >> i & 1)) {
+        a.dg[i] = (float2*)(sb + l.off[1 + i]);
+        a.gpart[i] = (double*)(sb + l.off[3 + i]);
+      }
+  }
   hipStream_t st = (hipStream_t)stream;
   int rc = dispatch_q(P, [&](auto c) { (void)xl::rs_launch<decltype(c)::value>(a, st); });
   if (rc) return rc;
@@ -402,7 +458,7 @@ int xl_czt(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream) {
     return fail(XL_EINVAL, "ncomp must equal nin without Ez construction");
   if (p->adjoint && (p->mask[0].kind || p->mask[1].kind || p->ez_mode || p->nin != p->ncomp))
     return fail(XL_EINVAL, "the adjoint CZT takes ncomp cotangents and no masks / Ez (use xl_pre_backward)");
-  if (int rc = check_mask(p->mask, p->nin)) return rc;
+  if (int rc = check_mask(p->mask, p->nin, p->batch)) return rc;
   const size_t need = cp.tables_bytes + cp.tiles_bytes + cp.ftab_bytes + 768;
   if (!ws || ws_bytes < need) return fail(XL_EWORKSPACE, "CZT workspace too small: need " + std::to_string(need));
   char* w = align256((char*)ws);
@@ -632,7 +688,7 @@ int xl_czt_gemm(const xl_czt_params* p, void* ws, size_t ws_bytes, void* stream)
     return fail(XL_EINVAL, "ncomp must equal nin without Ez construction");
   if (p->adjoint && (p->mask[0].kind || p->mask[1].kind || p->ez_mode || p->nin != p->ncomp))
     return fail(XL_EINVAL, "the adjoint CZT takes ncomp cotangents and no masks / Ez (use xl_pre_backward)");
-  if (int rc = check_mask(p->mask, p->nin)) return rc;
+  if (int rc = check_mask(p->mask, p->nin, p->batch)) return rc;
   const size_t need = gemm_ws(g);
   if (!ws || ws_bytes < need) return fail(XL_EWORKSPACE, "zoom-GEMM workspace too small: need " + std::to_string(need));
   if (!device_ready()) return fail(XL_EINVAL, "xl_init() has not run on this device");
@@ -757,7 +813,7 @@ static int pre_common(const xl_pre_params* p, xl::PreArgs* a) {
   if (p->ez_mode < XL_EZ_NONE || p->ez_mode > XL_EZ_HNA) return fail(XL_EINVAL, "unknown Ez mode");
   if ((p->ez_mode != XL_EZ_NONE || p->hna) && (p->nin < 2 || p->ncomp != 3))
     return fail(XL_EINVAL, "Ez construction / lens map need Ex, Ey and 3 output components");
-  if (int rc = check_mask(p->mask, p->nin)) return rc;
+  if (int rc = check_mask(p->mask, p->nin, p->batch)) return rc;
   std::memset(a, 0, sizeof(*a));
   a->in = (const float2*)p->in;
   a->in_cs = p->in_comp_stride;
@@ -807,10 +863,22 @@ int xl_pre_forward(const xl_pre_params* p, void* stream) {
   return hip_check(hipGetLastError(), "pre_forward launch");
 }
 
+size_t xl_pre_scalar_grad_elems(int32_t rows, int32_t cols) {
+  if (rows < 1 || cols < 1) return 0;
+  return 1 + (size_t)grid_for((int64_t)rows * cols).x;
+}
+
 int xl_pre_backward(const xl_pre_params* p, void* stream) {
   xl::PreArgs a;
   if (int rc = pre_common(p, &a)) return rc;
   if (!p->gout) return fail(XL_EINVAL, "null cotangent");
+  double* sbuf[4] = {nullptr, nullptr, nullptr, nullptr};
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      if (p->gparam[i][j] && (p->mask[i].forms >> (3 * j) & XL_PF_SCALAR)) {
+        a.scal |= 1 << (2 * i + j);
+        sbuf[2 * i + j] = (double*)p->gparam[i][j];
+      }
   const bool any_gparam = p->gparam[0][0] || p->gparam[0][1] || p->gparam[1][0] || p->gparam[1][1];
   if (p->batch > 1 && any_gparam && p->gparam_batch_stride < (int64_t)p->rows * p->cols)
     return fail(XL_EINVAL, "batched mask-parameter gradients need one array per member (gparam_batch_stride >= "
@@ -818,12 +886,18 @@ int xl_pre_backward(const xl_pre_params* p, void* stream) {
   xl::KTimer t(xl::K_PRE_BWD, (hipStream_t)stream);
   const int64_t npix = (int64_t)p->rows * p->cols;
   const bool lone_slm = !p->hna && p->ez_mode == XL_EZ_NONE && p->nin == 1 && p->ncomp == 1 &&
-                        p->mask[0].kind == XL_MASK_SLM && p->mask[1].kind == XL_MASK_NONE && !p->gparam[0][1];
-  if (lone_slm)
+                        p->mask[0].kind == XL_MASK_SLM && p->mask[1].kind == XL_MASK_NONE && !p->gparam[0][1] &&
+                        p->mask[0].forms == 0;
+  if (lone_slm) {
     hipLaunchKernelGGL(xl::pre_slm_backward_kernel, grid_for((npix + xl::kSlmUnroll - 1) / xl::kSlmUnroll, p->batch),
                        dim3(256), 0, (hipStream_t)stream, a);
-  else
-    hipLaunchKernelGGL(xl::pre_backward_kernel, grid_for(npix, p->batch), dim3(256), 0, (hipStream_t)stream, a);
+  } else {
+    const dim3 grid = grid_for(npix, p->batch);
+    hipLaunchKernelGGL(xl::pre_backward_kernel, grid, dim3(256), 0, (hipStream_t)stream, a);
+    if (a.scal)
+      hipLaunchKernelGGL(xl::pgrad_reduce_kernel, dim3(4), dim3(1024), 0, (hipStream_t)stream, sbuf[0], sbuf[1],
+                         sbuf[2], sbuf[3], (int)grid.x);
+  }
   return hip_check(hipGetLastError(), "pre_backward launch");
 }
 
@@ -836,6 +910,7 @@ int xl_slm_backward_adam(const xl_pre_params* p, const xl_adam_params* ad, void*
       p->mask[1].kind != XL_MASK_NONE || p->gparam[0][1] || p->gparam[1][0] || p->gparam[1][1])
     return fail(XL_EINVAL, "the fused SLM adjoint + Adam step takes one component and one XL_MASK_SLM stage");
   if (ad->param != p->mask[0].m0) return fail(XL_EINVAL, "Adam param must be the SLM phase array (mask[0].m0)");
+  if (p->mask[0].forms) return fail(XL_EINVAL, "the fused Adam step takes float32 phases (mask forms 0)");
   if (!ad->m || !ad->v || !ad->count) return fail(XL_EINVAL, "Adam m, v and count are required");
   if (p->batch > 1 && p->mask_batch_stride < (int64_t)p->rows * p->cols)
     return fail(XL_EINVAL, "a batched fused Adam step needs one phase array per member (mask_batch_stride >= "
@@ -1057,8 +1132,9 @@ size_t xl_dz_dot_workspace_bytes(void) { return (size_t)xl::kDzBlocks * sizeof(d
 
 int xl_dz_dot(const xl_dz_dot_params* p, void* ws, size_t ws_bytes, void* stream) {
   if (!p) return fail(XL_EINVAL, "null params");
-  if (!p->d || !p->g || !p->out || p->rows < 1 || p->cols < 1 || p->ncomp < 1 || p->ncomp > 3)
+  if (!p->out || p->rows < 1 || p->cols < 1 || p->ncomp < 0 || p->ncomp > 3 || (p->ncomp && (!p->d || !p->g)))
     return fail(XL_EINVAL, "bad distance-gradient arguments");
+  if (!p->ncomp && !p->pre) return fail(XL_EINVAL, "ncomp = 0 needs pre (the Ez_in term)");
   if (p->pre && (!p->gez || !p->x || !p->y)) return fail(XL_EINVAL, "the Ez_in term needs gez, x and y");
   const size_t need = xl_dz_dot_workspace_bytes();
   if (!ws || ws_bytes < need) return fail(XL_EWORKSPACE, "dz workspace too small: need " + std::to_string(need));

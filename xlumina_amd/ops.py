@@ -20,12 +20,20 @@ from . import _native as N
 
 @dataclass
 class MaskSpec:
-    """One Jones-mask stage applied before a propagation (see XL_MASK_*)."""
+    """One Jones-mask stage applied before a propagation (see XL_MASK_*).
+
+    p0 / p1 are float32 per-pixel arrays, float64 per-pixel arrays (read by
+    the kernels as they are) or one-element float64 tensors (one value for
+    every pixel, read from device memory).  Bit j of `unit` marks parameter j
+    as a raw value p of the angle 2 pi p - pi, the map the reference's
+    optimisers apply (optical_elements.py:1556-1559), formed in the kernels
+    for float64 parameters; gradients then reach the raw values."""
     kind: int
     p0: Optional[torch.Tensor] = None
     p1: Optional[torch.Tensor] = None
     s0: float = 0.0
     s1: float = 0.0
+    unit: int = 0
 
     def tensors(self):
         return (self.p0, self.p1)
@@ -81,12 +89,31 @@ def _prep_param(p, shape, device) -> Optional[torch.Tensor]:
     return p.contiguous()
 
 
-def fill_masks(dst, masks: Sequence[MaskSpec], shape, device, nin: int, batch: Optional[int] = None):
+def _param_with_form(p, shape, device, unit: bool, allow_f64: bool):
+    """(tensor, XL_PF_* form) of one mask parameter: float64 device tensors
+    are passed as they are (per pixel, or one value for every pixel), the
+    rest as float32 per-pixel arrays (a raw angle then mapped here)."""
+    if p is None:
+        return None, 0
+    if allow_f64 and isinstance(p, torch.Tensor) and p.dtype == torch.float64 and p.device == device:
+        if p.numel() == 1:
+            t, f = p.reshape(1), N.XL_PF_F64 | N.XL_PF_SCALAR
+        else:
+            t, f = (p if tuple(p.shape) == tuple(shape) else p.expand(shape)).contiguous(), N.XL_PF_F64
+        return t, f | (N.XL_PF_UNIT_ANGLE if unit else 0)
+    if unit:
+        p = torch.as_tensor(p, dtype=torch.float64, device=device) * (2 * math.pi) - math.pi
+    return _prep_param(p, shape, device), 0
+
+
+def fill_masks(dst, masks: Sequence[MaskSpec], shape, device, nin: int, batch: Optional[int] = None,
+               allow_f64: bool = True):
     """Fills the two xl_mask_stage slots; returns the tensors kept alive.
 
     With `batch`, a parameter may also be a (batch, rows, cols) float32 view
     whose rows are contiguous; every such parameter must share one batch
-    stride, returned as the second value ((keep, mask_batch_stride))."""
+    stride, returned as the second value ((keep, mask_batch_stride)).
+    Batched launches take float32 parameters only (allow_f64 False)."""
     if len(masks) > 2:
         raise ValueError("at most two fused mask stages")
     if batch is not None:
@@ -96,14 +123,16 @@ def fill_masks(dst, masks: Sequence[MaskSpec], shape, device, nin: int, batch: O
         st = dst[i]
         if i >= len(masks) or masks[i] is None:
             st.kind = N.XL_MASK_NONE
+            st.forms = 0
             st.m0 = st.m1 = None
             st.s0 = st.s1 = 0.0
             continue
         m = masks[i]
-        p0 = _prep_param(m.p0, shape, device)
-        p1 = _prep_param(m.p1, shape, device)
+        p0, f0 = _param_with_form(m.p0, shape, device, bool(m.unit & 1), allow_f64)
+        p1, f1 = _param_with_form(m.p1, shape, device, bool(m.unit & 2), allow_f64)
         keep += [p0, p1]
         st.kind = m.kind
+        st.forms = f0 | (f1 << 3)
         st.m0 = _ptr(p0) or None
         st.m1 = _ptr(p1) or None
         st.s0 = float(m.s0)
@@ -133,9 +162,9 @@ def _fill_masks_batched(dst, masks, shape, device, batch):
     if strides and any(kind == "shared" and p is not None for ps in simple for kind, p in ps):
         raise ValueError("mix of batched and shared per-pixel mask parameters")
     flat = [MaskSpec(m.kind, None if isinstance(m.p0, torch.Tensor) and m.p0.dim() == 3 else m.p0,
-                     None if isinstance(m.p1, torch.Tensor) and m.p1.dim() == 3 else m.p1, m.s0, m.s1)
+                     None if isinstance(m.p1, torch.Tensor) and m.p1.dim() == 3 else m.p1, m.s0, m.s1, m.unit)
             for m in masks]
-    keep += fill_masks(dst, flat, shape, device, 1)
+    keep += fill_masks(dst, flat, shape, device, 1, allow_f64=False)
     for i, ps in enumerate(simple):
         for j, (kind, p) in enumerate(ps):
             if kind == "batched":
@@ -233,12 +262,20 @@ def rs_hhat_bytes(n: int) -> int:
     return int(N.load().xl_rs_hhat_bytes(n))
 
 
+def rs_spec_bytes(n: int, ncomp: int) -> int:
+    """Bytes of the spectrum buffer of an n x n, ncomp-component RS propagation
+    whose gradients in z / lambda come from the column pass (spec_mode)."""
+    return int(N.load().xl_rs_spec_bytes(n, ncomp))
+
+
 def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Sequence[MaskSpec] = (),
                  ez_mode: int = N.XL_EZ_NONE, nin: Optional[int] = None, intensity: bool = False,
                  adjoint: bool = False, out: Optional[torch.Tensor] = None,
                  workspace: Optional[torch.Tensor] = None, z_dev: Optional[torch.Tensor] = None,
                  inten: Optional[torch.Tensor] = None, kernel_dz: int = 0,
-                 hhat: Optional[torch.Tensor] = None, hhat_mode: int = 0, noalloc: bool = False):
+                 hhat: Optional[torch.Tensor] = None, hhat_mode: int = 0, noalloc: bool = False,
+                 spec: Optional[torch.Tensor] = None, spec_mode: int = 0, spec_grads: int = 0,
+                 grad_out: Optional[torch.Tensor] = None):
     """RS_propagation_jit / VRS_propagation_jit (wave_optics.py:281-289,
     vectorized_optics.py:364-373) with fused masks and Ez construction.
 
@@ -251,6 +288,12 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
     hhat, hhat_mode: 1 builds the kernel spectrum into `hhat` (a device buffer
     of rs_hhat_bytes(n)) and keeps it; 2 reuses what a mode-1 call of the same
     geometry and distance left there (an adjoint call conjugates it).
+    spec, spec_mode: 1 keeps the spectra of the padded, masked input in `spec`
+    (a device buffer of rs_spec_bytes(n, ncomp)); 2, on the adjoint call of the
+    same geometry and distance, writes grad_out[0] = Re<gout, conv(E', dH/dz)>
+    (spec_grads bit 0) and grad_out[1] = Re<gout, conv(E', dH/dk)> (bit 1) into
+    the float64 device tensor grad_out (2 elements) by Parseval over those
+    spectra, with no derivative propagation.
     noalloc: run on the library's own per-device scratch
     (xl_rs_propagate_noalloc, n <= 1024) instead of a workspace; with `out`
     (and `inten`) given the call allocates nothing.  Such calls share one
@@ -306,6 +349,15 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
         if hhat is None or hhat.device != dev or hhat.numel() * hhat.element_size() < rs_hhat_bytes(n):
             raise ValueError("hhat must be a device buffer of rs_hhat_bytes(n) bytes on the field's device")
         p.hhat_mode, p.hhat = int(hhat_mode), hhat.data_ptr()
+    if spec_mode:
+        if spec is None or spec.device != dev or spec.numel() * spec.element_size() < rs_spec_bytes(n, ncomp):
+            raise ValueError("spec must be a device buffer of rs_spec_bytes(n, ncomp) bytes on the field's device")
+        p.spec_mode, p.spec = int(spec_mode), spec.data_ptr()
+        if spec_mode == 2:
+            if (grad_out is None or grad_out.device != dev or grad_out.dtype != torch.float64
+                    or grad_out.numel() < 2 or not grad_out.is_contiguous()):
+                raise ValueError("grad_out must be a contiguous float64 device tensor of 2 elements")
+            p.spec_grads, p.grad_out = int(spec_grads), grad_out.data_ptr()
     keep = fill_masks(p.mask, masks, (n, n), dev, nin)
     if noalloc:
         N.check(lib.xl_rs_propagate_noalloc(C.byref(p), _stream(dev)))
@@ -686,10 +738,13 @@ def pre_backward(src: torch.Tensor, gout: torch.Tensor, x, y, z: float = 0.0, ma
                  ez_mode: int = N.XL_EZ_NONE, hna: bool = False, nin: Optional[int] = None,
                  need_src: bool = True, need_params=((False, False), (False, False)),
                  hna_radius: float = 0.0, hna_f: float = 0.0, z_dev: Optional[torch.Tensor] = None,
-                 gparams=None):
+                 gparams=None, out: Optional[torch.Tensor] = None):
     """Adjoint of pre_forward: returns (grad of src[:nin], [[g00, g01], [g10, g11]]).
     `gparams` may hold preallocated (rows, cols) float32 outputs in the
-    [[g00, g01], [g10, g11]] slots that are needed."""
+    [[g00, g01], [g10, g11]] slots that are needed.  `out`: a contiguous
+    complex64 (C >= nin, rows, cols) tensor that receives the gradient of src
+    in its first nin planes and zeros in the others (the components the
+    operator does not read); it is then returned whole."""
     lib = N.load()
     src = _prep_field(src)
     gout = _prep_field(gout)
@@ -698,20 +753,40 @@ def pre_backward(src: torch.Tensor, gout: torch.Tensor, x, y, z: float = 0.0, ma
     p, keep = _pre_params(src, x, y, z, masks, ez_mode, hna, ncomp, nin, hna_radius, hna_f, z_dev)
     p.gout = gout.data_ptr()
     rows, cols = src.shape[-2:]
-    gsrc = torch.empty((nin, rows, cols), dtype=torch.complex64, device=src.device) if need_src else None
+    if out is not None and need_src:
+        if (out.dtype != torch.complex64 or out.device != src.device or out.dim() != 3 or out.shape[0] < nin
+                or tuple(out.shape[1:]) != (rows, cols) or not out.is_contiguous()):
+            raise ValueError(f"out must be a contiguous complex64 (>= {nin}, {rows}, {cols}) tensor on the field's device")
+        gsrc = out
+    else:
+        gsrc = torch.empty((nin, rows, cols), dtype=torch.complex64, device=src.device) if need_src else None
     p.out = _ptr(gsrc) or None
     gp = [[None, None], [None, None]]
     for i in range(2):
         for j in range(2):
             if need_params[i][j]:
+                # the gradient of the parameter as stored (see MaskSpec): float32 or float64
+                # per pixel, or for a one-value parameter the sum at element 0 of its buffer
+                form = (p.mask[i].forms >> (3 * j)) & 7
                 given = gparams[i][j] if gparams is not None else None
-                gp[i][j] = given if given is not None else torch.empty((rows, cols), dtype=torch.float32,
-                                                                       device=src.device)
-                p.gparam[i][j] = gp[i][j].data_ptr()
+                if form & N.XL_PF_SCALAR:
+                    buf = torch.empty(int(lib.xl_pre_scalar_grad_elems(rows, cols)), dtype=torch.float64,
+                                      device=src.device)
+                    gp[i][j] = buf[:1]
+                elif given is not None:
+                    if form and given.dtype != torch.float64:
+                        raise ValueError("a float64 mask parameter takes a float64 gradient buffer")
+                    buf = gp[i][j] = given
+                else:
+                    buf = gp[i][j] = torch.empty((rows, cols), dtype=torch.float64 if form else torch.float32,
+                                                 device=src.device)
+                p.gparam[i][j] = buf.data_ptr()
             else:
                 p.gparam[i][j] = None
     N.check(lib.xl_pre_backward(C.byref(p), _stream(src.device)))
     del keep
+    if gsrc is not None and gsrc.shape[0] > nin:
+        gsrc[nin:].zero_()
     return gsrc, gp
 
 
@@ -1004,27 +1079,37 @@ def beam_splitter(a: torch.Tensor, b: Optional[torch.Tensor], coef, want_c: bool
     return c, d
 
 
-def dz_dot(d: torch.Tensor, gout: torch.Tensor, pre: Optional[torch.Tensor] = None,
+def dz_dot(d: Optional[torch.Tensor], gout: Optional[torch.Tensor], pre: Optional[torch.Tensor] = None,
            gez: Optional[torch.Tensor] = None, x=None, y=None, z: float = 0.0,
            z_dev: Optional[torch.Tensor] = None) -> torch.Tensor:
     """0-d float64 device tensor: sum_c Re(conj(d_c) gout_c) over the planes
     of d and gout (C, rows, cols), plus, with `pre` (Ex', Ey') and `gez` (the
     cotangent of the VRS Ez_in), sum Re(conj(gez) Ex') X w + Re(conj(gez) Ey') Y w,
-    w = -z / (X^2 + Y^2 + z^2)^{3/2} (xl_dz_dot: one pass, fixed-order sums)."""
+    w = -z / (X^2 + Y^2 + z^2)^{3/2} (xl_dz_dot: one pass, fixed-order sums).
+    d = gout = None leaves the Ez_in term alone."""
     lib = N.load()
-    d = _gpu_c64(d, "d")
-    g = _gpu_c64(gout, "gout")
-    dev = d.device
-    N.ensure_device(dev)
-    if d.shape != g.shape or d.dim() < 2:
-        raise ValueError("d and gout must have the same (..., rows, cols) shape")
-    rows, cols = d.shape[-2:]
-    d3, g3 = d.reshape(-1, rows, cols), g.reshape(-1, rows, cols)
-    if d3.shape[0] > 3:
-        raise ValueError("at most 3 planes")
     p = N.DzDotParams()
-    p.d, p.g = d3.data_ptr(), g3.data_ptr()
-    p.comp_stride, p.ncomp, p.rows, p.cols = rows * cols, d3.shape[0], rows, cols
+    if d is None:
+        if gout is not None or pre is None:
+            raise ValueError("without d, pass gout=None and pre / gez (the Ez_in term alone)")
+        pre = _gpu_c64(pre, "pre")
+        dev = pre.device
+        N.ensure_device(dev)
+        rows, cols = pre.shape[-2:]
+        p.ncomp, p.rows, p.cols = 0, rows, cols
+    else:
+        d = _gpu_c64(d, "d")
+        g = _gpu_c64(gout, "gout")
+        dev = d.device
+        N.ensure_device(dev)
+        if d.shape != g.shape or d.dim() < 2:
+            raise ValueError("d and gout must have the same (..., rows, cols) shape")
+        rows, cols = d.shape[-2:]
+        d3, g3 = d.reshape(-1, rows, cols), g.reshape(-1, rows, cols)
+        if d3.shape[0] > 3:
+            raise ValueError("at most 3 planes")
+        p.d, p.g = d3.data_ptr(), g3.data_ptr()
+        p.comp_stride, p.ncomp, p.rows, p.cols = rows * cols, d3.shape[0], rows, cols
     if pre is not None:
         pre = _gpu_c64(pre, "pre")
         gez = _gpu_c64(gez, "gez")

@@ -24,12 +24,32 @@ from .wave_optics import ScalarLight, _grid
 from . import wave_optics as _wo
 
 
+class RawAngle:
+    """A mask angle given by its raw value p, the angle being 2 pi p - pi (the
+    map of the reference's optimisers, optical_elements.py:1556-1559).  With
+    p a float64 tensor the map runs inside the mask kernels and gradients
+    reach p directly; sSLM, sSLM_with_amplitude and LCD accept it for their
+    angles."""
+    __slots__ = ("p",)
+
+    def __init__(self, p):
+        self.p = p
+
+
 def _param(p, light):
     if p is None:
         return None
     if isinstance(p, torch.Tensor):
         return p.to(light.device)
     return torch.as_tensor(np.asarray(p, dtype=np.float32), device=light.device)
+
+
+def _angle(p, light):
+    """(tensor, is_raw) of a mask angle: a RawAngle keeps its float64 raw values."""
+    if isinstance(p, RawAngle):
+        t = p.p if isinstance(p.p, torch.Tensor) else torch.as_tensor(np.asarray(p.p, dtype=np.float64))
+        return t.to(light.device), True
+    return _param(p, light), False
 
 
 # -- (1) scalar -------------------------------------------------------------------------
@@ -96,8 +116,9 @@ def jones_sSLM_with_amplitude(alpha, phi, a1, a2):
 
 def sSLM(input_field: VectorizedLight, alpha_array=None, phi_array=None) -> VectorizedLight:
     """optical_elements.py:186-222: diag(e^{i alpha}, e^{i phi}) per pixel, Ez kept."""
-    return input_field._with_mask(MaskSpec(N.XL_MASK_SSLM, _param(alpha_array, input_field),
-                                           _param(phi_array, input_field)))
+    a, ra = _angle(alpha_array, input_field)
+    b, rb = _angle(phi_array, input_field)
+    return input_field._with_mask(MaskSpec(N.XL_MASK_SSLM, a, b, unit=int(ra) | 2 * int(rb)))
 
 
 def sSLM_with_amplitude(input_field: VectorizedLight, alpha_array=None, phi_array=None, A1_array=None,
@@ -110,11 +131,14 @@ def sSLM_with_amplitude(input_field: VectorizedLight, alpha_array=None, phi_arra
 
 def LCD(input_field: VectorizedLight, eta, theta) -> VectorizedLight:
     """optical_elements.py:266-305: retarder with eta/theta constant (floats) or per pixel; Ez kept."""
-    if not isinstance(eta, torch.Tensor) and not isinstance(theta, torch.Tensor) \
-            and np.ndim(eta) == 0 and np.ndim(theta) == 0:
+    plain = lambda v: not isinstance(v, (torch.Tensor, RawAngle)) and np.ndim(v) == 0
+    if plain(eta) and plain(theta):
         spec = MaskSpec(N.XL_MASK_LCD_SCALAR, None, None, float(eta), float(theta))
     else:
-        spec = MaskSpec(N.XL_MASK_LCD, _param(eta, input_field), _param(theta, input_field))
+        # one-value float64 tensors stay one value (read by the kernels from device memory)
+        e, re_ = _angle(eta, input_field)
+        t, rt = _angle(theta, input_field)
+        spec = MaskSpec(N.XL_MASK_LCD, e, t, unit=int(re_) | 2 * int(rt))
     return input_field._with_mask(spec)
 
 
@@ -457,6 +481,50 @@ def vectorized_fluorophores(i_ex, i_dep):
     return fluorescence(i_ex, i_dep)
 
 
+class _ComponentIntensity(torch.autograd.Function):
+    """torch.stack([|E_j[c]|^2 for each field E_j]) of (3, My, Mx) complex64
+    fields, written straight into the stack; backward: each field's
+    cotangent 2 g_j E_j[c] in plane c and zeros in the other two (one fill
+    and one product per field instead of the select / abs / pow / stack
+    backward chain)."""
+
+    @staticmethod
+    def forward(ctx, comp, *fields):
+        out = torch.empty((len(fields),) + tuple(fields[0].shape[-2:]), dtype=torch.float32,
+                          device=fields[0].device)
+        for j, e in enumerate(fields):
+            torch.abs(e[comp], out=out[j])
+            out[j].square_()
+        ctx.comp = comp
+        ctx.save_for_backward(*fields)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        c = ctx.comp
+        g2 = g * 2.0
+        grads = []
+        for j, e in enumerate(ctx.saved_tensors):
+            if not ctx.needs_input_grad[1 + j]:
+                grads.append(None)
+                continue
+            ge = torch.empty(e.shape, dtype=e.dtype, device=e.device)
+            if c == e.shape[0] - 1:
+                ge[:c].zero_()
+            else:
+                for i in range(e.shape[0]):
+                    if i != c:
+                        ge[i].zero_()
+            torch.mul(e[c], g2[j], out=ge[c])
+            grads.append(ge)
+        return (None, *grads)
+
+
+def _ez_intensities(dets):
+    """(6, My, Mx) stack of |Ez|^2 of the detector lights."""
+    return _ComponentIntensity.apply(2, *[d.E for d in dets])
+
+
 def _unit_angle(p):
     return p * (2 * math.pi) - math.pi          # raw (0, 1) -> (-pi, pi)
 
@@ -545,7 +613,7 @@ def hybrid_setup_sharp_focus(ls1, ls2, ls3, ls4, ls5, ls6, parameters, fixed_par
 
     dets = _table_3x3((ls1, ls2, ls3, ls4, ls5, ls6), block, [(b[4], b[5]) for b in blocks], bs, z4, z5,
                       lambda light: VCZT_objective_lens(light, r, f, xout, yout))
-    return torch.stack([d.Ez.abs() ** 2 for d in dets]), dets
+    return _ez_intensities(dets), dets
 
 
 def robust_discovery(ls1, ls2, ls3, ls4, ls5, ls6, parameters, fixed_params, noise_distances, noise_slms, noise_wps,
@@ -584,7 +652,7 @@ def robust_discovery(ls1, ls2, ls3, ls4, ls5, ls6, parameters, fixed_params, noi
 
     dets = _table_3x3((ls1, ls2, ls3, ls4, ls5, ls6), block, zs, bs, z4, z5,
                       lambda light: VCZT_objective_lens(light, r, f, xout, yout))
-    return torch.stack([d.Ez.abs() ** 2 for d in dets])
+    return _ez_intensities(dets)
 
 
 def _table_parameters(parameters, distance_offset):
@@ -595,7 +663,8 @@ def _table_parameters(parameters, distance_offset):
     blocks = []
     for i in range(3):
         ph1, ph2, eta, th, za, zb = P[6 * i: 6 * i + 6]
-        blocks.append((_unit_angle(ph1), _unit_angle(ph2), _unit_angle(eta), _unit_angle(th),
+        # the mask angles stay raw: their map to (-pi, pi) runs in the mask kernels
+        blocks.append((RawAngle(ph1), RawAngle(ph2), RawAngle(eta), RawAngle(th),
                        _distance(za, distance_offset), _distance(zb, distance_offset)))
     bs = [_unit_angle(p) for p in P[18:27]]
     return blocks, bs, _distance(P[27], distance_offset), _distance(P[28], distance_offset)
@@ -634,7 +703,7 @@ def _fixed_parameters(parameters, fixed_params, distance_offset):
     P = _as_tensors(parameters)
     z = [_distance(p, distance_offset) for p in P[0:8]]
     bs = [_unit_angle(p) for p in P[8:17]]
-    wp = [_unit_angle(p) for p in P[17:25]]
+    wp = [RawAngle(p) for p in P[17:25]]
     masks = ((ph11, ph12), (ph2, ph2), (ph3, ph3), (ph41, ph42))
     blocks = [(masks[i][0], masks[i][1], wp[2 * i], wp[2 * i + 1], z[2 * i], z[2 * i + 1]) for i in range(4)]
     return blocks, bs
@@ -691,7 +760,7 @@ def hybrid_setup_fixed_slms(ls1, ls2, ls3, ls4, ls5, ls6, parameters, fixed_para
     blocks, bs = _fixed_parameters(parameters, fixed_params, distance_offset)
     dets = _table_fixed((ls1, ls2, ls3, ls4, ls5, ls6), blocks, bs,
                         lambda light: VCZT_objective_lens(light, r, f, xout, yout))
-    return torch.stack([d.Ez.abs() ** 2 for d in dets]), dets
+    return _ez_intensities(dets), dets
 
 
 def hybrid_setup_fixed_slms_fluorophores(ls1, ls2, ls3, ls4, ls5, ls6, parameters, fixed_params,
@@ -766,7 +835,7 @@ def six_times_six_ansatz(ls1, ls2, ls3, ls4, ls5, ls6, ls7, ls8, ls9, ls10, ls11
             else:
                 dets[i] = olens(c)
         above = below
-    return torch.stack([d.Ez.abs() ** 2 for d in dets])
+    return _ez_intensities(dets)
 
 
 # -- noise for robust (noise-aware) tables (optical_elements.py:2127-2300) ---------------------
