@@ -196,7 +196,8 @@ typedef struct xl_pre_params {
   const double* z_dev;     /* optional device copy of z (see xl_rs_params); with batch > 1 one
                               distance per member, z_dev[b] */
   int32_t batch;           /* independent problems per launch (<= 1: one) */
-  int32_t reserved;
+  int32_t out_planes;      /* backward: planes of `out`; planes nin .. out_planes-1 (the components
+                              of the source the pre-op does not read) receive zeros (<= nin: none) */
   int64_t in_batch_stride;     /* complex elements (0: one input shared by the batch) */
   int64_t gout_batch_stride;   /* complex elements */
   int64_t out_batch_stride;    /* complex elements */
@@ -356,6 +357,8 @@ int xl_small_area(const xl_small_area_params* p, void* workspace, size_t workspa
  * (conj ca, conj da, conj cb, conj db).  c or d may be NULL; with
  * zero_ez = 1 component 2 of each output is set to 0 (the reference builds c
  * and d as new lights, Ez = 0). */
+#define XL_BS_THETA_RAW 4   /* added to theta_mode 1..3: theta is a raw angle parameter */
+
 typedef struct xl_bs_params {
   const void* a;               /* 2 components (Ex, Ey) of npix complex64 */
   const void* b;               /* or NULL */
@@ -369,7 +372,8 @@ typedef struct xl_bs_params {
   int32_t theta_mode;          /* 0: the coefficients above (or coef); 1: BS_symmetric (R, iT, iT, R),
                                   2: BS_symmetric_SI port 0 (R, 0, iT, 0), 3: port 1 (iT, 0, R, 0), with
                                   R = |sin t| - 0.01 |cos t|, T = 0.99 |cos t| formed on the device from
-                                  theta (optical_elements.py:376-383, 394-460) */
+                                  theta (optical_elements.py:376-383, 394-460); + XL_BS_THETA_RAW: theta
+                                  holds the raw parameter p of t = 2 pi p - pi (optical_elements.py:1590) */
   const float* coef;           /* optional device array (ca, cb, da, db as re, im pairs: 8 floats) read
                                   by the kernel instead of the values above (no host round trip when
                                   the coefficients are computed on the device) */
@@ -397,7 +401,8 @@ typedef struct xl_bs_dots_params {
   const double* theta;         /* with theta_mode > 0 (as in xl_bs_params): the splitter angle */
   int32_t theta_mode;
   int32_t reserved;
-  double* dtheta;              /* with theta_mode > 0: dL/dt = Re sum_k S_k dkappa_k/dt (one float64) */
+  double* dtheta;              /* with theta_mode > 0: dL/dt = Re sum_k S_k dkappa_k/dt (one float64);
+                                  with XL_BS_THETA_RAW, dL/dp = 2 pi dL/dt */
 } xl_bs_dots_params;
 size_t xl_bs_dots_workspace_bytes(int64_t npix);
 int xl_bs_dots(const xl_bs_dots_params* p, void* workspace, size_t workspace_bytes, void* stream);

@@ -173,3 +173,55 @@ def test_float64_forms_refused_in_batches(cuda_dev):
     p.batch = 1
     rc = lib.xl_rs_propagate(C.byref(p), ws.data_ptr(), ws.numel(), ops._stream(cuda_dev))
     assert rc == N.XL_EINVAL and b"XL_PF_F64" in lib.xl_last_error()
+
+
+@pytest.mark.parametrize("single", [False, True])
+def test_splitter_raw_angle(cuda_dev, single):
+    """BS_symmetric / BS_symmetric_SI with a raw angle p (t = 2 pi p - pi formed
+    in the kernels, XL_BS_THETA_RAW) against t mapped by torch: the same
+    outputs and the same dL/dp."""
+    from xlumina_amd import optical_elements as OE
+    from xlumina_amd.vectorized_optics import VectorizedLight
+    rng = np.random.default_rng(340 + single)
+    n = 48
+    x = np.linspace(-50.0, 50.0, n)
+
+    def light(seed):
+        L = VectorizedLight(x, x, 0.6, cuda_dev)
+        L.E = field(np.random.default_rng(seed), 3, n, cuda_dev)
+        return L
+
+    a, b = light(1), light(2)
+    w = torch.tensor(rng.random((2, 3, n, n)), dtype=torch.float32, device=cuda_dev)
+
+    def run(raw):
+        p = torch.tensor([0.37], dtype=torch.float64, device=cuda_dev, requires_grad=True)
+        th = OE.RawAngle(p) if raw else p * TWO_PI - math.pi
+        c, d = OE.BS_symmetric_SI(a, th) if single else OE.BS_symmetric(a, b, th)
+        L = (w[0] * c.E.abs() ** 2).sum() + (w[1] * d.E.abs() ** 2).sum()
+        return c.E.detach(), d.E.detach(), torch.autograd.grad(L, p)[0]
+
+    c64, d64, g64 = run(True)
+    c32, d32, g32 = run(False)
+    assert torch.equal(c64, c32) and torch.equal(d64, d32)
+    assert abs(float(g64) - float(g32)) <= 1e-9 * abs(float(g32))
+
+
+def test_vrs_source_cotangent_full_planes(cuda_dev):
+    """The source cotangent of a VRS from a 3-plane field: the pre-op adjoint
+    writes Ex, Ey and zeros for the Ez plane it does not read, equal to the
+    2-plane source's cotangent."""
+    rng = np.random.default_rng(350)
+    n = 80
+    x = np.linspace(-200.0, 200.0, n)
+    e3 = field(rng, 3, n, cuda_dev).requires_grad_(True)
+    e2 = e3.detach()[:2].clone().requires_grad_(True)
+    eta = torch.tensor(rng.random((n, n)), dtype=torch.float64, device=cuda_dev)
+    masks = [MaskSpec(N.XL_MASK_LCD, eta, eta, unit=3)]
+    w = torch.tensor(rng.random((3, n, n)), dtype=torch.float32, device=cuda_dev)
+    g3, = torch.autograd.grad((w * AG.rs(e3, x, x, 900.0, 0.6, masks=masks, ez_mode=N.XL_EZ_VRS, nin=2)
+                               .abs() ** 2).sum(), e3)
+    g2, = torch.autograd.grad((w * AG.rs(e2, x, x, 900.0, 0.6, masks=masks, ez_mode=N.XL_EZ_VRS, nin=2)
+                               .abs() ** 2).sum(), e2)
+    assert g3.shape == (3, n, n) and torch.count_nonzero(g3[2]) == 0
+    assert torch.equal(g3[:2], g2)

@@ -139,3 +139,32 @@ def test_czt_argument_checks_reject_bad_buffers():
         ops.check_czt_args((2, 8, 9), 2, 3, (8, 9), (5, 6), out.to(torch.complex128), cpu)
     with pytest.raises(ValueError):
         ops.check_workspace(torch.empty(16, dtype=torch.uint8), torch.device("meta"))
+
+
+def test_detector_intensity_stack_matches_torch_chain():
+    """The fused |E_j[2]|^2 stack and its backward (cotangent 2 g E in plane 2,
+    zeros in planes 0 and 1) against the select / abs / pow / stack chain."""
+    torch.manual_seed(3)
+    fields = [torch.randn(3, 5, 4, dtype=torch.complex64, requires_grad=True) for _ in range(3)]
+    w = torch.randn(3, 5, 4)
+    got = OE._ComponentIntensity.apply(2, *fields)
+    ref = torch.stack([e[2].abs() ** 2 for e in fields])
+    assert torch.allclose(got, ref, rtol=1e-6, atol=0)
+    g1 = torch.autograd.grad((got * w).sum(), fields)
+    g2 = torch.autograd.grad((ref * w).sum(), fields)
+    for a, b in zip(g1, g2):
+        assert torch.allclose(a, b, rtol=1e-5, atol=1e-6)
+        assert torch.count_nonzero(a[:2]) == 0
+
+
+def test_distances_mapped_together_match_one_by_one():
+    """_distances (one chain of launches for a table's distances) equals the
+    per-distance map (|p| 100 + offset) 1e4 in values and gradients."""
+    raw = [torch.tensor([v], dtype=torch.float64, requires_grad=True) for v in (0.3, -0.7, 0.05, 1.2)]
+    w = torch.tensor([1.0, -2.0, 0.5, 3.0], dtype=torch.float64)
+    a = OE._distances(raw, 8.9)
+    b = [OE._distance(p, 8.9) for p in raw]
+    assert all(x.shape == p.shape and torch.equal(x, y) for x, y, p in zip(a, b, raw))
+    ga = torch.autograd.grad(sum(wi * x.sum() for wi, x in zip(w, a)), raw)
+    gb = torch.autograd.grad(sum(wi * x.sum() for wi, x in zip(w, b)), raw)
+    assert all(torch.equal(x, y) for x, y in zip(ga, gb))

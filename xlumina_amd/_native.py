@@ -66,7 +66,7 @@ class PreParams(C.Structure):
                 ("gparam", (C.c_void_p * 2) * 2), ("rows", C.c_int32), ("cols", C.c_int32),
                 ("ez_mode", C.c_int32), ("hna", C.c_int32), ("x", C.c_void_p), ("y", C.c_void_p),
                 ("z", C.c_double), ("hna_radius", C.c_double), ("hna_f", C.c_double), ("mask", MaskStage * 2),
-                ("z_dev", C.c_void_p), ("batch", C.c_int32), ("reserved", C.c_int32),
+                ("z_dev", C.c_void_p), ("batch", C.c_int32), ("out_planes", C.c_int32),
                 ("in_batch_stride", C.c_int64), ("gout_batch_stride", C.c_int64), ("out_batch_stride", C.c_int64),
                 ("mask_batch_stride", C.c_int64), ("gparam_batch_stride", C.c_int64)]
 

@@ -21,9 +21,17 @@ struct BsArgs {
   int zero_ez;
   const float2* coef;  // optional device copy of (ca, cb, da, db)
   const double* theta; // theta_mode > 0: splitter angle
-  int theta_mode;      // 1 symmetric, 2 / 3 single input at port 0 / 1
+  int theta_mode;      // 1 symmetric, 2 / 3 single input at port 0 / 1; + BS_THETA_RAW: theta holds p, t = 2 pi p - pi
   int adjoint;
 };
+
+constexpr int BS_THETA_RAW = 4;
+
+// The splitter angle of theta_mode: as stored, or 2 pi p - pi of a raw p
+// (two float64 roundings, as the reference's parameter map).
+__device__ __forceinline__ double splitter_angle(int mode, double p) {
+  return (mode & BS_THETA_RAW) ? __dsub_rn(__dmul_rn(p, 6.283185307179586), 3.141592653589793) : p;
+}
 
 // Lossy splitter of angle t: R = |sin t| - 0.01 |cos t|, T = 0.99 |cos t|
 // (optical_elements.py:376-383) and its derivatives, in float64.
@@ -68,7 +76,7 @@ __global__ __launch_bounds__(256) void bs_kernel(BsArgs a) {
   if (a.theta_mode) {   // formed once per block in float64, shared through LDS
     __shared__ float2 k4[4];
     if (threadIdx.x == 0) {
-      splitter_coef(a.theta_mode, a.theta[0], ca, cb, da, db);
+      splitter_coef(a.theta_mode & 3, splitter_angle(a.theta_mode, a.theta[0]), ca, cb, da, db);
       if (a.adjoint) {   // (conj ca, conj da, conj cb, conj db)
         const float2 t = cb;
         ca = make_float2(ca.x, -ca.y);
@@ -183,14 +191,15 @@ __global__ __launch_bounds__(256) void bs_dots_reduce_kernel(BsDotsArgs a, int n
   if (a.dots && threadIdx.x < 8) a.dots[threadIdx.x] = red[threadIdx.x][0];
   if (a.theta_mode && a.dtheta && threadIdx.x == 0) {
     // dL/dt = Re sum_k S_k dkappa_k/dt with S = (conj(gc) a, conj(gc) b, conj(gd) a, conj(gd) b)
-    const SplitterRT k(a.theta[0]);
+    const SplitterRT k(splitter_angle(a.theta_mode, a.theta[0]));
     const double re0 = red[0][0], im0 = red[1][0], im1 = red[3][0];
     const double re2 = red[4][0], im2 = red[5][0], re3 = red[6][0];
+    const int mode = a.theta_mode & 3;
     double g;
-    if (a.theta_mode == 1) g = k.dR * (re0 + re3) - k.dT * (im1 + im2);
-    else if (a.theta_mode == 2) g = k.dR * re0 - k.dT * im2;
+    if (mode == 1) g = k.dR * (re0 + re3) - k.dT * (im1 + im2);
+    else if (mode == 2) g = k.dR * re0 - k.dT * im2;
     else g = -k.dT * im0 + k.dR * re2;
-    a.dtheta[0] = g;
+    a.dtheta[0] = (a.theta_mode & BS_THETA_RAW) ? 6.283185307179586 * g : g;   // dL/dp of a raw angle
   }
 }
 

@@ -25,6 +25,7 @@ struct PreArgs {
   int scal;                // bit 2i+j: gparam[i][j] is a scalar-form gradient (one launch row, blocks <= its
                            // buffer): per-block sums at element 1 + block, reduced into element 0
   int rows, cols, ez_mode, hna;
+  int out_planes;          // backward: planes nin .. out_planes-1 of out receive zeros
   const double* x;
   const double* y;
   double z, hna_radius, hna_f, hna_stm2;
@@ -246,6 +247,7 @@ __global__ __launch_bounds__(256) void pre_backward_kernel(PreArgs a0) {
       a.out[pix] = g0;
       if (a.nin > 1) a.out[a.in_cs + pix] = g1;
       if (a.nin > 2) a.out[2 * a.in_cs + pix] = g2;
+      for (int c = a.nin; c < a.out_planes; ++c) a.out[c * a.in_cs + pix] = make_float2(0.f, 0.f);
     }
   }
   // scalar-form parameters: this block's sum (times 2 pi for PF_UNIT) at

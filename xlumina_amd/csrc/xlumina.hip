@@ -835,6 +835,8 @@ static int pre_common(const xl_pre_params* p, xl::PreArgs* a) {
   a->hna_radius = p->hna_radius;
   a->hna_f = p->hna_f;
   a->hna_stm2 = p->hna_radius * p->hna_radius / (p->hna_radius * p->hna_radius + p->hna_f * p->hna_f);
+  if (p->out_planes < 0 || p->out_planes > 3) return fail(XL_EINVAL, "out_planes must be 0..3");
+  a->out_planes = p->out_planes;
   a->mask = to_mask(p->mask);
   if (p->batch < 0 || p->batch > 65535) return fail(XL_EINVAL, "batch must be 0..65535");
   if (p->batch > 1) {
@@ -887,7 +889,7 @@ int xl_pre_backward(const xl_pre_params* p, void* stream) {
   const int64_t npix = (int64_t)p->rows * p->cols;
   const bool lone_slm = !p->hna && p->ez_mode == XL_EZ_NONE && p->nin == 1 && p->ncomp == 1 &&
                         p->mask[0].kind == XL_MASK_SLM && p->mask[1].kind == XL_MASK_NONE && !p->gparam[0][1] &&
-                        p->mask[0].forms == 0;
+                        p->mask[0].forms == 0 && p->out_planes <= 1;
   if (lone_slm) {
     hipLaunchKernelGGL(xl::pre_slm_backward_kernel, grid_for((npix + xl::kSlmUnroll - 1) / xl::kSlmUnroll, p->batch),
                        dim3(256), 0, (hipStream_t)stream, a);
@@ -1182,7 +1184,8 @@ int xl_beam_splitter(const xl_bs_params* p, void* stream) {
   a.npix = p->npix;
   a.zero_ez = p->zero_ez ? 1 : 0;
   a.coef = (const float2*)p->coef;
-  if (p->theta_mode < 0 || p->theta_mode > 3) return fail(XL_EINVAL, "theta_mode must be 0..3");
+  if (p->theta_mode < 0 || p->theta_mode > 7 || p->theta_mode == XL_BS_THETA_RAW)
+    return fail(XL_EINVAL, "theta_mode must be 0..3, 1..3 + XL_BS_THETA_RAW");
   if (p->theta_mode && !p->theta) return fail(XL_EINVAL, "theta_mode needs the device angle theta");
   a.theta = p->theta;
   a.theta_mode = p->theta_mode;
@@ -1202,8 +1205,9 @@ size_t xl_bs_dots_workspace_bytes(int64_t npix) {
 int xl_bs_dots(const xl_bs_dots_params* p, void* ws, size_t ws_bytes, void* stream) {
   if (!p) return fail(XL_EINVAL, "null params");
   if (!p->a || p->npix < 1 || (!p->dots && !p->dtheta)) return fail(XL_EINVAL, "bad beam-splitter dot arguments");
-  if (p->theta_mode < 0 || p->theta_mode > 3 || (p->theta_mode && (!p->theta || !p->dtheta)))
-    return fail(XL_EINVAL, "theta_mode 1..3 needs theta and dtheta");
+  if (p->theta_mode < 0 || p->theta_mode > 7 || p->theta_mode == XL_BS_THETA_RAW ||
+      (p->theta_mode && (!p->theta || !p->dtheta)))
+    return fail(XL_EINVAL, "theta_mode 1..3 (+ XL_BS_THETA_RAW) needs theta and dtheta");
   const size_t need = xl_bs_dots_workspace_bytes(p->npix);
   if (!ws || ws_bytes < need)
     return fail(XL_EWORKSPACE, "beam-splitter dot workspace too small: need " + std::to_string(need));

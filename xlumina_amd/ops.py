@@ -758,6 +758,7 @@ def pre_backward(src: torch.Tensor, gout: torch.Tensor, x, y, z: float = 0.0, ma
                 or tuple(out.shape[1:]) != (rows, cols) or not out.is_contiguous()):
             raise ValueError(f"out must be a contiguous complex64 (>= {nin}, {rows}, {cols}) tensor on the field's device")
         gsrc = out
+        p.out_planes = out.shape[0]      # the kernel writes zeros beyond nin
     else:
         gsrc = torch.empty((nin, rows, cols), dtype=torch.complex64, device=src.device) if need_src else None
     p.out = _ptr(gsrc) or None
@@ -785,8 +786,6 @@ def pre_backward(src: torch.Tensor, gout: torch.Tensor, x, y, z: float = 0.0, ma
                 p.gparam[i][j] = None
     N.check(lib.xl_pre_backward(C.byref(p), _stream(src.device)))
     del keep
-    if gsrc is not None and gsrc.shape[0] > nin:
-        gsrc[nin:].zero_()
     return gsrc, gp
 
 

@@ -419,6 +419,8 @@ class _SplitterTheta(torch.autograd.Function):
 
 
 def _splitter(a: VectorizedLight, b: Optional[VectorizedLight], theta, mode):
+    if isinstance(theta, RawAngle):   # the kernels map the raw parameter (XL_BS_THETA_RAW)
+        theta, mode = theta.p, mode | 4
     th = _scalar(theta, a.device)
     if th.device != a.device:
         th = th.to(a.device)
@@ -533,6 +535,18 @@ def _distance(p, offset):
     return (torch.abs(p) * 100 + offset) * 1e4  # cm -> um
 
 
+def _distances(ps, offset):
+    """[_distance(p, offset) for p in ps], mapped together: one chain of
+    element-wise launches for all of a table's distances (each result a
+    one-element view of it) where the raw values are one-element tensors on
+    one device."""
+    if not ps or not all(isinstance(p, torch.Tensor) and p.numel() == 1 for p in ps) \
+            or len({p.device for p in ps}) != 1:
+        return [_distance(p, offset) for p in ps]
+    z = (torch.cat([p.reshape(1) for p in ps]).abs() * 100 + offset) * 1e4
+    return [v.reshape(p.shape) for v, p in zip(z.split(1), ps)]
+
+
 def _bs_pair(a, b, theta):
     """BS_symmetric(a, b); where one input port carries no light (a source the
     colour of a two-colour table does not have) the single-input splitter fed
@@ -567,6 +581,7 @@ def _table_3x3(ls, block, zs, bs, z4, z5, lens):
     objective lens map.  Returns the six detector lights."""
     ls1, ls2, ls3, ls4, ls5, ls6 = ls
     (z11, z12), (z21, z22), (z31, z32) = zs
+    s1, s2, s3 = z11 + z12, z21 + z22, z31 + z32   # each used twice
 
     def vrs(light, z):
         return light.VRS_propagation(z)[0]
@@ -574,21 +589,21 @@ def _table_3x3(ls, block, zs, bs, z4, z5, lens):
     c1, d1 = _bs_pair(ls1, ls4, bs[0])
     b2 = vrs(block(c1, 0), z12)
     c2, d2 = _bs_pair(ls2, b2, bs[1])
-    b3 = vrs(c2, z21 + z22)
+    b3 = vrs(c2, s2)
     c3, d3 = _bs_pair(ls3, b3, bs[2])
-    det_1 = lens(vrs(c3, z31 + z32))
+    det_1 = lens(vrs(c3, s3))
     a5, a6 = vrs(d2, z4), vrs(d3, z4)
     c4, d4 = _bs_pair(d1, ls5, bs[3])
-    b5 = vrs(c4, z11 + z12)
+    b5 = vrs(c4, s1)
     c5, d5 = BS_symmetric(a5, b5, bs[4])
     b6 = vrs(block(c5, 1), z22)
     c6, d6 = BS_symmetric(a6, b6, bs[5])
-    det_2 = lens(vrs(c6, z31 + z32))
+    det_2 = lens(vrs(c6, s3))
     a8, a9 = vrs(d5, z5), vrs(d6, z5)
     c7, d7 = _bs_pair(d4, ls6, bs[6])
-    b8 = vrs(c7, z11 + z12)
+    b8 = vrs(c7, s1)
     c8, d8 = BS_symmetric(a8, b8, bs[7])
-    b9 = vrs(c8, z21 + z22)
+    b9 = vrs(c8, s2)
     c9, d9 = BS_symmetric(a9, b9, bs[8])
     det_3 = lens(vrs(block(c9, 2), z32))
     return [det_1, det_2, det_3, lens(d7), lens(d8), lens(d9)]
@@ -642,7 +657,7 @@ def robust_discovery(ls1, ls2, ls3, ls4, ls5, ls6, parameters, fixed_params, noi
                        _unit_angle(q[4]) + noise_wps[2 * i], _unit_angle(q[5]) + noise_wps[2 * i + 1]))
         zs.append((_distance(q[6], distance_offset) + noise_distances[2 * i],
                    _distance(q[7], distance_offset) + noise_distances[2 * i + 1]))
-    bs = [_unit_angle(p) for p in P[24:33]]
+    bs = [RawAngle(p) for p in P[24:33]]
     z4 = _distance(P[33], distance_offset) + noise_distances[6]
     z5 = _distance(P[34], distance_offset) + noise_distances[7]
 
@@ -660,14 +675,14 @@ def _table_parameters(parameters, distance_offset):
     three blocks [phase_1, phase_2, eta, theta, z_1, z_2], nine splitter
     angles, z4, z5, mapped to radians and microns."""
     P = [p if isinstance(p, torch.Tensor) else torch.as_tensor(np.asarray(p, dtype=np.float64)) for p in parameters]
+    z = _distances([P[6 * i + k] for i in range(3) for k in (4, 5)] + [P[27], P[28]], distance_offset)
     blocks = []
     for i in range(3):
-        ph1, ph2, eta, th, za, zb = P[6 * i: 6 * i + 6]
-        # the mask angles stay raw: their map to (-pi, pi) runs in the mask kernels
-        blocks.append((RawAngle(ph1), RawAngle(ph2), RawAngle(eta), RawAngle(th),
-                       _distance(za, distance_offset), _distance(zb, distance_offset)))
-    bs = [_unit_angle(p) for p in P[18:27]]
-    return blocks, bs, _distance(P[27], distance_offset), _distance(P[28], distance_offset)
+        ph1, ph2, eta, th = P[6 * i: 6 * i + 4]
+        # the angles stay raw: their map to (-pi, pi) runs in the mask and splitter kernels
+        blocks.append((RawAngle(ph1), RawAngle(ph2), RawAngle(eta), RawAngle(th), z[2 * i], z[2 * i + 1]))
+    bs = [RawAngle(p) for p in P[18:27]]
+    return blocks, bs, z[6], z[7]
 
 
 def hybrid_setup_fluorophores(ls1, ls2, ls3, ls4, ls5, ls6, parameters, fixed_params, distance_offset=8.9):
@@ -701,8 +716,8 @@ def _fixed_parameters(parameters, fixed_params, distance_offset):
     per-block (phase_1, phase_2, eta, theta, z_1, z_2) and the splitter angles."""
     ph11, ph12, ph2, ph3, ph41, ph42 = fixed_params[4:10]
     P = _as_tensors(parameters)
-    z = [_distance(p, distance_offset) for p in P[0:8]]
-    bs = [_unit_angle(p) for p in P[8:17]]
+    z = _distances(P[0:8], distance_offset)
+    bs = [RawAngle(p) for p in P[8:17]]
     wp = [RawAngle(p) for p in P[17:25]]
     masks = ((ph11, ph12), (ph2, ph2), (ph3, ph3), (ph41, ph42))
     blocks = [(masks[i][0], masks[i][1], wp[2 * i], wp[2 * i + 1], z[2 * i], z[2 * i + 1]) for i in range(4)]
@@ -799,9 +814,9 @@ def six_times_six_ansatz(ls1, ls2, ls3, ls4, ls5, ls6, ls7, ls8, ls9, ls10, ls11
     r, f, xout, yout = fixed_params[:4]
     ph11, ph12, ph21, ph22 = fixed_params[4:8]
     P = _as_tensors(parameters)
-    z = [_distance(p, distance_offset) for p in P[0:12]]
-    bs = [_unit_angle(p) for p in P[12:48]]
-    e1, t1, e2, t2 = [_unit_angle(p) for p in P[48:52]]
+    z = _distances(P[0:12], distance_offset)
+    bs = [RawAngle(p) for p in P[12:48]]
+    e1, t1, e2, t2 = [RawAngle(p) for p in P[48:52]]
     across = [z[0], z[1] + z[2], z[3], z[4] + z[5], z[6]]
     down = z[7:12]
 
