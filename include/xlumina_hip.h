@@ -135,6 +135,13 @@ typedef struct xl_rs_params {
   int32_t spec_grads;      /* spec_mode 2: bit 0 z, bit 1 k */
   void* spec;
   double* grad_out;        /* spec_mode 2: 2 float64 on the device */
+  /* Optional: the quality factor of the distance propagated (z_dev[b] or z),
+   * (sqrt(wl^2 + rmax^2 + 2 wl sqrt(rmax^2 + z^2)) - rmax) / dr_real
+   * (wave_optics.py:187-191), written by the launch itself into
+   * quality_factor[b] (one float64 per member), so that a propagation to a
+   * distance held on the device reports it without a launch of its own. */
+  double* quality_factor;
+  double qf_rmax, qf_dr_real, qf_wavelength;
 } xl_rs_params;
 
 typedef struct xl_czt_params {
@@ -203,6 +210,12 @@ typedef struct xl_pre_params {
   int64_t out_batch_stride;    /* complex elements */
   int64_t mask_batch_stride;   /* float elements (0: shared mask parameters) */
   int64_t gparam_batch_stride; /* float elements */
+  /* backward with XL_EZ_VRS (one problem), optional: xl_pre_scalar_grad_elems(rows, cols)
+   * float64 whose element 0 receives sum Re(conj(g_Ez) Ex') X w + Re(conj(g_Ez) Ey') Y w,
+   * w = -z / (X^2 + Y^2 + z^2)^{3/2}: the distance dependence of the constructed
+   * Ez_in = (Ex' X + Ey' Y)/r (vectorized_optics.py:258-261) through its cotangent, the
+   * term xl_dz_dot forms from `pre` -- here without a second pass over the field. */
+  double* ez_dz;
 } xl_pre_params;
 
 const char* xl_last_error(void);

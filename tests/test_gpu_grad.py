@@ -545,3 +545,27 @@ def test_czt_grad_wrt_wavelength_vs_fd(cuda_dev, form, vectorial):
     h = 3e-8
     fd = (ref(wl0 + h) - ref(wl0 - h)) / (2 * h)
     assert abs(float(g) - fd) <= 2e-3 * abs(fd), (float(g), fd)
+
+
+@pytest.mark.parametrize("stages", [0, 1, 2])
+def test_pre_backward_ez_distance_term(cuda_dev, stages):
+    """The VRS Ez_in distance term formed in the pre-op adjoint's pass (ez_dz)
+    equals xl_dz_dot's `pre` term over the separately masked field, with no,
+    one and two mask stages."""
+    rng = np.random.default_rng(400 + stages)
+    n = 90
+    x = np.linspace(-150.0, 150.0, n)
+    src = torch.tensor(rng.standard_normal((2, n, n)) + 1j * rng.standard_normal((2, n, n)), dtype=torch.complex64,
+                       device=cuda_dev)
+    g = torch.tensor(rng.standard_normal((3, n, n)) + 1j * rng.standard_normal((3, n, n)), dtype=torch.complex64,
+                     device=cuda_dev)
+    t = [torch.tensor(rng.uniform(-np.pi, np.pi, (n, n)), dtype=torch.float32, device=cuda_dev) for _ in range(4)]
+    masks = [MaskSpec(N.XL_MASK_SSLM, t[0], t[1]), MaskSpec(N.XL_MASK_LCD, t[2], t[3])][:stages]
+    z = 700.0
+    buf = torch.full((ops.pre_scalar_grad_elems(n, n),), float("nan"), dtype=torch.float64, device=cuda_dev)
+    ops.pre_backward(src, g, x, x, z, masks=masks, ez_mode=N.XL_EZ_VRS, nin=2, ez_dz=buf)
+    pre = ops.pre_forward(src, x, x, z, masks=masks, ncomp=2, nin=2) if masks else src
+    ref = float(ops.dz_dot(None, None, pre=pre, gez=g[2], x=x, y=x, z=z))
+    # sum of the terms' magnitudes: |g_Ez| (|Ex'| + |Ey'|) max|X w|, |X w| <= 150 z / z^3
+    scale = float((g[2].abs() * pre[:2].abs().sum(0)).sum()) * 150.0 * z / z ** 3
+    assert abs(float(buf[0]) - ref) <= 1e-5 * scale, (float(buf[0]), ref, scale)

@@ -44,7 +44,8 @@ class RsParams(C.Structure):
                 ("hhat", C.c_void_p), ("batch", C.c_int32), ("reserved2", C.c_int32), ("in_batch_stride", C.c_int64),
                 ("out_batch_stride", C.c_int64), ("intensity_batch_stride", C.c_int64),
                 ("mask_batch_stride", C.c_int64), ("spec_mode", C.c_int32), ("spec_grads", C.c_int32),
-                ("spec", C.c_void_p), ("grad_out", C.c_void_p)]
+                ("spec", C.c_void_p), ("grad_out", C.c_void_p), ("quality_factor", C.c_void_p),
+                ("qf_rmax", C.c_double), ("qf_dr_real", C.c_double), ("qf_wavelength", C.c_double)]
 
 
 class CztParams(C.Structure):
@@ -68,7 +69,7 @@ class PreParams(C.Structure):
                 ("z", C.c_double), ("hna_radius", C.c_double), ("hna_f", C.c_double), ("mask", MaskStage * 2),
                 ("z_dev", C.c_void_p), ("batch", C.c_int32), ("out_planes", C.c_int32),
                 ("in_batch_stride", C.c_int64), ("gout_batch_stride", C.c_int64), ("out_batch_stride", C.c_int64),
-                ("mask_batch_stride", C.c_int64), ("gparam_batch_stride", C.c_int64)]
+                ("mask_batch_stride", C.c_int64), ("gparam_batch_stride", C.c_int64), ("ez_dz", C.c_void_p)]
 
 
 class MseParams(C.Structure):

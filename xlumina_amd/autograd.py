@@ -87,6 +87,7 @@ class RSConfig:
     nin: int
     mask_struct: tuple = field(default_factory=tuple)
     z_dev: Optional[torch.Tensor] = None   # float64 device scalar read by the kernels instead of z
+    qf_out: Optional[torch.Tensor] = None  # 0-d float64 device tensor the forward launch fills with the quality factor
 
 
 class _RSFunction(torch.autograd.Function):
@@ -105,7 +106,8 @@ class _RSFunction(torch.autograd.Function):
             spec = torch.empty(ops.rs_spec_bytes(n, ncomp), dtype=torch.uint8, device=src.device)
         out = ops.rs_propagate(src, cfg.x, cfg.y, cfg.z, cfg.wavelength, masks=masks, ez_mode=cfg.ez_mode,
                                nin=cfg.nin, z_dev=cfg.z_dev, hhat=hhat, hhat_mode=1 if hhat is not None else 0,
-                               spec=spec, spec_mode=1 if spec is not None else 0)
+                               spec=spec, spec_mode=1 if spec is not None else 0, quality_factor=cfg.qf_out)
+        cfg.qf_out = None   # the backward's launches report nothing
         ctx.cfg = cfg
         ctx.save_for_backward(src, *[p if p is not None else torch.empty(0) for p in params],
                               hhat if hhat is not None else torch.empty(0),
@@ -138,22 +140,28 @@ class _RSFunction(torch.autograd.Function):
         masks = _rebuild(cfg.mask_struct, params)
         need_params = ((ctx.needs_input_grad[2] and params[0] is not None, ctx.needs_input_grad[3] and params[1] is not None),
                        (ctx.needs_input_grad[4] and params[2] is not None, ctx.needs_input_grad[5] and params[3] is not None))
+        # the VRS Ez_in construction depends on z too: its term comes from the pre-op adjoint's pass
+        ez_dz = None
+        if want_z and cfg.ez_mode == N.XL_EZ_VRS:
+            ez_dz = torch.empty(ops.pre_scalar_grad_elems(*src.shape[-2:]), dtype=torch.float64, device=src.device)
+        if not masks and cfg.ez_mode == N.XL_EZ_NONE:
+            g_src, gp = g_pre, [[None, None], [None, None]]
+        else:
+            g_src, gp = ops.pre_backward(src, g_pre, cfg.x, cfg.y, cfg.z, masks=masks, ez_mode=cfg.ez_mode,
+                                         nin=cfg.nin, need_src=ctx.needs_input_grad[1], need_params=need_params,
+                                         z_dev=cfg.z_dev, out=_full_grad(ctx, src, cfg.nin), ez_dz=ez_dz)
         g_z = None
         if want_z:
+            # dL/dz = Re<g, conv(E', dH/dz)> (+ the Ez_in term, vectorized_optics.py:262-267)
             shape, dtype, device = ctx.z_meta
-            g_z = _rs_z_grad(cfg, src, masks, grads[0], g_pre).to(device=device, dtype=dtype).reshape(shape)
+            g_z = grads[0] if ez_dz is None else grads[0] + ez_dz[0]
+            g_z = g_z.to(device=device, dtype=dtype).reshape(shape)
         g_wl = None
         if want_wl:
             # dL/dlambda = -(k/lambda) Re<g, conv(E', dH/dk)>; Ez_in does not depend on lambda
             shape, dtype, device = ctx.wl_meta
             k = 2 * math.pi / cfg.wavelength
             g_wl = (-(k / cfg.wavelength) * grads[1]).to(device=device, dtype=dtype).reshape(shape)
-        if not masks and cfg.ez_mode == N.XL_EZ_NONE:
-            g_src, gp = g_pre, [[None, None], [None, None]]
-        else:
-            g_src, gp = ops.pre_backward(src, g_pre, cfg.x, cfg.y, cfg.z, masks=masks, ez_mode=cfg.ez_mode,
-                                         nin=cfg.nin, need_src=ctx.needs_input_grad[1], need_params=need_params,
-                                         z_dev=cfg.z_dev, out=_full_grad(ctx, src, cfg.nin))
         if g_src is not None and ctx.needs_input_grad[1]:
             if g_src.shape[0] != src.shape[0]:
                 full = torch.zeros_like(src)
@@ -164,25 +172,15 @@ class _RSFunction(torch.autograd.Function):
         return (None, g_src, *_param_grads(ctx, cfg.mask_struct, params, gp), g_z, g_wl)
 
 
-def _rs_z_grad(cfg: RSConfig, src, masks, g_prop, g_pre):
-    """dL/dz = Re<g, d out/dz>: g_prop = Re<g, conv(E', dH/dz)> (the
-    propagation's own dependence on z, from the adjoint's column pass) plus,
-    for VRS, the z-dependence of the constructed Ez_in = (Ex X + Ey Y)/r,
-    r = sqrt(X^2 + Y^2 + z^2) (vectorized_optics.py:262-267), through its
-    cotangent g_pre[2]."""
-    if cfg.ez_mode != N.XL_EZ_VRS:
-        return g_prop
-    # d Ez_in / dz = -(Ex' X + Ey' Y) z / r^3, through Ez_in's cotangent g_pre[2]
-    pre = ops.pre_forward(src, cfg.x, cfg.y, cfg.z, masks=masks, ncomp=2, nin=cfg.nin) if masks else src[:2]
-    return g_prop + ops.dz_dot(None, None, pre=pre, gez=g_pre[2], x=cfg.x, y=cfg.y, z=cfg.z, z_dev=cfg.z_dev)
-
-
 def rs(src: torch.Tensor, x, y, z, wavelength: float, masks: Sequence[MaskSpec] = (),
-       ez_mode: int = N.XL_EZ_NONE, nin: Optional[int] = None) -> torch.Tensor:
+       ez_mode: int = N.XL_EZ_NONE, nin: Optional[int] = None,
+       quality_factor: Optional[torch.Tensor] = None) -> torch.Tensor:
     """Differentiable RS / VRS propagation of src (C, n, n).  `z` may be a
     float or a one-element tensor; with requires_grad the distance receives
-    dL/dz (convolution with the analytic dH/dz).  A one-element `wavelength`
-    tensor with requires_grad receives dL/dlambda (convolution with dH/dk)."""
+    dL/dz (Parseval over the forward's and the adjoint's spectra with dH/dz).
+    A one-element `wavelength` tensor with requires_grad receives dL/dlambda
+    (the same with dH/dk).  `quality_factor`: optional 0-d float64 device
+    tensor filled by the forward launch (wave_optics.py:187-191)."""
     nin = src.shape[0] if nin is None else nin
     wl_t = wavelength if isinstance(wavelength, torch.Tensor) and wavelength.requires_grad else None
     if isinstance(wavelength, torch.Tensor):
@@ -193,9 +191,10 @@ def rs(src: torch.Tensor, x, y, z, wavelength: float, masks: Sequence[MaskSpec] 
         if z_t.numel() != 1:
             raise ValueError("the propagation distance must be a single value")
         z_dev = z_t.detach().to(device=src.device, dtype=torch.float64).reshape(1)
-        cfg = RSConfig(x, y, 0.0, float(wavelength), ez_mode, nin, _mask_structure(masks), z_dev)
+        cfg = RSConfig(x, y, 0.0, float(wavelength), ez_mode, nin, _mask_structure(masks), z_dev, quality_factor)
     else:
-        cfg = RSConfig(x, y, float(z), float(wavelength), ez_mode, nin, _mask_structure(masks))
+        cfg = RSConfig(x, y, float(z), float(wavelength), ez_mode, nin, _mask_structure(masks),
+                       qf_out=quality_factor)
     return _RSFunction.apply(cfg, src, *_flat_params(masks), z_t, wl_t)
 
 

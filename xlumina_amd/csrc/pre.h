@@ -22,8 +22,9 @@ struct PreArgs {
   float2* out;
   int64_t cs;
   float* gparam[2][2];     // float32 per pixel, or a double* in the float64 parameter forms
-  int scal;                // bit 2i+j: gparam[i][j] is a scalar-form gradient (one launch row, blocks <= its
-                           // buffer): per-block sums at element 1 + block, reduced into element 0
+  int scal;                // bit 2i+j: gparam[i][j] is a scalar-form gradient, bit 4: ezdz is set (one launch
+                           // row, blocks <= its buffer): per-block sums at element 1 + block, reduced into element 0
+  double* ezdz;            // XL_EZ_VRS: sum of Re(conj(g_Ez) Ex') X w + Re(conj(g_Ez) Ey') Y w, w = -z / r^3
   int rows, cols, ez_mode, hna;
   int out_planes;          // backward: planes nin .. out_planes-1 of out receive zeros
   const double* x;
@@ -190,6 +191,7 @@ __global__ __launch_bounds__(256) void pre_backward_kernel(PreArgs a0) {
   const PreArgs a = pre_member(a0, blockIdx.y);
   const int64_t npix = (int64_t)a.rows * a.cols;
   double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+  double acc_ez = 0.0;
   for (int64_t pix = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; pix < npix;
        pix += (int64_t)gridDim.x * blockDim.x) {
     // forward recompute of the stage inputs (the second stage's input only
@@ -222,6 +224,14 @@ __global__ __launch_bounds__(256) void pre_backward_kernel(PreArgs a0) {
         double dx = X / r, dy = Y / r;
         if (a.ez_mode == EZ_VCZT) { dx *= z / r; dy *= z / r; }
         cx = (float)dx; cy = (float)dy;
+        if (a.ezdz && a.ez_mode == EZ_VRS) {
+          // d Ez_in / dz = -(Ex' X + Ey' Y) z / r^3 through Ez_in's cotangent (vectorized_optics.py:258-261),
+          // with (Ex', Ey') the fully masked input (the last stage applied to m)
+          float2 f0 = m0, f1 = m1, f2 = m2;
+          apply_stage(a.mask.st[a.mask.st[1].kind != MASK_NONE ? 1 : 0], pix, f0, f1, f2);
+          const double w = -z / (r * r * r);
+          acc_ez += (double)(re_conj_mul(g2, f0) * (float)(X * w) + re_conj_mul(g2, f1) * (float)(Y * w));
+        }
       }
       g0 = c_add(g0, c_scale(g2, cx));
       g1 = c_add(g1, c_scale(g2, cy));
@@ -253,36 +263,37 @@ __global__ __launch_bounds__(256) void pre_backward_kernel(PreArgs a0) {
   // scalar-form parameters: this block's sum (times 2 pi for PF_UNIT) at
   // element 1 + block of the parameter's buffer; pgrad_reduce_kernel adds them
   if (a.scal) {
-    __shared__ double red[4][4];
+    __shared__ double red[5][4];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int k = 0; k < 5; ++k) {
+      if (!(a.scal >> k & 1)) continue;
+      double v = k < 4 ? acc[k >> 1][k & 1] : acc_ez;
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        if (!(a.scal >> (2 * i + j) & 1)) continue;
-        double v = acc[i][j];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-        if ((threadIdx.x & 63) == 0) red[2 * i + j][threadIdx.x >> 6] = v;
-      }
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = v;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          if (!(a.scal >> (2 * i + j) & 1)) continue;
-          const double v = red[2 * i + j][0] + red[2 * i + j][1] + red[2 * i + j][2] + red[2 * i + j][3];
-          const double sc = (param_form(a.mask.st[i], j) & PF_UNIT) ? 6.283185307179586 : 1.0;
-          reinterpret_cast<double*>(a.gparam[i][j])[1 + blockIdx.x] = sc * v;
+      for (int k = 0; k < 5; ++k) {
+        if (!(a.scal >> k & 1)) continue;
+        const double v = red[k][0] + red[k][1] + red[k][2] + red[k][3];
+        if (k == 4) {
+          a.ezdz[1 + blockIdx.x] = v;
+        } else {
+          const double sc = (param_form(a.mask.st[k >> 1], k & 1) & PF_UNIT) ? 6.283185307179586 : 1.0;
+          reinterpret_cast<double*>(a.gparam[k >> 1][k & 1])[1 + blockIdx.x] = sc * v;
         }
+      }
     }
   }
 }
 
 // buf[i][0] = sum of buf[i][1 .. n] for the scalar-form parameter gradients
-// (block i; null pointers skip), fixed order.
-__global__ __launch_bounds__(1024) void pgrad_reduce_kernel(double* b0, double* b1, double* b2, double* b3, int n) {
-  double* b = blockIdx.x == 0 ? b0 : blockIdx.x == 1 ? b1 : blockIdx.x == 2 ? b2 : b3;
+// and the Ez_in distance term (block i; null pointers skip), fixed order.
+__global__ __launch_bounds__(1024) void pgrad_reduce_kernel(double* b0, double* b1, double* b2, double* b3, double* b4,
+                                                            int n) {
+  double* b = blockIdx.x == 0 ? b0 : blockIdx.x == 1 ? b1 : blockIdx.x == 2 ? b2 : blockIdx.x == 3 ? b3 : b4;
   if (!b) return;   // whole block
   __shared__ double red[16];
   double s = 0.0;

@@ -64,7 +64,14 @@ struct RsArgs {
   float2* dg[2];
   double* gpart[2];
   double* grad_out;       // spec_mode 2: out[i] = sum of gpart[i]
+  double* qf_out;         // optional: quality factor of member b at qf_out[b]
+  double qf_rmax, qf_dr, qf_wl;
 };
+
+// Quality factor of a propagation to z (wave_optics.py:187-191).
+__device__ __forceinline__ double rs_quality_factor(double z, double rmax, double dr_real, double wl) {
+  return (sqrt(wl * wl + rmax * rmax + 2.0 * wl * sqrt(rmax * rmax + z * z)) - rmax) / dr_real;
+}
 
 // The arguments of batch member b (pointers offset by the batch strides).
 __device__ __forceinline__ RsArgs rs_member(const RsArgs& a0, int b) {
@@ -381,6 +388,8 @@ __device__ __forceinline__ void rs_col_dot_sums(const RsArgs& a, const double (&
 template <int P, int SPEC>
 __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_col_kernel(RsArgs a0) {
   const RsArgs a = rs_member(a0, blockIdx.y);
+  if (a.qf_out && blockIdx.x == 0 && threadIdx.x == 0)
+    a.qf_out[blockIdx.y] = rs_quality_factor(a.z_dev ? *a.z_dev : a.z, a.qf_rmax, a.qf_dr, a.qf_wl);
   constexpr int TPF = P / 16, LS = LdsLen<P>::value, CPB = Units<P>::CPB;
   using HO = HandOff<P, TPF>;
   __shared__ float2 lds[CPB * LS];

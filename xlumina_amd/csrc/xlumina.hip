@@ -392,6 +392,11 @@ int xl_rs_propagate(const xl_rs_params* p, void* ws, size_t ws_bytes, void* stre
   a.dg[0] = a.dg[1] = nullptr;
   a.gpart[0] = a.gpart[1] = nullptr;
   a.grad_out = p->grad_out;
+  a.qf_out = p->quality_factor;
+  a.qf_rmax = p->qf_rmax;
+  a.qf_dr = p->qf_dr_real;
+  a.qf_wl = p->qf_wavelength;
+  if (p->quality_factor && !(p->qf_dr_real > 0.0)) return fail(XL_EINVAL, "the quality factor needs qf_dr_real > 0");
   if (p->spec_mode) {
     const RsSpecLayout l = rs_spec_layout(n, ncomp);
     char* sb = align256((char*)p->spec);
@@ -874,13 +879,19 @@ int xl_pre_backward(const xl_pre_params* p, void* stream) {
   xl::PreArgs a;
   if (int rc = pre_common(p, &a)) return rc;
   if (!p->gout) return fail(XL_EINVAL, "null cotangent");
-  double* sbuf[4] = {nullptr, nullptr, nullptr, nullptr};
+  double* sbuf[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j)
       if (p->gparam[i][j] && (p->mask[i].forms >> (3 * j) & XL_PF_SCALAR)) {
         a.scal |= 1 << (2 * i + j);
         sbuf[2 * i + j] = (double*)p->gparam[i][j];
       }
+  if (p->ez_dz) {
+    if (p->ez_mode != XL_EZ_VRS || p->hna || p->batch > 1)
+      return fail(XL_EINVAL, "ez_dz is the XL_EZ_VRS construction's distance term of one problem");
+    a.scal |= 16;
+    a.ezdz = sbuf[4] = p->ez_dz;
+  }
   const bool any_gparam = p->gparam[0][0] || p->gparam[0][1] || p->gparam[1][0] || p->gparam[1][1];
   if (p->batch > 1 && any_gparam && p->gparam_batch_stride < (int64_t)p->rows * p->cols)
     return fail(XL_EINVAL, "batched mask-parameter gradients need one array per member (gparam_batch_stride >= "
@@ -897,8 +908,8 @@ int xl_pre_backward(const xl_pre_params* p, void* stream) {
     const dim3 grid = grid_for(npix, p->batch);
     hipLaunchKernelGGL(xl::pre_backward_kernel, grid, dim3(256), 0, (hipStream_t)stream, a);
     if (a.scal)
-      hipLaunchKernelGGL(xl::pgrad_reduce_kernel, dim3(4), dim3(1024), 0, (hipStream_t)stream, sbuf[0], sbuf[1],
-                         sbuf[2], sbuf[3], (int)grid.x);
+      hipLaunchKernelGGL(xl::pgrad_reduce_kernel, dim3(5), dim3(1024), 0, (hipStream_t)stream, sbuf[0], sbuf[1],
+                         sbuf[2], sbuf[3], sbuf[4], (int)grid.x);
   }
   return hip_check(hipGetLastError(), "pre_backward launch");
 }

@@ -191,6 +191,14 @@ def _qf_grid(x, y):
     return hit
 
 
+def quality_factor_buffer(z) -> Optional[torch.Tensor]:
+    """A 0-d float64 tensor for a propagation launch to fill with the quality
+    factor when z is a device tensor (no launch of its own); None otherwise."""
+    if isinstance(z, torch.Tensor) and z.device.type == "cuda":
+        return torch.empty((), dtype=torch.float64, device=z.device)
+    return None
+
+
 def quality_factor(x, y, wavelength, z):
     """wave_optics.py:187-191 in float64: a float for a float z; for a tensor z
     a tensor on z's device (computed there, so no host synchronisation)."""
@@ -275,7 +283,7 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
                  inten: Optional[torch.Tensor] = None, kernel_dz: int = 0,
                  hhat: Optional[torch.Tensor] = None, hhat_mode: int = 0, noalloc: bool = False,
                  spec: Optional[torch.Tensor] = None, spec_mode: int = 0, spec_grads: int = 0,
-                 grad_out: Optional[torch.Tensor] = None):
+                 grad_out: Optional[torch.Tensor] = None, quality_factor: Optional[torch.Tensor] = None):
     """RS_propagation_jit / VRS_propagation_jit (wave_optics.py:281-289,
     vectorized_optics.py:364-373) with fused masks and Ez construction.
 
@@ -294,6 +302,8 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
     (spec_grads bit 0) and grad_out[1] = Re<gout, conv(E', dH/dk)> (bit 1) into
     the float64 device tensor grad_out (2 elements) by Parseval over those
     spectra, with no derivative propagation.
+    quality_factor: optional 0-d float64 device tensor that the launch fills
+    with the quality factor of the distance propagated (wave_optics.py:187-191).
     noalloc: run on the library's own per-device scratch
     (xl_rs_propagate_noalloc, n <= 1024) instead of a workspace; with `out`
     (and `inten`) given the call allocates nothing.  Such calls share one
@@ -358,6 +368,11 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
                     or grad_out.numel() < 2 or not grad_out.is_contiguous()):
                 raise ValueError("grad_out must be a contiguous float64 device tensor of 2 elements")
             p.spec_grads, p.grad_out = int(spec_grads), grad_out.data_ptr()
+    if quality_factor is not None:
+        check_buffer(quality_factor, (), torch.float64, dev, "quality_factor")
+        dr_real, rmax = _qf_grid(x, y)
+        p.quality_factor, p.qf_rmax, p.qf_dr_real, p.qf_wavelength = (quality_factor.data_ptr(), float(rmax),
+                                                                      float(dr_real), float(wavelength))
     keep = fill_masks(p.mask, masks, (n, n), dev, nin)
     if noalloc:
         N.check(lib.xl_rs_propagate_noalloc(C.byref(p), _stream(dev)))
@@ -738,13 +753,16 @@ def pre_backward(src: torch.Tensor, gout: torch.Tensor, x, y, z: float = 0.0, ma
                  ez_mode: int = N.XL_EZ_NONE, hna: bool = False, nin: Optional[int] = None,
                  need_src: bool = True, need_params=((False, False), (False, False)),
                  hna_radius: float = 0.0, hna_f: float = 0.0, z_dev: Optional[torch.Tensor] = None,
-                 gparams=None, out: Optional[torch.Tensor] = None):
+                 gparams=None, out: Optional[torch.Tensor] = None, ez_dz: Optional[torch.Tensor] = None):
     """Adjoint of pre_forward: returns (grad of src[:nin], [[g00, g01], [g10, g11]]).
     `gparams` may hold preallocated (rows, cols) float32 outputs in the
     [[g00, g01], [g10, g11]] slots that are needed.  `out`: a contiguous
     complex64 (C >= nin, rows, cols) tensor that receives the gradient of src
     in its first nin planes and zeros in the others (the components the
-    operator does not read); it is then returned whole."""
+    operator does not read); it is then returned whole.  `ez_dz` (XL_EZ_VRS):
+    a float64 device buffer of pre_scalar_grad_elems(rows, cols) whose element 0
+    receives the distance dependence of the constructed Ez_in through its
+    cotangent (the `pre` term of dz_dot), formed in the same pass."""
     lib = N.load()
     src = _prep_field(src)
     gout = _prep_field(gout)
@@ -771,8 +789,7 @@ def pre_backward(src: torch.Tensor, gout: torch.Tensor, x, y, z: float = 0.0, ma
                 form = (p.mask[i].forms >> (3 * j)) & 7
                 given = gparams[i][j] if gparams is not None else None
                 if form & N.XL_PF_SCALAR:
-                    buf = torch.empty(int(lib.xl_pre_scalar_grad_elems(rows, cols)), dtype=torch.float64,
-                                      device=src.device)
+                    buf = torch.empty(pre_scalar_grad_elems(rows, cols), dtype=torch.float64, device=src.device)
                     gp[i][j] = buf[:1]
                 elif given is not None:
                     if form and given.dtype != torch.float64:
@@ -784,9 +801,20 @@ def pre_backward(src: torch.Tensor, gout: torch.Tensor, x, y, z: float = 0.0, ma
                 p.gparam[i][j] = buf.data_ptr()
             else:
                 p.gparam[i][j] = None
+    if ez_dz is not None:
+        if (ez_dz.dtype != torch.float64 or ez_dz.device != src.device or not ez_dz.is_contiguous()
+                or ez_dz.numel() < pre_scalar_grad_elems(rows, cols)):
+            raise ValueError("ez_dz must be a contiguous float64 device buffer of pre_scalar_grad_elems(rows, cols)")
+        p.ez_dz = ez_dz.data_ptr()
     N.check(lib.xl_pre_backward(C.byref(p), _stream(src.device)))
     del keep
     return gsrc, gp
+
+
+def pre_scalar_grad_elems(rows: int, cols: int) -> int:
+    """float64 elements of a pre_backward buffer summed over the pixels (a
+    one-value parameter's gradient, the Ez_in distance term)."""
+    return int(N.load().xl_pre_scalar_grad_elems(rows, cols))
 
 
 def uniform_phases(out: torch.Tensor, seeds: torch.Tensor, enable: Optional[torch.Tensor] = None) -> torch.Tensor:

@@ -84,10 +84,13 @@ class VectorizedLight(_GridMixin):
         with one shared kernel -> (light, quality factor)."""
         tic = time.perf_counter()
         src, masks = self._pending()
-        out = AG.rs(src, self.x, self.y, z, self.wavelength, masks=masks, ez_mode=N.XL_EZ_VRS, nin=2)
+        qf = ops.quality_factor_buffer(z)   # a device distance: the propagation launch reports it
+        out = AG.rs(src, self.x, self.y, z, self.wavelength, masks=masks, ez_mode=N.XL_EZ_VRS, nin=2,
+                    quality_factor=qf)
         light = VectorizedLight(self.x, self.y, self.wavelength, self.device)
         light._E = out
-        qf = ops.quality_factor(self.x, self.y, self.wavelength, z)
+        if qf is None:
+            qf = ops.quality_factor(self.x, self.y, self.wavelength, z)
         if _wo.VERBOSE:
             print(f"Time taken to perform one VRS propagation (in seconds): {(time.perf_counter() - tic):.4f}")
         return light, qf

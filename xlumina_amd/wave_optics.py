@@ -149,10 +149,12 @@ class ScalarLight(_GridMixin):
         """wave_optics.py:173-196: Rayleigh-Sommerfeld to distance z -> (light, quality factor)."""
         tic = time.perf_counter()
         src, masks = self._pending()
-        out = AG.rs(src, self.x, self.y, z, self.wavelength, masks=masks)
+        qf = ops.quality_factor_buffer(z)   # a device distance: the propagation launch reports it
+        out = AG.rs(src, self.x, self.y, z, self.wavelength, masks=masks, quality_factor=qf)
         light = ScalarLight(self.x, self.y, self.wavelength, self.device)
         light._field = out[0]
-        qf = ops.quality_factor(self.x, self.y, self.wavelength, z)
+        if qf is None:
+            qf = ops.quality_factor(self.x, self.y, self.wavelength, z)
         if VERBOSE:
             print(f"Time taken to perform one RS propagation (in seconds): {(time.perf_counter() - tic):.4f}")
         return light, qf
