@@ -71,6 +71,26 @@ __device__ __forceinline__ float2 bs_mac(float2 k, float2 v, float2 acc) {
 __global__ __launch_bounds__(256) void bs_kernel(BsArgs a) {
   const float2 zero = make_float2(0.f, 0.f);
   const int comp = blockIdx.y;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (comp == 2) {   // the zeroed Ez planes: no coefficients
+    for (; p < a.npix; p += stride) {
+      if (a.c) a.c[2 * a.ccs + p] = zero;
+      if (a.d) a.d[2 * a.dcs + p] = zero;
+    }
+    return;   // whole block
+  }
+  const float2* ap = a.a + comp * a.acs;
+  const float2* bp = a.b ? a.b + comp * a.bcs : nullptr;
+  float2* cp = a.c ? a.c + comp * a.ccs : nullptr;
+  float2* dp = a.d ? a.d + comp * a.dcs : nullptr;
+  // this thread's first samples are loaded before the block forms its
+  // coefficients, so the loads are in flight behind the float64 set-up
+  float2 va = zero, vb = zero;
+  if (p < a.npix) {
+    va = ap[p];
+    if (bp) vb = bp[p];
+  }
   float2 ca = a.coef ? a.coef[0] : a.ca, cb = a.coef ? a.coef[1] : a.cb;
   float2 da = a.coef ? a.coef[2] : a.da, db = a.coef ? a.coef[3] : a.db;
   if (a.theta_mode) {   // formed once per block in float64, shared through LDS
@@ -89,22 +109,14 @@ __global__ __launch_bounds__(256) void bs_kernel(BsArgs a) {
     __syncthreads();
     ca = k4[0]; cb = k4[1]; da = k4[2]; db = k4[3];
   }
-  if (comp == 2) {
-    for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < a.npix; p += (int64_t)gridDim.x * 256) {
-      if (a.c) a.c[2 * a.ccs + p] = zero;
-      if (a.d) a.d[2 * a.dcs + p] = zero;
-    }
-    return;
-  }
-  const float2* ap = a.a + comp * a.acs;
-  const float2* bp = a.b ? a.b + comp * a.bcs : nullptr;
-  float2* cp = a.c ? a.c + comp * a.ccs : nullptr;
-  float2* dp = a.d ? a.d + comp * a.dcs : nullptr;
-  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < a.npix; p += (int64_t)gridDim.x * 256) {
-    const float2 va = ap[p];
-    const float2 vb = bp ? bp[p] : zero;
+  for (; p < a.npix; p += stride) {
     if (cp) cp[p] = bs_mac(cb, vb, bs_mac(ca, va, zero));
     if (dp) dp[p] = bs_mac(db, vb, bs_mac(da, va, zero));
+    const int64_t q = p + stride;
+    if (q < a.npix) {
+      va = ap[q];
+      vb = bp ? bp[q] : zero;
+    }
   }
 }
 
