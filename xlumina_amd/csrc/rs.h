@@ -528,15 +528,21 @@ __global__ __launch_bounds__(2 * FftThreads<P>::value) void rs_final_kernel(RsAr
 // 128 B line; the blocks of the tile's other rows run next on the same XCD),
 // each last pass left in registers, where the first N samples are stored and
 // |E|^2 is accumulated over the components.
-template <int P>
+// ONE: one (component, row) unit per block (no |E|^2 to accumulate across
+// components), component-major so a tile's four rows still run back to back:
+// ncomp N blocks divide into the resident slots evenly where N row blocks
+// (2.67 rounds at P = 4096) leave a partly idle last round.
+template <int P, bool ONE = false>
 __global__ __launch_bounds__(P / 16) void rs_final1_kernel(RsArgs a0) {
   const RsArgs a = rs_member(a0, blockIdx.y);
   constexpr int TPF = P / 16, LS = LdsLen<P>::value;
   using LP = LastPass<P, TPF, false>;
   __shared__ float2 lds[LS];
   const int t = threadIdx.x;
-  const int row = xcd_remap(blockIdx.x, gridDim.x);
-  if (row >= a.n) return;   // whole block
+  const int unit = xcd_remap(blockIdx.x, gridDim.x);
+  const int c0 = ONE ? unit / a.n : 0;
+  const int row = ONE ? unit - c0 * a.n : unit;
+  if (row >= a.n || c0 >= a.ncomp) return;   // whole block
   const int nrg = (a.n + 3) >> 2, rg = row >> 2, rr = row & 3;
   // output columns < N <= P/2 are slots q < R/2 of the last pass
   constexpr int QH = LP::R / 2;
@@ -546,7 +552,7 @@ __global__ __launch_bounds__(P / 16) void rs_final1_kernel(RsArgs a0) {
 #pragma unroll
     for (int q = 0; q < QH; ++q) inten[b][q] = 0.f;
   NoIO none;
-  for (int c = 0; c < a.ncomp; ++c) {
+  for (int c = c0; c < (ONE ? c0 + 1 : a.ncomp); ++c) {
     const float2* tc = a.t + (int64_t)c * nrg * 4 * P + rg * 16 + rr * 4;
     auto load = [&](int col) -> float2 { return tc[(int64_t)(col >> 2) * nrg * 16 + (col & 3)]; };
     typename LP::Type o, unused;
@@ -562,12 +568,12 @@ __global__ __launch_bounds__(P / 16) void rs_final1_kernel(RsArgs a0) {
         if (col < a.n) {
           const float2 v = o.v[b][q];
           oc[col] = v;
-          inten[b][q] += v.x * v.x + v.y * v.y;
+          if constexpr (!ONE) inten[b][q] += v.x * v.x + v.y * v.y;
         }
       }
     }
   }
-  if (a.intensity) {
+  if (!ONE && a.intensity) {
 #pragma unroll
     for (int b = 0; b < LP::BPT; ++b) {
       if (!LP::valid(t, b)) continue;
@@ -681,8 +687,12 @@ inline hipError_t rs_launch(const RsArgs& a, hipStream_t st) {
   }
   {
     KTimer t(K_RS_FINAL, st);
-    if constexpr (P >= 1024)
-      hipLaunchKernelGGL(rs_final1_kernel<P>, dim3(a.n, nb), dim3(P / 16), 0, st, a);
+    if constexpr (P >= 1024) {
+      if (a.intensity)
+        hipLaunchKernelGGL((rs_final1_kernel<P, false>), dim3(a.n, nb), dim3(P / 16), 0, st, a);
+      else
+        hipLaunchKernelGGL((rs_final1_kernel<P, true>), dim3(a.n * a.ncomp, nb), dim3(P / 16), 0, st, a);
+    }
     else
       hipLaunchKernelGGL(rs_final_kernel<P>, dim3(rows / 2, nb), dim3(2 * FftThreads<P>::value), 0, st, a);
   }
