@@ -187,9 +187,26 @@ __device__ __forceinline__ void stage_backward(const MaskStage& m, int64_t pix, 
   }
 }
 
+// The pixel's row and column (32-bit arithmetic where the plane allows it).
+__device__ __forceinline__ void pix_row_col(int64_t pix, int cols, bool small, int& row, int& col) {
+  if (small) {
+    const int ip = (int)pix;
+    row = ip / cols;
+    col = ip - row * cols;
+  } else {
+    row = (int)(pix / cols);
+    col = (int)(pix - (int64_t)row * cols);
+  }
+}
+
+// MODE (the host's choice from ez_mode / hna): 0 no Ez construction or lens
+// map, 1 an Ez construction, 2 the high-NA lens map; each form compiles only
+// its own branch, which keeps the kernel arguments in scalar registers.
+template <int MODE>
 __global__ __launch_bounds__(256) void pre_backward_kernel(PreArgs a0) {
   const PreArgs a = pre_member(a0, blockIdx.y);
   const int64_t npix = (int64_t)a.rows * a.cols;
+  const bool small = npix < 0x7fffffff;
   double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
   double acc_ez = 0.0;
   for (int64_t pix = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; pix < npix;
@@ -205,15 +222,17 @@ __global__ __launch_bounds__(256) void pre_backward_kernel(PreArgs a0) {
     float2 g0 = a.gout[pix];
     float2 g1 = a.ncomp > 1 ? a.gout[a.cs + pix] : make_float2(0.f, 0.f);
     float2 g2 = a.ncomp > 2 ? a.gout[2 * a.cs + pix] : make_float2(0.f, 0.f);
-    if (a.hna) {
-      const int row = (int)(pix / a.cols), col = (int)(pix % a.cols);
+    if constexpr (MODE == 2) {
+      int row, col;
+      pix_row_col(pix, a.cols, small, row, col);
       float A[3][2];
       hna_map(a.x[col], a.y[row], a.hna_radius, a.hna_f, a.hna_stm2, A);
       const float2 h0 = c_add(c_add(c_scale(g0, A[0][0]), c_scale(g1, A[1][0])), c_scale(g2, A[2][0]));
       const float2 h1 = c_add(c_add(c_scale(g0, A[0][1]), c_scale(g1, A[1][1])), c_scale(g2, A[2][1]));
       g0 = h0; g1 = h1; g2 = make_float2(0.f, 0.f);
-    } else if (a.ez_mode != EZ_NONE) {
-      const int row = (int)(pix / a.cols), col = (int)(pix % a.cols);
+    } else if constexpr (MODE == 1) {
+      int row, col;
+      pix_row_col(pix, a.cols, small, row, col);
       const double X = a.x[col], Y = a.y[row], z = a.z_dev ? *a.z_dev : a.z;
       float cx, cy;
       if (a.ez_mode == EZ_HNA) {

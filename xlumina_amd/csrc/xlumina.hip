@@ -906,7 +906,12 @@ int xl_pre_backward(const xl_pre_params* p, void* stream) {
                        dim3(256), 0, (hipStream_t)stream, a);
   } else {
     const dim3 grid = grid_for(npix, p->batch);
-    hipLaunchKernelGGL(xl::pre_backward_kernel, grid, dim3(256), 0, (hipStream_t)stream, a);
+    if (p->hna)
+      hipLaunchKernelGGL(xl::pre_backward_kernel<2>, grid, dim3(256), 0, (hipStream_t)stream, a);
+    else if (p->ez_mode != XL_EZ_NONE)
+      hipLaunchKernelGGL(xl::pre_backward_kernel<1>, grid, dim3(256), 0, (hipStream_t)stream, a);
+    else
+      hipLaunchKernelGGL(xl::pre_backward_kernel<0>, grid, dim3(256), 0, (hipStream_t)stream, a);
     if (a.scal)
       hipLaunchKernelGGL(xl::pgrad_reduce_kernel, dim3(5), dim3(1024), 0, (hipStream_t)stream, sbuf[0], sbuf[1],
                          sbuf[2], sbuf[3], sbuf[4], (int)grid.x);
