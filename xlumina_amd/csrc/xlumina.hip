@@ -176,7 +176,10 @@ int plan_axis(int m, int M, double f1, double f2, double Dm, bool allow3, AxisPl
   // Bluestein length: the smaller of 2^k and (allow3) 3 * 2^k (768..3072)
   // covering mp; any length >= mp gives the same linear convolution
   int q = std::max(64, next_pow2(mp));
-  for (int q3 = 768; allow3 && q3 <= 3072; q3 *= 2)
+  // a power of two that holds both m and M in its halves runs the zero-half
+  // passes and the staged column I/O, and then beats the shorter 3 * 2^k
+  const bool half = 2 * std::max(m, M) <= q;
+  for (int q3 = 768; allow3 && !half && q3 <= 3072; q3 *= 2)
     if (q3 >= mp && q3 < q) {
       q = q3;
       break;
@@ -214,8 +217,9 @@ int plan_czt(const xl_czt_params* p, CztPlan* cp) {
   if (!p) return fail(XL_EINVAL, "null params");
   if (p->ny_in < 2 || p->nx_in < 2 || p->my < 1 || p->mx < 1) return fail(XL_EINVAL, "bad CZT plane sizes");
   if (!(p->dm != 0.0) || !std::isfinite(p->dm)) return fail(XL_EINVAL, "Dm must be finite and non-zero");
-  // 3 * 2^k lengths pay off in the column pass (y axis); the row pass keeps
-  // power-of-two lengths (measured faster there: three passes instead of four)
+  // 3 * 2^k lengths may serve the column pass (y axis) where the power of two
+  // would not be half empty; the row pass keeps power-of-two lengths
+  // (measured faster there: three passes instead of four)
   int rc = plan_axis(p->nx_in, p->mx, p->fx1, p->fx2, p->dm, false, &cp->x);
   if (rc) return rc;
   rc = plan_axis(p->ny_in, p->my, p->fy1, p->fy2, p->dm, true, &cp->y);
