@@ -68,6 +68,7 @@ extern "C" {
 #define XL_PF_F64 1
 #define XL_PF_SCALAR 2
 #define XL_PF_UNIT_ANGLE 4
+#define XL_RS_SPEC_PREBUILT 4   /* spec_grads bit: the derivative spectra are already in `spec` */
 
 typedef struct xl_mask_stage {
   int32_t kind;
@@ -123,16 +124,19 @@ typedef struct xl_rs_params {
    * is the spectrum of the padded, masked input component c and G_c that of
    * the output cotangent.
    *   spec_mode 1 (forward call): U_c is kept in `spec`
-   *     (xl_rs_spec_bytes(n, ncomp) bytes of device memory);
+   *     (xl_rs_spec_bytes(n, ncomp) bytes of device memory); the derivative
+   *     spectra named by spec_grads (bit 0: dH/dz, bit 1: dH/dk; 0: none) are
+   *     built there too, from the kernel rows the call evaluates for H;
    *   spec_mode 2 (the adjoint call of the same geometry and distance, its
    *     input the output cotangent): the derivative spectra named by
-   *     spec_grads (bit 0: dH/dz, bit 1: dH/dk) are built in `spec` and
+   *     spec_grads are built in `spec` -- or, with XL_RS_SPEC_PREBUILT set,
+   *     taken from there as the mode-1 call with those bits left them -- and
    *     grad_out[0] = Re<g, conv(E', dH/dz)>, grad_out[1] = Re<g, conv(E', dH/dk)>
    *     (float64, fixed summation order) are written for the bits set.
    * The Ez_in construction's own dependence on z is not included (xl_dz_dot
    * with ncomp = 0 adds it). */
   int32_t spec_mode;       /* 0, 1 or 2 */
-  int32_t spec_grads;      /* spec_mode 2: bit 0 z, bit 1 k */
+  int32_t spec_grads;      /* bit 0 z, bit 1 k; spec_mode 2 may add XL_RS_SPEC_PREBUILT */
   void* spec;
   double* grad_out;        /* spec_mode 2: 2 float64 on the device */
   /* Optional: the quality factor of the distance propagated (z_dev[b] or z),
@@ -216,6 +220,8 @@ typedef struct xl_pre_params {
    * Ez_in = (Ex' X + Ey' Y)/r (vectorized_optics.py:258-261) through its cotangent, the
    * term xl_dz_dot forms from `pre` -- here without a second pass over the field. */
   double* ez_dz;
+  const double* ez_dz_base; /* optional: one float64 on the device added to ez_dz[0] (the
+                               propagation's own term, so element 0 holds the whole gradient) */
 } xl_pre_params;
 
 const char* xl_last_error(void);

@@ -444,6 +444,23 @@ def test_rs_spectral_grads_match_derivative_propagation(cuda_dev, n, vectorial):
         ops.rs_propagate(g, x, y, z, 0.6, adjoint=True, hhat=hhat, hhat_mode=2, spec=spec, spec_mode=2,
                          spec_grads=2, grad_out=one)
         assert float(one[0]) == 7.0 and float(one[1]) == float(grads[1])
+        # derivative spectra built by the forward call from its own kernel rows
+        # (each subset of {dH/dz, dH/dk}), taken by the adjoint as prebuilt
+        for bits in (3, 1, 2):
+            spec2 = torch.empty_like(spec)
+            hhat2 = torch.empty_like(hhat)
+            out2 = ops.rs_propagate(src, x, y, z, 0.6, hhat=hhat2, hhat_mode=1, spec=spec2, spec_mode=1,
+                                    spec_grads=bits, **kw)
+            assert float((out2 - out).norm() / out.norm()) <= 1e-6
+            pre = torch.full((2,), 7.0, dtype=torch.float64, device=cuda_dev)
+            ops.rs_propagate(g, x, y, z, 0.6, adjoint=True, hhat=hhat2, hhat_mode=2, spec=spec2, spec_mode=2,
+                             spec_grads=bits | N.XL_RS_SPEC_PREBUILT, grad_out=pre)
+            for i in range(2):
+                if bits >> i & 1:
+                    scale = abs(float(grads[i])) + 1e-30
+                    assert abs(float(pre[i]) - float(grads[i])) <= 1e-6 * scale, (n, z, bits, i, float(pre[i]))
+                else:
+                    assert float(pre[i]) == 7.0
 
 
 def test_dz_dot_ez_term_alone(cuda_dev):
@@ -569,3 +586,7 @@ def test_pre_backward_ez_distance_term(cuda_dev, stages):
     # sum of the terms' magnitudes: |g_Ez| (|Ex'| + |Ey'|) max|X w|, |X w| <= 150 z / z^3
     scale = float((g[2].abs() * pre[:2].abs().sum(0)).sum()) * 150.0 * z / z ** 3
     assert abs(float(buf[0]) - ref) <= 1e-5 * scale, (float(buf[0]), ref, scale)
+    # with a base value (the propagation's own term) added by the same reduction
+    base = torch.tensor([0.375], dtype=torch.float64, device=cuda_dev)
+    ops.pre_backward(src, g, x, x, z, masks=masks, ez_mode=N.XL_EZ_VRS, nin=2, ez_dz=buf, ez_dz_base=base)
+    assert abs(float(buf[0]) - (ref + 0.375)) <= 1e-5 * scale + 1e-15

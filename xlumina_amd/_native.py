@@ -24,6 +24,7 @@ XL_OK, XL_EINVAL, XL_EHIP, XL_EWORKSPACE = range(4)
 XL_MASK_NONE, XL_MASK_SLM, XL_MASK_SSLM, XL_MASK_LCD, XL_MASK_LCD_SCALAR, XL_MASK_LP, XL_MASK_AMP = range(7)
 XL_EZ_NONE, XL_EZ_VRS, XL_EZ_VCZT, XL_EZ_HNA = range(4)
 XL_PF_F64, XL_PF_SCALAR, XL_PF_UNIT_ANGLE = 1, 2, 4
+XL_RS_SPEC_PREBUILT = 4   # spec_grads bit of an adjoint call: the forward built the derivative spectra
 XL_CZT_RS, XL_CZT_HNA, XL_CZT_PLAIN = range(3)
 XL_LOSS_INTENSITY, XL_LOSS_AMPLITUDE, XL_LOSS_PHASE = range(3)
 XL_RS_NOALLOC_MAX_N = 1024
@@ -69,7 +70,8 @@ class PreParams(C.Structure):
                 ("z", C.c_double), ("hna_radius", C.c_double), ("hna_f", C.c_double), ("mask", MaskStage * 2),
                 ("z_dev", C.c_void_p), ("batch", C.c_int32), ("out_planes", C.c_int32),
                 ("in_batch_stride", C.c_int64), ("gout_batch_stride", C.c_int64), ("out_batch_stride", C.c_int64),
-                ("mask_batch_stride", C.c_int64), ("gparam_batch_stride", C.c_int64), ("ez_dz", C.c_void_p)]
+                ("mask_batch_stride", C.c_int64), ("gparam_batch_stride", C.c_int64), ("ez_dz", C.c_void_p),
+                ("ez_dz_base", C.c_void_p)]
 
 
 class MseParams(C.Structure):

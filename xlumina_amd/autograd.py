@@ -101,12 +101,15 @@ class _RSFunction(torch.autograd.Function):
         hhat = spec = None
         if any(ctx.needs_input_grad[1:]):
             hhat = torch.empty(ops.rs_hhat_bytes(n), dtype=torch.uint8, device=src.device)
-        if ctx.needs_input_grad[6] or ctx.needs_input_grad[7]:
+        # the derivative spectra come from the forward's own kernel rows
+        spec_grads = (1 if ctx.needs_input_grad[6] else 0) | (2 if ctx.needs_input_grad[7] else 0)
+        if spec_grads:
             ncomp = 3 if cfg.ez_mode == N.XL_EZ_VRS else cfg.nin
             spec = torch.empty(ops.rs_spec_bytes(n, ncomp), dtype=torch.uint8, device=src.device)
         out = ops.rs_propagate(src, cfg.x, cfg.y, cfg.z, cfg.wavelength, masks=masks, ez_mode=cfg.ez_mode,
                                nin=cfg.nin, z_dev=cfg.z_dev, hhat=hhat, hhat_mode=1 if hhat is not None else 0,
-                               spec=spec, spec_mode=1 if spec is not None else 0, quality_factor=cfg.qf_out)
+                               spec=spec, spec_mode=1 if spec is not None else 0, spec_grads=spec_grads,
+                               quality_factor=cfg.qf_out)
         cfg.qf_out = None   # the backward's launches report nothing
         ctx.cfg = cfg
         ctx.save_for_backward(src, *[p if p is not None else torch.empty(0) for p in params],
@@ -133,7 +136,8 @@ class _RSFunction(torch.autograd.Function):
             grads = torch.empty(2, dtype=torch.float64, device=gout.device)
             g_pre = ops.rs_propagate(gout, cfg.x, cfg.y, cfg.z, cfg.wavelength, adjoint=True, z_dev=cfg.z_dev,
                                      hhat=hhat, hhat_mode=2 if hhat is not None else 0, spec=spec, spec_mode=2,
-                                     spec_grads=(1 if want_z else 0) | (2 if want_wl else 0), grad_out=grads)
+                                     spec_grads=(1 if want_z else 0) | (2 if want_wl else 0) | N.XL_RS_SPEC_PREBUILT,
+                                     grad_out=grads)
         else:
             g_pre = ops.rs_propagate(gout, cfg.x, cfg.y, cfg.z, cfg.wavelength, adjoint=True, z_dev=cfg.z_dev,
                                      hhat=hhat, hhat_mode=2 if hhat is not None else 0)
@@ -149,12 +153,13 @@ class _RSFunction(torch.autograd.Function):
         else:
             g_src, gp = ops.pre_backward(src, g_pre, cfg.x, cfg.y, cfg.z, masks=masks, ez_mode=cfg.ez_mode,
                                          nin=cfg.nin, need_src=ctx.needs_input_grad[1], need_params=need_params,
-                                         z_dev=cfg.z_dev, out=_full_grad(ctx, src, cfg.nin), ez_dz=ez_dz)
+                                         z_dev=cfg.z_dev, out=_full_grad(ctx, src, cfg.nin), ez_dz=ez_dz,
+                                         ez_dz_base=grads[:1] if ez_dz is not None else None)
         g_z = None
         if want_z:
             # dL/dz = Re<g, conv(E', dH/dz)> (+ the Ez_in term, vectorized_optics.py:262-267)
             shape, dtype, device = ctx.z_meta
-            g_z = grads[0] if ez_dz is None else grads[0] + ez_dz[0]
+            g_z = grads[0] if ez_dz is None else ez_dz[0]   # ez_dz[0] holds both terms
             g_z = g_z.to(device=device, dtype=dtype).reshape(shape)
         g_wl = None
         if want_wl:

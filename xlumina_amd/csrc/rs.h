@@ -60,8 +60,9 @@ struct RsArgs {
   // Re(conj(G) dg[i] U) over its column into gpart[i][unit] for each
   // derivative spectrum dg[i] (Hhat layout) that is set.
   int spec_mode;
+  int dg_built;           // spec_mode 2: the derivative spectra were built by the forward call
   float2* spec;
-  float2* dg[2];
+  float2* dg[2];          // spec_mode 1: derivative spectra to build with the kernel spectrum
   double* gpart[2];
   double* grad_out;       // spec_mode 2: out[i] = sum of gpart[i]
   double* qf_out;         // optional: quality factor of member b at qf_out[b]
@@ -154,6 +155,31 @@ __device__ __forceinline__ float2 rs_kernel_dk_value(double X, double Y, double 
   return make_float2(fr * e.x - fi * e.y, fr * e.y + fi * e.x);
 }
 
+// h with its derivatives in z (DM bit 0) and k (DM bit 1) at one lag, the
+// geometry (r, 1/r, the phase factor) formed once for all three; each value
+// is the expression of rs_kernel_value / rs_kernel_dz_value /
+// rs_kernel_dk_value.
+template <int DM>
+__device__ __forceinline__ void rs_kernel_with_derivatives(double X, double Y, double z, double k, float2& h,
+                                                           float2& hz, float2& hk) {
+  const double r2 = X * X + Y * Y + z * z;
+  const double ir = rsqrt_f64(r2), ir2 = ir * ir;
+  const double r = r2 * ir;
+  const double s = z > 0 ? 1.0 : -1.0;
+  const double turns = k * r * 0.15915494309189535;
+  const float2 e = expi_turns(s * turns);
+  auto rot = [&](float fr, float fi) { return make_float2(fr * e.x - fi * e.y, fr * e.y + fi * e.x); };
+  const double f = 0.15915494309189535 * z * ir * ir;
+  h = rot((float)(f * ir), (float)(-f * k));
+  if constexpr ((DM & 1) != 0) {
+    const double z2 = z * z;
+    const double cr = (r2 - 3.0 * z2) * ir2 * ir2 * ir + s * k * k * z2 * ir2 * ir;
+    const double ci = k * (s * z2 - (r2 - 2.0 * z2)) * ir2 * ir2;
+    hz = rot((float)(0.15915494309189535 * cr), (float)(0.15915494309189535 * ci));
+  }
+  if constexpr ((DM & 2) != 0) hk = rot((float)(f * s * k * r), (float)(f * (s - 1.0)));
+}
+
 // Hr tile index of (row r, column c): P/8 row groups
 __device__ __forceinline__ int64_t hr_idx(int r, int c, int P) {
   return ((int64_t)(c >> 2) * (P / 8) + (r >> 2)) * 16 + (r & 3) * 4 + (c & 3);
@@ -170,11 +196,16 @@ template <int P> struct Units {
 
 // One kernel row per group: H of rows 0..P/2-1 generated in float64 phase
 // (the array is even in x: columns above P/2 mirror), row FFT, stored as Hr
-// for kx <= P/2 (the transform of an even row is even).
-template <int P>
+// for kx <= P/2 (the transform of an even row is even).  DM != 0: the rows
+// of the derivative kernels in the bit set DM (bit 0 dH/dz, bit 1 dH/dk) are
+// formed from the same geometry, held in registers while H is transformed,
+// then transformed in turn into dg[i] (a forward call that will be
+// differentiated in z / lambda builds its derivative spectra this way).
+template <int P, int DM = 0>
 __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_hrow_kernel(RsArgs a0) {
   const RsArgs a = rs_member(a0, blockIdx.y);
   constexpr int TPF = P / 16, LS = LdsLen<P>::value, CPB = Units<P>::CPB;
+  constexpr int NCOL = (P / 2 + 1 + TPF - 1) / TPF;   // columns kx <= P/2 per thread
   __shared__ float2 lds[CPB * LS];
   const int g = threadIdx.x / TPF, t = threadIdx.x % TPF;
   float2* s = lds + g * LS;
@@ -182,23 +213,54 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_hrow_kernel(Rs
   const double z = a.z_dev ? *a.z_dev : a.z;
   double dy;
   const bool rowok = lag_of(a.y, r, a.n, P, &dy);
-  for (int c = t; c <= P / 2; c += TPF) {
+  float2 dv[2][NCOL];
+#pragma unroll
+  for (int i = 0; i < NCOL; ++i) {
+    const int c = t + i * TPF;
+    if (c > P / 2) break;
     double dx;
-    float2 v = make_float2(0.f, 0.f);
-    if (rowok && lag_of(a.x, c, a.n, P, &dx))
-      v = a.kernel_dz == 1   ? rs_kernel_dz_value(dx, dy, z, a.k)
-          : a.kernel_dz == 2 ? rs_kernel_dk_value(dx, dy, z, a.k)
-                             : rs_kernel_value(dx, dy, z, a.k);
+    float2 v = make_float2(0.f, 0.f), vz = v, vk = v;
+    if (rowok && lag_of(a.x, c, a.n, P, &dx)) {
+      if constexpr (DM == 0)
+        v = a.kernel_dz == 1   ? rs_kernel_dz_value(dx, dy, z, a.k)
+            : a.kernel_dz == 2 ? rs_kernel_dk_value(dx, dy, z, a.k)
+                               : rs_kernel_value(dx, dy, z, a.k);
+      else
+        rs_kernel_with_derivatives<DM>(dx, dy, z, a.k, v, vz, vk);
+    }
     s[lpad(c)] = v;
     if (c > 0 && c < P / 2) s[lpad(P - c)] = v;
+    dv[0][i] = vz;
+    dv[1][i] = vk;
   }
   __syncthreads();
   NoIO none;
-  auto store = [&](int c, float2 v) {
-    if (c <= P / 2) a.hr[hr_idx(r, c, P)] = v;
-  };
   typename LastPass<P, TPF, true>::Type unused;
-  fft_io<P, TPF, false, SRC_LDS, DST_FN>(s, t, none, store, unused);
+  {
+    auto store = [&](int c, float2 v) {
+      if (c <= P / 2) a.hr[hr_idx(r, c, P)] = v;
+    };
+    fft_io<P, TPF, false, SRC_LDS, DST_FN>(s, t, none, store, unused);
+  }
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    if constexpr (DM == 0) break;
+    if (!(DM >> d & 1)) continue;
+    __syncthreads();   // the last pass of other waves may still read LDS
+#pragma unroll
+    for (int i = 0; i < NCOL; ++i) {
+      const int c = t + i * TPF;
+      if (c > P / 2) break;
+      s[lpad(c)] = dv[d][i];
+      if (c > 0 && c < P / 2) s[lpad(P - c)] = dv[d][i];
+    }
+    __syncthreads();
+    float2* dst = a.dg[d];
+    auto store = [&](int c, float2 v) {
+      if (c <= P / 2) dst[hr_idx(r, c, P)] = v;
+    };
+    fft_io<P, TPF, false, SRC_LDS, DST_FN>(s, t, none, store, unused);
+  }
 }
 
 // One block per input row: the masks / Ez of each pixel are computed once by
@@ -633,12 +695,23 @@ inline hipError_t rs_launch(const RsArgs& a, hipStream_t st) {
   const dim3 blk(CPB * (P / 16));
   const int rows = ((a.n + 3) >> 2) * 4;
   const int nb = a.batch > 1 ? a.batch : 1;
+  // a forward call that keeps its spectra for a gradient builds the derivative
+  // spectra with its own kernel rows (one float64 geometry for all of them)
+  const int dm = (a.dg[0] ? 1 : 0) | (a.dg[1] ? 2 : 0);
+  const bool with_h = a.spec_mode == 1 && dm && !a.hreuse;
   if (!a.hreuse) {
     KTimer t(K_RS_HROW, st);
-    hipLaunchKernelGGL(rs_hrow_kernel<P>, dim3(P / 2 / CPB, nb), blk, 0, st, a);
+    if (with_h && dm == 1)
+      hipLaunchKernelGGL((rs_hrow_kernel<P, 1>), dim3(P / 2 / CPB, nb), blk, 0, st, a);
+    else if (with_h && dm == 2)
+      hipLaunchKernelGGL((rs_hrow_kernel<P, 2>), dim3(P / 2 / CPB, nb), blk, 0, st, a);
+    else if (with_h)
+      hipLaunchKernelGGL((rs_hrow_kernel<P, 3>), dim3(P / 2 / CPB, nb), blk, 0, st, a);
+    else
+      hipLaunchKernelGGL((rs_hrow_kernel<P, 0>), dim3(P / 2 / CPB, nb), blk, 0, st, a);
   }
   const int ncb = (P / 2 + 1 + CPB - 1) / CPB;   // columns kx <= P/2, grid padded to whole XCD rounds
-  if (a.spec_mode == 2) {
+  if ((a.spec_mode == 2 && !a.dg_built) || (a.spec_mode == 1 && dm && !with_h)) {
     // derivative spectra (dH/dz, dH/dk) with the spectrum passes, unconjugated
     for (int i = 0; i < 2; ++i) {
       if (!a.dg[i]) continue;
@@ -649,7 +722,7 @@ inline hipError_t rs_launch(const RsArgs& a, hipStream_t st) {
       d.adjoint = 0;
       d.hstored = 1;
       KTimer t(K_RS_HROW, st);
-      hipLaunchKernelGGL(rs_hrow_kernel<P>, dim3(P / 2 / CPB, nb), blk, 0, st, d);
+      hipLaunchKernelGGL((rs_hrow_kernel<P, 0>), dim3(P / 2 / CPB, nb), blk, 0, st, d);
       hipLaunchKernelGGL(rs_hcol_kernel<P>, dim3((ncb + 7) / 8 * 8, nb), blk, 0, st, d);
     }
   }
@@ -664,6 +737,15 @@ inline hipError_t rs_launch(const RsArgs& a, hipStream_t st) {
   if (!a.hreuse) {
     KTimer t(K_RS_HCOL, st);
     hipLaunchKernelGGL(rs_hcol_kernel<P>, dim3((ncb + 7) / 8 * 8, nb), blk, 0, st, a);
+    for (int i = 0; with_h && i < 2; ++i) {
+      if (!a.dg[i]) continue;
+      RsArgs d = a;   // the derivative row transforms, unconjugated
+      d.hr = a.dg[i];
+      d.hmid = a.dg[i] + (int64_t)(P / 2) * P;
+      d.adjoint = 0;
+      d.hstored = 1;
+      hipLaunchKernelGGL(rs_hcol_kernel<P>, dim3((ncb + 7) / 8 * 8, nb), blk, 0, st, d);
+    }
   }
   {
     KTimer t(K_RS_COL, st);

@@ -340,8 +340,10 @@ int xl_rs_propagate(const xl_rs_params* p, void* ws, size_t ws_bytes, void* stre
   if (p->spec_mode < 0 || p->spec_mode > 2) return fail(XL_EINVAL, "spec_mode must be 0, 1 or 2");
   if (p->spec_mode && (!p->spec || p->batch > 1 || p->kernel_dz))
     return fail(XL_EINVAL, "spec_mode 1 / 2 needs the spec buffer, one problem and kernel_dz = 0");
-  if (p->spec_mode == 2 && (p->spec_grads < 1 || p->spec_grads > 3 || !p->grad_out))
-    return fail(XL_EINVAL, "spec_mode 2 needs spec_grads 1..3 and grad_out");
+  if (p->spec_mode == 1 && (p->spec_grads < 0 || p->spec_grads > 3))
+    return fail(XL_EINVAL, "spec_mode 1 takes spec_grads 0..3");
+  if (p->spec_mode == 2 && ((p->spec_grads & 3) == 0 || p->spec_grads < 0 || p->spec_grads > 7 || !p->grad_out))
+    return fail(XL_EINVAL, "spec_mode 2 needs spec_grads 1..3 (| XL_RS_SPEC_PREBUILT) and grad_out");
   const int batch = std::max(1, (int)p->batch);
   const int64_t n2 = (int64_t)n * n;
   if (batch > 1) {
@@ -394,6 +396,7 @@ int xl_rs_propagate(const xl_rs_params* p, void* ws, size_t ws_bytes, void* stre
   a.spec_mode = p->spec_mode;
   a.spec = nullptr;
   a.dg[0] = a.dg[1] = nullptr;
+  a.dg_built = p->spec_mode == 2 && (p->spec_grads & XL_RS_SPEC_PREBUILT) ? 1 : 0;
   a.gpart[0] = a.gpart[1] = nullptr;
   a.grad_out = p->grad_out;
   a.qf_out = p->quality_factor;
@@ -406,9 +409,9 @@ int xl_rs_propagate(const xl_rs_params* p, void* ws, size_t ws_bytes, void* stre
     char* sb = align256((char*)p->spec);
     a.spec = (float2*)sb;
     for (int i = 0; i < 2; ++i)
-      if (p->spec_mode == 2 && (p->spec_grads >> i & 1)) {
+      if (p->spec_grads >> i & 1) {
         a.dg[i] = (float2*)(sb + l.off[1 + i]);
-        a.gpart[i] = (double*)(sb + l.off[3 + i]);
+        if (p->spec_mode == 2) a.gpart[i] = (double*)(sb + l.off[3 + i]);
       }
   }
   hipStream_t st = (hipStream_t)stream;
@@ -918,7 +921,7 @@ int xl_pre_backward(const xl_pre_params* p, void* stream) {
       hipLaunchKernelGGL(xl::pre_backward_kernel<0>, grid, dim3(256), 0, (hipStream_t)stream, a);
     if (a.scal)
       hipLaunchKernelGGL(xl::pgrad_reduce_kernel, dim3(5), dim3(1024), 0, (hipStream_t)stream, sbuf[0], sbuf[1],
-                         sbuf[2], sbuf[3], sbuf[4], (int)grid.x);
+                         sbuf[2], sbuf[3], sbuf[4], p->ez_dz ? p->ez_dz_base : nullptr, (int)grid.x);
   }
   return hip_check(hipGetLastError(), "pre_backward launch");
 }

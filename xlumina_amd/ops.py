@@ -297,11 +297,14 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
     of rs_hhat_bytes(n)) and keeps it; 2 reuses what a mode-1 call of the same
     geometry and distance left there (an adjoint call conjugates it).
     spec, spec_mode: 1 keeps the spectra of the padded, masked input in `spec`
-    (a device buffer of rs_spec_bytes(n, ncomp)); 2, on the adjoint call of the
-    same geometry and distance, writes grad_out[0] = Re<gout, conv(E', dH/dz)>
-    (spec_grads bit 0) and grad_out[1] = Re<gout, conv(E', dH/dk)> (bit 1) into
-    the float64 device tensor grad_out (2 elements) by Parseval over those
-    spectra, with no derivative propagation.
+    (a device buffer of rs_spec_bytes(n, ncomp)), and builds there the
+    derivative spectra named by spec_grads from the kernel rows it evaluates
+    for H; 2, on the adjoint call of the same geometry and distance, writes
+    grad_out[0] = Re<gout, conv(E', dH/dz)> (spec_grads bit 0) and
+    grad_out[1] = Re<gout, conv(E', dH/dk)> (bit 1) into the float64 device
+    tensor grad_out (2 elements) by Parseval over those spectra, with no
+    derivative propagation; spec_grads | XL_RS_SPEC_PREBUILT takes the
+    derivative spectra the mode-1 call built instead of building them.
     quality_factor: optional 0-d float64 device tensor that the launch fills
     with the quality factor of the distance propagated (wave_optics.py:187-191).
     noalloc: run on the library's own per-device scratch
@@ -362,12 +365,12 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
     if spec_mode:
         if spec is None or spec.device != dev or spec.numel() * spec.element_size() < rs_spec_bytes(n, ncomp):
             raise ValueError("spec must be a device buffer of rs_spec_bytes(n, ncomp) bytes on the field's device")
-        p.spec_mode, p.spec = int(spec_mode), spec.data_ptr()
+        p.spec_mode, p.spec, p.spec_grads = int(spec_mode), spec.data_ptr(), int(spec_grads)
         if spec_mode == 2:
             if (grad_out is None or grad_out.device != dev or grad_out.dtype != torch.float64
                     or grad_out.numel() < 2 or not grad_out.is_contiguous()):
                 raise ValueError("grad_out must be a contiguous float64 device tensor of 2 elements")
-            p.spec_grads, p.grad_out = int(spec_grads), grad_out.data_ptr()
+            p.grad_out = grad_out.data_ptr()
     if quality_factor is not None:
         check_buffer(quality_factor, (), torch.float64, dev, "quality_factor")
         dr_real, rmax = _qf_grid(x, y)
@@ -753,7 +756,8 @@ def pre_backward(src: torch.Tensor, gout: torch.Tensor, x, y, z: float = 0.0, ma
                  ez_mode: int = N.XL_EZ_NONE, hna: bool = False, nin: Optional[int] = None,
                  need_src: bool = True, need_params=((False, False), (False, False)),
                  hna_radius: float = 0.0, hna_f: float = 0.0, z_dev: Optional[torch.Tensor] = None,
-                 gparams=None, out: Optional[torch.Tensor] = None, ez_dz: Optional[torch.Tensor] = None):
+                 gparams=None, out: Optional[torch.Tensor] = None, ez_dz: Optional[torch.Tensor] = None,
+                 ez_dz_base: Optional[torch.Tensor] = None):
     """Adjoint of pre_forward: returns (grad of src[:nin], [[g00, g01], [g10, g11]]).
     `gparams` may hold preallocated (rows, cols) float32 outputs in the
     [[g00, g01], [g10, g11]] slots that are needed.  `out`: a contiguous
@@ -762,7 +766,8 @@ def pre_backward(src: torch.Tensor, gout: torch.Tensor, x, y, z: float = 0.0, ma
     operator does not read); it is then returned whole.  `ez_dz` (XL_EZ_VRS):
     a float64 device buffer of pre_scalar_grad_elems(rows, cols) whose element 0
     receives the distance dependence of the constructed Ez_in through its
-    cotangent (the `pre` term of dz_dot), formed in the same pass."""
+    cotangent (the `pre` term of dz_dot), formed in the same pass, plus the
+    float64 device value `ez_dz_base` (one element) when given."""
     lib = N.load()
     src = _prep_field(src)
     gout = _prep_field(gout)
@@ -806,6 +811,10 @@ def pre_backward(src: torch.Tensor, gout: torch.Tensor, x, y, z: float = 0.0, ma
                 or ez_dz.numel() < pre_scalar_grad_elems(rows, cols)):
             raise ValueError("ez_dz must be a contiguous float64 device buffer of pre_scalar_grad_elems(rows, cols)")
         p.ez_dz = ez_dz.data_ptr()
+        if ez_dz_base is not None:
+            if ez_dz_base.dtype != torch.float64 or ez_dz_base.device != src.device or ez_dz_base.numel() < 1:
+                raise ValueError("ez_dz_base must be a float64 device tensor")
+            p.ez_dz_base = ez_dz_base.data_ptr()
     N.check(lib.xl_pre_backward(C.byref(p), _stream(src.device)))
     del keep
     return gsrc, gp
