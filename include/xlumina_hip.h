@@ -146,6 +146,14 @@ typedef struct xl_rs_params {
    * distance held on the device reports it without a launch of its own. */
   double* quality_factor;
   double qf_rmax, qf_dr_real, qf_wavelength;
+  /* Optional: the derivative spectra of spec_mode 1 / 2 live in this buffer
+   * instead of `spec`: the spec buffer of another call of the same n, ncomp,
+   * x, y, z and k whose spec_grads covered these bits and which built them.  A
+   * spec_mode 1 call then builds none (pair it with hhat_mode 2 on that
+   * call's hhat to share its kernel spectrum too); a spec_mode 2 call needs
+   * XL_RS_SPEC_PREBUILT with it.  Two propagations over one distance share
+   * their kernel and derivative spectra this way. */
+  void* spec_dg;
 } xl_rs_params;
 
 typedef struct xl_czt_params {

@@ -590,3 +590,60 @@ def test_pre_backward_ez_distance_term(cuda_dev, stages):
     base = torch.tensor([0.375], dtype=torch.float64, device=cuda_dev)
     ops.pre_backward(src, g, x, x, z, masks=masks, ez_mode=N.XL_EZ_VRS, nin=2, ez_dz=buf, ez_dz_base=base)
     assert abs(float(buf[0]) - (ref + 0.375)) <= 1e-5 * scale + 1e-15
+
+
+@pytest.mark.parametrize("float_z", [False, True])
+def test_same_distance_propagations_share_spectra(cuda_dev, float_z):
+    """Two differentiated VRS over one distance (one tensor, or one float):
+    the second reuses the first's kernel spectrum and derivative spectra
+    (xl_rs_params.spec_dg).  Outputs and every gradient equal those of two
+    propagations over separate, equal distance tensors, which build their own;
+    the distance gradient of the shared form is the sum of the two."""
+    n = 96
+    x, y = R.space(150.0, n)
+    rng = np.random.default_rng(41)
+
+    def c64(*shape):
+        return torch.tensor(rng.standard_normal(shape) + 1j * rng.standard_normal(shape), dtype=torch.complex64,
+                            device=cuda_dev)
+
+    a, b = c64(2, n, n), c64(2, n, n)
+    eta = torch.tensor(rng.uniform(0, 2 * np.pi, (n, n)), dtype=torch.float32, device=cuda_dev)
+    th = torch.tensor(rng.uniform(0, 2 * np.pi, (n, n)), dtype=torch.float32, device=cuda_dev)
+    w = torch.tensor(rng.random((2, n, n)), dtype=torch.float32, device=cuda_dev)
+    z0 = 700.0
+
+    def run(z1, z2, share=True):
+        AG._SHARED_SPECTRA.clear()
+        sa, sb = a.clone().requires_grad_(True), b.clone().requires_grad_(True)
+        e = eta.clone().requires_grad_(True)
+        masks = [MaskSpec(N.XL_MASK_LCD, e, th)]
+        o1 = AG.rs(sa, x, y, z1, 0.65, masks=masks, ez_mode=N.XL_EZ_VRS, nin=2)
+        if not share:
+            AG._SHARED_SPECTRA.clear()   # the second builds its own spectra
+        o2 = AG.rs(sb, x, y, z2, 0.65, masks=masks, ez_mode=N.XL_EZ_VRS, nin=2)
+        entries = len(AG._SHARED_SPECTRA)
+        L = (w[0] * (o1.abs() ** 2).sum(0)).sum() + (w[1] * (o2.abs() ** 2).sum(0)).sum()
+        wrt = [sa, sb, e] + [t for t in (z1, z2) if isinstance(t, torch.Tensor)]
+        grads = torch.autograd.grad(L, wrt)
+        return o1.detach(), o2.detach(), grads, entries
+
+    if float_z:
+        s1, s2, gs, entries = run(z0, z0)
+        u1, u2, gu, _ = run(z0, z0, share=False)
+        assert entries == 1
+        for p, q in zip(gs, gu):
+            assert torch.equal(p, q)
+    else:
+        z = torch.tensor(z0, dtype=torch.float64, device=cuda_dev, requires_grad=True)
+        s1, s2, gs, entries = run(z, z)
+        assert entries == 1   # the second propagation found the first's spectra
+        za = torch.tensor(z0, dtype=torch.float64, device=cuda_dev, requires_grad=True)
+        zb = torch.tensor(z0, dtype=torch.float64, device=cuda_dev, requires_grad=True)
+        u1, u2, gu, entries_u = run(za, zb)
+        assert entries_u == 2   # separate tensors: each builds its own
+        for p, q in zip(gs[:3], gu[:3]):
+            assert torch.equal(p, q)
+        ref = float(gu[3]) + float(gu[4])
+        assert abs(float(gs[3]) - ref) <= 1e-12 * abs(ref), (float(gs[3]), ref)
+    assert torch.equal(s1, u1) and torch.equal(s2, u2)

@@ -46,7 +46,8 @@ class RsParams(C.Structure):
                 ("out_batch_stride", C.c_int64), ("intensity_batch_stride", C.c_int64),
                 ("mask_batch_stride", C.c_int64), ("spec_mode", C.c_int32), ("spec_grads", C.c_int32),
                 ("spec", C.c_void_p), ("grad_out", C.c_void_p), ("quality_factor", C.c_void_p),
-                ("qf_rmax", C.c_double), ("qf_dr_real", C.c_double), ("qf_wavelength", C.c_double)]
+                ("qf_rmax", C.c_double), ("qf_dr_real", C.c_double), ("qf_wavelength", C.c_double),
+                ("spec_dg", C.c_void_p)]
 
 
 class CztParams(C.Structure):

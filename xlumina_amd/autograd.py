@@ -14,6 +14,7 @@ recomputed in the backward pass, so a chain keeps one field per stage alive.
 from __future__ import annotations
 
 import math
+import weakref
 from dataclasses import dataclass, field
 from typing import Optional, Sequence, Tuple
 
@@ -90,6 +91,22 @@ class RSConfig:
     qf_out: Optional[torch.Tensor] = None  # 0-d float64 device tensor the forward launch fills with the quality factor
 
 
+# Kernel and derivative spectra of the differentiated forward propagations,
+# by geometry and distance: a later propagation over the same distance (one
+# distance tensor, unchanged, or one float) reuses them instead of building
+# its own.  An entry holds its key's tensors (so their addresses cannot be
+# reused by others while it lives) and weak references to the spectra, which
+# live as long as the first propagation's saved state.
+_SHARED_SPECTRA: dict = {}
+
+
+def _spectra_key(cfg: RSConfig, z_t, n: int, ncomp: int, dev):
+    xd, yd = ops.device_grid(cfg.x, dev), ops.device_grid(cfg.y, dev)
+    zk = ("t", z_t.data_ptr(), z_t._version, str(z_t.device)) if z_t is not None else ("f", float(cfg.z))
+    key = (zk, xd.data_ptr(), yd.data_ptr(), n, ncomp, float(cfg.wavelength), str(dev))
+    return key, (None if z_t is None else z_t.detach(), xd, yd)
+
+
 class _RSFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, cfg: RSConfig, src, p00, p01, p10, p11, z_t, wl_t):
@@ -98,20 +115,37 @@ class _RSFunction(torch.autograd.Function):
         # a step that will run the adjoint keeps the kernel spectrum for it, and,
         # when z or lambda receive gradients, the input spectra they are dotted with
         n = src.shape[-1]
-        hhat = spec = None
-        if any(ctx.needs_input_grad[1:]):
-            hhat = torch.empty(ops.rs_hhat_bytes(n), dtype=torch.uint8, device=src.device)
+        ncomp = 3 if cfg.ez_mode == N.XL_EZ_VRS else cfg.nin
+        hhat = spec = spec_dg = None
+        hhat_mode = 0
         # the derivative spectra come from the forward's own kernel rows
         spec_grads = (1 if ctx.needs_input_grad[6] else 0) | (2 if ctx.needs_input_grad[7] else 0)
+        key = None
+        if any(ctx.needs_input_grad[1:]):
+            key, keep = _spectra_key(cfg, z_t, n, ncomp, src.device)
+            hit = _SHARED_SPECTRA.get(key)
+            if hit is not None and hit["hhat"]() is not None:
+                # an earlier propagation over this distance built the spectra
+                hhat, hhat_mode = hit["hhat"](), 2
+                if spec_grads and (spec_grads & ~hit["bits"]) == 0 and hit["spec"] is not None:
+                    spec_dg = hit["spec"]()
+                key = None
+            else:
+                hhat, hhat_mode = torch.empty(ops.rs_hhat_bytes(n), dtype=torch.uint8, device=src.device), 1
         if spec_grads:
-            ncomp = 3 if cfg.ez_mode == N.XL_EZ_VRS else cfg.nin
             spec = torch.empty(ops.rs_spec_bytes(n, ncomp), dtype=torch.uint8, device=src.device)
         out = ops.rs_propagate(src, cfg.x, cfg.y, cfg.z, cfg.wavelength, masks=masks, ez_mode=cfg.ez_mode,
-                               nin=cfg.nin, z_dev=cfg.z_dev, hhat=hhat, hhat_mode=1 if hhat is not None else 0,
+                               nin=cfg.nin, z_dev=cfg.z_dev, hhat=hhat, hhat_mode=hhat_mode,
                                spec=spec, spec_mode=1 if spec is not None else 0, spec_grads=spec_grads,
-                               quality_factor=cfg.qf_out)
+                               quality_factor=cfg.qf_out, spec_dg=spec_dg)
+        if key is not None:
+            if len(_SHARED_SPECTRA) > 64:
+                _SHARED_SPECTRA.clear()
+            _SHARED_SPECTRA[key] = {"hhat": weakref.ref(hhat), "spec": weakref.ref(spec) if spec is not None else None,
+                                    "bits": spec_grads, "keep": keep}
         cfg.qf_out = None   # the backward's launches report nothing
         ctx.cfg = cfg
+        ctx.spec_dg = spec_dg
         ctx.save_for_backward(src, *[p if p is not None else torch.empty(0) for p in params],
                               hhat if hhat is not None else torch.empty(0),
                               spec if spec is not None else torch.empty(0))
@@ -137,7 +171,7 @@ class _RSFunction(torch.autograd.Function):
             g_pre = ops.rs_propagate(gout, cfg.x, cfg.y, cfg.z, cfg.wavelength, adjoint=True, z_dev=cfg.z_dev,
                                      hhat=hhat, hhat_mode=2 if hhat is not None else 0, spec=spec, spec_mode=2,
                                      spec_grads=(1 if want_z else 0) | (2 if want_wl else 0) | N.XL_RS_SPEC_PREBUILT,
-                                     grad_out=grads)
+                                     grad_out=grads, spec_dg=ctx.spec_dg)
         else:
             g_pre = ops.rs_propagate(gout, cfg.x, cfg.y, cfg.z, cfg.wavelength, adjoint=True, z_dev=cfg.z_dev,
                                      hhat=hhat, hhat_mode=2 if hhat is not None else 0)

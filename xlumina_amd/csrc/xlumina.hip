@@ -293,7 +293,7 @@ char* align256(char* p) { return (char*)(((uintptr_t)p + 255) & ~(uintptr_t)255)
 extern "C" {
 
 const char* xl_last_error(void) { return g_err.c_str(); }
-int xl_abi_version(void) { return 7; }
+int xl_abi_version(void) { return 8; }
 int xl_max_fft_length(void) { return xl::MAX_FFT; }
 int xl_init(void) { return ensure_init(); }
 
@@ -344,6 +344,8 @@ int xl_rs_propagate(const xl_rs_params* p, void* ws, size_t ws_bytes, void* stre
     return fail(XL_EINVAL, "spec_mode 1 takes spec_grads 0..3");
   if (p->spec_mode == 2 && ((p->spec_grads & 3) == 0 || p->spec_grads < 0 || p->spec_grads > 7 || !p->grad_out))
     return fail(XL_EINVAL, "spec_mode 2 needs spec_grads 1..3 (| XL_RS_SPEC_PREBUILT) and grad_out");
+  if (p->spec_dg && (!p->spec_mode || (p->spec_mode == 2 && !(p->spec_grads & XL_RS_SPEC_PREBUILT))))
+    return fail(XL_EINVAL, "spec_dg needs spec_mode 1, or spec_mode 2 with XL_RS_SPEC_PREBUILT");
   const int batch = std::max(1, (int)p->batch);
   const int64_t n2 = (int64_t)n * n;
   if (batch > 1) {
@@ -407,10 +409,12 @@ int xl_rs_propagate(const xl_rs_params* p, void* ws, size_t ws_bytes, void* stre
   if (p->spec_mode) {
     const RsSpecLayout l = rs_spec_layout(n, ncomp);
     char* sb = align256((char*)p->spec);
+    char* db = p->spec_dg ? align256((char*)p->spec_dg) : sb;   // where the derivative spectra live
     a.spec = (float2*)sb;
     for (int i = 0; i < 2; ++i)
       if (p->spec_grads >> i & 1) {
-        a.dg[i] = (float2*)(sb + l.off[1 + i]);
+        // a forward call with spec_dg builds nothing: the other call did
+        if (!(p->spec_mode == 1 && p->spec_dg)) a.dg[i] = (float2*)(db + l.off[1 + i]);
         if (p->spec_mode == 2) a.gpart[i] = (double*)(sb + l.off[3 + i]);
       }
   }

@@ -283,7 +283,8 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
                  inten: Optional[torch.Tensor] = None, kernel_dz: int = 0,
                  hhat: Optional[torch.Tensor] = None, hhat_mode: int = 0, noalloc: bool = False,
                  spec: Optional[torch.Tensor] = None, spec_mode: int = 0, spec_grads: int = 0,
-                 grad_out: Optional[torch.Tensor] = None, quality_factor: Optional[torch.Tensor] = None):
+                 grad_out: Optional[torch.Tensor] = None, quality_factor: Optional[torch.Tensor] = None,
+                 spec_dg: Optional[torch.Tensor] = None):
     """RS_propagation_jit / VRS_propagation_jit (wave_optics.py:281-289,
     vectorized_optics.py:364-373) with fused masks and Ez construction.
 
@@ -305,6 +306,10 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
     tensor grad_out (2 elements) by Parseval over those spectra, with no
     derivative propagation; spec_grads | XL_RS_SPEC_PREBUILT takes the
     derivative spectra the mode-1 call built instead of building them.
+    spec_dg: the spec buffer of another call (same n, ncomp, grid, distance
+    and wavelength, spec_grads covering these bits) whose derivative spectra
+    this call uses: a spec_mode 1 call then builds none, a spec_mode 2 call
+    (with XL_RS_SPEC_PREBUILT) reads them there.
     quality_factor: optional 0-d float64 device tensor that the launch fills
     with the quality factor of the distance propagated (wave_optics.py:187-191).
     noalloc: run on the library's own per-device scratch
@@ -366,6 +371,10 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
         if spec is None or spec.device != dev or spec.numel() * spec.element_size() < rs_spec_bytes(n, ncomp):
             raise ValueError("spec must be a device buffer of rs_spec_bytes(n, ncomp) bytes on the field's device")
         p.spec_mode, p.spec, p.spec_grads = int(spec_mode), spec.data_ptr(), int(spec_grads)
+        if spec_dg is not None:
+            if spec_dg.device != dev or spec_dg.numel() * spec_dg.element_size() < rs_spec_bytes(n, ncomp):
+                raise ValueError("spec_dg must be a device buffer of rs_spec_bytes(n, ncomp) bytes on the field's device")
+            p.spec_dg = spec_dg.data_ptr()
         if spec_mode == 2:
             if (grad_out is None or grad_out.device != dev or grad_out.dtype != torch.float64
                     or grad_out.numel() < 2 or not grad_out.is_contiguous()):
