@@ -647,3 +647,45 @@ def test_same_distance_propagations_share_spectra(cuda_dev, float_z):
         ref = float(gu[3]) + float(gu[4])
         assert abs(float(gs[3]) - ref) <= 1e-12 * abs(ref), (float(gs[3]), ref)
     assert torch.equal(s1, u1) and torch.equal(s2, u2)
+
+
+def test_czt_calls_of_one_geometry_share_staged_tables(cuda_dev):
+    """Two differentiated high-NA VCZT over equal geometries (separate
+    objects, as each lens call builds): the second forward and adjoint use
+    the axis tables the first staged (tables_mode 2).  Outputs and gradients
+    equal those of calls that stage their own."""
+    n = 128
+    x, y = R.space(1800.0, n)
+    xo = np.linspace(-2, 2, 40)
+    rng = np.random.default_rng(17)
+
+    def c64(*shape):
+        return torch.tensor(rng.standard_normal(shape) + 1j * rng.standard_normal(shape), dtype=torch.complex64,
+                            device=cuda_dev)
+
+    a, b = c64(2, n, n), c64(2, n, n)
+    w = torch.tensor(rng.random((2, 40, 40)), dtype=torch.float32, device=cuda_dev)
+
+    def run(share):
+        AG._STAGED_TABLES.clear()
+        sa, sb = a.clone().requires_grad_(True), b.clone().requires_grad_(True)
+        o1 = AG.czt(sa, ops.hna_geometry(x, y, xo, xo, 0.65, 1800.0, 2000.0), nin=2, form="bluestein")
+        if not share:
+            AG._STAGED_TABLES.clear()
+        o2 = AG.czt(sb, ops.hna_geometry(x, y, xo, xo, 0.65, 1800.0, 2000.0), nin=2, form="bluestein")
+        L = (w[0] * (o1.abs() ** 2).sum(0)).sum() + (w[1] * (o2.abs() ** 2).sum(0)).sum()
+        return o1.detach(), o2.detach(), torch.autograd.grad(L, (sa, sb))
+
+    s1, s2, gs = run(True)
+    u1, u2, gu = run(False)
+    assert torch.equal(s1, u1) and torch.equal(s2, u2)
+    assert torch.equal(gs[0], gu[0]) and torch.equal(gs[1], gu[1])
+    # the shared run really shared: one live entry while both calls' state lived
+    sa = a.clone().requires_grad_(True)
+    AG._STAGED_TABLES.clear()
+    o1 = AG.czt(sa, ops.hna_geometry(x, y, xo, xo, 0.65, 1800.0, 2000.0), nin=2, form="bluestein")
+    o2 = AG.czt(sa, ops.hna_geometry(x, y, xo, xo, 0.65, 1800.0, 2000.0), nin=2, form="bluestein")
+    assert len(AG._STAGED_TABLES) == 1
+    share = next(iter(AG._STAGED_TABLES.values()))()
+    assert share is not None and share.fwd is not None
+    del o1, o2

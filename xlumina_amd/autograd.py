@@ -247,12 +247,75 @@ class CZTConfig:
     form: str = "bluestein"   # resolved implementation, used for forward and adjoint
 
 
+class _StagedTables:
+    """Workspaces holding the staged Bluestein tables (chirps and their
+    spectra, functions of the geometry only) of one CZT geometry: one for the
+    forward calls, one for the adjoints.  Every differentiated call of the
+    geometry holds it, so it lives until the last of their backward passes;
+    the first call of each direction stages the tables, the others use them."""
+    __slots__ = ("fwd", "adj", "__weakref__")
+
+    def __init__(self):
+        self.fwd = None
+        self.adj = None
+
+
+_STAGED_TABLES: dict = {}
+
+
+def _geometry_key(g: ops.CztGeometry, mode):
+    """Content of a geometry (each call builds its own object)."""
+    return (mode, float(g.z), float(g.wavelength), float(g.dm), int(g.mode), float(g.hna_radius), float(g.hna_f),
+            g.x.tobytes(), g.y.tobytes(), g.xout.tobytes(), g.yout.tobytes())
+
+
+def _staged_tables(cfg: CZTConfig):
+    """The _StagedTables of cfg's geometry (None for the GEMM form)."""
+    if cfg.form != "bluestein":
+        return None
+    key = _geometry_key(cfg.geom, cfg.mode)
+    hit = _STAGED_TABLES.get(key)
+    share = hit() if hit is not None else None
+    if share is None:
+        share = _StagedTables()
+        if len(_STAGED_TABLES) > 64:
+            _STAGED_TABLES.clear()
+        _STAGED_TABLES[key] = weakref.ref(share)
+    return share
+
+
+def _czt_staged(share, attr: str, src, cfg: CZTConfig, **kw):
+    """ops.czt on the workspace share.<attr>: the first call stages the axis
+    tables there, later calls with the same arguments use them."""
+    if share is None:
+        return ops.czt(src, cfg.geom, mode=cfg.mode, form=cfg.form, **kw)
+    ws = getattr(share, attr)
+    adjoint, ez_mode = kw.get("adjoint", False), kw.get("ez_mode", N.XL_EZ_NONE)
+    nin = kw.get("nin") or src.shape[-3]
+    ncomp = 3 if (ez_mode == N.XL_EZ_VCZT or cfg.mode == N.XL_CZT_HNA) and not adjoint else nin
+    if cfg.mode != cfg.geom.mode:
+        return ops.czt(src, cfg.geom, mode=cfg.mode, form=cfg.form, **kw)
+    try:
+        need = ops.czt_workspace_bytes(cfg.geom, ncomp=ncomp, nin=nin, adjoint=adjoint, ez_mode=ez_mode)
+    except ValueError:
+        return ops.czt(src, cfg.geom, mode=cfg.mode, form=cfg.form, **kw)   # raises with the plan error
+    if ws is not None and ws.numel() >= need and ws.device == src.device:
+        return ops.czt(src, cfg.geom, mode=cfg.mode, form=cfg.form, workspace=ws, tables_mode=2, **kw)
+    ws = torch.empty(need, dtype=torch.uint8, device=src.device)
+    ops.czt(src, cfg.geom, mode=cfg.mode, form=cfg.form, workspace=ws, tables_mode=1, **kw)   # stage only
+    setattr(share, attr, ws)
+    return ops.czt(src, cfg.geom, mode=cfg.mode, form=cfg.form, workspace=ws, tables_mode=2, **kw)
+
+
 class _CZTFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, cfg: CZTConfig, src, p00, p01, p10, p11, z_t, wl_t):
         params = (p00, p01, p10, p11)
         masks = _rebuild(cfg.mask_struct, params)
-        out = ops.czt(src, cfg.geom, masks=masks, ez_mode=cfg.ez_mode, nin=cfg.nin, mode=cfg.mode, form=cfg.form)
+        # the calls of one geometry in a differentiated step stage its tables once
+        share = _staged_tables(cfg) if any(ctx.needs_input_grad[1:]) else None
+        out = _czt_staged(share, "fwd", src, cfg, masks=masks, ez_mode=cfg.ez_mode, nin=cfg.nin)
+        ctx.tables = share
         ctx.cfg = cfg
         keep_out = (z_t is not None and ctx.needs_input_grad[6]) or (wl_t is not None and ctx.needs_input_grad[7])
         ctx.save_for_backward(src, *[p if p is not None else torch.empty(0) for p in params],
@@ -269,7 +332,7 @@ class _CZTFunction(torch.autograd.Function):
         saved = ctx.saved_tensors
         src = saved[0]
         params = [saved[1 + i] if ctx.has[i] else None for i in range(4)]
-        g_pre = ops.czt(gout.contiguous(), g, adjoint=True, mode=cfg.mode, form=cfg.form)
+        g_pre = _czt_staged(ctx.tables, "adj", gout.contiguous(), cfg, adjoint=True)
         masks = _rebuild(cfg.mask_struct, params)
         hna = cfg.mode == N.XL_CZT_HNA
         g_z = None

@@ -43,7 +43,14 @@ def _ptr(t: Optional[torch.Tensor]) -> int:
     return 0 if t is None else t.data_ptr()
 
 
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def _stream(device) -> int:
+    """Handle of torch's current stream on `device` (a tensor's device): the
+    direct query where torch has it (a C call instead of a Stream object)."""
+    if _raw_stream is not None and device.index is not None:
+        return _raw_stream(device.index)
     return torch.cuda.current_stream(device).cuda_stream
 
 
@@ -604,7 +611,8 @@ def czt(src: torch.Tensor, g: CztGeometry, masks: Sequence[MaskSpec] = (), ez_mo
     (B, rows, cols) views or shared, output (B, ncomp, ...).
     `tables_mode` (Bluestein form, with a caller `workspace`): 1 only stages the
     axis tables into the workspace (src and out are not touched), 2 uses the
-    tables a mode-1 call with the same arguments staged there.
+    tables a mode-1 call with the same arguments staged there (a mode-0 call
+    with a caller workspace leaves the same tables there too).
     """
     if batch is not None:
         return _czt_batched(src, g, masks, ez_mode, nin, ncomp, adjoint, mode, out, workspace, batch)
