@@ -686,6 +686,6 @@ def test_czt_calls_of_one_geometry_share_staged_tables(cuda_dev):
     o1 = AG.czt(sa, ops.hna_geometry(x, y, xo, xo, 0.65, 1800.0, 2000.0), nin=2, form="bluestein")
     o2 = AG.czt(sa, ops.hna_geometry(x, y, xo, xo, 0.65, 1800.0, 2000.0), nin=2, form="bluestein")
     assert len(AG._STAGED_TABLES) == 1
-    share = next(iter(AG._STAGED_TABLES.values()))()
+    share = next(iter(AG._STAGED_TABLES.values()))[1]()
     assert share is not None and share.fwd is not None
     del o1, o2

@@ -263,24 +263,22 @@ class _StagedTables:
 _STAGED_TABLES: dict = {}
 
 
-def _geometry_key(g: ops.CztGeometry, mode):
-    """Content of a geometry (each call builds its own object)."""
-    return (mode, float(g.z), float(g.wavelength), float(g.dm), int(g.mode), float(g.hna_radius), float(g.hna_f),
-            g.x.tobytes(), g.y.tobytes(), g.xout.tobytes(), g.yout.tobytes())
-
-
 def _staged_tables(cfg: CZTConfig):
-    """The _StagedTables of cfg's geometry (None for the GEMM form)."""
+    """The _StagedTables of cfg's geometry, found by content (each call
+    builds its own geometry object); None for the GEMM form."""
     if cfg.form != "bluestein":
         return None
-    key = _geometry_key(cfg.geom, cfg.mode)
+    g = cfg.geom
+    grids = (g.x.tobytes(), g.y.tobytes(), g.xout.tobytes(), g.yout.tobytes())
+    key = (cfg.mode, float(g.z), float(g.wavelength), float(g.dm), int(g.mode), float(g.hna_radius), float(g.hna_f),
+           *(ops.content_key(b) for b in grids))
     hit = _STAGED_TABLES.get(key)
-    share = hit() if hit is not None else None
+    share = hit[1]() if hit is not None and hit[0] == grids else None
     if share is None:
         share = _StagedTables()
         if len(_STAGED_TABLES) > 64:
             _STAGED_TABLES.clear()
-        _STAGED_TABLES[key] = weakref.ref(share)
+        _STAGED_TABLES[key] = (grids, weakref.ref(share))
     return share
 
 
