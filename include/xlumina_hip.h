@@ -69,6 +69,8 @@ extern "C" {
 #define XL_PF_SCALAR 2
 #define XL_PF_UNIT_ANGLE 4
 #define XL_RS_SPEC_PREBUILT 4   /* spec_grads bit: the derivative spectra are already in `spec` */
+#define XL_RS_SPEC_PARTIALS 8   /* spec_grads bit of spec_mode 2: the per-unit partial sums stay in `spec`
+                                   (xl_rs_spec_partials) and grad_out is not written */
 
 typedef struct xl_mask_stage {
   int32_t kind;
@@ -228,8 +230,12 @@ typedef struct xl_pre_params {
    * Ez_in = (Ex' X + Ey' Y)/r (vectorized_optics.py:258-261) through its cotangent, the
    * term xl_dz_dot forms from `pre` -- here without a second pass over the field. */
   double* ez_dz;
-  const double* ez_dz_base; /* optional: one float64 on the device added to ez_dz[0] (the
+  const double* ez_dz_base; /* optional: float64 on the device added to ez_dz[0] (the
                                propagation's own term, so element 0 holds the whole gradient) */
+  int32_t ez_dz_base_n;     /* values at ez_dz_base, summed in a fixed order (0: one) -- e.g. the
+                               per-unit partial sums an XL_RS_SPEC_PARTIALS adjoint left in its spec
+                               buffer (xl_rs_spec_partials) */
+  int32_t reserved3;
 } xl_pre_params;
 
 const char* xl_last_error(void);
@@ -249,6 +255,10 @@ size_t xl_rs_hhat_bytes(int32_t n);
  * The gradients in z / lambda of RS_propagation (wave_optics.py:281-289
  * under jax.grad) without a second propagation. */
 size_t xl_rs_spec_bytes(int32_t n, int32_t ncomp);
+/* Byte offset, from the 256-byte-aligned start of a spec buffer, of the
+ * float64 per-unit partial sums of derivative i (0: z, 1: k) that a spec_mode 2
+ * call writes; *count receives their number.  -1 for bad arguments. */
+int64_t xl_rs_spec_partials(int32_t n, int32_t ncomp, int32_t i, int32_t* count);
 int xl_rs_propagate(const xl_rs_params* p, void* workspace, size_t workspace_bytes, void* stream);
 
 /* No counterpart in the reference.  xl_rs_propagate without a workspace

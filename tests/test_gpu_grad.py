@@ -461,6 +461,14 @@ def test_rs_spectral_grads_match_derivative_propagation(cuda_dev, n, vectorial):
                     assert abs(float(pre[i]) - float(grads[i])) <= 1e-6 * scale, (n, z, bits, i, float(pre[i]))
                 else:
                     assert float(pre[i]) == 7.0
+            # XL_RS_SPEC_PARTIALS: the per-unit partial sums stay in the spec
+            # buffer (grad_out untouched); their sum is the reduced value
+            ops.rs_propagate(g, x, y, z, 0.6, adjoint=True, hhat=hhat2, hhat_mode=2, spec=spec2, spec_mode=2,
+                             spec_grads=bits | N.XL_RS_SPEC_PREBUILT | N.XL_RS_SPEC_PARTIALS, grad_out=None)
+            for i in range(2):
+                if bits >> i & 1:
+                    part = float(ops.rs_spec_partials(spec2, n, ncomp, i).sum())
+                    assert abs(part - float(pre[i])) <= 1e-9 * abs(float(pre[i])), (n, z, bits, i, part)
 
 
 def test_dz_dot_ez_term_alone(cuda_dev):

@@ -106,7 +106,7 @@ def test_every_struct_field_matches_c():
 
 
 def test_host_side_queries(lib):
-    assert lib.xl_abi_version() == 8
+    assert lib.xl_abi_version() == 9
     assert lib.xl_max_fft_length() == 4096
     # RS workspace: ncomp * round4(n) * P + (P/2) * P + P complex64 (+ alignment slack)
     n, P = 1024, 2048
@@ -122,6 +122,17 @@ def test_host_side_queries(lib):
     assert lib.xl_rs_spec_bytes(20, 3) == 3 * 64 * 64 * 8 + 2 * (32 * 64 + 64) * 8 + 2 * 192 * 8 + 256
     assert lib.xl_rs_spec_bytes(20, 1) == 64 * 64 * 8 + 2 * (32 * 64 + 64) * 8 + 2 * 64 * 8 + 256
     assert lib.xl_rs_spec_bytes(1, 1) == 0 and lib.xl_rs_spec_bytes(64, 4) == 0 and lib.xl_rs_spec_bytes(4096, 1) == 0
+    # the partial sums are the last two regions: units float64 each, ending at the buffer's end
+    cnt = C.c_int32(0)
+    for nn, nc in ((1024, 3), (20, 3), (20, 1)):
+        total = lib.xl_rs_spec_bytes(nn, nc) - 256
+        o0 = lib.xl_rs_spec_partials(nn, nc, 0, C.byref(cnt))
+        units = cnt.value
+        o1 = lib.xl_rs_spec_partials(nn, nc, 1, C.byref(cnt))
+        assert cnt.value == units and o1 == o0 + (units * 8 + 255) // 256 * 256
+        assert total == o1 + (units * 8 + 255) // 256 * 256
+    assert units == 64 and lib.xl_rs_spec_partials(1024, 3, 2, C.byref(cnt)) == -1
+    assert lib.xl_rs_spec_partials(1, 1, 0, C.byref(cnt)) == -1
 
 
 def _czt_params(m, M):

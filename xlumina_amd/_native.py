@@ -25,6 +25,7 @@ XL_MASK_NONE, XL_MASK_SLM, XL_MASK_SSLM, XL_MASK_LCD, XL_MASK_LCD_SCALAR, XL_MAS
 XL_EZ_NONE, XL_EZ_VRS, XL_EZ_VCZT, XL_EZ_HNA = range(4)
 XL_PF_F64, XL_PF_SCALAR, XL_PF_UNIT_ANGLE = 1, 2, 4
 XL_RS_SPEC_PREBUILT = 4   # spec_grads bit of an adjoint call: the forward built the derivative spectra
+XL_RS_SPEC_PARTIALS = 8   # spec_grads bit of an adjoint call: partial sums stay in spec (xl_rs_spec_partials)
 XL_CZT_RS, XL_CZT_HNA, XL_CZT_PLAIN = range(3)
 XL_LOSS_INTENSITY, XL_LOSS_AMPLITUDE, XL_LOSS_PHASE = range(3)
 XL_RS_NOALLOC_MAX_N = 1024
@@ -72,7 +73,7 @@ class PreParams(C.Structure):
                 ("z_dev", C.c_void_p), ("batch", C.c_int32), ("out_planes", C.c_int32),
                 ("in_batch_stride", C.c_int64), ("gout_batch_stride", C.c_int64), ("out_batch_stride", C.c_int64),
                 ("mask_batch_stride", C.c_int64), ("gparam_batch_stride", C.c_int64), ("ez_dz", C.c_void_p),
-                ("ez_dz_base", C.c_void_p)]
+                ("ez_dz_base", C.c_void_p), ("ez_dz_base_n", C.c_int32), ("reserved3", C.c_int32)]
 
 
 class MseParams(C.Structure):
@@ -142,6 +143,7 @@ SIGNATURES = {
     "xl_rs_batch_workspace_bytes": (C.c_size_t, [C.c_int32, C.c_int32, C.c_int32]),
     "xl_rs_hhat_bytes": (C.c_size_t, [C.c_int32]),
     "xl_rs_spec_bytes": (C.c_size_t, [C.c_int32, C.c_int32]),
+    "xl_rs_spec_partials": (C.c_int64, [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]),
     "xl_pre_scalar_grad_elems": (C.c_size_t, [C.c_int32, C.c_int32]),
     "xl_rs_propagate": (C.c_int, [C.POINTER(RsParams), C.c_void_p, C.c_size_t, C.c_void_p]),
     "xl_rs_propagate_noalloc": (C.c_int, [C.POINTER(RsParams), C.c_void_p]),

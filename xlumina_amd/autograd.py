@@ -165,12 +165,17 @@ class _RSFunction(torch.autograd.Function):
         gout = gout.contiguous()
         want_z, want_wl = ctx.needs_input_grad[6], ctx.needs_input_grad[7]
         grads = None
+        # with the VRS Ez_in term the pre-op adjoint's reduction also sums the
+        # column pass's partial sums (one reduction launch instead of two)
+        ez_path = want_z and cfg.ez_mode == N.XL_EZ_VRS
+        partials = spec is not None and ez_path and not want_wl
         if spec is not None:
             # Re<g, conv(E', dH/dz)> and Re<g, conv(E', dH/dk)> by Parseval in the adjoint's column pass
-            grads = torch.empty(2, dtype=torch.float64, device=gout.device)
+            grads = None if partials else torch.empty(2, dtype=torch.float64, device=gout.device)
+            bits = (1 if want_z else 0) | (2 if want_wl else 0) | N.XL_RS_SPEC_PREBUILT
             g_pre = ops.rs_propagate(gout, cfg.x, cfg.y, cfg.z, cfg.wavelength, adjoint=True, z_dev=cfg.z_dev,
                                      hhat=hhat, hhat_mode=2 if hhat is not None else 0, spec=spec, spec_mode=2,
-                                     spec_grads=(1 if want_z else 0) | (2 if want_wl else 0) | N.XL_RS_SPEC_PREBUILT,
+                                     spec_grads=bits | (N.XL_RS_SPEC_PARTIALS if partials else 0),
                                      grad_out=grads, spec_dg=ctx.spec_dg)
         else:
             g_pre = ops.rs_propagate(gout, cfg.x, cfg.y, cfg.z, cfg.wavelength, adjoint=True, z_dev=cfg.z_dev,
@@ -179,16 +184,17 @@ class _RSFunction(torch.autograd.Function):
         need_params = ((ctx.needs_input_grad[2] and params[0] is not None, ctx.needs_input_grad[3] and params[1] is not None),
                        (ctx.needs_input_grad[4] and params[2] is not None, ctx.needs_input_grad[5] and params[3] is not None))
         # the VRS Ez_in construction depends on z too: its term comes from the pre-op adjoint's pass
-        ez_dz = None
-        if want_z and cfg.ez_mode == N.XL_EZ_VRS:
+        ez_dz = ez_base = None
+        if ez_path:
             ez_dz = torch.empty(ops.pre_scalar_grad_elems(*src.shape[-2:]), dtype=torch.float64, device=src.device)
+            ez_base = ops.rs_spec_partials(spec, src.shape[-1], 3, 0) if partials else grads[:1]
         if not masks and cfg.ez_mode == N.XL_EZ_NONE:
             g_src, gp = g_pre, [[None, None], [None, None]]
         else:
             g_src, gp = ops.pre_backward(src, g_pre, cfg.x, cfg.y, cfg.z, masks=masks, ez_mode=cfg.ez_mode,
                                          nin=cfg.nin, need_src=ctx.needs_input_grad[1], need_params=need_params,
                                          z_dev=cfg.z_dev, out=_full_grad(ctx, src, cfg.nin), ez_dz=ez_dz,
-                                         ez_dz_base=grads[:1] if ez_dz is not None else None)
+                                         ez_dz_base=ez_base)
         g_z = None
         if want_z:
             # dL/dz = Re<g, conv(E', dH/dz)> (+ the Ez_in term, vectorized_optics.py:262-267)

@@ -310,14 +310,16 @@ __global__ __launch_bounds__(256) void pre_backward_kernel(PreArgs a0) {
 
 // buf[i][0] = sum of buf[i][1 .. n] for the scalar-form parameter gradients
 // and the Ez_in distance term (block i; null pointers skip), fixed order;
-// base4, if set, is added to the Ez_in term's sum.
+// base4[0 .. nbase), if set, is added to the Ez_in term's sum.
 __global__ __launch_bounds__(1024) void pgrad_reduce_kernel(double* b0, double* b1, double* b2, double* b3, double* b4,
-                                                            const double* base4, int n) {
+                                                            const double* base4, int n, int nbase) {
   double* b = blockIdx.x == 0 ? b0 : blockIdx.x == 1 ? b1 : blockIdx.x == 2 ? b2 : blockIdx.x == 3 ? b3 : b4;
   if (!b) return;   // whole block
   __shared__ double red[16];
   double s = 0.0;
   for (int i = threadIdx.x; i < n; i += 1024) s += b[1 + i];
+  if (blockIdx.x == 4 && base4)
+    for (int i = threadIdx.x; i < nbase; i += 1024) s += base4[i];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
@@ -325,7 +327,6 @@ __global__ __launch_bounds__(1024) void pgrad_reduce_kernel(double* b0, double* 
   if (threadIdx.x == 0) {
     double v = 0.0;
     for (int w = 0; w < 16; ++w) v += red[w];
-    if (blockIdx.x == 4 && base4) v += base4[0];
     b[0] = v;
   }
 }

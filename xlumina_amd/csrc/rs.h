@@ -61,6 +61,7 @@ struct RsArgs {
   // derivative spectrum dg[i] (Hhat layout) that is set.
   int spec_mode;
   int dg_built;           // spec_mode 2: the derivative spectra were built by the forward call
+  int partials_only;      // spec_mode 2: leave the partial sums in gpart (no grad_out reduction)
   float2* spec;
   float2* dg[2];          // spec_mode 1: derivative spectra to build with the kernel spectrum
   double* gpart[2];
@@ -762,7 +763,7 @@ inline hipError_t rs_launch(const RsArgs& a, hipStream_t st) {
     else
       hipLaunchKernelGGL((rs_col_kernel<P, 0>), grid, blk, 0, st, a);
   }
-  if (a.spec_mode == 2) {
+  if (a.spec_mode == 2 && !a.partials_only) {
     KTimer t(K_LOSS, st);
     hipLaunchKernelGGL(rs_grad_reduce_kernel, dim3(2), dim3(1024), 0, st, a.gpart[0], a.gpart[1],
                        rs_col_units<P>(a.ncomp), a.grad_out);

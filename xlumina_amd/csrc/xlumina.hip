@@ -293,7 +293,7 @@ char* align256(char* p) { return (char*)(((uintptr_t)p + 255) & ~(uintptr_t)255)
 extern "C" {
 
 const char* xl_last_error(void) { return g_err.c_str(); }
-int xl_abi_version(void) { return 8; }
+int xl_abi_version(void) { return 9; }
 int xl_max_fft_length(void) { return xl::MAX_FFT; }
 int xl_init(void) { return ensure_init(); }
 
@@ -301,6 +301,13 @@ size_t xl_rs_hhat_bytes(int32_t n) {
   if (n < 2) return 0;
   const int64_t P = rs_fft_len(n);
   return (size_t)((P / 2) * P + P) * sizeof(float2);
+}
+
+int64_t xl_rs_spec_partials(int32_t n, int32_t ncomp, int32_t i, int32_t* count) {
+  if (!count || i < 0 || i > 1 || xl_rs_spec_bytes(n, ncomp) == 0) return -1;
+  const RsSpecLayout l = rs_spec_layout(n, ncomp);
+  *count = l.units;
+  return l.off[3 + i];
 }
 
 size_t xl_rs_spec_bytes(int32_t n, int32_t ncomp) {
@@ -342,8 +349,10 @@ int xl_rs_propagate(const xl_rs_params* p, void* ws, size_t ws_bytes, void* stre
     return fail(XL_EINVAL, "spec_mode 1 / 2 needs the spec buffer, one problem and kernel_dz = 0");
   if (p->spec_mode == 1 && (p->spec_grads < 0 || p->spec_grads > 3))
     return fail(XL_EINVAL, "spec_mode 1 takes spec_grads 0..3");
-  if (p->spec_mode == 2 && ((p->spec_grads & 3) == 0 || p->spec_grads < 0 || p->spec_grads > 7 || !p->grad_out))
-    return fail(XL_EINVAL, "spec_mode 2 needs spec_grads 1..3 (| XL_RS_SPEC_PREBUILT) and grad_out");
+  if (p->spec_mode == 2 && ((p->spec_grads & 3) == 0 || p->spec_grads < 0 || p->spec_grads > 15 ||
+                             (!p->grad_out && !(p->spec_grads & XL_RS_SPEC_PARTIALS))))
+    return fail(XL_EINVAL, "spec_mode 2 needs spec_grads 1..3 (| XL_RS_SPEC_PREBUILT | XL_RS_SPEC_PARTIALS) and "
+                           "grad_out unless XL_RS_SPEC_PARTIALS");
   if (p->spec_dg && (!p->spec_mode || (p->spec_mode == 2 && !(p->spec_grads & XL_RS_SPEC_PREBUILT))))
     return fail(XL_EINVAL, "spec_dg needs spec_mode 1, or spec_mode 2 with XL_RS_SPEC_PREBUILT");
   const int batch = std::max(1, (int)p->batch);
@@ -399,6 +408,7 @@ int xl_rs_propagate(const xl_rs_params* p, void* ws, size_t ws_bytes, void* stre
   a.spec = nullptr;
   a.dg[0] = a.dg[1] = nullptr;
   a.dg_built = p->spec_mode == 2 && (p->spec_grads & XL_RS_SPEC_PREBUILT) ? 1 : 0;
+  a.partials_only = p->spec_mode == 2 && (p->spec_grads & XL_RS_SPEC_PARTIALS) ? 1 : 0;
   a.gpart[0] = a.gpart[1] = nullptr;
   a.grad_out = p->grad_out;
   a.qf_out = p->quality_factor;
@@ -925,7 +935,8 @@ int xl_pre_backward(const xl_pre_params* p, void* stream) {
       hipLaunchKernelGGL(xl::pre_backward_kernel<0>, grid, dim3(256), 0, (hipStream_t)stream, a);
     if (a.scal)
       hipLaunchKernelGGL(xl::pgrad_reduce_kernel, dim3(5), dim3(1024), 0, (hipStream_t)stream, sbuf[0], sbuf[1],
-                         sbuf[2], sbuf[3], sbuf[4], p->ez_dz ? p->ez_dz_base : nullptr, (int)grid.x);
+                         sbuf[2], sbuf[3], sbuf[4], p->ez_dz ? p->ez_dz_base : nullptr, (int)grid.x,
+                         std::max(1, (int)p->ez_dz_base_n));
   }
   return hip_check(hipGetLastError(), "pre_backward launch");
 }

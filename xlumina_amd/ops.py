@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import weakref
 from dataclasses import dataclass
 from typing import Optional, Sequence
 
@@ -68,14 +69,29 @@ def content_key(b: bytes):
 
 
 _grid_cache: dict = {}
+_grid_ids: dict = {}   # (id(array), device) -> (weak reference, its bytes, device copy)
 
 
 def device_grid(arr, device) -> torch.Tensor:
-    """float64 device copy of a 1-D grid, cached by content."""
+    """float64 device copy of a 1-D grid, cached by content (a grid array
+    seen before is recognised by identity and a comparison of its bytes)."""
     if isinstance(arr, torch.Tensor):
         if arr.device == device and arr.dtype == torch.float64 and arr.is_contiguous():
             return arr
         arr = arr.detach().cpu().numpy()
+    if isinstance(arr, np.ndarray):
+        ent = _grid_ids.get((id(arr), device))
+        if ent is not None and ent[0]() is arr and ent[1] == arr.tobytes():
+            return ent[2]
+    t = _device_grid_by_content(arr, device)
+    if isinstance(arr, np.ndarray):
+        if len(_grid_ids) > 256:
+            _grid_ids.clear()
+        _grid_ids[(id(arr), device)] = (weakref.ref(arr), arr.tobytes(), t)
+    return t
+
+
+def _device_grid_by_content(arr, device) -> torch.Tensor:
     a = np.ascontiguousarray(np.asarray(arr, dtype=np.float64).ravel())
     ab = a.tobytes()
     key = (content_key(ab), str(device))
@@ -293,6 +309,18 @@ def rs_hhat_bytes(n: int) -> int:
     return int(N.load().xl_rs_hhat_bytes(n))
 
 
+def rs_spec_partials(spec: torch.Tensor, n: int, ncomp: int, i: int = 0) -> torch.Tensor:
+    """float64 view of the per-unit partial sums of derivative i (0: z, 1: k)
+    that a spec_mode 2 call writes into `spec` (with XL_RS_SPEC_PARTIALS they
+    are left there for a later reduction)."""
+    count = C.c_int32(0)
+    off = int(N.load().xl_rs_spec_partials(n, ncomp, i, C.byref(count)))
+    if off < 0:
+        raise ValueError(N.load().xl_last_error().decode())
+    start = (-spec.data_ptr()) % 256 + off   # the library aligns the buffer to 256 bytes
+    return spec[start:start + 8 * count.value].view(torch.float64)
+
+
 def rs_spec_bytes(n: int, ncomp: int) -> int:
     """Bytes of the spectrum buffer of an n x n, ncomp-component RS propagation
     whose gradients in z / lambda come from the column pass (spec_mode)."""
@@ -398,7 +426,7 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
             if spec_dg.device != dev or spec_dg.numel() * spec_dg.element_size() < rs_spec_bytes(n, ncomp):
                 raise ValueError("spec_dg must be a device buffer of rs_spec_bytes(n, ncomp) bytes on the field's device")
             p.spec_dg = spec_dg.data_ptr()
-        if spec_mode == 2:
+        if spec_mode == 2 and not (grad_out is None and spec_grads & N.XL_RS_SPEC_PARTIALS):
             if (grad_out is None or grad_out.device != dev or grad_out.dtype != torch.float64
                     or grad_out.numel() < 2 or not grad_out.is_contiguous()):
                 raise ValueError("grad_out must be a contiguous float64 device tensor of 2 elements")
@@ -800,7 +828,9 @@ def pre_backward(src: torch.Tensor, gout: torch.Tensor, x, y, z: float = 0.0, ma
     a float64 device buffer of pre_scalar_grad_elems(rows, cols) whose element 0
     receives the distance dependence of the constructed Ez_in through its
     cotangent (the `pre` term of dz_dot), formed in the same pass, plus the
-    float64 device value `ez_dz_base` (one element) when given."""
+    sum of the float64 device values `ez_dz_base` (fixed order) when given --
+    one value, or the partial sums an adjoint propagation left in its spec
+    buffer (rs_spec_partials)."""
     lib = N.load()
     src = _prep_field(src)
     gout = _prep_field(gout)
@@ -845,9 +875,10 @@ def pre_backward(src: torch.Tensor, gout: torch.Tensor, x, y, z: float = 0.0, ma
             raise ValueError("ez_dz must be a contiguous float64 device buffer of pre_scalar_grad_elems(rows, cols)")
         p.ez_dz = ez_dz.data_ptr()
         if ez_dz_base is not None:
-            if ez_dz_base.dtype != torch.float64 or ez_dz_base.device != src.device or ez_dz_base.numel() < 1:
-                raise ValueError("ez_dz_base must be a float64 device tensor")
-            p.ez_dz_base = ez_dz_base.data_ptr()
+            if (ez_dz_base.dtype != torch.float64 or ez_dz_base.device != src.device or ez_dz_base.numel() < 1
+                    or not ez_dz_base.is_contiguous()):
+                raise ValueError("ez_dz_base must be a contiguous float64 device tensor")
+            p.ez_dz_base, p.ez_dz_base_n = ez_dz_base.data_ptr(), ez_dz_base.numel()
     N.check(lib.xl_pre_backward(C.byref(p), _stream(src.device)))
     del keep
     return gsrc, gp
