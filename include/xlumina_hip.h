@@ -440,6 +440,15 @@ typedef struct xl_bs_dots_params {
   int32_t reserved;
   double* dtheta;              /* with theta_mode > 0: dL/dt = Re sum_k S_k dkappa_k/dt (one float64);
                                   with XL_BS_THETA_RAW, dL/dp = 2 pi dL/dt */
+  /* Optional, with theta_mode > 0: the input cotangents in the same pass (the
+   * xl_beam_splitter adjoint), ga = conj(ca) gc + conj(da) gd and
+   * gb = conj(cb) gc + conj(db) gd on components 0, 1 (an absent gc / gd
+   * counts as 0); planes 2 .. g_planes-1 of ga / gb receive zeros. */
+  void* ga;
+  void* gb;
+  int64_t ga_comp_stride, gb_comp_stride;
+  int32_t g_planes;
+  int32_t reserved2;
 } xl_bs_dots_params;
 size_t xl_bs_dots_workspace_bytes(int64_t npix);
 int xl_bs_dots(const xl_bs_dots_params* p, void* workspace, size_t workspace_bytes, void* stream);

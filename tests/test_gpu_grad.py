@@ -697,3 +697,41 @@ def test_czt_calls_of_one_geometry_share_staged_tables(cuda_dev):
     share = next(iter(AG._STAGED_TABLES.values()))[1]()
     assert share is not None and share.fwd is not None
     del o1, o2
+
+
+@pytest.mark.parametrize("mode", [1, 2, 1 | 4])
+def test_splitter_fused_backward_matches_separate(cuda_dev, mode):
+    """A splitter whose angle and input fields both need gradients forms them
+    in one pass (xl_bs_dots with ga / gb): the same values as the separate
+    adjoint (xl_beam_splitter) and angle-gradient (xl_bs_dots) passes,
+    including the zeroed third plane.  Modes: symmetric, single input, and
+    symmetric with a raw angle."""
+    from xlumina_amd import optical_elements as OE
+    n = 96
+    rng = np.random.default_rng(53 + mode)
+
+    def c64(*shape):
+        return torch.tensor(rng.standard_normal(shape) + 1j * rng.standard_normal(shape), dtype=torch.complex64,
+                            device=cuda_dev)
+
+    a0, b0 = c64(3, n, n), c64(3, n, n)
+    th0 = torch.tensor([0.37], dtype=torch.float64, device=cuda_dev)
+    w = torch.tensor(rng.random((2, 3, n, n)), dtype=torch.float32, device=cuda_dev)
+    two = mode & 3 == 1
+
+    def run(fields, angle):
+        a = a0.clone().requires_grad_(fields)
+        b = b0.clone().requires_grad_(fields) if two else None
+        th = th0.clone().requires_grad_(angle)
+        c, d = OE._SplitterTheta.apply(a, b, th, mode)
+        L = (w[0] * c.abs() ** 2).sum() + (w[1] * d.abs() ** 2).sum()
+        wrt = ([a] + ([b] if two else []) if fields else []) + ([th] if angle else [])
+        return torch.autograd.grad(L, wrt)
+
+    fused = run(True, True)
+    fields_only = run(True, False)
+    angle_only = run(False, True)
+    for p, q in zip(fused[:-1], fields_only):
+        assert torch.equal(p, q)
+        assert float(p[2].abs().max()) == 0.0
+    assert torch.equal(fused[-1], angle_only[0])

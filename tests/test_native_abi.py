@@ -106,7 +106,7 @@ def test_every_struct_field_matches_c():
 
 
 def test_host_side_queries(lib):
-    assert lib.xl_abi_version() == 9
+    assert lib.xl_abi_version() == 10
     assert lib.xl_max_fft_length() == 4096
     # RS workspace: ncomp * round4(n) * P + (P/2) * P + P complex64 (+ alignment slack)
     n, P = 1024, 2048

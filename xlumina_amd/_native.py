@@ -113,7 +113,9 @@ class BsDotsParams(C.Structure):
     _fields_ = [("gc", C.c_void_p), ("gd", C.c_void_p), ("gc_comp_stride", C.c_int64),
                 ("gd_comp_stride", C.c_int64), ("a", C.c_void_p), ("b", C.c_void_p), ("a_comp_stride", C.c_int64),
                 ("b_comp_stride", C.c_int64), ("dots", C.c_void_p), ("npix", C.c_int64), ("theta", C.c_void_p),
-                ("theta_mode", C.c_int32), ("reserved", C.c_int32), ("dtheta", C.c_void_p)]
+                ("theta_mode", C.c_int32), ("reserved", C.c_int32), ("dtheta", C.c_void_p),
+                ("ga", C.c_void_p), ("gb", C.c_void_p), ("ga_comp_stride", C.c_int64), ("gb_comp_stride", C.c_int64),
+                ("g_planes", C.c_int32), ("reserved2", C.c_int32)]
 
 
 class MomentParams(C.Structure):

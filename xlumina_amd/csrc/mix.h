@@ -135,6 +135,10 @@ struct BsDotsArgs {
   const double* theta;
   int theta_mode;
   double* dtheta;
+  float2* ga;          // optional: the input cotangents (theta_mode > 0)
+  float2* gb;
+  int64_t gacs, gbcs;
+  int g_planes;        // planes 2 .. g_planes-1 of ga / gb are zeroed
 };
 
 // conj(u) v accumulated into acc
@@ -144,25 +148,53 @@ __device__ __forceinline__ void cdot_acc(float2 u, float2 v, float& re, float& i
 }
 
 // grid kBsDotBlocks: partial[blk] = the block's sums of conj(gc) a, conj(gc) b,
-// conj(gd) a, conj(gd) b over both components.
+// conj(gd) a, conj(gd) b over both components.  With ga / gb set, the same
+// pass writes the input cotangents (the splitter's adjoint) from the gc / gd
+// it has loaded.
 __global__ __launch_bounds__(256) void bs_dots_kernel(BsDotsArgs a) {
   float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   const float2 zero = make_float2(0.f, 0.f);
+  const bool fields = a.ga || a.gb;
+  float2 ka = zero, kb = zero, kc = zero, kd = zero;   // conj(ca), conj(da), conj(cb), conj(db)
+  if (fields) {   // formed once per block in float64, shared through LDS
+    __shared__ float2 k4[4];
+    if (threadIdx.x == 0) {
+      float2 ca, cb, da, db;
+      splitter_coef(a.theta_mode & 3, splitter_angle(a.theta_mode, a.theta[0]), ca, cb, da, db);
+      k4[0] = make_float2(ca.x, -ca.y);
+      k4[1] = make_float2(da.x, -da.y);
+      k4[2] = make_float2(cb.x, -cb.y);
+      k4[3] = make_float2(db.x, -db.y);
+    }
+    __syncthreads();
+    ka = k4[0]; kb = k4[1]; kc = k4[2]; kd = k4[3];
+  }
 #pragma unroll 2
   for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < 2 * a.npix; e += (int64_t)gridDim.x * 256) {
     const int comp = e >= a.npix ? 1 : 0;
     const int64_t p = e - comp * a.npix;
     const float2 va = a.a[comp * a.acs + p];
     const float2 vb = a.b ? a.b[comp * a.bcs + p] : zero;
+    const float2 gcv = a.gc ? a.gc[comp * a.gccs + p] : zero;
+    const float2 gdv = a.gd ? a.gd[comp * a.gdcs + p] : zero;
     if (a.gc) {
-      const float2 g = a.gc[comp * a.gccs + p];
-      cdot_acc(g, va, s[0], s[1]);
-      cdot_acc(g, vb, s[2], s[3]);
+      cdot_acc(gcv, va, s[0], s[1]);
+      cdot_acc(gcv, vb, s[2], s[3]);
     }
     if (a.gd) {
-      const float2 g = a.gd[comp * a.gdcs + p];
-      cdot_acc(g, va, s[4], s[5]);
-      cdot_acc(g, vb, s[6], s[7]);
+      cdot_acc(gdv, va, s[4], s[5]);
+      cdot_acc(gdv, vb, s[6], s[7]);
+    }
+    if (a.ga) a.ga[comp * a.gacs + p] = bs_mac(kb, gdv, bs_mac(ka, gcv, zero));
+    if (a.gb) a.gb[comp * a.gbcs + p] = bs_mac(kd, gdv, bs_mac(kc, gcv, zero));
+  }
+  if (fields) {   // the planes the splitter does not act on
+    const int64_t nz = (int64_t)(a.g_planes > 2 ? a.g_planes - 2 : 0) * a.npix;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < nz; e += (int64_t)gridDim.x * 256) {
+      const int c = 2 + (int)(e / a.npix);
+      const int64_t p = e - (int64_t)(c - 2) * a.npix;
+      if (a.ga) a.ga[c * a.gacs + p] = zero;
+      if (a.gb) a.gb[c * a.gbcs + p] = zero;
     }
   }
   __shared__ float red[8][4];

@@ -293,7 +293,7 @@ char* align256(char* p) { return (char*)(((uintptr_t)p + 255) & ~(uintptr_t)255)
 extern "C" {
 
 const char* xl_last_error(void) { return g_err.c_str(); }
-int xl_abi_version(void) { return 9; }
+int xl_abi_version(void) { return 10; }
 int xl_max_fft_length(void) { return xl::MAX_FFT; }
 int xl_init(void) { return ensure_init(); }
 
@@ -1264,6 +1264,13 @@ int xl_bs_dots(const xl_bs_dots_params* p, void* ws, size_t ws_bytes, void* stre
   a.theta = p->theta;
   a.theta_mode = p->theta_mode;
   a.dtheta = p->dtheta;
+  if ((p->ga || p->gb) && (!p->theta_mode || p->g_planes < 2 || p->g_planes > 3))
+    return fail(XL_EINVAL, "the fused input cotangents need theta_mode and g_planes 2 or 3");
+  a.ga = (float2*)p->ga;
+  a.gb = (float2*)p->gb;
+  a.gacs = p->ga_comp_stride;
+  a.gbcs = p->gb_comp_stride;
+  a.g_planes = p->g_planes;
   hipStream_t st = (hipStream_t)stream;
   xl::KTimer t(xl::K_BS, st);
   hipLaunchKernelGGL(xl::bs_dots_kernel, dim3(xl::kBsDotBlocks), dim3(256), 0, st, a);

@@ -1230,11 +1230,15 @@ def dz_dot(d: Optional[torch.Tensor], gout: Optional[torch.Tensor], pre: Optiona
 
 
 def bs_dots(gc: Optional[torch.Tensor], gd: Optional[torch.Tensor], a: torch.Tensor,
-            b: Optional[torch.Tensor], theta: Optional[torch.Tensor] = None, theta_mode: int = 0) -> torch.Tensor:
+            b: Optional[torch.Tensor], theta: Optional[torch.Tensor] = None, theta_mode: int = 0,
+            want_a: bool = False, want_b: bool = False, ncomp_out: int = 3):
     """(4,) complex64 device tensor: sums over Ex, Ey and all pixels of
     conj(gc) a, conj(gc) b, conj(gd) a, conj(gd) b (absent terms are 0).
     With theta_mode 1-3 (the splitters of beam_splitter), the 0-d float64
-    device tensor dL/dtheta instead."""
+    device tensor dL/dtheta instead; with want_a / want_b as well, the tuple
+    (dL/dtheta, ga, gb) with the input cotangents of beam_splitter(...,
+    adjoint=True) formed in the same pass ((ncomp_out, rows, cols) each, or
+    None where not wanted)."""
     lib = N.load()
     a = _gpu_c64(a, "a")
     dev = a.device
@@ -1254,8 +1258,18 @@ def bs_dots(gc: Optional[torch.Tensor], gd: Optional[torch.Tensor], a: torch.Ten
     if theta_mode:
         dtheta = torch.empty((), dtype=torch.float64, device=dev)
         p.theta, p.theta_mode, p.dtheta = _theta_ptr(theta, dev), int(theta_mode), dtheta.data_ptr()
+        ga = gb = None
+        if want_a or want_b:
+            if not 2 <= ncomp_out <= 3:
+                raise ValueError("ncomp_out must be 2 or 3")
+            shape = (ncomp_out,) + tuple(a.shape[1:])
+            ga = torch.empty(shape, dtype=torch.complex64, device=dev) if want_a else None
+            gb = torch.empty(shape, dtype=torch.complex64, device=dev) if want_b else None
+            p.ga, p.gb = _ptr(ga), _ptr(gb)
+            p.ga_comp_stride = p.gb_comp_stride = npix
+            p.g_planes = ncomp_out
         N.check(lib.xl_bs_dots(C.byref(p), ws.data_ptr(), ws.numel(), _stream(dev)))
-        return dtheta
+        return (dtheta, ga, gb) if (want_a or want_b) else dtheta
     N.check(lib.xl_bs_dots(C.byref(p), ws.data_ptr(), ws.numel(), _stream(dev)))
     return torch.view_as_complex(dots.view(4, 2))
 

@@ -405,9 +405,15 @@ class _SplitterTheta(torch.autograd.Function):
         b = b if ctx.has_b else None
         gc = gc.contiguous() if gc is not None else None
         gd = gd.contiguous() if gd is not None else None
-        z = torch.zeros_like(a) if gc is None or gd is None else None
         ga = gb = gt = None
         need_b = ctx.has_b and ctx.needs_input_grad[1]
+        if ctx.needs_input_grad[2] and (ctx.needs_input_grad[0] or need_b):
+            # dL/dtheta and the input cotangents in one pass over gc, gd, a, b
+            gt, ga, gb = ops.bs_dots(gc, gd, a.detach(), None if b is None else b.detach(), theta=th,
+                                     theta_mode=ctx.mode, want_a=ctx.needs_input_grad[0], want_b=need_b,
+                                     ncomp_out=a.shape[0])
+            return ga, gb, gt.reshape(th.shape), None
+        z = torch.zeros_like(a) if gc is None or gd is None else None
         if ctx.needs_input_grad[0] or need_b:
             ga, gb = ops.beam_splitter(gc if gc is not None else z, gd if gd is not None else z, None,
                                        want_c=True, want_d=need_b, ncomp_out=a.shape[0], theta=th,
