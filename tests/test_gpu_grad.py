@@ -256,7 +256,7 @@ def test_captured_high_na_step_matches_eager(cuda_dev, form):
         step = workloads.CapturedStep(lambda: h.loss(alpha, phi, geom), [alpha, phi])
         for it in range(3):
             L, (ga, gp) = step.replay()
-            L, ga, gp = float(L), ga.clone(), gp.clone()
+            L, ga, gp = float(L.detach()), ga.clone(), gp.clone()
             a2 = alpha.detach().clone().requires_grad_(True)
             p2 = phi.detach().clone().requires_grad_(True)
             Le = h.loss(a2, p2, geom)
