@@ -309,7 +309,10 @@ __global__ __launch_bounds__(3 * (P / 16)) void rs_urow_kernel(RsArgs a0) {
 // computes the masks and Ez of exactly those 8 pixels once, keeps all
 // components in registers, and the components are transformed one after the
 // other from there (no LDS staging pass).
-template <int P>
+// ONE (inputs without masks or Ez construction: the adjoint's cotangent): one
+// (row, component) unit per block, the components of a row on consecutive
+// blocks, for grids whose row blocks alone leave resident slots empty.
+template <int P, bool ONE = false>
 __global__ __launch_bounds__(P / 16) void rs_urow1_kernel(RsArgs a0) {
   const RsArgs a = rs_member(a0, blockIdx.y);
   constexpr int TPF = P / 16, LS = LdsLen<P>::value;
@@ -318,6 +321,24 @@ __global__ __launch_bounds__(P / 16) void rs_urow1_kernel(RsArgs a0) {
   constexpr int H = R0 / 2;   // inputs r < H hold the N <= P/2 pixels
   __shared__ float2 lds[LS];
   const int t = threadIdx.x;
+  if constexpr (ONE) {
+    const int unit = xcd_remap(blockIdx.x, gridDim.x);
+    const int row = unit / a.ncomp, c = unit - row * a.ncomp;
+    const int nrg = (a.n + 3) >> 2, rg = row >> 2, rr = row & 3;
+    const float2* src = a.in + (int64_t)c * a.in_cs;
+    Regs<1, R0> rin;
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+      const int col = t + r * TPF;
+      rin.v[0][r] = (r < H && row < a.n && col < a.n) ? src[(int64_t)row * a.n + col] : make_float2(0.f, 0.f);
+    }
+    NoIO none;
+    float2* dst = a.t + (int64_t)c * nrg * 4 * P + rg * 16 + rr * 4;
+    auto store = [&](int col, float2 v) { dst[(int64_t)(col >> 2) * nrg * 16 + (col & 3)] = v; };
+    typename LastPass<P, TPF, true>::Type unused;
+    chain_io<P, TPF, false, true, 0, SRC_REG, DST_FN, true, false>(lds, t, none, store, rin, unused);
+    return;
+  }
   const int row = xcd_remap(blockIdx.x, gridDim.x);
   const int nrg = (a.n + 3) >> 2, rg = row >> 2, rr = row & 3;
   const int nc = a.ncomp;
@@ -729,7 +750,15 @@ inline hipError_t rs_launch(const RsArgs& a, hipStream_t st) {
   }
   if constexpr (P >= 1024) {
     KTimer t(K_RS_UROW, st);
-    hipLaunchKernelGGL(rs_urow1_kernel<P>, dim3(rows, nb), dim3(P / 16), 0, st, a);
+    bool one = false;
+    if constexpr (P <= 2048) {
+      // plain inputs (no masks, no Ez) over fewer row blocks than resident
+      // slots: one block per (row, component)
+      one = a.ncomp > 1 && a.nin == a.ncomp && a.ez_mode == EZ_NONE && !a.mask.st[0].kind && !a.mask.st[1].kind &&
+            rows * nb < 2048;
+      if (one) hipLaunchKernelGGL((rs_urow1_kernel<P, true>), dim3(rows * a.ncomp, nb), dim3(P / 16), 0, st, a);
+    }
+    if (!one) hipLaunchKernelGGL((rs_urow1_kernel<P, false>), dim3(rows, nb), dim3(P / 16), 0, st, a);
   } else {
     KTimer t(K_RS_UROW, st);
     hipLaunchKernelGGL(rs_urow_kernel<P>, dim3(rows, nb), dim3(a.ncomp * (P / 16)),
