@@ -753,3 +753,18 @@ def test_three_component_plain_rs_rows_per_component(cuda_dev):
         assert rel_l2(got[c], ref) <= 1e-5, c
     cot = torch.randn((3, n, n), dtype=torch.complex64, device=cuda_dev)
     assert dot_identity(lambda v: AG.rs(v, x, y, 5000.0, 0.65), src, cot) < 1e-5
+
+
+def test_vrs_without_masks_at_p2048(cuda_dev):
+    """Ex, Ey with the constructed Ez at P = 2048 (n = 600), no masks (the row
+    pass keeps one block per row: the Ez construction couples the
+    components).  All three output components against the float64 oracle."""
+    n = 600
+    x, y = R.space(900.0, n)
+    rng = np.random.default_rng(73)
+    ex, ey = rng.standard_normal((2, n, n)) + 1j * rng.standard_normal((2, n, n))
+    src = torch.from_numpy(np.stack([ex, ey]).astype(np.complex64)).to(cuda_dev)
+    got = ops.rs_propagate(src, x, y, 5000.0, 0.65, ez_mode=N.XL_EZ_VRS, nin=2).cpu().numpy()
+    (vx, vy, vz), _ = R.vrs_propagation(ex, ey, x, y, 0.65, 5000.0)
+    for c, ref in enumerate((vx, vy, vz)):
+        assert rel_l2(got[c], ref) <= 1e-5, c
