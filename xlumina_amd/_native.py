@@ -189,6 +189,9 @@ _inited_devices: set = set()
 def load(path: str | None = None):
     """Loads the shared library (no GPU needed)."""
     global _lib
+    lib = _lib
+    if lib is not None and path is None:   # loaded: no lock on the per-call path
+        return lib
     with _lock:
         if _lib is not None:
             return _lib

@@ -8,6 +8,7 @@ current stream.  There is no CPU path: non-GPU tensors raise.
 from __future__ import annotations
 
 import ctypes as C
+import functools
 import math
 import weakref
 from dataclasses import dataclass
@@ -300,10 +301,12 @@ def _z_ptr(z_dev: Optional[torch.Tensor], dev):
 # RS / VRS
 # ---------------------------------------------------------------------------
 
+@functools.lru_cache(maxsize=256)
 def rs_workspace_bytes(n: int, ncomp: int) -> int:
     return int(N.load().xl_rs_workspace_bytes(n, ncomp))
 
 
+@functools.lru_cache(maxsize=256)
 def rs_hhat_bytes(n: int) -> int:
     """Bytes of the kernel-spectrum buffer of an n x n RS propagation (hhat)."""
     return int(N.load().xl_rs_hhat_bytes(n))
@@ -321,6 +324,7 @@ def rs_spec_partials(spec: torch.Tensor, n: int, ncomp: int, i: int = 0) -> torc
     return spec[start:start + 8 * count.value].view(torch.float64)
 
 
+@functools.lru_cache(maxsize=256)
 def rs_spec_bytes(n: int, ncomp: int) -> int:
     """Bytes of the spectrum buffer of an n x n, ncomp-component RS propagation
     whose gradients in z / lambda come from the column pass (spec_mode)."""
@@ -391,7 +395,7 @@ def rs_propagate(src: torch.Tensor, x, y, z: float, wavelength: float, masks: Se
         if n > N.XL_RS_NOALLOC_MAX_N:
             raise ValueError(f"noalloc propagation takes grids up to {N.XL_RS_NOALLOC_MAX_N} per side, got {n}")
     else:
-        need = lib.xl_rs_workspace_bytes(n, ncomp)
+        need = rs_workspace_bytes(n, ncomp)
         if workspace is None or workspace.numel() < need:
             workspace = torch.empty(need, dtype=torch.uint8, device=dev)
     p = N.RsParams()
