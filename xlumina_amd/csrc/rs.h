@@ -181,6 +181,17 @@ __device__ __forceinline__ void rs_kernel_with_derivatives(double X, double Y, d
   if constexpr ((DM & 2) != 0) hk = rot((float)(f * s * k * r), (float)(f * (s - 1.0)));
 }
 
+// The kept input spectra are written once and read once, by another launch:
+// non-temporal accesses, so that they do not displace the tile lines the
+// column pass shares through the caches.
+__device__ __forceinline__ void stream_store(float2* p, float2 v) {
+  __builtin_nontemporal_store(v.x, &p->x);
+  __builtin_nontemporal_store(v.y, &p->y);
+}
+__device__ __forceinline__ float2 stream_load(const float2* p) {
+  return make_float2(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
+}
+
 // Hr tile index of (row r, column c): P/8 row groups
 __device__ __forceinline__ int64_t hr_idx(int r, int c, int P) {
   return ((int64_t)(c >> 2) * (P / 8) + (r >> 2)) * 16 + (r & 3) * 4 + (c & 3);
@@ -503,7 +514,7 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_col_kernel(RsA
 #pragma unroll
     for (int b = 0; b < HO::BPT; ++b)
 #pragma unroll
-      for (int q = 0; q < HO::R; ++q) uk.v[b][q] = active ? sp[(b * HO::R + q) * TPF] : make_float2(0.f, 0.f);
+      for (int q = 0; q < HO::R; ++q) uk.v[b][q] = active ? stream_load(sp + (b * HO::R + q) * TPF) : make_float2(0.f, 0.f);
   }
   // rows >= N <= P/2 are zero padding, and the rows kept are < N: the first
   // forward pass skips the upper half of its inputs, the last inverse pass
@@ -515,7 +526,7 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_col_kernel(RsA
 #pragma unroll
       for (int b = 0; b < HO::BPT; ++b)
 #pragma unroll
-        for (int q = 0; q < HO::R; ++q) sp[(b * HO::R + q) * TPF] = u.v[b][q];
+        for (int q = 0; q < HO::R; ++q) stream_store(sp + (b * HO::R + q) * TPF, u.v[b][q]);
     }
   }
   const float hs = (a.adjoint && a.hstored) ? -1.f : 1.f;   // stored spectrum: conjugate for the adjoint
@@ -532,7 +543,7 @@ __global__ __launch_bounds__(Units<P>::CPB * (P / 16), 4) void rs_col_kernel(RsA
         float2 h = kk == P / 2 ? a.hmid[hcol] : hc[(kk >> 2) * 16 + (kk & 3) * 4];
         h.y *= hs;
         if constexpr (GB != 0) {
-          const float2 uv = UPRE ? uk.v[b][q] : a.spec[(int64_t)unit * P + t + (b * HO::R + q) * TPF];
+          const float2 uv = UPRE ? uk.v[b][q] : stream_load(a.spec + (int64_t)unit * P + t + (b * HO::R + q) * TPF);
           const float2 gv = u.v[b][q];
           const int64_t di = kk == P / 2 ? (int64_t)(P / 2) * P + hcol : ho;
 #pragma unroll
